@@ -1,0 +1,181 @@
+"""ctypes binding of the engine's C ABI (include/bs_api.h).
+
+This is the binding a Python caller would add on the reference side; the C++
+drop-in (a SpringNetwork subclass) binds the same symbols directly.  Loading
+fails loudly when the built library is missing: there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbiospring_amd.so")
+GEN_PATH = os.path.join(_HERE, "libbsgen.so")
+
+BS_OK, BS_ERR_ARG, BS_ERR_NONFINITE, BS_ERR_HIP, BS_ERR_STATE, BS_ERR_NOGPU = range(6)
+STERIC_LINEAR, STERIC_AMBER_12_6, STERIC_LEWITT_8_6, STERIC_ZACHARIAS_8_6 = range(4)
+DIELECTRIC_CONSTANT, DIELECTRIC_DISTANCE = 0, 1
+FLAG_STATIC = 1
+E_SPRING, E_ELECTROSTATIC, E_STERIC, E_KINETIC, E_IMP, E_HYDROPHOBIC = range(6)
+NUM_ENERGIES = 6
+NS_STERIC, NS_ELECTROSTATIC, NS_HYDROPHOBIC = range(3)
+
+# Exported symbols, one per declaration in include/bs_api.h.
+EXPORTS = (
+    "bs_default_params", "bs_create", "bs_destroy", "bs_last_error", "bs_num_particles",
+    "bs_num_springs", "bs_set_particles", "bs_set_springs", "bs_set_params", "bs_setup",
+    "bs_step", "bs_compute_forces", "bs_get_state", "bs_set_state", "bs_get_energies",
+    "bs_last_nonfinite_particle", "bs_get_counters", "bs_profile", "bs_get_profile",
+    "bs_get_cells", "bs_get_pairs",
+)
+PROF_REBUILD, PROF_PAIR, PROF_SPRING = range(3)
+
+
+class Params(ctypes.Structure):
+    """Mirror of bs_params (field order and types must match the header)."""
+    _fields_ = [
+        ("timestep", ctypes.c_float),
+        ("neighborskin", ctypes.c_float),
+        ("spring_enable", ctypes.c_int32),
+        ("spring_scale", ctypes.c_float),
+        ("steric_enable", ctypes.c_int32),
+        ("steric_mode", ctypes.c_int32),
+        ("steric_cutoff", ctypes.c_float),
+        ("steric_gridscale", ctypes.c_float),
+        ("electrostatic_enable", ctypes.c_int32),
+        ("electrostatic_cutoff", ctypes.c_float),
+        ("electrostatic_scale", ctypes.c_float),
+        ("electrostatic_dielectric", ctypes.c_float),
+        ("dielectric_model", ctypes.c_int32),
+        ("viscosity_enable", ctypes.c_int32),
+        ("viscosity_value", ctypes.c_float),
+        ("imp_enable", ctypes.c_int32),
+        ("imp_scale", ctypes.c_float),
+        ("imp_upper_offset", ctypes.c_float),
+        ("imp_lower_offset", ctypes.c_float),
+        ("imp_upper_curvature", ctypes.c_float),
+        ("imp_lower_curvature", ctypes.c_float),
+        ("hydrophobicity_enable", ctypes.c_int32),
+        ("hydrophobicity_cutoff", ctypes.c_float),
+        ("hydrophobicity_scale", ctypes.c_float),
+        ("list_skin", ctypes.c_float),
+        ("rebuild_interval", ctypes.c_int32),
+        ("compensated_positions", ctypes.c_int32),
+    ]
+
+    def copy(self):
+        q = Params()
+        ctypes.pointer(q)[0] = self
+        return q
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+_f32p = ctypes.POINTER(ctypes.c_float)
+_f64p = ctypes.POINTER(ctypes.c_double)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_vp = ctypes.c_void_p
+
+
+def ptr(a, ctype):
+    """Pointer to a contiguous numpy array (None -> NULL)."""
+    if a is None:
+        return None
+    return a.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+_lib = None
+_gen = None
+
+
+def lib():
+    """The engine library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"engine library not built: {LIB_PATH} (run __graft_entry__.build())")
+    L = ctypes.CDLL(LIB_PATH)
+    L.bs_default_params.argtypes = [ctypes.POINTER(Params)]
+    L.bs_default_params.restype = None
+    L.bs_create.argtypes = [ctypes.POINTER(Params), ctypes.POINTER(_vp)]
+    L.bs_destroy.argtypes = [_vp]
+    L.bs_last_error.argtypes = [_vp]
+    L.bs_last_error.restype = ctypes.c_char_p
+    L.bs_num_particles.argtypes = [_vp]
+    L.bs_num_particles.restype = ctypes.c_int64
+    L.bs_num_springs.argtypes = [_vp]
+    L.bs_num_springs.restype = ctypes.c_int64
+    L.bs_set_particles.argtypes = [_vp, ctypes.c_int64] + [_f32p] * 9 + [_u8p]
+    L.bs_set_springs.argtypes = [_vp, ctypes.c_int64, _u32p, _u32p, _f32p, _f32p]
+    L.bs_set_params.argtypes = [_vp, ctypes.POINTER(Params)]
+    L.bs_setup.argtypes = [_vp]
+    L.bs_step.argtypes = [_vp, ctypes.c_int32, _f32p]
+    L.bs_compute_forces.argtypes = [_vp, _f32p, _f32p, _f64p]
+    L.bs_get_state.argtypes = [_vp, _f32p, _f32p]
+    L.bs_set_state.argtypes = [_vp, _f32p, _f32p]
+    L.bs_get_energies.argtypes = [_vp, _f64p]
+    L.bs_last_nonfinite_particle.argtypes = [_vp]
+    L.bs_last_nonfinite_particle.restype = ctypes.c_int64
+    L.bs_get_counters.argtypes = [_vp, _i64p]
+    L.bs_profile.argtypes = [_vp, ctypes.c_int32]
+    L.bs_get_profile.argtypes = [_vp, _f64p, _i64p]
+    L.bs_get_cells.argtypes = [_vp, ctypes.c_int32, _i64p, _i64p, _i64p, _i32p, _i32p, _i64p, _f64p]
+    L.bs_get_pairs.argtypes = [_vp, ctypes.c_int32, _u32p, ctypes.c_int64, _i64p]
+    _lib = L
+    return L
+
+
+def gen():
+    """The host-only workload generator library."""
+    global _gen
+    if _gen is not None:
+        return _gen
+    if not os.path.exists(GEN_PATH):
+        raise RuntimeError(f"generator library not built: {GEN_PATH}")
+    G = ctypes.CDLL(GEN_PATH)
+    G.bsg_params.argtypes = [ctypes.c_int, ctypes.POINTER(Params)]
+    G.bsg_params.restype = None
+    G.bsg_generate.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_int64]
+    G.bsg_generate.restype = _vp
+    G.bsg_num_particles.argtypes = [_vp]
+    G.bsg_num_particles.restype = ctypes.c_int64
+    G.bsg_num_springs.argtypes = [_vp]
+    G.bsg_num_springs.restype = ctypes.c_int64
+    G.bsg_copy.argtypes = [_vp] + [_f32p] * 9 + [_u8p, _u32p, _u32p, _f32p, _f32p]
+    G.bsg_copy.restype = None
+    G.bsg_free.argtypes = [_vp]
+    G.bsg_free.restype = None
+    _gen = G
+    return G
+
+
+def default_params():
+    p = Params()
+    lib().bs_default_params(ctypes.byref(p))
+    return p
+
+
+def check(code, handle=None, what=""):
+    if code != BS_OK:
+        msg = lib().bs_last_error(handle).decode() if handle else ""
+        raise EngineError(code, f"{what}: code {code}: {msg}")
+
+
+class EngineError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+def f32(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float32)
+
+
+def u32(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.uint32)

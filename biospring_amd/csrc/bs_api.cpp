@@ -1,0 +1,1245 @@
+// bs_api.cpp -- C ABI of the engine (declared in include/bs_api.h).
+//
+// Host side: owns device buffers and one stream per handle, builds the spring
+// CSR and the neighbour lists, and sequences the kernels of one step in the
+// order of SpringNetwork::computeStep (src/spn/SpringNetwork.cpp:251-268).
+#include "bs_engine.hpp"
+#include "bs_params_defaults.h"
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <numeric>
+#include <string>
+#include <vector>
+
+using namespace bs;
+
+namespace
+{
+
+struct List
+{
+    int terms = 0;
+    float rc = 0.0f;        // shared cutoff of the list's terms
+    float radius = 0.0f;    // rc + list_skin
+    uint32_t imask = 0, jmask = 0;
+    int32_t dims[3] = {1, 1, 1};
+    int32_t ncells = 1;
+    int32_t cap = 0;
+    int32_t *nbl = nullptr;
+    int32_t *nbn = nullptr;
+};
+
+template <typename T> T *dalloc(size_t count, hipError_t &err)
+{
+    void *p = nullptr;
+    if (count == 0)
+        count = 1;
+    err = hipMalloc(&p, count * sizeof(T));
+    return static_cast<T *>(p);
+}
+
+} // namespace
+
+struct bs_handle
+{
+    bs_params p{};
+    std::string err;
+    hipStream_t stream = nullptr;
+    Consts k{};
+
+    // host copies, original order
+    int64_t n = 0;
+    std::vector<float> x, v, mass, charge, radius, eps, sasa, transfer, hydro;
+    std::vector<uint8_t> flags;
+    std::vector<uint32_t> s1, s2;
+    std::vector<float> sl0, sk;
+    bool have_particles = false, have_springs = false, ready = false;
+
+    // internal order
+    std::vector<int32_t> perm, iperm;
+
+    // device
+    float4 *pos[2] = {nullptr, nullptr}, *plo[2] = {nullptr, nullptr};
+    float4 *vel = nullptr, *fnb = nullptr, *jprm = nullptr, *iprm = nullptr, *xref = nullptr, *fext = nullptr,
+           *fout = nullptr;
+    int32_t *sp_ptr = nullptr, *sp_j = nullptr, *perm_d = nullptr;
+    float *sp_k = nullptr, *sp_l0 = nullptr;
+    int64_t nedges = 0;
+    int32_t *cell_of = nullptr, *counts = nullptr, *cstart = nullptr, *cursor = nullptr, *ctmp = nullptr,
+            *cidx = nullptr;
+    int32_t cell_cap = 0;
+    double *box_part = nullptr, *box = nullptr;
+    int32_t box_blocks = 0;
+    std::vector<List> lists;
+    double *pair_part = nullptr;
+    int32_t pair_blocks = 0;
+    double *spring_part = nullptr;
+    int32_t spring_blocks = 0;
+    int32_t group = 16;
+    double *energies_d = nullptr;
+    int32_t *status = nullptr;   // [0] rebuild, [1] nonfinite id, [2] overflow, [3] max count
+    int32_t *counters = nullptr; // [0] step counter, [1] rebuilds
+    double energies[BS_NUM_ENERGIES] = {0, 0, 0, 0, 0, 0};
+    int cur = 0;
+    int64_t steps_done = 0;
+    int64_t nonfinite = -1;
+    int ext_uploaded = 0;
+
+    // event timing of bs_step (bs_profile)
+    bool prof = false;
+    std::vector<hipEvent_t> evpool;
+    size_t evused = 0;
+    struct Span
+    {
+        int cls;
+        size_t a, b;
+    };
+    std::vector<Span> spans;
+    double prof_ms[BS_PROF_NUM] = {0, 0, 0};
+    int64_t prof_n[BS_PROF_NUM] = {0, 0, 0};
+};
+
+namespace
+{
+
+int fail(bs_handle *h, int code, const char *fmt, ...)
+{
+    if (h)
+    {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        h->err = buf;
+    }
+    return code;
+}
+
+#define HIPCHK(h, expr)                                                                                           \
+    do                                                                                                            \
+    {                                                                                                             \
+        const hipError_t e_ = (expr);                                                                             \
+        if (e_ != hipSuccess)                                                                                     \
+            return fail((h), BS_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));                         \
+    } while (0)
+
+void free_device(bs_handle *h)
+{
+    auto f = [](void *p) {
+        if (p)
+            (void)hipFree(p);
+    };
+    for (int b = 0; b < 2; ++b)
+    {
+        f(h->pos[b]);
+        f(h->plo[b]);
+        h->pos[b] = h->plo[b] = nullptr;
+    }
+    f(h->vel); f(h->fnb); f(h->jprm); f(h->iprm); f(h->xref); f(h->fext); f(h->fout);
+    f(h->sp_ptr); f(h->sp_j); f(h->perm_d); f(h->sp_k); f(h->sp_l0);
+    f(h->cell_of); f(h->counts); f(h->cstart); f(h->cursor); f(h->ctmp); f(h->cidx);
+    f(h->box_part); f(h->box); f(h->pair_part); f(h->spring_part); f(h->energies_d); f(h->status); f(h->counters);
+    for (auto &l : h->lists)
+    {
+        f(l.nbl);
+        f(l.nbn);
+    }
+    h->lists.clear();
+    h->vel = h->fnb = h->jprm = h->iprm = h->xref = h->fext = h->fout = nullptr;
+    h->sp_ptr = h->sp_j = h->perm_d = nullptr;
+    h->sp_k = h->sp_l0 = nullptr;
+    h->cell_of = h->counts = h->cstart = h->cursor = h->ctmp = h->cidx = nullptr;
+    h->box_part = h->box = h->pair_part = h->spring_part = h->energies_d = nullptr;
+    h->status = h->counters = nullptr;
+    h->ready = false;
+}
+
+inline uint64_t spread3(uint64_t v)
+{
+    v &= 0x1fffff;
+    v = (v | v << 32) & 0x1f00000000ffffull;
+    v = (v | v << 16) & 0x1f0000ff0000ffull;
+    v = (v | v << 8) & 0x100f00f00f00f00full;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ull;
+    v = (v | v << 2) & 0x1249249249249249ull;
+    return v;
+}
+
+// Internal order: Morton order of a 4 A grid; ties keep the original order.
+void make_order(bs_handle *h)
+{
+    const int64_t n = h->n;
+    double lo[3] = {INFINITY, INFINITY, INFINITY};
+    for (int64_t i = 0; i < n; ++i)
+        for (int c = 0; c < 3; ++c)
+            lo[c] = std::min(lo[c], (double)h->x[3 * i + c]);
+    std::vector<std::pair<uint64_t, int32_t>> key((size_t)n);
+    const double cell = 4.0;
+    for (int64_t i = 0; i < n; ++i)
+    {
+        uint64_t q[3];
+        for (int c = 0; c < 3; ++c)
+        {
+            double t = std::floor(((double)h->x[3 * i + c] - lo[c]) / cell);
+            t = std::min(std::max(t, 0.0), 2097151.0);
+            q[c] = (uint64_t)t;
+        }
+        key[(size_t)i] = {spread3(q[0]) | spread3(q[1]) << 1 | spread3(q[2]) << 2, (int32_t)i};
+    }
+    std::sort(key.begin(), key.end());
+    h->perm.resize((size_t)n);
+    h->iperm.resize((size_t)n);
+    for (int64_t t = 0; t < n; ++t)
+    {
+        h->perm[(size_t)t] = key[(size_t)t].second;
+        h->iperm[(size_t)key[(size_t)t].second] = (int32_t)t;
+    }
+}
+
+uint32_t particle_flags(const bs_handle *h, int64_t i)
+{
+    uint32_t f = 0;
+    if (h->flags[(size_t)i] & BS_FLAG_STATIC)
+        f |= PF_STATIC;
+    if (h->charge[(size_t)i] != 0.0f)
+        f |= PF_CHARGED;
+    if (h->hydro[(size_t)i] != 0.0f)
+        f |= PF_HYDRO;
+    return f;
+}
+
+float4 jprm_of(const bs_handle *h, int64_t i)
+{
+    const bool sq = h->p.steric_mode == BS_STERIC_AMBER_12_6 || h->p.steric_mode == BS_STERIC_LEWITT_8_6;
+    const float r = h->radius[(size_t)i], e = h->eps[(size_t)i];
+    float4 q;
+    q.x = sq ? sqrtf(r) : r;
+    q.y = sq ? sqrtf(e) : e;
+    q.z = h->hydro[(size_t)i];
+    uint32_t fl = particle_flags(h, i);
+    std::memcpy(&q.w, &fl, 4);
+    return q;
+}
+
+int upload_positions(bs_handle *h)
+{
+    const int64_t n = h->n;
+    std::vector<float4> pv((size_t)n), vv((size_t)n);
+    for (int64_t t = 0; t < n; ++t)
+    {
+        const int64_t i = h->perm[(size_t)t];
+        pv[(size_t)t] = make_float4(h->x[3 * i], h->x[3 * i + 1], h->x[3 * i + 2], h->charge[(size_t)i]);
+        vv[(size_t)t] = make_float4(h->v[3 * i], h->v[3 * i + 1], h->v[3 * i + 2], 0.0f);
+    }
+    for (int b = 0; b < 2; ++b)
+    {
+        HIPCHK(h, hipMemcpyAsync(h->pos[b], pv.data(), sizeof(float4) * n, hipMemcpyHostToDevice, h->stream));
+        if (h->plo[b])
+            HIPCHK(h, hipMemsetAsync(h->plo[b], 0, sizeof(float4) * n, h->stream));
+    }
+    HIPCHK(h, hipMemcpyAsync(h->vel, vv.data(), sizeof(float4) * n, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return BS_OK;
+}
+
+void plan_lists(bs_handle *h)
+{
+    const bs_params &p = h->p;
+    bool any_q = false, all_q = true, any_h = false, all_h = true, same_qh = true;
+    for (int64_t i = 0; i < h->n; ++i)
+    {
+        const bool q = h->charge[(size_t)i] != 0.0f, hy = h->hydro[(size_t)i] != 0.0f;
+        any_q |= q;
+        all_q &= q;
+        any_h |= hy;
+        all_h &= hy;
+        same_qh &= (q == hy);
+    }
+    h->lists.clear();
+    int steric_idx = -1, elec_idx = -1;
+    if (p.steric_enable)
+    {
+        List l;
+        l.terms = TERM_STERIC;
+        l.rc = p.steric_cutoff;
+        h->lists.push_back(l);
+        steric_idx = 0;
+    }
+    if (p.electrostatic_enable && any_q)
+    {
+        if (steric_idx >= 0 && p.electrostatic_cutoff == p.steric_cutoff && all_q)
+            h->lists[(size_t)steric_idx].terms |= TERM_ELEC;
+        else
+        {
+            List l;
+            l.terms = TERM_ELEC;
+            l.rc = p.electrostatic_cutoff;
+            l.imask = l.jmask = PF_CHARGED;
+            h->lists.push_back(l);
+            elec_idx = (int)h->lists.size() - 1;
+        }
+    }
+    if (p.hydrophobicity_enable && any_h)
+    {
+        if (steric_idx >= 0 && p.hydrophobicity_cutoff == p.steric_cutoff && all_h)
+            h->lists[(size_t)steric_idx].terms |= TERM_HYDRO;
+        else if (elec_idx >= 0 && p.hydrophobicity_cutoff == p.electrostatic_cutoff && same_qh)
+            h->lists[(size_t)elec_idx].terms |= TERM_HYDRO;
+        else
+        {
+            List l;
+            l.terms = TERM_HYDRO;
+            l.rc = p.hydrophobicity_cutoff;
+            l.imask = l.jmask = PF_HYDRO;
+            h->lists.push_back(l);
+        }
+    }
+    for (auto &l : h->lists)
+        l.radius = l.rc + std::max(p.list_skin, 0.0f);
+}
+
+// Engine grid dimensions of a list from the current host positions.
+void plan_grid(bs_handle *h, List &l)
+{
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t i = 0; i < h->n; ++i)
+        for (int c = 0; c < 3; ++c)
+        {
+            lo[c] = std::min(lo[c], (double)h->x[3 * i + c]);
+            hi[c] = std::max(hi[c], (double)h->x[3 * i + c]);
+        }
+    int64_t total = 1;
+    for (int c = 0; c < 3; ++c)
+    {
+        int64_t d = (int64_t)std::floor((hi[c] - lo[c]) / l.radius);
+        d = std::max<int64_t>(1, std::min<int64_t>(d, 1024));
+        l.dims[c] = (int32_t)d;
+        total *= d;
+    }
+    while (total > (1 << 22))
+    { // keep the grid modest for sparse, spread-out systems
+        for (int c = 0; c < 3; ++c)
+            l.dims[c] = std::max(1, l.dims[c] / 2);
+        total = (int64_t)l.dims[0] * l.dims[1] * l.dims[2];
+    }
+    l.ncells = (int32_t)total;
+}
+
+int ensure_cells(bs_handle *h, int32_t ncells)
+{
+    if (ncells <= h->cell_cap)
+        return BS_OK;
+    hipError_t e;
+    auto f = [](void *p) {
+        if (p)
+            (void)hipFree(p);
+    };
+    f(h->counts);
+    f(h->cstart);
+    f(h->cursor);
+    h->counts = dalloc<int32_t>((size_t)ncells, e);
+    HIPCHK(h, e);
+    h->cstart = dalloc<int32_t>((size_t)ncells + 1, e);
+    HIPCHK(h, e);
+    h->cursor = dalloc<int32_t>((size_t)ncells, e);
+    HIPCHK(h, e);
+    h->cell_cap = ncells;
+    return BS_OK;
+}
+
+const int32_t *gate_of(bs_handle *h, bool gated) { return gated ? h->status : nullptr; }
+
+// Engine cell grid + Verlet list of list l.  gated: skipped on the device
+// unless the rebuild flag is set.
+void build_list(bs_handle *h, List &l, bool gated, int32_t cap_override)
+{
+    const int32_t n = (int32_t)h->n;
+    const int32_t *g = gate_of(h, gated);
+    hipStream_t s = h->stream;
+    launch_fill_i32(h->counts, 0, l.ncells, g, s);
+    launch_cell_ids(h->pos[h->cur], h->jprm, l.jmask, n, h->box, l.dims[0], l.dims[1], l.dims[2], 0.0, l.radius, 0,
+                    h->cell_of, h->counts, g, s);
+    launch_scan(h->counts, h->cstart, l.ncells, g, s);
+    launch_fill_i32(h->cursor, 0, l.ncells, g, s);
+    launch_scatter(h->cell_of, n, h->cstart, h->cursor, h->ctmp, g, s);
+    launch_cell_sort(h->cell_of, n, h->cstart, h->ctmp, nullptr, h->cidx, g, s);
+    ListArgs a{};
+    a.pos = h->pos[h->cur];
+    a.jprm = h->jprm;
+    a.cell_start = h->cstart;
+    a.cell_idx = h->cidx;
+    a.box = h->box;
+    a.sp_ptr = h->sp_ptr;
+    a.sp_j = h->sp_j;
+    a.nbl = l.nbl;
+    a.nbn = l.nbn;
+    a.status = h->status;
+    a.n = n;
+    a.npad = (int32_t)((h->n + 63) / 64 * 64);
+    a.cap = cap_override >= 0 ? cap_override : l.cap;
+    for (int c = 0; c < 3; ++c)
+        a.dims[c] = l.dims[c];
+    a.cell_edge = l.radius;
+    a.r2list = (float)((double)l.radius * l.radius * (1.0 + 1e-5));
+    a.i_mask = l.imask;
+    a.j_mask = l.jmask;
+    a.exclude_bonded = h->p.spring_enable ? 1 : 0;
+    a.gate = g;
+    launch_list_build(a, s);
+}
+
+void rebuild_chain(bs_handle *h, bool gated)
+{
+    if (h->lists.empty())
+    {
+        launch_clear_rebuild(h->status, h->counters, h->stream);
+        return;
+    }
+    const int32_t *g = gate_of(h, gated);
+    launch_box_reduce(h->pos[h->cur], (int32_t)h->n, h->box_part, h->box, h->box_blocks, g, h->stream);
+    for (auto &l : h->lists)
+        build_list(h, l, gated, -1);
+    launch_finish_rebuild(h->pos[h->cur], h->xref, (int32_t)h->n, h->status, h->counters, h->stream);
+    launch_clear_rebuild(h->status, h->counters, h->stream);
+}
+
+float rc2(float rc, double f) { return (float)((double)rc * rc * f); }
+
+void launch_pairs(bs_handle *h)
+{
+    const int32_t n = (int32_t)h->n;
+    const int32_t npad = (int32_t)((h->n + 63) / 64 * 64);
+    for (size_t li = 0; li < h->lists.size(); ++li)
+    {
+        const List &l = h->lists[li];
+        PairArgs a{};
+        a.pos = h->pos[h->cur];
+        a.jprm = h->jprm;
+        a.nbl = l.nbl;
+        a.nbn = l.nbn;
+        a.fnb = h->fnb;
+        a.epart = h->pair_part + (size_t)li * h->pair_blocks * 3;
+        a.n = n;
+        a.npad = npad;
+        a.accumulate = li > 0 ? 1 : 0;
+        a.rc = l.rc;
+        a.rc2lo = rc2(l.rc, 1.0 - 4e-6);
+        a.rc2hi = rc2(l.rc, 1.0 + 4e-6);
+        a.steric_mode = (l.terms & TERM_STERIC) ? h->p.steric_mode : -1;
+        a.steric_scale = h->p.steric_gridscale;
+        a.elec_scale = h->p.electrostatic_scale;
+        a.diel = h->p.electrostatic_dielectric;
+        a.hydro_scale = h->p.hydrophobicity_scale;
+        a.i_mask = l.imask;
+        a.j_mask = l.jmask;
+        a.k = h->k;
+        const int em = (l.terms & TERM_ELEC) ? (h->p.dielectric_model == BS_DIELECTRIC_DISTANCE ? 2 : 1) : 0;
+        launch_pair(a, em, (l.terms & TERM_HYDRO) ? 1 : 0, h->pair_blocks, h->stream);
+    }
+}
+
+void launch_springs(bs_handle *h, bool integrate)
+{
+    SpringArgs a{};
+    a.pos = h->pos[h->cur];
+    a.plo = h->p.compensated_positions ? h->plo[h->cur] : nullptr;
+    a.pos_out = h->pos[h->cur ^ 1];
+    a.plo_out = h->p.compensated_positions ? h->plo[h->cur ^ 1] : nullptr;
+    a.vel = h->vel;
+    a.fnb = h->lists.empty() ? nullptr : h->fnb;
+    a.fext = h->ext_uploaded ? h->fext : nullptr;
+    a.iprm = h->iprm;
+    a.jprm = h->jprm;
+    a.xref = h->xref;
+    a.sp_ptr = h->sp_ptr;
+    a.sp_j = h->sp_j;
+    a.sp_k = h->sp_k;
+    a.sp_l0 = h->sp_l0;
+    a.perm = h->perm_d;
+    a.fout = h->fout;
+    a.epart = h->spring_part;
+    a.status = h->status;
+    a.n = (int32_t)h->n;
+    a.spring_enable = h->p.spring_enable;
+    a.spring_scale = h->p.spring_scale;
+    a.dt = h->p.timestep;
+    a.visc_enable = h->p.viscosity_enable;
+    a.visc = h->p.viscosity_value;
+    a.imp_enable = h->p.imp_enable;
+    a.imp_scale = h->p.imp_scale;
+    a.imp_uo = h->p.imp_upper_offset;
+    a.imp_lo = h->p.imp_lower_offset;
+    a.imp_uc = h->p.imp_upper_curvature;
+    a.imp_lc = h->p.imp_lower_curvature;
+    const double hs = 0.5 * std::max(h->p.list_skin, 0.0f);
+    a.half_skin2 = (float)(hs * hs * (1.0 - 1e-5));
+    a.integrate = integrate ? 1 : 0;
+    a.k = h->k;
+    launch_spring(a, h->group, h->spring_blocks, h->stream);
+}
+
+// Records an event on the handle's stream and returns its pool index.
+size_t prof_mark(bs_handle *h)
+{
+    if (h->evused == h->evpool.size())
+    {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess)
+            return SIZE_MAX;
+        h->evpool.push_back(e);
+    }
+    const size_t idx = h->evused++;
+    if (hipEventRecord(h->evpool[idx], h->stream) != hipSuccess)
+        return SIZE_MAX;
+    return idx;
+}
+
+void prof_span(bs_handle *h, int cls, size_t a, size_t b)
+{
+    if (a != SIZE_MAX && b != SIZE_MAX)
+        h->spans.push_back({cls, a, b});
+}
+
+// After the stream has been synchronised: folds the recorded spans in.
+void prof_collect(bs_handle *h)
+{
+    for (const auto &s : h->spans)
+    {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, h->evpool[s.a], h->evpool[s.b]) == hipSuccess)
+        {
+            h->prof_ms[s.cls] += ms;
+            h->prof_n[s.cls] += 1;
+        }
+    }
+    h->spans.clear();
+    h->evused = 0;
+}
+
+int finish_call(bs_handle *h)
+{
+    launch_reduce_energies(h->pair_part, h->pair_blocks, (int32_t)h->lists.size(), h->spring_part,
+                           h->spring_blocks, h->energies_d, h->stream);
+    int32_t st[4];
+    HIPCHK(h, hipMemcpyAsync(h->energies, h->energies_d, sizeof(h->energies), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(st, h->status, sizeof st, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    prof_collect(h);
+    if (st[2] != 0)
+        return fail(h, BS_ERR_STATE, "neighbour list capacity exceeded (max %d > cap); rerun bs_setup", st[3]);
+    if (st[1] != INT_MAX)
+    {
+        h->nonfinite = st[1];
+        return fail(h, BS_ERR_NONFINITE, "Found non-finite position for particle %d.", st[1]);
+    }
+    return BS_OK;
+}
+
+int upload_ext(bs_handle *h, const float *ext3)
+{
+    h->ext_uploaded = 0;
+    if (!ext3)
+        return BS_OK;
+    std::vector<float4> fe((size_t)h->n);
+    for (int64_t t = 0; t < h->n; ++t)
+    {
+        const int64_t i = h->perm[(size_t)t];
+        fe[(size_t)t] = make_float4(ext3[3 * i], ext3[3 * i + 1], ext3[3 * i + 2], 0.0f);
+    }
+    HIPCHK(h, hipMemcpyAsync(h->fext, fe.data(), sizeof(float4) * h->n, hipMemcpyHostToDevice, h->stream));
+    h->ext_uploaded = 1;
+    return BS_OK;
+}
+
+int do_setup(bs_handle *h)
+{
+    if (!h->have_particles)
+        return fail(h, BS_ERR_STATE, "bs_setup: no particles");
+    free_device(h);
+    const int64_t n = h->n;
+    if (n > INT32_MAX / 64)
+        return fail(h, BS_ERR_ARG, "bs_setup: too many particles");
+    make_order(h);
+    hipError_t e;
+#define ALLOC(ptr, T, cnt)                                                                                        \
+    do                                                                                                            \
+    {                                                                                                             \
+        ptr = dalloc<T>((size_t)(cnt), e);                                                                        \
+        HIPCHK(h, e);                                                                                             \
+    } while (0)
+    for (int b = 0; b < 2; ++b)
+    {
+        ALLOC(h->pos[b], float4, n);
+        if (h->p.compensated_positions)
+            ALLOC(h->plo[b], float4, n);
+    }
+    ALLOC(h->vel, float4, n);
+    ALLOC(h->fnb, float4, n);
+    ALLOC(h->jprm, float4, n);
+    ALLOC(h->iprm, float4, n);
+    ALLOC(h->xref, float4, n);
+    ALLOC(h->fext, float4, n);
+    ALLOC(h->fout, float4, n);
+    ALLOC(h->perm_d, int32_t, n);
+    ALLOC(h->cell_of, int32_t, n);
+    ALLOC(h->ctmp, int32_t, n);
+    ALLOC(h->cidx, int32_t, n);
+    ALLOC(h->status, int32_t, 4);
+    ALLOC(h->counters, int32_t, 4);
+    ALLOC(h->energies_d, double, BS_NUM_ENERGIES);
+    h->box_blocks = (int32_t)std::min<int64_t>(1024, (n + BLOCK - 1) / BLOCK);
+    ALLOC(h->box_part, double, 6 * h->box_blocks);
+    ALLOC(h->box, double, 6);
+
+    // per-particle parameters
+    std::vector<float4> jp((size_t)n), ip((size_t)n);
+    for (int64_t t = 0; t < n; ++t)
+    {
+        const int64_t i = h->perm[(size_t)t];
+        jp[(size_t)t] = jprm_of(h, i);
+        ip[(size_t)t] = make_float4(h->mass[(size_t)i], h->sasa[(size_t)i], h->transfer[(size_t)i],
+                                    h->radius[(size_t)i]);
+    }
+    HIPCHK(h, hipMemcpyAsync(h->jprm, jp.data(), sizeof(float4) * n, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->iprm, ip.data(), sizeof(float4) * n, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->perm_d, h->perm.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+
+    // spring CSR, both directions, rows sorted by j (internal ids)
+    const int64_t m = (int64_t)h->s1.size();
+    std::vector<int32_t> ptr((size_t)n + 1, 0);
+    for (int64_t s = 0; s < m; ++s)
+    {
+        ptr[(size_t)h->iperm[h->s1[(size_t)s]] + 1]++;
+        ptr[(size_t)h->iperm[h->s2[(size_t)s]] + 1]++;
+    }
+    for (int64_t i = 0; i < n; ++i)
+        ptr[(size_t)i + 1] += ptr[(size_t)i];
+    h->nedges = ptr[(size_t)n];
+    struct Edge
+    {
+        int32_t j;
+        float k, l0;
+    };
+    std::vector<Edge> edges((size_t)h->nedges);
+    {
+        std::vector<int32_t> fill(ptr.begin(), ptr.end() - 1);
+        for (int64_t s = 0; s < m; ++s)
+        {
+            const int32_t a = h->iperm[h->s1[(size_t)s]], b = h->iperm[h->s2[(size_t)s]];
+            edges[(size_t)fill[(size_t)a]++] = {b, h->sk[(size_t)s], h->sl0[(size_t)s]};
+            edges[(size_t)fill[(size_t)b]++] = {a, h->sk[(size_t)s], h->sl0[(size_t)s]};
+        }
+    }
+    std::vector<int32_t> ej((size_t)h->nedges);
+    std::vector<float> ek((size_t)h->nedges), el((size_t)h->nedges);
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int64_t i = 0; i < n; ++i)
+    {
+        std::sort(edges.begin() + ptr[(size_t)i], edges.begin() + ptr[(size_t)i + 1],
+                  [](const Edge &a, const Edge &b) { return a.j < b.j; });
+        for (int32_t t = ptr[(size_t)i]; t < ptr[(size_t)i + 1]; ++t)
+        {
+            const int32_t j = edges[(size_t)t].j;
+            const bool js = (h->flags[(size_t)h->perm[(size_t)j]] & BS_FLAG_STATIC) != 0;
+            ej[(size_t)t] = js ? (int32_t)((uint32_t)j | 0x80000000u) : j;
+            ek[(size_t)t] = edges[(size_t)t].k;
+            el[(size_t)t] = edges[(size_t)t].l0;
+        }
+    }
+    ALLOC(h->sp_ptr, int32_t, n + 1);
+    ALLOC(h->sp_j, int32_t, h->nedges);
+    ALLOC(h->sp_k, float, h->nedges);
+    ALLOC(h->sp_l0, float, h->nedges);
+    HIPCHK(h, hipMemcpyAsync(h->sp_ptr, ptr.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, h->stream));
+    if (h->nedges)
+    {
+        HIPCHK(h, hipMemcpyAsync(h->sp_j, ej.data(), sizeof(int32_t) * h->nedges, hipMemcpyHostToDevice, h->stream));
+        HIPCHK(h, hipMemcpyAsync(h->sp_k, ek.data(), sizeof(float) * h->nedges, hipMemcpyHostToDevice, h->stream));
+        HIPCHK(h, hipMemcpyAsync(h->sp_l0, el.data(), sizeof(float) * h->nedges, hipMemcpyHostToDevice, h->stream));
+    }
+    int rc = upload_positions(h);
+    if (rc)
+        return rc;
+
+    // status / counters
+    const int32_t st0[4] = {1, INT_MAX, 0, 0}, ct0[4] = {0, 0, 0, 0};
+    HIPCHK(h, hipMemcpyAsync(h->status, st0, sizeof st0, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->counters, ct0, sizeof ct0, hipMemcpyHostToDevice, h->stream));
+
+    // kernel geometry
+    h->pair_blocks = (int32_t)((n + BLOCK - 1) / BLOCK);
+    h->spring_blocks = (int32_t)((n * h->group + BLOCK - 1) / BLOCK);
+    ALLOC(h->spring_part, double, 3 * h->spring_blocks);
+
+    // neighbour lists: a counting pass sizes the capacity
+    plan_lists(h);
+    ALLOC(h->pair_part, double, 3 * h->pair_blocks * std::max<size_t>(1, h->lists.size()));
+    int32_t maxcells = 1;
+    for (auto &l : h->lists)
+    {
+        plan_grid(h, l);
+        maxcells = std::max(maxcells, l.ncells);
+    }
+    rc = ensure_cells(h, maxcells);
+    if (rc)
+        return rc;
+    if (!h->lists.empty())
+        launch_box_reduce(h->pos[h->cur], (int32_t)n, h->box_part, h->box, h->box_blocks, nullptr, h->stream);
+    const int64_t npad = (n + 63) / 64 * 64;
+    for (auto &l : h->lists)
+    {
+        ALLOC(l.nbn, int32_t, n);
+        launch_fill_i32(h->status + 3, 0, 1, nullptr, h->stream);
+        build_list(h, l, false, 0);
+        int32_t st[4];
+        HIPCHK(h, hipMemcpyAsync(st, h->status, sizeof st, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        const int32_t mx = st[3];
+        l.cap = std::max<int32_t>(16, (int32_t)std::ceil(mx * 1.25) + 32);
+        ALLOC(l.nbl, int32_t, npad * (int64_t)l.cap);
+        launch_fill_i32(h->status + 2, 0, 1, nullptr, h->stream);
+    }
+    const int32_t st1[4] = {1, INT_MAX, 0, 0};
+    HIPCHK(h, hipMemcpyAsync(h->status, st1, sizeof st1, hipMemcpyHostToDevice, h->stream));
+    rebuild_chain(h, true);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+#undef ALLOC
+    h->ready = true;
+    h->cur = 0;
+    return BS_OK;
+}
+
+int ensure_ready(bs_handle *h)
+{
+    if (!h->ready)
+        return do_setup(h);
+    return BS_OK;
+}
+
+int kind_term(int kind) { return kind == BS_NS_ELECTROSTATIC ? TERM_ELEC : kind == BS_NS_HYDROPHOBIC ? TERM_HYDRO : TERM_STERIC; }
+
+} // namespace
+
+// ===========================================================================
+extern "C" {
+
+void bs_default_params(bs_params *p)
+{
+    if (p)
+        bs_params_fill_defaults(p);
+}
+
+int bs_create(const bs_params *p, bs_handle **out)
+{
+    if (!out)
+        return BS_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return BS_ERR_NOGPU;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess)
+        return BS_ERR_NOGPU;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess)
+        return BS_ERR_NOGPU;
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return BS_ERR_NOGPU;
+    bs_handle *h = new (std::nothrow) bs_handle();
+    if (!h)
+        return BS_ERR_ARG;
+    if (p)
+        h->p = *p;
+    else
+        bs_params_fill_defaults(&h->p);
+    h->k = make_consts();
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
+    {
+        delete h;
+        return BS_ERR_HIP;
+    }
+    *out = h;
+    return BS_OK;
+}
+
+int bs_destroy(bs_handle *h)
+{
+    if (!h)
+        return BS_OK;
+    if (h->stream)
+        (void)hipStreamSynchronize(h->stream);
+    for (hipEvent_t e : h->evpool)
+        (void)hipEventDestroy(e);
+    free_device(h);
+    if (h->stream)
+        (void)hipStreamDestroy(h->stream);
+    delete h;
+    return BS_OK;
+}
+
+const char *bs_last_error(const bs_handle *h)
+{
+    if (!h)
+        return "no handle (bs_create failed or was not called)";
+    return h->err.c_str();
+}
+
+int64_t bs_num_particles(const bs_handle *h) { return h ? h->n : 0; }
+int64_t bs_num_springs(const bs_handle *h) { return h ? (int64_t)h->s1.size() : 0; }
+
+int bs_set_particles(bs_handle *h, int64_t n, const float *x3, const float *v3, const float *mass,
+                     const float *charge, const float *radius, const float *epsilon, const float *sasa,
+                     const float *transfer, const float *hydrophobicity, const uint8_t *flags)
+{
+    if (!h)
+        return BS_ERR_ARG;
+    if (n <= 0 || !x3)
+        return fail(h, BS_ERR_ARG, "bs_set_particles: empty particle list");
+    free_device(h);
+    h->n = n;
+    auto take = [n](std::vector<float> &dst, const float *src, int64_t per, float dflt) {
+        dst.assign((size_t)(n * per), dflt);
+        if (src)
+            std::memcpy(dst.data(), src, sizeof(float) * n * per);
+    };
+    take(h->x, x3, 3, 0.0f);
+    take(h->v, v3, 3, 0.0f);
+    take(h->mass, mass, 1, 1.0f);
+    take(h->charge, charge, 1, 0.0f);
+    take(h->radius, radius, 1, 1.0f);
+    take(h->eps, epsilon, 1, 0.0f);
+    take(h->sasa, sasa, 1, 0.0f);
+    take(h->transfer, transfer, 1, 0.0f);
+    take(h->hydro, hydrophobicity, 1, 0.0f);
+    h->flags.assign((size_t)n, 0);
+    if (flags)
+        std::memcpy(h->flags.data(), flags, (size_t)n);
+    h->s1.clear();
+    h->s2.clear();
+    h->sl0.clear();
+    h->sk.clear();
+    h->have_particles = true;
+    h->steps_done = 0;
+    h->nonfinite = -1;
+    return BS_OK;
+}
+
+int bs_set_springs(bs_handle *h, int64_t m, const uint32_t *id1, const uint32_t *id2, const float *equilibrium,
+                   const float *stiffness)
+{
+    if (!h)
+        return BS_ERR_ARG;
+    if (!h->have_particles)
+        return fail(h, BS_ERR_STATE, "bs_set_springs: particles must be set before springs");
+    if (m < 0 || (m > 0 && (!id1 || !id2 || !equilibrium || !stiffness)))
+        return fail(h, BS_ERR_ARG, "bs_set_springs: bad arrays");
+    // SpringNetwork::addSpring: range and self checks, duplicates ignored.
+    std::vector<std::vector<uint32_t>> seen((size_t)h->n);
+    h->s1.clear();
+    h->s2.clear();
+    h->sl0.clear();
+    h->sk.clear();
+    for (int64_t s = 0; s < m; ++s)
+    {
+        const uint32_t a = id1[s], b = id2[s];
+        if (a >= (uint64_t)h->n || b >= (uint64_t)h->n)
+            return fail(h, BS_ERR_ARG, "SpringNetwork::addSpring: particle index out of range (spring %lld)",
+                        (long long)s);
+        if (a == b)
+            return fail(h, BS_ERR_ARG, "SpringNetwork::addSpring: a spring requires two distinct particles");
+        auto &sa = seen[a];
+        if (std::find(sa.begin(), sa.end(), b) != sa.end())
+            continue;
+        sa.push_back(b);
+        seen[b].push_back(a);
+        h->s1.push_back(a);
+        h->s2.push_back(b);
+        h->sl0.push_back(equilibrium[s]);
+        h->sk.push_back(stiffness[s]);
+    }
+    free_device(h);
+    h->have_springs = true;
+    return BS_OK;
+}
+
+int bs_set_params(bs_handle *h, const bs_params *p)
+{
+    if (!h || !p)
+        return BS_ERR_ARG;
+    if (h->ready)
+    {
+        // keep the current state, then rebuild lazily with the new parameters
+        int rc = bs_get_state(h, h->x.data(), h->v.data());
+        if (rc)
+            return rc;
+        free_device(h);
+    }
+    h->p = *p;
+    return BS_OK;
+}
+
+int bs_setup(bs_handle *h)
+{
+    if (!h)
+        return BS_ERR_ARG;
+    return do_setup(h);
+}
+
+int bs_step(bs_handle *h, int32_t nsteps, const float *ext_force3)
+{
+    if (!h)
+        return BS_ERR_ARG;
+    if (nsteps < 0)
+        return fail(h, BS_ERR_ARG, "bs_step: negative step count");
+    int rc = ensure_ready(h);
+    if (rc)
+        return rc;
+    rc = upload_ext(h, ext_force3);
+    if (rc)
+        return rc;
+    for (int32_t s = 0; s < nsteps; ++s)
+    {
+        // Bound the event pool: fold in what is recorded every 1024 steps.
+        if (h->prof && h->evused > 4096)
+        {
+            HIPCHK(h, hipStreamSynchronize(h->stream));
+            prof_collect(h);
+        }
+        const size_t e0 = h->prof ? prof_mark(h) : SIZE_MAX;
+        launch_prestep(h->status, h->counters, h->p.rebuild_interval, h->stream);
+        rebuild_chain(h, true);
+        const size_t e1 = h->prof ? prof_mark(h) : SIZE_MAX;
+        if (!h->lists.empty())
+            launch_pairs(h);
+        const size_t e2 = h->prof ? prof_mark(h) : SIZE_MAX;
+        launch_springs(h, true);
+        if (h->prof)
+        {
+            const size_t e3 = prof_mark(h);
+            prof_span(h, BS_PROF_REBUILD, e0, e1);
+            if (!h->lists.empty())
+                prof_span(h, BS_PROF_PAIR, e1, e2);
+            prof_span(h, BS_PROF_SPRING, e2, e3);
+        }
+        h->cur ^= 1;
+    }
+    h->steps_done += nsteps;
+    return finish_call(h);
+}
+
+int bs_compute_forces(bs_handle *h, const float *ext_force3, float *force3, double *energies)
+{
+    if (!h)
+        return BS_ERR_ARG;
+    int rc = ensure_ready(h);
+    if (rc)
+        return rc;
+    rc = upload_ext(h, ext_force3);
+    if (rc)
+        return rc;
+    rebuild_chain(h, true);
+    if (!h->lists.empty())
+        launch_pairs(h);
+    launch_springs(h, false);
+    rc = finish_call(h);
+    h->energies[BS_E_KINETIC] = 0.0;
+    if (rc)
+        return rc;
+    if (force3)
+    {
+        std::vector<float4> f((size_t)h->n);
+        HIPCHK(h, hipMemcpy(f.data(), h->fout, sizeof(float4) * h->n, hipMemcpyDeviceToHost));
+        for (int64_t t = 0; t < h->n; ++t)
+        {
+            const int64_t i = h->perm[(size_t)t];
+            const bool st = (h->flags[(size_t)i] & BS_FLAG_STATIC) != 0;
+            force3[3 * i] = st ? 0.0f : f[(size_t)t].x;
+            force3[3 * i + 1] = st ? 0.0f : f[(size_t)t].y;
+            force3[3 * i + 2] = st ? 0.0f : f[(size_t)t].z;
+        }
+    }
+    if (energies)
+        std::memcpy(energies, h->energies, sizeof(h->energies));
+    return BS_OK;
+}
+
+int bs_get_state(bs_handle *h, float *x3, float *v3)
+{
+    if (!h)
+        return BS_ERR_ARG;
+    if (!h->ready)
+    {
+        if (x3)
+            std::memcpy(x3, h->x.data(), sizeof(float) * 3 * h->n);
+        if (v3)
+            std::memcpy(v3, h->v.data(), sizeof(float) * 3 * h->n);
+        return BS_OK;
+    }
+    std::vector<float4> pv((size_t)h->n), vv((size_t)h->n);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipMemcpy(pv.data(), h->pos[h->cur], sizeof(float4) * h->n, hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(vv.data(), h->vel, sizeof(float4) * h->n, hipMemcpyDeviceToHost));
+    std::vector<float4> lv;
+    if (h->p.compensated_positions)
+    {
+        lv.resize((size_t)h->n);
+        HIPCHK(h, hipMemcpy(lv.data(), h->plo[h->cur], sizeof(float4) * h->n, hipMemcpyDeviceToHost));
+    }
+    for (int64_t t = 0; t < h->n; ++t)
+    {
+        const int64_t i = h->perm[(size_t)t];
+        float4 p = pv[(size_t)t];
+        if (!lv.empty())
+        { // round hi + lo to the nearest float
+            p.x = (float)((double)p.x + lv[(size_t)t].x);
+            p.y = (float)((double)p.y + lv[(size_t)t].y);
+            p.z = (float)((double)p.z + lv[(size_t)t].z);
+        }
+        if (x3)
+        {
+            x3[3 * i] = p.x;
+            x3[3 * i + 1] = p.y;
+            x3[3 * i + 2] = p.z;
+        }
+        if (v3)
+        {
+            v3[3 * i] = vv[(size_t)t].x;
+            v3[3 * i + 1] = vv[(size_t)t].y;
+            v3[3 * i + 2] = vv[(size_t)t].z;
+        }
+    }
+    return BS_OK;
+}
+
+int bs_set_state(bs_handle *h, const float *x3, const float *v3)
+{
+    if (!h)
+        return BS_ERR_ARG;
+    if (!h->have_particles)
+        return fail(h, BS_ERR_STATE, "bs_set_state: no particles");
+    if (h->ready)
+    {
+        // pull the current device state so that a NULL argument keeps it
+        int rc = bs_get_state(h, h->x.data(), h->v.data());
+        if (rc)
+            return rc;
+    }
+    if (x3)
+        std::memcpy(h->x.data(), x3, sizeof(float) * 3 * h->n);
+    if (v3)
+        std::memcpy(h->v.data(), v3, sizeof(float) * 3 * h->n);
+    if (!h->ready)
+        return BS_OK;
+    h->cur = 0;
+    int rc = upload_positions(h);
+    if (rc)
+        return rc;
+    const int32_t one = 1;
+    HIPCHK(h, hipMemcpy(h->status, &one, sizeof one, hipMemcpyHostToDevice));
+    return BS_OK;
+}
+
+int bs_get_energies(bs_handle *h, double *energies)
+{
+    if (!h || !energies)
+        return BS_ERR_ARG;
+    std::memcpy(energies, h->energies, sizeof(h->energies));
+    return BS_OK;
+}
+
+int64_t bs_last_nonfinite_particle(const bs_handle *h) { return h ? h->nonfinite : -1; }
+
+int bs_get_counters(const bs_handle *h, int64_t *out6)
+{
+    if (!h || !out6)
+        return BS_ERR_ARG;
+    for (int k = 0; k < 6; ++k)
+        out6[k] = 0;
+    out6[0] = h->steps_done;
+    out6[4] = h->nedges;
+    if (h->ready)
+    {
+        int32_t st[4], ct[4];
+        if (hipMemcpy(st, h->status, sizeof st, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(ct, h->counters, sizeof ct, hipMemcpyDeviceToHost) != hipSuccess)
+            return BS_ERR_HIP;
+        out6[1] = ct[1];
+        out6[2] = st[3];
+        int64_t cap = 0;
+        for (const auto &l : h->lists)
+            cap = std::max<int64_t>(cap, l.cap);
+        out6[3] = cap;
+    }
+    return BS_OK;
+}
+
+int bs_profile(bs_handle *h, int32_t enable)
+{
+    if (!h)
+        return BS_ERR_ARG;
+    h->prof = enable != 0;
+    h->spans.clear();
+    h->evused = 0;
+    for (int c = 0; c < BS_PROF_NUM; ++c)
+    {
+        h->prof_ms[c] = 0.0;
+        h->prof_n[c] = 0;
+    }
+    return BS_OK;
+}
+
+int bs_get_profile(const bs_handle *h, double *ms, int64_t *launches)
+{
+    if (!h)
+        return BS_ERR_ARG;
+    for (int c = 0; c < BS_PROF_NUM; ++c)
+    {
+        if (ms)
+            ms[c] = h->prof_ms[c];
+        if (launches)
+            launches[c] = h->prof_n[c];
+    }
+    return BS_OK;
+}
+
+int bs_get_cells(bs_handle *h, int32_t kind, int64_t *ncells_out, int64_t *nmembers_out, int64_t *cell_of_particle,
+                 int32_t *cell_start, int32_t *sorted_idx, int64_t dims_out[3], double box_out[6])
+{
+    if (!h || kind < 0 || kind > 2)
+        return BS_ERR_ARG;
+    int rc = ensure_ready(h);
+    if (rc)
+        return rc;
+    const int32_t n = (int32_t)h->n;
+    // box over every particle, FP64 (box.hpp:37-69)
+    launch_box_reduce(h->pos[h->cur], n, h->box_part, h->box, h->box_blocks, nullptr, h->stream);
+    double bx[6];
+    HIPCHK(h, hipMemcpyAsync(bx, h->box, sizeof bx, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    const float cutoff = kind == BS_NS_ELECTROSTATIC ? h->p.electrostatic_cutoff
+                         : kind == BS_NS_HYDROPHOBIC ? h->p.hydrophobicity_cutoff
+                                                     : h->p.steric_cutoff;
+    const float r = cutoff + h->p.neighborskin; // NeighborSearch::_search_radius, float
+    int64_t d[3];
+    for (int c = 0; c < 3; ++c)
+        d[c] = (int64_t)(size_t)(std::ceil((bx[3 + c] - bx[c]) / r) + 1); // nsearch.hpp:352-354
+    const int64_t ncells = d[0] * d[1] * d[2];
+    if (ncells > INT32_MAX - 1)
+        return fail(h, BS_ERR_ARG, "bs_get_cells: %lld cells exceed the engine limit", (long long)ncells);
+    rc = ensure_cells(h, (int32_t)ncells);
+    if (rc)
+        return rc;
+    const uint32_t mask = kind == BS_NS_ELECTROSTATIC ? PF_CHARGED : kind == BS_NS_HYDROPHOBIC ? PF_HYDRO : 0u;
+    hipStream_t s = h->stream;
+    launch_fill_i32(h->counts, 0, ncells, nullptr, s);
+    launch_cell_ids(h->pos[h->cur], h->jprm, mask, n, h->box, (int32_t)d[0], (int32_t)d[1], (int32_t)d[2], (double)r,
+                    r, 1, h->cell_of, h->counts, nullptr, s);
+    launch_scan(h->counts, h->cstart, (int32_t)ncells, nullptr, s);
+    launch_fill_i32(h->cursor, 0, ncells, nullptr, s);
+    launch_scatter(h->cell_of, n, h->cstart, h->cursor, h->ctmp, nullptr, s);
+    launch_cell_sort(h->cell_of, n, h->cstart, h->ctmp, h->perm_d, h->cidx, nullptr, s);
+    std::vector<int32_t> cof((size_t)n), cst((size_t)ncells + 1), cid((size_t)n);
+    HIPCHK(h, hipMemcpyAsync(cof.data(), h->cell_of, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+    HIPCHK(h, hipMemcpyAsync(cst.data(), h->cstart, sizeof(int32_t) * (ncells + 1), hipMemcpyDeviceToHost, s));
+    HIPCHK(h, hipMemcpyAsync(cid.data(), h->cidx, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+    HIPCHK(h, hipStreamSynchronize(s));
+    const int64_t nmem = cst[(size_t)ncells];
+    if (ncells_out)
+        *ncells_out = ncells;
+    if (nmembers_out)
+        *nmembers_out = nmem;
+    if (dims_out)
+        std::memcpy(dims_out, d, sizeof d);
+    if (box_out)
+        std::memcpy(box_out, bx, sizeof bx);
+    if (cell_of_particle)
+        for (int32_t t = 0; t < n; ++t)
+            cell_of_particle[h->perm[(size_t)t]] = cof[(size_t)t];
+    if (cell_start)
+        std::memcpy(cell_start, cst.data(), sizeof(int32_t) * (ncells + 1));
+    if (sorted_idx)
+        for (int64_t t = 0; t < nmem; ++t)
+            sorted_idx[t] = h->perm[(size_t)cid[(size_t)t]];
+    return BS_OK;
+}
+
+int bs_get_pairs(bs_handle *h, int32_t kind, uint32_t *ij, int64_t cap, int64_t *count)
+{
+    if (!h || kind < 0 || kind > 2 || !count)
+        return BS_ERR_ARG;
+    int rc = ensure_ready(h);
+    if (rc)
+        return rc;
+    *count = 0;
+    const int term = kind_term(kind);
+    const List *lst = nullptr;
+    for (const auto &l : h->lists)
+        if (l.terms & term)
+            lst = &l;
+    if (!lst)
+        return BS_OK;
+    rebuild_chain(h, true);
+    const int32_t n = (int32_t)h->n;
+    const int64_t npad = (h->n + 63) / 64 * 64;
+    const int64_t maxp = (int64_t)n * lst->cap;
+    hipError_t e;
+    uint32_t *out = dalloc<uint32_t>((size_t)(2 * maxp), e);
+    HIPCHK(h, e);
+    int32_t *cnt = dalloc<int32_t>(1, e);
+    if (e != hipSuccess)
+    {
+        (void)hipFree(out);
+        return fail(h, BS_ERR_HIP, "hipMalloc failed");
+    }
+    const uint32_t mask = kind == BS_NS_ELECTROSTATIC ? PF_CHARGED : kind == BS_NS_HYDROPHOBIC ? PF_HYDRO : 0u;
+    const float rc_k = kind == BS_NS_ELECTROSTATIC ? h->p.electrostatic_cutoff
+                       : kind == BS_NS_HYDROPHOBIC ? h->p.hydrophobicity_cutoff
+                                                   : h->p.steric_cutoff;
+    launch_fill_i32(cnt, 0, 1, nullptr, h->stream);
+    launch_pairs_extract(h->pos[h->cur], h->jprm, lst->nbl, lst->nbn, h->perm_d, n, (int32_t)npad, rc_k,
+                         rc2(rc_k, 1.0 - 4e-6), rc2(rc_k, 1.0 + 4e-6), mask, mask, kind != BS_NS_STERIC ? 1 : 0, out,
+                         cnt, maxp, h->stream);
+    int32_t c = 0;
+    hipError_t e1 = hipMemcpyAsync(&c, cnt, sizeof c, hipMemcpyDeviceToHost, h->stream);
+    hipError_t e2 = hipStreamSynchronize(h->stream);
+    std::vector<uint32_t> v;
+    if (e1 == hipSuccess && e2 == hipSuccess && !ij)
+    { // count only
+        (void)hipFree(out);
+        (void)hipFree(cnt);
+        *count = c;
+        return BS_OK;
+    }
+    if (e1 == hipSuccess && e2 == hipSuccess)
+    {
+        v.resize(2 * (size_t)std::min<int64_t>(c, maxp));
+        e1 = hipMemcpy(v.data(), out, sizeof(uint32_t) * v.size(), hipMemcpyDeviceToHost);
+    }
+    (void)hipFree(out);
+    (void)hipFree(cnt);
+    if (e1 != hipSuccess || e2 != hipSuccess)
+        return fail(h, BS_ERR_HIP, "bs_get_pairs: copy failed");
+    const int64_t np = (int64_t)v.size() / 2;
+    std::vector<uint64_t> keys((size_t)np);
+    for (int64_t t = 0; t < np; ++t)
+        keys[(size_t)t] = (uint64_t)v[2 * t] << 32 | v[2 * t + 1];
+    std::sort(keys.begin(), keys.end());
+    *count = np;
+    if (ij)
+        for (int64_t t = 0; t < np && t < cap; ++t)
+        {
+            ij[2 * t] = (uint32_t)(keys[(size_t)t] >> 32);
+            ij[2 * t + 1] = (uint32_t)keys[(size_t)t];
+        }
+    return BS_OK;
+}
+
+} // extern "C"

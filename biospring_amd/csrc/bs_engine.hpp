@@ -1,0 +1,153 @@
+// bs_engine.hpp -- device data layout and kernel launchers of the engine.
+//
+// Layout in HBM (all arrays in the engine's internal particle order, a
+// spatial Morton order fixed at setup; perm[k] = original id of internal k):
+//   pos[2]   float4[N]  x, y, z, charge           ping-pong between steps
+//   plo[2]   float4[N]  low words of x, y, z      compensated positions only
+//   vel      float4[N]  vx, vy, vz, 0
+//   fnb      float4[N]  non-bonded force of the current step
+//   jprm     float4[N]  sqrt(radius), sqrt(eps), hydrophobicity, flags (bits)
+//   iprm     float4[N]  mass, sasa, transfer, radius
+//   xref     float4[N]  positions at the last neighbour-list build
+//   springs  CSR over particles, both directions, each row sorted by j:
+//            sp_ptr int32[N+1], sp_j int32[E] (bit31 = j static), sp_k float[E],
+//            sp_l0 float[E]
+//   lists    ELL column-major: nbl[t * npad + i] int32, nbn[i]
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace bs
+{
+
+// Particle flag bits kept in jprm.w (as raw int bits).
+constexpr uint32_t PF_STATIC = 1u;
+constexpr uint32_t PF_CHARGED = 2u;
+constexpr uint32_t PF_HYDRO = 4u;
+constexpr uint32_t PF_RIGID = 8u;
+
+// Term bits of a neighbour list.
+constexpr int TERM_STERIC = 1;
+constexpr int TERM_ELEC = 2;
+constexpr int TERM_HYDRO = 4;
+
+// Subset filters.
+constexpr uint32_t SUB_ALL = 0u;
+
+struct Consts
+{
+    double c_spring;  // kJ/mol/A -> Da A fs^-2
+    double c_elec;    // e^2/A^2 -> Da A fs^-2 (before 1/(4 pi eps))
+    double c_kinetic; // Da A^2 fs^-2 -> kJ/mol
+    double c_coulomb; // 1/(4 pi eps0)
+    double e_coul;    // Coulomb energy prefactor for q1 q2 / d (d in A), kJ/mol
+    double pi;
+};
+
+Consts make_consts();
+
+// Parameters of one pair kernel launch.
+struct PairArgs
+{
+    const float4 *pos;
+    const float4 *jprm;
+    const int32_t *nbl;
+    const int32_t *nbn;
+    float4 *fnb;
+    double *epart;      // [nblocks * 3] steric, elec, hydro
+    int32_t n, npad;
+    int32_t accumulate; // add to fnb instead of overwriting
+    float rc, rc2lo, rc2hi;
+    int32_t steric_mode;
+    float steric_scale;
+    float elec_scale, diel;
+    float hydro_scale;
+    uint32_t i_mask, j_mask; // required flag bits (PF_CHARGED / PF_HYDRO) or 0
+    Consts k;
+};
+
+struct SpringArgs
+{
+    const float4 *pos;
+    const float4 *plo;   // nullable
+    float4 *pos_out;     // integrate mode
+    float4 *plo_out;     // nullable
+    float4 *vel;
+    const float4 *fnb;   // nullable (no pair terms)
+    const float4 *fext;  // nullable
+    const float4 *iprm;
+    const float4 *jprm;
+    const float4 *xref;
+    const int32_t *sp_ptr;
+    const int32_t *sp_j;
+    const float *sp_k;
+    const float *sp_l0;
+    const int32_t *perm;
+    float4 *fout;        // forces-only mode
+    double *epart;       // [nblocks * 3] spring, kinetic, imp
+    int32_t *status;     // [0] rebuild flag, [1] nonfinite min id, [2] overflow
+    int32_t n;
+    int32_t spring_enable;
+    float spring_scale;
+    float dt;
+    int32_t visc_enable;
+    float visc;
+    int32_t imp_enable;
+    float imp_scale, imp_uo, imp_lo, imp_uc, imp_lc;
+    float half_skin2;    // (list_skin/2)^2, conservative
+    int32_t integrate;
+    Consts k;
+};
+
+struct ListArgs
+{
+    const float4 *pos;
+    const float4 *jprm;
+    const int32_t *cell_start; // [ncells + 1]
+    const int32_t *cell_idx;   // sorted members
+    const double *box;         // [6] current box (device)
+    const int32_t *sp_ptr;
+    const int32_t *sp_j;
+    int32_t *nbl;
+    int32_t *nbn;
+    int32_t *status;           // [2] overflow, [3] max count
+    int32_t n, npad, cap;
+    int32_t dims[3];
+    float cell_edge;           // minimum edge (list radius)
+    float r2list;
+    uint32_t i_mask, j_mask;
+    int32_t exclude_bonded;
+    const int32_t *gate;       // nullable: skip unless *gate != 0
+};
+
+// ---- launchers (bs_kernels.hip) ----
+void launch_box_reduce(const float4 *pos, int32_t n, double *partial, double *box, int32_t nblocks,
+                       const int32_t *gate, hipStream_t s);
+void launch_cell_ids(const float4 *pos, const float4 *jprm, uint32_t mask, int32_t n, const double *box,
+                     int32_t dims0, int32_t dims1, int32_t dims2, double edge_exact, float min_edge,
+                     int32_t exact_mode, int32_t *cell_of, int32_t *counts, const int32_t *gate, hipStream_t s);
+void launch_scan(const int32_t *counts, int32_t *start, int32_t ncells, const int32_t *gate, hipStream_t s);
+void launch_scatter(const int32_t *cell_of, int32_t n, const int32_t *start, int32_t *cursor, int32_t *tmp_idx,
+                    const int32_t *gate, hipStream_t s);
+void launch_cell_sort(const int32_t *cell_of, int32_t n, const int32_t *start, const int32_t *tmp_idx,
+                      const int32_t *key, int32_t *out_idx, const int32_t *gate, hipStream_t s);
+void launch_list_build(const ListArgs &a, hipStream_t s);
+void launch_pair(const PairArgs &a, int32_t elec_mode, int32_t hydro, int32_t nblocks, hipStream_t s);
+void launch_spring(const SpringArgs &a, int32_t group, int32_t nblocks, hipStream_t s);
+void launch_reduce_energies(const double *pair_part, int32_t npair_blocks, int32_t nlists,
+                            const double *spring_part, int32_t nspring_blocks, double *energies, hipStream_t s);
+void launch_prestep(int32_t *status, int32_t *step_counter, int32_t interval, hipStream_t s);
+void launch_finish_rebuild(const float4 *pos, float4 *xref, int32_t n, int32_t *status, int32_t *counters,
+                           hipStream_t s);
+void launch_clear_rebuild(int32_t *status, int32_t *counters, hipStream_t s);
+void launch_fill_i32(int32_t *p, int32_t v, int64_t n, const int32_t *gate, hipStream_t s);
+void launch_pairs_extract(const float4 *pos, const float4 *jprm, const int32_t *nbl, const int32_t *nbn,
+                          const int32_t *perm, int32_t n, int32_t npad, float rc, float rc2lo, float rc2hi,
+                          uint32_t i_mask, uint32_t j_mask, int32_t nonzero, uint32_t *out, int32_t *count,
+                          int64_t cap, hipStream_t s);
+
+constexpr int32_t BLOCK = 256;
+
+} // namespace bs

@@ -1,0 +1,38 @@
+// Default parameters: the reference's defaultConfiguration()
+// (src/configuration/Configuration.hpp:153-209) plus the engine options.
+#ifndef BIOSPRING_AMD_PARAMS_DEFAULTS_H
+#define BIOSPRING_AMD_PARAMS_DEFAULTS_H
+
+#include <string.h>
+
+#include "../../include/bs_api.h"
+
+static inline void bs_params_fill_defaults(bs_params *p)
+{
+    memset(p, 0, sizeof *p);
+    p->timestep = 0.01f;          /* sim.timestep */
+    p->neighborskin = 0.0f;       /* sim.neighborskin */
+    p->spring_enable = 0;
+    p->spring_scale = 1.0f;
+    p->steric_enable = 0;
+    p->steric_mode = BS_STERIC_LINEAR;
+    p->steric_cutoff = 1.0f;
+    p->steric_gridscale = 1.0f;
+    p->electrostatic_enable = 0;
+    p->electrostatic_cutoff = 16.0f;
+    p->electrostatic_scale = 1.0f;
+    p->electrostatic_dielectric = 1.0f;
+    p->dielectric_model = BS_DIELECTRIC_CONSTANT;
+    p->viscosity_enable = 0;
+    p->viscosity_value = 1.0f;
+    p->imp_enable = 0;
+    p->imp_scale = 1.0f;
+    p->hydrophobicity_enable = 0;
+    p->hydrophobicity_cutoff = 15.0f;
+    p->hydrophobicity_scale = 1.0f;
+    p->list_skin = 1.0f;
+    p->rebuild_interval = 0;
+    p->compensated_positions = 0;
+}
+
+#endif

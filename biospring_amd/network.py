@@ -1,0 +1,186 @@
+"""Python mirror of biospring::spn::SpringNetwork's step API over the C ABI.
+
+Method names follow the reference (src/spn/SpringNetwork.h:303-397):
+setup, computeForces, computeStep, run, getParticlePosition, setForce and the
+energy getters.  Every call goes through libbiospring_amd.so; there is no CPU
+path.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _abi
+from ._abi import check, f32, ptr, u32
+
+
+class SpringNetwork:
+    """One simulation on the current HIP device."""
+
+    def __init__(self, params=None):
+        L = _abi.lib()
+        self._L = L
+        self._h = ctypes.c_void_p()
+        self.params = params.copy() if params is not None else _abi.default_params()
+        code = L.bs_create(ctypes.byref(self.params), ctypes.byref(self._h))
+        if code != _abi.BS_OK:
+            raise _abi.EngineError(code, f"bs_create failed with code {code} (no gfx950 device?)")
+        self._ext = None
+        self.n = 0
+
+    # -- construction ---------------------------------------------------------
+    def setParticles(self, x, v=None, mass=None, charge=None, radius=None, epsilon=None,
+                     sasa=None, transfer=None, hydrophobicity=None, flags=None):
+        x = f32(x).reshape(-1)
+        n = x.size // 3
+        arrs = [f32(a) for a in (v, mass, charge, radius, epsilon, sasa, transfer, hydrophobicity)]
+        fl = None if flags is None else np.ascontiguousarray(flags, dtype=np.uint8)
+        c = ctypes
+        check(self._L.bs_set_particles(self._h, n, ptr(x, c.c_float),
+                                       *[ptr(a, c.c_float) for a in arrs], ptr(fl, c.c_uint8)),
+              self._h, "bs_set_particles")
+        self._keep = (x, arrs, fl)
+        self.n = n
+
+    def setSprings(self, id1, id2, equilibrium, stiffness):
+        a, b, l0, k = u32(id1), u32(id2), f32(equilibrium), f32(stiffness)
+        c = ctypes
+        check(self._L.bs_set_springs(self._h, a.size, ptr(a, c.c_uint32), ptr(b, c.c_uint32),
+                                     ptr(l0, c.c_float), ptr(k, c.c_float)), self._h, "bs_set_springs")
+
+    @classmethod
+    def from_system(cls, system, params=None):
+        net = cls(params if params is not None else system.params)
+        net.setParticles(*system.particle_arrays())
+        net.setSprings(*system.spring_arrays())
+        return net
+
+    def setParams(self, params):
+        self.params = params.copy()
+        check(self._L.bs_set_params(self._h, ctypes.byref(self.params)), self._h, "bs_set_params")
+
+    def setup(self):
+        check(self._L.bs_setup(self._h), self._h, "bs_setup")
+
+    # -- step -------------------------------------------------------------------
+    def setForce(self, forces):
+        """External forces (n,3) added before every subsequent step's force
+        evaluation (SpringNetwork::setForce); None clears them."""
+        self._ext = None if forces is None else f32(forces).reshape(-1)
+
+    def computeStep(self):
+        self.run(1)
+
+    def run(self, nsteps):
+        check(self._L.bs_step(self._h, int(nsteps), ptr(self._ext, ctypes.c_float)), self._h, "bs_step")
+
+    def computeForces(self):
+        """Total forces (n,3) and the six FP64 energies at the current state."""
+        F = np.zeros(3 * self.n, np.float32)
+        E = np.zeros(_abi.NUM_ENERGIES, np.float64)
+        check(self._L.bs_compute_forces(self._h, ptr(self._ext, ctypes.c_float), ptr(F, ctypes.c_float),
+                                        ptr(E, ctypes.c_double)), self._h, "bs_compute_forces")
+        return F.reshape(-1, 3), E
+
+    # -- state ------------------------------------------------------------------
+    def getState(self):
+        x = np.zeros(3 * self.n, np.float32)
+        v = np.zeros(3 * self.n, np.float32)
+        check(self._L.bs_get_state(self._h, ptr(x, ctypes.c_float), ptr(v, ctypes.c_float)), self._h,
+              "bs_get_state")
+        return x.reshape(-1, 3), v.reshape(-1, 3)
+
+    def setState(self, x=None, v=None):
+        xx = None if x is None else f32(x).reshape(-1)
+        vv = None if v is None else f32(v).reshape(-1)
+        check(self._L.bs_set_state(self._h, ptr(xx, ctypes.c_float), ptr(vv, ctypes.c_float)), self._h,
+              "bs_set_state")
+
+    def getParticlePosition(self, i):
+        return self.getState()[0][i]
+
+    def getEnergies(self):
+        E = np.zeros(_abi.NUM_ENERGIES, np.float64)
+        check(self._L.bs_get_energies(self._h, ptr(E, ctypes.c_double)), self._h, "bs_get_energies")
+        return E
+
+    def getSpringEnergy(self):
+        return self.getEnergies()[_abi.E_SPRING]
+
+    def getElectrostaticEnergy(self):
+        return self.getEnergies()[_abi.E_ELECTROSTATIC]
+
+    def getStericEnergy(self):
+        return self.getEnergies()[_abi.E_STERIC]
+
+    def getKineticEnergy(self):
+        return self.getEnergies()[_abi.E_KINETIC]
+
+    def getIMPEnergy(self):
+        return self.getEnergies()[_abi.E_IMP]
+
+    def getHydrophobicEnergy(self):
+        return self.getEnergies()[_abi.E_HYDROPHOBIC]
+
+    def counters(self):
+        c = np.zeros(6, np.int64)
+        check(self._L.bs_get_counters(self._h, ptr(c, ctypes.c_int64)), self._h, "bs_get_counters")
+        return {"steps": int(c[0]), "rebuilds": int(c[1]), "max_list": int(c[2]), "list_cap": int(c[3]),
+                "edges": int(c[4])}
+
+    def profile(self, enable=True):
+        """Per-kernel-class HIP-event timing of run() (resets the totals)."""
+        check(self._L.bs_profile(self._h, 1 if enable else 0), self._h, "bs_profile")
+
+    def getProfile(self):
+        ms = np.zeros(3, np.float64)
+        n = np.zeros(3, np.int64)
+        check(self._L.bs_get_profile(self._h, ptr(ms, ctypes.c_double), ptr(n, ctypes.c_int64)), self._h,
+              "bs_get_profile")
+        names = ("rebuild", "pair", "spring")
+        return {names[c]: {"ms": float(ms[c]), "launches": int(n[c])} for c in range(3)}
+
+    def lastNonfiniteParticle(self):
+        return int(self._L.bs_last_nonfinite_particle(self._h))
+
+    # -- parity hooks -------------------------------------------------------------
+    def getCells(self, kind):
+        c = ctypes
+        nc, nm = c.c_int64(), c.c_int64()
+        check(self._L.bs_get_cells(self._h, kind, c.byref(nc), c.byref(nm), None, None, None, None, None),
+              self._h, "bs_get_cells")
+        cell_of = np.zeros(self.n, np.int64)
+        start = np.zeros(nc.value + 1, np.int32)
+        idx = np.zeros(max(nm.value, 1), np.int32)
+        dims = np.zeros(3, np.int64)
+        box = np.zeros(6, np.float64)
+        check(self._L.bs_get_cells(self._h, kind, c.byref(nc), c.byref(nm), ptr(cell_of, c.c_int64),
+                                   ptr(start, c.c_int32), ptr(idx, c.c_int32), ptr(dims, c.c_int64),
+                                   ptr(box, c.c_double)), self._h, "bs_get_cells")
+        return {"cell_of": cell_of, "start": start.astype(np.int64), "members": idx[:nm.value].astype(np.int64),
+                "dims": dims, "box": box}
+
+    def countPairs(self, kind):
+        """Number of processed pairs of search `kind` at the current state."""
+        cnt = ctypes.c_int64()
+        check(self._L.bs_get_pairs(self._h, kind, None, 0, ctypes.byref(cnt)), self._h, "bs_get_pairs")
+        return int(cnt.value)
+
+    def getPairs(self, kind):
+        c = ctypes
+        cnt = c.c_int64()
+        check(self._L.bs_get_pairs(self._h, kind, None, 0, c.byref(cnt)), self._h, "bs_get_pairs")
+        out = np.zeros(2 * max(cnt.value, 1), np.uint32)
+        check(self._L.bs_get_pairs(self._h, kind, ptr(out, c.c_uint32), cnt.value, c.byref(cnt)), self._h,
+              "bs_get_pairs")
+        return out[:2 * cnt.value].reshape(-1, 2)
+
+    def close(self):
+        if self._h:
+            self._L.bs_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

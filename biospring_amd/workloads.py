@@ -1,0 +1,89 @@
+"""Seeded synthetic spring networks (host generator, libbsgen.so).
+
+Workload 0: 2,000-bead CA elastic network (chain in a 30 A sphere, springs
+< 12 A, linear steric 4 A, Coulomb 16 A, 20% charged).  Workload 1: 250,000
+beads in a 140 A cube (springs < 8 A, Amber 12-6 + Coulomb with eps(r) = r,
+12 A cutoffs).  Workload 2: IMPALA CG beads.  See biospring_amd/csrc/bs_gen.cpp.
+"""
+import ctypes
+import hashlib
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _abi
+
+
+@dataclass
+class System:
+    x: np.ndarray
+    v: np.ndarray
+    mass: np.ndarray
+    charge: np.ndarray
+    radius: np.ndarray
+    eps: np.ndarray
+    sasa: np.ndarray
+    transfer: np.ndarray
+    hydro: np.ndarray
+    flags: np.ndarray
+    id1: np.ndarray
+    id2: np.ndarray
+    l0: np.ndarray
+    k: np.ndarray
+    params: _abi.Params = field(default_factory=_abi.Params)
+
+    @property
+    def n(self):
+        return int(self.mass.shape[0])
+
+    @property
+    def m(self):
+        return int(self.id1.shape[0])
+
+    def digest(self):
+        h = hashlib.sha256()
+        for a in (self.x, self.v, self.mass, self.charge, self.radius, self.eps, self.sasa,
+                  self.transfer, self.hydro, self.flags, self.id1, self.id2, self.l0, self.k):
+            h.update(np.ascontiguousarray(a).tobytes())
+        return h.hexdigest()[:16]
+
+    def particle_arrays(self):
+        return (self.x, self.v, self.mass, self.charge, self.radius, self.eps, self.sasa,
+                self.transfer, self.hydro, self.flags)
+
+    def spring_arrays(self):
+        return (self.id1, self.id2, self.l0, self.k)
+
+
+def params(config):
+    p = _abi.Params()
+    _abi.gen().bsg_params(config, ctypes.byref(p))
+    return p
+
+
+def generate(config, seed=1, n=0):
+    """Generates workload `config` with `seed`; n > 0 overrides the bead count
+    (same density)."""
+    G = _abi.gen()
+    h = G.bsg_generate(config, seed, n)
+    if not h:
+        raise ValueError(f"unknown workload {config}")
+    try:
+        nn = G.bsg_num_particles(h)
+        m = G.bsg_num_springs(h)
+        f = lambda k=1: np.zeros(nn * k, np.float32)
+        s = System(x=f(3), v=f(3), mass=f(), charge=f(), radius=f(), eps=f(), sasa=f(),
+                   transfer=f(), hydro=f(), flags=np.zeros(nn, np.uint8),
+                   id1=np.zeros(m, np.uint32), id2=np.zeros(m, np.uint32),
+                   l0=np.zeros(m, np.float32), k=np.zeros(m, np.float32))
+        P = _abi.ptr
+        c = ctypes
+        G.bsg_copy(h, P(s.x, c.c_float), P(s.v, c.c_float), P(s.mass, c.c_float),
+                   P(s.charge, c.c_float), P(s.radius, c.c_float), P(s.eps, c.c_float),
+                   P(s.sasa, c.c_float), P(s.transfer, c.c_float), P(s.hydro, c.c_float),
+                   P(s.flags, c.c_uint8), P(s.id1, c.c_uint32), P(s.id2, c.c_uint32),
+                   P(s.l0, c.c_float), P(s.k, c.c_float))
+    finally:
+        G.bsg_free(h)
+    s.params = params(config)
+    return s

@@ -1,0 +1,248 @@
+/*
+ * bs_api.h -- C ABI of the MI355X force-and-integrate engine for BioSpring
+ * spring networks.
+ *
+ * The engine replaces the per-step path of biospring::spn::SpringNetwork
+ * (src/spn/SpringNetwork.cpp:63-268): spring forces, the non-bonded pair
+ * forces found through the cell-list search (src/nsearch.hpp:155-436),
+ * viscosity, IMPALA membrane forces and the viscous semi-implicit Euler
+ * integrator.  A host-side subclass of SpringNetwork forwards its virtual
+ * step methods here; every entry point takes plain pointers and sizes, never
+ * throws, and reports failure through its return code plus bs_last_error().
+ *
+ * Units are the reference's: Angstrom, Dalton, femtosecond, kJ/mol, e
+ * (src/forcefield/constants.hpp:4-27).  Arrays of 3-vectors are packed
+ * x0,y0,z0,x1,... in float (positions, velocities, forces).
+ *
+ * Threading: calls on one handle must be serialised by the caller (the
+ * reference's main-thread model, src/spn/SpringNetwork.cpp:344-365).
+ * Different handles are independent.
+ */
+#ifndef BIOSPRING_AMD_BS_API_H
+#define BIOSPRING_AMD_BS_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- return codes ------------------------------------------------------ */
+#define BS_OK             0
+#define BS_ERR_ARG        1  /* bad argument: std::invalid_argument / out_of_range in the reference */
+#define BS_ERR_NONFINITE  2  /* non-finite position: logging::die in SpringNetwork.cpp:222-229 */
+#define BS_ERR_HIP        3  /* HIP runtime failure */
+#define BS_ERR_STATE      4  /* call out of order (e.g. step before setup), capacity exceeded */
+#define BS_ERR_NOGPU      5  /* no usable gfx950 device */
+
+/* ---- steric modes (SpringNetwork::_setupForceField, SpringNetwork.cpp:690-710) */
+#define BS_STERIC_LINEAR      0  /* "linear" and any unknown mode */
+#define BS_STERIC_AMBER_12_6  1  /* "lennard-jones-12-6Amber" */
+#define BS_STERIC_LEWITT_8_6  2  /* "lennard-jones-8-6Lewitt" */
+#define BS_STERIC_ZACHARIAS_8_6 3 /* "lennard-jones-8-6Zacharias" */
+
+/* ---- dielectric models ------------------------------------------------- */
+#define BS_DIELECTRIC_CONSTANT 0  /* reference: electrostatic.hpp:16-49 */
+#define BS_DIELECTRIC_DISTANCE 1  /* extension: eps(r) = eps * r (r in Angstrom); no reference counterpart */
+
+/* ---- per-particle flag bits (bs_set_particles) -------------------------- */
+#define BS_FLAG_STATIC  0x01u    /* Particle::isStatic (Particle.h:57) */
+
+/* ---- energy slots, FP64 (SpringNetwork::Energies, SpringNetwork.h:51-69) */
+#define BS_E_SPRING        0
+#define BS_E_ELECTROSTATIC 1
+#define BS_E_STERIC        2
+#define BS_E_KINETIC       3
+#define BS_E_IMP           4
+#define BS_E_HYDROPHOBIC   5
+#define BS_NUM_ENERGIES    6
+
+/* ---- neighbour-search kinds (SpringNetwork::NeighborSearch, SpringNetwork.h:79-100) */
+#define BS_NS_STERIC        0  /* all particles, steric.cutoff */
+#define BS_NS_ELECTROSTATIC 1  /* charged particles, electrostatic.cutoff */
+#define BS_NS_HYDROPHOBIC   2  /* hydrophobic particles, hydrophobicity.cutoff */
+
+/*
+ * Simulation parameters.  Field names and defaults follow the reference's
+ * Configuration (src/configuration/Configuration.hpp:153-209); the msp key
+ * is given beside each field.  bs_default_params() fills the defaults.
+ */
+typedef struct bs_params {
+    float timestep;               /* simulation.timestep (fs), default 0.01 */
+    float neighborskin;           /* simulation.neighborskin (A), default 0 */
+
+    int32_t spring_enable;        /* spring.enable, default 0 */
+    float spring_scale;           /* spring.scale, default 1 */
+
+    int32_t steric_enable;        /* steric.enable, default 0 */
+    int32_t steric_mode;          /* steric.mode -> BS_STERIC_*, default linear */
+    float steric_cutoff;          /* steric.cutoff (A), default 1 */
+    float steric_gridscale;       /* steric.gridscale, default 1 */
+
+    int32_t electrostatic_enable; /* electrostatic.enable, default 0 */
+    float electrostatic_cutoff;   /* electrostatic.cutoff (A), default 16 */
+    float electrostatic_scale;    /* electrostatic.scale, default 1 */
+    float electrostatic_dielectric; /* electrostatic.dielectric, default 1 */
+    int32_t dielectric_model;     /* BS_DIELECTRIC_*, default constant */
+
+    int32_t viscosity_enable;     /* viscosity.enable, default 0 */
+    float viscosity_value;        /* viscosity.value (Da/fs), default 1 */
+
+    int32_t imp_enable;           /* imp.enable, default 0 */
+    float imp_scale;              /* imp.scale, default 1 */
+    float imp_upper_offset;       /* ForceField::_impuppermebraneoffset (A), default 0 */
+    float imp_lower_offset;       /* ForceField::_implowermembraneoffset (A), default 0 */
+    float imp_upper_curvature;    /* ForceField::_uppermembtubecurv (1/A), default 0 */
+    float imp_lower_curvature;    /* ForceField::_lowermembtubecurv (1/A), default 0 */
+
+    int32_t hydrophobicity_enable; /* hydrophobicity.enable, default 0 */
+    float hydrophobicity_cutoff;  /* hydrophobicity.cutoff (A), default 15 */
+    float hydrophobicity_scale;   /* hydrophobicity.scale, default 1 */
+
+    /* Engine options with no reference counterpart. */
+    float list_skin;              /* Verlet-list margin (A) added to the largest cutoff; the list is
+                                     rebuilt whenever a particle drifts more than list_skin/2.  Pair
+                                     sets are exact for any value >= 0.  Default 1.0 */
+    int32_t rebuild_interval;     /* additionally rebuild every k steps (0 = only on drift). Default 0 */
+    int32_t compensated_positions; /* keep a float low word per coordinate (two-sum), default 0 */
+} bs_params;
+
+typedef struct bs_handle bs_handle;
+
+/* ---- lifetime ------------------------------------------------------------ */
+
+/* Fills *p with the reference defaults (Configuration.hpp:153-209). */
+void bs_default_params(bs_params *p);
+
+/* Creates a simulation on the current HIP device.  Replaces constructing a
+ * SpringNetwork (SpringNetwork.h:102-112). */
+int bs_create(const bs_params *p, bs_handle **out);
+
+/* Frees every device buffer and the stream; h may be NULL. */
+int bs_destroy(bs_handle *h);
+
+/* Message of the last failing call on h (static text if h is NULL). */
+const char *bs_last_error(const bs_handle *h);
+
+/* Number of particles / springs currently loaded. */
+int64_t bs_num_particles(const bs_handle *h);
+int64_t bs_num_springs(const bs_handle *h);
+
+/* ---- state upload ------------------------------------------------------ */
+
+/* Replaces SpringNetwork::addParticle (SpringNetwork.cpp:498-527) for a whole
+ * system.  Particle ids are the array indices.  Any pointer except x3 may be
+ * NULL, which selects the ParticleProperty defaults (ParticleProperty.h:20-25):
+ * v=0, mass 1, charge 0, radius 1, epsilon 0, sasa 0, transfer 0,
+ * hydrophobicity 0, flags 0 (dynamic).  isCharged = (charge != 0) and
+ * isHydrophobic = (hydrophobicity != 0) as in ParticleProperty.cpp:8-35.
+ * Must precede bs_set_springs, like the reference's addParticle. */
+int bs_set_particles(bs_handle *h, int64_t n, const float *x3, const float *v3,
+                     const float *mass, const float *charge, const float *radius,
+                     const float *epsilon, const float *sasa, const float *transfer,
+                     const float *hydrophobicity, const uint8_t *flags);
+
+/* Replaces SpringNetwork::addSpring (SpringNetwork.cpp:452-486) for a whole
+ * set: spring s links id1[s] and id2[s] with rest length equilibrium[s] (A)
+ * and stiffness[s] (kJ/mol/A^2).  Ids out of range or equal -> BS_ERR_ARG;
+ * a repeated pair is ignored, like the reference's isInSpringNeighbors test.
+ * The force acts along x[id2]-x[id1] exactly as Spring::computeForce
+ * (Spring.cpp:20-34). */
+int bs_set_springs(bs_handle *h, int64_t m, const uint32_t *id1, const uint32_t *id2,
+                   const float *equilibrium, const float *stiffness);
+
+/* Replaces SpringNetwork::setParams via setup(conf) (SpringNetwork.cpp:646-661).
+ * Takes effect at the next step boundary. */
+int bs_set_params(bs_handle *h, const bs_params *p);
+
+/* Builds device structures (spring CSR, exclusions, neighbour lists).
+ * Replaces SpringNetwork::setup (SpringNetwork.cpp:646-661). */
+int bs_setup(bs_handle *h);
+
+/* ---- step ---------------------------------------------------------------- */
+
+/* Advances nsteps steps of SpringNetwork::computeStep (SpringNetwork.cpp:251-268):
+ * forces (springs, pairs, viscosity, IMPALA), viscous Euler integration of every
+ * dynamic particle, neighbour upkeep.  ext_force3 (nullable, 3n floats) is added
+ * to the particle forces on every step before the force evaluation, like
+ * SpringNetwork::setForce from Interactor::syncSystemStateData (Interactor.cpp:23-31).
+ * Returns BS_ERR_NONFINITE if a position became non-finite; the offending
+ * particle is then reported by bs_last_nonfinite_particle. */
+int bs_step(bs_handle *h, int32_t nsteps, const float *ext_force3);
+
+/* SpringNetwork::computeForces (SpringNetwork.cpp:244-249) plus the force
+ * part of computeParticleForces, without integrating: writes the total force
+ * on every particle (3n floats, nullable; static particles get 0) and the
+ * energies (BS_NUM_ENERGIES doubles, nullable; kinetic slot = 0). */
+int bs_compute_forces(bs_handle *h, const float *ext_force3, float *force3, double *energies);
+
+/* ---- readback -------------------------------------------------------------- */
+
+/* Positions and velocities (3n floats each, either nullable).  Replaces
+ * SpringNetwork::getParticlePosition (SpringNetwork.cpp:430-436) for all
+ * particles at once.  Synchronising. */
+int bs_get_state(bs_handle *h, float *x3, float *v3);
+
+/* Overwrites positions/velocities of all particles (nullable each). */
+int bs_set_state(bs_handle *h, const float *x3, const float *v3);
+
+/* Energies of the last step (SpringNetwork::getSpringEnergy ... getters,
+ * SpringNetwork.h:157-162), accumulated in FP64. */
+int bs_get_energies(bs_handle *h, double *energies);
+
+/* Index of the first particle (lowest id) found with a non-finite position by
+ * the last bs_step, or -1. */
+int64_t bs_last_nonfinite_particle(const bs_handle *h);
+
+/* Counters: [0] steps done, [1] neighbour-list rebuilds, [2] max list length,
+ * [3] list capacity, [4] directed spring edges, [5] pairs in the last list. */
+int bs_get_counters(const bs_handle *h, int64_t *out6);
+
+/* ---- profiling --------------------------------------------------------------- */
+
+/* Kernel classes timed with HIP events on the handle's stream. */
+#define BS_PROF_REBUILD 0  /* neighbour-upkeep chain (box, binning, list build) */
+#define BS_PROF_PAIR    1  /* non-bonded pair kernels */
+#define BS_PROF_SPRING  2  /* spring gather + integrator (or force epilogue) */
+#define BS_PROF_NUM     3
+
+/* Turns per-kernel-class event timing of bs_step on (1) or off (0) and resets
+ * the totals.  Counterpart of the reference's wall-clock timeit::Profiler
+ * (src/timeit.hpp:16-168; "Average framerate", SpringNetwork.cpp:305-308). */
+int bs_profile(bs_handle *h, int32_t enable);
+
+/* Accumulated milliseconds and launch counts per BS_PROF_* class. */
+int bs_get_profile(const bs_handle *h, double *ms, int64_t *launches);
+
+/* ---- parity hooks ------------------------------------------------------------- */
+
+/* Rebuilds the reference cell list of search `kind` (BS_NS_*) on the GPU at the
+ * current positions, with the reference definitions: bounding box over all
+ * particles in FP64 (box.hpp:37-69), edge = cutoff + neighborskin,
+ * n_a = size_t(ceil(L_a/edge) + 1), cell = size_t((x - min)/edge) clamped
+ * (nsearch.hpp:277-293,342-390), id = cx + cy*nx + cz*nx*ny.
+ * Then copies out (each nullable):
+ *   cell_of_particle[n]  cell id per particle, -1 when not in this search;
+ *   cell_start[ncells+1] CSR offsets into sorted_idx;
+ *   sorted_idx[members]  particle ids per cell, ascending as in the reference;
+ *   dims[3], box[6] (min xyz, max xyz).
+ * *ncells and *nmembers receive the sizes; call first with NULL arrays. */
+int bs_get_cells(bs_handle *h, int32_t kind, int64_t *ncells, int64_t *nmembers,
+                 int64_t *cell_of_particle, int32_t *cell_start, int32_t *sorted_idx,
+                 int64_t dims[3], double box[6]);
+
+/* Processed non-bonded pairs of search `kind` at the current positions: the
+ * pairs the reference's Particle::add{Steric,Electrostatic,Hydrophobicity}Force
+ * (Particle.cpp:191-330,368-429) evaluates -- i dynamic, j static or j > i,
+ * not spring-bonded when springs are enabled, FP64 distance < cutoff
+ * (nsearch.hpp:230) and FP32 distance < cutoff (Particle.cpp:400-402), plus
+ * distance != 0 for electrostatic/hydrophobic.  Written as (min,max) id pairs
+ * sorted ascending.  *count receives the number; ij may be NULL to query it. */
+int bs_get_pairs(bs_handle *h, int32_t kind, uint32_t *ij, int64_t cap, int64_t *count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BIOSPRING_AMD_BS_API_H */

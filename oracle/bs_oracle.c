@@ -1,0 +1,1621 @@
+/*
+ * bs_oracle.c -- CPU restatement of BioSpring's per-step force-and-integrate
+ * path.  TEST INFRASTRUCTURE ONLY: the checker for the HIP engine.  Only the
+ * tests, the smoke check and the benchmark's CPU-baseline leg load it; the
+ * engine never calls into it.
+ *
+ * Two arithmetic modes share one enumeration of particles, springs and pairs:
+ *
+ *   ORC_F32  float-faithful: every float/double rounding step, loop order and
+ *            serial reduction order of the reference CPU path, so that it can
+ *            be compared bit-for-bit with the reference itself (oracle/_ref).
+ *   ORC_F64  the same algorithm in double precision: the FP64 oracle the GPU
+ *            forces and energies are held to (1e-5 relative).
+ *
+ * Pair decisions always follow the reference's two-stage test on float
+ * positions (nsearch.hpp:230 then Particle.cpp:400-402), in both modes, so the
+ * FP64 oracle evaluates exactly the reference's pair set.
+ *
+ * Reference files followed (paths relative to the BioSpring source root):
+ *   src/spn/SpringNetwork.cpp:63-241      step order, serial reductions
+ *   src/spn/Spring.cpp:20-39              spring force
+ *   src/spn/Particle.cpp:70-128,191-330,368-429  pair loops, viscosity, Euler
+ *   src/forcefield/ForceField.cpp:15-106  scales, IMPALA sign
+ *   src/forcefield/energy/{spring,steric,electrostatic,hydrophobic,imp}.hpp energy / force-module formulas
+ *   src/forcefield/constants.hpp:36-118   unit constants
+ *   src/forcefield/CombinationRules.hpp:18-31
+ *   src/nsearch.hpp:155-436, src/box.hpp:37-69, src/measure.hpp:163-196
+ *   src/Vector3f.h:45-144, src/Vector3f.cpp:9-11
+ *
+ * Parity status of each part is pinned by tests/test_oracle_*.py (golden
+ * vectors from the compiled reference, closed forms, finite differences,
+ * brute-force pair sets).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "../include/bs_api.h"
+
+#define ORC_F32 0
+#define ORC_F64 1
+
+/* ---------------------------------------------------------------------------
+ * Unit constants, evaluated in double in the same order as constants.hpp.
+ * ------------------------------------------------------------------------- */
+static const double K_KJ_TO_J = 1.0E3;
+static const double K_J_TO_KJ = 1.0E-3;
+static const double K_E_TO_C = 1.60217653E-19;
+static const double K_A_TO_M = 1.0E-10;
+static const double K_M_TO_A = 1.0E10;
+static const double K_DA_TO_KG = 1.66054E-27;
+static const double K_S_TO_FS = 1.0E15;
+static const double K_FS_TO_S = 1.0E-15;
+static const double K_PI = 3.1415927;
+static const double K_EPS0 = 8.854187817E-12;
+static const double K_NA = 6.02214169E23;
+static const double K_MIN_ELEC = 0.001;
+static const double K_MIN_VDW = 0.1;
+static const float K_IMP_ALPHA = 1.99f;
+static const float K_IMP_Z0 = 15.75f;
+static const float K_IMP_ALIP = -0.018f;
+
+static double c_coulomb(void) { return 1.0 / (4.0 * K_PI * K_EPS0); }
+static double c_newton(void) { return K_M_TO_A * (1.0 / K_DA_TO_KG) * K_FS_TO_S * K_FS_TO_S; }
+static double c_spring(void) { return K_KJ_TO_J / K_NA * K_M_TO_A * c_newton(); }
+static double c_elec(void) { return c_newton() * (K_E_TO_C * K_E_TO_C) / (K_EPS0 * K_A_TO_M * K_A_TO_M); }
+static double c_kinetic(void)
+{
+    return (K_A_TO_M * K_S_TO_FS) * (K_A_TO_M * K_S_TO_FS) * K_NA * K_DA_TO_KG * K_J_TO_KJ;
+}
+
+/* exported for tests: [spring, elec, kinetic, coulomb] */
+void orc_constants(double out[4])
+{
+    out[0] = c_spring();
+    out[1] = c_elec();
+    out[2] = c_kinetic();
+    out[3] = c_coulomb();
+}
+
+/* ---------------------------------------------------------------------------
+ * Geometry primitives with the reference rounding (Vector3f.cpp:9-11 ->
+ * measure.hpp:188-196; Vector3f.h:45-60).
+ * ------------------------------------------------------------------------- */
+static inline float norm_f(float x, float y, float z)
+{
+    const double dx = x, dy = y, dz = z;
+    if (fabs(dx) < 1e-40 && fabs(dy) < 1e-40 && fabs(dz) < 1e-40)
+        return 0.0f;
+    return (float)sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+static inline void normalize_f(float v[3])
+{
+    const float nrm = norm_f(v[0], v[1], v[2]);
+    if ((double)nrm > 1.0E-40) {
+        v[0] /= nrm;
+        v[1] /= nrm;
+        v[2] /= nrm;
+    } else {
+        v[0] = v[1] = v[2] = 0.0f;
+    }
+}
+
+static inline double norm_d(double x, double y, double z)
+{
+    if (fabs(x) < 1e-40 && fabs(y) < 1e-40 && fabs(z) < 1e-40)
+        return 0.0;
+    return sqrt(x * x + y * y + z * z);
+}
+
+static inline void normalize_d(double v[3])
+{
+    const double nrm = norm_d(v[0], v[1], v[2]);
+    if (nrm > 1.0E-40) {
+        v[0] /= nrm;
+        v[1] /= nrm;
+        v[2] /= nrm;
+    } else {
+        v[0] = v[1] = v[2] = 0.0;
+    }
+}
+
+/* ---------------------------------------------------------------------------
+ * Force-field formulas, float-faithful (energy/{spring,steric,electrostatic,hydrophobic,imp}.hpp).  The steric functions
+ * are compiled in the reference TU that sees `using namespace std`
+ * (reduce/ParticleGroup.h:9 before forcefield headers in SpringNetwork.cpp),
+ * so their pow(float, float) is powf; ForceField.cpp's IMPALA and
+ * hydrophobic formulas call the double exp/pow.
+ * ------------------------------------------------------------------------- */
+static inline float spring_energy_f(float d, float k, float l0)
+{
+    const float dv = d - l0; /* spring.hpp:17-21 */
+    return (float)(0.5 * k * dv * dv);
+}
+
+static inline float spring_fm_f(float d, float k, float l0)
+{
+    float fm = k * (d - l0); /* spring.hpp:24-29 */
+    fm = (float)(fm * c_spring());
+    return fm;
+}
+
+static float steric_energy_f(int mode, float ri, float rj, float ei, float ej, float d)
+{
+    switch (mode) {
+    case BS_STERIC_AMBER_12_6: { /* steric.hpp:60-72 */
+        if (d < K_MIN_VDW)
+            return 0.0f;
+        const float eij = (float)sqrt((double)(ei * ej));
+        const float rij = (float)sqrt((double)(ri * rj));
+        const float rep = eij * powf(rij / d, 12.0f);
+        const float att = (float)(-eij * 2.0 * powf(rij / d, 6.0f));
+        return rep + att;
+    }
+    case BS_STERIC_LEWITT_8_6: { /* steric.hpp:94-106 */
+        if (d < K_MIN_VDW)
+            return 0.0f;
+        const float eij = (float)sqrt((double)(ei * ej));
+        const float rij = (float)sqrt((double)(ri * rj));
+        const float rep = (float)(eij * 3.0 * powf(rij / d, 8.0f));
+        const float att = (float)(-eij * 4.0 * powf(rij / d, 6.0f));
+        return rep + att;
+    }
+    case BS_STERIC_ZACHARIAS_8_6: { /* steric.hpp:129-141 */
+        if (d < K_MIN_VDW)
+            return 0.0f;
+        const float eij = ei * ej;
+        const float rij = ri * rj;
+        const float rep = eij * powf(rij / d, 8.0f);
+        const float att = -eij * powf(rij / d, 6.0f);
+        return rep + att;
+    }
+    default: { /* linear, steric.hpp:27-36 */
+        const float eq = ri + rj;
+        const float dv = d - eq;
+        if (dv > 0)
+            return 0.0f;
+        return (float)(0.5 * 1.0f * dv * dv);
+    }
+    }
+}
+
+static float steric_fm_f(int mode, float ri, float rj, float ei, float ej, float d)
+{
+    switch (mode) {
+    case BS_STERIC_AMBER_12_6: { /* steric.hpp:74-87 */
+        if (d < K_MIN_VDW)
+            return 0.0f;
+        const float eij = (float)sqrt((double)(ei * ej));
+        const float rij = (float)sqrt((double)(ri * rj));
+        const float rep = (float)(-eij * 12.0 * (powf(rij, 12.0f) / powf(d, 13.0f)));
+        const float att = (float)(eij * 2.0 * 6.0 * (powf(rij, 6.0f) / powf(d, 7.0f)));
+        const float fm = rep + att;
+        return (float)(fm * c_spring());
+    }
+    case BS_STERIC_LEWITT_8_6: { /* steric.hpp:108-122 */
+        if (d < K_MIN_VDW)
+            return 0.0f;
+        const float eij = (float)sqrt((double)(ei * ej));
+        const float rij = (float)sqrt((double)(ri * rj));
+        const float rep = (float)(-eij * 3.0 * 8.0 * (powf(rij, 8.0f) / powf(d, 9.0f)));
+        const float att = (float)(eij * 4.0 * 6.0 * (powf(rij, 6.0f) / powf(d, 7.0f)));
+        const float fm = rep + att;
+        return (float)(fm * c_spring());
+    }
+    case BS_STERIC_ZACHARIAS_8_6: { /* steric.hpp:143-157 */
+        if (d < K_MIN_VDW)
+            return 0.0f;
+        const float eij = ei * ej;
+        const float rij = ri * rj;
+        const float rep = (float)(-eij * 8.0 * (powf(rij, 8.0f) / powf(d, 9.0f)));
+        const float att = (float)(eij * 6.0 * (powf(rij, 6.0f) / powf(d, 7.0f)));
+        const float fm = rep + att;
+        return (float)(fm * c_spring());
+    }
+    default: { /* linear, steric.hpp:39-49 */
+        const float eq = ri + rj;
+        const float dv = d - eq;
+        if (dv > 0)
+            return 0.0f;
+        const float fm = -1.0f * fabsf(dv);
+        return (float)(fm * c_spring());
+    }
+    }
+}
+
+/* Coulomb, electrostatic.hpp:16-49.  With the distance-dependent dielectric
+ * extension the relative permittivity becomes dielectric * d (d in A): the
+ * energy uses it in place of dielectric, and the force module, dE/dd, gains a
+ * factor 2. */
+static inline float elec_energy_f(float q1, float q2, float d, float diel, int ddd)
+{
+    if (d < K_MIN_ELEC)
+        return 0.0f;
+    const double k = c_coulomb();
+    const double c1 = q1 * K_E_TO_C;
+    const double c2 = q2 * K_E_TO_C;
+    const float eps = ddd ? diel * d : diel;
+    const float dm = (float)(d * K_A_TO_M);
+    double e = k * (c1 * c2) / (eps * dm);
+    e = e * K_NA;
+    e = e * 1.0E-3;
+    return (float)e;
+}
+
+static inline float elec_fm_f(float q1, float q2, float d, float diel, int ddd)
+{
+    if (d < K_MIN_ELEC)
+        return 0.0f;
+    const float eps = ddd ? diel * d : diel;
+    float fm = (float)(-(q1 * q2) / (4.0 * K_PI * eps * d * d));
+    fm = (float)(fm * c_elec());
+    if (ddd)
+        fm = 2.0f * fm;
+    return fm;
+}
+
+/* hydrophobic.hpp:20-39 (energy carries N_A*1e-3, the force does not). */
+static inline float hydro_energy_f(float h1, float h2, float d)
+{
+    double e = -(h1 * h2) * exp((double)-d);
+    e = e * K_NA;
+    e = e * 1.0E-3;
+    return (float)e;
+}
+
+static inline float hydro_fm_f(float h1, float h2, float d)
+{
+    float fm = (float)((h1 * h2) * exp((double)-d));
+    fm = (float)(fm * c_spring());
+    return fm;
+}
+
+/* IMPALA, imp.hpp:29-159, float-faithful. */
+typedef struct {
+    float up_off, lo_off, up_curv, lo_curv;
+} imp_geom;
+
+static inline int sgn_f(float c) { return (c > 0.0f) - (c < 0.0f); }
+
+static void imp_zprime_f(const imp_geom *g, float x, float y, float z, float *z_up, float *z_lo,
+                         float vup[3], float vlo[3], int *s_up, int *s_lo, float *r_up, float *r_lo)
+{
+    const float rup = g->up_curv == 0.0f ? 1000000.0f : fabsf(1 / g->up_curv);
+    const float rlo = g->lo_curv == 0.0f ? 1000000.0f : fabsf(1 / g->lo_curv);
+    const int sup = sgn_f(g->up_curv);
+    const int slo = sgn_f(g->lo_curv);
+    const float cup = g->up_off - sup * rup;
+    const float clo = -g->lo_off - slo * rlo;
+    vup[0] = x - 0.0f;
+    vup[1] = y - y;
+    vup[2] = z - cup;
+    vlo[0] = x - 0.0f;
+    vlo[1] = y - y;
+    vlo[2] = z - clo;
+    const float nup = norm_f(vup[0], vup[1], vup[2]);
+    const float nlo = norm_f(vlo[0], vlo[1], vlo[2]);
+    *z_up = sup == 0 ? z : (z > cup ? sup * nup + g->up_off - rup : -sup * nup + g->up_off - rup);
+    *z_lo = slo == 0 ? z : (z > clo ? slo * nlo - g->lo_off - rlo : -slo * nlo - g->lo_off - rlo);
+    *s_up = sup;
+    *s_lo = slo;
+    *r_up = rup;
+    *r_lo = rlo;
+}
+
+static inline int imp_is_flat(const imp_geom *g)
+{
+    return g->up_off == 0.0f && g->lo_off == 0.0f && g->up_curv == 0.0f && g->lo_curv == 0.0f;
+}
+
+static float imp_energy_f(const imp_geom *g, float x, float y, float z, float sasa, float tr)
+{
+    float zu, zl, vu[3], vl[3], ru, rl;
+    int su, sl;
+    imp_zprime_f(g, x, y, z, &zu, &zl, vu, vl, &su, &sl, &ru, &rl);
+    const double cz_up = 0.5 - 1.0 / (1.0 + exp((double)(K_IMP_ALPHA * (fabsf(zu - g->up_off) - K_IMP_Z0))));
+    const double cz_lo = 0.5 - 1.0 / (1.0 + exp((double)(K_IMP_ALPHA * (fabsf(zl + g->lo_off) - K_IMP_Z0))));
+    const double hu = -sasa * tr * cz_up;
+    const double hl = -sasa * tr * cz_lo;
+    const double lu = K_IMP_ALIP * sasa * cz_up;
+    const double ll = K_IMP_ALIP * sasa * cz_lo;
+    if (imp_is_flat(g))
+        return (float)(hu + lu);
+    return (float)(hu + lu + hl + ll);
+}
+
+static inline double imp_expo_f(float z, float off) { return exp((double)(K_IMP_ALPHA * (fabsf(z + off) - K_IMP_Z0))); }
+
+/* Returns the vector imp_force_vector() (before the ForceField sign/scale). */
+static void imp_force_f(const imp_geom *g, float x, float y, float z, float sasa, float tr, float out[3])
+{
+    float zu, zl, vu[3], vl[3], ru, rl;
+    int su, sl;
+    imp_zprime_f(g, x, y, z, &zu, &zl, vu, vl, &su, &sl, &ru, &rl);
+    double dcu = (K_IMP_ALPHA * (zu - g->up_off) * imp_expo_f(zu, -g->up_off)) /
+                 (pow(imp_expo_f(zu, -g->up_off) + 1, 2.0) * fabsf(zu - g->up_off));
+    double dcl = (K_IMP_ALPHA * (zl + g->lo_off) * imp_expo_f(zl, g->lo_off)) /
+                 (pow(imp_expo_f(zl, g->lo_off) + 1, 2.0) * fabsf(zl + g->lo_off));
+    if (isnan(dcu) || !isfinite(dcu))
+        dcu = 0.0;
+    if (isnan(dcl) || !isfinite(dcl))
+        dcl = 0.0;
+    float du[3] = {0.0f, 0.0f, 1.0f}, dl[3] = {0.0f, 0.0f, 1.0f};
+    if (su != 0)
+        memcpy(du, vu, sizeof du);
+    if (sl != 0)
+        memcpy(dl, vl, sizeof dl);
+    normalize_f(du);
+    normalize_f(dl);
+    const double hu = -sasa * tr * dcu;
+    const double hl = -sasa * tr * dcl;
+    const double lu = K_IMP_ALIP * sasa * dcu;
+    const double ll = K_IMP_ALIP * sasa * dcl;
+    const float mu = (float)((hu + lu) * c_spring());
+    const float ml = (float)((hl + ll) * c_spring());
+    const float fu[3] = {du[0] * mu, du[1] * mu, du[2] * mu};
+    const float fl[3] = {dl[0] * ml, dl[1] * ml, dl[2] * ml};
+    if (imp_is_flat(g)) {
+        out[0] = fu[0];
+        out[1] = fu[1];
+        out[2] = fu[2];
+    } else {
+        out[0] = fu[0] + fl[0];
+        out[1] = fu[1] + fl[1];
+        out[2] = fu[2] + fl[2];
+    }
+}
+
+/* ---------------------------------------------------------------------------
+ * The same formulas in double precision (FP64 oracle).
+ * ------------------------------------------------------------------------- */
+static double steric_energy_d(int mode, double ri, double rj, double ei, double ej, double d)
+{
+    switch (mode) {
+    case BS_STERIC_AMBER_12_6: {
+        if (d < K_MIN_VDW)
+            return 0.0;
+        const double eij = sqrt(ei * ej), s = sqrt(ri * rj) / d;
+        const double s6 = s * s * s * s * s * s;
+        return eij * s6 * s6 - eij * 2.0 * s6;
+    }
+    case BS_STERIC_LEWITT_8_6: {
+        if (d < K_MIN_VDW)
+            return 0.0;
+        const double eij = sqrt(ei * ej), s = sqrt(ri * rj) / d;
+        const double s2 = s * s, s6 = s2 * s2 * s2;
+        return eij * 3.0 * s6 * s2 - eij * 4.0 * s6;
+    }
+    case BS_STERIC_ZACHARIAS_8_6: {
+        if (d < K_MIN_VDW)
+            return 0.0;
+        const double eij = ei * ej, s = ri * rj / d;
+        const double s2 = s * s, s6 = s2 * s2 * s2;
+        return eij * s6 * s2 - eij * s6;
+    }
+    default: {
+        const double dv = d - (ri + rj);
+        if (dv > 0)
+            return 0.0;
+        return 0.5 * dv * dv;
+    }
+    }
+}
+
+static double steric_fm_d(int mode, double ri, double rj, double ei, double ej, double d)
+{
+    switch (mode) {
+    case BS_STERIC_AMBER_12_6: {
+        if (d < K_MIN_VDW)
+            return 0.0;
+        const double eij = sqrt(ei * ej), s = sqrt(ri * rj) / d;
+        const double s6 = s * s * s * s * s * s;
+        return (-12.0 * eij * s6 * s6 + 12.0 * eij * s6) / d * c_spring();
+    }
+    case BS_STERIC_LEWITT_8_6: {
+        if (d < K_MIN_VDW)
+            return 0.0;
+        const double eij = sqrt(ei * ej), s = sqrt(ri * rj) / d;
+        const double s2 = s * s, s6 = s2 * s2 * s2;
+        return (-24.0 * eij * s6 * s2 + 24.0 * eij * s6) / d * c_spring();
+    }
+    case BS_STERIC_ZACHARIAS_8_6: {
+        if (d < K_MIN_VDW)
+            return 0.0;
+        const double eij = ei * ej, s = ri * rj / d;
+        const double s2 = s * s, s6 = s2 * s2 * s2;
+        return (-8.0 * eij * s6 * s2 + 6.0 * eij * s6) / d * c_spring();
+    }
+    default: {
+        const double dv = d - (ri + rj);
+        if (dv > 0)
+            return 0.0;
+        return -fabs(dv) * c_spring();
+    }
+    }
+}
+
+static inline double elec_energy_d(double q1, double q2, double d, double diel, int ddd)
+{
+    if (d < K_MIN_ELEC)
+        return 0.0;
+    const double eps = ddd ? diel * d : diel;
+    return c_coulomb() * (q1 * K_E_TO_C) * (q2 * K_E_TO_C) / (eps * d * K_A_TO_M) * K_NA * 1.0E-3;
+}
+
+static inline double elec_fm_d(double q1, double q2, double d, double diel, int ddd)
+{
+    if (d < K_MIN_ELEC)
+        return 0.0;
+    const double eps = ddd ? diel * d : diel;
+    const double fm = -(q1 * q2) / (4.0 * K_PI * eps * d * d) * c_elec();
+    return ddd ? 2.0 * fm : fm;
+}
+
+static inline double hydro_energy_d(double h1, double h2, double d) { return -(h1 * h2) * exp(-d) * K_NA * 1.0E-3; }
+static inline double hydro_fm_d(double h1, double h2, double d) { return (h1 * h2) * exp(-d) * c_spring(); }
+
+static void imp_zprime_d(const imp_geom *g, double x, double y, double z, double *z_up, double *z_lo,
+                         double vup[3], double vlo[3], int *s_up, int *s_lo)
+{
+    const double rup = g->up_curv == 0.0f ? 1000000.0 : fabs(1.0 / g->up_curv);
+    const double rlo = g->lo_curv == 0.0f ? 1000000.0 : fabs(1.0 / g->lo_curv);
+    const int sup = sgn_f(g->up_curv), slo = sgn_f(g->lo_curv);
+    const double cup = g->up_off - sup * rup;
+    const double clo = -(double)g->lo_off - slo * rlo;
+    vup[0] = x;
+    vup[1] = 0.0;
+    vup[2] = z - cup;
+    vlo[0] = x;
+    vlo[1] = 0.0;
+    vlo[2] = z - clo;
+    const double nup = norm_d(vup[0], vup[1], vup[2]);
+    const double nlo = norm_d(vlo[0], vlo[1], vlo[2]);
+    *z_up = sup == 0 ? z : (z > cup ? sup * nup + g->up_off - rup : -sup * nup + g->up_off - rup);
+    *z_lo = slo == 0 ? z : (z > clo ? slo * nlo - g->lo_off - rlo : -slo * nlo - g->lo_off - rlo);
+    *s_up = sup;
+    *s_lo = slo;
+    (void)y;
+}
+
+static double imp_energy_d(const imp_geom *g, double x, double y, double z, double sasa, double tr)
+{
+    double zu, zl, vu[3], vl[3];
+    int su, sl;
+    imp_zprime_d(g, x, y, z, &zu, &zl, vu, vl, &su, &sl);
+    const double a = K_IMP_ALPHA, z0 = K_IMP_Z0, alip = K_IMP_ALIP;
+    const double cu = 0.5 - 1.0 / (1.0 + exp(a * (fabs(zu - g->up_off) - z0)));
+    const double cl = 0.5 - 1.0 / (1.0 + exp(a * (fabs(zl + g->lo_off) - z0)));
+    const double eu = (-sasa * tr + alip * sasa) * cu;
+    const double el = (-sasa * tr + alip * sasa) * cl;
+    return imp_is_flat(g) ? eu : eu + el;
+}
+
+static void imp_force_d(const imp_geom *g, double x, double y, double z, double sasa, double tr, double out[3])
+{
+    double zu, zl, vu[3], vl[3];
+    int su, sl;
+    imp_zprime_d(g, x, y, z, &zu, &zl, vu, vl, &su, &sl);
+    const double a = K_IMP_ALPHA, z0 = K_IMP_Z0, alip = K_IMP_ALIP;
+    const double wu = zu - g->up_off, wl = zl + g->lo_off;
+    const double xu = exp(a * (fabs(wu) - z0)), xl = exp(a * (fabs(wl) - z0));
+    double dcu = (a * wu * xu) / ((xu + 1) * (xu + 1) * fabs(wu));
+    double dcl = (a * wl * xl) / ((xl + 1) * (xl + 1) * fabs(wl));
+    if (!isfinite(dcu))
+        dcu = 0.0;
+    if (!isfinite(dcl))
+        dcl = 0.0;
+    double du[3] = {0.0, 0.0, 1.0}, dl[3] = {0.0, 0.0, 1.0};
+    if (su != 0)
+        memcpy(du, vu, sizeof du);
+    if (sl != 0)
+        memcpy(dl, vl, sizeof dl);
+    normalize_d(du);
+    normalize_d(dl);
+    const double mu = (-sasa * tr + alip * sasa) * dcu * c_spring();
+    const double ml = (-sasa * tr + alip * sasa) * dcl * c_spring();
+    for (int c = 0; c < 3; ++c)
+        out[c] = imp_is_flat(g) ? du[c] * mu : du[c] * mu + dl[c] * ml;
+}
+
+/* exported for closed-form / finite-difference tests of single terms. */
+double orc_term_energy(int term, int mode, const double *a, int na)
+{
+    /* term: 0 spring(d,k,l0) 1 steric(mode_s,ri,rj,ei,ej,d) 2 elec(q1,q2,d,diel,ddd)
+     *       3 hydro(h1,h2,d) 4 imp(x,y,z,sasa,tr,uo,lo,uc,lc) */
+    (void)na;
+    if (mode == ORC_F32) {
+        switch (term) {
+        case 0: return spring_energy_f((float)a[0], (float)a[1], (float)a[2]);
+        case 1: return steric_energy_f((int)a[0], (float)a[1], (float)a[2], (float)a[3], (float)a[4], (float)a[5]);
+        case 2: return elec_energy_f((float)a[0], (float)a[1], (float)a[2], (float)a[3], (int)a[4]);
+        case 3: return hydro_energy_f((float)a[0], (float)a[1], (float)a[2]);
+        case 4: {
+            imp_geom g = {(float)a[5], (float)a[6], (float)a[7], (float)a[8]};
+            return imp_energy_f(&g, (float)a[0], (float)a[1], (float)a[2], (float)a[3], (float)a[4]);
+        }
+        }
+    } else {
+        switch (term) {
+        case 0: { const double dv = a[0] - a[2]; return 0.5 * a[1] * dv * dv; }
+        case 1: return steric_energy_d((int)a[0], a[1], a[2], a[3], a[4], a[5]);
+        case 2: return elec_energy_d(a[0], a[1], a[2], a[3], (int)a[4]);
+        case 3: return hydro_energy_d(a[0], a[1], a[2]);
+        case 4: {
+            imp_geom g = {(float)a[5], (float)a[6], (float)a[7], (float)a[8]};
+            return imp_energy_d(&g, a[0], a[1], a[2], a[3], a[4]);
+        }
+        }
+    }
+    return 0.0;
+}
+
+/* Force module (or IMPALA force vector into out3) of a single term. */
+double orc_term_force(int term, int mode, const double *a, int na, double *out3)
+{
+    (void)na;
+    if (mode == ORC_F32) {
+        switch (term) {
+        case 0: return spring_fm_f((float)a[0], (float)a[1], (float)a[2]);
+        case 1: return steric_fm_f((int)a[0], (float)a[1], (float)a[2], (float)a[3], (float)a[4], (float)a[5]);
+        case 2: return elec_fm_f((float)a[0], (float)a[1], (float)a[2], (float)a[3], (int)a[4]);
+        case 3: return hydro_fm_f((float)a[0], (float)a[1], (float)a[2]);
+        case 4: {
+            imp_geom g = {(float)a[5], (float)a[6], (float)a[7], (float)a[8]};
+            float f[3];
+            imp_force_f(&g, (float)a[0], (float)a[1], (float)a[2], (float)a[3], (float)a[4], f);
+            for (int c = 0; c < 3; ++c)
+                out3[c] = f[c];
+            return 0.0;
+        }
+        }
+    } else {
+        switch (term) {
+        case 0: return a[1] * (a[0] - a[2]) * c_spring();
+        case 1: return steric_fm_d((int)a[0], a[1], a[2], a[3], a[4], a[5]);
+        case 2: return elec_fm_d(a[0], a[1], a[2], a[3], (int)a[4]);
+        case 3: return hydro_fm_d(a[0], a[1], a[2]);
+        case 4: {
+            imp_geom g = {(float)a[5], (float)a[6], (float)a[7], (float)a[8]};
+            imp_force_d(&g, a[0], a[1], a[2], a[3], a[4], out3);
+            return 0.0;
+        }
+        }
+    }
+    return 0.0;
+}
+
+/* ---------------------------------------------------------------------------
+ * System state.
+ * ------------------------------------------------------------------------- */
+typedef struct orc {
+    bs_params p;
+    int64_t n;
+    float *xf, *vf, *ff;    /* float state, 3n */
+    double *xd, *vd, *fd;   /* double state, 3n */
+    float *xdec;            /* float positions used for pair decisions, 3n */
+    float *mass, *charge, *radius, *eps, *sasa, *transfer, *hydro;
+    uint8_t *isstatic, *charged, *hydrophobic;
+    int64_t ns;             /* deduplicated springs, in addition order */
+    uint32_t *s1, *s2;
+    float *sl0, *sk;
+    int64_t nds;            /* dynamic springs (not both static) */
+    int64_t *dsp;
+    int64_t ndyn;
+    int64_t *dyn;
+    int64_t *bptr;          /* bonded-partner CSR, partners sorted */
+    uint32_t *bidx;
+    float ef[BS_NUM_ENERGIES];
+    double ed[BS_NUM_ENERGIES];
+    int64_t nonfinite;
+} orc;
+
+typedef struct {
+    double box[6];
+    int64_t dims[3];
+    float radius;           /* cell edge, float like NeighborSearch::_search_radius */
+    float cutoff;
+    int64_t ncells;
+    int64_t *start;         /* ncells+1 */
+    int64_t *members;       /* nmem, ids ascending within a cell */
+    int64_t nmem;
+    int64_t *cell_of;       /* n, -1 when not a member */
+} cells_t;
+
+static int cmp_u32(const void *a, const void *b)
+{
+    const uint32_t x = *(const uint32_t *)a, y = *(const uint32_t *)b;
+    return (x > y) - (x < y);
+}
+
+static int is_bonded(const orc *o, int64_t i, uint32_t j)
+{
+    int64_t lo = o->bptr[i], hi = o->bptr[i + 1];
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (o->bidx[mid] < j)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo < o->bptr[i + 1] && o->bidx[lo] == j;
+}
+
+orc *orc_create(const bs_params *p, int64_t n, const float *x3, const float *v3, const float *mass,
+                const float *charge, const float *radius, const float *eps, const float *sasa,
+                const float *transfer, const float *hydro, const uint8_t *flags, int64_t m,
+                const uint32_t *id1, const uint32_t *id2, const float *l0, const float *k)
+{
+    orc *o = (orc *)calloc(1, sizeof(orc));
+    o->p = *p;
+    o->n = n;
+    o->xf = (float *)malloc(sizeof(float) * 3 * n);
+    o->vf = (float *)calloc(3 * n, sizeof(float));
+    o->ff = (float *)calloc(3 * n, sizeof(float));
+    o->xd = (double *)malloc(sizeof(double) * 3 * n);
+    o->vd = (double *)calloc(3 * n, sizeof(double));
+    o->fd = (double *)calloc(3 * n, sizeof(double));
+    o->xdec = (float *)malloc(sizeof(float) * 3 * n);
+    o->mass = (float *)malloc(sizeof(float) * n);
+    o->charge = (float *)malloc(sizeof(float) * n);
+    o->radius = (float *)malloc(sizeof(float) * n);
+    o->eps = (float *)malloc(sizeof(float) * n);
+    o->sasa = (float *)malloc(sizeof(float) * n);
+    o->transfer = (float *)malloc(sizeof(float) * n);
+    o->hydro = (float *)malloc(sizeof(float) * n);
+    o->isstatic = (uint8_t *)malloc(n);
+    o->charged = (uint8_t *)malloc(n);
+    o->hydrophobic = (uint8_t *)malloc(n);
+    o->dyn = (int64_t *)malloc(sizeof(int64_t) * (n ? n : 1));
+    o->ndyn = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        for (int c = 0; c < 3; ++c) {
+            o->xf[3 * i + c] = x3[3 * i + c];
+            o->xd[3 * i + c] = x3[3 * i + c];
+            o->vf[3 * i + c] = v3 ? v3[3 * i + c] : 0.0f;
+            o->vd[3 * i + c] = o->vf[3 * i + c];
+        }
+        /* ParticleProperty defaults, ParticleProperty.h:20-25 */
+        o->mass[i] = mass ? mass[i] : 1.0f;
+        o->charge[i] = charge ? charge[i] : 0.0f;
+        o->radius[i] = radius ? radius[i] : 1.0f;
+        o->eps[i] = eps ? eps[i] : 0.0f;
+        o->sasa[i] = sasa ? sasa[i] : 0.0f;
+        o->transfer[i] = transfer ? transfer[i] : 0.0f;
+        o->hydro[i] = hydro ? hydro[i] : 0.0f;
+        o->isstatic[i] = flags ? (flags[i] & BS_FLAG_STATIC) != 0 : 0;
+        o->charged[i] = o->charge[i] != 0.0f;   /* ParticleProperty.cpp:8-16 */
+        o->hydrophobic[i] = o->hydro[i] != 0.0f; /* ParticleProperty.cpp:28-35 */
+        if (!o->isstatic[i])
+            o->dyn[o->ndyn++] = i;
+    }
+    /* Springs: dedup on the unordered pair like SpringNetwork::addSpring's
+     * isInSpringNeighbors test (SpringNetwork.cpp:459-486). */
+    int64_t *deg = (int64_t *)calloc(n + 1, sizeof(int64_t));
+    uint32_t *tmp = (uint32_t *)malloc(sizeof(uint32_t) * 2 * (m ? m : 1));
+    int64_t *tptr = (int64_t *)calloc(n + 1, sizeof(int64_t));
+    for (int64_t s = 0; s < m; ++s) {
+        deg[id1[s]]++;
+        deg[id2[s]]++;
+    }
+    for (int64_t i = 0; i < n; ++i)
+        tptr[i + 1] = tptr[i] + deg[i];
+    int64_t *fill = (int64_t *)calloc(n + 1, sizeof(int64_t));
+    for (int64_t s = 0; s < m; ++s) {
+        tmp[tptr[id1[s]] + fill[id1[s]]++] = id2[s];
+        tmp[tptr[id2[s]] + fill[id2[s]]++] = id1[s];
+    }
+    /* keep first occurrence per unordered pair: mark duplicates by sorting partners */
+    o->bptr = (int64_t *)calloc(n + 1, sizeof(int64_t));
+    o->bidx = (uint32_t *)malloc(sizeof(uint32_t) * 2 * (m ? m : 1));
+    for (int64_t i = 0; i < n; ++i) {
+        qsort(tmp + tptr[i], (size_t)(tptr[i + 1] - tptr[i]), sizeof(uint32_t), cmp_u32);
+        int64_t w = o->bptr[i];
+        for (int64_t e = tptr[i]; e < tptr[i + 1]; ++e)
+            if (e == tptr[i] || tmp[e] != tmp[e - 1])
+                o->bidx[w++] = tmp[e];
+        o->bptr[i + 1] = w;
+    }
+    /* Spring list in addition order, skipping repeats of an already added pair. */
+    o->s1 = (uint32_t *)malloc(sizeof(uint32_t) * (m ? m : 1));
+    o->s2 = (uint32_t *)malloc(sizeof(uint32_t) * (m ? m : 1));
+    o->sl0 = (float *)malloc(sizeof(float) * (m ? m : 1));
+    o->sk = (float *)malloc(sizeof(float) * (m ? m : 1));
+    o->dsp = (int64_t *)malloc(sizeof(int64_t) * (m ? m : 1));
+    /* seen-set via per-particle used flags on the sorted partner list */
+    uint8_t *used = (uint8_t *)calloc(o->bptr[n] ? o->bptr[n] : 1, 1);
+    o->ns = 0;
+    o->nds = 0;
+    for (int64_t s = 0; s < m; ++s) {
+        const uint32_t a = id1[s], b = id2[s];
+        /* locate b in a's sorted partner list */
+        int64_t lo = o->bptr[a], hi = o->bptr[a + 1];
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (o->bidx[mid] < b)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        if (used[lo])
+            continue;
+        used[lo] = 1;
+        int64_t lo2 = o->bptr[b], hi2 = o->bptr[b + 1];
+        while (lo2 < hi2) {
+            const int64_t mid = (lo2 + hi2) >> 1;
+            if (o->bidx[mid] < a)
+                lo2 = mid + 1;
+            else
+                hi2 = mid;
+        }
+        used[lo2] = 1;
+        o->s1[o->ns] = a;
+        o->s2[o->ns] = b;
+        o->sl0[o->ns] = l0[s];
+        o->sk[o->ns] = k[s];
+        if (!(o->isstatic[a] && o->isstatic[b]))
+            o->dsp[o->nds++] = o->ns;
+        o->ns++;
+    }
+    free(used);
+    free(deg);
+    free(tmp);
+    free(tptr);
+    free(fill);
+    o->nonfinite = -1;
+    return o;
+}
+
+void orc_destroy(orc *o)
+{
+    if (!o)
+        return;
+    free(o->xf); free(o->vf); free(o->ff); free(o->xd); free(o->vd); free(o->fd); free(o->xdec);
+    free(o->mass); free(o->charge); free(o->radius); free(o->eps); free(o->sasa); free(o->transfer);
+    free(o->hydro); free(o->isstatic); free(o->charged); free(o->hydrophobic); free(o->dyn);
+    free(o->bptr); free(o->bidx); free(o->s1); free(o->s2); free(o->sl0); free(o->sk); free(o->dsp);
+    free(o);
+}
+
+int64_t orc_num_springs(const orc *o) { return o->ns; }
+
+void orc_set_params(orc *o, const bs_params *p) { o->p = *p; }
+
+void orc_set_state(orc *o, const float *x3, const float *v3)
+{
+    for (int64_t i = 0; i < 3 * o->n; ++i) {
+        if (x3) {
+            o->xf[i] = x3[i];
+            o->xd[i] = x3[i];
+        }
+        if (v3) {
+            o->vf[i] = v3[i];
+            o->vd[i] = v3[i];
+        }
+    }
+}
+
+void orc_set_state_d(orc *o, const double *x3, const double *v3)
+{
+    for (int64_t i = 0; i < 3 * o->n; ++i) {
+        if (x3)
+            o->xd[i] = x3[i];
+        if (v3)
+            o->vd[i] = v3[i];
+    }
+}
+
+void orc_get_state(const orc *o, int mode, double *x3, double *v3)
+{
+    for (int64_t i = 0; i < 3 * o->n; ++i) {
+        if (x3)
+            x3[i] = mode == ORC_F32 ? (double)o->xf[i] : o->xd[i];
+        if (v3)
+            v3[i] = mode == ORC_F32 ? (double)o->vf[i] : o->vd[i];
+    }
+}
+
+/* ---------------------------------------------------------------------------
+ * Cell list, nsearch.hpp:342-390 (NeighborSearch::_build_grid).
+ * ------------------------------------------------------------------------- */
+static int member_of(const orc *o, int kind, int64_t i)
+{
+    if (kind == BS_NS_ELECTROSTATIC)
+        return o->charged[i];
+    if (kind == BS_NS_HYDROPHOBIC)
+        return o->hydrophobic[i];
+    return 1;
+}
+
+static float kind_cutoff(const orc *o, int kind)
+{
+    if (kind == BS_NS_ELECTROSTATIC)
+        return o->p.electrostatic_cutoff;
+    if (kind == BS_NS_HYDROPHOBIC)
+        return o->p.hydrophobicity_cutoff;
+    return o->p.steric_cutoff;
+}
+
+static inline int64_t cell_coord(double x, double mn, float r, int64_t na)
+{
+    int64_t c = (int64_t)(size_t)((x - mn) / r); /* nsearch.hpp:282-289 */
+    return c < na - 1 ? c : na - 1;
+}
+
+static void build_cells(const orc *o, int kind, const float *pos, cells_t *cl)
+{
+    const int64_t n = o->n;
+    /* Box(container) over every particle, box.hpp:37-69 */
+    double mn[3], mx[3];
+    for (int c = 0; c < 3; ++c)
+        mn[c] = mx[c] = pos[c];
+    for (int64_t i = 1; i < n; ++i)
+        for (int c = 0; c < 3; ++c) {
+            const double v = pos[3 * i + c];
+            if (v < mn[c])
+                mn[c] = v;
+            if (v > mx[c])
+                mx[c] = v;
+        }
+    cl->cutoff = kind_cutoff(o, kind);
+    cl->radius = cl->cutoff + o->p.neighborskin;
+    for (int c = 0; c < 3; ++c) {
+        cl->box[c] = mn[c];
+        cl->box[3 + c] = mx[c];
+        cl->dims[c] = (int64_t)(size_t)(ceil((mx[c] - mn[c]) / cl->radius) + 1);
+    }
+    cl->ncells = cl->dims[0] * cl->dims[1] * cl->dims[2];
+    cl->start = (int64_t *)calloc(cl->ncells + 1, sizeof(int64_t));
+    cl->cell_of = (int64_t *)malloc(sizeof(int64_t) * (n ? n : 1));
+    cl->nmem = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        if (!member_of(o, kind, i)) {
+            cl->cell_of[i] = -1;
+            continue;
+        }
+        const int64_t cx = cell_coord(pos[3 * i + 0], mn[0], cl->radius, cl->dims[0]);
+        const int64_t cy = cell_coord(pos[3 * i + 1], mn[1], cl->radius, cl->dims[1]);
+        const int64_t cz = cell_coord(pos[3 * i + 2], mn[2], cl->radius, cl->dims[2]);
+        cl->cell_of[i] = cx + cy * cl->dims[0] + cz * cl->dims[0] * cl->dims[1];
+        cl->start[cl->cell_of[i] + 1]++;
+        cl->nmem++;
+    }
+    for (int64_t c = 0; c < cl->ncells; ++c)
+        cl->start[c + 1] += cl->start[c];
+    cl->members = (int64_t *)malloc(sizeof(int64_t) * (cl->nmem ? cl->nmem : 1));
+    int64_t *cur = (int64_t *)malloc(sizeof(int64_t) * (cl->ncells ? cl->ncells : 1));
+    memcpy(cur, cl->start, sizeof(int64_t) * cl->ncells);
+    for (int64_t i = 0; i < n; ++i)
+        if (cl->cell_of[i] >= 0)
+            cl->members[cur[cl->cell_of[i]]++] = i; /* push_back in ascending id order */
+    free(cur);
+}
+
+static void free_cells(cells_t *cl)
+{
+    free(cl->start);
+    free(cl->members);
+    free(cl->cell_of);
+}
+
+static inline int64_t query_cell(const cells_t *cl, const float *xi, int c)
+{
+    return cell_coord(xi[c], cl->box[c], cl->radius, cl->dims[c]);
+}
+
+/* Calls visit(j) for every candidate of nsearch::for_each_neighbor
+ * (nsearch.hpp:214-234): 27 cells, x-outer z-inner, FP64 distance < cutoff. */
+#define FOR_EACH_NEIGHBOR(cl, pos, i, J, BODY)                                                         \
+    do {                                                                                            \
+        const float *xi_ = (pos) + 3 * (i);                                                         \
+        const int64_t qx_ = query_cell((cl), xi_, 0), qy_ = query_cell((cl), xi_, 1),               \
+                      qz_ = query_cell((cl), xi_, 2);                                               \
+        for (int dx_ = -1; dx_ <= 1; dx_++)                                                         \
+            for (int dy_ = -1; dy_ <= 1; dy_++)                                                     \
+                for (int dz_ = -1; dz_ <= 1; dz_++) {                                               \
+                    const int64_t ax_ = qx_ + dx_, ay_ = qy_ + dy_, az_ = qz_ + dz_;                 \
+                    if (ax_ < 0 || ax_ >= (cl)->dims[0] || ay_ < 0 || ay_ >= (cl)->dims[1] ||        \
+                        az_ < 0 || az_ >= (cl)->dims[2])                                            \
+                        continue;                                                                   \
+                    const int64_t cid_ = ax_ + ay_ * (cl)->dims[0] + az_ * (cl)->dims[0] * (cl)->dims[1]; \
+                    for (int64_t e_ = (cl)->start[cid_]; e_ < (cl)->start[cid_ + 1]; ++e_) {        \
+                        const int64_t J = (cl)->members[e_];                                        \
+                        if (J == (i))                                                               \
+                            continue;                                                               \
+                        const float *xj_ = (pos) + 3 * J;                                           \
+                        const double ddx_ = (double)xi_[0] - (double)xj_[0];                        \
+                        const double ddy_ = (double)xi_[1] - (double)xj_[1];                        \
+                        const double ddz_ = (double)xi_[2] - (double)xj_[2];                        \
+                        if (!(sqrt(ddx_ * ddx_ + ddy_ * ddy_ + ddz_ * ddz_) < (double)(cl)->cutoff)) \
+                            continue;                                                               \
+                        BODY                                                                        \
+                    }                                                                               \
+                }                                                                                   \
+    } while (0)
+
+/* Particle-level acceptance of a candidate (Particle.cpp:205-224,384-402):
+ * returns 1 and the float displacement/distance when the pair is processed. */
+static inline int pair_accept(const orc *o, int kind, int64_t i, int64_t j, const float *pos, float cutoff,
+                              float *d_out)
+{
+    if (!o->isstatic[j] && j < i)
+        return 0;
+    if (o->p.spring_enable && is_bonded(o, i, (uint32_t)j))
+        return 0;
+    if (j == i)
+        return 0;
+    const float f0 = pos[3 * j + 0] - pos[3 * i + 0];
+    const float f1 = pos[3 * j + 1] - pos[3 * i + 1];
+    const float f2 = pos[3 * j + 2] - pos[3 * i + 2];
+    const float d = norm_f(f0, f1, f2);
+    if (!(d < cutoff))
+        return 0;
+    if (kind != BS_NS_STERIC && !(d != 0.0f))
+        return 0;
+    *d_out = d;
+    return 1;
+}
+
+static int kind_enabled(const orc *o, int kind)
+{
+    if (kind == BS_NS_STERIC)
+        return o->p.steric_enable;
+    if (kind == BS_NS_ELECTROSTATIC)
+        return o->p.electrostatic_enable;
+    return o->p.hydrophobicity_enable;
+}
+
+static const float *decision_positions(orc *o, int mode)
+{
+    if (mode == ORC_F32)
+        return o->xf;
+    for (int64_t i = 0; i < 3 * o->n; ++i)
+        o->xdec[i] = (float)o->xd[i];
+    return o->xdec;
+}
+
+/* ---------------------------------------------------------------------------
+ * Pair enumeration for the parity hooks.
+ * ------------------------------------------------------------------------- */
+typedef struct {
+    uint32_t *ij;
+    int64_t cap, count;
+} pairbuf;
+
+static void pb_push(pairbuf *pb, uint32_t a, uint32_t b)
+{
+    if (pb->count >= pb->cap) {
+        pb->cap = pb->cap ? 2 * pb->cap : 1024;
+        pb->ij = (uint32_t *)realloc(pb->ij, sizeof(uint32_t) * 2 * pb->cap);
+    }
+    pb->ij[2 * pb->count] = a < b ? a : b;
+    pb->ij[2 * pb->count + 1] = a < b ? b : a;
+    pb->count++;
+}
+
+static int cmp_pair(const void *a, const void *b)
+{
+    const uint32_t *x = (const uint32_t *)a, *y = (const uint32_t *)b;
+    if (x[0] != y[0])
+        return (x[0] > y[0]) - (x[0] < y[0]);
+    return (x[1] > y[1]) - (x[1] < y[1]);
+}
+
+/* method 0: through the reference cell list; method 1: O(N^2) brute force
+ * (NeighborSearchO2, nsearch.hpp:74-102) with the same two-stage test.
+ * Returns the count; copies up to cap pairs into ij (sorted (min,max)). */
+int64_t orc_pairs(orc *o, int kind, int method, int mode, uint32_t *ij, int64_t cap)
+{
+    const float *pos = decision_positions(o, mode);
+    const float cutoff = kind_cutoff(o, kind);
+    pairbuf pb = {0, 0, 0};
+    cells_t cl;
+    if (method == 0)
+        build_cells(o, kind, pos, &cl);
+    for (int64_t t = 0; t < o->ndyn; ++t) {
+        const int64_t i = o->dyn[t];
+        if (kind == BS_NS_ELECTROSTATIC && !o->charged[i])
+            continue;
+        if (kind == BS_NS_HYDROPHOBIC && !o->hydrophobic[i])
+            continue;
+        if (method == 0) {
+            FOR_EACH_NEIGHBOR(&cl, pos, i, j, {
+                float d;
+                if (pair_accept(o, kind, i, j, pos, cutoff, &d))
+                    pb_push(&pb, (uint32_t)i, (uint32_t)j);
+            });
+        } else {
+            for (int64_t j = 0; j < o->n; ++j) {
+                if (j == i || !member_of(o, kind, j))
+                    continue;
+                const double ddx = (double)pos[3 * i] - (double)pos[3 * j];
+                const double ddy = (double)pos[3 * i + 1] - (double)pos[3 * j + 1];
+                const double ddz = (double)pos[3 * i + 2] - (double)pos[3 * j + 2];
+                if (!(sqrt(ddx * ddx + ddy * ddy + ddz * ddz) < (double)cutoff))
+                    continue;
+                float d;
+                if (pair_accept(o, kind, i, j, pos, cutoff, &d))
+                    pb_push(&pb, (uint32_t)i, (uint32_t)j);
+            }
+        }
+    }
+    if (method == 0)
+        free_cells(&cl);
+    if (pb.count)
+        qsort(pb.ij, (size_t)pb.count, 2 * sizeof(uint32_t), cmp_pair);
+    const int64_t cnt = pb.count;
+    if (ij)
+        memcpy(ij, pb.ij, sizeof(uint32_t) * 2 * (cnt < cap ? cnt : cap));
+    free(pb.ij);
+    return cnt;
+}
+
+/* Reference cell list of `kind` at the float positions of `mode`. */
+int64_t orc_cells(orc *o, int kind, int mode, int64_t *cell_of, int64_t *start, int64_t *members,
+                  int64_t dims[3], double box[6], int64_t *nmem)
+{
+    const float *pos = decision_positions(o, mode);
+    cells_t cl;
+    build_cells(o, kind, pos, &cl);
+    if (cell_of)
+        memcpy(cell_of, cl.cell_of, sizeof(int64_t) * o->n);
+    if (start)
+        memcpy(start, cl.start, sizeof(int64_t) * (cl.ncells + 1));
+    if (members)
+        memcpy(members, cl.members, sizeof(int64_t) * cl.nmem);
+    if (dims)
+        memcpy(dims, cl.dims, sizeof cl.dims);
+    if (box)
+        memcpy(box, cl.box, sizeof cl.box);
+    if (nmem)
+        *nmem = cl.nmem;
+    const int64_t nc = cl.ncells;
+    free_cells(&cl);
+    return nc;
+}
+
+/* ---------------------------------------------------------------------------
+ * Forces, float-faithful (SpringNetwork::computeForces).
+ * ------------------------------------------------------------------------- */
+typedef struct {
+    uint32_t target;
+    float f[3];
+    float e;
+} defer_f;
+
+typedef struct {
+    defer_f *v;
+    int64_t n, cap;
+} dvec_f;
+
+static void dvf_push(dvec_f *d, uint32_t t, const float f[3], float e)
+{
+    if (d->n >= d->cap) {
+        d->cap = d->cap ? 2 * d->cap : 256;
+        d->v = (defer_f *)realloc(d->v, sizeof(defer_f) * d->cap);
+    }
+    d->v[d->n].target = t;
+    d->v[d->n].f[0] = -f[0];
+    d->v[d->n].f[1] = -f[1];
+    d->v[d->n].f[2] = -f[2];
+    d->v[d->n].e = e;
+    d->n++;
+}
+
+typedef struct {
+    uint32_t target;
+    double f[3];
+    double e;
+} defer_d;
+
+typedef struct {
+    defer_d *v;
+    int64_t n, cap;
+} dvec_d;
+
+static void dvd_push(dvec_d *d, uint32_t t, const double f[3], double e)
+{
+    if (d->n >= d->cap) {
+        d->cap = d->cap ? 2 * d->cap : 256;
+        d->v = (defer_d *)realloc(d->v, sizeof(defer_d) * d->cap);
+    }
+    d->v[d->n].target = t;
+    d->v[d->n].f[0] = -f[0];
+    d->v[d->n].f[1] = -f[1];
+    d->v[d->n].f[2] = -f[2];
+    d->v[d->n].e = e;
+    d->n++;
+}
+
+static int max_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+static int thread_id(void)
+{
+#ifdef _OPENMP
+    return omp_get_thread_num();
+#else
+    return 0;
+#endif
+}
+
+/* One pair from particle i's side: float-faithful evaluation, Particle.cpp. */
+static inline void pair_eval_f(const orc *o, int kind, int64_t i, int64_t j, float d, float *fi, float *ei,
+                               dvec_f *def)
+{
+    const bs_params *p = &o->p;
+    float f[3] = {o->xf[3 * j] - o->xf[3 * i], o->xf[3 * j + 1] - o->xf[3 * i + 1],
+                  o->xf[3 * j + 2] - o->xf[3 * i + 2]};
+    float e, fm;
+    if (kind == BS_NS_STERIC) {
+        e = p->steric_gridscale *
+            steric_energy_f(p->steric_mode, o->radius[j], o->radius[i], o->eps[j], o->eps[i], d);
+        fm = p->steric_gridscale *
+             steric_fm_f(p->steric_mode, o->radius[j], o->radius[i], o->eps[j], o->eps[i], d);
+    } else if (kind == BS_NS_ELECTROSTATIC) {
+        e = p->electrostatic_scale *
+            elec_energy_f(o->charge[j], o->charge[i], d, p->electrostatic_dielectric, p->dielectric_model);
+        fm = p->electrostatic_scale *
+             elec_fm_f(o->charge[j], o->charge[i], d, p->electrostatic_dielectric, p->dielectric_model);
+    } else {
+        e = p->hydrophobicity_scale * hydro_energy_f(o->hydro[j], o->hydro[i], d);
+        fm = p->hydrophobicity_scale * hydro_fm_f(o->hydro[j], o->hydro[i], d);
+    }
+    normalize_f(f);
+    f[0] = f[0] * fm;
+    f[1] = f[1] * fm;
+    f[2] = f[2] * fm;
+    fi[0] = fi[0] + f[0];
+    fi[1] = fi[1] + f[1];
+    fi[2] = fi[2] + f[2];
+    if (!o->isstatic[j]) {
+        const float half = kind == BS_NS_ELECTROSTATIC ? 0.5f * e : e / 2.0f;
+        *ei += half;
+        dvf_push(def, (uint32_t)j, f, half);
+    } else {
+        *ei += e;
+    }
+}
+
+static inline void pair_eval_d(const orc *o, int kind, int64_t i, int64_t j, double *fi, double *ei, dvec_d *def)
+{
+    const bs_params *p = &o->p;
+    double f[3] = {o->xd[3 * j] - o->xd[3 * i], o->xd[3 * j + 1] - o->xd[3 * i + 1],
+                   o->xd[3 * j + 2] - o->xd[3 * i + 2]};
+    const double d = norm_d(f[0], f[1], f[2]);
+    double e, fm;
+    if (kind == BS_NS_STERIC) {
+        e = p->steric_gridscale *
+            steric_energy_d(p->steric_mode, o->radius[j], o->radius[i], o->eps[j], o->eps[i], d);
+        fm = p->steric_gridscale * steric_fm_d(p->steric_mode, o->radius[j], o->radius[i], o->eps[j], o->eps[i], d);
+    } else if (kind == BS_NS_ELECTROSTATIC) {
+        e = p->electrostatic_scale *
+            elec_energy_d(o->charge[j], o->charge[i], d, p->electrostatic_dielectric, p->dielectric_model);
+        fm = p->electrostatic_scale *
+             elec_fm_d(o->charge[j], o->charge[i], d, p->electrostatic_dielectric, p->dielectric_model);
+    } else {
+        e = p->hydrophobicity_scale * hydro_energy_d(o->hydro[j], o->hydro[i], d);
+        fm = p->hydrophobicity_scale * hydro_fm_d(o->hydro[j], o->hydro[i], d);
+    }
+    normalize_d(f);
+    for (int c = 0; c < 3; ++c) {
+        f[c] *= fm;
+        fi[c] += f[c];
+    }
+    if (!o->isstatic[j]) {
+        *ei += 0.5 * e;
+        dvd_push(def, (uint32_t)j, f, 0.5 * e);
+    } else {
+        *ei += e;
+    }
+}
+
+/* computeSpringForces + computeParticleForces, float-faithful.  ext3 (nullable)
+ * is added first (SpringNetwork::setForce in idleRun). */
+static void forces_f32(orc *o, const float *ext3)
+{
+    const int64_t n = o->n;
+    const bs_params *p = &o->p;
+    float *F = o->ff;
+    for (int64_t i = 0; i < 3 * n; ++i)
+        F[i] = 0.0f; /* resetForce at the end of the previous step */
+    if (ext3)
+        for (int64_t i = 0; i < 3 * n; ++i)
+            F[i] = F[i] + ext3[i];
+    for (int k = 0; k < BS_NUM_ENERGIES; ++k)
+        o->ef[k] = 0.0f;
+
+    /* --- springs, SpringNetwork.cpp:63-90 --- */
+    if (p->spring_enable) {
+        float *scratch = (float *)malloc(sizeof(float) * 3 * (o->nds ? o->nds : 1));
+        float *senergy = (float *)malloc(sizeof(float) * (o->nds ? o->nds : 1));
+#pragma omp parallel for schedule(static)
+        for (int64_t t = 0; t < o->nds; ++t) {
+            const int64_t s = o->dsp[t];
+            const uint32_t a = o->s1[s], b = o->s2[s];
+            senergy[t] = 0.0f;
+            scratch[3 * t] = scratch[3 * t + 1] = scratch[3 * t + 2] = 0.0f;
+            if (o->isstatic[a] && o->isstatic[b])
+                continue;
+            float d[3] = {o->xf[3 * b] - o->xf[3 * a], o->xf[3 * b + 1] - o->xf[3 * a + 1],
+                          o->xf[3 * b + 2] - o->xf[3 * a + 2]};
+            const float len = norm_f(d[0], d[1], d[2]);
+            senergy[t] = p->spring_scale * spring_energy_f(len, o->sk[s], o->sl0[s]);
+            normalize_f(d);
+            const float fm = p->spring_scale * spring_fm_f(len, o->sk[s], o->sl0[s]);
+            scratch[3 * t] = d[0] * fm;
+            scratch[3 * t + 1] = d[1] * fm;
+            scratch[3 * t + 2] = d[2] * fm;
+        }
+        float es = 0.0f;
+        for (int64_t t = 0; t < o->nds; ++t) {
+            const int64_t s = o->dsp[t];
+            const uint32_t a = o->s1[s], b = o->s2[s];
+            for (int c = 0; c < 3; ++c) {
+                F[3 * a + c] = F[3 * a + c] + scratch[3 * t + c];
+                F[3 * b + c] = F[3 * b + c] + -scratch[3 * t + c];
+            }
+            es += senergy[t];
+        }
+        o->ef[BS_E_SPRING] = es;
+        free(scratch);
+        free(senergy);
+    }
+
+    /* --- particle forces, SpringNetwork.cpp:93-198 --- */
+    const float *pos = o->xf;
+    cells_t cls[3];
+    int have[3] = {0, 0, 0};
+    for (int kind = 0; kind < 3; ++kind) {
+        if (!kind_enabled(o, kind))
+            continue;
+        if (kind != BS_NS_STERIC) {
+            int any = 0;
+            for (int64_t i = 0; i < n && !any; ++i)
+                any = member_of(o, kind, i);
+            if (!any)
+                continue; /* no search object when the subset is empty (SpringNetwork.cpp:729-735) */
+        }
+        build_cells(o, kind, pos, &cls[kind]);
+        have[kind] = 1;
+    }
+    const int nth = max_threads();
+    dvec_f *defs = (dvec_f *)calloc(3 * nth, sizeof(dvec_f));
+    float *pe = (float *)calloc(4 * (n ? n : 1), sizeof(float)); /* elec, steric, imp, hydro per particle */
+    const imp_geom g = {p->imp_upper_offset, p->imp_lower_offset, p->imp_upper_curvature, p->imp_lower_curvature};
+#pragma omp parallel for schedule(static)
+    for (int64_t t = 0; t < o->ndyn; ++t) {
+        const int tid = thread_id();
+        const int64_t i = o->dyn[t];
+        float *fi = F + 3 * i;
+        if (p->electrostatic_enable && o->charged[i] && have[BS_NS_ELECTROSTATIC]) {
+            const cells_t *cl = &cls[BS_NS_ELECTROSTATIC];
+            FOR_EACH_NEIGHBOR(cl, pos, i, j, {
+                float d;
+                if (pair_accept(o, BS_NS_ELECTROSTATIC, i, j, pos, cl->cutoff, &d))
+                    pair_eval_f(o, BS_NS_ELECTROSTATIC, i, j, d, fi, &pe[4 * i + 0], &defs[3 * tid + 1]);
+            });
+        }
+        if (p->steric_enable && have[BS_NS_STERIC]) {
+            const cells_t *cl = &cls[BS_NS_STERIC];
+            FOR_EACH_NEIGHBOR(cl, pos, i, j, {
+                float d;
+                if (pair_accept(o, BS_NS_STERIC, i, j, pos, cl->cutoff, &d))
+                    pair_eval_f(o, BS_NS_STERIC, i, j, d, fi, &pe[4 * i + 1], &defs[3 * tid + 0]);
+            });
+        }
+        if (p->viscosity_enable) { /* Particle.cpp:80-84 */
+            const float gam = p->viscosity_value;
+            for (int c = 0; c < 3; ++c)
+                fi[c] = fi[c] - o->vf[3 * i + c] * gam;
+        }
+        if (p->imp_enable) { /* Particle.cpp:259-267, ForceField.cpp:62-86 */
+            pe[4 * i + 2] = p->imp_scale * imp_energy_f(&g, pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], o->sasa[i],
+                                                         o->transfer[i]);
+            float fv[3];
+            imp_force_f(&g, pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], o->sasa[i], o->transfer[i], fv);
+            for (int c = 0; c < 3; ++c)
+                fi[c] = fi[c] + (-fv[c]) * p->imp_scale;
+        }
+        if (p->hydrophobicity_enable && o->hydrophobic[i] && have[BS_NS_HYDROPHOBIC]) {
+            const cells_t *cl = &cls[BS_NS_HYDROPHOBIC];
+            FOR_EACH_NEIGHBOR(cl, pos, i, j, {
+                float d;
+                if (pair_accept(o, BS_NS_HYDROPHOBIC, i, j, pos, cl->cutoff, &d))
+                    pair_eval_f(o, BS_NS_HYDROPHOBIC, i, j, d, fi, &pe[4 * i + 3], &defs[3 * tid + 2]);
+            });
+        }
+    }
+    /* deferred Newton-3 halves: steric, then electrostatic, then hydrophobic
+     * (SpringNetwork.cpp:139-141); thread buffers concatenate in particle order
+     * under schedule(static). */
+    float e_elec = 0.0f, e_ster = 0.0f, e_imp = 0.0f, e_hyd = 0.0f;
+    const int order[3] = {0, 1, 2};
+    float *acc[3] = {&e_ster, &e_elec, &e_hyd};
+    for (int q = 0; q < 3; ++q)
+        for (int t = 0; t < nth; ++t) {
+            const dvec_f *d = &defs[3 * t + order[q]];
+            for (int64_t e = 0; e < d->n; ++e) {
+                const defer_f *c = &d->v[e];
+                for (int k = 0; k < 3; ++k)
+                    F[3 * c->target + k] = F[3 * c->target + k] + c->f[k];
+                *acc[q] += c->e;
+            }
+        }
+    for (int64_t t = 0; t < o->ndyn; ++t) {
+        const int64_t i = o->dyn[t];
+        e_elec += pe[4 * i + 0];
+        e_ster += pe[4 * i + 1];
+        e_imp += pe[4 * i + 2];
+        e_hyd += pe[4 * i + 3];
+    }
+    o->ef[BS_E_ELECTROSTATIC] = e_elec;
+    o->ef[BS_E_STERIC] = e_ster;
+    o->ef[BS_E_IMP] = e_imp;
+    o->ef[BS_E_HYDROPHOBIC] = e_hyd;
+    for (int t = 0; t < 3 * nth; ++t)
+        free(defs[t].v);
+    free(defs);
+    free(pe);
+    for (int kind = 0; kind < 3; ++kind)
+        if (have[kind])
+            free_cells(&cls[kind]);
+}
+
+/* The same in double precision.  Pair decisions use float-rounded positions. */
+static void forces_f64(orc *o, const double *ext3)
+{
+    const int64_t n = o->n;
+    const bs_params *p = &o->p;
+    double *F = o->fd;
+    for (int64_t i = 0; i < 3 * n; ++i)
+        F[i] = ext3 ? ext3[i] : 0.0;
+    for (int k = 0; k < BS_NUM_ENERGIES; ++k)
+        o->ed[k] = 0.0;
+    if (p->spring_enable) {
+        double *scratch = (double *)malloc(sizeof(double) * 3 * (o->nds ? o->nds : 1));
+        double *senergy = (double *)malloc(sizeof(double) * (o->nds ? o->nds : 1));
+#pragma omp parallel for schedule(static)
+        for (int64_t t = 0; t < o->nds; ++t) {
+            const int64_t s = o->dsp[t];
+            const uint32_t a = o->s1[s], b = o->s2[s];
+            double d[3] = {o->xd[3 * b] - o->xd[3 * a], o->xd[3 * b + 1] - o->xd[3 * a + 1],
+                           o->xd[3 * b + 2] - o->xd[3 * a + 2]};
+            const double len = norm_d(d[0], d[1], d[2]);
+            const double ext = len - (double)o->sl0[s];
+            senergy[t] = p->spring_scale * 0.5 * o->sk[s] * ext * ext;
+            normalize_d(d);
+            const double fm = p->spring_scale * o->sk[s] * ext * c_spring();
+            for (int c = 0; c < 3; ++c)
+                scratch[3 * t + c] = d[c] * fm;
+        }
+        double es = 0.0;
+        for (int64_t t = 0; t < o->nds; ++t) {
+            const int64_t s = o->dsp[t];
+            for (int c = 0; c < 3; ++c) {
+                F[3 * o->s1[s] + c] += scratch[3 * t + c];
+                F[3 * o->s2[s] + c] -= scratch[3 * t + c];
+            }
+            es += senergy[t];
+        }
+        o->ed[BS_E_SPRING] = es;
+        free(scratch);
+        free(senergy);
+    }
+    const float *pos = decision_positions(o, ORC_F64);
+    cells_t cls[3];
+    int have[3] = {0, 0, 0};
+    for (int kind = 0; kind < 3; ++kind) {
+        if (!kind_enabled(o, kind))
+            continue;
+        if (kind != BS_NS_STERIC) {
+            int any = 0;
+            for (int64_t i = 0; i < n && !any; ++i)
+                any = member_of(o, kind, i);
+            if (!any)
+                continue;
+        }
+        build_cells(o, kind, pos, &cls[kind]);
+        have[kind] = 1;
+    }
+    const int nth = max_threads();
+    dvec_d *defs = (dvec_d *)calloc(3 * nth, sizeof(dvec_d));
+    double *pe = (double *)calloc(4 * (n ? n : 1), sizeof(double));
+    const imp_geom g = {p->imp_upper_offset, p->imp_lower_offset, p->imp_upper_curvature, p->imp_lower_curvature};
+#pragma omp parallel for schedule(static)
+    for (int64_t t = 0; t < o->ndyn; ++t) {
+        const int tid = thread_id();
+        const int64_t i = o->dyn[t];
+        double *fi = F + 3 * i;
+        if (p->electrostatic_enable && o->charged[i] && have[BS_NS_ELECTROSTATIC]) {
+            const cells_t *cl = &cls[BS_NS_ELECTROSTATIC];
+            FOR_EACH_NEIGHBOR(cl, pos, i, j, {
+                float d;
+                if (pair_accept(o, BS_NS_ELECTROSTATIC, i, j, pos, cl->cutoff, &d))
+                    pair_eval_d(o, BS_NS_ELECTROSTATIC, i, j, fi, &pe[4 * i + 0], &defs[3 * tid + 1]);
+            });
+        }
+        if (p->steric_enable && have[BS_NS_STERIC]) {
+            const cells_t *cl = &cls[BS_NS_STERIC];
+            FOR_EACH_NEIGHBOR(cl, pos, i, j, {
+                float d;
+                if (pair_accept(o, BS_NS_STERIC, i, j, pos, cl->cutoff, &d))
+                    pair_eval_d(o, BS_NS_STERIC, i, j, fi, &pe[4 * i + 1], &defs[3 * tid + 0]);
+            });
+        }
+        if (p->viscosity_enable)
+            for (int c = 0; c < 3; ++c)
+                fi[c] -= o->vd[3 * i + c] * (double)p->viscosity_value;
+        if (p->imp_enable) {
+            pe[4 * i + 2] = p->imp_scale * imp_energy_d(&g, o->xd[3 * i], o->xd[3 * i + 1], o->xd[3 * i + 2],
+                                                         o->sasa[i], o->transfer[i]);
+            double fv[3];
+            imp_force_d(&g, o->xd[3 * i], o->xd[3 * i + 1], o->xd[3 * i + 2], o->sasa[i], o->transfer[i], fv);
+            for (int c = 0; c < 3; ++c)
+                fi[c] -= fv[c] * p->imp_scale;
+        }
+        if (p->hydrophobicity_enable && o->hydrophobic[i] && have[BS_NS_HYDROPHOBIC]) {
+            const cells_t *cl = &cls[BS_NS_HYDROPHOBIC];
+            FOR_EACH_NEIGHBOR(cl, pos, i, j, {
+                float d;
+                if (pair_accept(o, BS_NS_HYDROPHOBIC, i, j, pos, cl->cutoff, &d))
+                    pair_eval_d(o, BS_NS_HYDROPHOBIC, i, j, fi, &pe[4 * i + 3], &defs[3 * tid + 2]);
+            });
+        }
+    }
+    double e_elec = 0.0, e_ster = 0.0, e_imp = 0.0, e_hyd = 0.0;
+    double *acc[3] = {&e_ster, &e_elec, &e_hyd};
+    for (int q = 0; q < 3; ++q)
+        for (int t = 0; t < nth; ++t) {
+            const dvec_d *d = &defs[3 * t + q];
+            for (int64_t e = 0; e < d->n; ++e) {
+                const defer_d *c = &d->v[e];
+                for (int k = 0; k < 3; ++k)
+                    F[3 * c->target + k] += c->f[k];
+                *acc[q] += c->e;
+            }
+        }
+    for (int64_t t = 0; t < o->ndyn; ++t) {
+        const int64_t i = o->dyn[t];
+        e_elec += pe[4 * i + 0];
+        e_ster += pe[4 * i + 1];
+        e_imp += pe[4 * i + 2];
+        e_hyd += pe[4 * i + 3];
+    }
+    o->ed[BS_E_ELECTROSTATIC] = e_elec;
+    o->ed[BS_E_STERIC] = e_ster;
+    o->ed[BS_E_IMP] = e_imp;
+    o->ed[BS_E_HYDROPHOBIC] = e_hyd;
+    for (int t = 0; t < 3 * nth; ++t)
+        free(defs[t].v);
+    free(defs);
+    free(pe);
+    for (int kind = 0; kind < 3; ++kind)
+        if (have[kind])
+            free_cells(&cls[kind]);
+}
+
+/* ---------------------------------------------------------------------------
+ * Integrator, SpringNetwork.cpp:202-241 and Particle.cpp:109-128.
+ * ------------------------------------------------------------------------- */
+static int integrate_f32(orc *o)
+{
+    const float dt = o->p.timestep;
+    float ke = 0.0f;
+    int64_t bad = -1;
+    for (int64_t t = 0; t < o->ndyn; ++t) {
+        const int64_t i = o->dyn[t];
+        float *x = o->xf + 3 * i, *v = o->vf + 3 * i, *f = o->ff + 3 * i;
+        const float m = o->mass[i];
+        if (m > 0.0f)
+            for (int c = 0; c < 3; ++c)
+                v[c] = v[c] + (f[c] / m) * dt;
+        const float vit = norm_f(v[0], v[1], v[2]);
+        const float kei = (float)(0.5f * m * (vit * vit) * c_kinetic());
+        for (int c = 0; c < 3; ++c)
+            x[c] = x[c] + v[c] * dt;
+        if ((!isfinite(x[0]) || !isfinite(x[1]) || !isfinite(x[2])) && bad < 0)
+            bad = i;
+        ke += kei;
+        f[0] = f[1] = f[2] = 0.0f;
+    }
+    o->ef[BS_E_KINETIC] += ke;
+    o->nonfinite = bad;
+    return bad < 0 ? BS_OK : BS_ERR_NONFINITE;
+}
+
+static int integrate_f64(orc *o)
+{
+    const double dt = o->p.timestep;
+    double ke = 0.0;
+    int64_t bad = -1;
+    for (int64_t t = 0; t < o->ndyn; ++t) {
+        const int64_t i = o->dyn[t];
+        double *x = o->xd + 3 * i, *v = o->vd + 3 * i, *f = o->fd + 3 * i;
+        const double m = o->mass[i];
+        if (m > 0.0)
+            for (int c = 0; c < 3; ++c)
+                v[c] += f[c] / m * dt;
+        ke += 0.5 * m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) * c_kinetic();
+        for (int c = 0; c < 3; ++c)
+            x[c] += v[c] * dt;
+        if ((!isfinite(x[0]) || !isfinite(x[1]) || !isfinite(x[2])) && bad < 0)
+            bad = i;
+        f[0] = f[1] = f[2] = 0.0;
+    }
+    o->ed[BS_E_KINETIC] += ke;
+    o->nonfinite = bad;
+    return bad < 0 ? BS_OK : BS_ERR_NONFINITE;
+}
+
+/* ---------------------------------------------------------------------------
+ * Public entry points.
+ * ------------------------------------------------------------------------- */
+
+/* Forces at the current state of `mode` into F3 (double, 3n) and energies
+ * into E6 (kinetic = 0).  ext3 nullable. */
+void orc_compute_forces(orc *o, int mode, const double *ext3, double *F3, double *E6)
+{
+    if (mode == ORC_F32) {
+        float *ef = NULL;
+        if (ext3) {
+            ef = (float *)malloc(sizeof(float) * 3 * o->n);
+            for (int64_t i = 0; i < 3 * o->n; ++i)
+                ef[i] = (float)ext3[i];
+        }
+        forces_f32(o, ef);
+        free(ef);
+        for (int64_t i = 0; i < 3 * o->n; ++i)
+            F3[i] = o->isstatic[i / 3] ? 0.0 : o->ff[i];
+        for (int k = 0; k < BS_NUM_ENERGIES; ++k)
+            E6[k] = o->ef[k];
+    } else {
+        forces_f64(o, ext3);
+        for (int64_t i = 0; i < 3 * o->n; ++i)
+            F3[i] = o->isstatic[i / 3] ? 0.0 : o->fd[i];
+        for (int k = 0; k < BS_NUM_ENERGIES; ++k)
+            E6[k] = o->ed[k];
+    }
+}
+
+/* nsteps of SpringNetwork::computeStep (without rigid bodies, probe, grids).
+ * Energies of the last step go to E6 (nullable). */
+int orc_step(orc *o, int mode, int nsteps, const double *ext3, double *E6)
+{
+    int rc = BS_OK;
+    float *ef = NULL;
+    if (ext3 && mode == ORC_F32) {
+        ef = (float *)malloc(sizeof(float) * 3 * o->n);
+        for (int64_t i = 0; i < 3 * o->n; ++i)
+            ef[i] = (float)ext3[i];
+    }
+    for (int s = 0; s < nsteps && rc == BS_OK; ++s) {
+        if (mode == ORC_F32) {
+            forces_f32(o, ef);
+            rc = integrate_f32(o);
+        } else {
+            forces_f64(o, ext3);
+            rc = integrate_f64(o);
+        }
+    }
+    free(ef);
+    if (E6)
+        for (int k = 0; k < BS_NUM_ENERGIES; ++k)
+            E6[k] = mode == ORC_F32 ? (double)o->ef[k] : o->ed[k];
+    return rc;
+}
+
+int64_t orc_last_nonfinite(const orc *o) { return o->nonfinite; }
+
+int orc_num_threads(void) { return max_threads(); }

@@ -1,0 +1,259 @@
+"""ctypes wrappers of the oracle (liboracle.so) and, where it was built, of the
+reference's own hot path (_ref/libbsref.so).  TEST INFRASTRUCTURE ONLY: used
+by tests/, the smoke check and the benchmark's CPU-baseline leg as the checker,
+never by the engine.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_PATH = os.path.join(HERE, "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libbsref.so")
+
+F32, F64 = 0, 1
+
+_f32p = ctypes.POINTER(ctypes.c_float)
+_f64p = ctypes.POINTER(ctypes.c_double)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_vp = ctypes.c_void_p
+
+
+def _p(a, t):
+    return None if a is None else a.ctypes.data_as(ctypes.POINTER(t))
+
+
+def _params_type():
+    from biospring_amd._abi import Params
+    return Params
+
+
+_olib = None
+_rlib = None
+
+
+def oracle_lib():
+    global _olib
+    if _olib is None:
+        if not os.path.exists(ORACLE_PATH):
+            raise RuntimeError(f"oracle not built: {ORACLE_PATH}")
+        L = ctypes.CDLL(ORACLE_PATH)
+        P = ctypes.POINTER(_params_type())
+        L.orc_create.argtypes = [P, ctypes.c_int64] + [_f32p] * 9 + [_u8p, ctypes.c_int64, _u32p, _u32p, _f32p, _f32p]
+        L.orc_create.restype = _vp
+        L.orc_destroy.argtypes = [_vp]
+        L.orc_set_params.argtypes = [_vp, P]
+        L.orc_set_state.argtypes = [_vp, _f32p, _f32p]
+        L.orc_set_state_d.argtypes = [_vp, _f64p, _f64p]
+        L.orc_get_state.argtypes = [_vp, ctypes.c_int, _f64p, _f64p]
+        L.orc_compute_forces.argtypes = [_vp, ctypes.c_int, _f64p, _f64p, _f64p]
+        L.orc_step.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _f64p, _f64p]
+        L.orc_pairs.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _u32p, ctypes.c_int64]
+        L.orc_pairs.restype = ctypes.c_int64
+        L.orc_cells.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _i64p, _i64p, _i64p, _i64p, _f64p, _i64p]
+        L.orc_cells.restype = ctypes.c_int64
+        L.orc_num_springs.argtypes = [_vp]
+        L.orc_num_springs.restype = ctypes.c_int64
+        L.orc_last_nonfinite.argtypes = [_vp]
+        L.orc_last_nonfinite.restype = ctypes.c_int64
+        L.orc_term_energy.argtypes = [ctypes.c_int, ctypes.c_int, _f64p, ctypes.c_int]
+        L.orc_term_energy.restype = ctypes.c_double
+        L.orc_term_force.argtypes = [ctypes.c_int, ctypes.c_int, _f64p, ctypes.c_int, _f64p]
+        L.orc_term_force.restype = ctypes.c_double
+        L.orc_constants.argtypes = [_f64p]
+        L.orc_num_threads.restype = ctypes.c_int
+        _olib = L
+    return _olib
+
+
+def ref_available():
+    return os.path.exists(REF_PATH)
+
+
+def ref_lib():
+    global _rlib
+    if _rlib is None:
+        if not ref_available():
+            raise RuntimeError(f"reference library not built: {REF_PATH}")
+        L = ctypes.CDLL(REF_PATH)
+        P = ctypes.POINTER(_params_type())
+        L.ref_create.argtypes = [P, ctypes.c_int64] + [_f32p] * 9 + [_u8p, ctypes.c_int64, _u32p, _u32p, _f32p, _f32p]
+        L.ref_create.restype = _vp
+        L.ref_destroy.argtypes = [_vp]
+        L.ref_compute_forces.argtypes = [_vp, _f32p, _f32p, _f32p]
+        L.ref_step.argtypes = [_vp, ctypes.c_int, _f32p, _f32p]
+        L.ref_get_state.argtypes = [_vp, _f32p, _f32p]
+        L.ref_cells.argtypes = [_vp, ctypes.c_int, ctypes.c_float, ctypes.c_float, _i64p, _i64p, _i64p, _i64p,
+                                _f64p, _i64p]
+        L.ref_cells.restype = ctypes.c_int64
+        L.ref_pairs.argtypes = [P, ctypes.c_int, ctypes.c_int64, _f32p, _f32p, _f32p, _u8p, ctypes.c_int64, _u32p,
+                                _u32p, _f32p, _f32p, _u32p, ctypes.c_int64]
+        L.ref_pairs.restype = ctypes.c_int64
+        L.ref_set_threads.argtypes = [ctypes.c_int]
+        _rlib = L
+    return _rlib
+
+
+def _sys_args(system):
+    c = ctypes
+    s = system
+    return [ctypes.c_int64(s.n), _p(s.x, c.c_float), _p(s.v, c.c_float), _p(s.mass, c.c_float),
+            _p(s.charge, c.c_float), _p(s.radius, c.c_float), _p(s.eps, c.c_float), _p(s.sasa, c.c_float),
+            _p(s.transfer, c.c_float), _p(s.hydro, c.c_float), _p(s.flags, c.c_uint8), ctypes.c_int64(s.m),
+            _p(s.id1, c.c_uint32), _p(s.id2, c.c_uint32), _p(s.l0, c.c_float), _p(s.k, c.c_float)]
+
+
+class Oracle:
+    """CPU restatement of the step (mode F32: float-faithful, F64: FP64)."""
+
+    def __init__(self, system, params=None):
+        self.L = oracle_lib()
+        self.sys = system
+        self.params = (params if params is not None else system.params).copy()
+        self.h = self.L.orc_create(ctypes.byref(self.params), *_sys_args(system))
+        self.n = system.n
+
+    def close(self):
+        if self.h:
+            self.L.orc_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def set_params(self, params):
+        self.params = params.copy()
+        self.L.orc_set_params(self.h, ctypes.byref(self.params))
+
+    def set_state(self, x=None, v=None):
+        xx = None if x is None else np.ascontiguousarray(x, np.float32).reshape(-1)
+        vv = None if v is None else np.ascontiguousarray(v, np.float32).reshape(-1)
+        self.L.orc_set_state(self.h, _p(xx, ctypes.c_float), _p(vv, ctypes.c_float))
+
+    def set_state_f64(self, x=None, v=None):
+        xx = None if x is None else np.ascontiguousarray(x, np.float64).reshape(-1)
+        vv = None if v is None else np.ascontiguousarray(v, np.float64).reshape(-1)
+        self.L.orc_set_state_d(self.h, _p(xx, ctypes.c_double), _p(vv, ctypes.c_double))
+
+    def state(self, mode):
+        x = np.zeros(3 * self.n)
+        v = np.zeros(3 * self.n)
+        self.L.orc_get_state(self.h, mode, _p(x, ctypes.c_double), _p(v, ctypes.c_double))
+        return x.reshape(-1, 3), v.reshape(-1, 3)
+
+    def compute_forces(self, mode, ext=None):
+        F = np.zeros(3 * self.n)
+        E = np.zeros(6)
+        e = None if ext is None else np.ascontiguousarray(ext, np.float64).reshape(-1)
+        self.L.orc_compute_forces(self.h, mode, _p(e, ctypes.c_double), _p(F, ctypes.c_double),
+                                  _p(E, ctypes.c_double))
+        return F.reshape(-1, 3), E
+
+    def step(self, mode, nsteps, ext=None):
+        E = np.zeros(6)
+        e = None if ext is None else np.ascontiguousarray(ext, np.float64).reshape(-1)
+        rc = self.L.orc_step(self.h, mode, nsteps, _p(e, ctypes.c_double), _p(E, ctypes.c_double))
+        return rc, E
+
+    def pairs(self, kind, method=0, mode=F32):
+        cnt = self.L.orc_pairs(self.h, kind, method, mode, None, 0)
+        out = np.zeros(2 * max(cnt, 1), np.uint32)
+        self.L.orc_pairs(self.h, kind, method, mode, _p(out, ctypes.c_uint32), cnt)
+        return out[:2 * cnt].reshape(-1, 2)
+
+    def cells(self, kind, mode=F32):
+        nm = ctypes.c_int64()
+        nc = self.L.orc_cells(self.h, kind, mode, None, None, None, None, None, ctypes.byref(nm))
+        cell_of = np.zeros(self.n, np.int64)
+        start = np.zeros(nc + 1, np.int64)
+        mem = np.zeros(max(nm.value, 1), np.int64)
+        dims = np.zeros(3, np.int64)
+        box = np.zeros(6)
+        self.L.orc_cells(self.h, kind, mode, _p(cell_of, ctypes.c_int64), _p(start, ctypes.c_int64),
+                         _p(mem, ctypes.c_int64), _p(dims, ctypes.c_int64), _p(box, ctypes.c_double),
+                         ctypes.byref(nm))
+        return {"cell_of": cell_of, "start": start, "members": mem[:nm.value], "dims": dims, "box": box}
+
+
+def term_energy(term, mode, args):
+    a = np.ascontiguousarray(args, np.float64)
+    return oracle_lib().orc_term_energy(term, mode, _p(a, ctypes.c_double), a.size)
+
+
+def term_force(term, mode, args):
+    a = np.ascontiguousarray(args, np.float64)
+    out = np.zeros(3)
+    fm = oracle_lib().orc_term_force(term, mode, _p(a, ctypes.c_double), a.size, _p(out, ctypes.c_double))
+    return fm, out
+
+
+def constants():
+    out = np.zeros(4)
+    oracle_lib().orc_constants(_p(out, ctypes.c_double))
+    return {"spring": out[0], "elec": out[1], "kinetic": out[2], "coulomb": out[3]}
+
+
+class Reference:
+    """The reference's own SpringNetwork, compiled from its sources."""
+
+    def __init__(self, system, params=None):
+        self.L = ref_lib()
+        self.sys = system
+        self.params = (params if params is not None else system.params).copy()
+        self.h = self.L.ref_create(ctypes.byref(self.params), *_sys_args(system))
+        self.n = system.n
+
+    def close(self):
+        if self.h:
+            self.L.ref_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def compute_forces(self, ext=None):
+        F = np.zeros(3 * self.n, np.float32)
+        E = np.zeros(6, np.float32)
+        e = None if ext is None else np.ascontiguousarray(ext, np.float32).reshape(-1)
+        self.L.ref_compute_forces(self.h, _p(e, ctypes.c_float), _p(F, ctypes.c_float), _p(E, ctypes.c_float))
+        return F.reshape(-1, 3), E
+
+    def step(self, nsteps, ext=None):
+        E = np.zeros(6, np.float32)
+        e = None if ext is None else np.ascontiguousarray(ext, np.float32).reshape(-1)
+        self.L.ref_step(self.h, nsteps, _p(e, ctypes.c_float), _p(E, ctypes.c_float))
+        return E
+
+    def state(self):
+        x = np.zeros(3 * self.n, np.float32)
+        v = np.zeros(3 * self.n, np.float32)
+        self.L.ref_get_state(self.h, _p(x, ctypes.c_float), _p(v, ctypes.c_float))
+        return x.reshape(-1, 3), v.reshape(-1, 3)
+
+    def cells(self, kind, cutoff, skin=0.0):
+        nm = ctypes.c_int64()
+        nc = self.L.ref_cells(self.h, kind, cutoff, skin, None, None, None, None, None, ctypes.byref(nm))
+        cell_of = np.zeros(self.n, np.int64)
+        start = np.zeros(nc + 1, np.int64)
+        mem = np.zeros(max(nm.value, 1), np.int64)
+        dims = np.zeros(3, np.int64)
+        box = np.zeros(6)
+        self.L.ref_cells(self.h, kind, cutoff, skin, _p(cell_of, ctypes.c_int64), _p(start, ctypes.c_int64),
+                         _p(mem, ctypes.c_int64), _p(dims, ctypes.c_int64), _p(box, ctypes.c_double),
+                         ctypes.byref(nm))
+        return {"cell_of": cell_of, "start": start, "members": mem[:nm.value], "dims": dims, "box": box}
+
+
+def ref_pairs(system, kind, params=None):
+    L = ref_lib()
+    p = (params if params is not None else system.params).copy()
+    s = system
+    c = ctypes
+    args = [ctypes.byref(p), kind, s.n, _p(s.x, c.c_float), _p(s.charge, c.c_float), _p(s.hydro, c.c_float),
+            _p(s.flags, c.c_uint8), s.m, _p(s.id1, c.c_uint32), _p(s.id2, c.c_uint32), _p(s.l0, c.c_float),
+            _p(s.k, c.c_float)]
+    cnt = L.ref_pairs(*args, None, 0)
+    out = np.zeros(2 * max(cnt, 1), np.uint32)
+    L.ref_pairs(*args, _p(out, c.c_uint32), cnt)
+    return out[:2 * cnt].reshape(-1, 2)

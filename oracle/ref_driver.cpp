@@ -1,0 +1,410 @@
+// ref_driver.cpp -- TEST INFRASTRUCTURE ONLY.
+//
+// A thin C interface over the reference's own SpringNetwork, compiled from the
+// reference sources where they lie (see oracle/Makefile, target `ref`) into
+// oracle/_ref/libbsref.so.  It is used to produce golden vectors for the
+// oracle and, optionally, as the CPU baseline.  Nothing here re-implements
+// physics: every number comes from the reference code.
+//
+//  - ref_create: SpringNetwork::addParticle / addSpring / setup
+//    (src/spn/SpringNetwork.cpp:452-527,646-661).
+//  - ref_compute_forces: idleRun + computeForces (SpringNetwork.cpp:244-249).
+//  - ref_step: computeStep (SpringNetwork.cpp:251-268).
+//  - ref_cells: the reference nsearch::NeighborSearch cell list
+//    (src/nsearch.hpp:277-293,342-390), read through a subclass.
+//  - ref_pairs: the processed pairs, recorded by a ForceField subclass whose
+//    energy callbacks see particle ids encoded in a per-particle property;
+//    the pair decisions are made by the reference's own Particle::add*Force.
+#include "SpringNetwork.h"
+#include "configuration/Configuration.hpp"
+#include "forcefield/ForceField.h"
+#include "rigidbody/RigidBodiesManager.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <mutex>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "../include/bs_api.h"
+
+using biospring::spn::Particle;
+using biospring::spn::SpringNetwork;
+
+namespace
+{
+
+const char * steric_mode_name(int m)
+{
+    switch (m)
+    {
+    case BS_STERIC_AMBER_12_6: return "lennard-jones-12-6Amber";
+    case BS_STERIC_LEWITT_8_6: return "lennard-jones-8-6Lewitt";
+    case BS_STERIC_ZACHARIAS_8_6: return "lennard-jones-8-6Zacharias";
+    default: return "linear";
+    }
+}
+
+biospring::configuration::Configuration make_config(const bs_params * p)
+{
+    auto c = biospring::configuration::defaultConfiguration();
+    c.sim.nbsteps = -1;
+    c.sim.timestep = p->timestep;
+    c.sim.neighborskin = p->neighborskin;
+    c.spring.enable = p->spring_enable != 0;
+    c.spring.scale = p->spring_scale;
+    c.steric.enable = p->steric_enable != 0;
+    c.steric.mode = steric_mode_name(p->steric_mode);
+    c.steric.cutoff = p->steric_cutoff;
+    c.steric.gridscale = p->steric_gridscale;
+    c.electrostatic.enable = p->electrostatic_enable != 0;
+    c.electrostatic.cutoff = p->electrostatic_cutoff;
+    c.electrostatic.scale = p->electrostatic_scale;
+    c.electrostatic.dielectric = p->electrostatic_dielectric;
+    c.viscosity.enable = p->viscosity_enable != 0;
+    c.viscosity.value = p->viscosity_value;
+    c.imp.enable = p->imp_enable != 0;
+    c.imp.scale = p->imp_scale;
+    c.hydrophobicity.enable = p->hydrophobicity_enable != 0;
+    c.hydrophobicity.cutoff = p->hydrophobicity_cutoff;
+    c.hydrophobicity.scale = p->hydrophobicity_scale;
+    return c;
+}
+
+// Records the (j, i) ids of each processed pair; ids are encoded as
+// epsilon (steric), |charge|-1 (electrostatic), |hydrophobicity|-1.
+class RecordingForceField : public biospring::forcefield::ForceField
+{
+  public:
+    mutable std::mutex mu;
+    mutable std::vector<uint32_t> ij;
+    int kind = BS_NS_STERIC;
+
+    float computeStericEnergy(float, float, float e1, float e2, float) const override
+    {
+        if (kind == BS_NS_STERIC)
+            push((uint32_t)e1, (uint32_t)e2);
+        return 0.0f;
+    }
+    float computeStericForceModule(float, float, float, float, float) const override { return 0.0f; }
+    float computeElectrostaticEnergy(float c1, float c2, float) const override
+    {
+        if (kind == BS_NS_ELECTROSTATIC)
+            push((uint32_t)(std::fabs(c1) - 1.0f), (uint32_t)(std::fabs(c2) - 1.0f));
+        return 0.0f;
+    }
+    float computeElectrostaticForceModule(float, float, float) const override { return 0.0f; }
+    float computeHydrophobicityEnergy(float h1, float h2, float) const override
+    {
+        if (kind == BS_NS_HYDROPHOBIC)
+            push((uint32_t)(std::fabs(h1) - 1.0f), (uint32_t)(std::fabs(h2) - 1.0f));
+        return 0.0f;
+    }
+    float computeHydrophobicityForceModule(float, float, float) const override { return 0.0f; }
+    float computeSpringEnergy(float, float, float) const override { return 0.0f; }
+    float computeSpringForceModule(float, float, float) const override { return 0.0f; }
+    float computeIMPEnergy(float, float, float, float, float) const override { return 0.0f; }
+    Vector3f computeIMPForceVector(float, float, float, float, float) const override { return Vector3f(); }
+
+  private:
+    void push(uint32_t a, uint32_t b) const
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        ij.push_back(std::min(a, b));
+        ij.push_back(std::max(a, b));
+    }
+};
+
+class RefNet : public SpringNetwork
+{
+  public:
+    void swap_forcefield(std::unique_ptr<biospring::forcefield::ForceField> ff) { _ff = std::move(ff); }
+    std::vector<Particle> & particles() { return _particles; }
+};
+
+using Container = std::vector<Particle>;
+
+struct ExposedSearch : public biospring::nsearch::NeighborSearch<Container>
+{
+    using Base = biospring::nsearch::NeighborSearch<Container>;
+    ExposedSearch(const Container & c, float cutoff, std::vector<size_t> inc, float skin)
+        : Base(c, cutoff, std::move(inc), skin)
+    {
+    }
+    ExposedSearch(const Container & c, float cutoff, float skin) : Base(c, cutoff, skin) {}
+    int64_t cell_of(size_t i) const { return (int64_t)_compute_cell(biospring::concepts::locatable::get_position(_system->at(i))); }
+    void dims(int64_t d[3]) const
+    {
+        d[0] = (int64_t)_ncells_x;
+        d[1] = (int64_t)_ncells_y;
+        d[2] = (int64_t)_ncells_z;
+    }
+    void box(double b[6]) const
+    {
+        for (int k = 0; k < 6; ++k)
+            b[k] = _box.boundaries()[k];
+    }
+    const std::vector<size_t> * cell(size_t id) const
+    {
+        auto it = _cells.find(id);
+        return it == _cells.end() ? nullptr : &it->second;
+    }
+};
+
+struct RefSys
+{
+    bs_params p;
+    std::unique_ptr<RefNet> net;
+};
+
+RefNet * build_net(const bs_params * p, int64_t n, const float * x3, const float * v3, const float * mass,
+                   const float * charge, const float * radius, const float * eps, const float * sasa,
+                   const float * transfer, const float * hydro, const uint8_t * flags, int64_t m,
+                   const uint32_t * id1, const uint32_t * id2, const float * l0, const float * k)
+{
+    auto * net = new RefNet();
+    for (int64_t i = 0; i < n; ++i)
+    {
+        Particle q;
+        q.setPosition(Vector3f(x3[3 * i], x3[3 * i + 1], x3[3 * i + 2]));
+        if (v3)
+            q.setVelocity(Vector3f(v3[3 * i], v3[3 * i + 1], v3[3 * i + 2]));
+        if (mass)
+            q.setMass(mass[i]);
+        if (charge)
+            q.setCharge(charge[i]);
+        if (radius)
+            q.setRadius(radius[i]);
+        if (eps)
+            q.setEpsilon(eps[i]);
+        if (sasa)
+            q.setSolventAccessibilitySurface(sasa[i]);
+        if (transfer)
+            q.setTransferEnergyByAccessibleSurface(transfer[i]);
+        if (hydro)
+            q.setHydrophobicity(hydro[i]);
+        q.setStatic(flags ? (flags[i] & BS_FLAG_STATIC) != 0 : false);
+        net->addParticle(q);
+    }
+    for (int64_t s = 0; s < m; ++s)
+        net->addSpring(id1[s], id2[s], l0[s], k[s]);
+    net->setup(make_config(p));
+    auto * ff = net->getForceField();
+    ff->setImpDoubleMembraneUpperMembOffset(p->imp_upper_offset);
+    ff->setImpDoubleMembraneLowerMembOffset(p->imp_lower_offset);
+    ff->setImpDoubleMembraneUpperMembTubeCurv(p->imp_upper_curvature);
+    ff->setImpDoubleMembraneLowerMembTubeCurv(p->imp_lower_curvature);
+    return net;
+}
+
+void read_energies(const SpringNetwork & net, float * E6)
+{
+    E6[BS_E_SPRING] = net.getSpringEnergy();
+    E6[BS_E_ELECTROSTATIC] = net.getElectrostaticEnergy();
+    E6[BS_E_STERIC] = net.getStericEnergy();
+    E6[BS_E_KINETIC] = net.getKineticEnergy();
+    E6[BS_E_IMP] = net.getIMPEnergy();
+    E6[BS_E_HYDROPHOBIC] = net.getHydrophobicEnergy();
+}
+
+} // namespace
+
+extern "C" {
+
+void ref_set_threads(int n)
+{
+#ifdef _OPENMP
+    omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
+void * ref_create(const bs_params * p, int64_t n, const float * x3, const float * v3, const float * mass,
+                  const float * charge, const float * radius, const float * eps, const float * sasa,
+                  const float * transfer, const float * hydro, const uint8_t * flags, int64_t m,
+                  const uint32_t * id1, const uint32_t * id2, const float * l0, const float * k)
+{
+    auto * s = new RefSys();
+    s->p = *p;
+    s->net.reset(build_net(p, n, x3, v3, mass, charge, radius, eps, sasa, transfer, hydro, flags, m, id1, id2, l0, k));
+    return s;
+}
+
+void ref_destroy(void * h) { delete static_cast<RefSys *>(h); }
+
+// idleRun + computeForces; forces of every particle (static ones too, as the
+// reference accumulates them) and the six energies.
+void ref_compute_forces(void * h, const float * ext3, float * F3, float * E6)
+{
+    auto * s = static_cast<RefSys *>(h);
+    RefNet & net = *s->net;
+    net.idleRun();
+    if (ext3)
+        for (unsigned i = 0; i < net.getNumberOfParticles(); ++i)
+        {
+            float f[3] = {ext3[3 * i], ext3[3 * i + 1], ext3[3 * i + 2]};
+            net.setForce(i, f);
+        }
+    net.computeForces();
+    for (unsigned i = 0; i < net.getNumberOfParticles(); ++i)
+    {
+        const Vector3f f = net.getParticle(i).getForce();
+        F3[3 * i] = f.getX();
+        F3[3 * i + 1] = f.getY();
+        F3[3 * i + 2] = f.getZ();
+    }
+    read_energies(net, E6);
+}
+
+// nsteps of computeStep, each preceded by setForce(ext) when ext3 != NULL
+// (the interactor path: forces are added before computeForces).
+int ref_step(void * h, int nsteps, const float * ext3, float * E6)
+{
+    auto * s = static_cast<RefSys *>(h);
+    RefNet & net = *s->net;
+    for (int t = 0; t < nsteps; ++t)
+    {
+        if (ext3)
+        {
+            // computeStep begins with idleRun, which resets energies and adds
+            // interactor forces; setForce before it is equivalent because the
+            // particle forces were reset at the end of the previous step.
+            for (unsigned i = 0; i < net.getNumberOfParticles(); ++i)
+            {
+                float f[3] = {ext3[3 * i], ext3[3 * i + 1], ext3[3 * i + 2]};
+                net.setForce(i, f);
+            }
+        }
+        net.computeStep();
+    }
+    if (E6)
+        read_energies(net, E6);
+    return 0;
+}
+
+void ref_get_state(void * h, float * x3, float * v3)
+{
+    auto * s = static_cast<RefSys *>(h);
+    RefNet & net = *s->net;
+    for (unsigned i = 0; i < net.getNumberOfParticles(); ++i)
+    {
+        const Particle & q = net.getParticle(i);
+        if (x3)
+        {
+            x3[3 * i] = q.getX();
+            x3[3 * i + 1] = q.getY();
+            x3[3 * i + 2] = q.getZ();
+        }
+        if (v3)
+        {
+            const Vector3f v = q.getVelocity();
+            v3[3 * i] = v.getX();
+            v3[3 * i + 1] = v.getY();
+            v3[3 * i + 2] = v.getZ();
+        }
+    }
+}
+
+// Cell list of search `kind` built by the reference NeighborSearch over the
+// network's current particles.  Returns the number of cells; cell_of[n]
+// (-1 for non-members), start[ncells+1], members[nmem], dims[3], box[6].
+int64_t ref_cells(void * h, int kind, float cutoff, float skin, int64_t * cell_of, int64_t * start,
+                  int64_t * members, int64_t * dims, double * box, int64_t * nmem)
+{
+    auto * s = static_cast<RefSys *>(h);
+    RefNet & net = *s->net;
+    const Container & parts = net.getParticles();
+    std::vector<size_t> inc;
+    for (size_t i = 0; i < parts.size(); ++i)
+    {
+        const bool member = kind == BS_NS_ELECTROSTATIC ? parts[i].isCharged()
+                            : kind == BS_NS_HYDROPHOBIC ? parts[i].isHydrophobic()
+                                                        : true;
+        if (member)
+            inc.push_back(i);
+    }
+    std::unique_ptr<ExposedSearch> ns;
+    if (kind == BS_NS_STERIC)
+        ns.reset(new ExposedSearch(parts, cutoff, skin));
+    else
+        ns.reset(new ExposedSearch(parts, cutoff, inc, skin));
+    int64_t d[3];
+    ns->dims(d);
+    const int64_t ncells = d[0] * d[1] * d[2];
+    if (dims)
+        std::memcpy(dims, d, sizeof d);
+    if (box)
+        ns->box(box);
+    std::vector<int64_t> member_flag(parts.size(), 0);
+    for (size_t i : inc)
+        member_flag[i] = 1;
+    if (cell_of)
+        for (size_t i = 0; i < parts.size(); ++i)
+            cell_of[i] = member_flag[i] ? ns->cell_of(i) : -1;
+    int64_t w = 0;
+    for (int64_t c = 0; c < ncells; ++c)
+    {
+        if (start)
+            start[c] = w;
+        const auto * v = ns->cell((size_t)c);
+        if (v)
+            for (size_t i : *v)
+            {
+                if (members)
+                    members[w] = (int64_t)i;
+                ++w;
+            }
+    }
+    if (start)
+        start[ncells] = w;
+    if (nmem)
+        *nmem = w;
+    return ncells;
+}
+
+// Processed pairs of `kind` at the current positions, recorded from the
+// reference's own pair loops.  Builds a copy of the system whose per-particle
+// property for that kind encodes the particle id (membership unchanged).
+int64_t ref_pairs(const bs_params * p, int kind, int64_t n, const float * x3, const float * charge,
+                  const float * hydro, const uint8_t * flags, int64_t m, const uint32_t * id1,
+                  const uint32_t * id2, const float * l0, const float * k, uint32_t * ij, int64_t cap)
+{
+    std::vector<float> enc_eps(n), enc_q(n), enc_h(n);
+    for (int64_t i = 0; i < n; ++i)
+    {
+        enc_eps[i] = (float)i;
+        const float q = charge ? charge[i] : 0.0f;
+        const float hh = hydro ? hydro[i] : 0.0f;
+        enc_q[i] = q != 0.0f ? (float)(i + 1) : 0.0f;
+        enc_h[i] = hh != 0.0f ? (float)(i + 1) : 0.0f;
+    }
+    bs_params q = *p;
+    q.viscosity_enable = 0;
+    q.imp_enable = 0;
+    std::unique_ptr<RefNet> net(build_net(&q, n, x3, nullptr, nullptr, enc_q.data(), nullptr, enc_eps.data(), nullptr,
+                                          nullptr, enc_h.data(), flags, m, id1, id2, l0, k));
+    auto * rec = new RecordingForceField();
+    rec->kind = kind;
+    net->swap_forcefield(std::unique_ptr<biospring::forcefield::ForceField>(rec));
+    net->idleRun();
+    net->computeParticleForces();
+    std::vector<std::pair<uint32_t, uint32_t>> v;
+    for (size_t t = 0; t + 1 < rec->ij.size(); t += 2)
+        v.emplace_back(rec->ij[t], rec->ij[t + 1]);
+    std::sort(v.begin(), v.end());
+    const int64_t cnt = (int64_t)v.size();
+    if (ij)
+        for (int64_t t = 0; t < cnt && t < cap; ++t)
+        {
+            ij[2 * t] = v[t].first;
+            ij[2 * t + 1] = v[t].second;
+        }
+    return cnt;
+}
+
+} // extern "C"

@@ -1,0 +1,149 @@
+"""GPU parity: the HIP engine, called through the C ABI, against the oracle.
+
+Bit-exact: reference cell lists and processed pair sets.  Within 1e-5 of the
+FP64 oracle: per-particle forces (floor 1e-6 max|F|) and energies.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from biospring_amd import _abi, workloads
+from biospring_amd.network import SpringNetwork
+from oracle import pyoracle as po
+from tests.parity import ENERGY_RTOL, FORCE_RTOL, energy_error, force_errors
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    # name: (workload, seed, n, param overrides)
+    "enm2k": (0, 1, 0, {}),
+    "lj3k_ddd": (1, 2, 3000, {}),
+    "lj3k_const": (1, 2, 3000, {"dielectric_model": 0}),
+    "lj20k": (1, 5, 20000, {}),
+}
+
+
+def _system(name):
+    cfg, seed, n, over = CASES[name]
+    s = workloads.generate(cfg, seed=seed, n=n)
+    for k, v in over.items():
+        setattr(s.params, k, v)
+    return s
+
+
+def _log(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_cells_bitexact(name):
+    s = _system(name)
+    _log("case", name, "n", s.n, "m", s.m)
+    net = SpringNetwork.from_system(s)
+    net.setup()
+    o = po.Oracle(s)
+    for kind in (_abi.NS_STERIC, _abi.NS_ELECTROSTATIC):
+        g = net.getCells(kind)
+        r = o.cells(kind)
+        for key in ("dims", "box", "cell_of", "start", "members"):
+            assert np.array_equal(np.asarray(g[key]), np.asarray(r[key])), (name, kind, key)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_pairs_bitexact(name):
+    s = _system(name)
+    _log("case", name)
+    net = SpringNetwork.from_system(s)
+    net.setup()
+    o = po.Oracle(s)
+    for kind in (_abi.NS_STERIC, _abi.NS_ELECTROSTATIC):
+        g = net.getPairs(kind)
+        r = o.pairs(kind, method=0)
+        _log("  kind", kind, "gpu", len(g), "oracle", len(r))
+        assert g.shape == r.shape and np.array_equal(g, r), (name, kind)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_forces_vs_fp64(name):
+    s = _system(name)
+    _log("case", name)
+    net = SpringNetwork.from_system(s)
+    F, E = net.computeForces()
+    o = po.Oracle(s)
+    Fd, Ed = o.compute_forces(po.F64)
+    dyn = (s.flags & 1) == 0
+    per, glob = force_errors(F, Fd, dyn)
+    _log("  force err per-particle max", per, "global", glob)
+    _log("  E gpu", E, "\n  E f64", Ed)
+    assert per <= FORCE_RTOL, per
+    for slot in range(6):
+        err = energy_error(E[slot], Ed[slot], floor=1e-3)
+        assert err <= ENERGY_RTOL, (slot, E[slot], Ed[slot], err)
+
+
+@pytest.mark.parametrize("name", ["enm2k", "lj3k_ddd"])
+def test_forces_after_steps(name):
+    """Move the system with the GPU, then compare forces at the moved state."""
+    s = _system(name)
+    net = SpringNetwork.from_system(s)
+    net.run(20)
+    x, v = net.getState()
+    F, E = net.computeForces()
+    o = po.Oracle(s)
+    o.set_state(x, v)
+    Fd, Ed = o.compute_forces(po.F64)
+    dyn = (s.flags & 1) == 0
+    per, glob = force_errors(F, Fd, dyn)
+    _log("  after 20 steps: force err", per, glob, "\n  E gpu", E, "\n  E f64", Ed)
+    assert per <= FORCE_RTOL
+    for slot in range(6):
+        assert energy_error(E[slot], Ed[slot], floor=1e-3) <= ENERGY_RTOL, slot
+
+
+@pytest.mark.parametrize("name", ["enm2k", "lj3k_ddd"])
+def test_trajectory_vs_fp64(name):
+    """100 steps: GPU (FP32) against the FP64 oracle trajectory."""
+    s = _system(name)
+    net = SpringNetwork.from_system(s)
+    o = po.Oracle(s)
+    for nsteps in (1, 10, 100):
+        done = net.counters()["steps"]
+        net.run(nsteps - done)
+        rc, Ed = o.step(po.F64, nsteps - done)
+        assert rc == 0
+        x, _ = net.getState()
+        xd, _ = o.state(po.F64)
+        dev = np.linalg.norm(x - xd, axis=1)
+        E = net.getEnergies()
+        _log(f"  step {nsteps}: rms dev {np.sqrt((dev ** 2).mean()):.3e} max {dev.max():.3e}")
+        _log("   E gpu", E, "\n   E f64", Ed)
+        assert dev.max() < 1e-3 * max(1, nsteps / 10)
+        for slot in range(6):
+            assert energy_error(E[slot], Ed[slot], floor=1e-2) <= 1e-4, (slot, E[slot], Ed[slot])
+
+
+def test_determinism():
+    s = _system("lj3k_ddd")
+    outs = []
+    for _ in range(2):
+        net = SpringNetwork.from_system(s)
+        net.run(15)
+        x, v = net.getState()
+        outs.append((x, v, net.getEnergies()))
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.array_equal(outs[0][1], outs[1][1])
+    assert np.array_equal(outs[0][2], outs[1][2])
+
+
+def test_nonfinite_reported():
+    s = _system("enm2k")
+    net = SpringNetwork.from_system(s)
+    ext = np.zeros((s.n, 3), np.float32)
+    ext[17, 0] = np.inf
+    net.setForce(ext)
+    with pytest.raises(_abi.EngineError) as ei:
+        net.run(1)
+    assert ei.value.code == _abi.BS_ERR_NONFINITE
+    assert net.lastNonfiniteParticle() == 17
