@@ -122,11 +122,16 @@ def main():
     }
 
     x, v = net.getState()
+    # Conservative force (springs + pairs): the viscous term -gamma v is an
+    # exact product in both codes and is compared separately in the tests.
+    pc = s.params.copy()
+    pc.viscosity_enable = 0
+    net.setParams(pc)
     F, E = net.computeForces()
     if not args.no_fp64 or not args.no_cpu:
         from oracle import pyoracle as po
         t = time.perf_counter()
-        o = po.Oracle(s)
+        o = po.Oracle(s, params=pc)
         o.set_state(x, v)
         log(f"oracle setup {time.perf_counter() - t:.1f}s")
         if not args.no_fp64:
@@ -141,8 +146,10 @@ def main():
                                         "global_rel_l2": float(np.linalg.norm(F[dyn] - Fd[dyn]) /
                                                                np.linalg.norm(Fd[dyn])),
                                         "energy_rel_max": float(max(erel)),
+                                        "forces": "springs + LJ + Coulomb (viscous term excluded)",
                                         "state": f"after {args.warmup + args.steps} steps"}
             log(f"fp64 check {time.perf_counter() - t:.1f}s")
+        o.set_params(s.params)
         if not args.no_cpu:
             o.set_state(x, v)
             t = time.perf_counter()

@@ -100,7 +100,12 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"engine library not built: {LIB_PATH} (run __graft_entry__.build())")
-    L = ctypes.CDLL(LIB_PATH)
+    _lib = bind(ctypes.CDLL(LIB_PATH))
+    return _lib
+
+
+def bind(L):
+    """Declares the C signatures of include/bs_api.h on a loaded library."""
     L.bs_default_params.argtypes = [ctypes.POINTER(Params)]
     L.bs_default_params.restype = None
     L.bs_create.argtypes = [ctypes.POINTER(Params), ctypes.POINTER(_vp)]
@@ -127,7 +132,6 @@ def lib():
     L.bs_get_profile.argtypes = [_vp, _f64p, _i64p]
     L.bs_get_cells.argtypes = [_vp, ctypes.c_int32, _i64p, _i64p, _i64p, _i32p, _i32p, _i64p, _f64p]
     L.bs_get_pairs.argtypes = [_vp, ctypes.c_int32, _u32p, ctypes.c_int64, _i64p]
-    _lib = L
     return L
 
 

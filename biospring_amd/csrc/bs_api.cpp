@@ -158,7 +158,28 @@ void free_device(bs_handle *h)
     h->cell_of = h->counts = h->cstart = h->cursor = h->ctmp = h->cidx = nullptr;
     h->box_part = h->box = h->pair_part = h->spring_part = h->energies_d = nullptr;
     h->status = h->counters = nullptr;
+    h->cell_cap = 0;
+    h->nedges = 0;
     h->ready = false;
+}
+
+// Host-side guard run before any kernel sequence: every buffer the kernels
+// touch must be allocated at the size they assume.
+int check_device(bs_handle *h)
+{
+    const void *need[] = {h->pos[0], h->pos[1], h->vel, h->fnb, h->jprm, h->iprm, h->xref, h->fext, h->fout,
+                          h->sp_ptr, h->sp_j, h->sp_k, h->sp_l0, h->perm_d, h->cell_of, h->ctmp, h->cidx,
+                          h->counts, h->cstart, h->cursor, h->box, h->box_part, h->pair_part, h->spring_part,
+                          h->energies_d, h->status, h->counters};
+    for (const void *q : need)
+        if (!q)
+            return fail(h, BS_ERR_STATE, "internal: device buffer missing");
+    if (h->p.compensated_positions && (!h->plo[0] || !h->plo[1]))
+        return fail(h, BS_ERR_STATE, "internal: compensated-position buffers missing");
+    for (const auto &l : h->lists)
+        if (!l.nbl || !l.nbn || l.cap <= 0 || l.ncells > h->cell_cap)
+            return fail(h, BS_ERR_STATE, "internal: neighbour list buffers missing");
+    return BS_OK;
 }
 
 inline uint64_t spread3(uint64_t v)
@@ -563,6 +584,7 @@ int do_setup(bs_handle *h)
     if (!h->have_particles)
         return fail(h, BS_ERR_STATE, "bs_setup: no particles");
     free_device(h);
+    h->cur = 0;
     const int64_t n = h->n;
     if (n > INT32_MAX / 64)
         return fail(h, BS_ERR_ARG, "bs_setup: too many particles");
@@ -720,8 +742,12 @@ int do_setup(bs_handle *h)
 int ensure_ready(bs_handle *h)
 {
     if (!h->ready)
-        return do_setup(h);
-    return BS_OK;
+    {
+        const int rc = do_setup(h);
+        if (rc)
+            return rc;
+    }
+    return check_device(h);
 }
 
 int kind_term(int kind) { return kind == BS_NS_ELECTROSTATIC ? TERM_ELEC : kind == BS_NS_HYDROPHOBIC ? TERM_HYDRO : TERM_STERIC; }

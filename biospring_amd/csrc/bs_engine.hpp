@@ -46,7 +46,23 @@ struct Consts
     double pi;
 };
 
-Consts make_consts();
+// Unit constants evaluated in double in the order of
+// src/forcefield/constants.hpp:36-118.
+inline Consts make_consts()
+{
+    const double kj_to_j = 1.0E3, j_to_kj = 1.0E-3, e2c = 1.60217653E-19, a2m = 1.0E-10, m2a = 1.0E10,
+                 da2kg = 1.66054E-27, s2fs = 1.0E15, fs2s = 1.0E-15, pi = 3.1415927, eps0 = 8.854187817E-12,
+                 na = 6.02214169E23;
+    const double newton = m2a * (1.0 / da2kg) * fs2s * fs2s;
+    Consts c;
+    c.c_spring = kj_to_j / na * m2a * newton;
+    c.c_elec = newton * (e2c * e2c) / (eps0 * a2m * a2m);
+    c.c_kinetic = (a2m * s2fs) * (a2m * s2fs) * na * da2kg * j_to_kj;
+    c.c_coulomb = 1.0 / (4.0 * pi * eps0);
+    c.e_coul = c.c_coulomb * (e2c * e2c) / a2m * na * 1.0E-3;
+    c.pi = pi;
+    return c;
+}
 
 // Parameters of one pair kernel launch.
 struct PairArgs

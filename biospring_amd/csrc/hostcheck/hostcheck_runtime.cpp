@@ -1,0 +1,542 @@
+// hostcheck_runtime.cpp -- CPU-side checker of the engine's host logic.
+//
+// Links the engine's real host code (bs_api.cpp) against a recording fake of
+// the HIP runtime and against launchers that, instead of launching, verify
+// that every buffer a kernel would touch is a live allocation of the extent
+// the kernel assumes.  The binning and list-build launchers also perform
+// their work on the CPU so that the host's read-backs (box, list lengths) see
+// consistent data.  Any violation aborts the process with a message.  Test
+// infrastructure only; never loaded by the engine.
+#include "hip/hip_runtime.h"
+
+#include "../bs_engine.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <mutex>
+#include <vector>
+
+namespace
+{
+std::map<uintptr_t, size_t> &allocs()
+{
+    static std::map<uintptr_t, size_t> m;
+    return m;
+}
+std::mutex &mu()
+{
+    static std::mutex m;
+    return m;
+}
+} // namespace
+
+bool hc_valid(const void *p, size_t bytes)
+{
+    std::lock_guard<std::mutex> lk(mu());
+    const uintptr_t a = (uintptr_t)p;
+    auto it = allocs().upper_bound(a);
+    if (it == allocs().begin())
+        return false;
+    --it;
+    return a >= it->first && a + bytes <= it->first + it->second;
+}
+
+void hc_fail(const char *what)
+{
+    std::fprintf(stderr, "HOSTCHECK VIOLATION: %s\n", what);
+    std::fflush(stderr);
+    std::abort();
+}
+
+static void need(const void *p, size_t bytes, const char *what)
+{
+    if (bytes == 0)
+        return;
+    if (!hc_valid(p, bytes))
+    {
+        char buf[256];
+        std::snprintf(buf, sizeof buf, "%s: %p (+%zu bytes) is not inside a live device allocation", what, p, bytes);
+        hc_fail(buf);
+    }
+}
+
+hipError_t hipMalloc(void **p, size_t bytes)
+{
+    *p = std::calloc(bytes ? bytes : 1, 1);
+    std::lock_guard<std::mutex> lk(mu());
+    allocs()[(uintptr_t)*p] = bytes;
+    return *p ? hipSuccess : hipErrorOutOfMemory;
+}
+
+hipError_t hipFree(void *p)
+{
+    if (!p)
+        return hipSuccess;
+    {
+        std::lock_guard<std::mutex> lk(mu());
+        auto it = allocs().find((uintptr_t)p);
+        if (it == allocs().end())
+            hc_fail("hipFree of a pointer that is not a live allocation");
+        allocs().erase(it);
+    }
+    std::free(p);
+    return hipSuccess;
+}
+
+hipError_t hipMemcpy(void *dst, const void *src, size_t bytes, hipMemcpyKind kind)
+{
+    if (kind == hipMemcpyHostToDevice)
+        need(dst, bytes, "hipMemcpy H2D destination");
+    if (kind == hipMemcpyDeviceToHost)
+        need(src, bytes, "hipMemcpy D2H source");
+    std::memcpy(dst, src, bytes);
+    return hipSuccess;
+}
+
+hipError_t hipMemcpyAsync(void *dst, const void *src, size_t bytes, hipMemcpyKind kind, hipStream_t)
+{
+    return hipMemcpy(dst, src, bytes, kind);
+}
+
+hipError_t hipMemsetAsync(void *dst, int v, size_t bytes, hipStream_t)
+{
+    need(dst, bytes, "hipMemsetAsync");
+    std::memset(dst, v, bytes);
+    return hipSuccess;
+}
+
+hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned)
+{
+    *s = reinterpret_cast<hipStream_t>(new int(1));
+    return hipSuccess;
+}
+hipError_t hipStreamDestroy(hipStream_t s)
+{
+    delete reinterpret_cast<int *>(s);
+    return hipSuccess;
+}
+hipError_t hipGetDeviceCount(int *n)
+{
+    *n = 1;
+    return hipSuccess;
+}
+hipError_t hipGetDevice(int *d)
+{
+    *d = 0;
+    return hipSuccess;
+}
+hipError_t hipGetDeviceProperties(hipDeviceProp_t *p, int)
+{
+    std::strcpy(p->gcnArchName, "gfx950:sramecc+:xnack-");
+    return hipSuccess;
+}
+hipError_t hipEventCreate(hipEvent_t *e)
+{
+    *e = reinterpret_cast<hipEvent_t>(new int(0));
+    return hipSuccess;
+}
+hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
+hipError_t hipEventElapsedTime(float *ms, hipEvent_t, hipEvent_t)
+{
+    *ms = 0.0f;
+    return hipSuccess;
+}
+hipError_t hipEventDestroy(hipEvent_t e)
+{
+    delete reinterpret_cast<int *>(e);
+    return hipSuccess;
+}
+const char *hipGetErrorString(hipError_t) { return "hostcheck error"; }
+
+extern "C" int64_t hc_live_allocations()
+{
+    std::lock_guard<std::mutex> lk(mu());
+    return (int64_t)allocs().size();
+}
+
+// ---------------------------------------------------------------------------
+// Launchers: extent checks + CPU work where the host reads results back.
+// ---------------------------------------------------------------------------
+namespace bs
+{
+
+static bool gated_off(const int32_t *gate)
+{
+    if (!gate)
+        return false;
+    need(gate, 4, "gate");
+    return *gate == 0;
+}
+
+static uint32_t fbits(float f)
+{
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return u;
+}
+
+void launch_box_reduce(const float4 *pos, int32_t n, double *partial, double *box, int32_t nblocks,
+                       const int32_t *gate, hipStream_t)
+{
+    need(pos, sizeof(float4) * n, "box: pos");
+    need(partial, sizeof(double) * 6 * nblocks, "box: partial");
+    need(box, sizeof(double) * 6, "box: box");
+    if (nblocks < 1 || nblocks > 4096)
+        hc_fail("box: bad block count");
+    if (gated_off(gate))
+        return;
+    for (int c = 0; c < 3; ++c)
+    {
+        box[c] = INFINITY;
+        box[3 + c] = -INFINITY;
+    }
+    for (int32_t i = 0; i < n; ++i)
+    {
+        const float xs[3] = {pos[i].x, pos[i].y, pos[i].z};
+        for (int c = 0; c < 3; ++c)
+        {
+            box[c] = std::min(box[c], (double)xs[c]);
+            box[3 + c] = std::max(box[3 + c], (double)xs[c]);
+        }
+    }
+}
+
+static int32_t internal_coord(float x, double mn, double len, int32_t dim, float min_edge)
+{
+    const double e = std::max((double)min_edge, len / (double)dim);
+    int32_t c = (int32_t)((x - mn) / e);
+    c = c < 0 ? 0 : c;
+    return c < dim - 1 ? c : dim - 1;
+}
+
+void launch_cell_ids(const float4 *pos, const float4 *jprm, uint32_t mask, int32_t n, const double *box, int32_t d0,
+                     int32_t d1, int32_t d2, double edge_exact, float min_edge, int32_t exact_mode, int32_t *cell_of,
+                     int32_t *counts, const int32_t *gate, hipStream_t)
+{
+    const int64_t nc = (int64_t)d0 * d1 * d2;
+    need(pos, sizeof(float4) * n, "cell_ids: pos");
+    need(jprm, sizeof(float4) * n, "cell_ids: jprm");
+    need(box, sizeof(double) * 6, "cell_ids: box");
+    need(cell_of, sizeof(int32_t) * n, "cell_ids: cell_of");
+    need(counts, sizeof(int32_t) * nc, "cell_ids: counts");
+    if (gated_off(gate))
+        return;
+    const int32_t dims[3] = {d0, d1, d2};
+    for (int32_t i = 0; i < n; ++i)
+    {
+        if (mask && !(fbits(jprm[i].w) & mask))
+        {
+            cell_of[i] = -1;
+            continue;
+        }
+        const float xs[3] = {pos[i].x, pos[i].y, pos[i].z};
+        int32_t c[3];
+        for (int a = 0; a < 3; ++a)
+        {
+            if (exact_mode)
+            {
+                const uint64_t u = (uint64_t)(((double)xs[a] - box[a]) / edge_exact);
+                c[a] = (int32_t)(u < (uint64_t)(dims[a] - 1) ? u : (uint64_t)(dims[a] - 1));
+            }
+            else
+                c[a] = internal_coord(xs[a], box[a], box[3 + a] - box[a], dims[a], min_edge);
+        }
+        const int64_t id = c[0] + (int64_t)c[1] * d0 + (int64_t)c[2] * d0 * d1;
+        if (id < 0 || id >= nc)
+            hc_fail("cell_ids: cell id out of range");
+        cell_of[i] = (int32_t)id;
+        counts[id]++;
+    }
+}
+
+void launch_scan(const int32_t *counts, int32_t *start, int32_t ncells, const int32_t *gate, hipStream_t)
+{
+    need(counts, sizeof(int32_t) * ncells, "scan: counts");
+    need(start, sizeof(int32_t) * (ncells + 1), "scan: start");
+    if (gated_off(gate))
+        return;
+    int32_t run = 0;
+    for (int32_t c = 0; c < ncells; ++c)
+    {
+        start[c] = run;
+        run += counts[c];
+    }
+    start[ncells] = run;
+}
+
+void launch_scatter(const int32_t *cell_of, int32_t n, const int32_t *start, int32_t *cursor, int32_t *tmp,
+                    const int32_t *gate, hipStream_t)
+{
+    need(cell_of, sizeof(int32_t) * n, "scatter: cell_of");
+    need(tmp, sizeof(int32_t) * n, "scatter: tmp");
+    if (gated_off(gate))
+        return;
+    for (int32_t i = 0; i < n; ++i)
+    {
+        const int32_t c = cell_of[i];
+        if (c < 0)
+            continue;
+        need(start + c, 8, "scatter: start[c..c+1]");
+        need(cursor + c, 4, "scatter: cursor[c]");
+        const int32_t slot = start[c] + cursor[c]++;
+        if (slot < 0 || slot >= n)
+            hc_fail("scatter: slot out of range");
+        tmp[slot] = i;
+    }
+}
+
+void launch_cell_sort(const int32_t *cell_of, int32_t n, const int32_t *start, const int32_t *tmp, const int32_t *key,
+                      int32_t *out, const int32_t *gate, hipStream_t)
+{
+    need(cell_of, sizeof(int32_t) * n, "cell_sort: cell_of");
+    need(tmp, sizeof(int32_t) * n, "cell_sort: tmp");
+    need(out, sizeof(int32_t) * n, "cell_sort: out");
+    if (key)
+        need(key, sizeof(int32_t) * n, "cell_sort: key");
+    if (gated_off(gate))
+        return;
+    for (int32_t i = 0; i < n; ++i)
+    {
+        const int32_t c = cell_of[i];
+        if (c < 0)
+            continue;
+        need(start + c, 8, "cell_sort: start");
+        const int32_t ki = key ? key[i] : i;
+        int32_t rank = 0;
+        for (int32_t t = start[c]; t < start[c + 1]; ++t)
+            rank += (key ? key[tmp[t]] : tmp[t]) < ki;
+        out[start[c] + rank] = i;
+    }
+}
+
+static bool bonded(const int32_t *sp_ptr, const int32_t *sp_j, int32_t i, int32_t j)
+{
+    for (int32_t e = sp_ptr[i]; e < sp_ptr[i + 1]; ++e)
+        if ((sp_j[e] & 0x7fffffff) == j)
+            return true;
+    return false;
+}
+
+void launch_list_build(const ListArgs &a, hipStream_t)
+{
+    const int64_t nc = (int64_t)a.dims[0] * a.dims[1] * a.dims[2];
+    need(a.pos, sizeof(float4) * a.n, "list: pos");
+    need(a.jprm, sizeof(float4) * a.n, "list: jprm");
+    need(a.cell_start, sizeof(int32_t) * (nc + 1), "list: cell_start");
+    need(a.cell_idx, sizeof(int32_t) * a.n, "list: cell_idx");
+    need(a.box, sizeof(double) * 6, "list: box");
+    need(a.sp_ptr, sizeof(int32_t) * (a.n + 1), "list: sp_ptr");
+    need(a.nbn, sizeof(int32_t) * a.n, "list: nbn");
+    need(a.status, sizeof(int32_t) * 4, "list: status");
+    if (a.cap > 0)
+        need(a.nbl, sizeof(int32_t) * (size_t)a.npad * a.cap, "list: nbl");
+    if (a.npad < a.n || a.npad % 64)
+        hc_fail("list: bad npad");
+    if (gated_off(a.gate))
+        return;
+    need(a.sp_j, sizeof(int32_t) * std::max(a.sp_ptr[a.n], 1), "list: sp_j");
+    for (int32_t i = 0; i < a.n; ++i)
+    {
+        const uint32_t fi = fbits(a.jprm[i].w);
+        int32_t cnt = 0;
+        if (!(fi & PF_STATIC) && (a.i_mask == 0u || (fi & a.i_mask)))
+        {
+            const float xs[3] = {a.pos[i].x, a.pos[i].y, a.pos[i].z};
+            int32_t c[3];
+            for (int q = 0; q < 3; ++q)
+                c[q] = internal_coord(xs[q], a.box[q], a.box[3 + q] - a.box[q], a.dims[q], a.cell_edge);
+            for (int dz = -1; dz <= 1; ++dz)
+                for (int dy = -1; dy <= 1; ++dy)
+                    for (int dx = -1; dx <= 1; ++dx)
+                    {
+                        const int32_t cx = c[0] + dx, cy = c[1] + dy, cz = c[2] + dz;
+                        if (cx < 0 || cy < 0 || cz < 0 || cx >= a.dims[0] || cy >= a.dims[1] || cz >= a.dims[2])
+                            continue;
+                        const int32_t cell = cx + cy * a.dims[0] + cz * a.dims[0] * a.dims[1];
+                        for (int32_t t = a.cell_start[cell]; t < a.cell_start[cell + 1]; ++t)
+                        {
+                            const int32_t j = a.cell_idx[t];
+                            if (j < 0 || j >= a.n)
+                                hc_fail("list: member out of range");
+                            if (j == i)
+                                continue;
+                            const float ddx = a.pos[j].x - xs[0], ddy = a.pos[j].y - xs[1], ddz = a.pos[j].z - xs[2];
+                            if (!(ddx * ddx + ddy * ddy + ddz * ddz < a.r2list))
+                                continue;
+                            if (a.j_mask && !(fbits(a.jprm[j].w) & a.j_mask))
+                                continue;
+                            if (a.exclude_bonded && bonded(a.sp_ptr, a.sp_j, i, j))
+                                continue;
+                            if (cnt < a.cap)
+                                a.nbl[(size_t)cnt * a.npad + i] = j;
+                            ++cnt;
+                        }
+                    }
+        }
+        a.nbn[i] = std::min(cnt, a.cap);
+        if (cnt > a.cap)
+            a.status[2] |= 1;
+        a.status[3] = std::max(a.status[3], cnt);
+    }
+}
+
+void launch_pair(const PairArgs &a, int32_t, int32_t, int32_t nb, hipStream_t)
+{
+    need(a.pos, sizeof(float4) * a.n, "pair: pos");
+    need(a.jprm, sizeof(float4) * a.n, "pair: jprm");
+    need(a.nbn, sizeof(int32_t) * a.n, "pair: nbn");
+    need(a.fnb, sizeof(float4) * a.n, "pair: fnb");
+    need(a.epart, sizeof(double) * 3 * nb, "pair: epart");
+    if ((int64_t)nb * BLOCK < a.n)
+        hc_fail("pair: grid too small");
+    for (int32_t i = 0; i < a.n; ++i)
+    {
+        const int32_t cnt = a.nbn[i];
+        if (cnt > 0)
+            need(a.nbl + (size_t)(cnt - 1) * a.npad + i, 4, "pair: nbl entry");
+        for (int32_t t = 0; t < cnt; ++t)
+        {
+            const int32_t j = a.nbl[(size_t)t * a.npad + i];
+            if (j < 0 || j >= a.n)
+                hc_fail("pair: neighbour index out of range");
+        }
+        if (!a.accumulate)
+            a.fnb[i] = make_float4(0, 0, 0, 0);
+    }
+    std::memset(a.epart, 0, sizeof(double) * 3 * nb);
+}
+
+void launch_spring(const SpringArgs &a, int32_t group, int32_t nb, hipStream_t)
+{
+    const int32_t n = a.n;
+    need(a.pos, sizeof(float4) * n, "spring: pos");
+    need(a.vel, sizeof(float4) * n, "spring: vel");
+    need(a.iprm, sizeof(float4) * n, "spring: iprm");
+    need(a.jprm, sizeof(float4) * n, "spring: jprm");
+    need(a.xref, sizeof(float4) * n, "spring: xref");
+    need(a.sp_ptr, sizeof(int32_t) * (n + 1), "spring: sp_ptr");
+    need(a.perm, sizeof(int32_t) * n, "spring: perm");
+    need(a.status, sizeof(int32_t) * 4, "spring: status");
+    need(a.epart, sizeof(double) * 3 * nb, "spring: epart");
+    if (a.fnb)
+        need(a.fnb, sizeof(float4) * n, "spring: fnb");
+    if (a.fext)
+        need(a.fext, sizeof(float4) * n, "spring: fext");
+    if (a.plo)
+        need(a.plo, sizeof(float4) * n, "spring: plo");
+    if (a.integrate)
+    {
+        need(a.pos_out, sizeof(float4) * n, "spring: pos_out");
+        if (a.plo)
+            need(a.plo_out, sizeof(float4) * n, "spring: plo_out");
+    }
+    else
+        need(a.fout, sizeof(float4) * n, "spring: fout");
+    if ((int64_t)nb * BLOCK < (int64_t)n * group)
+        hc_fail("spring: grid too small");
+    const int32_t ne = a.sp_ptr[n];
+    if (ne > 0)
+    {
+        need(a.sp_j, sizeof(int32_t) * ne, "spring: sp_j");
+        need(a.sp_k, sizeof(float) * ne, "spring: sp_k");
+        need(a.sp_l0, sizeof(float) * ne, "spring: sp_l0");
+    }
+    for (int32_t e = 0; e < ne; ++e)
+        if ((a.sp_j[e] & 0x7fffffff) >= n)
+            hc_fail("spring: edge index out of range");
+    if (a.integrate)
+        for (int32_t i = 0; i < n; ++i)
+            if (!(fbits(a.jprm[i].w) & PF_STATIC))
+                const_cast<float4 *>(a.pos_out)[i] = a.pos[i];
+    std::memset(a.epart, 0, sizeof(double) * 3 * nb);
+}
+
+void launch_reduce_energies(const double *pp, int32_t npb, int32_t nlists, const double *spp, int32_t nsb,
+                            double *energies, hipStream_t)
+{
+    need(pp, sizeof(double) * 3 * npb * std::max(nlists, 0), "reduce: pair partials");
+    if (spp)
+        need(spp, sizeof(double) * 3 * nsb, "reduce: spring partials");
+    need(energies, sizeof(double) * 6, "reduce: energies");
+    for (int k = 0; k < 6; ++k)
+        energies[k] = 0.0;
+}
+
+void launch_prestep(int32_t *status, int32_t *step, int32_t interval, hipStream_t)
+{
+    need(status, 16, "prestep: status");
+    need(step, 4, "prestep: step");
+    if (interval > 0 && *step > 0 && *step % interval == 0)
+        status[0] = 1;
+    *step += 1;
+}
+
+void launch_finish_rebuild(const float4 *pos, float4 *xref, int32_t n, int32_t *status, int32_t *, hipStream_t)
+{
+    need(pos, sizeof(float4) * n, "xref: pos");
+    need(xref, sizeof(float4) * n, "xref: xref");
+    need(status, 16, "xref: status");
+    if (status[0])
+        std::memcpy(xref, pos, sizeof(float4) * n);
+}
+
+void launch_clear_rebuild(int32_t *status, int32_t *counters, hipStream_t)
+{
+    need(status, 16, "clear: status");
+    need(counters, 8, "clear: counters");
+    if (status[0])
+    {
+        status[0] = 0;
+        counters[1] += 1;
+    }
+}
+
+void launch_fill_i32(int32_t *p, int32_t v, int64_t n, const int32_t *gate, hipStream_t)
+{
+    need(p, sizeof(int32_t) * n, "fill");
+    if (gated_off(gate))
+        return;
+    for (int64_t i = 0; i < n; ++i)
+        p[i] = v;
+}
+
+void launch_pairs_extract(const float4 *pos, const float4 *jprm, const int32_t *nbl, const int32_t *nbn,
+                          const int32_t *perm, int32_t n, int32_t npad, float rc, float, float, uint32_t i_mask,
+                          uint32_t j_mask, int32_t, uint32_t *out, int32_t *count, int64_t cap, hipStream_t)
+{
+    need(pos, sizeof(float4) * n, "extract: pos");
+    need(jprm, sizeof(float4) * n, "extract: jprm");
+    need(nbn, sizeof(int32_t) * n, "extract: nbn");
+    need(perm, sizeof(int32_t) * n, "extract: perm");
+    need(count, 4, "extract: count");
+    need(out, sizeof(uint32_t) * 2 * cap, "extract: out");
+    for (int32_t i = 0; i < n; ++i)
+    {
+        if (i_mask && !(fbits(jprm[i].w) & i_mask))
+            continue;
+        for (int32_t t = 0; t < nbn[i]; ++t)
+        {
+            const int32_t j = nbl[(size_t)t * npad + i];
+            const uint32_t fj = fbits(jprm[j].w);
+            if (j_mask && !(fj & j_mask))
+                continue;
+            if (!(fj & PF_STATIC) && perm[j] < perm[i])
+                continue;
+            const float dx = pos[j].x - pos[i].x, dy = pos[j].y - pos[i].y, dz = pos[j].z - pos[i].z;
+            if (!(std::sqrt((double)dx * dx + (double)dy * dy + (double)dz * dz) < rc))
+                continue;
+            const int32_t slot = (*count)++;
+            if (slot < cap)
+            {
+                out[2 * slot] = std::min(perm[i], perm[j]);
+                out[2 * slot + 1] = std::max(perm[i], perm[j]);
+            }
+        }
+    }
+}
+
+} // namespace bs

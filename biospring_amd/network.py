@@ -16,8 +16,8 @@ from ._abi import check, f32, ptr, u32
 class SpringNetwork:
     """One simulation on the current HIP device."""
 
-    def __init__(self, params=None):
-        L = _abi.lib()
+    def __init__(self, params=None, library=None):
+        L = library if library is not None else _abi.lib()
         self._L = L
         self._h = ctypes.c_void_p()
         self.params = params.copy() if params is not None else _abi.default_params()
@@ -48,8 +48,8 @@ class SpringNetwork:
                                      ptr(l0, c.c_float), ptr(k, c.c_float)), self._h, "bs_set_springs")
 
     @classmethod
-    def from_system(cls, system, params=None):
-        net = cls(params if params is not None else system.params)
+    def from_system(cls, system, params=None, library=None):
+        net = cls(params if params is not None else system.params, library=library)
         net.setParticles(*system.particle_arrays())
         net.setSprings(*system.spring_arrays())
         return net

@@ -83,15 +83,25 @@ def test_forces_vs_fp64(name):
         assert err <= ENERGY_RTOL, (slot, E[slot], Ed[slot], err)
 
 
-@pytest.mark.parametrize("name", ["enm2k", "lj3k_ddd"])
+@pytest.mark.parametrize("name", ["enm2k", "lj3k_ddd", "lj20k"])
 def test_forces_after_steps(name):
-    """Move the system with the GPU, then compare forces at the moved state."""
+    """Move the system with the GPU, then compare the conservative forces
+    (springs + pairs; viscosity off) and pair sets at the moved state.
+
+    The total force with the viscous term is F_cons(n) - gamma v(n), which in
+    the overdamped regime (gamma dt / m = 1) is a difference of nearly equal
+    vectors; the viscous product is exact in both codes, so parity is judged
+    on F_cons.
+    """
     s = _system(name)
     net = SpringNetwork.from_system(s)
     net.run(20)
     x, v = net.getState()
+    p = s.params.copy()
+    p.viscosity_enable = 0
+    net.setParams(p)
     F, E = net.computeForces()
-    o = po.Oracle(s)
+    o = po.Oracle(s, params=p)
     o.set_state(x, v)
     Fd, Ed = o.compute_forces(po.F64)
     dyn = (s.flags & 1) == 0
@@ -100,28 +110,42 @@ def test_forces_after_steps(name):
     assert per <= FORCE_RTOL
     for slot in range(6):
         assert energy_error(E[slot], Ed[slot], floor=1e-3) <= ENERGY_RTOL, slot
+    for kind in (_abi.NS_STERIC, _abi.NS_ELECTROSTATIC):
+        assert np.array_equal(net.getPairs(kind), o.pairs(kind, method=0)), kind
 
 
 @pytest.mark.parametrize("name", ["enm2k", "lj3k_ddd"])
 def test_trajectory_vs_fp64(name):
-    """100 steps: GPU (FP32) against the FP64 oracle trajectory."""
+    """100 steps: the GPU trajectory deviates from the FP64 oracle no more than
+    the reference's own float path (float-faithful oracle) does."""
     s = _system(name)
     net = SpringNetwork.from_system(s)
-    o = po.Oracle(s)
+    o64 = po.Oracle(s)
+    o32 = po.Oracle(s)
+    _, E0 = o64.compute_forces(po.F64)
+    done = 0
     for nsteps in (1, 10, 100):
-        done = net.counters()["steps"]
         net.run(nsteps - done)
-        rc, Ed = o.step(po.F64, nsteps - done)
-        assert rc == 0
+        rc, Ed = o64.step(po.F64, nsteps - done)
+        rc2, Ef = o32.step(po.F32, nsteps - done)
+        assert rc == 0 and rc2 == 0
+        done = nsteps
         x, _ = net.getState()
-        xd, _ = o.state(po.F64)
+        xd, _ = o64.state(po.F64)
+        xf, _ = o32.state(po.F32)
         dev = np.linalg.norm(x - xd, axis=1)
+        dref = np.linalg.norm(xf - xd, axis=1)
         E = net.getEnergies()
-        _log(f"  step {nsteps}: rms dev {np.sqrt((dev ** 2).mean()):.3e} max {dev.max():.3e}")
-        _log("   E gpu", E, "\n   E f64", Ed)
-        assert dev.max() < 1e-3 * max(1, nsteps / 10)
+        rms, rms_ref = np.sqrt((dev ** 2).mean()), np.sqrt((dref ** 2).mean())
+        _log(f"  step {nsteps}: gpu rms dev {rms:.3e} max {dev.max():.3e}; "
+             f"reference float path rms {rms_ref:.3e} max {dref.max():.3e}")
+        _log("   E gpu", E, "\n   E f64", Ed, "\n   E f32", Ef)
+        assert rms <= 3 * rms_ref + 1e-6
         for slot in range(6):
-            assert energy_error(E[slot], Ed[slot], floor=1e-2) <= 1e-4, (slot, E[slot], Ed[slot])
+            floor = 1e-3 * max(abs(E0[slot]), 1.0)
+            eg = energy_error(E[slot], Ed[slot], floor)
+            er = energy_error(Ef[slot], Ed[slot], floor)
+            assert eg <= max(1e-5, 3 * er), (slot, E[slot], Ed[slot], Ef[slot])
 
 
 def test_determinism():

@@ -1,0 +1,108 @@
+"""CPU check of the engine's host logic (buffer management, call sequencing).
+
+bs_api.cpp is compiled against a recording fake of the HIP runtime whose
+launchers verify that every buffer a kernel would touch is live and large
+enough (biospring_amd/csrc/hostcheck).  A violation aborts the child process,
+so each sequence runs in a subprocess and the test checks its exit status.
+"""
+import os
+import subprocess
+import sys
+import textwrap
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CSRC = os.path.join(ROOT, "biospring_amd", "csrc")
+HOSTCHECK = os.path.join(CSRC, "_build", "libbs_hostcheck.so")
+
+
+@pytest.fixture(scope="module")
+def hostcheck_lib():
+    subprocess.run(["make", "-C", CSRC, "hostcheck"], check=True, stdout=subprocess.DEVNULL)
+    assert os.path.exists(HOSTCHECK)
+    return HOSTCHECK
+
+
+def _run(body, lib):
+    code = textwrap.dedent(f"""
+        import ctypes, sys
+        sys.path.insert(0, {ROOT!r})
+        import numpy as np
+        from biospring_amd import _abi, workloads
+        from biospring_amd.network import SpringNetwork
+        L = _abi.bind(ctypes.CDLL({lib!r}))
+        L.hc_live_allocations.restype = ctypes.c_int64
+        """) + textwrap.dedent(body)
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
+    return r.stdout
+
+
+def test_setup_step_reparam_sequence(hostcheck_lib):
+    out = _run("""
+        s = workloads.generate(1, seed=3, n=3000)
+        base = L.hc_live_allocations()
+        net = SpringNetwork.from_system(s, library=L)
+        net.setup()
+        net.run(25)
+        p = s.params.copy(); p.viscosity_enable = 0
+        net.setParams(p)                      # frees device state, lazy re-setup
+        F, E = net.computeForces()
+        for kind in (0, 1, 2):
+            net.getCells(kind)
+        net.getPairs(0); net.getPairs(1); net.countPairs(0)
+        x, v = net.getState()
+        net.setState(x + 0.01, v)
+        net.run(3)
+        p2 = p.copy(); p2.electrostatic_cutoff = 9.0; p2.steric_cutoff = 6.0
+        net.setParams(p2)                     # different list plan
+        net.computeForces(); net.run(4); net.getPairs(1)
+        p3 = p2.copy(); p3.compensated_positions = 1; p3.hydrophobicity_enable = 1
+        net.setParams(p3)
+        net.run(3); net.computeForces()
+        net.profile(True); net.run(5); net.getProfile()
+        net.close()
+        print("leak", L.hc_live_allocations() - base)
+    """, hostcheck_lib)
+    assert "leak 0" in out, out
+
+
+def test_small_and_degenerate_systems(hostcheck_lib):
+    _run("""
+        # one particle, no springs; all static; no terms enabled
+        p = _abi.default_params(); p.steric_enable = 1; p.electrostatic_enable = 1
+        for n in (1, 2, 5):
+            net = SpringNetwork(p, library=L)
+            net.setParticles(np.arange(3 * n, dtype=np.float32), charge=np.ones(n, np.float32))
+            net.setup(); net.run(2); net.computeForces(); net.getPairs(0); net.getCells(1)
+            net.close()
+        net = SpringNetwork(p, library=L)
+        net.setParticles(np.zeros(30, np.float32), flags=np.ones(10, np.uint8))
+        net.run(2); net.computeForces(); net.close()
+        q = _abi.default_params()
+        net = SpringNetwork(q, library=L)
+        s = workloads.generate(0, seed=2, n=300)
+        net.setParticles(*s.particle_arrays()); net.setSprings(*s.spring_arrays())
+        net.run(3); net.computeForces(); net.getCells(0); net.close()
+    """, hostcheck_lib)
+
+
+def test_argument_errors(hostcheck_lib):
+    _run("""
+        net = SpringNetwork(library=L)
+        try:
+            net.setSprings([0], [1], [1.0], [1.0])
+            raise SystemExit("springs before particles accepted")
+        except _abi.EngineError as e:
+            assert e.code == _abi.BS_ERR_STATE
+        net.setParticles(np.zeros(6, np.float32))
+        for a, b in ((0, 2), (1, 1)):
+            try:
+                net.setSprings([a], [b], [1.0], [1.0])
+                raise SystemExit("bad spring accepted")
+            except _abi.EngineError as e:
+                assert e.code == _abi.BS_ERR_ARG, e
+        net.setSprings([0, 1, 0], [1, 0, 1], [1.0, 2.0, 3.0], [1.0, 1.0, 1.0])
+        assert L.bs_num_springs(net._h) == 1   # duplicates ignored like addSpring
+    """, hostcheck_lib)
