@@ -30,9 +30,9 @@ struct List
     uint32_t imask = 0, jmask = 0;
     int32_t dims[3] = {1, 1, 1};
     int32_t ncells = 1;
-    int32_t cap = 0;
-    int32_t *nbl = nullptr;
-    int32_t *nbn = nullptr;
+    int32_t cap = 0;          // entries per i-cluster
+    uint32_t *cl = nullptr;   // [ncl * cap] j | mask << 24
+    int32_t *cln = nullptr;   // [ncl]
 };
 
 template <typename T> T *dalloc(size_t count, hipError_t &err)
@@ -71,6 +71,9 @@ struct bs_handle
     int32_t *sp_ptr = nullptr, *sp_j = nullptr, *perm_d = nullptr;
     float *sp_k = nullptr, *sp_l0 = nullptr;
     int64_t nedges = 0;
+    int32_t ncl = 0;                                  // i-clusters of 8
+    int32_t *ex_ptr = nullptr;                        // [ncl + 1] static exclusions
+    uint32_t *ex_w = nullptr;                         // j << 8 | bonded mask
     int32_t *cell_of = nullptr, *counts = nullptr, *cstart = nullptr, *cursor = nullptr, *ctmp = nullptr,
             *cidx = nullptr;
     int32_t cell_cap = 0;
@@ -146,10 +149,13 @@ void free_device(bs_handle *h)
     f(h->sp_ptr); f(h->sp_j); f(h->perm_d); f(h->sp_k); f(h->sp_l0);
     f(h->cell_of); f(h->counts); f(h->cstart); f(h->cursor); f(h->ctmp); f(h->cidx);
     f(h->box_part); f(h->box); f(h->pair_part); f(h->spring_part); f(h->energies_d); f(h->status); f(h->counters);
+    f(h->ex_ptr); f(h->ex_w);
+    h->ex_ptr = nullptr;
+    h->ex_w = nullptr;
     for (auto &l : h->lists)
     {
-        f(l.nbl);
-        f(l.nbn);
+        f(l.cl);
+        f(l.cln);
     }
     h->lists.clear();
     h->vel = h->fnb = h->jprm = h->iprm = h->xref = h->fext = h->fout = nullptr;
@@ -170,14 +176,16 @@ int check_device(bs_handle *h)
     const void *need[] = {h->pos[0], h->pos[1], h->vel, h->fnb, h->jprm, h->iprm, h->xref, h->fext, h->fout,
                           h->sp_ptr, h->sp_j, h->sp_k, h->sp_l0, h->perm_d, h->cell_of, h->ctmp, h->cidx,
                           h->counts, h->cstart, h->cursor, h->box, h->box_part, h->pair_part, h->spring_part,
-                          h->energies_d, h->status, h->counters};
+                          h->energies_d, h->status, h->counters, h->ex_ptr, h->ex_w};
     for (const void *q : need)
         if (!q)
             return fail(h, BS_ERR_STATE, "internal: device buffer missing");
     if (h->p.compensated_positions && (!h->plo[0] || !h->plo[1]))
         return fail(h, BS_ERR_STATE, "internal: compensated-position buffers missing");
+    if (h->ncl != (int32_t)((h->n + CLUSTER - 1) / CLUSTER))
+        return fail(h, BS_ERR_STATE, "internal: cluster count mismatch");
     for (const auto &l : h->lists)
-        if (!l.nbl || !l.nbn || l.cap <= 0 || l.ncells > h->cell_cap)
+        if (!l.cl || !l.cln || l.cap <= 0 || l.ncells > h->cell_cap)
             return fail(h, BS_ERR_STATE, "internal: neighbour list buffers missing");
     return BS_OK;
 }
@@ -336,10 +344,12 @@ void plan_grid(bs_handle *h, List &l)
             lo[c] = std::min(lo[c], (double)h->x[3 * i + c]);
             hi[c] = std::max(hi[c], (double)h->x[3 * i + c]);
         }
+    // Binning grid of edge >= radius / 2: a cluster's search covers its
+    // bounding box grown by the radius, about 5-6 cells per axis.
     int64_t total = 1;
     for (int c = 0; c < 3; ++c)
     {
-        int64_t d = (int64_t)std::floor((hi[c] - lo[c]) / l.radius);
+        int64_t d = (int64_t)std::floor((hi[c] - lo[c]) / (0.5 * l.radius));
         d = std::max<int64_t>(1, std::min<int64_t>(d, 1024));
         l.dims[c] = (int32_t)d;
         total *= d;
@@ -385,8 +395,8 @@ void build_list(bs_handle *h, List &l, bool gated, int32_t cap_override)
     const int32_t *g = gate_of(h, gated);
     hipStream_t s = h->stream;
     launch_fill_i32(h->counts, 0, l.ncells, g, s);
-    launch_cell_ids(h->pos[h->cur], h->jprm, l.jmask, n, h->box, l.dims[0], l.dims[1], l.dims[2], 0.0, l.radius, 0,
-                    h->cell_of, h->counts, g, s);
+    launch_cell_ids(h->pos[h->cur], h->jprm, l.jmask, n, h->box, l.dims[0], l.dims[1], l.dims[2], 0.0,
+                    0.5f * l.radius, 0, h->cell_of, h->counts, g, s);
     launch_scan(h->counts, h->cstart, l.ncells, g, s);
     launch_fill_i32(h->cursor, 0, l.ncells, g, s);
     launch_scatter(h->cell_of, n, h->cstart, h->cursor, h->ctmp, g, s);
@@ -397,17 +407,18 @@ void build_list(bs_handle *h, List &l, bool gated, int32_t cap_override)
     a.cell_start = h->cstart;
     a.cell_idx = h->cidx;
     a.box = h->box;
-    a.sp_ptr = h->sp_ptr;
-    a.sp_j = h->sp_j;
-    a.nbl = l.nbl;
-    a.nbn = l.nbn;
+    a.ex_ptr = h->ex_ptr;
+    a.ex_w = h->ex_w;
+    a.cl = l.cl;
+    a.cln = l.cln;
     a.status = h->status;
     a.n = n;
-    a.npad = (int32_t)((h->n + 63) / 64 * 64);
+    a.ncl = h->ncl;
     a.cap = cap_override >= 0 ? cap_override : l.cap;
     for (int c = 0; c < 3; ++c)
         a.dims[c] = l.dims[c];
-    a.cell_edge = l.radius;
+    a.cell_edge = 0.5f * l.radius;
+    a.radius = l.radius;
     a.r2list = (float)((double)l.radius * l.radius * (1.0 + 1e-5));
     a.i_mask = l.imask;
     a.j_mask = l.jmask;
@@ -436,19 +447,19 @@ float rc2(float rc, double f) { return (float)((double)rc * rc * f); }
 void launch_pairs(bs_handle *h)
 {
     const int32_t n = (int32_t)h->n;
-    const int32_t npad = (int32_t)((h->n + 63) / 64 * 64);
     for (size_t li = 0; li < h->lists.size(); ++li)
     {
         const List &l = h->lists[li];
         PairArgs a{};
         a.pos = h->pos[h->cur];
         a.jprm = h->jprm;
-        a.nbl = l.nbl;
-        a.nbn = l.nbn;
+        a.cl = l.cl;
+        a.cln = l.cln;
         a.fnb = h->fnb;
         a.epart = h->pair_part + (size_t)li * h->pair_blocks * 3;
         a.n = n;
-        a.npad = npad;
+        a.ncl = h->ncl;
+        a.cap = l.cap;
         a.accumulate = li > 0 ? 1 : 0;
         a.rc = l.rc;
         a.rc2lo = rc2(l.rc, 1.0 - 4e-6);
@@ -586,8 +597,8 @@ int do_setup(bs_handle *h)
     free_device(h);
     h->cur = 0;
     const int64_t n = h->n;
-    if (n > INT32_MAX / 64)
-        return fail(h, BS_ERR_ARG, "bs_setup: too many particles");
+    if (n > (int64_t)CL_JMASK)
+        return fail(h, BS_ERR_ARG, "bs_setup: more than %u particles", CL_JMASK);
     make_order(h);
     hipError_t e;
 #define ALLOC(ptr, T, cnt)                                                                                        \
@@ -696,10 +707,51 @@ int do_setup(bs_handle *h)
     HIPCHK(h, hipMemcpyAsync(h->status, st0, sizeof st0, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(h->counters, ct0, sizeof ct0, hipMemcpyHostToDevice, h->stream));
 
-    // kernel geometry
-    h->pair_blocks = (int32_t)((n + BLOCK - 1) / BLOCK);
+    // kernel geometry: pair kernels run one wave per i-cluster
+    h->ncl = (int32_t)((n + CLUSTER - 1) / CLUSTER);
+    h->pair_blocks = (h->ncl + BLOCK / 64 - 1) / (BLOCK / 64);
     h->spring_blocks = (int32_t)((n * h->group + BLOCK - 1) / BLOCK);
     ALLOC(h->spring_part, double, 3 * h->spring_blocks);
+
+    // static exclusions per i-cluster: the union of its 8 spring rows as
+    // sorted (j, bonded-mask) words; springs never change during a run
+    {
+        std::vector<std::vector<uint32_t>> ex((size_t)h->ncl);
+#pragma omp parallel for schedule(dynamic, 64)
+        for (int32_t c = 0; c < h->ncl; ++c)
+        {
+            std::vector<uint32_t> &w = ex[(size_t)c];
+            for (int32_t k = 0; k < CLUSTER; ++k)
+            {
+                const int64_t i = (int64_t)c * CLUSTER + k;
+                if (i >= n)
+                    break;
+                for (int32_t t = ptr[(size_t)i]; t < ptr[(size_t)i + 1]; ++t)
+                    w.push_back((uint32_t)(ej[(size_t)t] & 0x7fffffff) << 8 | (1u << k));
+            }
+            std::sort(w.begin(), w.end());
+            size_t o = 0;
+            for (size_t t = 0; t < w.size(); ++t)
+            {
+                if (o > 0 && (w[o - 1] >> 8) == (w[t] >> 8))
+                    w[o - 1] |= w[t] & 0xFFu;
+                else
+                    w[o++] = w[t];
+            }
+            w.resize(o);
+        }
+        std::vector<int32_t> xp((size_t)h->ncl + 1, 0);
+        for (int32_t c = 0; c < h->ncl; ++c)
+            xp[(size_t)c + 1] = xp[(size_t)c] + (int32_t)ex[(size_t)c].size();
+        std::vector<uint32_t> xw((size_t)xp[(size_t)h->ncl]);
+        for (int32_t c = 0; c < h->ncl; ++c)
+            std::copy(ex[(size_t)c].begin(), ex[(size_t)c].end(), xw.begin() + xp[(size_t)c]);
+        ALLOC(h->ex_ptr, int32_t, h->ncl + 1);
+        ALLOC(h->ex_w, uint32_t, xw.size());
+        HIPCHK(h, hipMemcpy(h->ex_ptr, xp.data(), sizeof(int32_t) * xp.size(), hipMemcpyHostToDevice));
+        if (!xw.empty())
+            HIPCHK(h, hipMemcpy(h->ex_w, xw.data(), sizeof(uint32_t) * xw.size(), hipMemcpyHostToDevice));
+    }
 
     // neighbour lists: a counting pass sizes the capacity
     plan_lists(h);
@@ -715,18 +767,17 @@ int do_setup(bs_handle *h)
         return rc;
     if (!h->lists.empty())
         launch_box_reduce(h->pos[h->cur], (int32_t)n, h->box_part, h->box, h->box_blocks, nullptr, h->stream);
-    const int64_t npad = (n + 63) / 64 * 64;
     for (auto &l : h->lists)
     {
-        ALLOC(l.nbn, int32_t, n);
+        ALLOC(l.cln, int32_t, h->ncl);
         launch_fill_i32(h->status + 3, 0, 1, nullptr, h->stream);
         build_list(h, l, false, 0);
         int32_t st[4];
         HIPCHK(h, hipMemcpyAsync(st, h->status, sizeof st, hipMemcpyDeviceToHost, h->stream));
         HIPCHK(h, hipStreamSynchronize(h->stream));
         const int32_t mx = st[3];
-        l.cap = std::max<int32_t>(16, (int32_t)std::ceil(mx * 1.25) + 32);
-        ALLOC(l.nbl, int32_t, npad * (int64_t)l.cap);
+        l.cap = (std::max<int32_t>(16, (int32_t)std::ceil(mx * 1.25) + 64) + 7) / 8 * 8;
+        ALLOC(l.cl, uint32_t, (int64_t)h->ncl * l.cap);
         launch_fill_i32(h->status + 2, 0, 1, nullptr, h->stream);
     }
     const int32_t st1[4] = {1, INT_MAX, 0, 0};
@@ -1214,45 +1265,55 @@ int bs_get_pairs(bs_handle *h, int32_t kind, uint32_t *ij, int64_t cap, int64_t 
         return BS_OK;
     rebuild_chain(h, true);
     const int32_t n = (int32_t)h->n;
-    const int64_t npad = (h->n + 63) / 64 * 64;
-    const int64_t maxp = (int64_t)n * lst->cap;
-    hipError_t e;
-    uint32_t *out = dalloc<uint32_t>((size_t)(2 * maxp), e);
-    HIPCHK(h, e);
-    int32_t *cnt = dalloc<int32_t>(1, e);
-    if (e != hipSuccess)
-    {
-        (void)hipFree(out);
-        return fail(h, BS_ERR_HIP, "hipMalloc failed");
-    }
     const uint32_t mask = kind == BS_NS_ELECTROSTATIC ? PF_CHARGED : kind == BS_NS_HYDROPHOBIC ? PF_HYDRO : 0u;
     const float rc_k = kind == BS_NS_ELECTROSTATIC ? h->p.electrostatic_cutoff
                        : kind == BS_NS_HYDROPHOBIC ? h->p.hydrophobicity_cutoff
                                                    : h->p.steric_cutoff;
-    launch_fill_i32(cnt, 0, 1, nullptr, h->stream);
-    launch_pairs_extract(h->pos[h->cur], h->jprm, lst->nbl, lst->nbn, h->perm_d, n, (int32_t)npad, rc_k,
-                         rc2(rc_k, 1.0 - 4e-6), rc2(rc_k, 1.0 + 4e-6), mask, mask, kind != BS_NS_STERIC ? 1 : 0, out,
-                         cnt, maxp, h->stream);
+    hipError_t e;
+    int32_t *cnt = dalloc<int32_t>(1, e);
+    HIPCHK(h, e);
+    // pass 1 counts (writes nothing), pass 2 fills an exactly sized buffer
     int32_t c = 0;
-    hipError_t e1 = hipMemcpyAsync(&c, cnt, sizeof c, hipMemcpyDeviceToHost, h->stream);
-    hipError_t e2 = hipStreamSynchronize(h->stream);
+    uint32_t *out = nullptr;
+    hipError_t e1 = hipSuccess, e2 = hipSuccess;
+    for (int pass = 0; pass < 2 && e1 == hipSuccess && e2 == hipSuccess; ++pass)
+    {
+        if (pass == 1)
+        {
+            if (!ij || c == 0)
+                break;
+            out = dalloc<uint32_t>(2 * (size_t)c, e1);
+            if (e1 != hipSuccess)
+                break;
+        }
+        launch_fill_i32(cnt, 0, 1, nullptr, h->stream);
+        launch_pairs_extract(h->pos[h->cur], h->jprm, lst->cl, lst->cln, h->perm_d, n, h->ncl, lst->cap, rc_k,
+                             rc2(rc_k, 1.0 - 4e-6), rc2(rc_k, 1.0 + 4e-6), mask, mask, kind != BS_NS_STERIC ? 1 : 0,
+                             pass == 1 ? out : nullptr, cnt, pass == 1 ? c : 0, h->stream);
+        int32_t got = 0;
+        e1 = hipMemcpyAsync(&got, cnt, sizeof got, hipMemcpyDeviceToHost, h->stream);
+        e2 = hipStreamSynchronize(h->stream);
+        if (pass == 1 && got != c)
+            e1 = hipErrorInvalidValue;
+        c = got;
+    }
     std::vector<uint32_t> v;
     if (e1 == hipSuccess && e2 == hipSuccess && !ij)
     { // count only
-        (void)hipFree(out);
         (void)hipFree(cnt);
         *count = c;
         return BS_OK;
     }
-    if (e1 == hipSuccess && e2 == hipSuccess)
+    if (e1 == hipSuccess && e2 == hipSuccess && out)
     {
-        v.resize(2 * (size_t)std::min<int64_t>(c, maxp));
+        v.resize(2 * (size_t)c);
         e1 = hipMemcpy(v.data(), out, sizeof(uint32_t) * v.size(), hipMemcpyDeviceToHost);
     }
-    (void)hipFree(out);
+    if (out)
+        (void)hipFree(out);
     (void)hipFree(cnt);
     if (e1 != hipSuccess || e2 != hipSuccess)
-        return fail(h, BS_ERR_HIP, "bs_get_pairs: copy failed");
+        return fail(h, BS_ERR_HIP, "bs_get_pairs: extraction failed");
     const int64_t np = (int64_t)v.size() / 2;
     std::vector<uint64_t> keys((size_t)np);
     for (int64_t t = 0; t < np; ++t)

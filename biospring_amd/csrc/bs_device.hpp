@@ -1,0 +1,82 @@
+// bs_device.hpp -- device helpers shared by the engine's kernels.
+#pragma once
+
+#include "bs_engine.hpp"
+
+namespace bs
+{
+
+__device__ inline uint32_t fbits(float f) { return __float_as_uint(f); }
+
+__device__ inline bool gated_off(const int32_t *gate) { return gate != nullptr && *gate == 0; }
+
+__device__ inline double wave_sum(double v)
+{
+    for (int off = 32; off > 0; off >>= 1)
+        v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Block-wide sum, valid in thread 0.  Fixed order: deterministic.
+__device__ inline double block_sum(double v, double *sh)
+{
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0)
+        sh[wave] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w)
+            r += sh[w];
+    __syncthreads();
+    return r;
+}
+
+// The reference's two-stage cutoff test, exactly (nsearch.hpp:230 then
+// Particle.cpp:400-402 through Vector3f::norm, measure.hpp:163-196): an FP64
+// distance of exact float differences, then an FP64 norm of float-rounded
+// differences rounded to float; both must be below the float cutoff.
+// Products and sums are kept unfused, as in the reference's x86-64 build.
+__device__ __noinline__ bool exact_two_stage(float4 pi, float4 pj, float rc)
+{
+#pragma clang fp contract(off)
+    const double ax = (double)pi.x - (double)pj.x;
+    const double ay = (double)pi.y - (double)pj.y;
+    const double az = (double)pi.z - (double)pj.z;
+    const double s1 = ax * ax + ay * ay + az * az;
+    if (!(sqrt(s1) < (double)rc))
+        return false;
+    const double bx = (double)(pj.x - pi.x);
+    const double by = (double)(pj.y - pi.y);
+    const double bz = (double)(pj.z - pi.z);
+    float d = 0.0f;
+    if (!(fabs(bx) < 1e-40 && fabs(by) < 1e-40 && fabs(bz) < 1e-40))
+        d = (float)sqrt(bx * bx + by * by + bz * bz);
+    return d < rc;
+}
+
+// Fast path of the test: float r^2 outside a relative guard band of 4e-6
+// decides; inside it, the exact test runs.  The band is wider than the
+// float error of r^2 (about 3e-7 relative), so the decision is exact.
+__device__ inline bool in_cutoff(float4 pi, float4 pj, float rc, float rc2lo, float rc2hi, float r2)
+{
+    if (r2 < rc2lo)
+        return true;
+    if (r2 >= rc2hi)
+        return false;
+    return exact_two_stage(pi, pj, rc);
+}
+
+// Cell coordinate on the engine's binning grid: edge >= min_edge per axis and
+// the whole box covered by `dim` cells.
+__device__ inline int32_t internal_coord(float x, double mn, double len, int32_t dim, float min_edge)
+{
+    const double e = fmax((double)min_edge, len / (double)dim);
+    double q = ((double)x - mn) / e;
+    q = q > 0.0 ? q : 0.0; // also maps NaN to 0
+    int32_t c = q < (double)dim ? (int32_t)q : dim - 1;
+    return c < dim - 1 ? c : dim - 1;
+}
+
+} // namespace bs

@@ -64,16 +64,23 @@ inline Consts make_consts()
     return c;
 }
 
+// Neighbour lists are per i-cluster (8 consecutive internal particles): entry
+// t of cluster c is cl[c * cap + t] = j | (m << 24), where bit k of the 8-bit
+// m marks the pair (8c + k, j) as within the list radius at the last build,
+// valid and not spring-bonded.
+constexpr int32_t CLUSTER = 8;
+constexpr uint32_t CL_JMASK = 0x00FFFFFFu;
+
 // Parameters of one pair kernel launch.
 struct PairArgs
 {
     const float4 *pos;
     const float4 *jprm;
-    const int32_t *nbl;
-    const int32_t *nbn;
+    const uint32_t *cl;  // [ncl * cap]
+    const int32_t *cln;  // [ncl]
     float4 *fnb;
     double *epart;      // [nblocks * 3] steric, elec, hydro
-    int32_t n, npad;
+    int32_t n, ncl, cap;
     int32_t accumulate; // add to fnb instead of overwriting
     float rc, rc2lo, rc2hi;
     int32_t steric_mode;
@@ -124,14 +131,15 @@ struct ListArgs
     const int32_t *cell_start; // [ncells + 1]
     const int32_t *cell_idx;   // sorted members
     const double *box;         // [6] current box (device)
-    const int32_t *sp_ptr;
-    const int32_t *sp_j;
-    int32_t *nbl;
-    int32_t *nbn;
+    const int32_t *ex_ptr;     // [ncl + 1] static exclusions per i-cluster
+    const uint32_t *ex_w;      // j << 8 | bonded mask, sorted by j
+    uint32_t *cl;              // [ncl * cap]
+    int32_t *cln;              // [ncl]
     int32_t *status;           // [2] overflow, [3] max count
-    int32_t n, npad, cap;
+    int32_t n, ncl, cap;
     int32_t dims[3];
-    float cell_edge;           // minimum edge (list radius)
+    float cell_edge;           // minimum cell edge of the binning grid
+    float radius;              // list radius
     float r2list;
     uint32_t i_mask, j_mask;
     int32_t exclude_bonded;
@@ -159,10 +167,10 @@ void launch_finish_rebuild(const float4 *pos, float4 *xref, int32_t n, int32_t *
                            hipStream_t s);
 void launch_clear_rebuild(int32_t *status, int32_t *counters, hipStream_t s);
 void launch_fill_i32(int32_t *p, int32_t v, int64_t n, const int32_t *gate, hipStream_t s);
-void launch_pairs_extract(const float4 *pos, const float4 *jprm, const int32_t *nbl, const int32_t *nbn,
-                          const int32_t *perm, int32_t n, int32_t npad, float rc, float rc2lo, float rc2hi,
-                          uint32_t i_mask, uint32_t j_mask, int32_t nonzero, uint32_t *out, int32_t *count,
-                          int64_t cap, hipStream_t s);
+void launch_pairs_extract(const float4 *pos, const float4 *jprm, const uint32_t *cl, const int32_t *cln,
+                          const int32_t *perm, int32_t n, int32_t ncl, int32_t lcap, float rc, float rc2lo,
+                          float rc2hi, uint32_t i_mask, uint32_t j_mask, int32_t nonzero, uint32_t *out,
+                          int32_t *count, int64_t cap, hipStream_t s);
 
 constexpr int32_t BLOCK = 256;
 

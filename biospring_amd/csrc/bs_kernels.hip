@@ -1,10 +1,10 @@
-// bs_kernels.hip -- CDNA4 (gfx950) kernels of the force-and-integrate step.
+// bs_kernels.hip -- CDNA4 (gfx950) kernels of the step: cell binning, the
+// spring gather fused with the integrator, energy reductions, step control.
 //
 // Every kernel is wave64-shaped; none uses floating-point atomics, so the
 // forces and energies of a step are bitwise reproducible run to run.  Integer
-// atomics are used only for histogram counts, flags and the parity-hook pair
-// extraction (whose output is sorted on the host).
-#include "bs_engine.hpp"
+// atomics are used only for histogram counts and flags.
+#include "bs_device.hpp"
 
 #include <cmath>
 
@@ -13,65 +13,6 @@ namespace bs
 
 namespace
 {
-
-__device__ inline uint32_t fbits(float f) { return __float_as_uint(f); }
-
-__device__ inline bool gated_off(const int32_t *gate) { return gate != nullptr && *gate == 0; }
-
-__device__ inline double wave_sum(double v)
-{
-    for (int off = 32; off > 0; off >>= 1)
-        v += __shfl_xor(v, off, 64);
-    return v;
-}
-
-// Block-wide sum, valid in thread 0.  Fixed order: deterministic.
-__device__ inline double block_sum(double v, double *sh)
-{
-    v = wave_sum(v);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0)
-        sh[wave] = v;
-    __syncthreads();
-    double r = 0.0;
-    if (threadIdx.x == 0)
-        for (int w = 0; w < (int)(blockDim.x >> 6); ++w)
-            r += sh[w];
-    __syncthreads();
-    return r;
-}
-
-// The reference's two-stage cutoff test, exactly (nsearch.hpp:230 then
-// Particle.cpp:400-402 through Vector3f::norm, measure.hpp:163-196): an FP64
-// distance of exact float differences, then an FP64 norm of float-rounded
-// differences rounded to float; both must be below the float cutoff.
-// Products and sums are kept unfused, as in the reference's x86-64 build.
-__device__ __noinline__ bool exact_two_stage(float4 pi, float4 pj, float rc)
-{
-#pragma clang fp contract(off)
-    const double ax = (double)pi.x - (double)pj.x;
-    const double ay = (double)pi.y - (double)pj.y;
-    const double az = (double)pi.z - (double)pj.z;
-    const double s1 = ax * ax + ay * ay + az * az;
-    if (!(sqrt(s1) < (double)rc))
-        return false;
-    const double bx = (double)(pj.x - pi.x);
-    const double by = (double)(pj.y - pi.y);
-    const double bz = (double)(pj.z - pi.z);
-    float d = 0.0f;
-    if (!(fabs(bx) < 1e-40 && fabs(by) < 1e-40 && fabs(bz) < 1e-40))
-        d = (float)sqrt(bx * bx + by * by + bz * bz);
-    return d < rc;
-}
-
-__device__ inline bool in_cutoff(float4 pi, float4 pj, float rc, float rc2lo, float rc2hi, float r2)
-{
-    if (r2 < rc2lo)
-        return true;
-    if (r2 >= rc2hi)
-        return false;
-    return exact_two_stage(pi, pj, rc);
-}
 
 // ---------------------------------------------------------------------------
 // Bounding box, FP64 of float coordinates (box.hpp:37-69); min/max are exact.
@@ -138,17 +79,9 @@ __global__ void k_box_final(const double *partial, int32_t nb, double *box, cons
 
 // ---------------------------------------------------------------------------
 // Cell binning: ids, counting sort, deterministic in-cell order.
-// ---------------------------------------------------------------------------
-__device__ inline int32_t internal_coord(float x, double mn, double len, int32_t dim, float min_edge)
-{
-    const double e = fmax((double)min_edge, len / (double)dim);
-    int32_t c = (int32_t)((x - mn) / e);
-    c = c < 0 ? 0 : c;
-    return c < dim - 1 ? c : dim - 1;
-}
-
 // exact_mode 1: nsearch.hpp:277-293 (size_t((x - min)/edge), clamped), FP64.
 // exact_mode 0: the engine's own grid, edge >= min_edge on every axis.
+// ---------------------------------------------------------------------------
 __global__ void k_cell_ids(const float4 *pos, const float4 *jprm, uint32_t mask, int32_t n, const double *box,
                            int32_t d0, int32_t d1, int32_t d2, double edge_exact, float min_edge, int32_t exact_mode,
                            int32_t *cell_of, int32_t *counts, const int32_t *gate)
@@ -173,7 +106,7 @@ __global__ void k_cell_ids(const float4 *pos, const float4 *jprm, uint32_t mask,
         if (exact_mode)
         {
             const double q = ((double)xs[a] - box[a]) / edge_exact;
-            const uint64_t u = (uint64_t)q;
+            const uint64_t u = q > 0.0 ? (uint64_t)q : 0u;
             c[a] = (int32_t)(u < (uint64_t)(dims[a] - 1) ? u : (uint64_t)(dims[a] - 1));
         }
         else
@@ -276,231 +209,12 @@ __global__ void k_cell_sort(const int32_t *cell_of, int32_t n, const int32_t *st
 }
 
 // ---------------------------------------------------------------------------
-// Verlet list build: full list (both directions), spring-bonded pairs
-// excluded, radius = largest cutoff + list skin.
-// ---------------------------------------------------------------------------
-__device__ inline bool bonded(const int32_t *sp_ptr, const int32_t *sp_j, int32_t i, int32_t j)
-{
-    int32_t lo = sp_ptr[i], hi = sp_ptr[i + 1];
-    while (lo < hi)
-    {
-        const int32_t mid = (lo + hi) >> 1;
-        if ((sp_j[mid] & 0x7fffffff) < j)
-            lo = mid + 1;
-        else
-            hi = mid;
-    }
-    return lo < sp_ptr[i + 1] && (sp_j[lo] & 0x7fffffff) == j;
-}
-
-__global__ void __launch_bounds__(BLOCK) k_list_build(ListArgs a)
-{
-    if (gated_off(a.gate))
-        return;
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.n)
-        return;
-    const uint32_t fi = fbits(a.jprm[i].w);
-    int32_t cnt = 0;
-    if (!(fi & PF_STATIC) && (a.i_mask == 0u || (fi & a.i_mask)))
-    {
-        const float4 pi = a.pos[i];
-        const float xs[3] = {pi.x, pi.y, pi.z};
-        int32_t c[3];
-        for (int q = 0; q < 3; ++q)
-            c[q] = internal_coord(xs[q], a.box[q], a.box[3 + q] - a.box[q], a.dims[q], a.cell_edge);
-        for (int dz = -1; dz <= 1; ++dz)
-        {
-            const int32_t cz = c[2] + dz;
-            if (cz < 0 || cz >= a.dims[2])
-                continue;
-            for (int dy = -1; dy <= 1; ++dy)
-            {
-                const int32_t cy = c[1] + dy;
-                if (cy < 0 || cy >= a.dims[1])
-                    continue;
-                for (int dx = -1; dx <= 1; ++dx)
-                {
-                    const int32_t cx = c[0] + dx;
-                    if (cx < 0 || cx >= a.dims[0])
-                        continue;
-                    const int32_t cell = cx + cy * a.dims[0] + cz * a.dims[0] * a.dims[1];
-                    for (int32_t t = a.cell_start[cell]; t < a.cell_start[cell + 1]; ++t)
-                    {
-                        const int32_t j = a.cell_idx[t];
-                        if (j == i)
-                            continue;
-                        const float4 pj = a.pos[j];
-                        const float ddx = pj.x - pi.x, ddy = pj.y - pi.y, ddz = pj.z - pi.z;
-                        const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                        if (!(r2 < a.r2list))
-                            continue;
-                        if (a.j_mask != 0u && !(fbits(a.jprm[j].w) & a.j_mask))
-                            continue;
-                        if (a.exclude_bonded && bonded(a.sp_ptr, a.sp_j, i, j))
-                            continue;
-                        if (cnt < a.cap)
-                            a.nbl[(size_t)cnt * a.npad + i] = j;
-                        ++cnt;
-                    }
-                }
-            }
-        }
-    }
-    a.nbn[i] = cnt < a.cap ? cnt : a.cap;
-    if (cnt > a.cap)
-        atomicOr(&a.status[2], 1);
-    if (cnt > 0)
-        atomicMax(&a.status[3], cnt);
-}
-
-// ---------------------------------------------------------------------------
-// Pair forces over the list.  One lane per particle i; forces on i only
-// (each dynamic-dynamic pair is visited from both sides), energies weighted
-// 1/2 for a dynamic partner and 1 for a static one (Particle.cpp:231-249).
-// Formulas: energy/steric.hpp, electrostatic.hpp, hydrophobic.hpp.
-// ---------------------------------------------------------------------------
-template <int SMODE, int EMODE, int HYDRO>
-__global__ void __launch_bounds__(BLOCK) k_pair(PairArgs a)
-{
-    __shared__ double sh[BLOCK / 64];
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    double es = 0.0, ee = 0.0, eh = 0.0;
-    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-    const float cs = (float)a.k.c_spring;
-    // Coulomb prefactors: energy kJ/mol for q1 q2 / (eps d), force module
-    // q1 q2 / (4 pi eps d^2) * C_elec (electrostatic.hpp:16-49).
-    const float ke = (float)(a.k.e_coul / a.diel) * a.elec_scale;
-    const float kf = (float)(a.k.c_elec / (4.0 * a.k.pi * a.diel)) * a.elec_scale;
-    if (i < a.n)
-    {
-        const int32_t cnt = a.nbn[i];
-        const float4 pi = a.pos[i];
-        const float4 ri = a.jprm[i];
-        const uint32_t fli = fbits(ri.w);
-        for (int32_t t = 0; t < cnt; ++t)
-        {
-            const int32_t j = a.nbl[(size_t)t * a.npad + i];
-            const float4 pj = a.pos[j];
-            const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
-            const float r2 = dx * dx + dy * dy + dz * dz;
-            if (!in_cutoff(pi, pj, a.rc, a.rc2lo, a.rc2hi, r2))
-                continue;
-            const float4 rj = a.jprm[j];
-            const uint32_t flj = fbits(rj.w);
-            const float w = (flj & PF_STATIC) ? 1.0f : 0.5f;
-            float fscal = 0.0f; // force = (dx,dy,dz) * fscal
-            if (r2 > 0.0f)
-            {
-                float inv = rsqrtf(r2);
-                inv = inv * (1.5f - 0.5f * r2 * inv * inv);
-                const float inv2 = inv * inv;
-                if (SMODE >= 0)
-                {
-                    float e = 0.0f, f = 0.0f;
-                    if (SMODE == 0)
-                    { // linear: ri + rj from raw radii
-                        const float d = r2 * inv;
-                        const float dv = d - (ri.x + rj.x);
-                        if (!(dv > 0.0f))
-                        {
-                            e = 0.5f * dv * dv;
-                            f = dv * cs * inv; // -|dv| * C / d
-                        }
-                    }
-                    else if (r2 >= 0.01f)
-                    {
-                        if (SMODE == 1)
-                        { // Amber 12-6: sqrt(r), sqrt(eps) per bead
-                            const float eij = ri.y * rj.y;
-                            const float R = ri.x * rj.x;
-                            const float s2 = R * R * inv2, s6 = s2 * s2 * s2, s12 = s6 * s6;
-                            e = eij * (s12 - 2.0f * s6);
-                            f = 12.0f * eij * (s6 - s12) * inv2 * cs;
-                        }
-                        else if (SMODE == 2)
-                        { // Lewitt 8-6
-                            const float eij = ri.y * rj.y;
-                            const float R = ri.x * rj.x;
-                            const float s2 = R * R * inv2, s6 = s2 * s2 * s2, s8 = s6 * s2;
-                            e = eij * (3.0f * s8 - 4.0f * s6);
-                            f = 24.0f * eij * (s6 - s8) * inv2 * cs;
-                        }
-                        else
-                        { // Zacharias 8-6: raw r, eps, product rules
-                            const float eij = ri.y * rj.y;
-                            const float R = ri.x * rj.x;
-                            const float s2 = R * R * inv2, s6 = s2 * s2 * s2, s8 = s6 * s2;
-                            e = eij * (s8 - s6);
-                            f = eij * (6.0f * s6 - 8.0f * s8) * inv2 * cs;
-                        }
-                    }
-                    es += (double)(w * a.steric_scale * e);
-                    fscal += a.steric_scale * f;
-                }
-                if (EMODE > 0 && (fli & PF_CHARGED) && (flj & PF_CHARGED) && r2 >= 1.0e-6f)
-                {
-                    const float qq = pi.w * pj.w;
-                    if (EMODE == 1)
-                    {
-                        ee += (double)(w * ke * qq * inv);
-                        fscal -= kf * qq * inv * inv2;
-                    }
-                    else
-                    { // eps(r) = eps * r
-                        ee += (double)(w * ke * qq * inv2);
-                        fscal -= 2.0f * kf * qq * inv2 * inv2;
-                    }
-                }
-                if (HYDRO && (fli & PF_HYDRO) && (flj & PF_HYDRO))
-                {
-                    const float d = r2 * inv;
-                    const float hh = ri.z * rj.z * __expf(-d);
-                    eh += (double)(w * a.hydro_scale) * (-(double)hh * 6.02214169E23 * 1.0E-3);
-                    fscal += a.hydro_scale * hh * cs * inv;
-                }
-            }
-            else if (SMODE == 0)
-            {
-                const float s = ri.x + rj.x; // coincident pair: linear energy, zero force
-                es += (double)(w * a.steric_scale * 0.5f * s * s);
-            }
-            fx += dx * fscal;
-            fy += dy * fscal;
-            fz += dz * fscal;
-        }
-    }
-    const double bs = block_sum(es, sh);
-    const double be = block_sum(ee, sh);
-    const double bh = block_sum(eh, sh);
-    if (threadIdx.x == 0)
-    {
-        a.epart[3 * blockIdx.x + 0] = bs;
-        a.epart[3 * blockIdx.x + 1] = be;
-        a.epart[3 * blockIdx.x + 2] = bh;
-    }
-    if (i < a.n)
-    {
-        float4 f = make_float4(fx, fy, fz, 0.0f);
-        if (a.accumulate)
-        {
-            const float4 o = a.fnb[i];
-            f.x += o.x;
-            f.y += o.y;
-            f.z += o.z;
-        }
-        a.fnb[i] = f;
-    }
-}
-
-// ---------------------------------------------------------------------------
 // IMPALA (imp.hpp:29-159), FP32 geometry with FP64 energy/derivative, as the
 // reference; returns the energy and adds the force (ForceField.cpp:80-86).
 // ---------------------------------------------------------------------------
-__device__ inline double imp_eval(float x, float y, float z, float sasa, float tr, float uo, float lo, float uc,
-                                  float lc, float scale, double c_spring, float f[3])
+__device__ inline double imp_eval(float x, float z, float sasa, float tr, float uo, float lo, float uc, float lc,
+                                  float scale, double c_spring, float f[3])
 {
-    (void)y;
     const float alpha = 1.99f, z0 = 15.75f, alip = -0.018f;
     const bool flat = uo == 0.0f && lo == 0.0f && uc == 0.0f && lc == 0.0f;
     const float ru = uc == 0.0f ? 1000000.0f : fabsf(1.0f / uc);
@@ -544,12 +258,44 @@ __device__ inline double imp_eval(float x, float y, float z, float sasa, float t
     return (double)scale * (flat ? eu : eu + el);
 }
 
+// One spring edge seen from row i: force on i and the energy share.  The
+// extension len - L0 is formed from an FP64 length (rsqrt seed plus one
+// Newton step) so that it keeps full accuracy near rest.
+template <bool COMP>
+__device__ inline void spring_edge(float4 pi, float4 li, float4 pj, const float4 *plo, int32_t j, float k, float l0,
+                                   bool jstatic, double cs, float scale, float &fx, float &fy, float &fz, double &e)
+{
+    double dx = (double)pj.x - (double)pi.x;
+    double dy = (double)pj.y - (double)pi.y;
+    double dz = (double)pj.z - (double)pi.z;
+    if (COMP)
+    {
+        const float4 lj = plo[j];
+        dx += (double)lj.x - (double)li.x;
+        dy += (double)lj.y - (double)li.y;
+        dz += (double)lj.z - (double)li.z;
+    }
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    double len = 0.0;
+    float inv = 0.0f;
+    if (r2 > 0.0)
+    {
+        inv = rsqrtf((float)r2);
+        const double s = r2 * (double)inv;
+        len = fma(fma(-s, s, r2), 0.5 * (double)inv, s);
+    }
+    const double ext = len - (double)l0;
+    const float fm = (float)((double)k * ext * cs) * inv;
+    fx += (float)dx * fm;
+    fy += (float)dy * fm;
+    fz += (float)dz * fm;
+    e += (jstatic ? 1.0 : 0.5) * 0.5 * (double)scale * (double)k * ext * ext;
+}
+
 // ---------------------------------------------------------------------------
-// Springs (CSR gather, G lanes per row) fused with viscosity, external and
-// IMPALA forces and the viscous semi-implicit Euler step
-// (SpringNetwork.cpp:63-90,202-241; Spring.cpp:20-34; Particle.cpp:80-128).
-// The extension len - L0 is formed from an FP64 length so that it keeps full
-// accuracy when the spring is near rest.
+// Springs (CSR gather, G lanes per row, 4 edges in flight per lane) fused with
+// viscosity, external and IMPALA forces and the viscous semi-implicit Euler
+// step (SpringNetwork.cpp:63-90,202-241; Spring.cpp:20-34; Particle.cpp:80-128).
 // ---------------------------------------------------------------------------
 template <int G, bool INTEGRATE, bool COMP>
 __global__ void __launch_bounds__(BLOCK) k_spring(SpringArgs a)
@@ -575,37 +321,29 @@ __global__ void __launch_bounds__(BLOCK) k_spring(SpringArgs a)
     {
         const int32_t beg = a.sp_ptr[row], end = a.sp_ptr[row + 1];
         const double cs = a.k.c_spring * (double)a.spring_scale;
-        for (int32_t e = beg + lane; e < end; e += G)
+        constexpr int U = 4;
+        for (int32_t e0 = beg + lane; e0 < end; e0 += U * G)
         {
-            const int32_t jr = a.sp_j[e];
-            const int32_t j = jr & 0x7fffffff;
-            const float4 pj = a.pos[j];
-            double dx = (double)pj.x - (double)pi.x;
-            double dy = (double)pj.y - (double)pi.y;
-            double dz = (double)pj.z - (double)pi.z;
-            if (COMP)
+            int32_t jr[U];
+            float kk[U], ll[U];
+            float4 pj[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
             {
-                const float4 lj = a.plo[j];
-                dx += (double)lj.x - (double)li.x;
-                dy += (double)lj.y - (double)li.y;
-                dz += (double)lj.z - (double)li.z;
+                const int32_t e = e0 + u * G;
+                const bool ok = e < end;
+                jr[u] = ok ? a.sp_j[e] : 0;
+                kk[u] = ok ? a.sp_k[e] : 0.0f;
+                ll[u] = ok ? a.sp_l0[e] : 0.0f;
             }
-            const double r2 = dx * dx + dy * dy + dz * dz;
-            const double k = a.sp_k[e];
-            double len = 0.0;
-            float inv = 0.0f;
-            if (r2 > 0.0)
-            {
-                inv = rsqrtf((float)r2);
-                const double s = r2 * (double)inv;
-                len = fma(fma(-s, s, r2), 0.5 * (double)inv, s);
-            }
-            const double ext = len - (double)a.sp_l0[e];
-            const float fm = (float)(k * ext * cs) * inv;
-            fx += (float)dx * fm;
-            fy += (float)dy * fm;
-            fz += (float)dz * fm;
-            espr += (jr < 0 ? 1.0 : 0.5) * 0.5 * (double)a.spring_scale * k * ext * ext;
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                pj[u] = a.pos[jr[u] & 0x7fffffff];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (e0 + u * G < end)
+                    spring_edge<COMP>(pi, li, pj[u], a.plo, jr[u] & 0x7fffffff, kk[u], ll[u], jr[u] < 0, cs,
+                                      a.spring_scale, fx, fy, fz, espr);
         }
     }
     // reduce the force over the G lanes of the row
@@ -642,7 +380,7 @@ __global__ void __launch_bounds__(BLOCK) k_spring(SpringArgs a)
         if (a.imp_enable)
         {
             float f[3] = {fx, fy, fz};
-            eimp += imp_eval(pi.x, pi.y, pi.z, ip.y, ip.z, a.imp_uo, a.imp_lo, a.imp_uc, a.imp_lc, a.imp_scale,
+            eimp += imp_eval(pi.x, pi.z, ip.y, ip.z, a.imp_uo, a.imp_lo, a.imp_uc, a.imp_lc, a.imp_scale,
                              a.k.c_spring, f);
             fx = f[0];
             fy = f[1];
@@ -773,45 +511,6 @@ __global__ void k_fill_i32(int32_t *p, int32_t v, int64_t n, const int32_t *gate
         p[i] = v;
 }
 
-// Processed pairs of one search kind, as (min, max) original ids.
-__global__ void k_pairs_extract(const float4 *pos, const float4 *jprm, const int32_t *nbl, const int32_t *nbn,
-                                const int32_t *perm, int32_t n, int32_t npad, float rc, float rc2lo, float rc2hi,
-                                uint32_t i_mask, uint32_t j_mask, int32_t nonzero, uint32_t *out, int32_t *count,
-                                int64_t cap)
-{
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n)
-        return;
-    const uint32_t fi = fbits(jprm[i].w);
-    if (i_mask != 0u && !(fi & i_mask))
-        return;
-    const float4 pi = pos[i];
-    const int32_t cnt = nbn[i];
-    for (int32_t t = 0; t < cnt; ++t)
-    {
-        const int32_t j = nbl[(size_t)t * npad + i];
-        const uint32_t fj = fbits(jprm[j].w);
-        if (j_mask != 0u && !(fj & j_mask))
-            continue;
-        if (!(fj & PF_STATIC) && perm[j] < perm[i])
-            continue;
-        const float4 pj = pos[j];
-        const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        if (!in_cutoff(pi, pj, rc, rc2lo, rc2hi, r2))
-            continue;
-        if (nonzero && !(dx != 0.0f || dy != 0.0f || dz != 0.0f))
-            continue;
-        const int32_t slot = atomicAdd(count, 1);
-        if (slot < cap)
-        {
-            const uint32_t a = (uint32_t)perm[i], b = (uint32_t)perm[j];
-            out[2 * (size_t)slot] = a < b ? a : b;
-            out[2 * (size_t)slot + 1] = a < b ? b : a;
-        }
-    }
-}
-
 inline int32_t nblk(int64_t n, int32_t b = BLOCK) { return (int32_t)((n + b - 1) / b); }
 
 } // namespace
@@ -851,39 +550,6 @@ void launch_cell_sort(const int32_t *cell_of, int32_t n, const int32_t *start, c
 {
     if (n > 0)
         k_cell_sort<<<nblk(n), BLOCK, 0, s>>>(cell_of, n, start, tmp, key, out, gate);
-}
-
-void launch_list_build(const ListArgs &a, hipStream_t s)
-{
-    if (a.n > 0)
-        k_list_build<<<nblk(a.n), BLOCK, 0, s>>>(a);
-}
-
-template <int S, int E, int H> static void pair3(const PairArgs &a, int32_t nb, hipStream_t s)
-{
-    k_pair<S, E, H><<<nb, BLOCK, 0, s>>>(a);
-}
-
-template <int S> static void pair2(const PairArgs &a, int32_t em, int32_t h, int32_t nb, hipStream_t s)
-{
-    if (em == 0)
-        h ? pair3<S, 0, 1>(a, nb, s) : pair3<S, 0, 0>(a, nb, s);
-    else if (em == 1)
-        h ? pair3<S, 1, 1>(a, nb, s) : pair3<S, 1, 0>(a, nb, s);
-    else
-        h ? pair3<S, 2, 1>(a, nb, s) : pair3<S, 2, 0>(a, nb, s);
-}
-
-void launch_pair(const PairArgs &a, int32_t elec_mode, int32_t hydro, int32_t nb, hipStream_t s)
-{
-    switch (a.steric_mode)
-    {
-    case -1: pair2<-1>(a, elec_mode, hydro, nb, s); break;
-    case 0: pair2<0>(a, elec_mode, hydro, nb, s); break;
-    case 1: pair2<1>(a, elec_mode, hydro, nb, s); break;
-    case 2: pair2<2>(a, elec_mode, hydro, nb, s); break;
-    default: pair2<3>(a, elec_mode, hydro, nb, s); break;
-    }
 }
 
 template <int G> static void spring_g(const SpringArgs &a, int32_t nb, hipStream_t s)
@@ -937,16 +603,6 @@ void launch_fill_i32(int32_t *p, int32_t v, int64_t n, const int32_t *gate, hipS
         const int64_t b = (n + BLOCK - 1) / BLOCK;
         k_fill_i32<<<(int32_t)(b < 4096 ? b : 4096), BLOCK, 0, s>>>(p, v, n, gate);
     }
-}
-
-void launch_pairs_extract(const float4 *pos, const float4 *jprm, const int32_t *nbl, const int32_t *nbn,
-                          const int32_t *perm, int32_t n, int32_t npad, float rc, float rc2lo, float rc2hi,
-                          uint32_t i_mask, uint32_t j_mask, int32_t nonzero, uint32_t *out, int32_t *count,
-                          int64_t cap, hipStream_t s)
-{
-    if (n > 0)
-        k_pairs_extract<<<nblk(n), BLOCK, 0, s>>>(pos, jprm, nbl, nbn, perm, n, npad, rc, rc2lo, rc2hi, i_mask,
-                                                   j_mask, nonzero, out, count, cap);
 }
 
 } // namespace bs

@@ -1,0 +1,392 @@
+// bs_pair.hip -- non-bonded pair lists and pair forces on CDNA4 (gfx950).
+//
+// Particles are grouped into i-clusters of 8 consecutive internal indices
+// (the internal order is spatial).  A neighbour list holds, per i-cluster,
+// entries j | m << 24 where the 8-bit m marks which of the cluster's particles
+// have j within the list radius and not spring-bonded.  Kernels run one wave64
+// per i-cluster with lanes laid out as 8 i x 8 entries, so that every load of
+// a j's data serves 8 lanes, and accumulate forces on i only (each pair is
+// visited from both sides): no atomics, bitwise reproducible.
+#include "bs_device.hpp"
+
+#include <cmath>
+
+namespace bs
+{
+
+namespace
+{
+
+constexpr int WAVES = BLOCK / 64;
+
+__device__ inline uint32_t bonded_mask(const int32_t *ex_ptr, const uint32_t *ex_w, int32_t ci, int32_t j)
+{
+    int32_t lo = ex_ptr[ci];
+    const int32_t end = ex_ptr[ci + 1];
+    int32_t hi = end;
+    while (lo < hi)
+    {
+        const int32_t mid = (lo + hi) >> 1;
+        if ((int32_t)(ex_w[mid] >> 8) < j)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return (lo < end && (int32_t)(ex_w[lo] >> 8) == j) ? (ex_w[lo] & 0xFFu) : 0u;
+}
+
+// ---------------------------------------------------------------------------
+// List build: one wave per i-cluster, one candidate j per lane.  Candidates
+// come from the cells covering the cluster's bounding box grown by the list
+// radius, so the search is complete for any cluster shape.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(BLOCK) k_cluster_list(ListArgs a)
+{
+    if (gated_off(a.gate))
+        return;
+    const int lane = threadIdx.x & 63;
+    const int32_t ci = blockIdx.x * WAVES + (threadIdx.x >> 6);
+    if (ci >= a.ncl)
+        return; // whole wave
+    const int32_t il = lane & 7;
+    const int32_t iself = ci * CLUSTER + il;
+    float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
+    uint32_t fi = PF_STATIC;
+    if (iself < a.n)
+    {
+        pi = a.pos[iself];
+        fi = fbits(a.jprm[iself].w);
+    }
+    const bool vi = lane < CLUSTER && iself < a.n && !(fi & PF_STATIC) && (a.i_mask == 0u || (fi & a.i_mask));
+    const uint32_t vmask = (uint32_t)(__ballot(vi) & 0xFFull);
+    int32_t count = 0;
+    if (vmask != 0u)
+    {
+        float xi[CLUSTER], yi[CLUSTER], zi[CLUSTER];
+        float bmin[3] = {INFINITY, INFINITY, INFINITY}, bmax[3] = {-INFINITY, -INFINITY, -INFINITY};
+#pragma unroll
+        for (int k = 0; k < CLUSTER; ++k)
+        {
+            xi[k] = __shfl(pi.x, k, 64);
+            yi[k] = __shfl(pi.y, k, 64);
+            zi[k] = __shfl(pi.z, k, 64);
+            if ((vmask >> k) & 1u)
+            {
+                bmin[0] = fminf(bmin[0], xi[k]);
+                bmin[1] = fminf(bmin[1], yi[k]);
+                bmin[2] = fminf(bmin[2], zi[k]);
+                bmax[0] = fmaxf(bmax[0], xi[k]);
+                bmax[1] = fmaxf(bmax[1], yi[k]);
+                bmax[2] = fmaxf(bmax[2], zi[k]);
+            }
+        }
+        int32_t clo[3], chi[3];
+        for (int q = 0; q < 3; ++q)
+        {
+            const double mn = a.box[q], len = a.box[3 + q] - a.box[q];
+            clo[q] = internal_coord(bmin[q] - a.radius, mn, len, a.dims[q], a.cell_edge);
+            chi[q] = internal_coord(bmax[q] + a.radius, mn, len, a.dims[q], a.cell_edge);
+        }
+        for (int32_t cz = clo[2]; cz <= chi[2]; ++cz)
+            for (int32_t cy = clo[1]; cy <= chi[1]; ++cy)
+            {
+                // cells cx = clo..chi are contiguous in the cell array
+                const int32_t row = cy * a.dims[0] + cz * a.dims[0] * a.dims[1];
+                const int32_t s = a.cell_start[row + clo[0]];
+                const int32_t e = a.cell_start[row + chi[0] + 1];
+                for (int32_t t = s; t < e; t += 64)
+                {
+                    const int32_t idx = t + lane;
+                    uint32_t m = 0u;
+                    int32_t j = 0;
+                    if (idx < e)
+                    {
+                        j = a.cell_idx[idx];
+                        const float4 pj = a.pos[j];
+                        const uint32_t fj = fbits(a.jprm[j].w);
+                        if (a.j_mask == 0u || (fj & a.j_mask))
+                        {
+#pragma unroll
+                            for (int k = 0; k < CLUSTER; ++k)
+                            {
+                                const float dx = pj.x - xi[k], dy = pj.y - yi[k], dz = pj.z - zi[k];
+                                const float r2 = dx * dx + dy * dy + dz * dz;
+                                if (r2 < a.r2list && j != ci * CLUSTER + k)
+                                    m |= 1u << k;
+                            }
+                            m &= vmask;
+                            if (m != 0u && a.exclude_bonded)
+                                m &= ~bonded_mask(a.ex_ptr, a.ex_w, ci, j);
+                        }
+                    }
+                    const uint64_t bal = __ballot(m != 0u);
+                    if (bal != 0ull)
+                    {
+                        const int32_t off = __popcll(bal & ((1ull << lane) - 1ull));
+                        if (m != 0u)
+                        {
+                            const int32_t slot = count + off;
+                            if (slot < a.cap)
+                                a.cl[(size_t)ci * a.cap + slot] = (uint32_t)j | (m << 24);
+                        }
+                        count += __popcll(bal);
+                    }
+                }
+            }
+    }
+    if (lane == 0)
+    {
+        a.cln[ci] = count < a.cap ? count : a.cap;
+        if (count > a.cap)
+            atomicOr(&a.status[2], 1);
+        if (count > 0)
+            atomicMax(&a.status[3], count);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Pair forces.  Energies are weighted 1/2 for a dynamic partner and 1 for a
+// static one (Particle.cpp:231-249).  Formulas: energy/steric.hpp,
+// electrostatic.hpp, hydrophobic.hpp; per-bead sqrt(r), sqrt(eps) for the
+// Lorentz-Berthelot / Good-Hope modes (CombinationRules.hpp:18-31).
+// ---------------------------------------------------------------------------
+template <int SMODE, int EMODE, int HYDRO>
+__global__ void __launch_bounds__(BLOCK) k_pair(PairArgs a)
+{
+    __shared__ double sh[WAVES];
+    const int lane = threadIdx.x & 63;
+    const int32_t ci = blockIdx.x * WAVES + (threadIdx.x >> 6);
+    const int32_t il = lane & 7, jl = lane >> 3;
+    const int32_t i = ci * CLUSTER + il;
+    double es = 0.0, ee = 0.0, eh = 0.0;
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    const float cs = (float)a.k.c_spring;
+    // Coulomb prefactors: energy kJ/mol for q1 q2 / (eps d), force module
+    // q1 q2 / (4 pi eps d^2) * C_elec (electrostatic.hpp:16-49).
+    const float ke = (float)(a.k.e_coul / a.diel) * a.elec_scale;
+    const float kf = (float)(a.k.c_elec / (4.0 * a.k.pi * a.diel)) * a.elec_scale;
+    if (ci < a.ncl && i < a.n)
+    {
+        const float4 pi = a.pos[i];
+        const float4 ri = a.jprm[i];
+        const uint32_t fli = fbits(ri.w);
+        const int32_t cnt = a.cln[ci];
+        const uint32_t *L = a.cl + (size_t)ci * a.cap;
+        for (int32_t t = 0; t < cnt; t += 8)
+        {
+            const int32_t e = t + jl;
+            const uint32_t w = e < cnt ? L[e] : 0u;
+            if (!((w >> (24 + il)) & 1u))
+                continue;
+            const int32_t j = (int32_t)(w & CL_JMASK);
+            const float4 pj = a.pos[j];
+            const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            if (!in_cutoff(pi, pj, a.rc, a.rc2lo, a.rc2hi, r2))
+                continue;
+            const float4 rj = a.jprm[j];
+            const uint32_t flj = fbits(rj.w);
+            const float wgt = (flj & PF_STATIC) ? 1.0f : 0.5f;
+            float fscal = 0.0f; // force on i = (dx, dy, dz) * fscal
+            if (r2 > 0.0f)
+            {
+                float inv = rsqrtf(r2);
+                inv = inv * (1.5f - 0.5f * r2 * inv * inv);
+                const float inv2 = inv * inv;
+                if (SMODE >= 0)
+                {
+                    float en = 0.0f, f = 0.0f;
+                    if (SMODE == 0)
+                    { // linear: ri + rj from raw radii
+                        const float d = r2 * inv;
+                        const float dv = d - (ri.x + rj.x);
+                        if (!(dv > 0.0f))
+                        {
+                            en = 0.5f * dv * dv;
+                            f = dv * cs * inv; // -|dv| * C / d
+                        }
+                    }
+                    else if (r2 >= 0.01f)
+                    {
+                        const float eij = ri.y * rj.y;
+                        const float R = ri.x * rj.x;
+                        const float s2 = R * R * inv2, s6 = s2 * s2 * s2;
+                        if (SMODE == 1)
+                        { // Amber 12-6
+                            const float s12 = s6 * s6;
+                            en = eij * (s12 - 2.0f * s6);
+                            f = 12.0f * eij * (s6 - s12) * inv2 * cs;
+                        }
+                        else if (SMODE == 2)
+                        { // Lewitt 8-6
+                            const float s8 = s6 * s2;
+                            en = eij * (3.0f * s8 - 4.0f * s6);
+                            f = 24.0f * eij * (s6 - s8) * inv2 * cs;
+                        }
+                        else
+                        { // Zacharias 8-6: raw r, eps, product rules
+                            const float s8 = s6 * s2;
+                            en = eij * (s8 - s6);
+                            f = eij * (6.0f * s6 - 8.0f * s8) * inv2 * cs;
+                        }
+                    }
+                    es += (double)(wgt * a.steric_scale * en);
+                    fscal += a.steric_scale * f;
+                }
+                if (EMODE > 0 && (fli & PF_CHARGED) && (flj & PF_CHARGED) && r2 >= 1.0e-6f)
+                {
+                    const float qq = pi.w * pj.w;
+                    if (EMODE == 1)
+                    {
+                        ee += (double)(wgt * ke * qq * inv);
+                        fscal -= kf * qq * inv * inv2;
+                    }
+                    else
+                    { // eps(r) = eps * r
+                        ee += (double)(wgt * ke * qq * inv2);
+                        fscal -= 2.0f * kf * qq * inv2 * inv2;
+                    }
+                }
+                if (HYDRO && (fli & PF_HYDRO) && (flj & PF_HYDRO))
+                {
+                    const float d = r2 * inv;
+                    const float hh = ri.z * rj.z * __expf(-d);
+                    eh += (double)(wgt * a.hydro_scale) * (-(double)hh * 6.02214169E23 * 1.0E-3);
+                    fscal += a.hydro_scale * hh * cs * inv;
+                }
+            }
+            else if (SMODE == 0)
+            {
+                const float s = ri.x + rj.x; // coincident pair: linear energy, zero force
+                es += (double)(wgt * a.steric_scale * 0.5f * s * s);
+            }
+            fx += dx * fscal;
+            fy += dy * fscal;
+            fz += dz * fscal;
+        }
+    }
+    // sum over the 8 entry lanes of each i (lanes il, il+8, ..., il+56)
+    for (int off = 8; off < 64; off <<= 1)
+    {
+        fx += __shfl_xor(fx, off, 64);
+        fy += __shfl_xor(fy, off, 64);
+        fz += __shfl_xor(fz, off, 64);
+    }
+    const double bs = block_sum(es, sh);
+    const double be = block_sum(ee, sh);
+    const double bh = block_sum(eh, sh);
+    if (threadIdx.x == 0)
+    {
+        a.epart[3 * blockIdx.x + 0] = bs;
+        a.epart[3 * blockIdx.x + 1] = be;
+        a.epart[3 * blockIdx.x + 2] = bh;
+    }
+    if (ci < a.ncl && jl == 0 && i < a.n)
+    {
+        float4 f = make_float4(fx, fy, fz, 0.0f);
+        if (a.accumulate)
+        {
+            const float4 o = a.fnb[i];
+            f.x += o.x;
+            f.y += o.y;
+            f.z += o.z;
+        }
+        a.fnb[i] = f;
+    }
+}
+
+// Processed pairs of one search kind, as (min, max) original ids
+// (Particle.cpp:205-224: i dynamic, j static or later in id order).
+__global__ void __launch_bounds__(BLOCK) k_pairs_extract(const float4 *pos, const float4 *jprm, const uint32_t *cl,
+                                                         const int32_t *cln, const int32_t *perm, int32_t n,
+                                                         int32_t ncl, int32_t lcap, float rc, float rc2lo,
+                                                         float rc2hi, uint32_t i_mask, uint32_t j_mask,
+                                                         int32_t nonzero, uint32_t *out, int32_t *count, int64_t cap)
+{
+    const int lane = threadIdx.x & 63;
+    const int32_t ci = blockIdx.x * WAVES + (threadIdx.x >> 6);
+    const int32_t il = lane & 7, jl = lane >> 3;
+    const int32_t i = ci * CLUSTER + il;
+    if (ci >= ncl || i >= n)
+        return;
+    const uint32_t fi = fbits(jprm[i].w);
+    if (i_mask != 0u && !(fi & i_mask))
+        return;
+    const float4 pi = pos[i];
+    const int32_t cnt = cln[ci];
+    for (int32_t t = jl; t < cnt; t += 8)
+    {
+        const uint32_t w = cl[(size_t)ci * lcap + t];
+        if (!((w >> (24 + il)) & 1u))
+            continue;
+        const int32_t j = (int32_t)(w & CL_JMASK);
+        const uint32_t fj = fbits(jprm[j].w);
+        if (j_mask != 0u && !(fj & j_mask))
+            continue;
+        if (!(fj & PF_STATIC) && perm[j] < perm[i])
+            continue;
+        const float4 pj = pos[j];
+        const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        if (!in_cutoff(pi, pj, rc, rc2lo, rc2hi, r2))
+            continue;
+        if (nonzero && !(dx != 0.0f || dy != 0.0f || dz != 0.0f))
+            continue;
+        const int32_t slot = atomicAdd(count, 1);
+        if (slot < cap)
+        {
+            const uint32_t u = (uint32_t)perm[i], v = (uint32_t)perm[j];
+            out[2 * (size_t)slot] = u < v ? u : v;
+            out[2 * (size_t)slot + 1] = u < v ? v : u;
+        }
+    }
+}
+
+inline int32_t cluster_blocks(int32_t ncl) { return (ncl + WAVES - 1) / WAVES; }
+
+} // namespace
+
+void launch_list_build(const ListArgs &a, hipStream_t s)
+{
+    if (a.ncl > 0)
+        k_cluster_list<<<cluster_blocks(a.ncl), BLOCK, 0, s>>>(a);
+}
+
+template <int S, int E, int H> static void pair3(const PairArgs &a, int32_t nb, hipStream_t s)
+{
+    k_pair<S, E, H><<<nb, BLOCK, 0, s>>>(a);
+}
+
+template <int S> static void pair2(const PairArgs &a, int32_t em, int32_t h, int32_t nb, hipStream_t s)
+{
+    if (em == 0)
+        h ? pair3<S, 0, 1>(a, nb, s) : pair3<S, 0, 0>(a, nb, s);
+    else if (em == 1)
+        h ? pair3<S, 1, 1>(a, nb, s) : pair3<S, 1, 0>(a, nb, s);
+    else
+        h ? pair3<S, 2, 1>(a, nb, s) : pair3<S, 2, 0>(a, nb, s);
+}
+
+void launch_pair(const PairArgs &a, int32_t elec_mode, int32_t hydro, int32_t nb, hipStream_t s)
+{
+    switch (a.steric_mode)
+    {
+    case -1: pair2<-1>(a, elec_mode, hydro, nb, s); break;
+    case 0: pair2<0>(a, elec_mode, hydro, nb, s); break;
+    case 1: pair2<1>(a, elec_mode, hydro, nb, s); break;
+    case 2: pair2<2>(a, elec_mode, hydro, nb, s); break;
+    default: pair2<3>(a, elec_mode, hydro, nb, s); break;
+    }
+}
+
+void launch_pairs_extract(const float4 *pos, const float4 *jprm, const uint32_t *cl, const int32_t *cln,
+                          const int32_t *perm, int32_t n, int32_t ncl, int32_t lcap, float rc, float rc2lo,
+                          float rc2hi, uint32_t i_mask, uint32_t j_mask, int32_t nonzero, uint32_t *out,
+                          int32_t *count, int64_t cap, hipStream_t s)
+{
+    if (ncl > 0)
+        k_pairs_extract<<<cluster_blocks(ncl), BLOCK, 0, s>>>(pos, jprm, cl, cln, perm, n, ncl, lcap, rc, rc2lo,
+                                                               rc2hi, i_mask, j_mask, nonzero, out, count, cap);
+}
+
+} // namespace bs

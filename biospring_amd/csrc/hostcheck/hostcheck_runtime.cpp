@@ -313,14 +313,8 @@ void launch_cell_sort(const int32_t *cell_of, int32_t n, const int32_t *start, c
     }
 }
 
-static bool bonded(const int32_t *sp_ptr, const int32_t *sp_j, int32_t i, int32_t j)
-{
-    for (int32_t e = sp_ptr[i]; e < sp_ptr[i + 1]; ++e)
-        if ((sp_j[e] & 0x7fffffff) == j)
-            return true;
-    return false;
-}
-
+// Cluster list build on the CPU with the kernel's semantics (search range =
+// cells covering the cluster bounding box grown by the radius).
 void launch_list_build(const ListArgs &a, hipStream_t)
 {
     const int64_t nc = (int64_t)a.dims[0] * a.dims[1] * a.dims[2];
@@ -329,58 +323,91 @@ void launch_list_build(const ListArgs &a, hipStream_t)
     need(a.cell_start, sizeof(int32_t) * (nc + 1), "list: cell_start");
     need(a.cell_idx, sizeof(int32_t) * a.n, "list: cell_idx");
     need(a.box, sizeof(double) * 6, "list: box");
-    need(a.sp_ptr, sizeof(int32_t) * (a.n + 1), "list: sp_ptr");
-    need(a.nbn, sizeof(int32_t) * a.n, "list: nbn");
+    need(a.cln, sizeof(int32_t) * a.ncl, "list: cln");
     need(a.status, sizeof(int32_t) * 4, "list: status");
     if (a.cap > 0)
-        need(a.nbl, sizeof(int32_t) * (size_t)a.npad * a.cap, "list: nbl");
-    if (a.npad < a.n || a.npad % 64)
-        hc_fail("list: bad npad");
+        need(a.cl, sizeof(uint32_t) * (size_t)a.ncl * a.cap, "list: cl");
+    if ((int64_t)a.ncl * CLUSTER < a.n || (int64_t)(a.ncl - 1) * CLUSTER >= a.n)
+        hc_fail("list: bad cluster count");
+    if (a.exclude_bonded)
+        need(a.ex_ptr, sizeof(int32_t) * (a.ncl + 1), "list: ex_ptr");
     if (gated_off(a.gate))
         return;
-    need(a.sp_j, sizeof(int32_t) * std::max(a.sp_ptr[a.n], 1), "list: sp_j");
-    for (int32_t i = 0; i < a.n; ++i)
+    if (a.exclude_bonded)
+        need(a.ex_w, sizeof(uint32_t) * std::max(a.ex_ptr[a.ncl], 1), "list: ex_w");
+    for (int32_t ci = 0; ci < a.ncl; ++ci)
     {
-        const uint32_t fi = fbits(a.jprm[i].w);
-        int32_t cnt = 0;
-        if (!(fi & PF_STATIC) && (a.i_mask == 0u || (fi & a.i_mask)))
+        uint32_t vmask = 0;
+        float xi[CLUSTER], yi[CLUSTER], zi[CLUSTER];
+        double bmin[3] = {INFINITY, INFINITY, INFINITY}, bmax[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int k = 0; k < CLUSTER; ++k)
         {
-            const float xs[3] = {a.pos[i].x, a.pos[i].y, a.pos[i].z};
-            int32_t c[3];
-            for (int q = 0; q < 3; ++q)
-                c[q] = internal_coord(xs[q], a.box[q], a.box[3 + q] - a.box[q], a.dims[q], a.cell_edge);
-            for (int dz = -1; dz <= 1; ++dz)
-                for (int dy = -1; dy <= 1; ++dy)
-                    for (int dx = -1; dx <= 1; ++dx)
-                    {
-                        const int32_t cx = c[0] + dx, cy = c[1] + dy, cz = c[2] + dz;
-                        if (cx < 0 || cy < 0 || cz < 0 || cx >= a.dims[0] || cy >= a.dims[1] || cz >= a.dims[2])
-                            continue;
-                        const int32_t cell = cx + cy * a.dims[0] + cz * a.dims[0] * a.dims[1];
-                        for (int32_t t = a.cell_start[cell]; t < a.cell_start[cell + 1]; ++t)
-                        {
-                            const int32_t j = a.cell_idx[t];
-                            if (j < 0 || j >= a.n)
-                                hc_fail("list: member out of range");
-                            if (j == i)
-                                continue;
-                            const float ddx = a.pos[j].x - xs[0], ddy = a.pos[j].y - xs[1], ddz = a.pos[j].z - xs[2];
-                            if (!(ddx * ddx + ddy * ddy + ddz * ddz < a.r2list))
-                                continue;
-                            if (a.j_mask && !(fbits(a.jprm[j].w) & a.j_mask))
-                                continue;
-                            if (a.exclude_bonded && bonded(a.sp_ptr, a.sp_j, i, j))
-                                continue;
-                            if (cnt < a.cap)
-                                a.nbl[(size_t)cnt * a.npad + i] = j;
-                            ++cnt;
-                        }
-                    }
+            const int32_t i = ci * CLUSTER + k;
+            xi[k] = yi[k] = zi[k] = 0.0f;
+            if (i >= a.n)
+                continue;
+            const uint32_t fi = fbits(a.jprm[i].w);
+            xi[k] = a.pos[i].x;
+            yi[k] = a.pos[i].y;
+            zi[k] = a.pos[i].z;
+            if (!(fi & PF_STATIC) && (a.i_mask == 0u || (fi & a.i_mask)))
+            {
+                vmask |= 1u << k;
+                const float v[3] = {xi[k], yi[k], zi[k]};
+                for (int q = 0; q < 3; ++q)
+                {
+                    bmin[q] = std::min(bmin[q], (double)v[q]);
+                    bmax[q] = std::max(bmax[q], (double)v[q]);
+                }
+            }
         }
-        a.nbn[i] = std::min(cnt, a.cap);
-        if (cnt > a.cap)
+        int32_t count = 0;
+        if (vmask)
+        {
+            int32_t clo[3], chi[3];
+            for (int q = 0; q < 3; ++q)
+            {
+                const double len = a.box[3 + q] - a.box[q];
+                clo[q] = internal_coord((float)bmin[q] - a.radius, a.box[q], len, a.dims[q], a.cell_edge);
+                chi[q] = internal_coord((float)bmax[q] + a.radius, a.box[q], len, a.dims[q], a.cell_edge);
+            }
+            for (int32_t cz = clo[2]; cz <= chi[2]; ++cz)
+                for (int32_t cy = clo[1]; cy <= chi[1]; ++cy)
+                {
+                    const int32_t row = cy * a.dims[0] + cz * a.dims[0] * a.dims[1];
+                    if (row + chi[0] + 1 > nc)
+                        hc_fail("list: cell row out of range");
+                    for (int32_t t = a.cell_start[row + clo[0]]; t < a.cell_start[row + chi[0] + 1]; ++t)
+                    {
+                        const int32_t j = a.cell_idx[t];
+                        if (j < 0 || j >= a.n)
+                            hc_fail("list: member out of range");
+                        if (a.j_mask && !(fbits(a.jprm[j].w) & a.j_mask))
+                            continue;
+                        uint32_t m = 0;
+                        for (int k = 0; k < CLUSTER; ++k)
+                        {
+                            const float dx = a.pos[j].x - xi[k], dy = a.pos[j].y - yi[k], dz = a.pos[j].z - zi[k];
+                            if (dx * dx + dy * dy + dz * dz < a.r2list && j != ci * CLUSTER + k)
+                                m |= 1u << k;
+                        }
+                        m &= vmask;
+                        if (m && a.exclude_bonded)
+                            for (int32_t x = a.ex_ptr[ci]; x < a.ex_ptr[ci + 1]; ++x)
+                                if ((int32_t)(a.ex_w[x] >> 8) == j)
+                                    m &= ~(a.ex_w[x] & 0xFFu);
+                        if (!m)
+                            continue;
+                        if (count < a.cap)
+                            a.cl[(size_t)ci * a.cap + count] = (uint32_t)j | (m << 24);
+                        ++count;
+                    }
+                }
+        }
+        a.cln[ci] = std::min(count, a.cap);
+        if (count > a.cap)
             a.status[2] |= 1;
-        a.status[3] = std::max(a.status[3], cnt);
+        a.status[3] = std::max(a.status[3], count);
     }
 }
 
@@ -388,25 +415,29 @@ void launch_pair(const PairArgs &a, int32_t, int32_t, int32_t nb, hipStream_t)
 {
     need(a.pos, sizeof(float4) * a.n, "pair: pos");
     need(a.jprm, sizeof(float4) * a.n, "pair: jprm");
-    need(a.nbn, sizeof(int32_t) * a.n, "pair: nbn");
+    need(a.cln, sizeof(int32_t) * a.ncl, "pair: cln");
+    need(a.cl, sizeof(uint32_t) * (size_t)a.ncl * a.cap, "pair: cl");
     need(a.fnb, sizeof(float4) * a.n, "pair: fnb");
     need(a.epart, sizeof(double) * 3 * nb, "pair: epart");
-    if ((int64_t)nb * BLOCK < a.n)
+    if ((int64_t)nb * (BLOCK / 64) < a.ncl)
         hc_fail("pair: grid too small");
-    for (int32_t i = 0; i < a.n; ++i)
+    for (int32_t ci = 0; ci < a.ncl; ++ci)
     {
-        const int32_t cnt = a.nbn[i];
-        if (cnt > 0)
-            need(a.nbl + (size_t)(cnt - 1) * a.npad + i, 4, "pair: nbl entry");
-        for (int32_t t = 0; t < cnt; ++t)
+        if (a.cln[ci] < 0 || a.cln[ci] > a.cap)
+            hc_fail("pair: list length out of range");
+        for (int32_t t = 0; t < a.cln[ci]; ++t)
         {
-            const int32_t j = a.nbl[(size_t)t * a.npad + i];
-            if (j < 0 || j >= a.n)
+            const uint32_t w = a.cl[(size_t)ci * a.cap + t];
+            if ((int32_t)(w & CL_JMASK) >= a.n)
                 hc_fail("pair: neighbour index out of range");
+            for (int k = 0; k < CLUSTER; ++k)
+                if (((w >> (24 + k)) & 1u) && ci * CLUSTER + k >= a.n)
+                    hc_fail("pair: mask bit on a padding lane");
         }
+    }
+    for (int32_t i = 0; i < a.n; ++i)
         if (!a.accumulate)
             a.fnb[i] = make_float4(0, 0, 0, 0);
-    }
     std::memset(a.epart, 0, sizeof(double) * 3 * nb);
 }
 
@@ -504,30 +535,46 @@ void launch_fill_i32(int32_t *p, int32_t v, int64_t n, const int32_t *gate, hipS
         p[i] = v;
 }
 
-void launch_pairs_extract(const float4 *pos, const float4 *jprm, const int32_t *nbl, const int32_t *nbn,
-                          const int32_t *perm, int32_t n, int32_t npad, float rc, float, float, uint32_t i_mask,
-                          uint32_t j_mask, int32_t, uint32_t *out, int32_t *count, int64_t cap, hipStream_t)
+void launch_pairs_extract(const float4 *pos, const float4 *jprm, const uint32_t *cl, const int32_t *cln,
+                          const int32_t *perm, int32_t n, int32_t ncl, int32_t lcap, float rc, float, float,
+                          uint32_t i_mask, uint32_t j_mask, int32_t, uint32_t *out, int32_t *count, int64_t cap,
+                          hipStream_t)
 {
     need(pos, sizeof(float4) * n, "extract: pos");
     need(jprm, sizeof(float4) * n, "extract: jprm");
-    need(nbn, sizeof(int32_t) * n, "extract: nbn");
+    need(cln, sizeof(int32_t) * ncl, "extract: cln");
+    need(cl, sizeof(uint32_t) * (size_t)ncl * lcap, "extract: cl");
     need(perm, sizeof(int32_t) * n, "extract: perm");
     need(count, 4, "extract: count");
-    need(out, sizeof(uint32_t) * 2 * cap, "extract: out");
+    if (cap > 0)
+        need(out, sizeof(uint32_t) * 2 * cap, "extract: out");
     for (int32_t i = 0; i < n; ++i)
     {
         if (i_mask && !(fbits(jprm[i].w) & i_mask))
             continue;
-        for (int32_t t = 0; t < nbn[i]; ++t)
+        const int32_t ci = i / CLUSTER, il = i % CLUSTER;
+        for (int32_t t = 0; t < cln[ci]; ++t)
         {
-            const int32_t j = nbl[(size_t)t * npad + i];
+            const uint32_t w = cl[(size_t)ci * lcap + t];
+            if (!((w >> (24 + il)) & 1u))
+                continue;
+            const int32_t j = (int32_t)(w & CL_JMASK);
             const uint32_t fj = fbits(jprm[j].w);
             if (j_mask && !(fj & j_mask))
                 continue;
             if (!(fj & PF_STATIC) && perm[j] < perm[i])
                 continue;
+            // the reference's two-stage test (nsearch.hpp:230, Particle.cpp:400-402)
+            const double ax = (double)pos[i].x - (double)pos[j].x, ay = (double)pos[i].y - (double)pos[j].y,
+                         az = (double)pos[i].z - (double)pos[j].z;
+            if (!(std::sqrt(ax * ax + ay * ay + az * az) < (double)rc))
+                continue;
             const float dx = pos[j].x - pos[i].x, dy = pos[j].y - pos[i].y, dz = pos[j].z - pos[i].z;
-            if (!(std::sqrt((double)dx * dx + (double)dy * dy + (double)dz * dz) < rc))
+            const double bx = dx, by = dy, bz = dz;
+            const float d = (std::fabs(bx) < 1e-40 && std::fabs(by) < 1e-40 && std::fabs(bz) < 1e-40)
+                                ? 0.0f
+                                : (float)std::sqrt(bx * bx + by * by + bz * bz);
+            if (!(d < rc))
                 continue;
             const int32_t slot = (*count)++;
             if (slot < cap)

@@ -88,6 +88,28 @@ def test_small_and_degenerate_systems(hostcheck_lib):
     """, hostcheck_lib)
 
 
+def test_cluster_lists_cover_reference_pairs(hostcheck_lib):
+    """The host-built exclusion lists and the cluster search ranges, run
+    through the checker's CPU emulation of the list build, reproduce the
+    oracle's processed pair sets exactly (also after motion and a rebuild)."""
+    _run("""
+        from oracle import pyoracle as po
+        for cfg, n, seed in ((0, 0, 2), (1, 3000, 4), (1, 12000, 5)):
+            s = workloads.generate(cfg, seed=seed, n=n)
+            net = SpringNetwork.from_system(s, library=L)
+            net.setup()
+            o = po.Oracle(s)
+            for kind in (0, 1):
+                assert np.array_equal(net.getPairs(kind), o.pairs(kind)), (cfg, n, kind)
+            rng = np.random.default_rng(seed)
+            x = s.x.reshape(-1, 3) + rng.normal(0, 0.3, (s.n, 3)).astype(np.float32)
+            net.setState(x)
+            o.set_state(x)
+            for kind in (0, 1):
+                assert np.array_equal(net.getPairs(kind), o.pairs(kind)), (cfg, n, kind, "moved")
+    """, hostcheck_lib)
+
+
 def test_argument_errors(hostcheck_lib):
     _run("""
         net = SpringNetwork(library=L)
