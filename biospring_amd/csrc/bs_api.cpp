@@ -66,8 +66,8 @@ struct bs_handle
 
     // device
     float4 *pos[2] = {nullptr, nullptr}, *plo[2] = {nullptr, nullptr};
-    float4 *vel = nullptr, *fnb = nullptr, *jprm = nullptr, *iprm = nullptr, *xref = nullptr, *fext = nullptr,
-           *fout = nullptr;
+    float4 *vel = nullptr, *jprm = nullptr, *iprm = nullptr, *xref = nullptr, *fext = nullptr, *fout = nullptr;
+    double4 *fnb = nullptr;
     int32_t *sp_ptr = nullptr, *sp_j = nullptr, *perm_d = nullptr;
     float *sp_k = nullptr, *sp_l0 = nullptr;
     int64_t nedges = 0;
@@ -158,7 +158,8 @@ void free_device(bs_handle *h)
         f(l.cln);
     }
     h->lists.clear();
-    h->vel = h->fnb = h->jprm = h->iprm = h->xref = h->fext = h->fout = nullptr;
+    h->vel = h->jprm = h->iprm = h->xref = h->fext = h->fout = nullptr;
+    h->fnb = nullptr;
     h->sp_ptr = h->sp_j = h->perm_d = nullptr;
     h->sp_k = h->sp_l0 = nullptr;
     h->cell_of = h->counts = h->cstart = h->cursor = h->ctmp = h->cidx = nullptr;
@@ -468,6 +469,10 @@ void launch_pairs(bs_handle *h)
         a.steric_scale = h->p.steric_gridscale;
         a.elec_scale = h->p.electrostatic_scale;
         a.diel = h->p.electrostatic_dielectric;
+        // Coulomb prefactors (electrostatic.hpp:16-49): energy e_C q1 q2 / (eps d),
+        // force module q1 q2 / (4 pi eps d^2) * C_elec, both times the scale
+        a.ke = h->k.e_coul / a.diel * a.elec_scale;
+        a.kf = h->k.c_elec / (4.0 * h->k.pi * a.diel) * a.elec_scale;
         a.hydro_scale = h->p.hydrophobicity_scale;
         a.i_mask = l.imask;
         a.j_mask = l.jmask;
@@ -614,7 +619,7 @@ int do_setup(bs_handle *h)
             ALLOC(h->plo[b], float4, n);
     }
     ALLOC(h->vel, float4, n);
-    ALLOC(h->fnb, float4, n);
+    ALLOC(h->fnb, double4, n);
     ALLOC(h->jprm, float4, n);
     ALLOC(h->iprm, float4, n);
     ALLOC(h->xref, float4, n);
@@ -627,7 +632,7 @@ int do_setup(bs_handle *h)
     ALLOC(h->status, int32_t, 4);
     ALLOC(h->counters, int32_t, 4);
     ALLOC(h->energies_d, double, BS_NUM_ENERGIES);
-    h->box_blocks = (int32_t)std::min<int64_t>(1024, (n + BLOCK - 1) / BLOCK);
+    h->box_blocks = (int32_t)std::min<int64_t>(BLOCK, (n + BLOCK - 1) / BLOCK); // one per final-reduce thread
     ALLOC(h->box_part, double, 6 * h->box_blocks);
     ALLOC(h->box, double, 6);
 
@@ -776,7 +781,7 @@ int do_setup(bs_handle *h)
         HIPCHK(h, hipMemcpyAsync(st, h->status, sizeof st, hipMemcpyDeviceToHost, h->stream));
         HIPCHK(h, hipStreamSynchronize(h->stream));
         const int32_t mx = st[3];
-        l.cap = (std::max<int32_t>(16, (int32_t)std::ceil(mx * 1.25) + 64) + 7) / 8 * 8;
+        l.cap = (std::max<int32_t>(16, (int32_t)std::ceil(mx * 1.25) + 64) + 15) / 16 * 16;
         ALLOC(l.cl, uint32_t, (int64_t)h->ncl * l.cap);
         launch_fill_i32(h->status + 2, 0, 1, nullptr, h->stream);
     }

@@ -5,14 +5,16 @@
 //   pos[2]   float4[N]  x, y, z, charge           ping-pong between steps
 //   plo[2]   float4[N]  low words of x, y, z      compensated positions only
 //   vel      float4[N]  vx, vy, vz, 0
-//   fnb      float4[N]  non-bonded force of the current step
+//   fnb      double4[N] non-bonded force of the current step (FP64 sums)
 //   jprm     float4[N]  sqrt(radius), sqrt(eps), hydrophobicity, flags (bits)
 //   iprm     float4[N]  mass, sasa, transfer, radius
 //   xref     float4[N]  positions at the last neighbour-list build
 //   springs  CSR over particles, both directions, each row sorted by j:
 //            sp_ptr int32[N+1], sp_j int32[E] (bit31 = j static), sp_k float[E],
 //            sp_l0 float[E]
-//   lists    ELL column-major: nbl[t * npad + i] int32, nbn[i]
+//   lists    per i-cluster of 8 particles: cl[c * cap + t] = j | mask << 24,
+//            cln[c] entries, zero words up to the next multiple of 16
+//   ex       static exclusions per i-cluster: ex_ptr[c], ex_w = j << 8 | mask
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -76,9 +78,9 @@ struct PairArgs
 {
     const float4 *pos;
     const float4 *jprm;
-    const uint32_t *cl;  // [ncl * cap]
+    const uint32_t *cl;  // [ncl * cap], zero-padded to a multiple of 16 entries
     const int32_t *cln;  // [ncl]
-    float4 *fnb;
+    double4 *fnb;       // non-bonded force, FP64
     double *epart;      // [nblocks * 3] steric, elec, hydro
     int32_t n, ncl, cap;
     int32_t accumulate; // add to fnb instead of overwriting
@@ -86,6 +88,7 @@ struct PairArgs
     int32_t steric_mode;
     float steric_scale;
     float elec_scale, diel;
+    double ke, kf;      // Coulomb energy / force prefactors incl. scale and 1/eps
     float hydro_scale;
     uint32_t i_mask, j_mask; // required flag bits (PF_CHARGED / PF_HYDRO) or 0
     Consts k;
@@ -98,7 +101,7 @@ struct SpringArgs
     float4 *pos_out;     // integrate mode
     float4 *plo_out;     // nullable
     float4 *vel;
-    const float4 *fnb;   // nullable (no pair terms)
+    const double4 *fnb;  // nullable (no pair terms)
     const float4 *fext;  // nullable
     const float4 *iprm;
     const float4 *jprm;

@@ -63,17 +63,36 @@ __global__ void k_box_partial(const float4 *pos, int32_t n, double *partial, con
     }
 }
 
-__global__ void k_box_final(const double *partial, int32_t nb, double *box, const int32_t *gate)
+// Combines up to BLOCK block partials (one per thread) with wave shuffles.
+__global__ void __launch_bounds__(BLOCK) k_box_final(const double *partial, int32_t nb, double *box,
+                                                     const int32_t *gate)
 {
     if (gated_off(gate))
         return;
-    if (threadIdx.x < 6)
+    __shared__ double sh[6][BLOCK / 64];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    double v[6];
+    for (int c = 0; c < 3; ++c)
     {
-        const int c = threadIdx.x;
-        double v = partial[c];
-        for (int b = 1; b < nb; ++b)
-            v = c < 3 ? fmin(v, partial[6 * b + c]) : fmax(v, partial[6 * b + c]);
-        box[c] = v;
+        v[c] = t < nb ? partial[6 * t + c] : INFINITY;
+        v[3 + c] = t < nb ? partial[6 * t + 3 + c] : -INFINITY;
+    }
+    for (int off = 32; off > 0; off >>= 1)
+        for (int c = 0; c < 3; ++c)
+        {
+            v[c] = fmin(v[c], __shfl_xor(v[c], off, 64));
+            v[3 + c] = fmax(v[3 + c], __shfl_xor(v[3 + c], off, 64));
+        }
+    if (lane == 0)
+        for (int c = 0; c < 6; ++c)
+            sh[c][wave] = v[c];
+    __syncthreads();
+    if (t < 6)
+    {
+        double r = sh[t][0];
+        for (int w = 1; w < BLOCK / 64; ++w)
+            r = t < 3 ? fmin(r, sh[t][w]) : fmax(r, sh[t][w]);
+        box[t] = r;
     }
 }
 
@@ -263,7 +282,7 @@ __device__ inline double imp_eval(float x, float z, float sasa, float tr, float 
 // Newton step) so that it keeps full accuracy near rest.
 template <bool COMP>
 __device__ inline void spring_edge(float4 pi, float4 li, float4 pj, const float4 *plo, int32_t j, float k, float l0,
-                                   bool jstatic, double cs, float scale, float &fx, float &fy, float &fz, double &e)
+                                   bool jstatic, double cs, float scale, double &fx, double &fy, double &fz, double &e)
 {
     double dx = (double)pj.x - (double)pi.x;
     double dy = (double)pj.y - (double)pi.y;
@@ -285,10 +304,11 @@ __device__ inline void spring_edge(float4 pi, float4 li, float4 pj, const float4
         len = fma(fma(-s, s, r2), 0.5 * (double)inv, s);
     }
     const double ext = len - (double)l0;
-    const float fm = (float)((double)k * ext * cs) * inv;
-    fx += (float)dx * fm;
-    fy += (float)dy * fm;
-    fz += (float)dz * fm;
+    const double y = inv; // 1/len refined in FP64 for the unit vector
+    const double fm = (double)k * ext * cs * (y * fma(-0.5 * r2, y * y, 1.5));
+    fx = fma(dx, fm, fx);
+    fy = fma(dy, fm, fy);
+    fz = fma(dz, fm, fz);
     e += (jstatic ? 1.0 : 0.5) * 0.5 * (double)scale * (double)k * ext * ext;
 }
 
@@ -305,7 +325,7 @@ __global__ void __launch_bounds__(BLOCK) k_spring(SpringArgs a)
     const int32_t row = tid / G;
     const int32_t lane = tid % G;
     double espr = 0.0, eke = 0.0, eimp = 0.0;
-    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
     const bool active = row < a.n;
     float4 pi = make_float4(0.f, 0.f, 0.f, 0.f), li = make_float4(0.f, 0.f, 0.f, 0.f);
     uint32_t fli = PF_STATIC;
@@ -356,36 +376,41 @@ __global__ void __launch_bounds__(BLOCK) k_spring(SpringArgs a)
     if (dyn && lane == 0)
     {
         float4 v = a.vel[row];
+        double sx = fx, sy = fy, sz = fz;
         if (a.fnb)
         {
-            const float4 fn = a.fnb[row];
-            fx += fn.x;
-            fy += fn.y;
-            fz += fn.z;
+            const double4 fn = a.fnb[row];
+            sx += fn.x;
+            sy += fn.y;
+            sz += fn.z;
         }
         if (a.fext)
         {
             const float4 fe = a.fext[row];
-            fx += fe.x;
-            fy += fe.y;
-            fz += fe.z;
+            sx += fe.x;
+            sy += fe.y;
+            sz += fe.z;
         }
+        // the conservative sum is rounded once; the viscous and IMPALA terms
+        // follow in FP32 as in the reference (Particle.cpp:80-84, 259-267)
+        float fxf = (float)sx, fyf = (float)sy, fzf = (float)sz;
         if (a.visc_enable)
         {
-            fx -= v.x * a.visc;
-            fy -= v.y * a.visc;
-            fz -= v.z * a.visc;
+            fxf -= v.x * a.visc;
+            fyf -= v.y * a.visc;
+            fzf -= v.z * a.visc;
         }
         const float4 ip = a.iprm[row];
         if (a.imp_enable)
         {
-            float f[3] = {fx, fy, fz};
+            float f[3] = {fxf, fyf, fzf};
             eimp += imp_eval(pi.x, pi.z, ip.y, ip.z, a.imp_uo, a.imp_lo, a.imp_uc, a.imp_lc, a.imp_scale,
                              a.k.c_spring, f);
-            fx = f[0];
-            fy = f[1];
-            fz = f[2];
+            fxf = f[0];
+            fyf = f[1];
+            fzf = f[2];
         }
+        const float fx = fxf, fy = fyf, fz = fzf;
         if (!INTEGRATE)
         {
             a.fout[row] = make_float4(fx, fy, fz, 0.0f);
@@ -521,7 +546,7 @@ void launch_box_reduce(const float4 *pos, int32_t n, double *partial, double *bo
                        const int32_t *gate, hipStream_t s)
 {
     k_box_partial<<<nblocks, BLOCK, 0, s>>>(pos, n, partial, gate);
-    k_box_final<<<1, 64, 0, s>>>(partial, nblocks, box, gate);
+    k_box_final<<<1, BLOCK, 0, s>>>(partial, nblocks, box, gate);
 }
 
 void launch_cell_ids(const float4 *pos, const float4 *jprm, uint32_t mask, int32_t n, const double *box,

@@ -134,6 +134,14 @@ __global__ void __launch_bounds__(BLOCK) k_cluster_list(ListArgs a)
                 }
             }
     }
+    // zero words up to the next multiple of 16, read unconditionally by the
+    // pair kernel (the capacity is a multiple of 16)
+    if (count < a.cap)
+    {
+        const int32_t pad = ((count + 15) & ~15) < a.cap ? ((count + 15) & ~15) : a.cap;
+        for (int32_t slot = count + lane; slot < pad; slot += 64)
+            a.cl[(size_t)ci * a.cap + slot] = 0u;
+    }
     if (lane == 0)
     {
         a.cln[ci] = count < a.cap ? count : a.cap;
@@ -145,11 +153,115 @@ __global__ void __launch_bounds__(BLOCK) k_cluster_list(ListArgs a)
 }
 
 // ---------------------------------------------------------------------------
-// Pair forces.  Energies are weighted 1/2 for a dynamic partner and 1 for a
-// static one (Particle.cpp:231-249).  Formulas: energy/steric.hpp,
+// One pair from i's side.  Energies are weighted 1/2 for a dynamic partner
+// and 1 for a static one (Particle.cpp:231-249).  Formulas: energy/steric.hpp,
 // electrostatic.hpp, hydrophobic.hpp; per-bead sqrt(r), sqrt(eps) for the
-// Lorentz-Berthelot / Good-Hope modes (CombinationRules.hpp:18-31).
+// Lorentz-Berthelot / Good-Hope modes (CombinationRules.hpp:18-31).  The
+// force is accumulated in FP64 (same rate as FP32 on CDNA4), which keeps
+// small net forces accurate when many pair terms cancel.
 // ---------------------------------------------------------------------------
+template <int SMODE, int EMODE, int HYDRO>
+__device__ inline void interact(bool on, float4 pi, float4 ri, uint32_t fli, float4 pj, float4 rj,
+                                const PairArgs &a, float cs, double &fx, double &fy, double &fz, double &es,
+                                double &ee, double &eh)
+{
+    if (!on)
+        return;
+    const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    if (!in_cutoff(pi, pj, a.rc, a.rc2lo, a.rc2hi, r2))
+        return;
+    const uint32_t flj = fbits(rj.w);
+    const float wgt = (flj & PF_STATIC) ? 1.0f : 0.5f;
+    float fscal = 0.0f;   // force on i = (dx, dy, dz) * (fscal + fscd)
+    double fscd = 0.0;
+    if (r2 > 0.0f)
+    {
+        float inv = __builtin_amdgcn_rsqf(r2);
+        inv = inv * (1.5f - 0.5f * r2 * inv * inv);
+        const float inv2 = inv * inv;
+        if (SMODE >= 0)
+        {
+            float en = 0.0f, f = 0.0f;
+            if (SMODE == 0)
+            { // linear: ri + rj from raw radii
+                const float d = r2 * inv;
+                const float dv = d - (ri.x + rj.x);
+                if (!(dv > 0.0f))
+                {
+                    en = 0.5f * dv * dv;
+                    f = dv * cs * inv; // -|dv| * C / d
+                }
+            }
+            else if (r2 >= 0.01f)
+            {
+                const float eij = ri.y * rj.y;
+                const float R = ri.x * rj.x;
+                const float s2 = R * R * inv2, s6 = s2 * s2 * s2;
+                if (SMODE == 1)
+                { // Amber 12-6
+                    const float s12 = s6 * s6;
+                    en = eij * (s12 - 2.0f * s6);
+                    f = 12.0f * eij * (s6 - s12) * inv2 * cs;
+                }
+                else if (SMODE == 2)
+                { // Lewitt 8-6
+                    const float s8 = s6 * s2;
+                    en = eij * (3.0f * s8 - 4.0f * s6);
+                    f = 24.0f * eij * (s6 - s8) * inv2 * cs;
+                }
+                else
+                { // Zacharias 8-6: raw r, eps, product rules
+                    const float s8 = s6 * s2;
+                    en = eij * (s8 - s6);
+                    f = eij * (6.0f * s6 - 8.0f * s8) * inv2 * cs;
+                }
+            }
+            es += (double)(wgt * a.steric_scale * en);
+            fscal += a.steric_scale * f;
+        }
+        if (EMODE > 0 && (fli & PF_CHARGED) && (flj & PF_CHARGED) && r2 >= 1.0e-6f)
+        {
+            // Coulomb dominates the non-bonded force: its scalar is formed in
+            // FP64 from 1/d refined by one FP64 Newton step of the FP32 seed
+            const double xd = dx, yd = dy, zd = dz;
+            const double r2d = fma(xd, xd, fma(yd, yd, zd * zd));
+            const double y = inv;
+            const double invd = y * fma(-0.5 * r2d, y * y, 1.5);
+            const double qq = (double)pi.w * (double)pj.w;
+            if (EMODE == 1)
+            {
+                ee += (double)wgt * a.ke * qq * invd;
+                fscd -= a.kf * qq * invd * invd * invd;
+            }
+            else
+            { // eps(r) = eps * r
+                const double i2 = invd * invd;
+                ee += (double)wgt * a.ke * qq * i2;
+                fscd -= 2.0 * a.kf * qq * i2 * i2;
+            }
+        }
+        if (HYDRO && (fli & PF_HYDRO) && (flj & PF_HYDRO))
+        {
+            const float d = r2 * inv;
+            const float hh = ri.z * rj.z * __expf(-d);
+            eh += (double)(wgt * a.hydro_scale) * (-(double)hh * 6.02214169E23 * 1.0E-3);
+            fscal += a.hydro_scale * hh * cs * inv;
+        }
+    }
+    else if (SMODE == 0)
+    {
+        const float s = ri.x + rj.x; // coincident pair: linear energy, zero force
+        es += (double)(wgt * a.steric_scale * 0.5f * s * s);
+    }
+    const double fs = (double)fscal + fscd;
+    fx = fma((double)dx, fs, fx);
+    fy = fma((double)dy, fs, fy);
+    fz = fma((double)dz, fs, fz);
+}
+
+// Pair forces: two list words per lane per iteration, loads issued without
+// branches (the lists are zero-padded, a zero word is a masked-out entry).
 template <int SMODE, int EMODE, int HYDRO>
 __global__ void __launch_bounds__(BLOCK) k_pair(PairArgs a)
 {
@@ -159,110 +271,28 @@ __global__ void __launch_bounds__(BLOCK) k_pair(PairArgs a)
     const int32_t il = lane & 7, jl = lane >> 3;
     const int32_t i = ci * CLUSTER + il;
     double es = 0.0, ee = 0.0, eh = 0.0;
-    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
     const float cs = (float)a.k.c_spring;
-    // Coulomb prefactors: energy kJ/mol for q1 q2 / (eps d), force module
-    // q1 q2 / (4 pi eps d^2) * C_elec (electrostatic.hpp:16-49).
-    const float ke = (float)(a.k.e_coul / a.diel) * a.elec_scale;
-    const float kf = (float)(a.k.c_elec / (4.0 * a.k.pi * a.diel)) * a.elec_scale;
-    if (ci < a.ncl && i < a.n)
+    if (ci < a.ncl)
     {
-        const float4 pi = a.pos[i];
-        const float4 ri = a.jprm[i];
+        const bool iv = i < a.n;
+        const float4 pi = iv ? a.pos[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const float4 ri = iv ? a.jprm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
         const uint32_t fli = fbits(ri.w);
-        const int32_t cnt = a.cln[ci];
+        const uint32_t ibit = 1u << (24 + il);
+        const int32_t cnt16 = (a.cln[ci] + 15) & ~15;
         const uint32_t *L = a.cl + (size_t)ci * a.cap;
-        for (int32_t t = 0; t < cnt; t += 8)
+        for (int32_t t = 0; t < cnt16; t += 16)
         {
-            const int32_t e = t + jl;
-            const uint32_t w = e < cnt ? L[e] : 0u;
-            if (!((w >> (24 + il)) & 1u))
-                continue;
-            const int32_t j = (int32_t)(w & CL_JMASK);
-            const float4 pj = a.pos[j];
-            const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
-            const float r2 = dx * dx + dy * dy + dz * dz;
-            if (!in_cutoff(pi, pj, a.rc, a.rc2lo, a.rc2hi, r2))
-                continue;
-            const float4 rj = a.jprm[j];
-            const uint32_t flj = fbits(rj.w);
-            const float wgt = (flj & PF_STATIC) ? 1.0f : 0.5f;
-            float fscal = 0.0f; // force on i = (dx, dy, dz) * fscal
-            if (r2 > 0.0f)
-            {
-                float inv = rsqrtf(r2);
-                inv = inv * (1.5f - 0.5f * r2 * inv * inv);
-                const float inv2 = inv * inv;
-                if (SMODE >= 0)
-                {
-                    float en = 0.0f, f = 0.0f;
-                    if (SMODE == 0)
-                    { // linear: ri + rj from raw radii
-                        const float d = r2 * inv;
-                        const float dv = d - (ri.x + rj.x);
-                        if (!(dv > 0.0f))
-                        {
-                            en = 0.5f * dv * dv;
-                            f = dv * cs * inv; // -|dv| * C / d
-                        }
-                    }
-                    else if (r2 >= 0.01f)
-                    {
-                        const float eij = ri.y * rj.y;
-                        const float R = ri.x * rj.x;
-                        const float s2 = R * R * inv2, s6 = s2 * s2 * s2;
-                        if (SMODE == 1)
-                        { // Amber 12-6
-                            const float s12 = s6 * s6;
-                            en = eij * (s12 - 2.0f * s6);
-                            f = 12.0f * eij * (s6 - s12) * inv2 * cs;
-                        }
-                        else if (SMODE == 2)
-                        { // Lewitt 8-6
-                            const float s8 = s6 * s2;
-                            en = eij * (3.0f * s8 - 4.0f * s6);
-                            f = 24.0f * eij * (s6 - s8) * inv2 * cs;
-                        }
-                        else
-                        { // Zacharias 8-6: raw r, eps, product rules
-                            const float s8 = s6 * s2;
-                            en = eij * (s8 - s6);
-                            f = eij * (6.0f * s6 - 8.0f * s8) * inv2 * cs;
-                        }
-                    }
-                    es += (double)(wgt * a.steric_scale * en);
-                    fscal += a.steric_scale * f;
-                }
-                if (EMODE > 0 && (fli & PF_CHARGED) && (flj & PF_CHARGED) && r2 >= 1.0e-6f)
-                {
-                    const float qq = pi.w * pj.w;
-                    if (EMODE == 1)
-                    {
-                        ee += (double)(wgt * ke * qq * inv);
-                        fscal -= kf * qq * inv * inv2;
-                    }
-                    else
-                    { // eps(r) = eps * r
-                        ee += (double)(wgt * ke * qq * inv2);
-                        fscal -= 2.0f * kf * qq * inv2 * inv2;
-                    }
-                }
-                if (HYDRO && (fli & PF_HYDRO) && (flj & PF_HYDRO))
-                {
-                    const float d = r2 * inv;
-                    const float hh = ri.z * rj.z * __expf(-d);
-                    eh += (double)(wgt * a.hydro_scale) * (-(double)hh * 6.02214169E23 * 1.0E-3);
-                    fscal += a.hydro_scale * hh * cs * inv;
-                }
-            }
-            else if (SMODE == 0)
-            {
-                const float s = ri.x + rj.x; // coincident pair: linear energy, zero force
-                es += (double)(wgt * a.steric_scale * 0.5f * s * s);
-            }
-            fx += dx * fscal;
-            fy += dy * fscal;
-            fz += dz * fscal;
+            const uint32_t w0 = L[t + jl];
+            const uint32_t w1 = L[t + 8 + jl];
+            const int32_t j0 = (int32_t)(w0 & CL_JMASK), j1 = (int32_t)(w1 & CL_JMASK);
+            const float4 pj0 = a.pos[j0];
+            const float4 pj1 = a.pos[j1];
+            const float4 rj0 = a.jprm[j0];
+            const float4 rj1 = a.jprm[j1];
+            interact<SMODE, EMODE, HYDRO>((w0 & ibit) != 0u, pi, ri, fli, pj0, rj0, a, cs, fx, fy, fz, es, ee, eh);
+            interact<SMODE, EMODE, HYDRO>((w1 & ibit) != 0u, pi, ri, fli, pj1, rj1, a, cs, fx, fy, fz, es, ee, eh);
         }
     }
     // sum over the 8 entry lanes of each i (lanes il, il+8, ..., il+56)
@@ -283,10 +313,10 @@ __global__ void __launch_bounds__(BLOCK) k_pair(PairArgs a)
     }
     if (ci < a.ncl && jl == 0 && i < a.n)
     {
-        float4 f = make_float4(fx, fy, fz, 0.0f);
+        double4 f = make_double4(fx, fy, fz, 0.0);
         if (a.accumulate)
         {
-            const float4 o = a.fnb[i];
+            const double4 o = a.fnb[i];
             f.x += o.x;
             f.y += o.y;
             f.z += o.z;
@@ -296,7 +326,8 @@ __global__ void __launch_bounds__(BLOCK) k_pair(PairArgs a)
 }
 
 // Processed pairs of one search kind, as (min, max) original ids
-// (Particle.cpp:205-224: i dynamic, j static or later in id order).
+// (Particle.cpp:205-224: i dynamic, j static or later in id order).  One
+// counter update per wave keeps the atomic traffic low.
 __global__ void __launch_bounds__(BLOCK) k_pairs_extract(const float4 *pos, const float4 *jprm, const uint32_t *cl,
                                                          const int32_t *cln, const int32_t *perm, int32_t n,
                                                          int32_t ncl, int32_t lcap, float rc, float rc2lo,
@@ -305,39 +336,56 @@ __global__ void __launch_bounds__(BLOCK) k_pairs_extract(const float4 *pos, cons
 {
     const int lane = threadIdx.x & 63;
     const int32_t ci = blockIdx.x * WAVES + (threadIdx.x >> 6);
+    if (ci >= ncl)
+        return; // whole wave
     const int32_t il = lane & 7, jl = lane >> 3;
     const int32_t i = ci * CLUSTER + il;
-    if (ci >= ncl || i >= n)
-        return;
-    const uint32_t fi = fbits(jprm[i].w);
-    if (i_mask != 0u && !(fi & i_mask))
-        return;
-    const float4 pi = pos[i];
+    bool iok = i < n;
+    const uint32_t fi = iok ? fbits(jprm[i].w) : 0u;
+    iok = iok && (i_mask == 0u || (fi & i_mask));
+    const float4 pi = iok ? pos[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const int32_t pi_id = iok ? perm[i] : 0;
     const int32_t cnt = cln[ci];
-    for (int32_t t = jl; t < cnt; t += 8)
+    for (int32_t t0 = 0; t0 < cnt; t0 += 8)
     {
-        const uint32_t w = cl[(size_t)ci * lcap + t];
-        if (!((w >> (24 + il)) & 1u))
-            continue;
-        const int32_t j = (int32_t)(w & CL_JMASK);
-        const uint32_t fj = fbits(jprm[j].w);
-        if (j_mask != 0u && !(fj & j_mask))
-            continue;
-        if (!(fj & PF_STATIC) && perm[j] < perm[i])
-            continue;
-        const float4 pj = pos[j];
-        const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        if (!in_cutoff(pi, pj, rc, rc2lo, rc2hi, r2))
-            continue;
-        if (nonzero && !(dx != 0.0f || dy != 0.0f || dz != 0.0f))
-            continue;
-        const int32_t slot = atomicAdd(count, 1);
-        if (slot < cap)
+        const int32_t t = t0 + jl;
+        bool keep = false;
+        uint32_t u = 0u, v = 0u;
+        if (iok && t < cnt)
         {
-            const uint32_t u = (uint32_t)perm[i], v = (uint32_t)perm[j];
-            out[2 * (size_t)slot] = u < v ? u : v;
-            out[2 * (size_t)slot + 1] = u < v ? v : u;
+            const uint32_t w = cl[(size_t)ci * lcap + t];
+            if ((w >> (24 + il)) & 1u)
+            {
+                const int32_t j = (int32_t)(w & CL_JMASK);
+                const uint32_t fj = fbits(jprm[j].w);
+                const int32_t pj_id = perm[j];
+                if ((j_mask == 0u || (fj & j_mask)) && ((fj & PF_STATIC) || pi_id < pj_id))
+                {
+                    const float4 pj = pos[j];
+                    const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+                    const float r2 = dx * dx + dy * dy + dz * dz;
+                    keep = in_cutoff(pi, pj, rc, rc2lo, rc2hi, r2) &&
+                           !(nonzero && !(dx != 0.0f || dy != 0.0f || dz != 0.0f));
+                    u = (uint32_t)(pi_id < pj_id ? pi_id : pj_id);
+                    v = (uint32_t)(pi_id < pj_id ? pj_id : pi_id);
+                }
+            }
+        }
+        const uint64_t bal = __ballot(keep);
+        if (bal == 0ull)
+            continue;
+        int32_t base = 0;
+        if (lane == 0)
+            base = atomicAdd(count, (int32_t)__popcll(bal));
+        base = __shfl(base, 0, 64);
+        if (keep)
+        {
+            const int64_t slot = (int64_t)base + __popcll(bal & ((1ull << lane) - 1ull));
+            if (slot < cap)
+            {
+                out[2 * slot] = u;
+                out[2 * slot + 1] = v;
+            }
         }
     }
 }

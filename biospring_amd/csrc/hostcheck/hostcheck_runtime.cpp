@@ -404,6 +404,10 @@ void launch_list_build(const ListArgs &a, hipStream_t)
                     }
                 }
         }
+        if (a.cap % 16)
+            hc_fail("list: capacity not a multiple of 16");
+        for (int32_t slot = count; slot < std::min((count + 15) & ~15, a.cap); ++slot)
+            a.cl[(size_t)ci * a.cap + slot] = 0u;
         a.cln[ci] = std::min(count, a.cap);
         if (count > a.cap)
             a.status[2] |= 1;
@@ -417,7 +421,7 @@ void launch_pair(const PairArgs &a, int32_t, int32_t, int32_t nb, hipStream_t)
     need(a.jprm, sizeof(float4) * a.n, "pair: jprm");
     need(a.cln, sizeof(int32_t) * a.ncl, "pair: cln");
     need(a.cl, sizeof(uint32_t) * (size_t)a.ncl * a.cap, "pair: cl");
-    need(a.fnb, sizeof(float4) * a.n, "pair: fnb");
+    need(a.fnb, sizeof(double4) * a.n, "pair: fnb");
     need(a.epart, sizeof(double) * 3 * nb, "pair: epart");
     if ((int64_t)nb * (BLOCK / 64) < a.ncl)
         hc_fail("pair: grid too small");
@@ -425,6 +429,12 @@ void launch_pair(const PairArgs &a, int32_t, int32_t, int32_t nb, hipStream_t)
     {
         if (a.cln[ci] < 0 || a.cln[ci] > a.cap)
             hc_fail("pair: list length out of range");
+        const int32_t cnt16 = (a.cln[ci] + 15) & ~15;
+        if (cnt16 > a.cap)
+            hc_fail("pair: padded list length exceeds the capacity");
+        for (int32_t t = a.cln[ci]; t < cnt16; ++t)
+            if (a.cl[(size_t)ci * a.cap + t] != 0u)
+                hc_fail("pair: list padding is not zero");
         for (int32_t t = 0; t < a.cln[ci]; ++t)
         {
             const uint32_t w = a.cl[(size_t)ci * a.cap + t];
@@ -437,7 +447,7 @@ void launch_pair(const PairArgs &a, int32_t, int32_t, int32_t nb, hipStream_t)
     }
     for (int32_t i = 0; i < a.n; ++i)
         if (!a.accumulate)
-            a.fnb[i] = make_float4(0, 0, 0, 0);
+            a.fnb[i] = make_double4(0, 0, 0, 0);
     std::memset(a.epart, 0, sizeof(double) * 3 * nb);
 }
 
@@ -454,7 +464,7 @@ void launch_spring(const SpringArgs &a, int32_t group, int32_t nb, hipStream_t)
     need(a.status, sizeof(int32_t) * 4, "spring: status");
     need(a.epart, sizeof(double) * 3 * nb, "spring: epart");
     if (a.fnb)
-        need(a.fnb, sizeof(float4) * n, "spring: fnb");
+        need(a.fnb, sizeof(double4) * n, "spring: fnb");
     if (a.fext)
         need(a.fext, sizeof(float4) * n, "spring: fext");
     if (a.plo)
