@@ -8,3 +8,7 @@ if ROOT not in sys.path:
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) device")
+    # the oracle is the checker of every test: a plain gcc build of one C file
+    if not os.path.exists(os.path.join(ROOT, "oracle", "liboracle.so")):
+        import subprocess
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)

@@ -1,0 +1,191 @@
+#!/usr/bin/env python3
+"""Writes the golden fixtures of tests/golden/ from the reference itself.
+
+The reference's own SpringNetwork is compiled from its sources into
+oracle/_ref/libbsref.so (`make -C oracle ref`); this script drives it on small
+seeded systems and stores inputs and outputs as compressed .npz files:
+
+  forces/energies of one SpringNetwork::computeForces (SpringNetwork.cpp:244-249),
+  processed pair sets recorded from Particle::add*Force (Particle.cpp:191-429),
+  nsearch cell lists (nsearch.hpp:342-390),
+  positions/velocities/energies after computeStep (SpringNetwork.cpp:251-268).
+
+The reference runs with one OpenMP thread so that its kinetic-energy reduction
+order is fixed.  Run from the repository root:  python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from biospring_amd import _abi, workloads  # noqa: E402
+from oracle import pyoracle as po  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+SYS_KEYS = ("x", "v", "mass", "charge", "radius", "eps", "sasa", "transfer", "hydro", "flags",
+            "id1", "id2", "l0", "k")
+
+
+def params_array(p):
+    return np.array([getattr(p, f) for f, _ in _abi.Params._fields_], dtype=np.float64)
+
+
+def store(name, system, params, **outputs):
+    arrays = {f"in_{k}": getattr(system, k) for k in SYS_KEYS}
+    arrays["params"] = params_array(params)
+    arrays.update(outputs)
+    path = os.path.join(OUT, f"{name}.npz")
+    np.savez_compressed(path, **arrays)
+    print(f"{name}: n={system.n} m={system.m} -> {os.path.getsize(path) / 1024:.0f} KiB")
+
+
+def ref_case(name, system, params, kinds=(0, 1), steps=(1, 5), cells=True, variants=()):
+    """Reference outputs for `params`, plus for each (tag, params) variant the
+    forces, energies and pair sets under keys prefixed by the tag."""
+    out = {}
+    for tag, vp in variants:
+        ref = po.Reference(system, vp)
+        F, E = ref.compute_forces()
+        ref.close()
+        out[f"{tag}__forces"], out[f"{tag}__energies"] = F, E
+        out[f"{tag}__params"] = params_array(vp)
+        for kind in (0, 1):
+            if (vp.steric_enable, vp.electrostatic_enable)[kind]:
+                out[f"{tag}__pairs_{kind}"] = po.ref_pairs(system, kind, vp)
+    ref = po.Reference(system, params)
+    F, E = ref.compute_forces()
+    out["forces"], out["energies"] = F, E
+    ref.close()
+    for kind in kinds:
+        out[f"pairs_{kind}"] = po.ref_pairs(system, kind, params)
+        if cells:
+            ref = po.Reference(system, params)
+            cutoff = (params.steric_cutoff, params.electrostatic_cutoff, params.hydrophobicity_cutoff)[kind]
+            c = ref.cells(kind, cutoff, params.neighborskin)
+            for key in ("cell_of", "start", "members", "dims", "box"):
+                out[f"cells_{kind}_{key}"] = c[key]
+            ref.close()
+    ref = po.Reference(system, params)
+    done = 0
+    for s in steps:
+        E = ref.step(s - done)
+        done = s
+        x, v = ref.state()
+        out[f"x_{s}"], out[f"v_{s}"], out[f"E_{s}"] = x, v, E
+    ref.close()
+    out["steps"] = np.array(steps, np.int64)
+    store(name, system, params, **out)
+
+
+def subset(system, n):
+    """First n beads of a generated system, springs among them kept."""
+    s = system
+    keep = (s.id1 < n) & (s.id2 < n)
+    cut = workloads.System(x=s.x[:3 * n].copy(), v=s.v[:3 * n].copy(), mass=s.mass[:n].copy(),
+                           charge=s.charge[:n].copy(), radius=s.radius[:n].copy(), eps=s.eps[:n].copy(),
+                           sasa=s.sasa[:n].copy(), transfer=s.transfer[:n].copy(), hydro=s.hydro[:n].copy(),
+                           flags=s.flags[:n].copy(), id1=s.id1[keep].copy(), id2=s.id2[keep].copy(),
+                           l0=s.l0[keep].copy(), k=s.k[keep].copy())
+    cut.params = s.params.copy()
+    return cut
+
+
+def main():
+    po.ref_lib().ref_set_threads(1)
+
+    # CA elastic network (workload 0 rules, 600 beads): springs, linear steric
+    # (identically zero: all pairs < 4 A are bonded), Coulomb, viscosity.
+    s = workloads.generate(0, seed=11, n=600)
+    ref_case("enm600", s, s.params, steps=(1, 5, 20))
+
+    # Amber 12-6 + Coulomb (constant dielectric: the reference has no eps(r)).
+    # Variants on the same system: every steric mode (Coulomb off), and no
+    # springs with shorter cutoffs (pairs under 8 A then interact).
+    s = workloads.generate(1, seed=12, n=800)
+    p = s.params.copy()
+    p.dielectric_model = 0
+    variants = []
+    for mode, tag in ((0, "linear"), (1, "amber"), (2, "lewitt"), (3, "zacharias")):
+        q = p.copy()
+        q.steric_mode = mode
+        q.electrostatic_enable = 0
+        variants.append((tag, q))
+    q = p.copy()
+    q.spring_enable = 0
+    q.steric_cutoff = 6.0
+    q.electrostatic_cutoff = 7.0
+    variants.append(("nosprings", q))
+    ref_case("lj800", s, p, steps=(1, 5), variants=variants)
+
+    # 25% static beads
+    rng = np.random.default_rng(13)
+    t = workloads.generate(1, seed=13, n=800)
+    t.flags = (rng.random(t.n) < 0.25).astype(np.uint8)
+    pt = t.params.copy()
+    pt.dielectric_model = 0
+    ref_case("static800", t, pt, steps=(1, 5))
+
+    # pseudo-hydrophobic pair term (Particle.cpp:269-330)
+    h = workloads.generate(0, seed=14, n=600)
+    hv = rng.uniform(-1.0, 1.0, h.n).astype(np.float32)
+    hv[rng.random(h.n) < 0.5] = 0.0
+    h.hydro = hv
+    ph = h.params.copy()
+    ph.hydrophobicity_enable = 1
+    ph.hydrophobicity_cutoff = 8.0
+    ref_case("hydro600", h, ph, kinds=(0, 1, 2), steps=(1, 3))
+
+    # IMPALA membranes (imp.hpp): flat slab, double flat, curved double
+    m = workloads.generate(0, seed=15, n=300)
+    m.x = (m.x.reshape(-1, 3) * np.array([1.0, 1.0, 1.4], np.float32)).reshape(-1)
+    m.sasa = rng.uniform(10.0, 100.0, m.n).astype(np.float32)
+    m.transfer = rng.uniform(-1.0, 1.0, m.n).astype(np.float32)
+    for tag, (uo, lo, uc, lc) in (("flat", (0, 0, 0, 0)), ("double", (30.0, 30.0, 0, 0)),
+                                  ("curved", (30.0, 30.0, 1.0 / 200.0, -1.0 / 200.0))):
+        pm = m.params.copy()
+        pm.imp_enable = 1
+        pm.imp_upper_offset, pm.imp_lower_offset = uo, lo
+        pm.imp_upper_curvature, pm.imp_lower_curvature = uc, lc
+        ref_case(f"imp_{tag}", m, pm, kinds=(), steps=(1, 5), cells=False)
+
+    # two-particle scans of the reference regression tests
+    # (test-spring-energy.cpp:82-99, test-steric-energy.cpp:108-160,
+    # test-electrostatic-energy.cpp:78-135): energy of one pair vs distance
+    xs = np.arange(0.01, 5.0, 0.01, dtype=np.float32)
+    scans = {}
+    for tag, setup in (("spring", dict(spring_enable=1)),
+                       ("linear", dict(steric_enable=1, steric_mode=0, steric_cutoff=16.0)),
+                       ("amber", dict(steric_enable=1, steric_mode=1, steric_cutoff=16.0)),
+                       ("coulomb", dict(electrostatic_enable=1, electrostatic_cutoff=16.0))):
+        es, fs = [], []
+        for d in xs:
+            two = workloads.System(
+                x=np.array([0, 0, 0, d, 0, 0], np.float32), v=np.zeros(6, np.float32),
+                mass=np.full(2, 12.01, np.float32), charge=np.array([0.5973, -0.5679], np.float32),
+                radius=np.full(2, 1.908, np.float32), eps=np.full(2, 0.086, np.float32),
+                sasa=np.zeros(2, np.float32), transfer=np.zeros(2, np.float32), hydro=np.zeros(2, np.float32),
+                flags=np.zeros(2, np.uint8), id1=np.array([0], np.uint32), id2=np.array([1], np.uint32),
+                l0=np.array([0.0], np.float32), k=np.array([1.0], np.float32))
+            pp = workloads.params(99)  # reference defaults, no workload terms
+            pp.spring_enable = 0
+            pp.viscosity_enable = 0
+            for k2, v2 in setup.items():
+                setattr(pp, k2, v2)
+            two.params = pp
+            ref = po.Reference(two, pp)
+            F, E = ref.compute_forces()
+            ref.close()
+            es.append(E)
+            fs.append(F[0])
+        scans[f"{tag}_E"] = np.array(es)
+        scans[f"{tag}_F0"] = np.array(fs)
+    np.savez_compressed(os.path.join(OUT, "scans.npz"), d=xs, **scans)
+    print("scans written")
+
+
+if __name__ == "__main__":
+    main()

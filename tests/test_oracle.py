@@ -1,0 +1,244 @@
+"""Pins the oracle (oracle/bs_oracle.c) before it is trusted as the checker.
+
+* Float-faithful mode against golden vectors written by the reference itself
+  (tests/golden/make_golden.py): forces, energies, processed pair sets, cell
+  lists and multi-step positions/velocities, bit for bit.
+* Pair sets through the cell list against the O(N^2) brute force
+  (NeighborSearchO2, nsearch.hpp:74-102).
+* FP64 mode: central finite differences of its own total energy
+  (F = -dE/dx * C for springs, steric, Coulomb and IMPALA), closed forms of
+  the reference's regression tests, and agreement with the float path.
+"""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from biospring_amd import _abi, workloads
+from oracle import pyoracle as po
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+KEYS = ("x", "v", "mass", "charge", "radius", "eps", "sasa", "transfer", "hydro", "flags", "id1", "id2", "l0", "k")
+
+
+def params_from(arr):
+    p = _abi.Params()
+    for (name, ctype), v in zip(_abi.Params._fields_, arr):
+        setattr(p, name, int(v) if ctype is not _abi.ctypes.c_float else float(v))
+    return p
+
+
+def load(name):
+    z = np.load(os.path.join(GOLDEN, f"{name}.npz"))
+    s = workloads.System(**{k: z[f"in_{k}"] for k in KEYS})
+    s.params = params_from(z["params"])
+    return s, z
+
+
+CASES = sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.npz")) if "scans" not in p)
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_forces_energies_bitexact(name):
+    s, z = load(name)
+    o = po.Oracle(s)
+    F, E = o.compute_forces(po.F32)
+    dyn = (s.flags & 1) == 0
+    assert np.array_equal(F[dyn].astype(np.float32), z["forces"][dyn]), name
+    assert np.array_equal(E.astype(np.float32), z["energies"]), (E, z["energies"])
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_pairs_and_cells_bitexact(name):
+    s, z = load(name)
+    o = po.Oracle(s)
+    for kind in (0, 1, 2):
+        if f"pairs_{kind}" not in z:
+            continue
+        assert np.array_equal(o.pairs(kind, method=0), z[f"pairs_{kind}"]), (name, kind)
+        if f"cells_{kind}_cell_of" in z:
+            c = o.cells(kind)
+            for key in ("cell_of", "start", "members", "dims", "box"):
+                assert np.array_equal(np.asarray(c[key]), z[f"cells_{kind}_{key}"]), (name, kind, key)
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_trajectory_bitexact(name):
+    s, z = load(name)
+    o = po.Oracle(s)
+    done = 0
+    for st in z["steps"]:
+        rc, E = o.step(po.F32, int(st) - done)
+        assert rc == 0
+        done = int(st)
+        x, v = o.state(po.F32)
+        assert np.array_equal(x.astype(np.float32), z[f"x_{st}"]), (name, st)
+        assert np.array_equal(v.astype(np.float32), z[f"v_{st}"]), (name, st)
+        assert np.array_equal(E.astype(np.float32), z[f"E_{st}"]), (name, st, E, z[f"E_{st}"])
+
+
+def test_variants_bitexact():
+    s, z = load("lj800")
+    tags = sorted({k.split("__")[0] for k in z.files if "__" in k})
+    assert tags
+    for tag in tags:
+        p = params_from(z[f"{tag}__params"])
+        o = po.Oracle(s, params=p)
+        F, E = o.compute_forces(po.F32)
+        assert np.array_equal(F.astype(np.float32), z[f"{tag}__forces"]), tag
+        assert np.array_equal(E.astype(np.float32), z[f"{tag}__energies"]), tag
+        for kind in (0, 1):
+            if f"{tag}__pairs_{kind}" in z:
+                assert np.array_equal(o.pairs(kind), z[f"{tag}__pairs_{kind}"]), (tag, kind)
+
+
+@pytest.mark.parametrize("cfg,n,seed", [(0, 700, 3), (1, 1200, 4)])
+def test_pairs_cells_vs_bruteforce(cfg, n, seed):
+    s = workloads.generate(cfg, seed=seed, n=n)
+    s.flags = (np.random.default_rng(seed).random(s.n) < 0.2).astype(np.uint8)
+    o = po.Oracle(s)
+    for kind in (0, 1):
+        assert np.array_equal(o.pairs(kind, method=0), o.pairs(kind, method=1)), kind
+
+
+def test_scans_match_reference_and_closed_forms():
+    """Two-particle scans of the reference regression tests."""
+    z = np.load(os.path.join(GOLDEN, "scans.npz"))
+    d = z["d"].astype(np.float64)
+    # spring: E = 1/2 k d^2 with k = 1, L0 = 0 (test-spring-energy.cpp:82-99, 4 ULP)
+    assert np.allclose(z["spring_E"][:, 0], 0.5 * d.astype(np.float32) ** 2, rtol=4 * 2 ** -23)
+    # linear steric: E = 1/2 (d - 2r)^2 for overlap (test-steric-energy.cpp:108-127)
+    r = 1.908
+    lin = np.where(d - 2 * r > 0, 0.0, 0.5 * (d - 2 * r) ** 2)
+    assert np.allclose(z["linear_E"][:, 2], lin, rtol=1e-6, atol=1e-6)
+    # Coulomb closed form: k q1 q2 / d N_A 1e-3 (test-electrostatic-energy.cpp:78-135)
+    kc = 8.9875517873681764e9
+    e = 1.60217653e-19
+    coul = kc * (0.5973 * e) * (-0.5679 * e) / (d * 1e-10) * 6.02214169e23 * 1e-3
+    assert np.allclose(z["coulomb_E"][:, 1], coul, rtol=1e-5)
+    # Amber: eps[(R/d)^12 - 2 (R/d)^6] with R = (r_i + r_j)/2 = r, zero below
+    # the 0.1 A floor (steric.hpp:62, constants.hpp:86), with the reference
+    # test's absolute bands (test-steric-energy.cpp:129-160, energies < 1e15);
+    # a float (R/d)^12 carries ~12 rounding errors of R/d, so 1e-6 relative
+    # is also accepted against this FP64 closed form.
+    amb = np.where(d < 0.1, 0.0, 0.086 * ((r / d) ** 12 - 2 * (r / d) ** 6))
+    got = z["amber_E"][:, 2].astype(np.float64)
+    band = np.where(got < 1e4, 1e-3, np.where(got < 1e5, 1e-2, np.where(got < 1e8, 1e1, 1e7)))
+    err = np.abs(got - amb)
+    ok = (err <= band) | (err <= 1e-6 * np.abs(amb))
+    assert ok[got < 1e15].all()
+    assert (got < 1e15).sum() > 400
+    # the oracle reproduces every scan point exactly (float path)
+    for tag, slot in (("spring", 0), ("linear", 2), ("amber", 2), ("coulomb", 1)):
+        for t in range(0, len(d), 7):
+            two = _two_particle(float(z["d"][t]))
+            p = workloads.params(99)
+            p.spring_enable = 0
+            p.viscosity_enable = 0
+            if tag == "spring":
+                p.spring_enable = 1
+            elif tag == "coulomb":
+                p.electrostatic_enable = 1
+                p.electrostatic_cutoff = 16.0
+            else:
+                p.steric_enable = 1
+                p.steric_mode = 0 if tag == "linear" else 1
+                p.steric_cutoff = 16.0
+            o = po.Oracle(two, params=p)
+            F, E = o.compute_forces(po.F32)
+            assert np.float32(E[slot]) == z[f"{tag}_E"][t, slot], (tag, t)
+            assert np.array_equal(F[0].astype(np.float32), z[f"{tag}_F0"][t]), (tag, t)
+
+
+def _two_particle(dist):
+    return workloads.System(
+        x=np.array([0, 0, 0, dist, 0, 0], np.float32), v=np.zeros(6, np.float32),
+        mass=np.full(2, 12.01, np.float32), charge=np.array([0.5973, -0.5679], np.float32),
+        radius=np.full(2, 1.908, np.float32), eps=np.full(2, 0.086, np.float32),
+        sasa=np.zeros(2, np.float32), transfer=np.zeros(2, np.float32), hydro=np.zeros(2, np.float32),
+        flags=np.zeros(2, np.uint8), id1=np.array([0], np.uint32), id2=np.array([1], np.uint32),
+        l0=np.array([0.0], np.float32), k=np.array([1.0], np.float32))
+
+
+# ---------------------------------------------------------------------------
+# FP64 mode
+# ---------------------------------------------------------------------------
+def _total_energy(o, x):
+    o.set_state_f64(x)
+    _, E = o.compute_forces(po.F64)
+    return E[_abi.E_SPRING] + E[_abi.E_ELECTROSTATIC] + E[_abi.E_STERIC] + E[_abi.E_IMP]
+
+
+@pytest.mark.parametrize("setup", ["enm", "amber", "lewitt", "zacharias", "linear_nosprings", "ddd", "imp_curved"])
+def test_fp64_forces_are_energy_gradients(setup):
+    rng = np.random.default_rng(5)
+    if setup == "enm":
+        s = workloads.generate(0, seed=6, n=300)
+        p = s.params.copy()
+    else:
+        s = workloads.generate(1, seed=7, n=400)
+        p = s.params.copy()
+        p.dielectric_model = 1 if setup == "ddd" else 0
+        if setup in ("amber", "lewitt", "zacharias"):
+            p.steric_mode = {"amber": 1, "lewitt": 2, "zacharias": 3}[setup]
+        if setup == "linear_nosprings":
+            p.steric_mode = 0
+            p.spring_enable = 0
+            p.steric_cutoff = 8.0
+        if setup == "imp_curved":
+            s.sasa = rng.uniform(10, 100, s.n).astype(np.float32)
+            s.transfer = rng.uniform(-1, 1, s.n).astype(np.float32)
+            p.imp_enable = 1
+            p.imp_upper_offset, p.imp_lower_offset = 5.0, 8.0
+            p.imp_upper_curvature, p.imp_lower_curvature = 1 / 50.0, -1 / 60.0
+    p.viscosity_enable = 0
+    # move off the rest lengths so that the springs carry force
+    x0 = (s.x.reshape(-1, 3) + rng.normal(0, 0.2, (s.n, 3))).astype(np.float32).astype(np.float64)
+    o = po.Oracle(s, params=p)
+    o.set_state_f64(x0)
+    F, _ = o.compute_forces(po.F64)
+    c = po.constants()["spring"]
+    h = 1e-5
+    pairs0 = [o.pairs(k, mode=po.F64) for k in (0, 1)]
+    checked = 0
+    for i in rng.choice(s.n, 12, replace=False):
+        for a in range(3):
+            xp, xm = x0.copy(), x0.copy()
+            xp[i, a] += h
+            xm[i, a] -= h
+            o.set_state_f64(xp)
+            same = all(np.array_equal(o.pairs(k, mode=po.F64), pairs0[k]) for k in (0, 1))
+            o.set_state_f64(xm)
+            same = same and all(np.array_equal(o.pairs(k, mode=po.F64), pairs0[k]) for k in (0, 1))
+            if not same:
+                continue  # a pair crosses the hard cutoff: no derivative there
+            g = (_total_energy(o, xp) - _total_energy(o, xm)) / (2 * h)
+            want = -g * c
+            scale = max(abs(F[i, a]), 1e-3 * np.abs(F[i]).max(), 1e-12)
+            assert abs(F[i, a] - want) <= 2e-5 * scale + 1e-9 * np.abs(F).max(), (setup, i, a, F[i, a], want)
+            checked += 1
+    assert checked >= 30, checked
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_fp64_close_to_float_path(name):
+    """FP64 and float-faithful forces agree to float precision: a transposed
+    operand or dropped term in either would show up here."""
+    s, _ = load(name)
+    o = po.Oracle(s)
+    F32, E32 = o.compute_forces(po.F32)
+    F64, E64 = o.compute_forces(po.F64)
+    dyn = (s.flags & 1) == 0
+    glob = np.linalg.norm(F32[dyn] - F64[dyn]) / max(np.linalg.norm(F64[dyn]), 1e-30)
+    assert glob < 1e-4, glob
+    for k in range(6):
+        assert abs(E32[k] - E64[k]) <= 1e-4 * max(abs(E64[k]), 1.0), (k, E32[k], E64[k])
+
+
+def test_constants_match_reference_values():
+    """Derived unit constants (constants.hpp:93-118) as listed for the path."""
+    c = po.constants()
+    assert c["spring"] == pytest.approx(9.999992838e-5, rel=1e-9)
+    assert c["elec"] == pytest.approx(1.745913244, rel=1e-9)
+    assert c["kinetic"] == pytest.approx(1.000000716e4, rel=1e-9)
