@@ -20,6 +20,7 @@ FLAG_STATIC = 1
 E_SPRING, E_ELECTROSTATIC, E_STERIC, E_KINETIC, E_IMP, E_HYDROPHOBIC = range(6)
 NUM_ENERGIES = 6
 NS_STERIC, NS_ELECTROSTATIC, NS_HYDROPHOBIC = range(3)
+GRID_POTENTIAL, GRID_DENSITY = 0, 1
 
 # Exported symbols, one per declaration in include/bs_api.h.
 EXPORTS = (
@@ -62,6 +63,11 @@ class Params(ctypes.Structure):
         ("list_skin", ctypes.c_float),
         ("rebuild_interval", ctypes.c_int32),
         ("compensated_positions", ctypes.c_int32),
+        ("rigidbody_enable", ctypes.c_int32),
+        ("potentialgrid_enable", ctypes.c_int32),
+        ("potentialgrid_scale", ctypes.c_float),
+        ("densitygrid_enable", ctypes.c_int32),
+        ("densitygrid_scale", ctypes.c_float),
     ]
 
     def copy(self):

@@ -33,6 +33,11 @@ static inline void bs_params_fill_defaults(bs_params *p)
     p->list_skin = 1.0f;
     p->rebuild_interval = 0;
     p->compensated_positions = 0;
+    p->rigidbody_enable = 0;      /* rigidbody.enable */
+    p->potentialgrid_enable = 0;  /* GridSetting defaults, Setting.hpp:192-202 */
+    p->potentialgrid_scale = 1.0f;
+    p->densitygrid_enable = 0;
+    p->densitygrid_scale = 1.0f;
 }
 
 #endif

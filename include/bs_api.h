@@ -63,6 +63,10 @@ extern "C" {
 #define BS_NS_ELECTROSTATIC 1  /* charged particles, electrostatic.cutoff */
 #define BS_NS_HYDROPHOBIC   2  /* hydrophobic particles, hydrophobicity.cutoff */
 
+/* ---- grids (SpringNetwork::_grids, SpringNetwork.h:45-49) ---------------- */
+#define BS_GRID_POTENTIAL   0  /* electrostatic potential, potentialgrid.* */
+#define BS_GRID_DENSITY     1  /* density, densitygrid.* */
+
 /*
  * Simulation parameters.  Field names and defaults follow the reference's
  * Configuration (src/configuration/Configuration.hpp:153-209); the msp key
@@ -106,6 +110,16 @@ typedef struct bs_params {
                                      sets are exact for any value >= 0.  Default 1.0 */
     int32_t rebuild_interval;     /* additionally rebuild every k steps (0 = only on drift). Default 0 */
     int32_t compensated_positions; /* keep a float low word per coordinate (two-sum), default 0 */
+
+    /* Reference options read by the step beyond the force-field block above. */
+    int32_t rigidbody_enable;     /* rigidbody.enable: aggregate and solve the bodies of
+                                     bs_add_rigid_body each step (SpringNetwork.cpp:184-192,261-262), default 0 */
+    int32_t potentialgrid_enable; /* potentialgrid.enable: electrostatic field force of grid
+                                     BS_GRID_POTENTIAL (Particle.cpp:161-189); needs electrostatic_enable, default 0 */
+    float potentialgrid_scale;    /* potentialgrid.scale (ForceField::_forcefieldscale), default 1 */
+    int32_t densitygrid_enable;   /* densitygrid.enable: density field force of grid BS_GRID_DENSITY
+                                     (Particle.cpp:136-159), default 0 */
+    float densitygrid_scale;      /* densitygrid.scale, default 1 */
 } bs_params;
 
 typedef struct bs_handle bs_handle;

@@ -44,6 +44,10 @@
 #define ORC_F32 0
 #define ORC_F64 1
 
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
 /* ---------------------------------------------------------------------------
  * Unit constants, evaluated in double in the same order as constants.hpp.
  * ------------------------------------------------------------------------- */
@@ -613,7 +617,44 @@ typedef struct orc {
     float ef[BS_NUM_ENERGIES];
     double ed[BS_NUM_ENERGIES];
     int64_t nonfinite;
+    /* rigid bodies (RigidBody.cpp, RigidBodiesManager.cpp) */
+    int nbodies;
+    struct rbody *bodies;
+    int32_t *body_of;       /* n, -1 for flexible particles */
+    int64_t *local_of;      /* n, index inside the particle's body */
+    /* grids: [BS_GRID_POTENTIAL], [BS_GRID_DENSITY] (PotentialGrid.hpp) */
+    struct ogrid *grid[2];
+    float *burying;         /* ParticleProperty::_burying, default 1 */
 } orc;
+
+/* A dense grid as PotentialGrid holds it after loading: per cell the scalar
+ * and its gradient vector (PotentialGrid::compute_gradient), x-major with z
+ * fastest (DenseGrid::_data[i][j][k]). */
+typedef struct ogrid {
+    double origin[3], cell[3], bmax[3];
+    int64_t shape[3];
+    float *cf;    /* 4 per cell: scalar, vector (float-faithful gradient) */
+    double *cd;   /* 4 per cell: scalar, vector (FP64 gradient) */
+} ogrid;
+
+/* One rigid body, RigidBody.h:156-183.  The F32 state follows the reference's
+ * Vector3f/Quaternion (float) and Matrix/mass/dt (double) types; the F64 state
+ * is the same algorithm in double. */
+typedef struct rbody {
+    int64_t n;
+    int64_t *ids;           /* particle ids in body order (_particulesIds) */
+    double M;               /* _mass, double sum of float masses */
+    double iinv[9];         /* _ibodyinv */
+    float X[3], v[3], a[3], L[3], w[3], q[4], F[3], T[3]; /* q = (x, y, z, w) */
+    float *p0;              /* 3n, _p0 */
+    double Xd[3], vd[3], Ld[3], wd[3], qd[4], Fd[3], Td[3];
+    double *p0d;
+} rbody;
+
+static void potential_field_f(const orc *o, int64_t i, float *fi, float *ei);
+static void potential_field_d(const orc *o, int64_t i, const float *pos, double *fi, double *ei);
+static void density_field_f(const orc *o, int64_t i, float *fi);
+static void density_field_d(const orc *o, int64_t i, const float *pos, double *fi);
 
 typedef struct {
     double box[6];
@@ -673,6 +714,14 @@ orc *orc_create(const bs_params *p, int64_t n, const float *x3, const float *v3,
     o->hydrophobic = (uint8_t *)malloc(n);
     o->dyn = (int64_t *)malloc(sizeof(int64_t) * (n ? n : 1));
     o->ndyn = 0;
+    o->body_of = (int32_t *)malloc(sizeof(int32_t) * (n ? n : 1));
+    o->local_of = (int64_t *)malloc(sizeof(int64_t) * (n ? n : 1));
+    o->burying = (float *)malloc(sizeof(float) * (n ? n : 1));
+    for (int64_t i = 0; i < n; ++i) {
+        o->body_of[i] = -1;
+        o->local_of[i] = -1;
+        o->burying[i] = 1.0f;
+    }
     for (int64_t i = 0; i < n; ++i) {
         for (int c = 0; c < 3; ++c) {
             o->xf[3 * i + c] = x3[3 * i + c];
@@ -771,6 +820,8 @@ orc *orc_create(const bs_params *p, int64_t n, const float *x3, const float *v3,
     return o;
 }
 
+static void free_bodies(orc *o);
+
 void orc_destroy(orc *o)
 {
     if (!o)
@@ -779,6 +830,14 @@ void orc_destroy(orc *o)
     free(o->mass); free(o->charge); free(o->radius); free(o->eps); free(o->sasa); free(o->transfer);
     free(o->hydro); free(o->isstatic); free(o->charged); free(o->hydrophobic); free(o->dyn);
     free(o->bptr); free(o->bidx); free(o->s1); free(o->s2); free(o->sl0); free(o->sk); free(o->dsp);
+    free_bodies(o);
+    free(o->body_of); free(o->local_of); free(o->burying);
+    for (int k = 0; k < 2; ++k)
+        if (o->grid[k]) {
+            free(o->grid[k]->cf);
+            free(o->grid[k]->cd);
+            free(o->grid[k]);
+        }
     free(o);
 }
 
@@ -1246,8 +1305,8 @@ static void forces_f32(orc *o, const float *ext3)
             const uint32_t a = o->s1[s], b = o->s2[s];
             senergy[t] = 0.0f;
             scratch[3 * t] = scratch[3 * t + 1] = scratch[3 * t + 2] = 0.0f;
-            if (o->isstatic[a] && o->isstatic[b])
-                continue;
+            if ((o->body_of[a] >= 0 && o->body_of[b] >= 0) || (o->isstatic[a] && o->isstatic[b]))
+                continue; /* Spring.cpp:24-25 */
             float d[3] = {o->xf[3 * b] - o->xf[3 * a], o->xf[3 * b + 1] - o->xf[3 * a + 1],
                           o->xf[3 * b + 2] - o->xf[3 * a + 2]};
             const float len = norm_f(d[0], d[1], d[2]);
@@ -1307,6 +1366,10 @@ static void forces_f32(orc *o, const float *ext3)
                     pair_eval_f(o, BS_NS_ELECTROSTATIC, i, j, d, fi, &pe[4 * i + 0], &defs[3 * tid + 1]);
             });
         }
+        if (p->electrostatic_enable && p->potentialgrid_enable && o->grid[BS_GRID_POTENTIAL])
+            potential_field_f(o, i, fi, &pe[4 * i + 0]);
+        if (p->densitygrid_enable && o->grid[BS_GRID_DENSITY])
+            density_field_f(o, i, fi);
         if (p->steric_enable && have[BS_NS_STERIC]) {
             const cells_t *cl = &cls[BS_NS_STERIC];
             FOR_EACH_NEIGHBOR(cl, pos, i, j, {
@@ -1446,6 +1509,10 @@ static void forces_f64(orc *o, const double *ext3)
                     pair_eval_d(o, BS_NS_ELECTROSTATIC, i, j, fi, &pe[4 * i + 0], &defs[3 * tid + 1]);
             });
         }
+        if (p->electrostatic_enable && p->potentialgrid_enable && o->grid[BS_GRID_POTENTIAL])
+            potential_field_d(o, i, pos, fi, &pe[4 * i + 0]);
+        if (p->densitygrid_enable && o->grid[BS_GRID_DENSITY])
+            density_field_d(o, i, pos, fi);
         if (p->steric_enable && have[BS_NS_STERIC]) {
             const cells_t *cl = &cls[BS_NS_STERIC];
             FOR_EACH_NEIGHBOR(cl, pos, i, j, {
@@ -1507,6 +1574,599 @@ static void forces_f64(orc *o, const double *ext3)
 }
 
 /* ---------------------------------------------------------------------------
+ * Rigid bodies.
+ *   aggregation  RigidBody::computeParticleForceAndTorque  RigidBody.cpp:567-580
+ *   solve        RigidBody::solve                           RigidBody.cpp:430-484
+ *   placement    RigidBody::integrateParticleVelocity       RigidBody.cpp:600-635
+ *   setup        RigidBody::init / RigidBodiesManager        RigidBody.cpp:57-124, RigidBodiesManager.cpp:11-43
+ *   quaternions  Quaternion.cpp:11-96, Quaternion.h:17-65
+ * The placement uses the particle's index inside its body.  The reference
+ * passes the index in the dynamic-particle list (SpringNetwork.cpp:216-218),
+ * which coincides for the single body SpringNetwork::run creates and is wrong
+ * for several bodies; the oracle follows the intended (local) index.
+ * ------------------------------------------------------------------------- */
+
+/* Quaternion product a*b, Quaternion::operator* (Quaternion.cpp:80-92). */
+static void qmul_f(const float a[4], const float b[4], float o[4])
+{
+    float ax1[3], ax2[3], ax3[3];
+    for (int c = 0; c < 3; ++c) {
+        ax1[c] = b[c] * a[3];
+        ax2[c] = a[c] * b[3];
+    }
+    ax3[0] = a[1] * b[2] - a[2] * b[1];
+    ax3[1] = a[2] * b[0] - a[0] * b[2];
+    ax3[2] = a[0] * b[1] - a[1] * b[0];
+    const float dot = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+    const float w = a[3] * b[3] - dot;
+    for (int c = 0; c < 3; ++c)
+        o[c] = (ax1[c] + ax2[c]) + ax3[c];
+    o[3] = w;
+}
+
+static void qmul_d(const double a[4], const double b[4], double o[4])
+{
+    double t[4];
+    t[0] = b[0] * a[3] + a[0] * b[3] + (a[1] * b[2] - a[2] * b[1]);
+    t[1] = b[1] * a[3] + a[1] * b[3] + (a[2] * b[0] - a[0] * b[2]);
+    t[2] = b[2] * a[3] + a[2] * b[3] + (a[0] * b[1] - a[1] * b[0]);
+    t[3] = a[3] * b[3] - (a[0] * b[0] + a[1] * b[1] + a[2] * b[2]);
+    memcpy(o, t, sizeof t);
+}
+
+/* Quaternion::normalize: only above a magnitude of 0.005. */
+static void qnormalize_f(float q[4])
+{
+    const float mag = sqrtf(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+    if (mag > 0.005)
+        for (int c = 0; c < 4; ++c)
+            q[c] /= mag;
+}
+
+static void qnormalize_d(double q[4])
+{
+    const double mag = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+    if (mag > 0.005)
+        for (int c = 0; c < 4; ++c)
+            q[c] /= mag;
+}
+
+/* Quaternion::inverse: conjugate / |q|^2, with |q| = sqrt(w^2 + x^2 + y^2 + z^2). */
+static void qinverse_f(const float q[4], float o[4])
+{
+    float av = sqrtf(q[3] * q[3] + q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+    av *= av;
+    av = 1 / av;
+    o[3] = q[3] * av;
+    for (int c = 0; c < 3; ++c)
+        o[c] = -q[c] * av;
+}
+
+static void qinverse_d(const double q[4], double o[4])
+{
+    const double av = 1.0 / (q[3] * q[3] + q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+    o[3] = q[3] * av;
+    for (int c = 0; c < 3; ++c)
+        o[c] = -q[c] * av;
+}
+
+/* Quaternion::fromAxisAngle; the axis is used as given (not normalised). */
+static void qaxis_f(const float ax[3], float angle, float q[4])
+{
+    const float s = (float)sin(angle / 2.0);
+    const float c = (float)cos(angle / 2.0);
+    q[0] = ax[0] * s;
+    q[1] = ax[1] * s;
+    q[2] = ax[2] * s;
+    q[3] = c;
+    qnormalize_f(q);
+}
+
+static void qaxis_d(const double ax[3], double angle, double q[4])
+{
+    const double s = sin(angle / 2.0), c = cos(angle / 2.0);
+    q[0] = ax[0] * s;
+    q[1] = ax[1] * s;
+    q[2] = ax[2] * s;
+    q[3] = c;
+    qnormalize_d(q);
+}
+
+/* Springs evaluated each step: not both static (SpringNetwork.cpp:478-483)
+ * and not both rigid (RigidBody::UpdateSpringsState, RigidBody.cpp:42-53);
+ * removal keeps the order of the others (SpringNetwork.h:197). */
+static void rebuild_dynamic_springs(orc *o)
+{
+    o->nds = 0;
+    for (int64_t s = 0; s < o->ns; ++s) {
+        const uint32_t a = o->s1[s], b = o->s2[s];
+        if (o->isstatic[a] && o->isstatic[b])
+            continue;
+        if (o->body_of[a] >= 0 && o->body_of[b] >= 0)
+            continue;
+        o->dsp[o->nds++] = s;
+    }
+}
+
+static void free_bodies(orc *o)
+{
+    for (int b = 0; b < o->nbodies; ++b) {
+        free(o->bodies[b].ids);
+        free(o->bodies[b].p0);
+        free(o->bodies[b].p0d);
+    }
+    free(o->bodies);
+    o->bodies = NULL;
+    o->nbodies = 0;
+    for (int64_t i = 0; i < o->n; ++i) {
+        o->body_of[i] = -1;
+        o->local_of[i] = -1;
+    }
+}
+
+/* RigidBodiesManager::InitRigidBodies: a new body from `ids` unless one of
+ * them already belongs to a body (then nothing happens); an empty list
+ * removes every body.  Returns the number of bodies, or -1 for an id out of
+ * range.  The body's state is initialised from the current F32 and F64
+ * positions (RigidBody::init, RigidBody.cpp:57-124). */
+int orc_add_rigid_body(orc *o, int64_t n, const int64_t *ids)
+{
+    if (n <= 0) {
+        free_bodies(o);
+        rebuild_dynamic_springs(o);
+        return 0;
+    }
+    for (int64_t t = 0; t < n; ++t) {
+        if (ids[t] < 0 || ids[t] >= o->n)
+            return -1;
+        if (o->body_of[ids[t]] >= 0)
+            return o->nbodies;
+    }
+    o->bodies = (rbody *)realloc(o->bodies, sizeof(rbody) * (o->nbodies + 1));
+    rbody *B = &o->bodies[o->nbodies];
+    memset(B, 0, sizeof *B);
+    B->n = n;
+    B->ids = (int64_t *)malloc(sizeof(int64_t) * n);
+    B->p0 = (float *)malloc(sizeof(float) * 3 * n);
+    B->p0d = (double *)malloc(sizeof(double) * 3 * n);
+    memcpy(B->ids, ids, sizeof(int64_t) * n);
+    /* computeTotalMass, computeBarycentre (RigidBody.cpp:128-154) */
+    double M = 0.0;
+    for (int64_t t = 0; t < n; ++t)
+        M += o->mass[ids[t]];
+    B->M = M;
+    float bary[3] = {0.0f, 0.0f, 0.0f};
+    double tm = 0.0;
+    for (int64_t t = 0; t < n; ++t) {
+        const int64_t i = ids[t];
+        for (int c = 0; c < 3; ++c)
+            bary[c] = bary[c] + o->xf[3 * i + c] * o->mass[i];
+        tm = tm + o->mass[i];
+    }
+    const float inv = (float)(1.0 / tm);
+    for (int c = 0; c < 3; ++c)
+        B->X[c] = bary[c] * inv;
+    double bd[3] = {0.0, 0.0, 0.0};
+    for (int64_t t = 0; t < n; ++t)
+        for (int c = 0; c < 3; ++c)
+            bd[c] += o->xd[3 * ids[t] + c] * o->mass[ids[t]];
+    for (int c = 0; c < 3; ++c)
+        B->Xd[c] = bd[c] / M;
+    /* computeAllLocalPositions (RigidBody.cpp:166-176) */
+    for (int64_t t = 0; t < n; ++t)
+        for (int c = 0; c < 3; ++c) {
+            B->p0[3 * t + c] = o->xf[3 * ids[t] + c] - B->X[c];
+            B->p0d[3 * t + c] = o->xd[3 * ids[t] + c] - B->Xd[c];
+        }
+    /* I_body = 2/5 M dmax^2 Id with dmax = 1; its inverse (RigidBody.cpp:103-108) */
+    memset(B->iinv, 0, sizeof B->iinv);
+    const double ib = (2.0 / 5.0) * M * 1.0 * 1.0;
+    B->iinv[0] = B->iinv[4] = B->iinv[8] = 1.0 / ib;
+    B->q[3] = 1.0f;
+    B->qd[3] = 1.0;
+    for (int64_t t = 0; t < n; ++t) {
+        o->body_of[ids[t]] = o->nbodies;
+        o->local_of[ids[t]] = t;
+    }
+    o->nbodies++;
+    rebuild_dynamic_springs(o);
+    return o->nbodies;
+}
+
+int orc_num_bodies(const orc *o) { return o->nbodies; }
+
+/* Body state: X, v, L, omega, q(x, y, z, w) -> 16 values. */
+int orc_rigid_state(const orc *o, int mode, int b, double *out16)
+{
+    if (b < 0 || b >= o->nbodies)
+        return -1;
+    const rbody *B = &o->bodies[b];
+    for (int c = 0; c < 3; ++c) {
+        out16[c] = mode == ORC_F32 ? B->X[c] : B->Xd[c];
+        out16[3 + c] = mode == ORC_F32 ? B->v[c] : B->vd[c];
+        out16[6 + c] = mode == ORC_F32 ? B->L[c] : B->Ld[c];
+        out16[9 + c] = mode == ORC_F32 ? B->w[c] : B->wd[c];
+    }
+    for (int c = 0; c < 4; ++c)
+        out16[12 + c] = mode == ORC_F32 ? B->q[c] : B->qd[c];
+    return 0;
+}
+
+/* computeParticleForceAndTorque over the dynamic rigid particles in
+ * particle order (SpringNetwork.cpp:186-192), after every force is in. */
+static void rigid_aggregate_f(orc *o)
+{
+    for (int64_t t = 0; t < o->ndyn; ++t) {
+        const int64_t i = o->dyn[t];
+        const int b = o->body_of[i];
+        if (b < 0)
+            continue;
+        rbody *B = &o->bodies[b];
+        const float *f = o->ff + 3 * i;
+        float r[3];
+        for (int c = 0; c < 3; ++c)
+            r[c] = o->xf[3 * i + c] - B->X[c];
+        const float tq[3] = {r[1] * f[2] - r[2] * f[1], r[2] * f[0] - r[0] * f[2], r[0] * f[1] - r[1] * f[0]};
+        for (int c = 0; c < 3; ++c) {
+            B->F[c] += f[c];
+            B->T[c] += 0.0f + tq[c];
+        }
+    }
+}
+
+static void rigid_aggregate_d(orc *o)
+{
+    for (int64_t t = 0; t < o->ndyn; ++t) {
+        const int64_t i = o->dyn[t];
+        const int b = o->body_of[i];
+        if (b < 0)
+            continue;
+        rbody *B = &o->bodies[b];
+        const double *f = o->fd + 3 * i;
+        double r[3];
+        for (int c = 0; c < 3; ++c)
+            r[c] = o->xd[3 * i + c] - B->Xd[c];
+        B->Fd[0] += f[0];
+        B->Fd[1] += f[1];
+        B->Fd[2] += f[2];
+        B->Td[0] += r[1] * f[2] - r[2] * f[1];
+        B->Td[1] += r[2] * f[0] - r[0] * f[2];
+        B->Td[2] += r[0] * f[1] - r[1] * f[0];
+    }
+}
+
+/* RigidBody::solve without restraints, sampling or Monte Carlo. */
+static void rigid_solve_f(orc *o)
+{
+    const double dt = o->p.timestep;
+    const float fdt = (float)dt;
+    for (int b = 0; b < o->nbodies; ++b) {
+        rbody *B = &o->bodies[b];
+        const float fm = (float)B->M;
+        for (int c = 0; c < 3; ++c) {
+            B->a[c] = B->F[c] / fm;
+            B->v[c] += B->a[c] * fdt;
+        }
+        for (int c = 0; c < 3; ++c)
+            B->X[c] += B->v[c] * fdt + ((B->a[c] * 0.5f) * fdt) * fdt;
+        for (int c = 0; c < 3; ++c)
+            B->L[c] += B->T[c] * fdt;
+        for (int r = 0; r < 3; ++r)
+            B->w[r] = (float)(B->iinv[3 * r] * B->L[0] + B->iinv[3 * r + 1] * B->L[1] + B->iinv[3 * r + 2] * B->L[2]);
+        const float ang = (float)((double)norm_f(B->w[0], B->w[1], B->w[2]) * dt);
+        float dq[4], nq[4];
+        qaxis_f(B->w, ang, dq);
+        qmul_f(dq, B->q, nq);
+        qnormalize_f(nq);
+        memcpy(B->q, nq, sizeof nq);
+        memset(B->F, 0, sizeof B->F);
+        memset(B->T, 0, sizeof B->T);
+    }
+}
+
+static void rigid_solve_d(orc *o)
+{
+    const double dt = o->p.timestep;
+    for (int b = 0; b < o->nbodies; ++b) {
+        rbody *B = &o->bodies[b];
+        double a[3];
+        for (int c = 0; c < 3; ++c) {
+            a[c] = B->Fd[c] / B->M;
+            B->vd[c] += a[c] * dt;
+        }
+        for (int c = 0; c < 3; ++c)
+            B->Xd[c] += B->vd[c] * dt + a[c] * 0.5 * dt * dt;
+        for (int c = 0; c < 3; ++c)
+            B->Ld[c] += B->Td[c] * dt;
+        for (int r = 0; r < 3; ++r)
+            B->wd[r] = B->iinv[3 * r] * B->Ld[0] + B->iinv[3 * r + 1] * B->Ld[1] + B->iinv[3 * r + 2] * B->Ld[2];
+        const double ang = norm_d(B->wd[0], B->wd[1], B->wd[2]) * dt;
+        double dq[4], nq[4];
+        qaxis_d(B->wd, ang, dq);
+        qmul_d(dq, B->qd, nq);
+        qnormalize_d(nq);
+        memcpy(B->qd, nq, sizeof nq);
+        memset(B->Fd, 0, sizeof B->Fd);
+        memset(B->Td, 0, sizeof B->Td);
+    }
+}
+
+/* New position X + q (p0, 0) q^-1 and velocity (x_new - x_prev) / dt. */
+static void rigid_place_f(orc *o, int64_t i)
+{
+    const rbody *B = &o->bodies[o->body_of[i]];
+    const int64_t l = o->local_of[i];
+    const float P[4] = {B->p0[3 * l], B->p0[3 * l + 1], B->p0[3 * l + 2], 0.0f};
+    float t1[4], qi[4], r[4];
+    qmul_f(B->q, P, t1);
+    qinverse_f(B->q, qi);
+    qmul_f(t1, qi, r);
+    const float fdt = (float)o->p.timestep;
+    for (int c = 0; c < 3; ++c) {
+        const float prev = o->xf[3 * i + c];
+        const float nx = B->X[c] + r[c];
+        o->xf[3 * i + c] = nx;
+        o->vf[3 * i + c] = (nx - prev) / fdt;
+    }
+}
+
+static void rigid_place_d(orc *o, int64_t i)
+{
+    const rbody *B = &o->bodies[o->body_of[i]];
+    const int64_t l = o->local_of[i];
+    const double P[4] = {B->p0d[3 * l], B->p0d[3 * l + 1], B->p0d[3 * l + 2], 0.0};
+    double t1[4], qi[4], r[4];
+    qmul_d(B->qd, P, t1);
+    qinverse_d(B->qd, qi);
+    qmul_d(t1, qi, r);
+    const double dt = o->p.timestep;
+    for (int c = 0; c < 3; ++c) {
+        const double prev = o->xd[3 * i + c];
+        const double nx = B->Xd[c] + r[c];
+        o->xd[3 * i + c] = nx;
+        o->vd[3 * i + c] = (nx - prev) / dt;
+    }
+}
+
+/* ---------------------------------------------------------------------------
+ * Grids.  PotentialGrid::compute_gradient (PotentialGrid.cpp:15-58),
+ * GridCoordinatesSystem::cell_coordinates / is_out_of_grid
+ * (GridCoordinatesSystem.hpp:85-93,206-218), the field forces of
+ * Particle.cpp:136-189 and electrostatic_field_energy
+ * (energy/electrostatic_field.hpp:15-23).
+ * ------------------------------------------------------------------------- */
+static const double K_BOLTZ = 1.3806504E-23;
+
+static double grad_scale(void) { return K_BOLTZ * K_M_TO_A * c_newton(); }
+
+/* kind: BS_GRID_POTENTIAL or BS_GRID_DENSITY; scalar holds shape[0]*shape[1]*
+ * shape[2] values, z fastest.  Returns 0, or -1 for a bad argument. */
+int orc_set_grid(orc *o, int kind, const double *origin, const int64_t *shape, const double *cell,
+                 const float *scalar)
+{
+    if (kind < 0 || kind > 1)
+        return -1;
+    if (o->grid[kind]) {
+        free(o->grid[kind]->cf);
+        free(o->grid[kind]->cd);
+        free(o->grid[kind]);
+        o->grid[kind] = NULL;
+    }
+    if (!scalar)
+        return 0;
+    for (int c = 0; c < 3; ++c)
+        if (shape[c] <= 0 || !(cell[c] > 0.0))
+            return -1;
+    ogrid *g = (ogrid *)calloc(1, sizeof(ogrid));
+    const int64_t nx = shape[0], ny = shape[1], nz = shape[2], nc = nx * ny * nz;
+    for (int c = 0; c < 3; ++c) {
+        g->origin[c] = origin[c];
+        g->cell[c] = cell[c];
+        g->shape[c] = shape[c];
+        g->bmax[c] = origin[c] + (double)shape[c] * cell[c]; /* Box(origin, shape * cell) */
+    }
+    g->cf = (float *)malloc(sizeof(float) * 4 * nc);
+    g->cd = (double *)malloc(sizeof(double) * 4 * nc);
+    const float fscale = (float)grad_scale();
+    const double dscale = grad_scale();
+    const float d3[3] = {(float)cell[0], (float)cell[1], (float)cell[2]};
+#define GIDX(i, j, k) (((i) * ny + (j)) * nz + (k))
+    for (int64_t i = 0; i < nx; ++i)
+        for (int64_t j = 0; j < ny; ++j)
+            for (int64_t k = 0; k < nz; ++k) {
+                const int64_t id = GIDX(i, j, k);
+                const float cur = scalar[id];
+                float gf[3] = {0.0f, 0.0f, 0.0f};
+                double gd[3] = {0.0, 0.0, 0.0};
+                const int64_t nb[3][2] = {{i > 0 && i < nx - 1 ? GIDX(i - 1, j, k) : -1, i < nx - 1 ? GIDX(i + 1, j, k) : -1},
+                                          {j > 0 && j < ny - 1 ? GIDX(i, j - 1, k) : -1, j < ny - 1 ? GIDX(i, j + 1, k) : -1},
+                                          {k > 0 && k < nz - 1 ? GIDX(i, j, k - 1) : -1, k < nz - 1 ? GIDX(i, j, k + 1) : -1}};
+                for (int a = 0; a < 3; ++a) {
+                    if (nb[a][0] < 0)
+                        continue;
+                    const float prv = scalar[nb[a][0]], nxt = scalar[nb[a][1]];
+                    gf[a] = (float)(((cur - prv) + (nxt - cur)) / (d3[a] * 2.0));
+                    gd[a] = ((double)nxt - (double)prv) / (2.0 * cell[a]);
+                }
+                g->cf[4 * id] = cur;
+                g->cd[4 * id] = cur;
+                for (int a = 0; a < 3; ++a) {
+                    g->cf[4 * id + 1 + a] = gf[a] * -fscale;
+                    g->cd[4 * id + 1 + a] = gd[a] * -dscale;
+                }
+            }
+#undef GIDX
+    o->grid[kind] = g;
+    return 0;
+}
+
+void orc_set_burying(orc *o, const float *burying)
+{
+    for (int64_t i = 0; i < o->n; ++i)
+        o->burying[i] = burying ? burying[i] : 1.0f;
+}
+
+/* Cell index of a position, -1 when off the grid. */
+static int64_t grid_cell(const ogrid *g, double x, double y, double z)
+{
+    const double p[3] = {x, y, z};
+    for (int c = 0; c < 3; ++c)
+        if (p[c] < g->origin[c] || p[c] > g->bmax[c] - 1e-6)
+            return -1;
+    int64_t q[3];
+    for (int c = 0; c < 3; ++c) {
+        q[c] = (int)((p[c] - g->origin[c]) / g->cell[c]);
+        if (q[c] < 0 || q[c] >= g->shape[c])
+            return -1; /* DenseGrid::at would throw */
+    }
+    return (q[0] * g->shape[1] + q[1]) * g->shape[2] + q[2];
+}
+
+/* Electrostatic field (potential grid) force and energy, float-faithful. */
+static void potential_field_f(const orc *o, int64_t i, float *fi, float *ei)
+{
+    const ogrid *g = o->grid[BS_GRID_POTENTIAL];
+    const float *x = o->xf + 3 * i;
+    const int64_t c = grid_cell(g, x[0], x[1], x[2]);
+    if (c < 0)
+        return;
+    const float s = o->p.potentialgrid_scale, q = o->charge[i];
+    for (int a = 0; a < 3; ++a)
+        fi[a] = fi[a] + (g->cf[4 * c + 1 + a] * q) * s;
+    float e = g->cf[4 * c] * q;
+    e = (float)(e * K_BOLTZ);
+    e = (float)(e * K_J_TO_KJ);
+    e = (float)(e * K_NA);
+    *ei += s * e;
+}
+
+static void potential_field_d(const orc *o, int64_t i, const float *pos, double *fi, double *ei)
+{
+    const ogrid *g = o->grid[BS_GRID_POTENTIAL];
+    const int64_t c = grid_cell(g, pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+    if (c < 0)
+        return;
+    const double s = o->p.potentialgrid_scale, q = o->charge[i];
+    for (int a = 0; a < 3; ++a)
+        fi[a] += g->cd[4 * c + 1 + a] * q * s;
+    *ei += s * g->cd[4 * c] * q * K_BOLTZ * K_J_TO_KJ * K_NA;
+}
+
+static void density_field_f(const orc *o, int64_t i, float *fi)
+{
+    const ogrid *g = o->grid[BS_GRID_DENSITY];
+    const float *x = o->xf + 3 * i;
+    const int64_t c = grid_cell(g, x[0], x[1], x[2]);
+    if (c < 0)
+        return;
+    const float s = o->p.densitygrid_scale, bu = o->burying[i];
+    for (int a = 0; a < 3; ++a)
+        fi[a] = fi[a] + (g->cf[4 * c + 1 + a] * bu) * s;
+}
+
+static void density_field_d(const orc *o, int64_t i, const float *pos, double *fi)
+{
+    const ogrid *g = o->grid[BS_GRID_DENSITY];
+    const int64_t c = grid_cell(g, pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+    if (c < 0)
+        return;
+    const double s = o->p.densitygrid_scale, bu = o->burying[i];
+    for (int a = 0; a < 3; ++a)
+        fi[a] += g->cd[4 * c + 1 + a] * bu * s;
+}
+
+/* ---------------------------------------------------------------------------
+ * Insertion vector, InsertionVector.cpp:18-78 with the centroid of every
+ * particle (measure.hpp:121-140).  out6 = angle (deg), roll angle (deg),
+ * insertion depth (centroid z), vector (3).
+ * ------------------------------------------------------------------------- */
+int orc_insertion(const orc *o, int mode, int64_t i1, int64_t i2, double *out6)
+{
+    if (i1 < 0 || i2 < 0 || i1 >= o->n || i2 >= o->n)
+        return -1;
+    double cs[3] = {0.0, 0.0, 0.0};
+    for (int64_t i = 0; i < o->n; ++i)
+        for (int c = 0; c < 3; ++c)
+            cs[c] += mode == ORC_F32 ? (double)o->xf[3 * i + c] : o->xd[3 * i + c];
+    const double f = o->n ? 1.0 / (double)o->n : 0.0;
+    for (int c = 0; c < 3; ++c)
+        cs[c] *= f;
+    if (mode == ORC_F32) {
+        const float *p1 = o->xf + 3 * i1, *p2 = o->xf + 3 * i2;
+        float v[3] = {p2[0] - p1[0], p2[1] - p1[1], p2[2] - p1[2]};
+        float angle = acosf(-v[2] / norm_f(v[0], v[1], v[2]));
+        angle = (float)(angle * (180.0 / M_PI));
+        angle = (float)(angle - 90.0);
+        const float bary[3] = {(float)cs[0], (float)cs[1], (float)cs[2]};
+        const float n2sq = (float)pow(norm_f(v[0], v[1], v[2]), 2.0);
+        float proj[3], n1[3], n3[3], ref[3], n2[3];
+        for (int c = 0; c < 3; ++c) /* term-by-term products, as written */
+            proj[c] = p1[c] + v[c] * (((bary[c] - p1[c]) * v[c]) / n2sq);
+        for (int c = 0; c < 3; ++c)
+            n1[c] = proj[c] - bary[c];
+        normalize_f(n1);
+        n3[0] = n1[1] * v[2] - n1[2] * v[1];
+        n3[1] = n1[2] * v[0] - n1[0] * v[2];
+        n3[2] = n1[0] * v[1] - n1[1] * v[0];
+        normalize_f(n3);
+        const float am = (float)(atan2f(n3[2], n1[2]) - M_PI);
+        const float r = (float)sqrt(((double)bary[0] - proj[0]) * ((double)bary[0] - proj[0]) +
+                                    ((double)bary[1] - proj[1]) * ((double)bary[1] - proj[1]) +
+                                    ((double)bary[2] - proj[2]) * ((double)bary[2] - proj[2]));
+        const float ca = cosf(am), sa = sinf(am);
+        for (int c = 0; c < 3; ++c)
+            ref[c] = (proj[c] + (n1[c] * r) * ca) + (n3[c] * r) * sa;
+        for (int c = 0; c < 3; ++c)
+            n2[c] = proj[c] - ref[c];
+        normalize_f(n2);
+        const float dot12 = n1[0] * n2[0] + n1[1] * n2[1] + n1[2] * n2[2];
+        const float arad = acosf(dot12 / (norm_f(n1[0], n1[1], n1[2]) * norm_f(n2[0], n2[1], n2[2])));
+        const float c12[3] = {n1[1] * n2[2] - n1[2] * n2[1], n1[2] * n2[0] - n1[0] * n2[2],
+                              n1[0] * n2[1] - n1[1] * n2[0]};
+        const float s = c12[0] * (p2[0] - proj[0]) + c12[1] * (p2[1] - proj[1]) + c12[2] * (p2[2] - proj[2]);
+        const float adeg = (float)((arad * 180) / M_PI);
+        out6[0] = angle;
+        out6[1] = s > 0.0 ? 360 - adeg : adeg;
+        out6[2] = (float)cs[2];
+        for (int c = 0; c < 3; ++c)
+            out6[3 + c] = v[c];
+    } else {
+        const double *p1 = o->xd + 3 * i1, *p2 = o->xd + 3 * i2;
+        double v[3] = {p2[0] - p1[0], p2[1] - p1[1], p2[2] - p1[2]};
+        const double vn = norm_d(v[0], v[1], v[2]);
+        out6[0] = acos(-v[2] / vn) * (180.0 / M_PI) - 90.0;
+        double proj[3], n1[3], n3[3], ref[3], n2[3];
+        for (int c = 0; c < 3; ++c)
+            proj[c] = p1[c] + v[c] * (((cs[c] - p1[c]) * v[c]) / (vn * vn));
+        for (int c = 0; c < 3; ++c)
+            n1[c] = proj[c] - cs[c];
+        normalize_d(n1);
+        n3[0] = n1[1] * v[2] - n1[2] * v[1];
+        n3[1] = n1[2] * v[0] - n1[0] * v[2];
+        n3[2] = n1[0] * v[1] - n1[1] * v[0];
+        normalize_d(n3);
+        const double am = atan2(n3[2], n1[2]) - M_PI;
+        const double r = norm_d(cs[0] - proj[0], cs[1] - proj[1], cs[2] - proj[2]);
+        for (int c = 0; c < 3; ++c)
+            ref[c] = proj[c] + n1[c] * r * cos(am) + n3[c] * r * sin(am);
+        for (int c = 0; c < 3; ++c)
+            n2[c] = proj[c] - ref[c];
+        normalize_d(n2);
+        const double dot12 = n1[0] * n2[0] + n1[1] * n2[1] + n1[2] * n2[2];
+        const double arad = acos(dot12 / (norm_d(n1[0], n1[1], n1[2]) * norm_d(n2[0], n2[1], n2[2])));
+        const double c12[3] = {n1[1] * n2[2] - n1[2] * n2[1], n1[2] * n2[0] - n1[0] * n2[2],
+                               n1[0] * n2[1] - n1[1] * n2[0]};
+        const double s = c12[0] * (p2[0] - proj[0]) + c12[1] * (p2[1] - proj[1]) + c12[2] * (p2[2] - proj[2]);
+        const double adeg = arad * 180.0 / M_PI;
+        out6[1] = s > 0.0 ? 360.0 - adeg : adeg;
+        out6[2] = cs[2];
+        for (int c = 0; c < 3; ++c)
+            out6[3 + c] = v[c];
+    }
+    return 0;
+}
+
+/* ---------------------------------------------------------------------------
  * Integrator, SpringNetwork.cpp:202-241 and Particle.cpp:109-128.
  * ------------------------------------------------------------------------- */
 static int integrate_f32(orc *o)
@@ -1518,13 +2178,18 @@ static int integrate_f32(orc *o)
         const int64_t i = o->dyn[t];
         float *x = o->xf + 3 * i, *v = o->vf + 3 * i, *f = o->ff + 3 * i;
         const float m = o->mass[i];
-        if (m > 0.0f)
+        float kei = 0.0f; /* a rigid particle's kinetic energy is reset before it is summed */
+        if (o->body_of[i] >= 0) {
+            rigid_place_f(o, i);
+        } else {
+            if (m > 0.0f)
+                for (int c = 0; c < 3; ++c)
+                    v[c] = v[c] + (f[c] / m) * dt;
+            const float vit = norm_f(v[0], v[1], v[2]);
+            kei = (float)(0.5f * m * (vit * vit) * c_kinetic());
             for (int c = 0; c < 3; ++c)
-                v[c] = v[c] + (f[c] / m) * dt;
-        const float vit = norm_f(v[0], v[1], v[2]);
-        const float kei = (float)(0.5f * m * (vit * vit) * c_kinetic());
-        for (int c = 0; c < 3; ++c)
-            x[c] = x[c] + v[c] * dt;
+                x[c] = x[c] + v[c] * dt;
+        }
         if ((!isfinite(x[0]) || !isfinite(x[1]) || !isfinite(x[2])) && bad < 0)
             bad = i;
         ke += kei;
@@ -1544,12 +2209,16 @@ static int integrate_f64(orc *o)
         const int64_t i = o->dyn[t];
         double *x = o->xd + 3 * i, *v = o->vd + 3 * i, *f = o->fd + 3 * i;
         const double m = o->mass[i];
-        if (m > 0.0)
+        if (o->body_of[i] >= 0) {
+            rigid_place_d(o, i);
+        } else {
+            if (m > 0.0)
+                for (int c = 0; c < 3; ++c)
+                    v[c] += f[c] / m * dt;
+            ke += 0.5 * m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) * c_kinetic();
             for (int c = 0; c < 3; ++c)
-                v[c] += f[c] / m * dt;
-        ke += 0.5 * m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) * c_kinetic();
-        for (int c = 0; c < 3; ++c)
-            x[c] += v[c] * dt;
+                x[c] += v[c] * dt;
+        }
         if ((!isfinite(x[0]) || !isfinite(x[1]) || !isfinite(x[2])) && bad < 0)
             bad = i;
         f[0] = f[1] = f[2] = 0.0;
@@ -1589,7 +2258,7 @@ void orc_compute_forces(orc *o, int mode, const double *ext3, double *F3, double
     }
 }
 
-/* nsteps of SpringNetwork::computeStep (without rigid bodies, probe, grids).
+/* nsteps of SpringNetwork::computeStep (without the probe).
  * Energies of the last step go to E6 (nullable). */
 int orc_step(orc *o, int mode, int nsteps, const double *ext3, double *E6)
 {
@@ -1603,9 +2272,17 @@ int orc_step(orc *o, int mode, int nsteps, const double *ext3, double *E6)
     for (int s = 0; s < nsteps && rc == BS_OK; ++s) {
         if (mode == ORC_F32) {
             forces_f32(o, ef);
+            if (o->p.rigidbody_enable) {
+                rigid_aggregate_f(o);
+                rigid_solve_f(o);
+            }
             rc = integrate_f32(o);
         } else {
             forces_f64(o, ext3);
+            if (o->p.rigidbody_enable) {
+                rigid_aggregate_d(o);
+                rigid_solve_d(o);
+            }
             rc = integrate_f64(o);
         }
     }

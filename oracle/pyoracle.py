@@ -65,6 +65,17 @@ def oracle_lib():
         L.orc_term_force.restype = ctypes.c_double
         L.orc_constants.argtypes = [_f64p]
         L.orc_num_threads.restype = ctypes.c_int
+        L.orc_add_rigid_body.argtypes = [_vp, ctypes.c_int64, _i64p]
+        L.orc_add_rigid_body.restype = ctypes.c_int
+        L.orc_num_bodies.argtypes = [_vp]
+        L.orc_num_bodies.restype = ctypes.c_int
+        L.orc_rigid_state.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _f64p]
+        L.orc_rigid_state.restype = ctypes.c_int
+        L.orc_set_grid.argtypes = [_vp, ctypes.c_int, _f64p, _i64p, _f64p, _f32p]
+        L.orc_set_grid.restype = ctypes.c_int
+        L.orc_set_burying.argtypes = [_vp, _f32p]
+        L.orc_insertion.argtypes = [_vp, ctypes.c_int, ctypes.c_int64, ctypes.c_int64, _f64p]
+        L.orc_insertion.restype = ctypes.c_int
         _olib = L
     return _olib
 
@@ -93,6 +104,13 @@ def ref_lib():
                                 _u32p, _f32p, _f32p, _u32p, ctypes.c_int64]
         L.ref_pairs.restype = ctypes.c_int64
         L.ref_set_threads.argtypes = [ctypes.c_int]
+        L.ref_add_rigid_body.argtypes = [_vp, ctypes.c_int64, _i64p]
+        L.ref_add_rigid_body.restype = ctypes.c_int
+        L.ref_rigid_state.argtypes = [ctypes.c_int, _f64p]
+        L.ref_rigid_state.restype = ctypes.c_int
+        L.ref_set_grid.argtypes = [_vp, ctypes.c_int, _f64p, _i64p, _f64p, _f32p, ctypes.c_double]
+        L.ref_set_burying.argtypes = [_vp, _f32p]
+        L.ref_insertion.argtypes = [_vp, ctypes.c_int64, ctypes.c_int64, _f64p]
         _rlib = L
     return _rlib
 
@@ -163,6 +181,40 @@ class Oracle:
         self.L.orc_pairs(self.h, kind, method, mode, _p(out, ctypes.c_uint32), cnt)
         return out[:2 * cnt].reshape(-1, 2)
 
+    def add_rigid_body(self, ids):
+        a = np.ascontiguousarray(ids, np.int64)
+        r = self.L.orc_add_rigid_body(self.h, a.size, _p(a, ctypes.c_int64))
+        if r < 0:
+            raise ValueError("rigid body id out of range")
+        return r
+
+    def rigid_state(self, mode, b):
+        """X, v, L, omega (3 each) and q (x, y, z, w) of body b."""
+        out = np.zeros(16)
+        if self.L.orc_rigid_state(self.h, mode, b, _p(out, ctypes.c_double)) != 0:
+            raise IndexError(b)
+        return out
+
+    def set_grid(self, kind, origin, shape, cell, scalar):
+        o = np.ascontiguousarray(origin, np.float64)
+        sh = np.ascontiguousarray(shape, np.int64)
+        c = np.ascontiguousarray(cell, np.float64)
+        v = None if scalar is None else np.ascontiguousarray(scalar, np.float32).reshape(-1)
+        if self.L.orc_set_grid(self.h, kind, _p(o, ctypes.c_double), _p(sh, ctypes.c_int64),
+                               _p(c, ctypes.c_double), _p(v, ctypes.c_float)) != 0:
+            raise ValueError("bad grid")
+
+    def set_burying(self, burying):
+        b = np.ascontiguousarray(burying, np.float32)
+        self.L.orc_set_burying(self.h, _p(b, ctypes.c_float))
+
+    def insertion(self, mode, i1, i2):
+        """angle, roll angle, depth, vector (3)."""
+        out = np.zeros(6)
+        if self.L.orc_insertion(self.h, mode, i1, i2, _p(out, ctypes.c_double)) != 0:
+            raise IndexError((i1, i2))
+        return out
+
     def cells(self, kind, mode=F32):
         nm = ctypes.c_int64()
         nc = self.L.orc_cells(self.h, kind, mode, None, None, None, None, None, ctypes.byref(nm))
@@ -230,6 +282,34 @@ class Reference:
         v = np.zeros(3 * self.n, np.float32)
         self.L.ref_get_state(self.h, _p(x, ctypes.c_float), _p(v, ctypes.c_float))
         return x.reshape(-1, 3), v.reshape(-1, 3)
+
+    def add_rigid_body(self, ids):
+        a = np.ascontiguousarray(ids, np.int64)
+        return self.L.ref_add_rigid_body(self.h, a.size, _p(a, ctypes.c_int64))
+
+    def rigid_state(self, b):
+        """X, (zeros), omega of body b."""
+        out = np.zeros(9)
+        if self.L.ref_rigid_state(b, _p(out, ctypes.c_double)) != 0:
+            raise IndexError(b)
+        return out
+
+    def set_grid(self, kind, origin, shape, cell, scalar, scale=1.0):
+        o = np.ascontiguousarray(origin, np.float64)
+        sh = np.ascontiguousarray(shape, np.int64)
+        c = np.ascontiguousarray(cell, np.float64)
+        v = np.ascontiguousarray(scalar, np.float32).reshape(-1)
+        self.L.ref_set_grid(self.h, kind, _p(o, ctypes.c_double), _p(sh, ctypes.c_int64), _p(c, ctypes.c_double),
+                            _p(v, ctypes.c_float), scale)
+
+    def set_burying(self, burying):
+        b = np.ascontiguousarray(burying, np.float32)
+        self.L.ref_set_burying(self.h, _p(b, ctypes.c_float))
+
+    def insertion(self, i1, i2):
+        out = np.zeros(6)
+        self.L.ref_insertion(self.h, i1, i2, _p(out, ctypes.c_double))
+        return out
 
     def cells(self, kind, cutoff, skin=0.0):
         nm = ctypes.c_int64()

@@ -15,9 +15,19 @@
 //  - ref_pairs: the processed pairs, recorded by a ForceField subclass whose
 //    energy callbacks see particle ids encoded in a per-particle property;
 //    the pair decisions are made by the reference's own Particle::add*Force.
+//  - ref_add_rigid_body: RigidBodiesManager::InitRigidBodies
+//    (src/rigidbody/RigidBodiesManager.cpp:11-43); the manager's collection is
+//    global, so one reference object with bodies may exist at a time.
+//  - ref_set_grid: a PotentialGrid filled cell by cell, then
+//    PotentialGrid::compute_gradient (src/grid/PotentialGrid.cpp:20-60), as
+//    the DX reader does after parsing (src/IO/OpenDXReader.cpp:146-173).
+//  - ref_insertion: InsertionVector::computeVector/Angle/RollAngle
+//    (src/InsertionVector.cpp:18-78).
 #include "SpringNetwork.h"
 #include "configuration/Configuration.hpp"
 #include "forcefield/ForceField.h"
+#include "InsertionVector.h"
+#include "grid/PotentialGrid.hpp"
 #include "rigidbody/RigidBodiesManager.h"
 
 #include <algorithm>
@@ -72,6 +82,7 @@ biospring::configuration::Configuration make_config(const bs_params * p)
     c.hydrophobicity.enable = p->hydrophobicity_enable != 0;
     c.hydrophobicity.cutoff = p->hydrophobicity_cutoff;
     c.hydrophobicity.scale = p->hydrophobicity_scale;
+    c.rigidbody.enable = p->rigidbody_enable != 0;
     return c;
 }
 
@@ -124,6 +135,22 @@ class RefNet : public SpringNetwork
   public:
     void swap_forcefield(std::unique_ptr<biospring::forcefield::ForceField> ff) { _ff = std::move(ff); }
     std::vector<Particle> & particles() { return _particles; }
+    // Grid options are switched on after setup(): setup would otherwise read
+    // the grids from DX files (SpringNetwork.cpp:712-747).
+    void enable_grid(int kind, double scale)
+    {
+        if (kind == BS_GRID_POTENTIAL)
+        {
+            _config.potentialgrid.enable = true;
+            _config.potentialgrid.scale = scale;
+            _ff->setForceFieldScale(scale);
+        }
+        else
+        {
+            _config.densitygrid.enable = true;
+            _config.densitygrid.scale = scale;
+        }
+    }
 };
 
 using Container = std::vector<Particle>;
@@ -235,7 +262,84 @@ void * ref_create(const bs_params * p, int64_t n, const float * x3, const float 
     return s;
 }
 
-void ref_destroy(void * h) { delete static_cast<RefSys *>(h); }
+void ref_destroy(void * h)
+{
+    biospring::rigidbody::RigidBodiesManager::CleanRigidBodies();
+    delete static_cast<RefSys *>(h);
+}
+
+// Bodies are made rigid through the reference manager; returns the number of
+// bodies in its (global) collection.
+int ref_add_rigid_body(void * h, int64_t n, const int64_t * ids)
+{
+    auto * s = static_cast<RefSys *>(h);
+    std::vector<unsigned> v;
+    for (int64_t t = 0; t < n; ++t)
+        v.push_back((unsigned)ids[t]);
+    biospring::rigidbody::RigidBodiesManager::InitRigidBodies(s->net.get(), v);
+    return (int)biospring::rigidbody::RigidBodiesManager::getCollection().size();
+}
+
+// Body state: position, velocity, omega (3 each); the reference keeps L private.
+int ref_rigid_state(int b, double * out9)
+{
+    const auto & col = biospring::rigidbody::RigidBodiesManager::getCollection();
+    if (b < 0 || (size_t)b >= col.size())
+        return -1;
+    const Vector3f X = col[b]->getPos(), w = col[b]->getOmega();
+    out9[0] = X.getX();
+    out9[1] = X.getY();
+    out9[2] = X.getZ();
+    out9[6] = w.getX();
+    out9[7] = w.getY();
+    out9[8] = w.getZ();
+    out9[3] = out9[4] = out9[5] = 0.0;
+    return 0;
+}
+
+// Grid `kind` with origin, shape and cell size, scalar values z fastest.
+void ref_set_grid(void * h, int kind, const double * origin, const int64_t * shape, const double * cell,
+                  const float * scalar, double scale)
+{
+    auto * s = static_cast<RefSys *>(h);
+    RefNet & net = *s->net;
+    biospring::grid::PotentialGrid & g = kind == BS_GRID_POTENTIAL ? net.getPotentialGrid() : net.getDensityGrid();
+    std::array<double, 6> box{origin[0], origin[1], origin[2], origin[0] + shape[0] * cell[0],
+                              origin[1] + shape[1] * cell[1], origin[2] + shape[2] * cell[2]};
+    g.reshape(box, {cell[0], cell[1], cell[2]});
+    int64_t t = 0;
+    for (int i = 0; i < shape[0]; ++i)
+        for (int j = 0; j < shape[1]; ++j)
+            for (int k = 0; k < shape[2]; ++k)
+                g.at(biospring::grid::discrete_coordinates(i, j, k)).scalar = scalar[t++];
+    g.compute_gradient();
+    net.enable_grid(kind, scale);
+}
+
+void ref_set_burying(void * h, const float * burying)
+{
+    auto * s = static_cast<RefSys *>(h);
+    for (unsigned i = 0; i < s->net->getNumberOfParticles(); ++i)
+        s->net->getParticle(i).setBurying(burying[i]);
+}
+
+// angle, roll angle, insertion depth, vector
+void ref_insertion(void * h, int64_t i1, int64_t i2, double * out6)
+{
+    auto * s = static_cast<RefSys *>(h);
+    RefNet & net = *s->net;
+    InsertionVector iv(net, net.getParticle((unsigned)i1), net.getParticle((unsigned)i2));
+    iv.computeVector();
+    iv.computeAngle();
+    iv.computeRollAngle();
+    out6[0] = iv.getAngle();
+    out6[1] = iv.getRollAngle();
+    out6[2] = iv.getInsertionDepth();
+    const Vector3f v = iv.getVector();
+    out6[3] = v.getX();
+    out6[4] = v.getY();
+    out6[5] = v.getZ();
+}
 
 // idleRun + computeForces; forces of every particle (static ones too, as the
 // reference accumulates them) and the six energies.

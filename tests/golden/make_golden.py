@@ -25,6 +25,7 @@ from biospring_amd import _abi, workloads  # noqa: E402
 from oracle import pyoracle as po  # noqa: E402
 
 OUT = os.path.dirname(os.path.abspath(__file__))
+ONLY = set(sys.argv[1:])  # case names to (re)write; all when empty
 
 SYS_KEYS = ("x", "v", "mass", "charge", "radius", "eps", "sasa", "transfer", "hydro", "flags",
             "id1", "id2", "l0", "k")
@@ -43,9 +44,13 @@ def store(name, system, params, **outputs):
     print(f"{name}: n={system.n} m={system.m} -> {os.path.getsize(path) / 1024:.0f} KiB")
 
 
-def ref_case(name, system, params, kinds=(0, 1), steps=(1, 5), cells=True, variants=()):
+def ref_case(name, system, params, kinds=(0, 1), steps=(1, 5), cells=True, variants=(), prep=None, extra=None):
     """Reference outputs for `params`, plus for each (tag, params) variant the
-    forces, energies and pair sets under keys prefixed by the tag."""
+    forces, energies and pair sets under keys prefixed by the tag.  `prep(ref)`
+    is applied to every reference object after creation (rigid bodies, grids);
+    `extra(ref, step)` returns arrays stored after each recorded step."""
+    if ONLY and name not in ONLY:
+        return
     out = {}
     for tag, vp in variants:
         ref = po.Reference(system, vp)
@@ -57,6 +62,8 @@ def ref_case(name, system, params, kinds=(0, 1), steps=(1, 5), cells=True, varia
             if (vp.steric_enable, vp.electrostatic_enable)[kind]:
                 out[f"{tag}__pairs_{kind}"] = po.ref_pairs(system, kind, vp)
     ref = po.Reference(system, params)
+    if prep:
+        prep(ref)
     F, E = ref.compute_forces()
     out["forces"], out["energies"] = F, E
     ref.close()
@@ -70,15 +77,30 @@ def ref_case(name, system, params, kinds=(0, 1), steps=(1, 5), cells=True, varia
                 out[f"cells_{kind}_{key}"] = c[key]
             ref.close()
     ref = po.Reference(system, params)
+    if prep:
+        prep(ref)
     done = 0
     for s in steps:
         E = ref.step(s - done)
         done = s
         x, v = ref.state()
         out[f"x_{s}"], out[f"v_{s}"], out[f"E_{s}"] = x, v, E
+        if extra:
+            for k, a in extra(ref, s).items():
+                out[f"{k}_{s}"] = a
     ref.close()
     out["steps"] = np.array(steps, np.int64)
     store(name, system, params, **out)
+
+
+def add_arrays(name, **arrays):
+    """Stores the inputs a case needs beyond the system (bodies, grids)."""
+    if ONLY and name not in ONLY:
+        return
+    path = os.path.join(OUT, f"{name}.npz")
+    z = dict(np.load(path))
+    z.update(arrays)
+    np.savez_compressed(path, **z)
 
 
 def subset(system, n):
@@ -152,6 +174,60 @@ def main():
         pm.imp_upper_curvature, pm.imp_lower_curvature = uc, lc
         ref_case(f"imp_{tag}", m, pm, kinds=(), steps=(1, 5), cells=False)
 
+    # one rigid body (RigidBodiesManager::InitRigidBodies) made of the first
+    # 120 dynamic beads, flexible beads around it; and one body of every
+    # dynamic bead, as SpringNetwork::run makes it (SpringNetwork.cpp:281-284).
+    # The reference places rigid beads by their index in the dynamic list,
+    # which equals their index in the body in both layouts.
+    r = workloads.generate(0, seed=16, n=300)
+    r.v = rng.normal(0.0, 0.01, r.v.shape).astype(np.float32)
+    pr = r.params.copy()
+    pr.rigidbody_enable = 1
+    body = np.arange(120)
+
+    def rigid_extra(ref, step):
+        return {"body": ref.rigid_state(0)}
+
+    ref_case("rigid300", r, pr, steps=(1, 5, 20), prep=lambda ref: ref.add_rigid_body(body), extra=rigid_extra)
+    add_arrays("rigid300", body_ids=body.astype(np.int64))
+    ra = workloads.generate(0, seed=17, n=200)
+    ra.flags = (rng.random(ra.n) < 0.1).astype(np.uint8)
+    pa = ra.params.copy()
+    pa.rigidbody_enable = 1
+    alldyn = np.nonzero(ra.flags == 0)[0]
+    ref_case("rigidall200", ra, pa, kinds=(), cells=False, steps=(1, 5, 20),
+             prep=lambda ref: ref.add_rigid_body(alldyn), extra=rigid_extra)
+    add_arrays("rigidall200", body_ids=alldyn.astype(np.int64))
+
+    # electrostatic potential and density grids (Particle.cpp:136-189) with
+    # per-bead burying; a few beads lie off the grids.  The insertion vector
+    # (InsertionVector.cpp:18-78) is recorded after each step.
+    g = workloads.generate(0, seed=18, n=300)
+    pg = g.params.copy()
+    pg.potentialgrid_enable = 1
+    pg.potentialgrid_scale = 0.7
+    pg.densitygrid_enable = 1
+    pg.densitygrid_scale = 1.3
+    gx = g.x.reshape(-1, 3)
+    g_origin = (gx.min(0) - 1.0 + np.array([3.0, 0.0, 0.0])).astype(np.float64)
+    g_shape = np.array([24, 20, 28], np.int64)
+    g_cell = np.array([2.5, 3.0, 2.0])
+    phi = rng.normal(0.0, 50.0, g_shape).astype(np.float32)
+    dens = rng.uniform(0.0, 5.0, g_shape).astype(np.float32)
+    bur = rng.uniform(0.2, 2.0, g.n).astype(np.float32)
+
+    def grid_prep(ref):
+        ref.set_grid(0, g_origin, g_shape, g_cell, phi, 0.7)
+        ref.set_grid(1, g_origin, g_shape, g_cell, dens, 1.3)
+        ref.set_burying(bur)
+
+    ref_case("grid300", g, pg, steps=(1, 5), prep=grid_prep,
+             extra=lambda ref, st: {"ivec": ref.insertion(3, 250)})
+    add_arrays("grid300", grid_origin=g_origin, grid_shape=g_shape, grid_cell=g_cell, grid_phi=phi,
+               grid_density=dens, burying=bur, ivec_ids=np.array([3, 250], np.int64))
+
+    if ONLY and "scans" not in ONLY:
+        return
     # two-particle scans of the reference regression tests
     # (test-spring-energy.cpp:82-99, test-steric-energy.cpp:108-160,
     # test-electrostatic-energy.cpp:78-135): energy of one pair vs distance

@@ -23,7 +23,9 @@ KEYS = ("x", "v", "mass", "charge", "radius", "eps", "sasa", "transfer", "hydro"
 
 
 def params_from(arr):
-    p = _abi.Params()
+    """Params from a stored array; fields added after the array was written
+    keep their defaults."""
+    p = workloads.params(99)
     for (name, ctype), v in zip(_abi.Params._fields_, arr):
         setattr(p, name, int(v) if ctype is not _abi.ctypes.c_float else float(v))
     return p
@@ -36,13 +38,25 @@ def load(name):
     return s, z
 
 
+def prepare(o, z):
+    """Applies the case's rigid body, grids and burying to an oracle or an
+    engine object (same method names)."""
+    if "body_ids" in z:
+        o.add_rigid_body(z["body_ids"])
+    if "grid_phi" in z:
+        o.set_grid(_abi.GRID_POTENTIAL, z["grid_origin"], z["grid_shape"], z["grid_cell"], z["grid_phi"])
+        o.set_grid(_abi.GRID_DENSITY, z["grid_origin"], z["grid_shape"], z["grid_cell"], z["grid_density"])
+        o.set_burying(z["burying"])
+    return o
+
+
 CASES = sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.npz")) if "scans" not in p)
 
 
 @pytest.mark.parametrize("name", CASES)
 def test_forces_energies_bitexact(name):
     s, z = load(name)
-    o = po.Oracle(s)
+    o = prepare(po.Oracle(s), z)
     F, E = o.compute_forces(po.F32)
     dyn = (s.flags & 1) == 0
     assert np.array_equal(F[dyn].astype(np.float32), z["forces"][dyn]), name
@@ -52,7 +66,7 @@ def test_forces_energies_bitexact(name):
 @pytest.mark.parametrize("name", CASES)
 def test_pairs_and_cells_bitexact(name):
     s, z = load(name)
-    o = po.Oracle(s)
+    o = prepare(po.Oracle(s), z)
     for kind in (0, 1, 2):
         if f"pairs_{kind}" not in z:
             continue
@@ -66,7 +80,7 @@ def test_pairs_and_cells_bitexact(name):
 @pytest.mark.parametrize("name", CASES)
 def test_trajectory_bitexact(name):
     s, z = load(name)
-    o = po.Oracle(s)
+    o = prepare(po.Oracle(s), z)
     done = 0
     for st in z["steps"]:
         rc, E = o.step(po.F32, int(st) - done)
@@ -76,6 +90,13 @@ def test_trajectory_bitexact(name):
         assert np.array_equal(x.astype(np.float32), z[f"x_{st}"]), (name, st)
         assert np.array_equal(v.astype(np.float32), z[f"v_{st}"]), (name, st)
         assert np.array_equal(E.astype(np.float32), z[f"E_{st}"]), (name, st, E, z[f"E_{st}"])
+        if f"body_{st}" in z:  # body position and angular velocity
+            b = o.rigid_state(po.F32, 0)
+            assert np.array_equal(b[0:3], z[f"body_{st}"][0:3]), (name, st)
+            assert np.array_equal(b[9:12], z[f"body_{st}"][6:9]), (name, st)
+        if f"ivec_{st}" in z:  # insertion vector: angle, roll angle, depth, vector
+            i1, i2 = z["ivec_ids"]
+            assert np.array_equal(o.insertion(po.F32, int(i1), int(i2)), z[f"ivec_{st}"]), (name, st)
 
 
 def test_variants_bitexact():
@@ -225,8 +246,8 @@ def test_fp64_forces_are_energy_gradients(setup):
 def test_fp64_close_to_float_path(name):
     """FP64 and float-faithful forces agree to float precision: a transposed
     operand or dropped term in either would show up here."""
-    s, _ = load(name)
-    o = po.Oracle(s)
+    s, z = load(name)
+    o = prepare(po.Oracle(s), z)
     F32, E32 = o.compute_forces(po.F32)
     F64, E64 = o.compute_forces(po.F64)
     dyn = (s.flags & 1) == 0
@@ -242,3 +263,83 @@ def test_constants_match_reference_values():
     assert c["spring"] == pytest.approx(9.999992838e-5, rel=1e-9)
     assert c["elec"] == pytest.approx(1.745913244, rel=1e-9)
     assert c["kinetic"] == pytest.approx(1.000000716e4, rel=1e-9)
+
+
+# ---------------------------------------------------------------------------
+# Rigid bodies beyond what a reference run can pin
+# ---------------------------------------------------------------------------
+def _pairwise(x, ids):
+    p = x[ids]
+    return np.linalg.norm(p[:, None, :] - p[None, :, :], axis=-1)
+
+
+@pytest.mark.parametrize("mode", [po.F32, po.F64])
+def test_rigid_motion_is_rigid(mode):
+    """Every intra-body distance is preserved (a wrong quaternion product or a
+    transposed rotation would stretch the body), and the first solve moves
+    the centre of mass by the closed form a dt (v + a dt/2) with v0 = 0."""
+    s, z = load("rigid300")
+    o = prepare(po.Oracle(s), z)
+    ids = z["body_ids"]
+    x0, _ = o.state(mode)
+    d0 = _pairwise(x0, ids)
+    p = s.params.copy()
+    p.rigidbody_enable = 0
+    probe = po.Oracle(s, params=p)
+    probe.add_rigid_body(ids)
+    F, _ = probe.compute_forces(po.F64)
+    M = s.mass[ids].astype(np.float64).sum()
+    a = F[ids].sum(0) / M
+    X0 = o.rigid_state(mode, 0)[0:3]
+    o.step(mode, 1)
+    X1 = o.rigid_state(mode, 0)[0:3]
+    dt = float(s.params.timestep)
+    tol = 1e-5 * np.abs(a).max() * dt * dt + (1e-6 if mode == po.F32 else 1e-12)
+    assert np.allclose(X1 - X0, a * dt * dt * 1.5, rtol=1e-4, atol=tol)
+    o.step(mode, 19)
+    x, _ = o.state(mode)
+    d = _pairwise(x, ids)
+    assert np.abs(d - d0).max() < (2e-4 if mode == po.F32 else 1e-9), np.abs(d - d0).max()
+    assert np.abs(x[ids] - x0[ids]).max() > 1e-3  # it did move
+
+
+def test_two_bodies_move_independently():
+    """Two bodies out of each other's reach evolve as each does alone (checks
+    per-body bookkeeping: ids, local indices, solve order).  Not bit for bit:
+    the shared cell grid changes the order in which pair terms are summed."""
+    a = workloads.generate(0, seed=31, n=150)
+    b = workloads.generate(0, seed=32, n=150)
+    shift = np.array([200.0, 0.0, 0.0], np.float32)
+    n1 = a.n
+    both = workloads.System(
+        x=np.concatenate([a.x, (b.x.reshape(-1, 3) + shift).reshape(-1)]), v=np.concatenate([a.v, b.v]),
+        mass=np.concatenate([a.mass, b.mass]), charge=np.concatenate([a.charge, b.charge]),
+        radius=np.concatenate([a.radius, b.radius]), eps=np.concatenate([a.eps, b.eps]),
+        sasa=np.concatenate([a.sasa, b.sasa]), transfer=np.concatenate([a.transfer, b.transfer]),
+        hydro=np.concatenate([a.hydro, b.hydro]), flags=np.concatenate([a.flags, b.flags]),
+        id1=np.concatenate([a.id1, b.id1 + n1]), id2=np.concatenate([a.id2, b.id2 + n1]),
+        l0=np.concatenate([a.l0, b.l0]), k=np.concatenate([a.k, b.k]))
+    p = a.params.copy()
+    p.rigidbody_enable = 1
+    ia, ib = np.arange(10, 90), np.arange(20, 110)
+    for mode in (po.F32, po.F64):
+        o = po.Oracle(both, params=p)
+        assert o.add_rigid_body(ib + n1) == 1  # second system's body first
+        assert o.add_rigid_body(ia) == 2
+        assert o.add_rigid_body(np.array([15, 500])) == 2  # overlaps body 2: ignored
+        oa, ob = po.Oracle(a, params=p), po.Oracle(b, params=p)
+        oa.add_rigid_body(ia)
+        ob.set_state((b.x.reshape(-1, 3) + shift))
+        ob.add_rigid_body(ib)
+        o.step(mode, 10)
+        oa.step(mode, 10)
+        ob.step(mode, 10)
+        x, _ = o.state(mode)
+        rt = 1e-5 if mode == po.F32 else 1e-10
+        assert np.allclose(x[:n1], oa.state(mode)[0], rtol=0, atol=rt * 100), mode
+        assert np.allclose(x[n1:], ob.state(mode)[0], rtol=0, atol=rt * 300), mode
+        for got, want in ((o.rigid_state(mode, 1), oa.rigid_state(mode, 0)),
+                          (o.rigid_state(mode, 0), ob.rigid_state(mode, 0))):
+            for sl in (slice(0, 3), slice(3, 6), slice(6, 9), slice(9, 12), slice(12, 16)):
+                scale = np.abs(want[sl]).max()
+                assert np.abs(got[sl] - want[sl]).max() <= rt * max(scale, 1e-30), (mode, sl, got, want)
