@@ -28,7 +28,8 @@ EXPORTS = (
     "bs_num_springs", "bs_set_particles", "bs_set_springs", "bs_set_params", "bs_setup",
     "bs_step", "bs_compute_forces", "bs_get_state", "bs_set_state", "bs_get_energies",
     "bs_last_nonfinite_particle", "bs_get_counters", "bs_profile", "bs_get_profile",
-    "bs_get_cells", "bs_get_pairs",
+    "bs_get_cells", "bs_get_pairs", "bs_add_rigid_body", "bs_num_rigid_bodies", "bs_get_rigid_body",
+    "bs_set_grid", "bs_set_burying", "bs_get_insertion_vector",
 )
 PROF_REBUILD, PROF_PAIR, PROF_SPRING = range(3)
 
@@ -138,6 +139,13 @@ def bind(L):
     L.bs_get_profile.argtypes = [_vp, _f64p, _i64p]
     L.bs_get_cells.argtypes = [_vp, ctypes.c_int32, _i64p, _i64p, _i64p, _i32p, _i32p, _i64p, _f64p]
     L.bs_get_pairs.argtypes = [_vp, ctypes.c_int32, _u32p, ctypes.c_int64, _i64p]
+    L.bs_add_rigid_body.argtypes = [_vp, ctypes.c_int64, _i64p]
+    L.bs_num_rigid_bodies.argtypes = [_vp]
+    L.bs_num_rigid_bodies.restype = ctypes.c_int32
+    L.bs_get_rigid_body.argtypes = [_vp, ctypes.c_int32, _f64p]
+    L.bs_set_grid.argtypes = [_vp, ctypes.c_int32, _f64p, _i64p, _f64p, _f32p]
+    L.bs_set_burying.argtypes = [_vp, _f32p]
+    L.bs_get_insertion_vector.argtypes = [_vp, ctypes.c_int64, ctypes.c_int64, _f64p]
     return L
 
 

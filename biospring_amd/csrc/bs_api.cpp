@@ -64,6 +64,28 @@ struct bs_handle
     // internal order
     std::vector<int32_t> perm, iperm;
 
+    // rigid bodies (host copies; the state survives a re-setup)
+    struct BodyHost
+    {
+        std::vector<int64_t> ids; // original ids, body order
+        bool init = false;
+        RigidBody st{};
+        std::vector<double> p0;   // 3 per member, body frame
+    };
+    std::vector<BodyHost> bodies;
+    std::vector<int32_t> body_of; // [n] body index or -1
+    std::vector<float> burying;   // [n] ParticleProperty::_burying
+
+    // field grids (PotentialGrid after compute_gradient), per BS_GRID_*
+    struct GridHost
+    {
+        bool set = false;
+        double origin[3] = {0, 0, 0}, cell[3] = {1, 1, 1};
+        int64_t shape[3] = {0, 0, 0};
+        std::vector<float4> cells; // gradient force vector, scalar
+    };
+    GridHost grids[2];
+
     // device
     float4 *pos[2] = {nullptr, nullptr}, *plo[2] = {nullptr, nullptr};
     float4 *vel = nullptr, *jprm = nullptr, *iprm = nullptr, *xref = nullptr, *fext = nullptr, *fout = nullptr;
@@ -84,6 +106,13 @@ struct bs_handle
     int32_t pair_blocks = 0;
     double *spring_part = nullptr;
     int32_t spring_blocks = 0;
+    RigidBody *bodies_d = nullptr;
+    double4 *rx = nullptr, *rp0 = nullptr, *frig = nullptr;
+    int32_t *slot_p = nullptr, *slot_b = nullptr, *bstart = nullptr;
+    int32_t nr = 0;
+    float4 *grid_d[2] = {nullptr, nullptr};
+    double *cent_part = nullptr, *cent = nullptr;
+    int32_t cent_blocks = 0;
     int32_t group = 16;
     double *energies_d = nullptr;
     int32_t *status = nullptr;   // [0] rebuild, [1] nonfinite id, [2] overflow, [3] max count
@@ -152,6 +181,14 @@ void free_device(bs_handle *h)
     f(h->ex_ptr); f(h->ex_w);
     h->ex_ptr = nullptr;
     h->ex_w = nullptr;
+    f(h->bodies_d); f(h->rx); f(h->rp0); f(h->frig); f(h->slot_p); f(h->slot_b); f(h->bstart);
+    f(h->grid_d[0]); f(h->grid_d[1]); f(h->cent_part); f(h->cent);
+    h->bodies_d = nullptr;
+    h->rx = h->rp0 = h->frig = nullptr;
+    h->slot_p = h->slot_b = h->bstart = nullptr;
+    h->nr = 0;
+    h->grid_d[0] = h->grid_d[1] = nullptr;
+    h->cent_part = h->cent = nullptr;
     for (auto &l : h->lists)
     {
         f(l.cl);
@@ -183,6 +220,12 @@ int check_device(bs_handle *h)
             return fail(h, BS_ERR_STATE, "internal: device buffer missing");
     if (h->p.compensated_positions && (!h->plo[0] || !h->plo[1]))
         return fail(h, BS_ERR_STATE, "internal: compensated-position buffers missing");
+    if (!h->bodies.empty() && (!h->bodies_d || !h->frig || !h->bstart || (h->nr > 0 && (!h->rx || !h->rp0 ||
+                                                                                        !h->slot_p || !h->slot_b))))
+        return fail(h, BS_ERR_STATE, "internal: rigid-body buffers missing");
+    for (int g = 0; g < 2; ++g)
+        if (h->grids[g].set && !h->grid_d[g])
+            return fail(h, BS_ERR_STATE, "internal: grid buffer missing");
     if (h->ncl != (int32_t)((h->n + CLUSTER - 1) / CLUSTER))
         return fail(h, BS_ERR_STATE, "internal: cluster count mismatch");
     for (const auto &l : h->lists)
@@ -242,6 +285,8 @@ uint32_t particle_flags(const bs_handle *h, int64_t i)
         f |= PF_CHARGED;
     if (h->hydro[(size_t)i] != 0.0f)
         f |= PF_HYDRO;
+    if (!h->body_of.empty() && h->body_of[(size_t)i] >= 0)
+        f |= PF_RIGID;
     return f;
 }
 
@@ -482,6 +527,24 @@ void launch_pairs(bs_handle *h)
     }
 }
 
+GridView grid_view(const bs_handle *h, int kind, bool on, float scale)
+{
+    GridView g{};
+    const auto &G = h->grids[kind];
+    if (!on || !G.set)
+        return g;
+    g.cells = h->grid_d[kind];
+    for (int c = 0; c < 3; ++c)
+    {
+        g.origin[c] = G.origin[c];
+        g.cell[c] = G.cell[c];
+        g.bmax[c] = G.origin[c] + (double)G.shape[c] * G.cell[c]; // Box(origin, shape * cell)
+        g.shape[c] = (int32_t)G.shape[c];
+    }
+    g.scale = scale;
+    return g;
+}
+
 void launch_springs(bs_handle *h, bool integrate)
 {
     SpringArgs a{};
@@ -519,7 +582,40 @@ void launch_springs(bs_handle *h, bool integrate)
     a.half_skin2 = (float)(hs * hs * (1.0 - 1e-5));
     a.integrate = integrate ? 1 : 0;
     a.k = h->k;
+    a.frig = h->frig;
+    // the electrostatic field needs electrostatic.enable (SpringNetwork.cpp:107-113)
+    a.gpot = grid_view(h, BS_GRID_POTENTIAL, h->p.electrostatic_enable && h->p.potentialgrid_enable,
+                       h->p.potentialgrid_scale);
+    a.gden = grid_view(h, BS_GRID_DENSITY, h->p.densitygrid_enable, h->p.densitygrid_scale);
+    a.e_field = 1.3806504E-23 * 1.0E-3 * 6.02214169E23; // k_B, J -> kJ, N_A (constants.hpp:70-73)
     launch_spring(a, h->group, h->spring_blocks, h->stream);
+}
+
+void launch_bodies(bs_handle *h)
+{
+    if (h->bodies.empty())
+        return;
+    RigidArgs a{};
+    a.pos = h->pos[h->cur];
+    a.pos_out = h->pos[h->cur ^ 1];
+    a.vel = h->vel;
+    a.frig = h->frig;
+    a.rx = h->rx;
+    a.p0 = h->rp0;
+    a.slot_p = h->slot_p;
+    a.slot_b = h->slot_b;
+    a.bstart = h->bstart;
+    a.bodies = h->bodies_d;
+    a.xref = h->xref;
+    a.perm = h->perm_d;
+    a.status = h->status;
+    a.nr = h->nr;
+    a.nb = (int32_t)h->bodies.size();
+    a.solve = h->p.rigidbody_enable;
+    a.dt = (double)h->p.timestep;
+    const double hs = 0.5 * std::max(h->p.list_skin, 0.0f);
+    a.half_skin2 = (float)(hs * hs * (1.0 - 1e-5));
+    launch_rigid(a, h->stream);
 }
 
 // Records an event on the handle's stream and returns its pool index.
@@ -643,7 +739,7 @@ int do_setup(bs_handle *h)
         const int64_t i = h->perm[(size_t)t];
         jp[(size_t)t] = jprm_of(h, i);
         ip[(size_t)t] = make_float4(h->mass[(size_t)i], h->sasa[(size_t)i], h->transfer[(size_t)i],
-                                    h->radius[(size_t)i]);
+                                    h->burying.empty() ? 1.0f : h->burying[(size_t)i]);
     }
     HIPCHK(h, hipMemcpyAsync(h->jprm, jp.data(), sizeof(float4) * n, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(h->iprm, ip.data(), sizeof(float4) * n, hipMemcpyHostToDevice, h->stream));
@@ -692,20 +788,123 @@ int do_setup(bs_handle *h)
             el[(size_t)t] = edges[(size_t)t].l0;
         }
     }
+    // The force CSR leaves out springs between two rigid particles
+    // (RigidBody::UpdateSpringsState, RigidBody.cpp:42-53; Spring.cpp:24);
+    // the exclusions below keep every spring.
+    std::vector<int32_t> fptr, fej;
+    std::vector<float> fek, fel;
+    const bool any_rigid = !h->bodies.empty();
+    if (any_rigid)
+    {
+        fptr.assign((size_t)n + 1, 0);
+        for (int64_t i = 0; i < n; ++i)
+        {
+            const bool ri = h->body_of[(size_t)h->perm[(size_t)i]] >= 0;
+            for (int32_t t = ptr[(size_t)i]; t < ptr[(size_t)i + 1]; ++t)
+            {
+                const int32_t j = ej[(size_t)t] & 0x7fffffff;
+                if (ri && h->body_of[(size_t)h->perm[(size_t)j]] >= 0)
+                    continue;
+                fej.push_back(ej[(size_t)t]);
+                fek.push_back(ek[(size_t)t]);
+                fel.push_back(el[(size_t)t]);
+            }
+            fptr[(size_t)i + 1] = (int32_t)fej.size();
+        }
+    }
+    const std::vector<int32_t> &up_ptr = any_rigid ? fptr : ptr, &up_j = any_rigid ? fej : ej;
+    const std::vector<float> &up_k = any_rigid ? fek : ek, &up_l = any_rigid ? fel : el;
+    h->nedges = up_ptr[(size_t)n];
     ALLOC(h->sp_ptr, int32_t, n + 1);
     ALLOC(h->sp_j, int32_t, h->nedges);
     ALLOC(h->sp_k, float, h->nedges);
     ALLOC(h->sp_l0, float, h->nedges);
-    HIPCHK(h, hipMemcpyAsync(h->sp_ptr, ptr.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->sp_ptr, up_ptr.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, h->stream));
     if (h->nedges)
     {
-        HIPCHK(h, hipMemcpyAsync(h->sp_j, ej.data(), sizeof(int32_t) * h->nedges, hipMemcpyHostToDevice, h->stream));
-        HIPCHK(h, hipMemcpyAsync(h->sp_k, ek.data(), sizeof(float) * h->nedges, hipMemcpyHostToDevice, h->stream));
-        HIPCHK(h, hipMemcpyAsync(h->sp_l0, el.data(), sizeof(float) * h->nedges, hipMemcpyHostToDevice, h->stream));
+        HIPCHK(h, hipMemcpyAsync(h->sp_j, up_j.data(), sizeof(int32_t) * h->nedges, hipMemcpyHostToDevice, h->stream));
+        HIPCHK(h, hipMemcpyAsync(h->sp_k, up_k.data(), sizeof(float) * h->nedges, hipMemcpyHostToDevice, h->stream));
+        HIPCHK(h, hipMemcpyAsync(h->sp_l0, up_l.data(), sizeof(float) * h->nedges, hipMemcpyHostToDevice, h->stream));
     }
     int rc = upload_positions(h);
     if (rc)
         return rc;
+
+    // rigid bodies: state from the current positions on first use
+    // (RigidBody::init, RigidBody.cpp:57-124), slots for their dynamic members
+    if (!h->bodies.empty())
+    {
+        const int32_t nb = (int32_t)h->bodies.size();
+        std::vector<RigidBody> st((size_t)nb);
+        std::vector<int32_t> sp, sb, bs((size_t)nb + 1, 0);
+        std::vector<double4> p0v, rxv;
+        for (int32_t b = 0; b < nb; ++b)
+        {
+            auto &B = h->bodies[(size_t)b];
+            if (!B.init)
+            {
+                RigidBody r{};
+                double M = 0.0, X[3] = {0, 0, 0};
+                for (int64_t id : B.ids)
+                    M += h->mass[(size_t)id];
+                for (int64_t id : B.ids)
+                    for (int c = 0; c < 3; ++c)
+                        X[c] += (double)h->x[3 * id + c] * h->mass[(size_t)id];
+                r.M = M;
+                for (int c = 0; c < 3; ++c)
+                    r.X[c] = X[c] / M;
+                const double ib = (2.0 / 5.0) * M; // I_body = 2/5 M dmax^2 with dmax = 1
+                r.iinv[0] = r.iinv[4] = r.iinv[8] = 1.0 / ib;
+                r.q[3] = 1.0;
+                B.p0.resize(3 * B.ids.size());
+                for (size_t t = 0; t < B.ids.size(); ++t)
+                    for (int c = 0; c < 3; ++c)
+                        B.p0[3 * t + c] = (double)h->x[3 * B.ids[t] + c] - r.X[c];
+                B.st = r;
+                B.init = true;
+            }
+            st[(size_t)b] = B.st;
+            bs[(size_t)b] = (int32_t)sp.size();
+            for (size_t t = 0; t < B.ids.size(); ++t)
+            {
+                const int64_t id = B.ids[t];
+                if (h->flags[(size_t)id] & BS_FLAG_STATIC)
+                    continue;
+                sp.push_back(h->iperm[(size_t)id]);
+                sb.push_back(b);
+                p0v.push_back(make_double4(B.p0[3 * t], B.p0[3 * t + 1], B.p0[3 * t + 2], 0.0));
+                rxv.push_back(make_double4(h->x[3 * id], h->x[3 * id + 1], h->x[3 * id + 2], 0.0));
+            }
+        }
+        bs[(size_t)nb] = (int32_t)sp.size();
+        h->nr = (int32_t)sp.size();
+        ALLOC(h->bodies_d, RigidBody, nb);
+        ALLOC(h->bstart, int32_t, nb + 1);
+        ALLOC(h->frig, double4, n);
+        ALLOC(h->slot_p, int32_t, h->nr);
+        ALLOC(h->slot_b, int32_t, h->nr);
+        ALLOC(h->rp0, double4, h->nr);
+        ALLOC(h->rx, double4, h->nr);
+        HIPCHK(h, hipMemcpy(h->bodies_d, st.data(), sizeof(RigidBody) * nb, hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemcpy(h->bstart, bs.data(), sizeof(int32_t) * (nb + 1), hipMemcpyHostToDevice));
+        if (h->nr > 0)
+        {
+            HIPCHK(h, hipMemcpy(h->slot_p, sp.data(), sizeof(int32_t) * h->nr, hipMemcpyHostToDevice));
+            HIPCHK(h, hipMemcpy(h->slot_b, sb.data(), sizeof(int32_t) * h->nr, hipMemcpyHostToDevice));
+            HIPCHK(h, hipMemcpy(h->rp0, p0v.data(), sizeof(double4) * h->nr, hipMemcpyHostToDevice));
+            HIPCHK(h, hipMemcpy(h->rx, rxv.data(), sizeof(double4) * h->nr, hipMemcpyHostToDevice));
+        }
+    }
+    for (int g = 0; g < 2; ++g)
+        if (h->grids[g].set)
+        {
+            const auto &G = h->grids[g];
+            ALLOC(h->grid_d[g], float4, G.cells.size());
+            HIPCHK(h, hipMemcpy(h->grid_d[g], G.cells.data(), sizeof(float4) * G.cells.size(), hipMemcpyHostToDevice));
+        }
+    h->cent_blocks = (int32_t)std::min<int64_t>(1024, (n + BLOCK - 1) / BLOCK);
+    ALLOC(h->cent_part, double, 3 * h->cent_blocks);
+    ALLOC(h->cent, double, 3);
 
     // status / counters
     const int32_t st0[4] = {1, INT_MAX, 0, 0}, ct0[4] = {0, 0, 0, 0};
@@ -716,7 +915,7 @@ int do_setup(bs_handle *h)
     h->ncl = (int32_t)((n + CLUSTER - 1) / CLUSTER);
     h->pair_blocks = (h->ncl + BLOCK / 64 - 1) / (BLOCK / 64);
     h->spring_blocks = (int32_t)((n * h->group + BLOCK - 1) / BLOCK);
-    ALLOC(h->spring_part, double, 3 * h->spring_blocks);
+    ALLOC(h->spring_part, double, 4 * h->spring_blocks);
 
     // static exclusions per i-cluster: the union of its 8 spring rows as
     // sorted (j, bonded-mask) words; springs never change during a run
@@ -807,6 +1006,26 @@ int ensure_ready(bs_handle *h)
 }
 
 int kind_term(int kind) { return kind == BS_NS_ELECTROSTATIC ? TERM_ELEC : kind == BS_NS_HYDROPHOBIC ? TERM_HYDRO : TERM_STERIC; }
+
+// Before a re-setup: the device state (positions, velocities, body states)
+// back into the host copies, then the device buffers released.
+int pull_and_release(bs_handle *h)
+{
+    if (!h->ready)
+        return BS_OK;
+    int rc = bs_get_state(h, h->x.data(), h->v.data());
+    if (rc)
+        return rc;
+    if (!h->bodies.empty() && h->bodies_d)
+    {
+        std::vector<RigidBody> st(h->bodies.size());
+        HIPCHK(h, hipMemcpy(st.data(), h->bodies_d, sizeof(RigidBody) * st.size(), hipMemcpyDeviceToHost));
+        for (size_t b = 0; b < st.size(); ++b)
+            h->bodies[b].st = st[b];
+    }
+    free_device(h);
+    return BS_OK;
+}
 
 } // namespace
 
@@ -904,6 +1123,9 @@ int bs_set_particles(bs_handle *h, int64_t n, const float *x3, const float *v3, 
     h->flags.assign((size_t)n, 0);
     if (flags)
         std::memcpy(h->flags.data(), flags, (size_t)n);
+    h->bodies.clear();
+    h->body_of.assign((size_t)n, -1);
+    h->burying.assign((size_t)n, 1.0f);
     h->s1.clear();
     h->s2.clear();
     h->sl0.clear();
@@ -956,14 +1178,10 @@ int bs_set_params(bs_handle *h, const bs_params *p)
 {
     if (!h || !p)
         return BS_ERR_ARG;
-    if (h->ready)
-    {
-        // keep the current state, then rebuild lazily with the new parameters
-        int rc = bs_get_state(h, h->x.data(), h->v.data());
-        if (rc)
-            return rc;
-        free_device(h);
-    }
+    // keep the current state, then rebuild lazily with the new parameters
+    int rc = pull_and_release(h);
+    if (rc)
+        return rc;
     h->p = *p;
     return BS_OK;
 }
@@ -1003,6 +1221,7 @@ int bs_step(bs_handle *h, int32_t nsteps, const float *ext_force3)
             launch_pairs(h);
         const size_t e2 = h->prof ? prof_mark(h) : SIZE_MAX;
         launch_springs(h, true);
+        launch_bodies(h);
         if (h->prof)
         {
             const size_t e3 = prof_mark(h);
@@ -1331,6 +1550,200 @@ int bs_get_pairs(bs_handle *h, int32_t kind, uint32_t *ij, int64_t cap, int64_t 
             ij[2 * t] = (uint32_t)(keys[(size_t)t] >> 32);
             ij[2 * t + 1] = (uint32_t)keys[(size_t)t];
         }
+    return BS_OK;
+}
+
+int bs_add_rigid_body(bs_handle *h, int64_t n, const int64_t *ids)
+{
+    if (!h)
+        return BS_ERR_ARG;
+    if (!h->have_particles)
+        return fail(h, BS_ERR_STATE, "bs_add_rigid_body: particles must be set first");
+    if (n < 0 || (n > 0 && !ids))
+        return fail(h, BS_ERR_ARG, "bs_add_rigid_body: bad id list");
+    std::vector<int64_t> v(ids, ids + n);
+    for (int64_t id : v)
+        if (id < 0 || id >= h->n)
+            return fail(h, BS_ERR_ARG, "bs_add_rigid_body: particle index %lld out of range", (long long)id);
+    {
+        std::vector<int64_t> u(v);
+        std::sort(u.begin(), u.end());
+        if (std::adjacent_find(u.begin(), u.end()) != u.end())
+            return fail(h, BS_ERR_ARG, "bs_add_rigid_body: repeated particle index");
+    }
+    int rc = pull_and_release(h);
+    if (rc)
+        return rc;
+    if (n == 0)
+    { // RigidBodiesManager::InitRigidBodies with no ids clears the collection
+        h->bodies.clear();
+        h->body_of.assign((size_t)h->n, -1);
+        return BS_OK;
+    }
+    for (int64_t id : v)
+        if (h->body_of[(size_t)id] >= 0)
+            return BS_OK; // already rigid: no new body (RigidBodiesManager.cpp:19-38)
+    const int32_t b = (int32_t)h->bodies.size();
+    h->bodies.emplace_back();
+    h->bodies.back().ids = v;
+    for (int64_t id : v)
+        h->body_of[(size_t)id] = b;
+    return BS_OK;
+}
+
+int32_t bs_num_rigid_bodies(const bs_handle *h) { return h ? (int32_t)h->bodies.size() : 0; }
+
+int bs_get_rigid_body(bs_handle *h, int32_t b, double *out16)
+{
+    if (!h || !out16)
+        return BS_ERR_ARG;
+    if (b < 0 || b >= (int32_t)h->bodies.size())
+        return fail(h, BS_ERR_ARG, "bs_get_rigid_body: no body %d", b);
+    int rc = ensure_ready(h);
+    if (rc)
+        return rc;
+    RigidBody r;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipMemcpy(&r, h->bodies_d + b, sizeof r, hipMemcpyDeviceToHost));
+    for (int c = 0; c < 3; ++c)
+    {
+        out16[c] = r.X[c];
+        out16[3 + c] = r.v[c];
+        out16[6 + c] = r.L[c];
+        out16[9 + c] = r.w[c];
+    }
+    for (int c = 0; c < 4; ++c)
+        out16[12 + c] = r.q[c];
+    return BS_OK;
+}
+
+int bs_set_grid(bs_handle *h, int32_t kind, const double *origin, const int64_t *shape, const double *cell,
+                const float *scalar)
+{
+    if (!h)
+        return BS_ERR_ARG;
+    if (kind != BS_GRID_POTENTIAL && kind != BS_GRID_DENSITY)
+        return fail(h, BS_ERR_ARG, "bs_set_grid: unknown grid kind %d", kind);
+    if (scalar && (!origin || !shape || !cell))
+        return fail(h, BS_ERR_ARG, "bs_set_grid: origin, shape and cell are required");
+    if (scalar)
+        for (int c = 0; c < 3; ++c)
+            if (shape[c] <= 0 || shape[c] > (1 << 20) || !(cell[c] > 0.0))
+                return fail(h, BS_ERR_ARG, "bs_set_grid: bad shape or cell size");
+    if (scalar && shape[0] * shape[1] * shape[2] > (int64_t)1 << 31)
+        return fail(h, BS_ERR_ARG, "bs_set_grid: grid too large");
+    int rc = pull_and_release(h);
+    if (rc)
+        return rc;
+    auto &G = h->grids[kind];
+    G = bs_handle::GridHost{};
+    if (!scalar)
+        return BS_OK;
+    const int64_t nx = shape[0], ny = shape[1], nz = shape[2];
+    G.cells.resize((size_t)(nx * ny * nz));
+    // PotentialGrid::compute_gradient (PotentialGrid.cpp:20-60): central
+    // differences inside, zero on the faces, times -k_B * m->A * N->Da A fs^-2
+    const double newton = 1.0E10 * (1.0 / 1.66054E-27) * 1.0E-15 * 1.0E-15;
+    const double scale = 1.3806504E-23 * 1.0E10 * newton;
+    auto at = [&](int64_t i, int64_t j, int64_t k) { return (double)scalar[(i * ny + j) * nz + k]; };
+    for (int64_t i = 0; i < nx; ++i)
+        for (int64_t j = 0; j < ny; ++j)
+            for (int64_t k = 0; k < nz; ++k)
+            {
+                double g[3] = {0.0, 0.0, 0.0};
+                if (i > 0 && i < nx - 1)
+                    g[0] = (at(i + 1, j, k) - at(i - 1, j, k)) / (2.0 * cell[0]);
+                if (j > 0 && j < ny - 1)
+                    g[1] = (at(i, j + 1, k) - at(i, j - 1, k)) / (2.0 * cell[1]);
+                if (k > 0 && k < nz - 1)
+                    g[2] = (at(i, j, k + 1) - at(i, j, k - 1)) / (2.0 * cell[2]);
+                G.cells[(size_t)((i * ny + j) * nz + k)] =
+                    make_float4((float)(-g[0] * scale), (float)(-g[1] * scale), (float)(-g[2] * scale),
+                                (float)at(i, j, k));
+            }
+    for (int c = 0; c < 3; ++c)
+    {
+        G.origin[c] = origin[c];
+        G.cell[c] = cell[c];
+        G.shape[c] = shape[c];
+    }
+    G.set = true;
+    return BS_OK;
+}
+
+int bs_set_burying(bs_handle *h, const float *burying)
+{
+    if (!h)
+        return BS_ERR_ARG;
+    if (!h->have_particles)
+        return fail(h, BS_ERR_STATE, "bs_set_burying: particles must be set first");
+    int rc = pull_and_release(h);
+    if (rc)
+        return rc;
+    h->burying.assign((size_t)h->n, 1.0f);
+    if (burying)
+        std::memcpy(h->burying.data(), burying, sizeof(float) * h->n);
+    return BS_OK;
+}
+
+int bs_get_insertion_vector(bs_handle *h, int64_t i1, int64_t i2, double *out6)
+{
+    if (!h || !out6)
+        return BS_ERR_ARG;
+    if (i1 < 0 || i2 < 0 || i1 >= h->n || i2 >= h->n)
+        return fail(h, BS_ERR_ARG, "Insertion vector could not be initialized. One or both of the pair of "
+                                   "particles defining the insertion vector have an invalid identifier.");
+    int rc = ensure_ready(h);
+    if (rc)
+        return rc;
+    launch_centroid(h->pos[h->cur], (int32_t)h->n, h->cent_part, h->cent_blocks, h->cent, h->stream);
+    double cs[3];
+    float4 q1, q2;
+    HIPCHK(h, hipMemcpyAsync(cs, h->cent, sizeof cs, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(&q1, h->pos[h->cur] + h->iperm[(size_t)i1], sizeof q1, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(&q2, h->pos[h->cur] + h->iperm[(size_t)i2], sizeof q2, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    // InsertionVector::computeVector / computeAngle / computeRollAngle
+    // (InsertionVector.cpp:18-78), in FP64; the projection uses the
+    // term-by-term vector products the reference writes.
+    const double p1[3] = {q1.x, q1.y, q1.z}, p2[3] = {q2.x, q2.y, q2.z};
+    const double v[3] = {p2[0] - p1[0], p2[1] - p1[1], p2[2] - p1[2]};
+    auto nrm = [](const double *a) { return std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]); };
+    auto unit = [&](double *a) {
+        const double l = nrm(a);
+        for (int c = 0; c < 3; ++c)
+            a[c] = l > 1e-40 ? a[c] / l : 0.0;
+    };
+    const double vn = nrm(v);
+    out6[0] = std::acos(-v[2] / vn) * (180.0 / M_PI) - 90.0;
+    double proj[3], n1[3], n3[3], ref[3], n2[3];
+    for (int c = 0; c < 3; ++c)
+        proj[c] = p1[c] + v[c] * (((cs[c] - p1[c]) * v[c]) / (vn * vn));
+    for (int c = 0; c < 3; ++c)
+        n1[c] = proj[c] - cs[c];
+    unit(n1);
+    n3[0] = n1[1] * v[2] - n1[2] * v[1];
+    n3[1] = n1[2] * v[0] - n1[0] * v[2];
+    n3[2] = n1[0] * v[1] - n1[1] * v[0];
+    unit(n3);
+    const double am = std::atan2(n3[2], n1[2]) - M_PI;
+    const double d[3] = {cs[0] - proj[0], cs[1] - proj[1], cs[2] - proj[2]};
+    const double r = nrm(d);
+    for (int c = 0; c < 3; ++c)
+        ref[c] = proj[c] + n1[c] * r * std::cos(am) + n3[c] * r * std::sin(am);
+    for (int c = 0; c < 3; ++c)
+        n2[c] = proj[c] - ref[c];
+    unit(n2);
+    const double dot12 = n1[0] * n2[0] + n1[1] * n2[1] + n1[2] * n2[2];
+    const double arad = std::acos(dot12 / (nrm(n1) * nrm(n2)));
+    const double c12[3] = {n1[1] * n2[2] - n1[2] * n2[1], n1[2] * n2[0] - n1[0] * n2[2],
+                           n1[0] * n2[1] - n1[1] * n2[0]};
+    const double sgn = c12[0] * (p2[0] - proj[0]) + c12[1] * (p2[1] - proj[1]) + c12[2] * (p2[2] - proj[2]);
+    const double adeg = arad * 180.0 / M_PI;
+    out6[1] = sgn > 0.0 ? 360.0 - adeg : adeg;
+    out6[2] = cs[2];
+    for (int c = 0; c < 3; ++c)
+        out6[3 + c] = v[c];
     return BS_OK;
 }
 

@@ -7,7 +7,7 @@
 //   vel      float4[N]  vx, vy, vz, 0
 //   fnb      double4[N] non-bonded force of the current step (FP64 sums)
 //   jprm     float4[N]  sqrt(radius), sqrt(eps), hydrophobicity, flags (bits)
-//   iprm     float4[N]  mass, sasa, transfer, radius
+//   iprm     float4[N]  mass, sasa, transfer, burying
 //   xref     float4[N]  positions at the last neighbour-list build
 //   springs  CSR over particles, both directions, each row sorted by j:
 //            sp_ptr int32[N+1], sp_j int32[E] (bit31 = j static), sp_k float[E],
@@ -15,6 +15,12 @@
 //   lists    per i-cluster of 8 particles: cl[c * cap + t] = j | mask << 24,
 //            cln[c] entries, zero words up to the next multiple of 16
 //   ex       static exclusions per i-cluster: ex_ptr[c], ex_w = j << 8 | mask
+//   grids    per field, float4 per cell: gradient force vector (x, y, z) and
+//            the scalar, x-major with z fastest (DenseGrid::_data[i][j][k])
+//   rigid    per dynamic rigid particle ("slot", grouped by body in body
+//            order): slot_p int32, slot_b int32, p0 double4 (body frame),
+//            rx double4 (FP64 position); bodies RigidBody[nb]; frig double4[N]
+//            total force of rigid particles in the current step
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -94,6 +100,44 @@ struct PairArgs
     Consts k;
 };
 
+// Rigid body state (RigidBody.h:156-183) in FP64; q = (x, y, z, w).
+struct RigidBody
+{
+    double M;
+    double iinv[9];
+    double X[3], v[3], L[3], w[3], q[4];
+};
+
+// A field grid as the kernels read it (PotentialGrid after compute_gradient).
+struct GridView
+{
+    const float4 *cells; // nullptr: field off
+    double origin[3], bmax[3], cell[3];
+    int32_t shape[3];
+    float scale;
+};
+
+struct RigidArgs
+{
+    const float4 *pos;      // positions at the start of the step (charge in .w)
+    float4 *pos_out;
+    float4 *vel;
+    const double4 *frig;    // [N] total force of each rigid particle
+    double4 *rx;            // [nr] FP64 position of each slot
+    const double4 *p0;      // [nr] offset in the body frame
+    const int32_t *slot_p;  // [nr] internal particle index
+    const int32_t *slot_b;  // [nr] body index
+    const int32_t *bstart;  // [nb + 1] slot range of each body
+    RigidBody *bodies;      // [nb]
+    const float4 *xref;
+    const int32_t *perm;
+    int32_t *status;
+    int32_t nr, nb;
+    int32_t solve;          // rigidbody.enable: aggregate and solve before placing
+    double dt;
+    float half_skin2;
+};
+
 struct SpringArgs
 {
     const float4 *pos;
@@ -112,7 +156,10 @@ struct SpringArgs
     const float *sp_l0;
     const int32_t *perm;
     float4 *fout;        // forces-only mode
-    double *epart;       // [nblocks * 3] spring, kinetic, imp
+    double4 *frig;       // integrate mode: total force of rigid particles (not integrated here)
+    GridView gpot, gden; // electrostatic potential field, density field
+    double e_field;      // potential * charge -> kJ/mol (electrostatic_field.hpp:15-23)
+    double *epart;       // [nblocks * 4] spring, kinetic, imp, field
     int32_t *status;     // [0] rebuild flag, [1] nonfinite min id, [2] overflow
     int32_t n;
     int32_t spring_enable;
@@ -169,6 +216,8 @@ void launch_prestep(int32_t *status, int32_t *step_counter, int32_t interval, hi
 void launch_finish_rebuild(const float4 *pos, float4 *xref, int32_t n, int32_t *status, int32_t *counters,
                            hipStream_t s);
 void launch_clear_rebuild(int32_t *status, int32_t *counters, hipStream_t s);
+void launch_rigid(const RigidArgs &a, hipStream_t s);
+void launch_centroid(const float4 *pos, int32_t n, double *partial, int32_t nblocks, double *out3, hipStream_t s);
 void launch_fill_i32(int32_t *p, int32_t v, int64_t n, const int32_t *gate, hipStream_t s);
 void launch_pairs_extract(const float4 *pos, const float4 *jprm, const uint32_t *cl, const int32_t *cln,
                           const int32_t *perm, int32_t n, int32_t ncl, int32_t lcap, float rc, float rc2lo,

@@ -228,26 +228,29 @@ __global__ void k_cell_sort(const int32_t *cell_of, int32_t n, const int32_t *st
 }
 
 // ---------------------------------------------------------------------------
-// IMPALA (imp.hpp:29-159), FP32 geometry with FP64 energy/derivative, as the
-// reference; returns the energy and adds the force (ForceField.cpp:80-86).
+// IMPALA (imp.hpp:29-159, ForceField.cpp:62-86) in FP64: returns the energy
+// and adds the force (the negated gradient vector times the scale) to f.
+// The curved geometry z' = sqrt(x^2 + (z - c)^2) - R cancels two numbers of
+// the size of the tube radius, so it is formed in double.
 // ---------------------------------------------------------------------------
-__device__ inline double imp_eval(float x, float z, float sasa, float tr, float uo, float lo, float uc, float lc,
-                                  float scale, double c_spring, float f[3])
+__device__ inline double imp_eval(float xf, float zf, float sasa, float tr, float uo, float lo, float uc, float lc,
+                                  float scale, double c_spring, double f[3])
 {
-    const float alpha = 1.99f, z0 = 15.75f, alip = -0.018f;
+    const double alpha = (double)1.99f, z0 = 15.75, alip = (double)-0.018f;
     const bool flat = uo == 0.0f && lo == 0.0f && uc == 0.0f && lc == 0.0f;
-    const float ru = uc == 0.0f ? 1000000.0f : fabsf(1.0f / uc);
-    const float rl = lc == 0.0f ? 1000000.0f : fabsf(1.0f / lc);
+    const double x = xf, z = zf;
+    const double ru = uc == 0.0f ? 1000000.0 : fabs(1.0 / (double)uc);
+    const double rl = lc == 0.0f ? 1000000.0 : fabs(1.0 / (double)lc);
     const int su = (uc > 0.0f) - (uc < 0.0f), sl = (lc > 0.0f) - (lc < 0.0f);
-    const float cu = uo - su * ru, cl = -lo - sl * rl;
-    const float vu[3] = {x, 0.0f, z - cu}, vl[3] = {x, 0.0f, z - cl};
-    const float nu = sqrtf(vu[0] * vu[0] + vu[2] * vu[2]);
-    const float nl = sqrtf(vl[0] * vl[0] + vl[2] * vl[2]);
-    const float zu = su == 0 ? z : (z > cu ? su * nu + uo - ru : -su * nu + uo - ru);
-    const float zl = sl == 0 ? z : (z > cl ? sl * nl - lo - rl : -sl * nl - lo - rl);
-    const double wu = (double)zu - uo, wl = (double)zl + lo;
+    const double cu = (double)uo - su * ru, cl = -(double)lo - sl * rl;
+    const double vu0 = x, vu2 = z - cu, vl0 = x, vl2 = z - cl;
+    const double nu = sqrt(vu0 * vu0 + vu2 * vu2);
+    const double nl = sqrt(vl0 * vl0 + vl2 * vl2);
+    const double zu = su == 0 ? z : (z > cu ? su * nu + uo - ru : -su * nu + uo - ru);
+    const double zl = sl == 0 ? z : (z > cl ? sl * nl - lo - rl : -sl * nl - lo - rl);
+    const double wu = zu - uo, wl = zl + lo;
     const double xu = exp(alpha * (fabs(wu) - z0)), xl = exp(alpha * (fabs(wl) - z0));
-    const double pre = (double)(-sasa * tr) + (double)(alip * sasa);
+    const double pre = -(double)sasa * tr + alip * sasa;
     const double eu = pre * (0.5 - 1.0 / (1.0 + xu));
     const double el = pre * (0.5 - 1.0 / (1.0 + xl));
     double dcu = alpha * wu * xu / ((xu + 1.0) * (xu + 1.0) * fabs(wu));
@@ -256,24 +259,22 @@ __device__ inline double imp_eval(float x, float z, float sasa, float tr, float 
         dcu = 0.0;
     if (!isfinite(dcl))
         dcl = 0.0;
-    float du[3] = {0.0f, 0.0f, 1.0f}, dl[3] = {0.0f, 0.0f, 1.0f};
+    double du[3] = {0.0, 0.0, 1.0}, dl[3] = {0.0, 0.0, 1.0};
     if (su != 0)
     {
-        const float inv = nu > 0.0f ? 1.0f / nu : 0.0f;
-        du[0] = vu[0] * inv;
-        du[1] = 0.0f;
-        du[2] = vu[2] * inv;
+        const double inv = nu > 1e-40 ? 1.0 / nu : 0.0;
+        du[0] = vu0 * inv;
+        du[2] = vu2 * inv;
     }
     if (sl != 0)
     {
-        const float inv = nl > 0.0f ? 1.0f / nl : 0.0f;
-        dl[0] = vl[0] * inv;
-        dl[1] = 0.0f;
-        dl[2] = vl[2] * inv;
+        const double inv = nl > 1e-40 ? 1.0 / nl : 0.0;
+        dl[0] = vl0 * inv;
+        dl[2] = vl2 * inv;
     }
-    const float mu = (float)(pre * dcu * c_spring), ml = (float)(pre * dcl * c_spring);
+    const double mu = pre * dcu * c_spring, ml = pre * dcl * c_spring;
     for (int c = 0; c < 3; ++c)
-        f[c] -= scale * (flat ? du[c] * mu : du[c] * mu + dl[c] * ml);
+        f[c] -= (double)scale * (flat ? du[c] * mu : du[c] * mu + dl[c] * ml);
     return (double)scale * (flat ? eu : eu + el);
 }
 
@@ -312,6 +313,23 @@ __device__ inline void spring_edge(float4 pi, float4 li, float4 pj, const float4
     e += (jstatic ? 1.0 : 0.5) * 0.5 * (double)scale * (double)k * ext * ext;
 }
 
+// Cell of a field grid holding a position, -1 off the grid
+// (GridCoordinatesSystem.hpp:85-93,206-218: off when x < min or x > max - 1e-6).
+__device__ inline int64_t grid_cell(const GridView &g, float x, float y, float z)
+{
+    const double p[3] = {x, y, z};
+    int64_t q[3];
+    for (int c = 0; c < 3; ++c)
+    {
+        if (!(p[c] >= g.origin[c]) || p[c] > g.bmax[c] - 1e-6)
+            return -1;
+        q[c] = (int64_t)(int32_t)((p[c] - g.origin[c]) / g.cell[c]);
+        if (q[c] >= g.shape[c])
+            return -1;
+    }
+    return (q[0] * g.shape[1] + q[1]) * g.shape[2] + q[2];
+}
+
 // ---------------------------------------------------------------------------
 // Springs (CSR gather, G lanes per row, 4 edges in flight per lane) fused with
 // viscosity, external and IMPALA forces and the viscous semi-implicit Euler
@@ -324,7 +342,7 @@ __global__ void __launch_bounds__(BLOCK) k_spring(SpringArgs a)
     const int32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     const int32_t row = tid / G;
     const int32_t lane = tid % G;
-    double espr = 0.0, eke = 0.0, eimp = 0.0;
+    double espr = 0.0, eke = 0.0, eimp = 0.0, efld = 0.0;
     double fx = 0.0, fy = 0.0, fz = 0.0;
     const bool active = row < a.n;
     float4 pi = make_float4(0.f, 0.f, 0.f, 0.f), li = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -391,8 +409,45 @@ __global__ void __launch_bounds__(BLOCK) k_spring(SpringArgs a)
             sy += fe.y;
             sz += fe.z;
         }
-        // the conservative sum is rounded once; the viscous and IMPALA terms
-        // follow in FP32 as in the reference (Particle.cpp:80-84, 259-267)
+        const float4 ip = a.iprm[row];
+        // field forces: nearest-lower-cell lookup, no interpolation
+        // (Particle.cpp:136-189); off-grid particles get nothing
+        if (a.gpot.cells)
+        {
+            const int64_t c = grid_cell(a.gpot, pi.x, pi.y, pi.z);
+            if (c >= 0)
+            {
+                const float4 g = a.gpot.cells[c];
+                const double qs = (double)pi.w * (double)a.gpot.scale;
+                sx += (double)g.x * qs;
+                sy += (double)g.y * qs;
+                sz += (double)g.z * qs;
+                efld += (double)g.w * qs * a.e_field;
+            }
+        }
+        if (a.gden.cells)
+        {
+            const int64_t c = grid_cell(a.gden, pi.x, pi.y, pi.z);
+            if (c >= 0)
+            {
+                const float4 g = a.gden.cells[c];
+                const double bs = (double)ip.w * (double)a.gden.scale;
+                sx += (double)g.x * bs;
+                sy += (double)g.y * bs;
+                sz += (double)g.z * bs;
+            }
+        }
+        if (a.imp_enable)
+        {
+            double f[3] = {sx, sy, sz};
+            eimp += imp_eval(pi.x, pi.z, ip.y, ip.z, a.imp_uo, a.imp_lo, a.imp_uc, a.imp_lc, a.imp_scale,
+                             a.k.c_spring, f);
+            sx = f[0];
+            sy = f[1];
+            sz = f[2];
+        }
+        // the position-dependent sum is rounded once; the viscous term
+        // -gamma v follows in FP32 (Particle.cpp:80-84)
         float fxf = (float)sx, fyf = (float)sy, fzf = (float)sz;
         if (a.visc_enable)
         {
@@ -400,20 +455,17 @@ __global__ void __launch_bounds__(BLOCK) k_spring(SpringArgs a)
             fyf -= v.y * a.visc;
             fzf -= v.z * a.visc;
         }
-        const float4 ip = a.iprm[row];
-        if (a.imp_enable)
-        {
-            float f[3] = {fxf, fyf, fzf};
-            eimp += imp_eval(pi.x, pi.z, ip.y, ip.z, a.imp_uo, a.imp_lo, a.imp_uc, a.imp_lc, a.imp_scale,
-                             a.k.c_spring, f);
-            fxf = f[0];
-            fyf = f[1];
-            fzf = f[2];
-        }
         const float fx = fxf, fy = fyf, fz = fzf;
         if (!INTEGRATE)
         {
             a.fout[row] = make_float4(fx, fy, fz, 0.0f);
+        }
+        else if (fli & PF_RIGID)
+        {
+            // placed by its body after the body solve (k_rigid_place); a rigid
+            // particle's kinetic energy is reset before it is summed
+            // (RigidBody.cpp:629, SpringNetwork.cpp:232)
+            a.frig[row] = make_double4(fx, fy, fz, 0.0);
         }
         else
         {
@@ -466,11 +518,155 @@ __global__ void __launch_bounds__(BLOCK) k_spring(SpringArgs a)
     const double b0 = block_sum(espr, sh);
     const double b1 = block_sum(eke, sh);
     const double b2 = block_sum(eimp, sh);
+    const double b3 = block_sum(efld, sh);
     if (threadIdx.x == 0)
     {
-        a.epart[3 * blockIdx.x + 0] = b0;
-        a.epart[3 * blockIdx.x + 1] = b1;
-        a.epart[3 * blockIdx.x + 2] = b2;
+        a.epart[4 * blockIdx.x + 0] = b0;
+        a.epart[4 * blockIdx.x + 1] = b1;
+        a.epart[4 * blockIdx.x + 2] = b2;
+        a.epart[4 * blockIdx.x + 3] = b3;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Rigid bodies.  k_rigid_solve: one workgroup per body sums F and the torque
+// (x_i - X) x F_i over its dynamic particles (RigidBody.cpp:567-580) in a
+// fixed order, then one lane advances the body (RigidBody.cpp:430-484: linear
+// Euler with the a dt^2/2 term, L += tau dt, omega = I^-1 L, rotation by the
+// quaternion of axis omega -- not normalised -- and angle |omega| dt).
+// k_rigid_place: x_i = X + q (p0_i, 0) q^-1 and v_i = (x_i - x_prev) / dt
+// (RigidBody.cpp:600-635).  Everything in FP64; positions stored in float.
+// ---------------------------------------------------------------------------
+__device__ inline void qmul(const double a[4], const double b[4], double o[4])
+{
+    o[0] = b[0] * a[3] + a[0] * b[3] + (a[1] * b[2] - a[2] * b[1]);
+    o[1] = b[1] * a[3] + a[1] * b[3] + (a[2] * b[0] - a[0] * b[2]);
+    o[2] = b[2] * a[3] + a[2] * b[3] + (a[0] * b[1] - a[1] * b[0]);
+    o[3] = a[3] * b[3] - (a[0] * b[0] + a[1] * b[1] + a[2] * b[2]);
+}
+
+__device__ inline void qnormalize(double q[4])
+{
+    const double m = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+    if (m > 0.005) // Quaternion::normalize (Quaternion.cpp:11-21)
+        for (int c = 0; c < 4; ++c)
+            q[c] /= m;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_rigid_solve(RigidArgs a)
+{
+    __shared__ double sh[BLOCK / 64];
+    const int32_t b = blockIdx.x;
+    RigidBody &B = a.bodies[b];
+    const double X0 = B.X[0], X1 = B.X[1], X2 = B.X[2];
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int32_t s = a.bstart[b] + threadIdx.x; s < a.bstart[b + 1]; s += blockDim.x)
+    {
+        const double4 f = a.frig[a.slot_p[s]];
+        const double4 x = a.rx[s];
+        const double r0 = x.x - X0, r1 = x.y - X1, r2 = x.z - X2;
+        acc[0] += f.x;
+        acc[1] += f.y;
+        acc[2] += f.z;
+        acc[3] += r1 * f.z - r2 * f.y;
+        acc[4] += r2 * f.x - r0 * f.z;
+        acc[5] += r0 * f.y - r1 * f.x;
+    }
+    double tot[6];
+    for (int k = 0; k < 6; ++k)
+        tot[k] = block_sum(acc[k], sh);
+    if (threadIdx.x != 0)
+        return;
+    const double dt = a.dt;
+    double acc3[3];
+    for (int c = 0; c < 3; ++c)
+    {
+        acc3[c] = tot[c] / B.M;
+        B.v[c] += acc3[c] * dt;
+    }
+    for (int c = 0; c < 3; ++c)
+        B.X[c] += B.v[c] * dt + acc3[c] * 0.5 * dt * dt;
+    for (int c = 0; c < 3; ++c)
+        B.L[c] += tot[3 + c] * dt;
+    for (int r = 0; r < 3; ++r)
+        B.w[r] = B.iinv[3 * r] * B.L[0] + B.iinv[3 * r + 1] * B.L[1] + B.iinv[3 * r + 2] * B.L[2];
+    const double wn = sqrt(B.w[0] * B.w[0] + B.w[1] * B.w[1] + B.w[2] * B.w[2]);
+    const double half = 0.5 * wn * dt;
+    const double sn = sin(half), cn = cos(half);
+    double dq[4] = {B.w[0] * sn, B.w[1] * sn, B.w[2] * sn, cn};
+    qnormalize(dq);
+    double nq[4];
+    qmul(dq, B.q, nq);
+    qnormalize(nq);
+    for (int c = 0; c < 4; ++c)
+        B.q[c] = nq[c];
+}
+
+__global__ void k_rigid_place(RigidArgs a)
+{
+    const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.nr)
+        return;
+    const int32_t p = a.slot_p[s];
+    const RigidBody &B = a.bodies[a.slot_b[s]];
+    const double4 p0 = a.p0[s];
+    const double q[4] = {B.q[0], B.q[1], B.q[2], B.q[3]};
+    const double P[4] = {p0.x, p0.y, p0.z, 0.0};
+    const double n2 = q[3] * q[3] + q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
+    const double qi[4] = {-q[0] / n2, -q[1] / n2, -q[2] / n2, q[3] / n2};
+    double t[4], r[4];
+    qmul(q, P, t);
+    qmul(t, qi, r);
+    const double4 xp = a.rx[s];
+    const double nx = B.X[0] + r[0], ny = B.X[1] + r[1], nz = B.X[2] + r[2];
+    a.rx[s] = make_double4(nx, ny, nz, 0.0);
+    const float4 xn = make_float4((float)nx, (float)ny, (float)nz, a.pos[p].w);
+    float4 v = a.vel[p];
+    v.x = (float)((nx - xp.x) / a.dt);
+    v.y = (float)((ny - xp.y) / a.dt);
+    v.z = (float)((nz - xp.z) / a.dt);
+    a.vel[p] = v;
+    a.pos_out[p] = xn;
+    if (!isfinite(xn.x) || !isfinite(xn.y) || !isfinite(xn.z))
+        atomicMin(&a.status[1], a.perm[p]);
+    const float4 xr = a.xref[p];
+    const float ex = xn.x - xr.x, ey = xn.y - xr.y, ez = xn.z - xr.z;
+    if (!(ex * ex + ey * ey + ez * ez <= a.half_skin2))
+        atomicOr(&a.status[0], 1);
+}
+
+// Sum of positions in FP64, block partials then one fixed-order final sum.
+__global__ void __launch_bounds__(BLOCK) k_centroid_partial(const float4 *pos, int32_t n, double *partial)
+{
+    __shared__ double sh[BLOCK / 64];
+    double acc[3] = {0, 0, 0};
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    {
+        const float4 p = pos[i];
+        acc[0] += p.x;
+        acc[1] += p.y;
+        acc[2] += p.z;
+    }
+    for (int c = 0; c < 3; ++c)
+    {
+        const double t = block_sum(acc[c], sh);
+        if (threadIdx.x == 0)
+            partial[3 * blockIdx.x + c] = t;
+    }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_centroid_final(const double *partial, int32_t nb, int32_t n, double *out)
+{
+    __shared__ double sh[BLOCK / 64];
+    double acc[3] = {0, 0, 0};
+    for (int32_t b = threadIdx.x; b < nb; b += blockDim.x)
+        for (int c = 0; c < 3; ++c)
+            acc[c] += partial[3 * b + c];
+    for (int c = 0; c < 3; ++c)
+    {
+        const double t = block_sum(acc[c], sh);
+        if (threadIdx.x == 0)
+            out[c] = n > 0 ? t / (double)n : 0.0;
     }
 }
 
@@ -491,9 +687,10 @@ __global__ void __launch_bounds__(BLOCK) k_reduce_energies(const double *pp, int
     if (spp)
         for (int32_t b = threadIdx.x; b < nsb; b += blockDim.x)
         {
-            acc[0] += spp[3 * b + 0];
-            acc[3] += spp[3 * b + 1];
-            acc[4] += spp[3 * b + 2];
+            acc[0] += spp[4 * b + 0];
+            acc[3] += spp[4 * b + 1];
+            acc[4] += spp[4 * b + 2];
+            acc[1] += spp[4 * b + 3];
         }
     for (int k = 0; k < 6; ++k)
     {
@@ -619,6 +816,22 @@ void launch_finish_rebuild(const float4 *pos, float4 *xref, int32_t n, int32_t *
 void launch_clear_rebuild(int32_t *status, int32_t *counters, hipStream_t s)
 {
     k_clear_rebuild<<<1, 1, 0, s>>>(status, counters);
+}
+
+void launch_rigid(const RigidArgs &a, hipStream_t s)
+{
+    if (a.nb <= 0)
+        return;
+    if (a.solve)
+        k_rigid_solve<<<a.nb, BLOCK, 0, s>>>(a);
+    if (a.nr > 0)
+        k_rigid_place<<<nblk(a.nr), BLOCK, 0, s>>>(a);
+}
+
+void launch_centroid(const float4 *pos, int32_t n, double *partial, int32_t nblocks, double *out3, hipStream_t s)
+{
+    k_centroid_partial<<<nblocks, BLOCK, 0, s>>>(pos, n, partial);
+    k_centroid_final<<<1, BLOCK, 0, s>>>(partial, nblocks, n, out3);
 }
 
 void launch_fill_i32(int32_t *p, int32_t v, int64_t n, const int32_t *gate, hipStream_t s)

@@ -462,7 +462,15 @@ void launch_spring(const SpringArgs &a, int32_t group, int32_t nb, hipStream_t)
     need(a.sp_ptr, sizeof(int32_t) * (n + 1), "spring: sp_ptr");
     need(a.perm, sizeof(int32_t) * n, "spring: perm");
     need(a.status, sizeof(int32_t) * 4, "spring: status");
-    need(a.epart, sizeof(double) * 3 * nb, "spring: epart");
+    need(a.epart, sizeof(double) * 4 * nb, "spring: epart");
+    for (const GridView *g : {&a.gpot, &a.gden})
+        if (g->cells)
+            need(g->cells, sizeof(float4) * (size_t)g->shape[0] * g->shape[1] * g->shape[2], "spring: grid");
+    bool any_rigid = false;
+    for (int32_t i = 0; i < n; ++i)
+        any_rigid |= (fbits(a.jprm[i].w) & PF_RIGID) != 0;
+    if (any_rigid && a.integrate)
+        need(a.frig, sizeof(double4) * n, "spring: frig");
     if (a.fnb)
         need(a.fnb, sizeof(double4) * n, "spring: fnb");
     if (a.fext)
@@ -491,9 +499,9 @@ void launch_spring(const SpringArgs &a, int32_t group, int32_t nb, hipStream_t)
             hc_fail("spring: edge index out of range");
     if (a.integrate)
         for (int32_t i = 0; i < n; ++i)
-            if (!(fbits(a.jprm[i].w) & PF_STATIC))
+            if (!(fbits(a.jprm[i].w) & (PF_STATIC | PF_RIGID)))
                 const_cast<float4 *>(a.pos_out)[i] = a.pos[i];
-    std::memset(a.epart, 0, sizeof(double) * 3 * nb);
+    std::memset(a.epart, 0, sizeof(double) * 4 * nb);
 }
 
 void launch_reduce_energies(const double *pp, int32_t npb, int32_t nlists, const double *spp, int32_t nsb,
@@ -501,10 +509,63 @@ void launch_reduce_energies(const double *pp, int32_t npb, int32_t nlists, const
 {
     need(pp, sizeof(double) * 3 * npb * std::max(nlists, 0), "reduce: pair partials");
     if (spp)
-        need(spp, sizeof(double) * 3 * nsb, "reduce: spring partials");
+        need(spp, sizeof(double) * 4 * nsb, "reduce: spring partials");
     need(energies, sizeof(double) * 6, "reduce: energies");
     for (int k = 0; k < 6; ++k)
         energies[k] = 0.0;
+}
+
+void launch_rigid(const RigidArgs &a, hipStream_t)
+{
+    if (a.nb <= 0)
+        return;
+    need(a.bodies, sizeof(RigidBody) * a.nb, "rigid: bodies");
+    need(a.bstart, sizeof(int32_t) * (a.nb + 1), "rigid: bstart");
+    if (a.bstart[0] != 0 || a.bstart[a.nb] != a.nr)
+        hc_fail("rigid: slot ranges do not cover the slots");
+    for (int32_t b = 0; b < a.nb; ++b)
+        if (a.bstart[b + 1] < a.bstart[b])
+            hc_fail("rigid: slot ranges not ascending");
+    if (a.nr == 0)
+        return;
+    need(a.slot_p, sizeof(int32_t) * a.nr, "rigid: slot_p");
+    need(a.slot_b, sizeof(int32_t) * a.nr, "rigid: slot_b");
+    need(a.p0, sizeof(double4) * a.nr, "rigid: p0");
+    need(a.rx, sizeof(double4) * a.nr, "rigid: rx");
+    need(a.status, sizeof(int32_t) * 4, "rigid: status");
+    int32_t n = 0;
+    for (int32_t t = 0; t < a.nr; ++t)
+    {
+        if (a.slot_b[t] < 0 || a.slot_b[t] >= a.nb || t < a.bstart[a.slot_b[t]] || t >= a.bstart[a.slot_b[t] + 1])
+            hc_fail("rigid: slot outside its body's range");
+        n = std::max(n, a.slot_p[t] + 1);
+        if (a.slot_p[t] < 0)
+            hc_fail("rigid: negative particle index");
+    }
+    need(a.pos, sizeof(float4) * n, "rigid: pos");
+    need(a.pos_out, sizeof(float4) * n, "rigid: pos_out");
+    need(a.vel, sizeof(float4) * n, "rigid: vel");
+    need(a.frig, sizeof(double4) * n, "rigid: frig");
+    need(a.xref, sizeof(float4) * n, "rigid: xref");
+    need(a.perm, sizeof(int32_t) * n, "rigid: perm");
+    for (int32_t t = 0; t < a.nr; ++t)
+        const_cast<float4 *>(a.pos_out)[a.slot_p[t]] = a.pos[a.slot_p[t]];
+}
+
+void launch_centroid(const float4 *pos, int32_t n, double *partial, int32_t nblocks, double *out3, hipStream_t)
+{
+    need(pos, sizeof(float4) * n, "centroid: pos");
+    need(partial, sizeof(double) * 3 * nblocks, "centroid: partial");
+    need(out3, sizeof(double) * 3, "centroid: out");
+    double c[3] = {0, 0, 0};
+    for (int32_t i = 0; i < n; ++i)
+    {
+        c[0] += pos[i].x;
+        c[1] += pos[i].y;
+        c[2] += pos[i].z;
+    }
+    for (int k = 0; k < 3; ++k)
+        out3[k] = n > 0 ? c[k] / n : 0.0;
 }
 
 void launch_prestep(int32_t *status, int32_t *step, int32_t interval, hipStream_t)

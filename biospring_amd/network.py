@@ -174,6 +174,45 @@ class SpringNetwork:
               "bs_get_pairs")
         return out[:2 * cnt.value].reshape(-1, 2)
 
+    # -- rigid bodies (RigidBodiesManager, src/rigidbody/RigidBodiesManager.cpp:11-60) --
+    def addRigidBody(self, ids):
+        """InitRigidBodies(this, ids): a new body unless an id is already rigid."""
+        a = np.ascontiguousarray(ids, np.int64)
+        check(self._L.bs_add_rigid_body(self._h, a.size, ptr(a, ctypes.c_int64)), self._h, "bs_add_rigid_body")
+        return self.numRigidBodies()
+
+    def numRigidBodies(self):
+        return int(self._L.bs_num_rigid_bodies(self._h))
+
+    def getRigidBody(self, b):
+        """X, v, L, omega (3 each) and q (x, y, z, w) of body b."""
+        out = np.zeros(16)
+        check(self._L.bs_get_rigid_body(self._h, b, ptr(out, ctypes.c_double)), self._h, "bs_get_rigid_body")
+        return out
+
+    # -- fields (SpringNetwork::getPotentialGrid / getDensityGrid, SpringNetwork.h:179-184) --
+    def setGrid(self, kind, origin, shape, cell, scalar):
+        o = np.ascontiguousarray(origin, np.float64)
+        sh = np.ascontiguousarray(shape, np.int64)
+        c = np.ascontiguousarray(cell, np.float64)
+        v = None if scalar is None else np.ascontiguousarray(scalar, np.float32).reshape(-1)
+        if v is not None and v.size != int(np.prod(sh)):
+            raise ValueError("scalar size does not match the grid shape")
+        check(self._L.bs_set_grid(self._h, kind, ptr(o, ctypes.c_double), ptr(sh, ctypes.c_int64),
+                                  ptr(c, ctypes.c_double), ptr(v, ctypes.c_float)), self._h, "bs_set_grid")
+
+    def setBurying(self, burying):
+        b = None if burying is None else f32(burying)
+        check(self._L.bs_set_burying(self._h, ptr(b, ctypes.c_float)), self._h, "bs_set_burying")
+
+    # -- insertion vector (src/InsertionVector.cpp:18-78) --
+    def getInsertionVector(self, i1, i2):
+        """angle (deg), roll angle (deg), insertion depth, vector (3)."""
+        out = np.zeros(6)
+        check(self._L.bs_get_insertion_vector(self._h, int(i1), int(i2), ptr(out, ctypes.c_double)), self._h,
+              "bs_get_insertion_vector")
+        return out
+
     def close(self):
         if self._h:
             self._L.bs_destroy(self._h)

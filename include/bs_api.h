@@ -255,6 +255,63 @@ int bs_get_cells(bs_handle *h, int32_t kind, int64_t *ncells, int64_t *nmembers,
  * sorted ascending.  *count receives the number; ij may be NULL to query it. */
 int bs_get_pairs(bs_handle *h, int32_t kind, uint32_t *ij, int64_t cap, int64_t *count);
 
+/* ---- rigid bodies ------------------------------------------------------------- */
+
+/* Makes particles ids[0..n) one rigid body: RigidBodiesManager::InitRigidBodies
+ * (RigidBodiesManager.cpp:11-43).  As there, nothing happens when one of the
+ * ids already belongs to a body, and n = 0 removes every body.  The body's
+ * mass, barycentre and body-frame offsets are taken from the positions at the
+ * next bs_setup / step (RigidBody::init, RigidBody.cpp:57-124; isotropic
+ * inertia 2/5 M).  Springs between two rigid particles are no longer evaluated
+ * (RigidBody.cpp:42-53).  Each step with rigidbody_enable sums the force and
+ * torque of the body's dynamic particles (RigidBody.cpp:567-580), advances the
+ * body (RigidBody.cpp:430-484) and places its particles
+ * (RigidBody.cpp:600-635); with rigidbody_enable = 0 the particles are only
+ * placed.  The reference places by the particle's index in the dynamic list
+ * (SpringNetwork.cpp:216-218), which is the body index for the single body
+ * SpringNetwork::run creates; this engine uses the index in the body, which
+ * is also right for several bodies. */
+int bs_add_rigid_body(bs_handle *h, int64_t n, const int64_t *ids);
+
+/* Number of rigid bodies. */
+int32_t bs_num_rigid_bodies(const bs_handle *h);
+
+/* State of body b: position X, velocity, angular momentum L, angular velocity
+ * omega (3 each) and orientation quaternion (x, y, z, w) -> out16
+ * (RigidBody.h:156-183). */
+int bs_get_rigid_body(bs_handle *h, int32_t b, double *out16);
+
+/* ---- field grids -------------------------------------------------------------- */
+
+/* Sets grid `kind` (BS_GRID_*) from its scalar values, shape[0]*shape[1]*shape[2]
+ * floats with z fastest, as PotentialGrid holds them after opendx::readGrid
+ * (OpenDXReader.cpp:146-173); the force vector of each cell is the central
+ * difference gradient times -k_B (PotentialGrid::compute_gradient,
+ * PotentialGrid.cpp:20-60).  Each step, for the potential grid (with
+ * electrostatic_enable and potentialgrid_enable) F += vector * q * scale and
+ * E_elec += scalar * q * k_B * N_A / 1000 * scale (Particle.cpp:161-189); for
+ * the density grid (densitygrid_enable) F += vector * burying * scale
+ * (Particle.cpp:136-159).  The cell is the nearest lower one, no
+ * interpolation; a particle off the grid gets nothing
+ * (GridCoordinatesSystem.hpp:85-93,206-218).  scalar = NULL removes the grid. */
+int bs_set_grid(bs_handle *h, int32_t kind, const double origin[3], const int64_t shape[3],
+                const double cell[3], const float *scalar);
+
+/* Per-particle burying factor of the density field (ParticleProperty::_burying,
+ * ParticleProperty.h:60-61; default 1).  NULL restores the default. */
+int bs_set_burying(bs_handle *h, const float *burying);
+
+/* ---- insertion vector --------------------------------------------------------- */
+
+/* InsertionVector::computeVector / computeAngle / computeRollAngle
+ * (InsertionVector.cpp:18-78) for particles i1, i2 at the current positions,
+ * with the centroid of all particles (measure.hpp:121-140) reduced on the GPU:
+ * out6 = angle (deg), roll angle (deg), insertion depth (centroid z),
+ * vector p2 - p1 (3).  Replaces SpringNetwork::_updateInsertionVector
+ * (SpringNetwork.cpp:52-59), which the reference runs every step; here it
+ * runs when asked for. */
+int bs_get_insertion_vector(bs_handle *h, int64_t i1, int64_t i2, double *out6);
+
 #ifdef __cplusplus
 }
 #endif

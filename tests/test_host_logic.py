@@ -128,3 +128,62 @@ def test_argument_errors(hostcheck_lib):
         net.setSprings([0, 1, 0], [1, 0, 1], [1.0, 2.0, 3.0], [1.0, 1.0, 1.0])
         assert L.bs_num_springs(net._h) == 1   # duplicates ignored like addSpring
     """, hostcheck_lib)
+
+
+def test_rigid_bodies_grids_sequence(hostcheck_lib):
+    """Rigid bodies, field grids, burying and the insertion vector through
+    setup, steps, re-parametrisation and body/grid changes."""
+    out = _run("""
+        s = workloads.generate(0, seed=4, n=900)
+        s.flags[::17] = 1
+        base = L.hc_live_allocations()
+        p = s.params.copy(); p.rigidbody_enable = 1
+        p.potentialgrid_enable = 1; p.densitygrid_enable = 1
+        net = SpringNetwork.from_system(s, params=p, library=L)
+        assert net.addRigidBody(np.arange(0, 200)) == 1
+        assert net.addRigidBody(np.arange(150, 300)) == 1     # overlaps: ignored
+        assert net.addRigidBody(np.arange(400, 520)) == 2
+        x = s.x.reshape(-1, 3)
+        org = x.min(0) - 2.0; shape = np.array([12, 10, 14]); cell = np.array([3.0, 3.5, 2.5])
+        phi = np.random.default_rng(1).normal(size=shape).astype(np.float32)
+        net.setGrid(_abi.GRID_POTENTIAL, org, shape, cell, phi)
+        net.setGrid(_abi.GRID_DENSITY, org, shape, cell, phi * 0.5)
+        net.setBurying(np.linspace(0.5, 2.0, s.n, dtype=np.float32))
+        net.setup(); net.run(5); net.computeForces()
+        b = net.getRigidBody(1)
+        iv = net.getInsertionVector(3, 800)
+        p2 = p.copy(); p2.rigidbody_enable = 0
+        net.setParams(p2); net.run(3)                         # body state kept across re-setup
+        net.addRigidBody(np.arange(600, 650)); net.run(2)
+        net.setGrid(_abi.GRID_DENSITY, org, shape, cell, None); net.run(2)
+        net.addRigidBody([]); net.run(2)                      # removes every body
+        assert net.numRigidBodies() == 0
+        net.close()
+        print("leak", L.hc_live_allocations() - base)
+    """, hostcheck_lib)
+    assert "leak 0" in out, out
+
+
+def test_rigid_and_grid_argument_errors(hostcheck_lib):
+    _run("""
+        s = workloads.generate(0, seed=5, n=100)
+        net = SpringNetwork.from_system(s, library=L)
+        for bad in ([0, 100], [-1], [3, 3]):
+            try:
+                net.addRigidBody(bad)
+                raise SystemExit("accepted %r" % (bad,))
+            except _abi.EngineError as e:
+                assert e.code == _abi.BS_ERR_ARG
+        for shape, cell in (([0, 2, 2], [1, 1, 1]), ([2, 2, 2], [1, 0, 1])):
+            try:
+                net.setGrid(0, [0, 0, 0], shape, cell, np.zeros(max(int(np.prod(shape)), 1), np.float32))
+                raise SystemExit("accepted grid")
+            except (_abi.EngineError, ValueError):
+                pass
+        try:
+            net.getInsertionVector(0, 100)
+            raise SystemExit("accepted ivec")
+        except _abi.EngineError as e:
+            assert e.code == _abi.BS_ERR_ARG
+        net.close()
+    """, hostcheck_lib)
