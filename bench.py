@@ -11,6 +11,14 @@ in HBM.
 Printed JSON (one line): throughput, the roofline of the dominant kernel (HIP
 events on the engine's stream over the timed region), the FP64-oracle force
 error at the final state, and the oracle's CPU time on a bounded sample.
+
+Two helper modes serve the HBM-traffic measurement of the kernels
+(rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE):
+  --copy-calibration    a device-to-device copy of a known size, the yardstick
+                        that converts the counters' KiB into bytes;
+  --pmc-summary F C W D reads the four counter runs (FETCH bench/copy dirs,
+                        WRITE bench/copy dirs) and writes
+                        profiles/pmc_traffic.json, which later bench lines cite.
 """
 import argparse
 import ctypes
@@ -30,8 +38,72 @@ FP32_PEAK_TFLOPS = 157.3    # MI355X FP32 vector/matrix, TFLOP/s
 PAIR_FLOP = 40              # algorithmic flops per unique LJ+Coulomb pair
 
 
+COPY_BYTES = 1 << 30        # copy-calibration size
+# Reference (compiled from its sources, -O3 OpenMP) / float-faithful oracle
+# step time on the same 250k input with 8 OpenMP threads each, measured on an
+# 8-core x86-64 host: 9.64 s / 8.06 s per step.
+REF_OVER_ORACLE = 1.196
+
+
 def hip_sync():
     ctypes.CDLL("libamdhip64.so").hipDeviceSynchronize()
+
+
+def copy_calibration(reps=5):
+    """hipMemcpy device-to-device of COPY_BYTES (a blit kernel) `reps` times."""
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(COPY_BYTES)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(COPY_BYTES)) == 0
+    hip.hipMemset(a, 1, ctypes.c_size_t(COPY_BYTES))
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(COPY_BYTES), 3) == 0  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_bytes": COPY_BYTES, "reps": reps}))
+
+
+def _counter_means(d, counter):
+    """Mean counter value per kernel name over every dispatch in the
+    rocprofv3 CSV output under directory d."""
+    import csv
+    import glob
+    acc = {}
+    for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+        with open(f) as fh:
+            for r in csv.DictReader(fh):
+                if r.get("Counter_Name") != counter:
+                    continue
+                k = r["Kernel_Name"]
+                acc.setdefault(k, []).append(float(r["Counter_Value"]))
+    return {k: sum(v) / len(v) for k, v in acc.items()}
+
+
+def pmc_summary(fetch_dir, fetch_copy_dir, write_dir, write_copy_dir):
+    def pick(m, key):
+        c = [v for k, v in m.items() if key in k]
+        return c[0] if c else None
+    out = {}
+    cf = pick(_counter_means(fetch_copy_dir, "FETCH_SIZE"), "copy")
+    cw = pick(_counter_means(write_copy_dir, "WRITE_SIZE"), "copy")
+    sf = COPY_BYTES / (cf * 1024.0)
+    sw = COPY_BYTES / (cw * 1024.0)
+    fm = _counter_means(fetch_dir, "FETCH_SIZE")
+    wm = _counter_means(write_dir, "WRITE_SIZE")
+    for name, key in (("pair", "k_pair<"), ("spring_integrate", "k_spring<16, true"),
+                      ("cluster_list", "k_cluster_list")):
+        f, w = pick(fm, key), pick(wm, key)
+        if f is None or w is None:
+            continue
+        out[name] = {"fetch_bytes": f * 1024.0 * sf, "write_bytes": w * 1024.0 * sw,
+                     "bytes": f * 1024.0 * sf + w * 1024.0 * sw}
+    out["calibration"] = {"copy_bytes": COPY_BYTES, "copy_fetch_kib": cf, "copy_write_kib": cw,
+                          "fetch_scale": sf, "write_scale": sw}
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    with open(path, "w") as fh:
+        json.dump(out, fh, indent=1)
+    print(json.dumps(out))
 
 
 def log(*a):
@@ -45,10 +117,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=250000)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--pmc-summary", nargs=4, metavar=("FETCH", "FETCH_COPY", "WRITE", "WRITE_COPY"))
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration()
+    if args.pmc_summary:
+        return pmc_summary(*args.pmc_summary)
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is a design note only",
                           "n_gpus": args.gpus}))
@@ -99,9 +177,14 @@ def main():
     dom = "pair" if pair_ms >= spring_ms else "spring_integrate"
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "pmc_traffic.json")
-    if os.path.exists(tfile):
+    if os.path.exists(tfile) and args.n == 250000:
+        # HBM bytes per launch from the committed FETCH_SIZE / WRITE_SIZE runs
         with open(tfile) as fh:
-            traffic = json.load(fh).get(dom)
+            tt = json.load(fh)
+        for name in ("pair", "spring_integrate"):
+            if name in tt:
+                kernels[name]["traffic_bytes"] = tt[name]["bytes"]
+        traffic = tt.get(dom, {}).get("bytes")
     d = kernels[dom]
     roofline = {"bound": d["bound"], "achieved": d["achieved"], "peak": d["peak"], "unit": d["unit"],
                 "frac": d["frac"], "traffic": traffic, "kernel": dom}
@@ -157,8 +240,10 @@ def main():
             dt = time.perf_counter() - t
             out["cpu_baseline"] = {"value": n * args.cpu_steps / dt, "unit": "particle-steps/s",
                                    "cores": int(po.oracle_lib().orc_num_threads()), "kind": "port",
-                                   "sample": f"{args.cpu_steps} full step(s) of the same 250k workload, "
+                                   "sample": f"{args.cpu_steps} full step(s) of the same {n}-bead workload, "
                                              f"float-faithful oracle, OpenMP"}
+            if REF_OVER_ORACLE:
+                out["cpu_baseline"]["reference_equivalent"] = n * args.cpu_steps / dt / REF_OVER_ORACLE
     print(json.dumps(out), flush=True)
 
 
