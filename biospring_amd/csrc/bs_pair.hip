@@ -159,109 +159,117 @@ __global__ void __launch_bounds__(BLOCK) k_cluster_list(ListArgs a)
 // Lorentz-Berthelot / Good-Hope modes (CombinationRules.hpp:18-31).  The
 // force is accumulated in FP64 (same rate as FP32 on CDNA4), which keeps
 // small net forces accurate when many pair terms cancel.
+//
+// Branch-free: every lane evaluates the terms with a safe distance and the
+// results are selected by the pair's predicates.  Across a wave the pairs are
+// half inside the cutoff at best, so branching would not skip work; it only
+// costs exec-mask bookkeeping and keeps loads from being issued early.  The
+// exact two-stage test runs for pairs in the guard band around rc^2 only.
 // ---------------------------------------------------------------------------
 template <int SMODE, int EMODE, int HYDRO>
-__device__ inline void interact(bool on, float4 pi, float4 ri, uint32_t fli, float4 pj, float4 rj,
-                                const PairArgs &a, float cs, double &fx, double &fy, double &fz, double &es,
-                                double &ee, double &eh)
+__device__ inline void interact(bool masked, float4 pi, float4 ri, uint32_t fli, float4 pj, float4 rj,
+                                const PairArgs &a, float cs, double qi, double &fx, double &fy, double &fz,
+                                double &es, double &ee, double &eh)
 {
-    if (!on)
-        return;
     const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
     const float r2 = dx * dx + dy * dy + dz * dz;
-    if (!in_cutoff(pi, pj, a.rc, a.rc2lo, a.rc2hi, r2))
-        return;
+    bool on = masked && r2 < a.rc2lo;
+    if (masked && r2 >= a.rc2lo && r2 < a.rc2hi) // rare: decide exactly
+        on = exact_two_stage(pi, pj, a.rc);
     const uint32_t flj = fbits(rj.w);
     const float wgt = (flj & PF_STATIC) ? 1.0f : 0.5f;
-    float fscal = 0.0f;   // force on i = (dx, dy, dz) * (fscal + fscd)
+    const bool pos2 = r2 > 0.0f;
+    const float r2s = pos2 ? r2 : 1.0f;
+    float inv = __builtin_amdgcn_rsqf(r2s);
+    inv = inv * (1.5f - 0.5f * r2s * inv * inv);
+    const float inv2 = inv * inv;
+    float fscal = 0.0f; // force on i = (dx, dy, dz) * (fscal + fscd)
     double fscd = 0.0;
-    if (r2 > 0.0f)
+    if (SMODE >= 0)
     {
-        float inv = __builtin_amdgcn_rsqf(r2);
-        inv = inv * (1.5f - 0.5f * r2 * inv * inv);
-        const float inv2 = inv * inv;
-        if (SMODE >= 0)
-        {
-            float en = 0.0f, f = 0.0f;
-            if (SMODE == 0)
-            { // linear: ri + rj from raw radii
-                const float d = r2 * inv;
-                const float dv = d - (ri.x + rj.x);
-                if (!(dv > 0.0f))
-                {
-                    en = 0.5f * dv * dv;
-                    f = dv * cs * inv; // -|dv| * C / d
-                }
-            }
-            else if (r2 >= 0.01f)
-            {
-                const float eij = ri.y * rj.y;
-                const float R = ri.x * rj.x;
-                const float s2 = R * R * inv2, s6 = s2 * s2 * s2;
-                if (SMODE == 1)
-                { // Amber 12-6
-                    const float s12 = s6 * s6;
-                    en = eij * (s12 - 2.0f * s6);
-                    f = 12.0f * eij * (s6 - s12) * inv2 * cs;
-                }
-                else if (SMODE == 2)
-                { // Lewitt 8-6
-                    const float s8 = s6 * s2;
-                    en = eij * (3.0f * s8 - 4.0f * s6);
-                    f = 24.0f * eij * (s6 - s8) * inv2 * cs;
-                }
-                else
-                { // Zacharias 8-6: raw r, eps, product rules
-                    const float s8 = s6 * s2;
-                    en = eij * (s8 - s6);
-                    f = eij * (6.0f * s6 - 8.0f * s8) * inv2 * cs;
-                }
-            }
-            es += (double)(wgt * a.steric_scale * en);
-            fscal += a.steric_scale * f;
+        float en = 0.0f, f = 0.0f;
+        if (SMODE == 0)
+        { // linear: ri + rj from raw radii; a coincident pair has energy, no force
+            const float sr = ri.x + rj.x;
+            const float d = r2s * inv;
+            const float dv = pos2 ? d - sr : -sr;
+            const bool ov = !(dv > 0.0f);
+            en = ov ? 0.5f * dv * dv : 0.0f;
+            f = (ov && pos2) ? dv * cs * inv : 0.0f; // -|dv| * C / d
         }
-        if (EMODE > 0 && (fli & PF_CHARGED) && (flj & PF_CHARGED) && r2 >= 1.0e-6f)
+        else
         {
-            // Coulomb dominates the non-bonded force: its scalar is formed in
-            // FP64 from 1/d refined by one FP64 Newton step of the FP32 seed
-            const double xd = dx, yd = dy, zd = dz;
-            const double r2d = fma(xd, xd, fma(yd, yd, zd * zd));
-            const double y = inv;
-            const double invd = y * fma(-0.5 * r2d, y * y, 1.5);
-            const double qq = (double)pi.w * (double)pj.w;
-            if (EMODE == 1)
-            {
-                ee += (double)wgt * a.ke * qq * invd;
-                fscd -= a.kf * qq * invd * invd * invd;
+            const float eij = ri.y * rj.y;
+            const float R = ri.x * rj.x;
+            const float s2 = R * R * inv2, s6 = s2 * s2 * s2;
+            float e0, f0;
+            if (SMODE == 1)
+            { // Amber 12-6
+                const float s12 = s6 * s6;
+                e0 = eij * (s12 - 2.0f * s6);
+                f0 = 12.0f * eij * (s6 - s12) * inv2 * cs;
+            }
+            else if (SMODE == 2)
+            { // Lewitt 8-6
+                const float s8 = s6 * s2;
+                e0 = eij * (3.0f * s8 - 4.0f * s6);
+                f0 = 24.0f * eij * (s6 - s8) * inv2 * cs;
             }
             else
-            { // eps(r) = eps * r
-                const double i2 = invd * invd;
-                ee += (double)wgt * a.ke * qq * i2;
-                fscd -= 2.0 * a.kf * qq * i2 * i2;
+            { // Zacharias 8-6: raw r, eps, product rules
+                const float s8 = s6 * s2;
+                e0 = eij * (s8 - s6);
+                f0 = eij * (6.0f * s6 - 8.0f * s8) * inv2 * cs;
             }
+            const bool far = r2 >= 0.01f; // 0.1 A floor (steric.hpp:62)
+            en = far ? e0 : 0.0f;
+            f = far ? f0 : 0.0f;
         }
-        if (HYDRO && (fli & PF_HYDRO) && (flj & PF_HYDRO))
-        {
-            const float d = r2 * inv;
-            const float hh = ri.z * rj.z * __expf(-d);
-            eh += (double)(wgt * a.hydro_scale) * (-(double)hh * 6.02214169E23 * 1.0E-3);
-            fscal += a.hydro_scale * hh * cs * inv;
-        }
+        es += on ? (double)(wgt * a.steric_scale * en) : 0.0;
+        fscal += a.steric_scale * f;
     }
-    else if (SMODE == 0)
+    if (EMODE > 0)
     {
-        const float s = ri.x + rj.x; // coincident pair: linear energy, zero force
-        es += (double)(wgt * a.steric_scale * 0.5f * s * s);
+        // Coulomb dominates the non-bonded force: its scalar is formed in
+        // FP64 from 1/d refined by one FP64 Newton step of the FP32 seed
+        const bool qon = (fli & PF_CHARGED) && (flj & PF_CHARGED) && r2 >= 1.0e-6f;
+        const double xd = dx, yd = dy, zd = dz;
+        const double r2d = qon ? fma(xd, xd, fma(yd, yd, zd * zd)) : 1.0;
+        const double y = inv;
+        const double invd = y * fma(-0.5 * r2d, y * y, 1.5);
+        const double qq = qon ? qi * (double)pj.w : 0.0;
+        if (EMODE == 1)
+        {
+            const double e = (double)wgt * a.ke * qq * invd;
+            ee += on ? e : 0.0;
+            fscd = -(a.kf * qq * invd * invd * invd);
+        }
+        else
+        { // eps(r) = eps * r
+            const double i2 = invd * invd;
+            const double e = (double)wgt * a.ke * qq * i2;
+            ee += on ? e : 0.0;
+            fscd = -(2.0 * a.kf * qq * i2 * i2);
+        }
     }
-    const double fs = (double)fscal + fscd;
+    if (HYDRO)
+    {
+        const bool hon = (fli & PF_HYDRO) && (flj & PF_HYDRO) && pos2;
+        const float d = r2s * inv;
+        const float hh = hon ? ri.z * rj.z * __expf(-d) : 0.0f;
+        eh += on ? (double)(wgt * a.hydro_scale) * (-(double)hh * 6.02214169E23 * 1.0E-3) : 0.0;
+        fscal += a.hydro_scale * hh * cs * inv;
+    }
+    const double fs = on ? (double)fscal + fscd : 0.0;
     fx = fma((double)dx, fs, fx);
     fy = fma((double)dy, fs, fy);
     fz = fma((double)dz, fs, fz);
 }
 
-// Pair forces: two list words per lane per iteration, loads issued without
-// branches (the lists are zero-padded, a zero word is a masked-out entry).
+// Pair forces of one i-cluster per wave.  Two list words per lane per
+// iteration; while iteration t computes, the list words of iteration t+2 and
+// the j data of iteration t+1 are in flight (zero-padded lists: a zero word
+// is a masked-out entry pointing at particle 0).
 template <int SMODE, int EMODE, int HYDRO>
 __global__ void __launch_bounds__(BLOCK) k_pair(PairArgs a)
 {
@@ -279,20 +287,45 @@ __global__ void __launch_bounds__(BLOCK) k_pair(PairArgs a)
         const float4 pi = iv ? a.pos[i] : make_float4(0.f, 0.f, 0.f, 0.f);
         const float4 ri = iv ? a.jprm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
         const uint32_t fli = fbits(ri.w);
+        const double qi = (double)pi.w;
         const uint32_t ibit = 1u << (24 + il);
         const int32_t cnt16 = (a.cln[ci] + 15) & ~15;
         const uint32_t *L = a.cl + (size_t)ci * a.cap;
+        // words of iteration t (w), t+1 (n); j data of iteration t (pj, rj).
+        // Look-ahead reads past the end are clamped to the list's last 16
+        // words (valid entries, never computed with), so every load is
+        // unconditional and can be issued ahead of its use.
+        const int32_t last = cnt16 > 16 ? cnt16 - 16 : 0;
+        uint32_t w0 = 0u, w1 = 0u, n0 = 0u, n1 = 0u;
+        if (cnt16 > 0)
+        {
+            const int32_t o = min(16, last);
+            w0 = L[jl];
+            w1 = L[8 + jl];
+            n0 = L[o + jl];
+            n1 = L[o + 8 + jl];
+        }
+        float4 pj0 = a.pos[w0 & CL_JMASK], rj0 = a.jprm[w0 & CL_JMASK];
+        float4 pj1 = a.pos[w1 & CL_JMASK], rj1 = a.jprm[w1 & CL_JMASK];
         for (int32_t t = 0; t < cnt16; t += 16)
         {
-            const uint32_t w0 = L[t + jl];
-            const uint32_t w1 = L[t + 8 + jl];
-            const int32_t j0 = (int32_t)(w0 & CL_JMASK), j1 = (int32_t)(w1 & CL_JMASK);
-            const float4 pj0 = a.pos[j0];
-            const float4 pj1 = a.pos[j1];
-            const float4 rj0 = a.jprm[j0];
-            const float4 rj1 = a.jprm[j1];
-            interact<SMODE, EMODE, HYDRO>((w0 & ibit) != 0u, pi, ri, fli, pj0, rj0, a, cs, fx, fy, fz, es, ee, eh);
-            interact<SMODE, EMODE, HYDRO>((w1 & ibit) != 0u, pi, ri, fli, pj1, rj1, a, cs, fx, fy, fz, es, ee, eh);
+            const int32_t o = min(t + 32, last); // words of iteration t+2
+            const uint32_t m0 = L[o + jl];
+            const uint32_t m1 = L[o + 8 + jl];
+            const float4 qj0 = a.pos[n0 & CL_JMASK], sj0 = a.jprm[n0 & CL_JMASK];
+            const float4 qj1 = a.pos[n1 & CL_JMASK], sj1 = a.jprm[n1 & CL_JMASK];
+            interact<SMODE, EMODE, HYDRO>((w0 & ibit) != 0u, pi, ri, fli, pj0, rj0, a, cs, qi, fx, fy, fz, es, ee,
+                                          eh);
+            interact<SMODE, EMODE, HYDRO>((w1 & ibit) != 0u, pi, ri, fli, pj1, rj1, a, cs, qi, fx, fy, fz, es, ee,
+                                          eh);
+            w0 = n0;
+            w1 = n1;
+            n0 = m0;
+            n1 = m1;
+            pj0 = qj0;
+            rj0 = sj0;
+            pj1 = qj1;
+            rj1 = sj1;
         }
     }
     // sum over the 8 entry lanes of each i (lanes il, il+8, ..., il+56)

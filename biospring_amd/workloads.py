@@ -87,3 +87,35 @@ def generate(config, seed=1, n=0):
         G.bsg_free(h)
     s.params = params(config)
     return s
+
+
+def rigid_groups(system, nbodies=4, size=2000):
+    """Disjoint compact groups for workload 2's rigid bodies: around each of
+    `nbodies` centres spread over the system's xy extent (z = its mean), the
+    `size` closest beads not already taken.  Deterministic."""
+    x = system.x.reshape(-1, 3).astype(np.float64)
+    lo, hi = x.min(0), x.max(0)
+    mid = 0.5 * (lo + hi)
+    q = 0.25 * (hi - lo)
+    centres = [mid + np.array([sx * q[0], sy * q[1], 0.0]) for sx, sy in ((-1, -1), (1, -1), (-1, 1), (1, 1))]
+    taken = np.zeros(system.n, bool)
+    groups = []
+    for c in centres[:nbodies]:
+        d = np.linalg.norm(x - c, axis=1)
+        d[taken] = np.inf
+        ids = np.sort(np.argsort(d, kind="stable")[:min(size, system.n)])
+        taken[ids] = True
+        groups.append(ids.astype(np.int64))
+    return groups
+
+
+def impala_params(system, membrane="double"):
+    """Workload 2 parameters: IMPALA (flat slab, or a curved double membrane
+    of tube radius 200 A with 30 A offsets) with rigid bodies enabled."""
+    p = system.params.copy()
+    p.imp_enable = 1
+    p.rigidbody_enable = 1
+    if membrane == "double":
+        p.imp_upper_offset, p.imp_lower_offset = 30.0, 30.0
+        p.imp_upper_curvature, p.imp_lower_curvature = 1.0 / 200.0, -1.0 / 200.0
+    return p

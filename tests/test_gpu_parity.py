@@ -185,3 +185,63 @@ def test_nonfinite_reported():
         net.run(1)
     assert ei.value.code == _abi.BS_ERR_NONFINITE
     assert net.lastNonfiniteParticle() == 17
+
+
+def _impala_rigid(n=6000, size=300):
+    s = workloads.generate(2, seed=3, n=n)
+    s.params = workloads.impala_params(s, "double")
+    return s, workloads.rigid_groups(s, 4, size)
+
+
+def test_impala_rigid_bodies():
+    """Workload 2 at reduced size: curved double IMPALA membrane, four rigid
+    bodies among flexible beads.  Forces vs FP64 at the start and after 30
+    steps (viscosity off for the comparison), body states and trajectory vs
+    the FP64 oracle with the float-faithful oracle as the yardstick."""
+    s, groups = _impala_rigid()
+    _log("impala+rigid n", s.n, "m", s.m, "bodies", [len(g) for g in groups])
+    net = SpringNetwork.from_system(s)
+    o64, o32 = po.Oracle(s), po.Oracle(s)
+    for g in groups:
+        net.addRigidBody(g)
+        o64.add_rigid_body(g)
+        o32.add_rigid_body(g)
+    assert net.numRigidBodies() == 4
+    F, E = net.computeForces()
+    Fd, Ed = o64.compute_forces(po.F64)
+    per, glob = force_errors(F, Fd, (s.flags & 1) == 0)
+    _log(f"  forces at start: per-particle {per:.3g} global {glob:.3g}")
+    assert per <= FORCE_RTOL
+    for slot in range(6):
+        assert energy_error(E[slot], Ed[slot], floor=1e-3) <= ENERGY_RTOL, (slot, E[slot], Ed[slot])
+    net.run(30)
+    o64.step(po.F64, 30)
+    o32.step(po.F32, 30)
+    x, _ = net.getState()
+    xd, _ = o64.state(po.F64)
+    xf, _ = o32.state(po.F32)
+    dev, dref = np.abs(x - xd).max(), np.abs(xf - xd).max()
+    _log(f"  30 steps: gpu max dev {dev:.3e}, float path {dref:.3e}")
+    assert dev <= 3 * dref + 1e-6
+    for b in range(4):
+        bg, bd = net.getRigidBody(b), o64.rigid_state(po.F64, b)
+        bf = o32.rigid_state(po.F32, b)
+        for sl in (slice(0, 3), slice(9, 12), slice(12, 16)):
+            eg, ef = np.abs(bg[sl] - bd[sl]).max(), np.abs(bf[sl] - bd[sl]).max()
+            assert eg <= 3 * ef + 1e-7 * max(np.abs(bd[sl]).max(), 1e-9), (b, sl, bg[sl], bd[sl], bf[sl])
+    # intra-body distances of the GPU state are those of the start
+    x0 = s.x.reshape(-1, 3).astype(np.float64)
+    for g in groups:
+        d0 = np.linalg.norm(x0[g][:, None] - x0[g][None], axis=-1)
+        d1 = np.linalg.norm(x[g][:, None].astype(np.float64) - x[g][None], axis=-1)
+        assert np.abs(d1 - d0).max() < 1e-3
+    p = s.params.copy()
+    p.viscosity_enable = 0
+    net.setParams(p)
+    o64.set_params(p)
+    o64.set_state(x, net.getState()[1])
+    F, E = net.computeForces()
+    Fd, Ed = o64.compute_forces(po.F64)
+    per, glob = force_errors(F, Fd, (s.flags & 1) == 0)
+    _log(f"  forces after 30 steps: per-particle {per:.3g} global {glob:.3g}")
+    assert per <= FORCE_RTOL
