@@ -29,7 +29,8 @@ EXPORTS = (
     "bs_step", "bs_compute_forces", "bs_get_state", "bs_set_state", "bs_get_energies",
     "bs_last_nonfinite_particle", "bs_get_counters", "bs_profile", "bs_get_profile",
     "bs_get_cells", "bs_get_pairs", "bs_add_rigid_body", "bs_num_rigid_bodies", "bs_get_rigid_body",
-    "bs_set_grid", "bs_set_burying", "bs_get_insertion_vector",
+    "bs_set_grid", "bs_set_burying", "bs_get_insertion_vector", "bs_set_systems", "bs_num_systems",
+    "bs_get_system_energies", "bs_autotune",
 )
 PROF_REBUILD, PROF_PAIR, PROF_SPRING = range(3)
 
@@ -69,6 +70,7 @@ class Params(ctypes.Structure):
         ("potentialgrid_scale", ctypes.c_float),
         ("densitygrid_enable", ctypes.c_int32),
         ("densitygrid_scale", ctypes.c_float),
+        ("graph_steps", ctypes.c_int32),
     ]
 
     def copy(self):
@@ -146,6 +148,11 @@ def bind(L):
     L.bs_set_grid.argtypes = [_vp, ctypes.c_int32, _f64p, _i64p, _f64p, _f32p]
     L.bs_set_burying.argtypes = [_vp, _f32p]
     L.bs_get_insertion_vector.argtypes = [_vp, ctypes.c_int64, ctypes.c_int64, _f64p]
+    L.bs_set_systems.argtypes = [_vp, ctypes.c_int32, _i64p]
+    L.bs_num_systems.argtypes = [_vp]
+    L.bs_num_systems.restype = ctypes.c_int32
+    L.bs_get_system_energies.argtypes = [_vp, _f64p]
+    L.bs_autotune.argtypes = [_vp, ctypes.c_int32, _i32p, _i32p]
     return L
 
 

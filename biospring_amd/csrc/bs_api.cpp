@@ -102,10 +102,23 @@ struct bs_handle
     double *box_part = nullptr, *box = nullptr;
     int32_t box_blocks = 0;
     std::vector<List> lists;
-    double *pair_part = nullptr;
     int32_t pair_blocks = 0;
-    double *spring_part = nullptr;
     int32_t spring_blocks = 0;
+    // energies: per particle on energy steps, summed per segment of at most
+    // SEG contiguous particles of one system, then per system
+    double4 *epp = nullptr, *ese = nullptr;
+    int32_t *seg_start = nullptr, *sys_seg = nullptr;
+    double *seg_part = nullptr, *sys_E = nullptr;
+    int32_t nseg = 0;
+    bool last_call_forces_only = false;
+    // captured multi-step launch sequence (graph_steps)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int32_t graph_len = 0;
+    int graph_ext = 0;
+    // independent systems (original-id offsets) and their binning offsets
+    std::vector<int64_t> sys_off;
+    float4 *shift = nullptr;
     RigidBody *bodies_d = nullptr;
     double4 *rx = nullptr, *rp0 = nullptr, *frig = nullptr;
     int32_t *slot_p = nullptr, *slot_b = nullptr, *bstart = nullptr;
@@ -162,8 +175,20 @@ int fail(bs_handle *h, int code, const char *fmt, ...)
             return fail((h), BS_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));                         \
     } while (0)
 
+void drop_graph(bs_handle *h)
+{
+    if (h->graph_exec)
+        (void)hipGraphExecDestroy(h->graph_exec);
+    if (h->graph)
+        (void)hipGraphDestroy(h->graph);
+    h->graph_exec = nullptr;
+    h->graph = nullptr;
+    h->graph_len = 0;
+}
+
 void free_device(bs_handle *h)
 {
+    drop_graph(h);
     auto f = [](void *p) {
         if (p)
             (void)hipFree(p);
@@ -177,7 +202,13 @@ void free_device(bs_handle *h)
     f(h->vel); f(h->fnb); f(h->jprm); f(h->iprm); f(h->xref); f(h->fext); f(h->fout);
     f(h->sp_ptr); f(h->sp_j); f(h->perm_d); f(h->sp_k); f(h->sp_l0);
     f(h->cell_of); f(h->counts); f(h->cstart); f(h->cursor); f(h->ctmp); f(h->cidx);
-    f(h->box_part); f(h->box); f(h->pair_part); f(h->spring_part); f(h->energies_d); f(h->status); f(h->counters);
+    f(h->box_part); f(h->box); f(h->energies_d); f(h->status); f(h->counters);
+    f(h->epp); f(h->ese); f(h->seg_start); f(h->sys_seg); f(h->seg_part); f(h->sys_E); f(h->shift);
+    h->epp = h->ese = nullptr;
+    h->seg_start = h->sys_seg = nullptr;
+    h->seg_part = h->sys_E = nullptr;
+    h->shift = nullptr;
+    h->nseg = 0;
     f(h->ex_ptr); f(h->ex_w);
     h->ex_ptr = nullptr;
     h->ex_w = nullptr;
@@ -200,7 +231,7 @@ void free_device(bs_handle *h)
     h->sp_ptr = h->sp_j = h->perm_d = nullptr;
     h->sp_k = h->sp_l0 = nullptr;
     h->cell_of = h->counts = h->cstart = h->cursor = h->ctmp = h->cidx = nullptr;
-    h->box_part = h->box = h->pair_part = h->spring_part = h->energies_d = nullptr;
+    h->box_part = h->box = h->energies_d = nullptr;
     h->status = h->counters = nullptr;
     h->cell_cap = 0;
     h->nedges = 0;
@@ -213,8 +244,9 @@ int check_device(bs_handle *h)
 {
     const void *need[] = {h->pos[0], h->pos[1], h->vel, h->fnb, h->jprm, h->iprm, h->xref, h->fext, h->fout,
                           h->sp_ptr, h->sp_j, h->sp_k, h->sp_l0, h->perm_d, h->cell_of, h->ctmp, h->cidx,
-                          h->counts, h->cstart, h->cursor, h->box, h->box_part, h->pair_part, h->spring_part,
-                          h->energies_d, h->status, h->counters, h->ex_ptr, h->ex_w};
+                          h->counts, h->cstart, h->cursor, h->box, h->box_part, h->epp, h->ese, h->seg_start,
+                          h->sys_seg, h->seg_part, h->sys_E, h->energies_d, h->status, h->counters, h->ex_ptr,
+                          h->ex_w};
     for (const void *q : need)
         if (!q)
             return fail(h, BS_ERR_STATE, "internal: device buffer missing");
@@ -226,6 +258,8 @@ int check_device(bs_handle *h)
     for (int g = 0; g < 2; ++g)
         if (h->grids[g].set && !h->grid_d[g])
             return fail(h, BS_ERR_STATE, "internal: grid buffer missing");
+    if (h->sys_off.size() > 2 && !h->shift)
+        return fail(h, BS_ERR_STATE, "internal: system offsets missing");
     if (h->ncl != (int32_t)((h->n + CLUSTER - 1) / CLUSTER))
         return fail(h, BS_ERR_STATE, "internal: cluster count mismatch");
     for (const auto &l : h->lists)
@@ -248,23 +282,30 @@ inline uint64_t spread3(uint64_t v)
 // Internal order: Morton order of a 4 A grid; ties keep the original order.
 void make_order(bs_handle *h)
 {
+    // Morton order of 4 A cells within each system; systems stay contiguous
+    // in their given order, so each is one range of internal indices.
     const int64_t n = h->n;
-    double lo[3] = {INFINITY, INFINITY, INFINITY};
-    for (int64_t i = 0; i < n; ++i)
-        for (int c = 0; c < 3; ++c)
-            lo[c] = std::min(lo[c], (double)h->x[3 * i + c]);
-    std::vector<std::pair<uint64_t, int32_t>> key((size_t)n);
+    std::vector<std::pair<std::pair<int32_t, uint64_t>, int32_t>> key((size_t)n);
     const double cell = 4.0;
-    for (int64_t i = 0; i < n; ++i)
+    const int32_t nsys = (int32_t)h->sys_off.size() - 1;
+    for (int32_t sy = 0; sy < nsys; ++sy)
     {
-        uint64_t q[3];
-        for (int c = 0; c < 3; ++c)
+        const int64_t b = h->sys_off[(size_t)sy], e = h->sys_off[(size_t)sy + 1];
+        double lo[3] = {INFINITY, INFINITY, INFINITY};
+        for (int64_t i = b; i < e; ++i)
+            for (int c = 0; c < 3; ++c)
+                lo[c] = std::min(lo[c], (double)h->x[3 * i + c]);
+        for (int64_t i = b; i < e; ++i)
         {
-            double t = std::floor(((double)h->x[3 * i + c] - lo[c]) / cell);
-            t = std::min(std::max(t, 0.0), 2097151.0);
-            q[c] = (uint64_t)t;
+            uint64_t q[3];
+            for (int c = 0; c < 3; ++c)
+            {
+                double t = std::floor(((double)h->x[3 * i + c] - lo[c]) / cell);
+                t = std::min(std::max(t, 0.0), 2097151.0);
+                q[c] = (uint64_t)t;
+            }
+            key[(size_t)i] = {{sy, spread3(q[0]) | spread3(q[1]) << 1 | spread3(q[2]) << 2}, (int32_t)i};
         }
-        key[(size_t)i] = {spread3(q[0]) | spread3(q[1]) << 1 | spread3(q[2]) << 2, (int32_t)i};
     }
     std::sort(key.begin(), key.end());
     h->perm.resize((size_t)n);
@@ -274,6 +315,47 @@ void make_order(bs_handle *h)
         h->perm[(size_t)t] = key[(size_t)t].second;
         h->iperm[(size_t)key[(size_t)t].second] = (int32_t)t;
     }
+}
+
+// Binning offsets of a batch: system s is moved to site s of a cubic lattice
+// whose spacing exceeds every system's extent plus twice the largest list
+// radius and a 20 A margin, so no neighbour search reaches across systems.
+// The offsets apply to binning only; forces use each system's own
+// coordinates.  shift.w carries the system index (as raw bits).
+std::vector<float4> system_shifts(const bs_handle *h, float max_radius)
+{
+    const int32_t nsys = (int32_t)h->sys_off.size() - 1;
+    std::vector<std::array<double, 6>> bb((size_t)nsys);
+    double ext = 0.0;
+    for (int32_t sy = 0; sy < nsys; ++sy)
+    {
+        auto &b = bb[(size_t)sy];
+        b = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+        for (int64_t i = h->sys_off[(size_t)sy]; i < h->sys_off[(size_t)sy + 1]; ++i)
+            for (int c = 0; c < 3; ++c)
+            {
+                b[c] = std::min(b[c], (double)h->x[3 * i + c]);
+                b[3 + c] = std::max(b[3 + c], (double)h->x[3 * i + c]);
+            }
+        for (int c = 0; c < 3; ++c)
+            ext = std::max(ext, b[3 + c] - b[c]);
+    }
+    const double spacing = std::ceil(ext + 2.0 * max_radius + 20.0);
+    const int32_t side = (int32_t)std::ceil(std::cbrt((double)nsys) - 1e-9);
+    std::vector<float4> out((size_t)h->n);
+    for (int32_t sy = 0; sy < nsys; ++sy)
+    {
+        const int32_t site[3] = {sy % side, (sy / side) % side, sy / (side * side)};
+        float o[3];
+        for (int c = 0; c < 3; ++c)
+            o[c] = (float)(site[c] * spacing - bb[(size_t)sy][c]);
+        uint32_t tag = (uint32_t)sy;
+        float w;
+        std::memcpy(&w, &tag, 4);
+        for (int64_t i = h->sys_off[(size_t)sy]; i < h->sys_off[(size_t)sy + 1]; ++i)
+            out[(size_t)h->iperm[(size_t)i]] = make_float4(o[0], o[1], o[2], w);
+    }
+    return out;
 }
 
 uint32_t particle_flags(const bs_handle *h, int64_t i)
@@ -381,15 +463,26 @@ void plan_lists(bs_handle *h)
 }
 
 // Engine grid dimensions of a list from the current host positions.
-void plan_grid(bs_handle *h, List &l)
+void plan_grid(bs_handle *h, List &l, const std::vector<float4> &shift)
 {
     double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (int64_t i = 0; i < h->n; ++i)
+    {
+        float o[3] = {0.0f, 0.0f, 0.0f};
+        if (!shift.empty())
+        {
+            const float4 q = shift[(size_t)h->iperm[(size_t)i]];
+            o[0] = q.x;
+            o[1] = q.y;
+            o[2] = q.z;
+        }
         for (int c = 0; c < 3; ++c)
         {
-            lo[c] = std::min(lo[c], (double)h->x[3 * i + c]);
-            hi[c] = std::max(hi[c], (double)h->x[3 * i + c]);
+            const double v = (double)(h->x[3 * i + c] + o[c]);
+            lo[c] = std::min(lo[c], v);
+            hi[c] = std::max(hi[c], v);
         }
+    }
     // Binning grid of edge >= radius / 2: a cluster's search covers its
     // bounding box grown by the radius, about 5-6 cells per axis.
     int64_t total = 1;
@@ -441,7 +534,7 @@ void build_list(bs_handle *h, List &l, bool gated, int32_t cap_override)
     const int32_t *g = gate_of(h, gated);
     hipStream_t s = h->stream;
     launch_fill_i32(h->counts, 0, l.ncells, g, s);
-    launch_cell_ids(h->pos[h->cur], h->jprm, l.jmask, n, h->box, l.dims[0], l.dims[1], l.dims[2], 0.0,
+    launch_cell_ids(h->pos[h->cur], h->shift, h->jprm, l.jmask, n, h->box, l.dims[0], l.dims[1], l.dims[2], 0.0,
                     0.5f * l.radius, 0, h->cell_of, h->counts, g, s);
     launch_scan(h->counts, h->cstart, l.ncells, g, s);
     launch_fill_i32(h->cursor, 0, l.ncells, g, s);
@@ -449,6 +542,7 @@ void build_list(bs_handle *h, List &l, bool gated, int32_t cap_override)
     launch_cell_sort(h->cell_of, n, h->cstart, h->ctmp, nullptr, h->cidx, g, s);
     ListArgs a{};
     a.pos = h->pos[h->cur];
+    a.shift = h->shift;
     a.jprm = h->jprm;
     a.cell_start = h->cstart;
     a.cell_idx = h->cidx;
@@ -481,7 +575,7 @@ void rebuild_chain(bs_handle *h, bool gated)
         return;
     }
     const int32_t *g = gate_of(h, gated);
-    launch_box_reduce(h->pos[h->cur], (int32_t)h->n, h->box_part, h->box, h->box_blocks, g, h->stream);
+    launch_box_reduce(h->pos[h->cur], h->shift, (int32_t)h->n, h->box_part, h->box, h->box_blocks, g, h->stream);
     for (auto &l : h->lists)
         build_list(h, l, gated, -1);
     launch_finish_rebuild(h->pos[h->cur], h->xref, (int32_t)h->n, h->status, h->counters, h->stream);
@@ -490,7 +584,8 @@ void rebuild_chain(bs_handle *h, bool gated)
 
 float rc2(float rc, double f) { return (float)((double)rc * rc * f); }
 
-void launch_pairs(bs_handle *h)
+// energy: also sum this evaluation's energies (the last step of a call).
+void launch_pairs(bs_handle *h, bool energy)
 {
     const int32_t n = (int32_t)h->n;
     for (size_t li = 0; li < h->lists.size(); ++li)
@@ -502,11 +597,12 @@ void launch_pairs(bs_handle *h)
         a.cl = l.cl;
         a.cln = l.cln;
         a.fnb = h->fnb;
-        a.epart = h->pair_part + (size_t)li * h->pair_blocks * 3;
+        a.epp = h->epp;
         a.n = n;
         a.ncl = h->ncl;
         a.cap = l.cap;
         a.accumulate = li > 0 ? 1 : 0;
+        a.energy = energy ? 1 : 0;
         a.rc = l.rc;
         a.rc2lo = rc2(l.rc, 1.0 - 4e-6);
         a.rc2hi = rc2(l.rc, 1.0 + 4e-6);
@@ -545,7 +641,7 @@ GridView grid_view(const bs_handle *h, int kind, bool on, float scale)
     return g;
 }
 
-void launch_springs(bs_handle *h, bool integrate)
+void launch_springs(bs_handle *h, bool integrate, bool energy)
 {
     SpringArgs a{};
     a.pos = h->pos[h->cur];
@@ -564,7 +660,7 @@ void launch_springs(bs_handle *h, bool integrate)
     a.sp_l0 = h->sp_l0;
     a.perm = h->perm_d;
     a.fout = h->fout;
-    a.epart = h->spring_part;
+    a.eps = h->ese;
     a.status = h->status;
     a.n = (int32_t)h->n;
     a.spring_enable = h->p.spring_enable;
@@ -581,6 +677,7 @@ void launch_springs(bs_handle *h, bool integrate)
     const double hs = 0.5 * std::max(h->p.list_skin, 0.0f);
     a.half_skin2 = (float)(hs * hs * (1.0 - 1e-5));
     a.integrate = integrate ? 1 : 0;
+    a.energy = energy ? 1 : 0;
     a.k = h->k;
     a.frig = h->frig;
     // the electrostatic field needs electrostatic.enable (SpringNetwork.cpp:107-113)
@@ -658,8 +755,8 @@ void prof_collect(bs_handle *h)
 
 int finish_call(bs_handle *h)
 {
-    launch_reduce_energies(h->pair_part, h->pair_blocks, (int32_t)h->lists.size(), h->spring_part,
-                           h->spring_blocks, h->energies_d, h->stream);
+    launch_energies(h->lists.empty() ? nullptr : h->epp, h->ese, h->seg_start, h->nseg, h->sys_seg,
+                    (int32_t)h->sys_off.size() - 1, h->seg_part, h->sys_E, h->energies_d, h->stream);
     int32_t st[4];
     HIPCHK(h, hipMemcpyAsync(h->energies, h->energies_d, sizeof(h->energies), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipMemcpyAsync(st, h->status, sizeof st, hipMemcpyDeviceToHost, h->stream));
@@ -672,6 +769,51 @@ int finish_call(bs_handle *h)
         h->nonfinite = st[1];
         return fail(h, BS_ERR_NONFINITE, "Found non-finite position for particle %d.", st[1]);
     }
+    return BS_OK;
+}
+
+// One step: rebuild gate, (gated) list rebuild, pair forces, springs +
+// integration, rigid bodies.  energy: also write the per-particle energies.
+void launch_step(bs_handle *h, bool energy)
+{
+    const size_t e0 = h->prof ? prof_mark(h) : SIZE_MAX;
+    launch_prestep(h->status, h->counters, h->p.rebuild_interval, h->stream);
+    rebuild_chain(h, true);
+    const size_t e1 = h->prof ? prof_mark(h) : SIZE_MAX;
+    if (!h->lists.empty())
+        launch_pairs(h, energy);
+    const size_t e2 = h->prof ? prof_mark(h) : SIZE_MAX;
+    launch_springs(h, true, energy);
+    launch_bodies(h);
+    if (h->prof)
+    {
+        const size_t e3 = prof_mark(h);
+        prof_span(h, BS_PROF_REBUILD, e0, e1);
+        if (!h->lists.empty())
+            prof_span(h, BS_PROF_PAIR, e1, e2);
+        prof_span(h, BS_PROF_SPRING, e2, e3);
+    }
+    h->cur ^= 1;
+}
+
+// Captures G steps (G even, starting at cur = 0) into a HIP graph.  The step
+// holds no host synchronisation and no memset nodes; rebuild decisions stay
+// on the device.
+int capture_graph(bs_handle *h, int32_t G)
+{
+    drop_graph(h);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed));
+    const int cur0 = h->cur;
+    for (int32_t s = 0; s < G; ++s)
+        launch_step(h, false);
+    h->cur = cur0;
+    hipGraph_t g = nullptr;
+    HIPCHK(h, hipStreamEndCapture(h->stream, &g));
+    h->graph = g;
+    HIPCHK(h, hipGraphInstantiate(&h->graph_exec, g, nullptr, nullptr, 0));
+    h->graph_len = G;
+    h->graph_ext = h->ext_uploaded;
     return BS_OK;
 }
 
@@ -906,6 +1048,28 @@ int do_setup(bs_handle *h)
     ALLOC(h->cent_part, double, 3 * h->cent_blocks);
     ALLOC(h->cent, double, 3);
 
+    // energy segments: at most SEG contiguous particles of one system each
+    {
+        constexpr int64_t SEG = 2048;
+        const int32_t nsys = (int32_t)h->sys_off.size() - 1;
+        std::vector<int32_t> ss, sg((size_t)nsys + 1, 0);
+        for (int32_t sy = 0; sy < nsys; ++sy)
+        {
+            sg[(size_t)sy] = (int32_t)ss.size();
+            for (int64_t b = h->sys_off[(size_t)sy]; b < h->sys_off[(size_t)sy + 1]; b += SEG)
+                ss.push_back((int32_t)b);
+        }
+        sg[(size_t)nsys] = (int32_t)ss.size();
+        h->nseg = (int32_t)ss.size();
+        ss.push_back((int32_t)n);
+        ALLOC(h->seg_start, int32_t, ss.size());
+        ALLOC(h->sys_seg, int32_t, sg.size());
+        ALLOC(h->seg_part, double, 6 * std::max(h->nseg, 1));
+        ALLOC(h->sys_E, double, 6 * nsys);
+        HIPCHK(h, hipMemcpy(h->seg_start, ss.data(), sizeof(int32_t) * ss.size(), hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemcpy(h->sys_seg, sg.data(), sizeof(int32_t) * sg.size(), hipMemcpyHostToDevice));
+    }
+
     // status / counters
     const int32_t st0[4] = {1, INT_MAX, 0, 0}, ct0[4] = {0, 0, 0, 0};
     HIPCHK(h, hipMemcpyAsync(h->status, st0, sizeof st0, hipMemcpyHostToDevice, h->stream));
@@ -915,7 +1079,8 @@ int do_setup(bs_handle *h)
     h->ncl = (int32_t)((n + CLUSTER - 1) / CLUSTER);
     h->pair_blocks = (h->ncl + BLOCK / 64 - 1) / (BLOCK / 64);
     h->spring_blocks = (int32_t)((n * h->group + BLOCK - 1) / BLOCK);
-    ALLOC(h->spring_part, double, 4 * h->spring_blocks);
+    ALLOC(h->epp, double4, n);
+    ALLOC(h->ese, double4, n);
 
     // static exclusions per i-cluster: the union of its 8 spring rows as
     // sorted (j, bonded-mask) words; springs never change during a run
@@ -959,18 +1124,28 @@ int do_setup(bs_handle *h)
 
     // neighbour lists: a counting pass sizes the capacity
     plan_lists(h);
-    ALLOC(h->pair_part, double, 3 * h->pair_blocks * std::max<size_t>(1, h->lists.size()));
+    std::vector<float4> shift;
+    if (h->sys_off.size() > 2)
+    {
+        float maxr = 0.0f;
+        for (const auto &l : h->lists)
+            maxr = std::max(maxr, l.radius);
+        shift = system_shifts(h, maxr);
+        ALLOC(h->shift, float4, n);
+        HIPCHK(h, hipMemcpy(h->shift, shift.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+    }
     int32_t maxcells = 1;
     for (auto &l : h->lists)
     {
-        plan_grid(h, l);
+        plan_grid(h, l, shift);
         maxcells = std::max(maxcells, l.ncells);
     }
     rc = ensure_cells(h, maxcells);
     if (rc)
         return rc;
     if (!h->lists.empty())
-        launch_box_reduce(h->pos[h->cur], (int32_t)n, h->box_part, h->box, h->box_blocks, nullptr, h->stream);
+        launch_box_reduce(h->pos[h->cur], h->shift, (int32_t)n, h->box_part, h->box, h->box_blocks, nullptr,
+                          h->stream);
     for (auto &l : h->lists)
     {
         ALLOC(l.cln, int32_t, h->ncl);
@@ -1126,6 +1301,7 @@ int bs_set_particles(bs_handle *h, int64_t n, const float *x3, const float *v3, 
     h->bodies.clear();
     h->body_of.assign((size_t)n, -1);
     h->burying.assign((size_t)n, 1.0f);
+    h->sys_off = {0, n};
     h->s1.clear();
     h->s2.clear();
     h->sl0.clear();
@@ -1199,13 +1375,34 @@ int bs_step(bs_handle *h, int32_t nsteps, const float *ext_force3)
         return BS_ERR_ARG;
     if (nsteps < 0)
         return fail(h, BS_ERR_ARG, "bs_step: negative step count");
+    if (nsteps == 0)
+        return BS_OK;
     int rc = ensure_ready(h);
     if (rc)
         return rc;
     rc = upload_ext(h, ext_force3);
     if (rc)
         return rc;
-    for (int32_t s = 0; s < nsteps; ++s)
+    int32_t s = 0;
+    const int32_t G = (h->p.graph_steps + 1) / 2 * 2;
+    if (G > 0 && !h->prof && nsteps > G)
+    {
+        // replay a captured graph of G steps (from cur = 0 back to cur = 0)
+        for (; s < nsteps - 1 && h->cur != 0; ++s)
+            launch_step(h, false);
+        if (nsteps - 1 - s >= G)
+        {
+            if (!h->graph_exec || h->graph_len != G || h->graph_ext != h->ext_uploaded)
+            {
+                rc = capture_graph(h, G);
+                if (rc)
+                    return rc;
+            }
+            for (; nsteps - 1 - s >= G; s += G)
+                HIPCHK(h, hipGraphLaunch(h->graph_exec, h->stream));
+        }
+    }
+    for (; s < nsteps; ++s)
     {
         // Bound the event pool: fold in what is recorded every 1024 steps.
         if (h->prof && h->evused > 4096)
@@ -1213,26 +1410,10 @@ int bs_step(bs_handle *h, int32_t nsteps, const float *ext_force3)
             HIPCHK(h, hipStreamSynchronize(h->stream));
             prof_collect(h);
         }
-        const size_t e0 = h->prof ? prof_mark(h) : SIZE_MAX;
-        launch_prestep(h->status, h->counters, h->p.rebuild_interval, h->stream);
-        rebuild_chain(h, true);
-        const size_t e1 = h->prof ? prof_mark(h) : SIZE_MAX;
-        if (!h->lists.empty())
-            launch_pairs(h);
-        const size_t e2 = h->prof ? prof_mark(h) : SIZE_MAX;
-        launch_springs(h, true);
-        launch_bodies(h);
-        if (h->prof)
-        {
-            const size_t e3 = prof_mark(h);
-            prof_span(h, BS_PROF_REBUILD, e0, e1);
-            if (!h->lists.empty())
-                prof_span(h, BS_PROF_PAIR, e1, e2);
-            prof_span(h, BS_PROF_SPRING, e2, e3);
-        }
-        h->cur ^= 1;
+        launch_step(h, s + 1 == nsteps);
     }
     h->steps_done += nsteps;
+    h->last_call_forces_only = false;
     return finish_call(h);
 }
 
@@ -1248,10 +1429,11 @@ int bs_compute_forces(bs_handle *h, const float *ext_force3, float *force3, doub
         return rc;
     rebuild_chain(h, true);
     if (!h->lists.empty())
-        launch_pairs(h);
-    launch_springs(h, false);
+        launch_pairs(h, true);
+    launch_springs(h, false, true);
     rc = finish_call(h);
     h->energies[BS_E_KINETIC] = 0.0;
+    h->last_call_forces_only = true;
     if (rc)
         return rc;
     if (force3)
@@ -1416,12 +1598,14 @@ int bs_get_cells(bs_handle *h, int32_t kind, int64_t *ncells_out, int64_t *nmemb
 {
     if (!h || kind < 0 || kind > 2)
         return BS_ERR_ARG;
+    if (h->sys_off.size() > 2)
+        return fail(h, BS_ERR_STATE, "bs_get_cells: the reference cell list is defined for a single system");
     int rc = ensure_ready(h);
     if (rc)
         return rc;
     const int32_t n = (int32_t)h->n;
     // box over every particle, FP64 (box.hpp:37-69)
-    launch_box_reduce(h->pos[h->cur], n, h->box_part, h->box, h->box_blocks, nullptr, h->stream);
+    launch_box_reduce(h->pos[h->cur], nullptr, n, h->box_part, h->box, h->box_blocks, nullptr, h->stream);
     double bx[6];
     HIPCHK(h, hipMemcpyAsync(bx, h->box, sizeof bx, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -1441,7 +1625,7 @@ int bs_get_cells(bs_handle *h, int32_t kind, int64_t *ncells_out, int64_t *nmemb
     const uint32_t mask = kind == BS_NS_ELECTROSTATIC ? PF_CHARGED : kind == BS_NS_HYDROPHOBIC ? PF_HYDRO : 0u;
     hipStream_t s = h->stream;
     launch_fill_i32(h->counts, 0, ncells, nullptr, s);
-    launch_cell_ids(h->pos[h->cur], h->jprm, mask, n, h->box, (int32_t)d[0], (int32_t)d[1], (int32_t)d[2], (double)r,
+    launch_cell_ids(h->pos[h->cur], nullptr, h->jprm, mask, n, h->box, (int32_t)d[0], (int32_t)d[1], (int32_t)d[2], (double)r,
                     r, 1, h->cell_of, h->counts, nullptr, s);
     launch_scan(h->counts, h->cstart, (int32_t)ncells, nullptr, s);
     launch_fill_i32(h->cursor, 0, ncells, nullptr, s);
@@ -1744,6 +1928,143 @@ int bs_get_insertion_vector(bs_handle *h, int64_t i1, int64_t i2, double *out6)
     out6[2] = cs[2];
     for (int c = 0; c < 3; ++c)
         out6[3 + c] = v[c];
+    return BS_OK;
+}
+
+int bs_set_systems(bs_handle *h, int32_t nsys, const int64_t *offsets)
+{
+    if (!h)
+        return BS_ERR_ARG;
+    if (!h->have_particles)
+        return fail(h, BS_ERR_STATE, "bs_set_systems: particles must be set first");
+    if (nsys < 1 || !offsets || offsets[0] != 0 || offsets[nsys] != h->n)
+        return fail(h, BS_ERR_ARG, "bs_set_systems: offsets must run from 0 to the particle count");
+    for (int32_t s = 0; s < nsys; ++s)
+        if (offsets[s + 1] <= offsets[s])
+            return fail(h, BS_ERR_ARG, "bs_set_systems: empty or unordered system %d", s);
+    std::vector<int32_t> sys((size_t)h->n);
+    for (int32_t s = 0; s < nsys; ++s)
+        for (int64_t i = offsets[s]; i < offsets[s + 1]; ++i)
+            sys[(size_t)i] = s;
+    for (size_t t = 0; t < h->s1.size(); ++t)
+        if (sys[h->s1[t]] != sys[h->s2[t]])
+            return fail(h, BS_ERR_ARG, "bs_set_systems: spring %zu joins two systems", t);
+    for (const auto &B : h->bodies)
+        for (int64_t id : B.ids)
+            if (sys[(size_t)id] != sys[(size_t)B.ids[0]])
+                return fail(h, BS_ERR_ARG, "bs_set_systems: a rigid body spans two systems");
+    int rc = pull_and_release(h);
+    if (rc)
+        return rc;
+    h->sys_off.assign(offsets, offsets + nsys + 1);
+    return BS_OK;
+}
+
+int32_t bs_num_systems(const bs_handle *h) { return h ? (int32_t)h->sys_off.size() - 1 : 0; }
+
+int bs_get_system_energies(bs_handle *h, double *out)
+{
+    if (!h || !out)
+        return BS_ERR_ARG;
+    const int32_t nsys = (int32_t)h->sys_off.size() - 1;
+    if (!h->ready || !h->sys_E)
+    {
+        for (int32_t k = 0; k < 6 * nsys; ++k)
+            out[k] = 0.0;
+        return BS_OK;
+    }
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipMemcpy(out, h->sys_E, sizeof(double) * 6 * nsys, hipMemcpyDeviceToHost));
+    if (h->last_call_forces_only)
+        for (int32_t s = 0; s < nsys; ++s)
+            out[6 * s + BS_E_KINETIC] = 0.0;
+    return BS_OK;
+}
+
+int bs_autotune(bs_handle *h, int32_t steps, int32_t *group_out, int32_t *graph_out)
+{
+    if (!h)
+        return BS_ERR_ARG;
+    if (steps < 1)
+        return fail(h, BS_ERR_ARG, "bs_autotune: steps must be positive");
+    int rc = ensure_ready(h);
+    if (rc)
+        return rc;
+    // trials run on a copy of the current system, so h is left as it is
+    std::vector<float> x((size_t)(3 * h->n)), v((size_t)(3 * h->n));
+    rc = bs_get_state(h, x.data(), v.data());
+    if (rc)
+        return rc;
+    static const int32_t groups[] = {4, 8, 16, 32};
+    static const int32_t graphs[] = {0, 16};
+    double best = INFINITY;
+    int32_t bg = h->group, bgr = h->p.graph_steps;
+    for (int32_t gs : graphs)
+        for (int32_t g : groups)
+        {
+            bs_handle *t = nullptr;
+            bs_params p = h->p;
+            p.graph_steps = gs;
+            if (bs_create(&p, &t) != BS_OK)
+                return fail(h, BS_ERR_HIP, "bs_autotune: could not create a trial handle");
+            t->n = h->n;
+            t->x = x;
+            t->v = v;
+            t->mass = h->mass;
+            t->charge = h->charge;
+            t->radius = h->radius;
+            t->eps = h->eps;
+            t->sasa = h->sasa;
+            t->transfer = h->transfer;
+            t->hydro = h->hydro;
+            t->flags = h->flags;
+            t->s1 = h->s1;
+            t->s2 = h->s2;
+            t->sl0 = h->sl0;
+            t->sk = h->sk;
+            t->bodies = h->bodies;
+            t->body_of = h->body_of;
+            t->burying = h->burying;
+            for (int k = 0; k < 2; ++k)
+                t->grids[k] = h->grids[k];
+            t->sys_off = h->sys_off;
+            t->have_particles = t->have_springs = true;
+            t->group = g;
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            int trc = do_setup(t);
+            if (trc == BS_OK)
+                trc = bs_step(t, 2, nullptr);
+            if (trc == BS_OK && hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess)
+            {
+                (void)hipEventRecord(e0, t->stream);
+                trc = bs_step(t, steps, nullptr);
+                (void)hipEventRecord(e1, t->stream);
+                float ms = 0.0f;
+                if (trc == BS_OK && hipEventSynchronize(e1) == hipSuccess &&
+                    hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms < best)
+                {
+                    best = ms;
+                    bg = g;
+                    bgr = gs;
+                }
+            }
+            if (e0)
+                (void)hipEventDestroy(e0);
+            if (e1)
+                (void)hipEventDestroy(e1);
+            bs_destroy(t);
+            if (trc != BS_OK)
+                return fail(h, trc, "bs_autotune: trial failed");
+        }
+    h->group = bg;
+    h->spring_blocks = (int32_t)((h->n * h->group + BLOCK - 1) / BLOCK);
+    if (h->p.graph_steps != bgr)
+        drop_graph(h);
+    h->p.graph_steps = bgr;
+    if (group_out)
+        *group_out = bg;
+    if (graph_out)
+        *graph_out = bgr;
     return BS_OK;
 }
 

@@ -87,9 +87,10 @@ struct PairArgs
     const uint32_t *cl;  // [ncl * cap], zero-padded to a multiple of 16 entries
     const int32_t *cln;  // [ncl]
     double4 *fnb;       // non-bonded force, FP64
-    double *epart;      // [nblocks * 3] steric, elec, hydro
+    double4 *epp;       // [N] per-particle steric, elec, hydro energy (energy steps)
     int32_t n, ncl, cap;
     int32_t accumulate; // add to fnb instead of overwriting
+    int32_t energy;     // also write epp (the last step of a call only)
     float rc, rc2lo, rc2hi;
     int32_t steric_mode;
     float steric_scale;
@@ -159,7 +160,7 @@ struct SpringArgs
     double4 *frig;       // integrate mode: total force of rigid particles (not integrated here)
     GridView gpot, gden; // electrostatic potential field, density field
     double e_field;      // potential * charge -> kJ/mol (electrostatic_field.hpp:15-23)
-    double *epart;       // [nblocks * 4] spring, kinetic, imp, field
+    double4 *eps;        // [N] per-particle spring, kinetic, imp, field energy (energy steps)
     int32_t *status;     // [0] rebuild flag, [1] nonfinite min id, [2] overflow
     int32_t n;
     int32_t spring_enable;
@@ -171,12 +172,14 @@ struct SpringArgs
     float imp_scale, imp_uo, imp_lo, imp_uc, imp_lc;
     float half_skin2;    // (list_skin/2)^2, conservative
     int32_t integrate;
+    int32_t energy;      // also write eps (the last step of a call only)
     Consts k;
 };
 
 struct ListArgs
 {
     const float4 *pos;
+    const float4 *shift;       // nullable: per-particle offset of its system for binning
     const float4 *jprm;
     const int32_t *cell_start; // [ncells + 1]
     const int32_t *cell_idx;   // sorted members
@@ -197,11 +200,14 @@ struct ListArgs
 };
 
 // ---- launchers (bs_kernels.hip) ----
-void launch_box_reduce(const float4 *pos, int32_t n, double *partial, double *box, int32_t nblocks,
-                       const int32_t *gate, hipStream_t s);
-void launch_cell_ids(const float4 *pos, const float4 *jprm, uint32_t mask, int32_t n, const double *box,
-                     int32_t dims0, int32_t dims1, int32_t dims2, double edge_exact, float min_edge,
-                     int32_t exact_mode, int32_t *cell_of, int32_t *counts, const int32_t *gate, hipStream_t s);
+// shift (nullable): per-particle offset added to positions for binning, which
+// keeps independent systems of a batch apart on the engine's grid.
+void launch_box_reduce(const float4 *pos, const float4 *shift, int32_t n, double *partial, double *box,
+                       int32_t nblocks, const int32_t *gate, hipStream_t s);
+void launch_cell_ids(const float4 *pos, const float4 *shift, const float4 *jprm, uint32_t mask, int32_t n,
+                     const double *box, int32_t dims0, int32_t dims1, int32_t dims2, double edge_exact,
+                     float min_edge, int32_t exact_mode, int32_t *cell_of, int32_t *counts, const int32_t *gate,
+                     hipStream_t s);
 void launch_scan(const int32_t *counts, int32_t *start, int32_t ncells, const int32_t *gate, hipStream_t s);
 void launch_scatter(const int32_t *cell_of, int32_t n, const int32_t *start, int32_t *cursor, int32_t *tmp_idx,
                     const int32_t *gate, hipStream_t s);
@@ -210,8 +216,12 @@ void launch_cell_sort(const int32_t *cell_of, int32_t n, const int32_t *start, c
 void launch_list_build(const ListArgs &a, hipStream_t s);
 void launch_pair(const PairArgs &a, int32_t elec_mode, int32_t hydro, int32_t nblocks, hipStream_t s);
 void launch_spring(const SpringArgs &a, int32_t group, int32_t nblocks, hipStream_t s);
-void launch_reduce_energies(const double *pair_part, int32_t npair_blocks, int32_t nlists,
-                            const double *spring_part, int32_t nspring_blocks, double *energies, hipStream_t s);
+// Per-system energies from the per-particle ones: segments of contiguous
+// particles (never crossing a system) are summed by one workgroup each, then
+// every system's segments in order; out_sys[nsys * 6], out_total[6].
+void launch_energies(const double4 *epp, const double4 *eps, const int32_t *seg_start, int32_t nseg,
+                     const int32_t *sys_seg, int32_t nsys, double *seg_part, double *out_sys, double *out_total,
+                     hipStream_t s);
 void launch_prestep(int32_t *status, int32_t *step_counter, int32_t interval, hipStream_t s);
 void launch_finish_rebuild(const float4 *pos, float4 *xref, int32_t n, int32_t *status, int32_t *counters,
                            hipStream_t s);

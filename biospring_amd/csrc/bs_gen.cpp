@@ -18,12 +18,15 @@
 //      12 A cutoffs; viscosity 1 Da/fs; dt = 1 fs.
 //   2  IMPALA: CG beads in 4-bead residues, sasa U[10,100] A^2, transfer
 //      U[-1,1]; springs < 10 A; membrane set by the caller.
+//   3  one system of a batch: workload 0's CA network with the spring cutoff
+//      drawn from U[9,14] A by the system's seed (the caller draws the sizes).
 // Any workload accepts an explicit bead count (n > 0) at the same density.
 #include <algorithm>
 #include <array>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <unordered_map>
 #include <vector>
 
 #include "bs_params_defaults.h"
@@ -204,25 +207,48 @@ void uniform_beads(System & s, int64_t n, double L, double dmin, Rng & rng)
 void chain_in_sphere(System & s, int64_t n, double b, double R, double dmin, Rng & rng)
 {
     s.resize(n);
-    const double lo[3] = {-R, -R, -R}, hi[3] = {R, R, R};
     std::vector<std::array<float, 3>> pos;
     pos.reserve(n);
     const double d2min = dmin * dmin;
+    // hashed grid of edge dmin over the placed beads: the self-avoidance test
+    // looks at the 27 cells around a trial point
+    std::unordered_map<uint64_t, std::vector<int64_t>> grid;
+    auto cell = [&](double v) { return (int64_t)std::floor(v / dmin) + (1 << 20); };
+    auto key = [](int64_t a, int64_t b, int64_t c) { return (uint64_t)a << 42 | (uint64_t)b << 21 | (uint64_t)c; };
     auto fits = [&](const double * p, int64_t upto) {
-        // brute force over a grid rebuilt lazily would be faster; n is small.
-        for (int64_t j = 0; j + 1 < upto; ++j)
-        {
-            const double dx = pos[j][0] - p[0], dy = pos[j][1] - p[1], dz = pos[j][2] - p[2];
-            if (dx * dx + dy * dy + dz * dz < d2min)
-                return false;
-        }
+        const int64_t c0 = cell(p[0]), c1 = cell(p[1]), c2 = cell(p[2]);
+        for (int64_t a = c0 - 1; a <= c0 + 1; ++a)
+            for (int64_t bb = c1 - 1; bb <= c1 + 1; ++bb)
+                for (int64_t c = c2 - 1; c <= c2 + 1; ++c)
+                {
+                    const auto it = grid.find(key(a, bb, c));
+                    if (it == grid.end())
+                        continue;
+                    for (int64_t j : it->second)
+                    {
+                        if (j + 1 >= upto) // the chain predecessor is not tested
+                            continue;
+                        const double dx = pos[j][0] - p[0], dy = pos[j][1] - p[1], dz = pos[j][2] - p[2];
+                        if (dx * dx + dy * dy + dz * dz < d2min)
+                            return false;
+                    }
+                }
         return true;
     };
-    pos.push_back({0.0f, 0.0f, 0.0f});
+    auto push = [&](const std::array<float, 3> & q) {
+        grid[key(cell(q[0]), cell(q[1]), cell(q[2]))].push_back((int64_t)pos.size());
+        pos.push_back(q);
+    };
+    auto pop = [&]() {
+        const auto & q = pos.back();
+        grid[key(cell(q[0]), cell(q[1]), cell(q[2]))].pop_back(); // the largest index is last in its cell
+        pos.pop_back();
+    };
+    push({0.0f, 0.0f, 0.0f});
     int fails = 0;
     while ((int64_t)pos.size() < n)
     {
-        const auto & prev = pos.back();
+        const auto prev = pos.back();
         bool placed = false;
         for (int t = 0; t < 400 && !placed; ++t)
         {
@@ -237,7 +263,7 @@ void chain_in_sphere(System & s, int64_t n, double b, double R, double dmin, Rng
             const double pd[3] = {pf[0], pf[1], pf[2]};
             if (!fits(pd, (int64_t)pos.size()))
                 continue;
-            pos.push_back({pf[0], pf[1], pf[2]});
+            push({pf[0], pf[1], pf[2]});
             placed = true;
         }
         if (!placed)
@@ -245,11 +271,9 @@ void chain_in_sphere(System & s, int64_t n, double b, double R, double dmin, Rng
             ++fails;
             const int64_t back = std::min<int64_t>((int64_t)pos.size() - 1, 1 + fails % 8);
             for (int64_t t = 0; t < back; ++t)
-                pos.pop_back();
+                pop();
         }
     }
-    (void)lo;
-    (void)hi;
     for (int64_t i = 0; i < n; ++i)
         for (int c = 0; c < 3; ++c)
             s.x[3 * i + c] = pos[i][c];
@@ -272,6 +296,7 @@ void bsg_params(int config, bs_params * p)
     switch (config)
     {
     case 0:
+    case 3:
         p->steric_enable = 1;
         p->steric_mode = BS_STERIC_LINEAR;
         p->steric_cutoff = 4.0f;
@@ -308,15 +333,17 @@ bsg_system * bsg_generate(int config, uint64_t seed, int64_t n)
     switch (config)
     {
     case 0:
+    case 3:
     {
         if (n <= 0)
             n = 2000;
+        const double cut = config == 3 ? rng.uniform(9.0, 14.0) : 12.0;
         const double R = 30.0 * std::cbrt((double)n / 2000.0);
         chain_in_sphere(*s, n, 3.8, R, 3.0, rng);
         for (int64_t i = 0; i < n; ++i)
             if (rng.uniform() < 0.2)
                 s->charge[i] = rng.uniform() < 0.5 ? -1.0f : 1.0f;
-        cutoff_springs(*s, 12.0, rng, 0, 0.1);
+        cutoff_springs(*s, cut, rng, 0, 0.1);
         break;
     }
     case 1:

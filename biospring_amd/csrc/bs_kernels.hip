@@ -17,7 +17,8 @@ namespace
 // ---------------------------------------------------------------------------
 // Bounding box, FP64 of float coordinates (box.hpp:37-69); min/max are exact.
 // ---------------------------------------------------------------------------
-__global__ void k_box_partial(const float4 *pos, int32_t n, double *partial, const int32_t *gate)
+__global__ void k_box_partial(const float4 *pos, const float4 *shift, int32_t n, double *partial,
+                              const int32_t *gate)
 {
     if (gated_off(gate))
         return;
@@ -25,7 +26,14 @@ __global__ void k_box_partial(const float4 *pos, int32_t n, double *partial, con
     float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     {
-        const float4 p = pos[i];
+        float4 p = pos[i];
+        if (shift)
+        {
+            const float4 o = shift[i];
+            p.x += o.x;
+            p.y += o.y;
+            p.z += o.z;
+        }
         mn[0] = fminf(mn[0], p.x);
         mn[1] = fminf(mn[1], p.y);
         mn[2] = fminf(mn[2], p.z);
@@ -101,7 +109,8 @@ __global__ void __launch_bounds__(BLOCK) k_box_final(const double *partial, int3
 // exact_mode 1: nsearch.hpp:277-293 (size_t((x - min)/edge), clamped), FP64.
 // exact_mode 0: the engine's own grid, edge >= min_edge on every axis.
 // ---------------------------------------------------------------------------
-__global__ void k_cell_ids(const float4 *pos, const float4 *jprm, uint32_t mask, int32_t n, const double *box,
+__global__ void k_cell_ids(const float4 *pos, const float4 *shift, const float4 *jprm, uint32_t mask, int32_t n,
+                           const double *box,
                            int32_t d0, int32_t d1, int32_t d2, double edge_exact, float min_edge, int32_t exact_mode,
                            int32_t *cell_of, int32_t *counts, const int32_t *gate)
 {
@@ -116,7 +125,14 @@ __global__ void k_cell_ids(const float4 *pos, const float4 *jprm, uint32_t mask,
         cell_of[i] = -1;
         return;
     }
-    const float4 p = pos[i];
+    float4 p = pos[i];
+    if (shift)
+    {
+        const float4 o = shift[i];
+        p.x += o.x;
+        p.y += o.y;
+        p.z += o.z;
+    }
     int32_t c[3];
     const float xs[3] = {p.x, p.y, p.z};
     const int32_t dims[3] = {d0, d1, d2};
@@ -283,7 +299,8 @@ __device__ inline double imp_eval(float xf, float zf, float sasa, float tr, floa
 // Newton step) so that it keeps full accuracy near rest.
 template <bool COMP>
 __device__ inline void spring_edge(float4 pi, float4 li, float4 pj, const float4 *plo, int32_t j, float k, float l0,
-                                   bool jstatic, double cs, float scale, double &fx, double &fy, double &fz, double &e)
+                                   bool jstatic, double cs, float scale, bool energy, double &fx, double &fy,
+                                   double &fz, double &e)
 {
     double dx = (double)pj.x - (double)pi.x;
     double dy = (double)pj.y - (double)pi.y;
@@ -310,7 +327,8 @@ __device__ inline void spring_edge(float4 pi, float4 li, float4 pj, const float4
     fx = fma(dx, fm, fx);
     fy = fma(dy, fm, fy);
     fz = fma(dz, fm, fz);
-    e += (jstatic ? 1.0 : 0.5) * 0.5 * (double)scale * (double)k * ext * ext;
+    if (energy)
+        e += (jstatic ? 1.0 : 0.5) * 0.5 * (double)scale * (double)k * ext * ext;
 }
 
 // Cell of a field grid holding a position, -1 off the grid
@@ -338,7 +356,6 @@ __device__ inline int64_t grid_cell(const GridView &g, float x, float y, float z
 template <int G, bool INTEGRATE, bool COMP>
 __global__ void __launch_bounds__(BLOCK) k_spring(SpringArgs a)
 {
-    __shared__ double sh[BLOCK / 64];
     const int32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     const int32_t row = tid / G;
     const int32_t lane = tid % G;
@@ -381,15 +398,17 @@ __global__ void __launch_bounds__(BLOCK) k_spring(SpringArgs a)
             for (int u = 0; u < U; ++u)
                 if (e0 + u * G < end)
                     spring_edge<COMP>(pi, li, pj[u], a.plo, jr[u] & 0x7fffffff, kk[u], ll[u], jr[u] < 0, cs,
-                                      a.spring_scale, fx, fy, fz, espr);
+                                      a.spring_scale, a.energy != 0, fx, fy, fz, espr);
         }
     }
-    // reduce the force over the G lanes of the row
+    // reduce the force (and the spring energy) over the G lanes of the row
     for (int off = G / 2; off > 0; off >>= 1)
     {
         fx += __shfl_xor(fx, off, G);
         fy += __shfl_xor(fy, off, G);
         fz += __shfl_xor(fz, off, G);
+        if (a.energy)
+            espr += __shfl_xor(espr, off, G);
     }
     if (dyn && lane == 0)
     {
@@ -515,17 +534,8 @@ __global__ void __launch_bounds__(BLOCK) k_spring(SpringArgs a)
             a.pos_out[row] = xn;
         }
     }
-    const double b0 = block_sum(espr, sh);
-    const double b1 = block_sum(eke, sh);
-    const double b2 = block_sum(eimp, sh);
-    const double b3 = block_sum(efld, sh);
-    if (threadIdx.x == 0)
-    {
-        a.epart[4 * blockIdx.x + 0] = b0;
-        a.epart[4 * blockIdx.x + 1] = b1;
-        a.epart[4 * blockIdx.x + 2] = b2;
-        a.epart[4 * blockIdx.x + 3] = b3;
-    }
+    if (a.energy && active && lane == 0)
+        a.eps[row] = make_double4(espr, eke, eimp, efld);
 }
 
 // ---------------------------------------------------------------------------
@@ -670,33 +680,56 @@ __global__ void __launch_bounds__(BLOCK) k_centroid_final(const double *partial,
     }
 }
 
-// Sums the block partials in a fixed order.  energies: spring, elec, steric,
-// kinetic, imp, hydro.  pair partials are [list][block][steric, elec, hydro].
-__global__ void __launch_bounds__(BLOCK) k_reduce_energies(const double *pp, int32_t npb, int32_t nlists,
-                                                           const double *spp, int32_t nsb, double *energies)
+// Energies per segment of contiguous particles (one workgroup each), then per
+// system over its segments in order, then the total over systems in order.
+// epp: steric, elec, hydro; eps: spring, kinetic, imp, field (elec).
+// Slots: spring, elec, steric, kinetic, imp, hydro.
+__global__ void __launch_bounds__(BLOCK) k_seg_energies(const double4 *epp, const double4 *eps,
+                                                        const int32_t *seg_start, double *seg_part)
 {
     __shared__ double sh[BLOCK / 64];
     double acc[6] = {0, 0, 0, 0, 0, 0};
-    for (int32_t l = 0; l < nlists; ++l)
-        for (int32_t b = threadIdx.x; b < npb; b += blockDim.x)
+    for (int32_t i = seg_start[blockIdx.x] + threadIdx.x; i < seg_start[blockIdx.x + 1]; i += blockDim.x)
+    {
+        if (epp)
         {
-            acc[2] += pp[(size_t)(l * npb + b) * 3 + 0];
-            acc[1] += pp[(size_t)(l * npb + b) * 3 + 1];
-            acc[5] += pp[(size_t)(l * npb + b) * 3 + 2];
+            const double4 e = epp[i];
+            acc[2] += e.x;
+            acc[1] += e.y;
+            acc[5] += e.z;
         }
-    if (spp)
-        for (int32_t b = threadIdx.x; b < nsb; b += blockDim.x)
-        {
-            acc[0] += spp[4 * b + 0];
-            acc[3] += spp[4 * b + 1];
-            acc[4] += spp[4 * b + 2];
-            acc[1] += spp[4 * b + 3];
-        }
+        const double4 f = eps[i];
+        acc[0] += f.x;
+        acc[3] += f.y;
+        acc[4] += f.z;
+        acc[1] += f.w;
+    }
     for (int k = 0; k < 6; ++k)
     {
-        const double s = block_sum(acc[k], sh);
+        const double t = block_sum(acc[k], sh);
         if (threadIdx.x == 0)
-            energies[k] = s;
+            seg_part[6 * blockIdx.x + k] = t;
+    }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_sys_energies(const double *seg_part, const int32_t *sys_seg,
+                                                        int32_t nsys, double *out_sys, double *out_total)
+{
+    for (int32_t s = threadIdx.x; s < nsys; s += blockDim.x)
+        for (int k = 0; k < 6; ++k)
+        {
+            double t = 0.0;
+            for (int32_t g = sys_seg[s]; g < sys_seg[s + 1]; ++g)
+                t += seg_part[6 * g + k];
+            out_sys[6 * s + k] = t;
+        }
+    __syncthreads();
+    if (threadIdx.x < 6)
+    {
+        double t = 0.0;
+        for (int32_t s = 0; s < nsys; ++s)
+            t += out_sys[6 * s + threadIdx.x];
+        out_total[threadIdx.x] = t;
     }
 }
 
@@ -739,20 +772,20 @@ inline int32_t nblk(int64_t n, int32_t b = BLOCK) { return (int32_t)((n + b - 1)
 
 // ---- launchers ----
 
-void launch_box_reduce(const float4 *pos, int32_t n, double *partial, double *box, int32_t nblocks,
-                       const int32_t *gate, hipStream_t s)
+void launch_box_reduce(const float4 *pos, const float4 *shift, int32_t n, double *partial, double *box,
+                       int32_t nblocks, const int32_t *gate, hipStream_t s)
 {
-    k_box_partial<<<nblocks, BLOCK, 0, s>>>(pos, n, partial, gate);
+    k_box_partial<<<nblocks, BLOCK, 0, s>>>(pos, shift, n, partial, gate);
     k_box_final<<<1, BLOCK, 0, s>>>(partial, nblocks, box, gate);
 }
 
-void launch_cell_ids(const float4 *pos, const float4 *jprm, uint32_t mask, int32_t n, const double *box,
-                     int32_t d0, int32_t d1, int32_t d2, double edge_exact, float min_edge, int32_t exact_mode,
-                     int32_t *cell_of, int32_t *counts, const int32_t *gate, hipStream_t s)
+void launch_cell_ids(const float4 *pos, const float4 *shift, const float4 *jprm, uint32_t mask, int32_t n,
+                     const double *box, int32_t d0, int32_t d1, int32_t d2, double edge_exact, float min_edge,
+                     int32_t exact_mode, int32_t *cell_of, int32_t *counts, const int32_t *gate, hipStream_t s)
 {
     if (n > 0)
-        k_cell_ids<<<nblk(n), BLOCK, 0, s>>>(pos, jprm, mask, n, box, d0, d1, d2, edge_exact, min_edge, exact_mode,
-                                              cell_of, counts, gate);
+        k_cell_ids<<<nblk(n), BLOCK, 0, s>>>(pos, shift, jprm, mask, n, box, d0, d1, d2, edge_exact, min_edge,
+                                              exact_mode, cell_of, counts, gate);
 }
 
 void launch_scan(const int32_t *counts, int32_t *start, int32_t ncells, const int32_t *gate, hipStream_t s)
@@ -794,10 +827,13 @@ void launch_spring(const SpringArgs &a, int32_t group, int32_t nb, hipStream_t s
     }
 }
 
-void launch_reduce_energies(const double *pp, int32_t npb, int32_t nlists, const double *spp, int32_t nsb,
-                            double *energies, hipStream_t s)
+void launch_energies(const double4 *epp, const double4 *eps, const int32_t *seg_start, int32_t nseg,
+                     const int32_t *sys_seg, int32_t nsys, double *seg_part, double *out_sys, double *out_total,
+                     hipStream_t s)
 {
-    k_reduce_energies<<<1, BLOCK, 0, s>>>(pp, npb, nlists, spp, nsb, energies);
+    if (nseg > 0)
+        k_seg_energies<<<nseg, BLOCK, 0, s>>>(epp, eps, seg_start, seg_part);
+    k_sys_energies<<<1, BLOCK, 0, s>>>(seg_part, sys_seg, nsys, out_sys, out_total);
 }
 
 void launch_prestep(int32_t *status, int32_t *step, int32_t interval, hipStream_t s)

@@ -51,17 +51,29 @@ __global__ void __launch_bounds__(BLOCK) k_cluster_list(ListArgs a)
     const int32_t il = lane & 7;
     const int32_t iself = ci * CLUSTER + il;
     float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
-    uint32_t fi = PF_STATIC;
+    uint32_t fi = PF_STATIC, tag = 0u;
     if (iself < a.n)
     {
         pi = a.pos[iself];
         fi = fbits(a.jprm[iself].w);
+        if (a.shift)
+        { // independent systems: binning in shifted coordinates
+            const float4 o = a.shift[iself];
+            pi.x += o.x;
+            pi.y += o.y;
+            pi.z += o.z;
+            tag = fbits(o.w);
+        }
     }
     const bool vi = lane < CLUSTER && iself < a.n && !(fi & PF_STATIC) && (a.i_mask == 0u || (fi & a.i_mask));
-    const uint32_t vmask = (uint32_t)(__ballot(vi) & 0xFFull);
+    uint32_t todo = (uint32_t)(__ballot(vi) & 0xFFull);
     int32_t count = 0;
-    if (vmask != 0u)
+    // a cluster that straddles two systems searches once per system
+    while (todo != 0u)
     {
+        const uint32_t tag0 = __shfl(tag, __ffs(todo) - 1, 64);
+        const uint32_t vmask = (uint32_t)(__ballot(lane < CLUSTER && ((todo >> lane) & 1u) && tag == tag0) & 0xFFull);
+        todo &= ~vmask;
         float xi[CLUSTER], yi[CLUSTER], zi[CLUSTER];
         float bmin[3] = {INFINITY, INFINITY, INFINITY}, bmax[3] = {-INFINITY, -INFINITY, -INFINITY};
 #pragma unroll
@@ -102,7 +114,14 @@ __global__ void __launch_bounds__(BLOCK) k_cluster_list(ListArgs a)
                     if (idx < e)
                     {
                         j = a.cell_idx[idx];
-                        const float4 pj = a.pos[j];
+                        float4 pj = a.pos[j];
+                        if (a.shift)
+                        {
+                            const float4 o = a.shift[j];
+                            pj.x += o.x;
+                            pj.y += o.y;
+                            pj.z += o.z;
+                        }
                         const uint32_t fj = fbits(a.jprm[j].w);
                         if (a.j_mask == 0u || (fj & a.j_mask))
                         {
@@ -166,10 +185,10 @@ __global__ void __launch_bounds__(BLOCK) k_cluster_list(ListArgs a)
 // costs exec-mask bookkeeping and keeps loads from being issued early.  The
 // exact two-stage test runs for pairs in the guard band around rc^2 only.
 // ---------------------------------------------------------------------------
-template <int SMODE, int EMODE, int HYDRO>
+template <int SMODE, int EMODE, int HYDRO, bool ENERGY>
 __device__ inline void interact(bool masked, float4 pi, float4 ri, uint32_t fli, float4 pj, float4 rj,
-                                const PairArgs &a, float cs, double qi, double &fx, double &fy, double &fz,
-                                double &es, double &ee, double &eh)
+                                const PairArgs &a, float cs, double qe, double qf, double &fx, double &fy,
+                                double &fz, double &es, double &ee, double &eh)
 {
     const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
     const float r2 = dx * dx + dy * dy + dz * dz;
@@ -225,7 +244,8 @@ __device__ inline void interact(bool masked, float4 pi, float4 ri, uint32_t fli,
             en = far ? e0 : 0.0f;
             f = far ? f0 : 0.0f;
         }
-        es += on ? (double)(wgt * a.steric_scale * en) : 0.0;
+        if (ENERGY)
+            es += on ? (double)(wgt * a.steric_scale * en) : 0.0;
         fscal += a.steric_scale * f;
     }
     if (EMODE > 0)
@@ -237,19 +257,19 @@ __device__ inline void interact(bool masked, float4 pi, float4 ri, uint32_t fli,
         const double r2d = qon ? fma(xd, xd, fma(yd, yd, zd * zd)) : 1.0;
         const double y = inv;
         const double invd = y * fma(-0.5 * r2d, y * y, 1.5);
-        const double qq = qon ? qi * (double)pj.w : 0.0;
+        const double qj = qon ? (double)pj.w : 0.0;
         if (EMODE == 1)
         {
-            const double e = (double)wgt * a.ke * qq * invd;
-            ee += on ? e : 0.0;
-            fscd = -(a.kf * qq * invd * invd * invd);
+            if (ENERGY)
+                ee += on ? (double)wgt * qe * qj * invd : 0.0;
+            fscd = -(qf * qj * invd * invd * invd);
         }
         else
         { // eps(r) = eps * r
             const double i2 = invd * invd;
-            const double e = (double)wgt * a.ke * qq * i2;
-            ee += on ? e : 0.0;
-            fscd = -(2.0 * a.kf * qq * i2 * i2);
+            if (ENERGY)
+                ee += on ? (double)wgt * qe * qj * i2 : 0.0;
+            fscd = -(2.0 * qf * qj * i2 * i2);
         }
     }
     if (HYDRO)
@@ -257,7 +277,8 @@ __device__ inline void interact(bool masked, float4 pi, float4 ri, uint32_t fli,
         const bool hon = (fli & PF_HYDRO) && (flj & PF_HYDRO) && pos2;
         const float d = r2s * inv;
         const float hh = hon ? ri.z * rj.z * __expf(-d) : 0.0f;
-        eh += on ? (double)(wgt * a.hydro_scale) * (-(double)hh * 6.02214169E23 * 1.0E-3) : 0.0;
+        if (ENERGY)
+            eh += on ? (double)(wgt * a.hydro_scale) * (-(double)hh * 6.02214169E23 * 1.0E-3) : 0.0;
         fscal += a.hydro_scale * hh * cs * inv;
     }
     const double fs = on ? (double)fscal + fscd : 0.0;
@@ -270,10 +291,9 @@ __device__ inline void interact(bool masked, float4 pi, float4 ri, uint32_t fli,
 // iteration; while iteration t computes, the list words of iteration t+2 and
 // the j data of iteration t+1 are in flight (zero-padded lists: a zero word
 // is a masked-out entry pointing at particle 0).
-template <int SMODE, int EMODE, int HYDRO>
+template <int SMODE, int EMODE, int HYDRO, bool ENERGY>
 __global__ void __launch_bounds__(BLOCK) k_pair(PairArgs a)
 {
-    __shared__ double sh[WAVES];
     const int lane = threadIdx.x & 63;
     const int32_t ci = blockIdx.x * WAVES + (threadIdx.x >> 6);
     const int32_t il = lane & 7, jl = lane >> 3;
@@ -287,7 +307,8 @@ __global__ void __launch_bounds__(BLOCK) k_pair(PairArgs a)
         const float4 pi = iv ? a.pos[i] : make_float4(0.f, 0.f, 0.f, 0.f);
         const float4 ri = iv ? a.jprm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
         const uint32_t fli = fbits(ri.w);
-        const double qi = (double)pi.w;
+        const double qe = (double)pi.w * a.ke; // Coulomb prefactors with q_i folded in
+        const double qf = (double)pi.w * a.kf;
         const uint32_t ibit = 1u << (24 + il);
         const int32_t cnt16 = (a.cln[ci] + 15) & ~15;
         const uint32_t *L = a.cl + (size_t)ci * a.cap;
@@ -314,10 +335,10 @@ __global__ void __launch_bounds__(BLOCK) k_pair(PairArgs a)
             const uint32_t m1 = L[o + 8 + jl];
             const float4 qj0 = a.pos[n0 & CL_JMASK], sj0 = a.jprm[n0 & CL_JMASK];
             const float4 qj1 = a.pos[n1 & CL_JMASK], sj1 = a.jprm[n1 & CL_JMASK];
-            interact<SMODE, EMODE, HYDRO>((w0 & ibit) != 0u, pi, ri, fli, pj0, rj0, a, cs, qi, fx, fy, fz, es, ee,
-                                          eh);
-            interact<SMODE, EMODE, HYDRO>((w1 & ibit) != 0u, pi, ri, fli, pj1, rj1, a, cs, qi, fx, fy, fz, es, ee,
-                                          eh);
+            interact<SMODE, EMODE, HYDRO, ENERGY>((w0 & ibit) != 0u, pi, ri, fli, pj0, rj0, a, cs, qe, qf, fx, fy,
+                                                  fz, es, ee, eh);
+            interact<SMODE, EMODE, HYDRO, ENERGY>((w1 & ibit) != 0u, pi, ri, fli, pj1, rj1, a, cs, qe, qf, fx, fy,
+                                                  fz, es, ee, eh);
             w0 = n0;
             w1 = n1;
             n0 = m0;
@@ -334,15 +355,24 @@ __global__ void __launch_bounds__(BLOCK) k_pair(PairArgs a)
         fx += __shfl_xor(fx, off, 64);
         fy += __shfl_xor(fy, off, 64);
         fz += __shfl_xor(fz, off, 64);
+        if (ENERGY)
+        {
+            es += __shfl_xor(es, off, 64);
+            ee += __shfl_xor(ee, off, 64);
+            eh += __shfl_xor(eh, off, 64);
+        }
     }
-    const double bs = block_sum(es, sh);
-    const double be = block_sum(ee, sh);
-    const double bh = block_sum(eh, sh);
-    if (threadIdx.x == 0)
+    if (ENERGY && ci < a.ncl && jl == 0 && i < a.n)
     {
-        a.epart[3 * blockIdx.x + 0] = bs;
-        a.epart[3 * blockIdx.x + 1] = be;
-        a.epart[3 * blockIdx.x + 2] = bh;
+        double4 e = make_double4(es, ee, eh, 0.0);
+        if (a.accumulate)
+        {
+            const double4 o = a.epp[i];
+            e.x += o.x;
+            e.y += o.y;
+            e.z += o.z;
+        }
+        a.epp[i] = e;
     }
     if (ci < a.ncl && jl == 0 && i < a.n)
     {
@@ -435,7 +465,10 @@ void launch_list_build(const ListArgs &a, hipStream_t s)
 
 template <int S, int E, int H> static void pair3(const PairArgs &a, int32_t nb, hipStream_t s)
 {
-    k_pair<S, E, H><<<nb, BLOCK, 0, s>>>(a);
+    if (a.energy)
+        k_pair<S, E, H, true><<<nb, BLOCK, 0, s>>>(a);
+    else
+        k_pair<S, E, H, false><<<nb, BLOCK, 0, s>>>(a);
 }
 
 template <int S> static void pair2(const PairArgs &a, int32_t em, int32_t h, int32_t nb, hipStream_t s)

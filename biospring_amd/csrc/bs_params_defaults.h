@@ -38,6 +38,7 @@ static inline void bs_params_fill_defaults(bs_params *p)
     p->potentialgrid_scale = 1.0f;
     p->densitygrid_enable = 0;
     p->densitygrid_scale = 1.0f;
+    p->graph_steps = 0;
 }
 
 #endif

@@ -49,4 +49,19 @@ hipError_t hipEventCreate(hipEvent_t *e);
 hipError_t hipEventRecord(hipEvent_t e, hipStream_t s);
 hipError_t hipEventElapsedTime(float *ms, hipEvent_t a, hipEvent_t b);
 hipError_t hipEventDestroy(hipEvent_t e);
+hipError_t hipEventSynchronize(hipEvent_t e);
 const char *hipGetErrorString(hipError_t e);
+// Graphs: capture is not emulated (the launchers here run at once); the
+// calls report "not supported", which the engine turns into an error.
+typedef struct ihipGraph_t *hipGraph_t;
+typedef struct ihipGraphExec_t *hipGraphExec_t;
+typedef struct ihipGraphNode_t *hipGraphNode_t;
+enum hipStreamCaptureMode { hipStreamCaptureModeGlobal = 0, hipStreamCaptureModeThreadLocal = 1,
+                            hipStreamCaptureModeRelaxed = 2 };
+enum { hipErrorNotSupported = 801 };
+hipError_t hipStreamBeginCapture(hipStream_t s, hipStreamCaptureMode mode);
+hipError_t hipStreamEndCapture(hipStream_t s, hipGraph_t *g);
+hipError_t hipGraphInstantiate(hipGraphExec_t *e, hipGraph_t g, hipGraphNode_t *err, char *log, size_t n);
+hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t s);
+hipError_t hipGraphExecDestroy(hipGraphExec_t e);
+hipError_t hipGraphDestroy(hipGraph_t g);

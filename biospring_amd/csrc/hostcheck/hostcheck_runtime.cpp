@@ -161,6 +161,22 @@ extern "C" int64_t hc_live_allocations()
 // ---------------------------------------------------------------------------
 // Launchers: extent checks + CPU work where the host reads results back.
 // ---------------------------------------------------------------------------
+hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) { return hipErrorNotSupported; }
+hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t *g)
+{
+    *g = nullptr;
+    return hipErrorNotSupported;
+}
+hipError_t hipGraphInstantiate(hipGraphExec_t *e, hipGraph_t, hipGraphNode_t *, char *, size_t)
+{
+    *e = nullptr;
+    return hipErrorNotSupported;
+}
+hipError_t hipGraphLaunch(hipGraphExec_t, hipStream_t) { return hipErrorNotSupported; }
+hipError_t hipGraphExecDestroy(hipGraphExec_t) { return hipSuccess; }
+hipError_t hipGraphDestroy(hipGraph_t) { return hipSuccess; }
+
 namespace bs
 {
 
@@ -179,10 +195,24 @@ static uint32_t fbits(float f)
     return u;
 }
 
-void launch_box_reduce(const float4 *pos, int32_t n, double *partial, double *box, int32_t nblocks,
-                       const int32_t *gate, hipStream_t)
+static float4 shifted(const float4 *pos, const float4 *shift, int32_t i)
+{
+    float4 p = pos[i];
+    if (shift)
+    {
+        p.x += shift[i].x;
+        p.y += shift[i].y;
+        p.z += shift[i].z;
+    }
+    return p;
+}
+
+void launch_box_reduce(const float4 *pos, const float4 *shift, int32_t n, double *partial, double *box,
+                       int32_t nblocks, const int32_t *gate, hipStream_t)
 {
     need(pos, sizeof(float4) * n, "box: pos");
+    if (shift)
+        need(shift, sizeof(float4) * n, "box: shift");
     need(partial, sizeof(double) * 6 * nblocks, "box: partial");
     need(box, sizeof(double) * 6, "box: box");
     if (nblocks < 1 || nblocks > 4096)
@@ -196,7 +226,8 @@ void launch_box_reduce(const float4 *pos, int32_t n, double *partial, double *bo
     }
     for (int32_t i = 0; i < n; ++i)
     {
-        const float xs[3] = {pos[i].x, pos[i].y, pos[i].z};
+        const float4 p = shifted(pos, shift, i);
+        const float xs[3] = {p.x, p.y, p.z};
         for (int c = 0; c < 3; ++c)
         {
             box[c] = std::min(box[c], (double)xs[c]);
@@ -213,12 +244,14 @@ static int32_t internal_coord(float x, double mn, double len, int32_t dim, float
     return c < dim - 1 ? c : dim - 1;
 }
 
-void launch_cell_ids(const float4 *pos, const float4 *jprm, uint32_t mask, int32_t n, const double *box, int32_t d0,
-                     int32_t d1, int32_t d2, double edge_exact, float min_edge, int32_t exact_mode, int32_t *cell_of,
-                     int32_t *counts, const int32_t *gate, hipStream_t)
+void launch_cell_ids(const float4 *pos, const float4 *shift, const float4 *jprm, uint32_t mask, int32_t n,
+                     const double *box, int32_t d0, int32_t d1, int32_t d2, double edge_exact, float min_edge,
+                     int32_t exact_mode, int32_t *cell_of, int32_t *counts, const int32_t *gate, hipStream_t)
 {
     const int64_t nc = (int64_t)d0 * d1 * d2;
     need(pos, sizeof(float4) * n, "cell_ids: pos");
+    if (shift)
+        need(shift, sizeof(float4) * n, "cell_ids: shift");
     need(jprm, sizeof(float4) * n, "cell_ids: jprm");
     need(box, sizeof(double) * 6, "cell_ids: box");
     need(cell_of, sizeof(int32_t) * n, "cell_ids: cell_of");
@@ -233,7 +266,8 @@ void launch_cell_ids(const float4 *pos, const float4 *jprm, uint32_t mask, int32
             cell_of[i] = -1;
             continue;
         }
-        const float xs[3] = {pos[i].x, pos[i].y, pos[i].z};
+        const float4 p = shifted(pos, shift, i);
+        const float xs[3] = {p.x, p.y, p.z};
         int32_t c[3];
         for (int a = 0; a < 3; ++a)
         {
@@ -335,11 +369,12 @@ void launch_list_build(const ListArgs &a, hipStream_t)
         return;
     if (a.exclude_bonded)
         need(a.ex_w, sizeof(uint32_t) * std::max(a.ex_ptr[a.ncl], 1), "list: ex_w");
+    if (a.shift)
+        need(a.shift, sizeof(float4) * a.n, "list: shift");
     for (int32_t ci = 0; ci < a.ncl; ++ci)
     {
-        uint32_t vmask = 0;
+        uint32_t valid = 0, tag[CLUSTER] = {};
         float xi[CLUSTER], yi[CLUSTER], zi[CLUSTER];
-        double bmin[3] = {INFINITY, INFINITY, INFINITY}, bmax[3] = {-INFINITY, -INFINITY, -INFINITY};
         for (int k = 0; k < CLUSTER; ++k)
         {
             const int32_t i = ci * CLUSTER + k;
@@ -347,23 +382,33 @@ void launch_list_build(const ListArgs &a, hipStream_t)
             if (i >= a.n)
                 continue;
             const uint32_t fi = fbits(a.jprm[i].w);
-            xi[k] = a.pos[i].x;
-            yi[k] = a.pos[i].y;
-            zi[k] = a.pos[i].z;
+            const float4 p = shifted(a.pos, a.shift, i);
+            xi[k] = p.x;
+            yi[k] = p.y;
+            zi[k] = p.z;
+            tag[k] = a.shift ? fbits(a.shift[i].w) : 0u;
             if (!(fi & PF_STATIC) && (a.i_mask == 0u || (fi & a.i_mask)))
-            {
-                vmask |= 1u << k;
-                const float v[3] = {xi[k], yi[k], zi[k]};
-                for (int q = 0; q < 3; ++q)
-                {
-                    bmin[q] = std::min(bmin[q], (double)v[q]);
-                    bmax[q] = std::max(bmax[q], (double)v[q]);
-                }
-            }
+                valid |= 1u << k;
         }
         int32_t count = 0;
-        if (vmask)
-        {
+        uint32_t todo = valid;
+        while (todo)
+        { // one search per system among the cluster's particles
+            const uint32_t t0 = tag[__builtin_ctz(todo)];
+            uint32_t vmask = 0;
+            double bmin[3] = {INFINITY, INFINITY, INFINITY}, bmax[3] = {-INFINITY, -INFINITY, -INFINITY};
+            for (int k = 0; k < CLUSTER; ++k)
+                if (((todo >> k) & 1u) && tag[k] == t0)
+                {
+                    vmask |= 1u << k;
+                    const float v[3] = {xi[k], yi[k], zi[k]};
+                    for (int q = 0; q < 3; ++q)
+                    {
+                        bmin[q] = std::min(bmin[q], (double)v[q]);
+                        bmax[q] = std::max(bmax[q], (double)v[q]);
+                    }
+                }
+            todo &= ~vmask;
             int32_t clo[3], chi[3];
             for (int q = 0; q < 3; ++q)
             {
@@ -384,10 +429,11 @@ void launch_list_build(const ListArgs &a, hipStream_t)
                             hc_fail("list: member out of range");
                         if (a.j_mask && !(fbits(a.jprm[j].w) & a.j_mask))
                             continue;
+                        const float4 pj = shifted(a.pos, a.shift, j);
                         uint32_t m = 0;
                         for (int k = 0; k < CLUSTER; ++k)
                         {
-                            const float dx = a.pos[j].x - xi[k], dy = a.pos[j].y - yi[k], dz = a.pos[j].z - zi[k];
+                            const float dx = pj.x - xi[k], dy = pj.y - yi[k], dz = pj.z - zi[k];
                             if (dx * dx + dy * dy + dz * dz < a.r2list && j != ci * CLUSTER + k)
                                 m |= 1u << k;
                         }
@@ -422,7 +468,8 @@ void launch_pair(const PairArgs &a, int32_t, int32_t, int32_t nb, hipStream_t)
     need(a.cln, sizeof(int32_t) * a.ncl, "pair: cln");
     need(a.cl, sizeof(uint32_t) * (size_t)a.ncl * a.cap, "pair: cl");
     need(a.fnb, sizeof(double4) * a.n, "pair: fnb");
-    need(a.epart, sizeof(double) * 3 * nb, "pair: epart");
+    if (a.energy)
+        need(a.epp, sizeof(double4) * a.n, "pair: epp");
     if ((int64_t)nb * (BLOCK / 64) < a.ncl)
         hc_fail("pair: grid too small");
     for (int32_t ci = 0; ci < a.ncl; ++ci)
@@ -447,8 +494,11 @@ void launch_pair(const PairArgs &a, int32_t, int32_t, int32_t nb, hipStream_t)
     }
     for (int32_t i = 0; i < a.n; ++i)
         if (!a.accumulate)
+        {
             a.fnb[i] = make_double4(0, 0, 0, 0);
-    std::memset(a.epart, 0, sizeof(double) * 3 * nb);
+            if (a.energy)
+                a.epp[i] = make_double4(0, 0, 0, 0);
+        }
 }
 
 void launch_spring(const SpringArgs &a, int32_t group, int32_t nb, hipStream_t)
@@ -462,7 +512,8 @@ void launch_spring(const SpringArgs &a, int32_t group, int32_t nb, hipStream_t)
     need(a.sp_ptr, sizeof(int32_t) * (n + 1), "spring: sp_ptr");
     need(a.perm, sizeof(int32_t) * n, "spring: perm");
     need(a.status, sizeof(int32_t) * 4, "spring: status");
-    need(a.epart, sizeof(double) * 4 * nb, "spring: epart");
+    if (a.energy)
+        need(a.eps, sizeof(double4) * n, "spring: eps");
     for (const GridView *g : {&a.gpot, &a.gden})
         if (g->cells)
             need(g->cells, sizeof(float4) * (size_t)g->shape[0] * g->shape[1] * g->shape[2], "spring: grid");
@@ -501,18 +552,33 @@ void launch_spring(const SpringArgs &a, int32_t group, int32_t nb, hipStream_t)
         for (int32_t i = 0; i < n; ++i)
             if (!(fbits(a.jprm[i].w) & (PF_STATIC | PF_RIGID)))
                 const_cast<float4 *>(a.pos_out)[i] = a.pos[i];
-    std::memset(a.epart, 0, sizeof(double) * 4 * nb);
+    if (a.energy)
+        for (int32_t i = 0; i < n; ++i)
+            a.eps[i] = make_double4(0, 0, 0, 0);
 }
 
-void launch_reduce_energies(const double *pp, int32_t npb, int32_t nlists, const double *spp, int32_t nsb,
-                            double *energies, hipStream_t)
+void launch_energies(const double4 *epp, const double4 *eps, const int32_t *seg_start, int32_t nseg,
+                     const int32_t *sys_seg, int32_t nsys, double *seg_part, double *out_sys, double *out_total,
+                     hipStream_t)
 {
-    need(pp, sizeof(double) * 3 * npb * std::max(nlists, 0), "reduce: pair partials");
-    if (spp)
-        need(spp, sizeof(double) * 4 * nsb, "reduce: spring partials");
-    need(energies, sizeof(double) * 6, "reduce: energies");
+    need(seg_start, sizeof(int32_t) * (nseg + 1), "energies: seg_start");
+    need(sys_seg, sizeof(int32_t) * (nsys + 1), "energies: sys_seg");
+    need(seg_part, sizeof(double) * 6 * std::max(nseg, 1), "energies: seg_part");
+    need(out_sys, sizeof(double) * 6 * nsys, "energies: out_sys");
+    need(out_total, sizeof(double) * 6, "energies: out_total");
+    const int32_t n = seg_start[nseg];
+    need(eps, sizeof(double4) * n, "energies: eps");
+    if (epp)
+        need(epp, sizeof(double4) * n, "energies: epp");
+    if (sys_seg[0] != 0 || sys_seg[nsys] != nseg)
+        hc_fail("energies: system segment ranges do not cover the segments");
+    for (int32_t g = 0; g < nseg; ++g)
+        if (seg_start[g + 1] <= seg_start[g])
+            hc_fail("energies: empty segment");
+    for (int k = 0; k < 6 * nsys; ++k)
+        out_sys[k] = 0.0;
     for (int k = 0; k < 6; ++k)
-        energies[k] = 0.0;
+        out_total[k] = 0.0;
 }
 
 void launch_rigid(const RigidArgs &a, hipStream_t)

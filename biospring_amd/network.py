@@ -213,6 +213,29 @@ class SpringNetwork:
               "bs_get_insertion_vector")
         return out
 
+    # -- batches of independent systems --
+    def setSystems(self, offsets):
+        """Particles [offsets[s], offsets[s+1]) form independent system s."""
+        o = np.ascontiguousarray(offsets, np.int64)
+        check(self._L.bs_set_systems(self._h, o.size - 1, ptr(o, ctypes.c_int64)), self._h, "bs_set_systems")
+
+    def numSystems(self):
+        return int(self._L.bs_num_systems(self._h))
+
+    def getSystemEnergies(self):
+        """(nsys, 6) energies of each system after the last step."""
+        out = np.zeros(6 * self.numSystems())
+        check(self._L.bs_get_system_energies(self._h, ptr(out, ctypes.c_double)), self._h,
+              "bs_get_system_energies")
+        return out.reshape(-1, 6)
+
+    def autotune(self, steps=20):
+        """Picks the fastest launch configuration on a copy of the system;
+        returns (lanes per spring row, graph steps)."""
+        g, gr = ctypes.c_int32(), ctypes.c_int32()
+        check(self._L.bs_autotune(self._h, int(steps), ctypes.byref(g), ctypes.byref(gr)), self._h, "bs_autotune")
+        return int(g.value), int(gr.value)
+
     def close(self):
         if self._h:
             self._L.bs_destroy(self._h)

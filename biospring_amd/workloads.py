@@ -119,3 +119,24 @@ def impala_params(system, membrane="double"):
         p.imp_upper_offset, p.imp_lower_offset = 30.0, 30.0
         p.imp_upper_curvature, p.imp_lower_curvature = 1.0 / 200.0, -1.0 / 200.0
     return p
+
+
+def batch(nsys=512, seed=1, nmin=200, nmax=20000):
+    """Workload 3: `nsys` independent CA networks, sizes log-uniform in
+    [nmin, nmax], each generated like workload 0 from its own seed with a
+    spring cutoff drawn from U[9, 14] A.  Returns the concatenated system
+    and the particle offsets of the systems (nsys + 1)."""
+    rng = np.random.default_rng(seed)
+    sizes = np.rint(np.exp(rng.uniform(np.log(nmin), np.log(nmax), nsys))).astype(np.int64)
+    parts = [generate(3, seed=int(seed) * 100003 + s, n=int(sizes[s])) for s in range(nsys)]
+    off = np.zeros(nsys + 1, np.int64)
+    off[1:] = np.cumsum([p.n for p in parts])
+    cat = lambda k: np.concatenate([getattr(p, k) for p in parts])
+    out = System(x=cat("x"), v=cat("v"), mass=cat("mass"), charge=cat("charge"), radius=cat("radius"),
+                 eps=cat("eps"), sasa=cat("sasa"), transfer=cat("transfer"), hydro=cat("hydro"),
+                 flags=cat("flags"),
+                 id1=np.concatenate([p.id1 + np.uint32(off[s]) for s, p in enumerate(parts)]),
+                 id2=np.concatenate([p.id2 + np.uint32(off[s]) for s, p in enumerate(parts)]),
+                 l0=cat("l0"), k=cat("k"))
+    out.params = parts[0].params.copy()
+    return out, off, parts

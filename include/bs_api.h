@@ -120,6 +120,12 @@ typedef struct bs_params {
     int32_t densitygrid_enable;   /* densitygrid.enable: density field force of grid BS_GRID_DENSITY
                                      (Particle.cpp:136-159), default 0 */
     float densitygrid_scale;      /* densitygrid.scale, default 1 */
+
+    /* Engine option: bs_step replays a captured HIP graph of this many steps
+       (rounded up to even; the last step of a call, which also sums the
+       energies, is launched directly).  0 launches every kernel directly.
+       Results are identical either way.  Default 0 */
+    int32_t graph_steps;
 } bs_params;
 
 typedef struct bs_handle bs_handle;
@@ -311,6 +317,32 @@ int bs_set_burying(bs_handle *h, const float *burying);
  * (SpringNetwork.cpp:52-59), which the reference runs every step; here it
  * runs when asked for. */
 int bs_get_insertion_vector(bs_handle *h, int64_t i1, int64_t i2, double *out6);
+
+/* ---- batches of independent systems ------------------------------------------ */
+
+/* Declares the particles as nsys independent systems: system s is particles
+ * [offsets[s], offsets[s+1]) (offsets[0] = 0, offsets[nsys] = n).  Springs and
+ * rigid bodies must stay inside one system.  Systems never interact; each is
+ * stepped exactly as alone (the parameters are shared), in one launch
+ * sequence over all of them.  Per-system energies: bs_get_system_energies.
+ * No reference counterpart (the reference runs one network per process);
+ * serves many small networks on one GPU. */
+int bs_set_systems(bs_handle *h, int32_t nsys, const int64_t *offsets);
+
+/* Number of systems (1 unless bs_set_systems declared more). */
+int32_t bs_num_systems(const bs_handle *h);
+
+/* The six energies of every system after the last bs_step / bs_compute_forces,
+ * out[6 * s + BS_E_*]. */
+int bs_get_system_energies(bs_handle *h, double *out);
+
+/* ---- tuning ---------------------------------------------------------------------- */
+
+/* Times `steps` steps of a copy of the current system for each launch
+ * configuration (lanes per spring row 4/8/16/32; direct launches or a 16-step
+ * graph) and keeps the fastest for h; h's state is not advanced.  The chosen
+ * values go to *group_out / *graph_out (nullable).  No reference counterpart. */
+int bs_autotune(bs_handle *h, int32_t steps, int32_t *group_out, int32_t *graph_out);
 
 #ifdef __cplusplus
 }

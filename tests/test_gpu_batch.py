@@ -1,0 +1,70 @@
+"""Batches of independent systems (workload 3 at reduced size) on the GPU:
+each system against its own FP64 oracle -- forces, per-system energies and a
+short trajectory -- with the float-faithful oracle as the yardstick."""
+import sys
+
+import numpy as np
+import pytest
+
+from biospring_amd import workloads
+from biospring_amd.network import SpringNetwork
+from oracle import pyoracle as po
+from tests.parity import ENERGY_RTOL, FORCE_RTOL, energy_error, force_errors
+
+pytestmark = pytest.mark.gpu
+
+
+def _log(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def test_batch_matches_each_system_alone():
+    s, off, parts = workloads.batch(12, seed=5, nmax=2500)
+    _log("batch n", s.n, "m", s.m, "sizes", np.diff(off).tolist())
+    net = SpringNetwork.from_system(s)
+    net.setSystems(off)
+    F, E = net.computeForces()
+    Es = net.getSystemEnergies()
+    assert np.allclose(Es.sum(0), E, rtol=1e-12, atol=1e-12)
+    for k, p in enumerate(parts):
+        o = po.Oracle(p)
+        Fd, Ed = o.compute_forces(po.F64)
+        sl = slice(off[k], off[k + 1])
+        per, glob = force_errors(F[sl], Fd, (p.flags & 1) == 0)
+        assert per <= FORCE_RTOL, (k, per)
+        for slot in range(6):
+            assert energy_error(Es[k, slot], Ed[slot], floor=1e-3) <= ENERGY_RTOL, (k, slot, Es[k, slot], Ed[slot])
+    net.run(40)
+    x, _ = net.getState()
+    Es = net.getSystemEnergies()
+    worst = 0.0
+    for k, p in enumerate(parts):
+        o64, o32 = po.Oracle(p), po.Oracle(p)
+        o64.step(po.F64, 40)
+        o32.step(po.F32, 40)
+        xd, _ = o64.state(po.F64)
+        xf, _ = o32.state(po.F32)
+        sl = slice(off[k], off[k + 1])
+        dev = np.abs(x[sl] - xd).max()
+        dref = np.abs(xf - xd).max()
+        worst = max(worst, dev / max(dref, 1e-12))
+        assert dev <= 3 * dref + 1e-6, (k, dev, dref)
+    _log(f"  40 steps: worst gpu/float-path deviation ratio {worst:.2f}")
+
+
+def test_batch_graph_replay_matches_direct_launches():
+    s, off, _ = workloads.batch(6, seed=6, nmax=1500)
+    a = SpringNetwork.from_system(s)
+    a.setSystems(off)
+    p = s.params.copy()
+    p.graph_steps = 4
+    b = SpringNetwork.from_system(s, params=p)
+    b.setSystems(off)
+    for n in (7, 10, 1):
+        a.run(n)
+        b.run(n)
+        xa, va = a.getState()
+        xb, vb = b.getState()
+        assert np.array_equal(xa, xb) and np.array_equal(va, vb), n
+        assert np.array_equal(a.getSystemEnergies(), b.getSystemEnergies()), n

@@ -187,3 +187,32 @@ def test_rigid_and_grid_argument_errors(hostcheck_lib):
             assert e.code == _abi.BS_ERR_ARG
         net.close()
     """, hostcheck_lib)
+
+
+def test_batch_of_systems_sequence(hostcheck_lib):
+    out = _run("""
+        s, off, parts = workloads.batch(9, seed=3, nmax=1500)
+        base = L.hc_live_allocations()
+        net = SpringNetwork.from_system(s, library=L)
+        net.setSystems(off)
+        assert net.numSystems() == 9
+        net.setup(); net.run(4); net.computeForces()
+        E = net.getSystemEnergies(); assert E.shape == (9, 6)
+        net.addRigidBody(np.arange(off[2], off[2] + 50)); net.run(2)
+        try:
+            net.getCells(0)
+            raise SystemExit("cells accepted for a batch")
+        except _abi.EngineError:
+            pass
+        net.getPairs(1)
+        bad = off.copy(); bad[3] = bad[2]
+        try:
+            net.setSystems(bad)
+            raise SystemExit("accepted empty system")
+        except _abi.EngineError as e:
+            assert e.code == _abi.BS_ERR_ARG
+        net.setSystems([0, s.n]); net.run(2)
+        net.close()
+        print("leak", L.hc_live_allocations() - base)
+    """, hostcheck_lib)
+    assert "leak 0" in out, out
