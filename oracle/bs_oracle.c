@@ -2229,6 +2229,269 @@ static int integrate_f64(orc *o)
 }
 
 /* ---------------------------------------------------------------------------
+ * Adjoint (no reference counterpart): gradients of the trajectory loss
+ *   L = (1/n) sum_i |x_T,i - y_i|^2
+ * after T FP64 steps with respect to spring stiffness k, rest length L0 and
+ * particle charge q.  Reverse sweep of the semi-implicit Euler step
+ * v' = v + dt/m (F(x) - gamma v), x' = x + dt v':
+ *   a = lv + dt lx,  w = dt/m a,  lx += H(x) w,  lv = (1 - gamma dt/m) a,
+ * with H = dF/dx (symmetric: the terms are conservative) and the parameter
+ * gradients sum_i w_i . dF_i/dtheta.  Each pair term acts along d = x_j - x_i
+ * with F_i = phi(r) d, phi = fm / r (fm: the force modules above), so
+ *   H w |_i = phi dw + (phi'/r) (d . dw) d,  dw = w_j - w_i,  phi' = (fm' - phi)/r.
+ * The pair set of each step is the forward's (frozen at each state).
+ * Terms: springs, the four steric modes, Coulomb (both dielectric models),
+ * viscosity.  IMPALA, the pseudo-hydrophobic term, grids and rigid bodies
+ * are not differentiated (orc_adjoint returns NAN when they are on).
+ * ------------------------------------------------------------------------- */
+static double steric_dfm_d(int mode, double ri, double rj, double ei, double ej, double d)
+{
+    switch (mode) {
+    case BS_STERIC_AMBER_12_6: {
+        if (d < K_MIN_VDW)
+            return 0.0;
+        const double eij = sqrt(ei * ej), R = sqrt(ri * rj), s = R / d;
+        const double s6 = s * s * s * s * s * s;
+        return 12.0 * eij * (-7.0 * s6 + 13.0 * s6 * s6) / (d * d) * c_spring();
+    }
+    case BS_STERIC_LEWITT_8_6: {
+        if (d < K_MIN_VDW)
+            return 0.0;
+        const double eij = sqrt(ei * ej), R = sqrt(ri * rj), s = R / d;
+        const double s2 = s * s, s6 = s2 * s2 * s2;
+        return 24.0 * eij * (-7.0 * s6 + 9.0 * s6 * s2) / (d * d) * c_spring();
+    }
+    case BS_STERIC_ZACHARIAS_8_6: {
+        if (d < K_MIN_VDW)
+            return 0.0;
+        const double eij = ei * ej, s = ri * rj / d;
+        const double s2 = s * s, s6 = s2 * s2 * s2;
+        return eij * (-42.0 * s6 + 72.0 * s6 * s2) / (d * d) * c_spring();
+    }
+    default: {
+        const double dv = d - (ri + rj);
+        return dv > 0 ? 0.0 : c_spring();
+    }
+    }
+}
+
+static double elec_dfm_d(double q1, double q2, double d, double diel, int ddd)
+{
+    if (d < K_MIN_ELEC)
+        return 0.0;
+    const double kq = (q1 * q2) / (4.0 * K_PI * diel) * c_elec();
+    return ddd ? 6.0 * kq / (d * d * d * d) : 2.0 * kq / (d * d * d);
+}
+
+/* One accepted pair of the adjoint sweep (see adjoint_terms). */
+static void adjoint_pair(orc *o, int kind, int64_t i, int64_t j, const float *pos, float cutoff, const double *w,
+                         double *hw, double *dq)
+{
+    const bs_params *p = &o->p;
+    float dfl;
+    if (!pair_accept(o, kind, i, j, pos, cutoff, &dfl))
+        return;
+    double d[3], dw[3];
+    for (int c = 0; c < 3; ++c) {
+        d[c] = o->xd[3 * j + c] - o->xd[3 * i + c];
+        dw[c] = w[3 * j + c] - w[3 * i + c];
+    }
+    const double r = norm_d(d[0], d[1], d[2]);
+    if (r == 0.0)
+        return;
+    double fm, dfm;
+    if (kind == BS_NS_STERIC) {
+        fm = p->steric_gridscale * steric_fm_d(p->steric_mode, o->radius[j], o->radius[i], o->eps[j], o->eps[i], r);
+        dfm = p->steric_gridscale *
+              steric_dfm_d(p->steric_mode, o->radius[j], o->radius[i], o->eps[j], o->eps[i], r);
+    } else {
+        fm = p->electrostatic_scale *
+             elec_fm_d(o->charge[j], o->charge[i], r, p->electrostatic_dielectric, p->dielectric_model);
+        dfm = p->electrostatic_scale *
+              elec_dfm_d(o->charge[j], o->charge[i], r, p->electrostatic_dielectric, p->dielectric_model);
+    }
+    const double phi = fm / r, dphi = (dfm - phi) / r;
+    const double dd = d[0] * dw[0] + d[1] * dw[1] + d[2] * dw[2];
+    for (int c = 0; c < 3; ++c) {
+        const double hv = phi * dw[c] + dphi / r * dd * d[c];
+        hw[3 * i + c] += hv;
+        if (!o->isstatic[j])
+            hw[3 * j + c] -= hv;
+    }
+    if (kind == BS_NS_ELECTROSTATIC) {
+        const double qq = (double)o->charge[i] * o->charge[j];
+        if (qq != 0.0) {
+            const double g = -(phi / qq) * dd; /* c (w_i - w_j) . d */
+            dq[i] += g * o->charge[j];
+            dq[j] += g * o->charge[i];
+        }
+    }
+}
+
+/* H w and the parameter gradients at the current F64 state; hw[3n] is added to. */
+static void adjoint_terms(orc *o, const double *w, double *hw, double *dk, double *dl0, double *dq)
+{
+    const bs_params *p = &o->p;
+    const int64_t n = o->n;
+    if (p->spring_enable)
+        for (int64_t t = 0; t < o->nds; ++t) {
+            const int64_t s = o->dsp[t];
+            const uint32_t a = o->s1[s], b = o->s2[s];
+            double d[3], dw[3];
+            for (int c = 0; c < 3; ++c) {
+                d[c] = o->xd[3 * b + c] - o->xd[3 * a + c];
+                dw[c] = w[3 * b + c] - w[3 * a + c];
+            }
+            const double r = norm_d(d[0], d[1], d[2]);
+            if (r == 0.0)
+                continue;
+            const double C = c_spring() * p->spring_scale;
+            const double fm = o->sk[s] * (r - o->sl0[s]) * C, dfm = o->sk[s] * C;
+            const double phi = fm / r, dphi = (dfm - phi) / r;
+            const double dd = d[0] * dw[0] + d[1] * dw[1] + d[2] * dw[2];
+            for (int c = 0; c < 3; ++c) {
+                const double hv = phi * dw[c] + dphi / r * dd * d[c];
+                if (!o->isstatic[a])
+                    hw[3 * a + c] += hv;
+                if (!o->isstatic[b])
+                    hw[3 * b + c] -= hv;
+            }
+            const double wu = -dd / r; /* (w_a - w_b) . d / r */
+            dk[s] += wu * (r - o->sl0[s]) * C;
+            dl0[s] += -wu * o->sk[s] * C;
+        }
+    const float *pos = decision_positions(o, ORC_F64);
+    for (int kind = 0; kind < 2; ++kind) {
+        if (!kind_enabled(o, kind))
+            continue;
+        int any = kind == BS_NS_STERIC;
+        for (int64_t i = 0; i < n && !any; ++i)
+            any = member_of(o, kind, i);
+        if (!any)
+            continue;
+        cells_t cl;
+        build_cells(o, kind, pos, &cl);
+        for (int64_t t = 0; t < o->ndyn; ++t) {
+            const int64_t i = o->dyn[t];
+            if (kind == BS_NS_ELECTROSTATIC && !o->charged[i])
+                continue;
+            FOR_EACH_NEIGHBOR(&cl, pos, i, j, { adjoint_pair(o, kind, i, j, pos, cl.cutoff, w, hw, dq); });
+        }
+        free_cells(&cl);
+    }
+}
+
+/* Runs nsteps FP64 steps from the current F64 state (which advances), then
+ * the reverse sweep.  Returns the loss; dk, dl0 per spring (addition order),
+ * dq per particle.  NAN if a term without an adjoint is enabled. */
+double orc_adjoint(orc *o, int nsteps, const double *y3, double *dk, double *dl0, double *dq)
+{
+    const bs_params *p = &o->p;
+    if (p->imp_enable || p->hydrophobicity_enable || o->nbodies > 0 ||
+        (p->electrostatic_enable && p->potentialgrid_enable && o->grid[BS_GRID_POTENTIAL]) ||
+        (p->densitygrid_enable && o->grid[BS_GRID_DENSITY]))
+        return NAN;
+    const int64_t n = o->n;
+    double *xs = (double *)malloc(sizeof(double) * 3 * n * (size_t)(nsteps + 1));
+    memcpy(xs, o->xd, sizeof(double) * 3 * n);
+    for (int s = 0; s < nsteps; ++s) {
+        forces_f64(o, NULL);
+        integrate_f64(o);
+        memcpy(xs + (size_t)(s + 1) * 3 * n, o->xd, sizeof(double) * 3 * n);
+    }
+    double loss = 0.0;
+    for (int64_t k = 0; k < 3 * n; ++k) {
+        const double e = o->xd[k] - y3[k];
+        loss += e * e;
+    }
+    loss /= (double)n;
+    double *lx = (double *)calloc(3 * n, sizeof(double)), *lv = (double *)calloc(3 * n, sizeof(double));
+    double *w = (double *)calloc(3 * n, sizeof(double)), *hw = (double *)calloc(3 * n, sizeof(double));
+    for (int64_t i = 0; i < n; ++i)
+        if (!o->isstatic[i])
+            for (int c = 0; c < 3; ++c)
+                lx[3 * i + c] = 2.0 * (o->xd[3 * i + c] - y3[3 * i + c]) / (double)n;
+    memset(dk, 0, sizeof(double) * o->ns);
+    memset(dl0, 0, sizeof(double) * o->ns);
+    memset(dq, 0, sizeof(double) * n);
+    const double dt = p->timestep, gam = p->viscosity_enable ? p->viscosity_value : 0.0;
+    for (int s = nsteps - 1; s >= 0; --s) {
+        for (int64_t i = 0; i < n; ++i) {
+            const double m = o->mass[i];
+            for (int c = 0; c < 3; ++c) {
+                const double a = lv[3 * i + c] + dt * lx[3 * i + c];
+                w[3 * i + c] = (!o->isstatic[i] && m > 0.0) ? dt / m * a : 0.0;
+                lv[3 * i + c] = a; /* finished below */
+                hw[3 * i + c] = 0.0;
+            }
+        }
+        memcpy(o->xd, xs + (size_t)s * 3 * n, sizeof(double) * 3 * n);
+        adjoint_terms(o, w, hw, dk, dl0, dq);
+        for (int64_t i = 0; i < n; ++i) {
+            if (o->isstatic[i])
+                continue;
+            const double m = o->mass[i];
+            for (int c = 0; c < 3; ++c) {
+                lx[3 * i + c] += hw[3 * i + c];
+                if (m > 0.0)
+                    lv[3 * i + c] *= 1.0 - gam * dt / m;
+            }
+        }
+    }
+    memcpy(o->xd, xs + (size_t)nsteps * 3 * n, sizeof(double) * 3 * n);
+    free(xs);
+    free(lx);
+    free(lv);
+    free(w);
+    free(hw);
+    return loss;
+}
+
+/* Loss of nsteps FP64 steps from the given positions/velocities (the state
+ * advances), for finite differences. */
+double orc_loss(orc *o, int nsteps, const double *y3)
+{
+    for (int s = 0; s < nsteps; ++s) {
+        forces_f64(o, NULL);
+        integrate_f64(o);
+    }
+    double loss = 0.0;
+    for (int64_t k = 0; k < 3 * o->n; ++k) {
+        const double e = o->xd[k] - y3[k];
+        loss += e * e;
+    }
+    return loss / (double)o->n;
+}
+
+/* Replaces spring k / L0 (addition order of the deduplicated springs) and
+ * charges (nullable each); charged flags are kept. */
+void orc_set_params_arrays(orc *o, const float *k, const float *l0, const float *q)
+{
+    for (int64_t s = 0; s < o->ns; ++s) {
+        if (k)
+            o->sk[s] = k[s];
+        if (l0)
+            o->sl0[s] = l0[s];
+    }
+    if (q)
+        for (int64_t i = 0; i < o->n; ++i)
+            o->charge[i] = q[i];
+}
+
+/* H w at the current F64 state (forces' Jacobian times w), for tests. */
+void orc_hessian_vec(orc *o, const double *w, double *hw)
+{
+    double *dk = (double *)calloc(o->ns ? o->ns : 1, sizeof(double));
+    double *dl = (double *)calloc(o->ns ? o->ns : 1, sizeof(double));
+    double *dq = (double *)calloc(o->n, sizeof(double));
+    memset(hw, 0, sizeof(double) * 3 * o->n);
+    adjoint_terms(o, w, hw, dk, dl, dq);
+    free(dk);
+    free(dl);
+    free(dq);
+}
+
+/* ---------------------------------------------------------------------------
  * Public entry points.
  * ------------------------------------------------------------------------- */
 

@@ -76,6 +76,12 @@ def oracle_lib():
         L.orc_set_burying.argtypes = [_vp, _f32p]
         L.orc_insertion.argtypes = [_vp, ctypes.c_int, ctypes.c_int64, ctypes.c_int64, _f64p]
         L.orc_insertion.restype = ctypes.c_int
+        L.orc_adjoint.argtypes = [_vp, ctypes.c_int, _f64p, _f64p, _f64p, _f64p]
+        L.orc_adjoint.restype = ctypes.c_double
+        L.orc_loss.argtypes = [_vp, ctypes.c_int, _f64p]
+        L.orc_loss.restype = ctypes.c_double
+        L.orc_set_params_arrays.argtypes = [_vp, _f32p, _f32p, _f32p]
+        L.orc_hessian_vec.argtypes = [_vp, _f64p, _f64p]
         _olib = L
     return _olib
 
@@ -214,6 +220,30 @@ class Oracle:
         if self.L.orc_insertion(self.h, mode, i1, i2, _p(out, ctypes.c_double)) != 0:
             raise IndexError((i1, i2))
         return out
+
+    def adjoint(self, nsteps, target):
+        """FP64 forward of nsteps from the current F64 state, then the reverse
+        sweep: (loss, dL/dk, dL/dL0 per spring, dL/dq per particle)."""
+        y = np.ascontiguousarray(target, np.float64).reshape(-1)
+        ns = int(self.L.orc_num_springs(self.h))
+        dk, dl, dq = np.zeros(max(ns, 1)), np.zeros(max(ns, 1)), np.zeros(self.n)
+        loss = self.L.orc_adjoint(self.h, nsteps, _p(y, ctypes.c_double), _p(dk, ctypes.c_double),
+                                  _p(dl, ctypes.c_double), _p(dq, ctypes.c_double))
+        return loss, dk[:ns], dl[:ns], dq
+
+    def loss(self, nsteps, target):
+        y = np.ascontiguousarray(target, np.float64).reshape(-1)
+        return self.L.orc_loss(self.h, nsteps, _p(y, ctypes.c_double))
+
+    def set_arrays(self, k=None, l0=None, q=None):
+        a = [None if v is None else np.ascontiguousarray(v, np.float32) for v in (k, l0, q)]
+        self.L.orc_set_params_arrays(self.h, *[_p(v, ctypes.c_float) for v in a])
+
+    def hessian_vec(self, w):
+        ww = np.ascontiguousarray(w, np.float64).reshape(-1)
+        out = np.zeros(3 * self.n)
+        self.L.orc_hessian_vec(self.h, _p(ww, ctypes.c_double), _p(out, ctypes.c_double))
+        return out.reshape(-1, 3)
 
     def cells(self, kind, mode=F32):
         nm = ctypes.c_int64()

@@ -343,3 +343,71 @@ def test_two_bodies_move_independently():
             for sl in (slice(0, 3), slice(3, 6), slice(6, 9), slice(9, 12), slice(12, 16)):
                 scale = np.abs(want[sl]).max()
                 assert np.abs(got[sl] - want[sl]).max() <= rt * max(scale, 1e-30), (mode, sl, got, want)
+
+
+# ---------------------------------------------------------------------------
+# FP64 adjoint (no reference counterpart): pinned by finite differences
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("setup", ["enm", "amber", "lewitt", "zacharias", "linear", "ddd"])
+def test_hessian_vector_is_force_derivative(setup):
+    """H w of the adjoint sweep equals the central difference of the forces
+    along w, for springs, every steric mode and both dielectric models."""
+    if setup == "enm":
+        s = workloads.generate(0, seed=6, n=250)
+        p = s.params.copy()
+    else:
+        s = workloads.generate(1, seed=7, n=350)
+        p = s.params.copy()
+        p.dielectric_model = 1 if setup == "ddd" else 0
+        p.steric_mode = {"amber": 1, "lewitt": 2, "zacharias": 3, "linear": 0, "ddd": 1}[setup]
+        if setup == "linear":
+            p.spring_enable = 0
+            p.steric_cutoff = 8.0
+    p.viscosity_enable = 0
+    rng = np.random.default_rng(1)
+    x0 = (s.x.reshape(-1, 3) + rng.normal(0, 0.2, (s.n, 3))).astype(np.float32).astype(np.float64)
+    o = po.Oracle(s, params=p)
+    o.set_state_f64(x0)
+    w = rng.normal(size=(s.n, 3))
+    w[(s.flags & 1) == 1] = 0
+    Hw = o.hessian_vec(w)
+    h = 1e-6
+    o.set_state_f64(x0 + h * w)
+    Fp, _ = o.compute_forces(po.F64)
+    o.set_state_f64(x0 - h * w)
+    Fm, _ = o.compute_forces(po.F64)
+    fd = (Fp - Fm) / (2 * h)
+    assert np.linalg.norm(Hw - fd) / np.linalg.norm(fd) < 1e-7
+
+
+def test_adjoint_gradients_match_finite_differences():
+    """dL/dk, dL/dL0 and dL/dq of the loss after 20 steps (Amber + Coulomb with
+    eps(r) = r, viscosity on) against central differences with h = 1e-3."""
+    s = workloads.generate(1, seed=9, n=400)
+    p = s.params.copy()
+    x0 = s.x.reshape(-1, 3).astype(np.float64)
+    y = x0 + 0.5 * np.sin(x0 / 7.0)
+    T, h = 20, 1e-3
+    loss, dk, dl, dq = po.Oracle(s, params=p).adjoint(T, y)
+    assert loss > 0
+
+    def fd(**kw):
+        (name, arr), = kw.items()
+        lo = po.Oracle(s, params=p)
+        lo.set_arrays(**{name: arr})
+        return lo.loss(T, y)
+
+    rng = np.random.default_rng(2)
+    for si in rng.choice(s.m, 4, replace=False):
+        for name, base, g in (("k", s.k, dk), ("l0", s.l0, dl)):
+            ap, am = base.copy(), base.copy()
+            ap[si] += h
+            am[si] -= h
+            ref = (fd(**{name: ap}) - fd(**{name: am})) / (2 * h)
+            assert abs(g[si] - ref) <= 5e-4 * abs(ref) + 1e-14, (name, si, g[si], ref)
+    for i in rng.choice(np.nonzero(s.charge != 0)[0], 4, replace=False):
+        qp, qm = s.charge.copy(), s.charge.copy()
+        qp[i] += h
+        qm[i] -= h
+        ref = (fd(q=qp) - fd(q=qm)) / (2 * h)
+        assert abs(dq[i] - ref) <= 5e-4 * abs(ref) + 1e-14, (i, dq[i], ref)
