@@ -40,10 +40,14 @@ WORKLOADS = {
        "springs < 10 A, linear steric 4 A",
     3: "512 independent CA networks ({n} beads in all, 200-20,000 each, springs < 9-14 A), "
        "linear steric 4 A + Coulomb 16 A, one launch sequence replayed from a HIP graph",
+    4: "adjoint of {T} steps (checkpoints every {K}) of a {n}-bead network, Amber 12-6 + Coulomb eps(r)=r, "
+       "12 A cutoffs, springs < 10 A; loss = MSD to a target displaced along 3 smooth modes (5 A)",
 }
 HBM_PEAK_GBS = 8000.0       # MI355X HBM3E, GB/s
 FP32_PEAK_TFLOPS = 157.3    # MI355X FP32 vector/matrix, TFLOP/s
 PAIR_FLOP = 40              # algorithmic flops per unique LJ+Coulomb pair
+FP64_PEAK_TFLOPS = 78.6     # MI355X FP64 vector, TFLOP/s
+ADJ_PAIR_FLOP = 140         # adjoint: FP64 flops per unique LJ+Coulomb pair (both sides, H w and dL/dq)
 
 
 COPY_BYTES = 1 << 30        # copy-calibration size
@@ -123,10 +127,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", type=int, default=1, choices=(1, 2, 3),
+    ap.add_argument("--workload", type=int, default=1, choices=(1, 2, 3, 4),
                     help="1: 250k-bead LJ + Coulomb network (headline); 2: 50k CG beads, IMPALA curved double "
                          "membrane, 4 rigid bodies of 2,000 beads; 3: 512 independent CA networks of "
-                         "200-20,000 beads stepped together from a HIP graph")
+                         "200-20,000 beads stepped together from a HIP graph; 4: adjoint gradients of a "
+                         "200-step trajectory loss on a 100k-bead network (--steps is the trajectory length)")
+    ap.add_argument("--checkpoint", type=int, default=20, help="workload 4: checkpoint interval")
     ap.add_argument("--systems", type=int, default=512, help="workload 3: number of systems")
     ap.add_argument("--autotune", action="store_true", help="pick the launch configuration first")
     ap.add_argument("--n", type=int, default=0, help="bead count (default: the workload's own)")
@@ -146,6 +152,8 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
+    if args.workload == 4:
+        return bench_adjoint(args)
     from biospring_amd import _abi, workloads
     from biospring_amd.network import SpringNetwork
 
@@ -309,6 +317,141 @@ def main():
                                              + ", float-faithful oracle, OpenMP"}
             if REF_OVER_ORACLE:
                 out["cpu_baseline"]["reference_equivalent"] = done / dt / REF_OVER_ORACLE
+    print(json.dumps(out), flush=True)
+
+
+def bench_adjoint(args):
+    """Workload 4: the checkpointed adjoint (forward, recomputation and the
+    FP64 reverse sweep) timed against the plain forward of the same steps,
+    with directional central differences of the GPU's own loss at full size
+    and the FP64 oracle's gradient on a 2,000-bead network of the same kind."""
+    from biospring_amd import _abi, workloads
+    from biospring_amd.network import SpringNetwork
+
+    T, K = args.steps, args.checkpoint
+    t = time.perf_counter()
+    s = workloads.generate(4, seed=args.seed, n=args.n)
+    y = workloads.fitting_target(s, seed=args.seed)
+    n = s.n
+    log(f"generated n={n} springs={s.m} in {time.perf_counter() - t:.1f}s digest={s.digest()}")
+    net = SpringNetwork.from_system(s)
+    net.setup()
+    x0, v0 = net.getState()
+    pairs_unique = net.countPairs(_abi.NS_STERIC)
+    net.adjoint(min(max(args.warmup, 1), T), y, K)
+    net.setState(x0, v0)
+    net.profile(True)
+    net.adjoint(T, y, K)
+    prof = net.getProfile()
+    net.profile(False)
+    edges = net.counters()["edges"]
+
+    def timed(fn):
+        net.setState(x0, v0)
+        hip_sync()
+        t0 = time.perf_counter()
+        r = fn()
+        hip_sync()
+        return time.perf_counter() - t0, r
+
+    t_fwd, _ = timed(lambda: net.run(T))
+    t_adj, (loss, dk, dl, dq) = timed(lambda: net.adjoint(T, y, K))
+    value = n * T / t_adj
+    log(f"forward {t_fwd * 1e3:.1f} ms, adjoint {t_adj * 1e3:.1f} ms, ratio {t_adj / t_fwd:.2f}")
+
+    ap_ms = prof["adj_pair"]["ms"] / max(prof["adj_pair"]["launches"], 1)
+    as_ms = prof["adj_spring"]["ms"] / max(prof["adj_spring"]["launches"], 1)
+    f_alg = ADJ_PAIR_FLOP * pairs_unique
+    owned = int(s.m)  # springs with a dynamic end: one read-modify-write of dk, dL0 each per step
+    b_alg = 4 * (n + 1) + 12 * edges + 32 * owned + 280 * n
+    kernels = {
+        "adj_pair": {"avg_ms": ap_ms, "bound": "mfma", "achieved": f_alg / (ap_ms * 1e-3) / 1e12,
+                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "alg_flops": f_alg,
+                     "note": "FP64 vector ALU work priced against the FP64 peak (pairs at the initial state)"},
+        "adj_spring": {"avg_ms": as_ms, "bound": "hbm", "achieved": b_alg / (as_ms * 1e-3) / 1e9,
+                       "peak": HBM_PEAK_GBS, "unit": "GB/s", "alg_bytes": b_alg},
+    }
+    for k in kernels.values():
+        k["frac"] = k["achieved"] / k["peak"]
+    kernels["forward_pair_avg_ms"] = prof["pair"]["ms"] / max(prof["pair"]["launches"], 1)
+    kernels["forward_spring_avg_ms"] = prof["spring"]["ms"] / max(prof["spring"]["launches"], 1)
+    kernels["rebuild_avg_ms"] = prof["rebuild"]["ms"] / max(prof["rebuild"]["launches"], 1)
+    dom = "adj_pair" if ap_ms >= as_ms else "adj_spring"
+    d = kernels[dom]
+    roofline = {"bound": d["bound"], "achieved": d["achieved"], "peak": d["peak"], "unit": d["unit"],
+                "frac": d["frac"], "traffic": None, "kernel": dom}
+
+    # directional central differences of the GPU loss (all springs / charges at once)
+    eps = 1e-2
+    rng = np.random.default_rng(7)
+    rk, rl, rq = rng.normal(size=s.m), rng.normal(size=s.m), rng.normal(size=n) * (s.charge != 0)
+    k64, l64, q64 = s.k.astype(np.float64), s.l0.astype(np.float64), s.charge.astype(np.float64)
+
+    def loss_with(k=None, l0=None, q=None):
+        if k is not None or l0 is not None:
+            net.setSprings(s.id1, s.id2, s.l0 if l0 is None else l0.astype(np.float32),
+                           s.k if k is None else k.astype(np.float32))
+        if q is not None:
+            net.setParticles(x0, v0, s.mass, q.astype(np.float32), s.radius, s.eps, s.sasa, s.transfer,
+                             s.hydro, s.flags)
+            net.setSprings(*s.spring_arrays())
+        net.setState(x0, v0)
+        net.run(T)
+        x, _ = net.getState()
+        r = float(np.mean(np.sum((x.astype(np.float64) - y) ** 2, axis=1)))
+        if q is not None:
+            net.setParticles(*s.particle_arrays())
+        net.setSprings(*s.spring_arrays())
+        return r
+
+    fd = {}
+    t = time.perf_counter()
+    for name, pred, f in (("k", float(np.dot(dk, k64 * rk)), lambda e: loss_with(k=k64 * (1 + e * rk))),
+                          ("l0", float(np.dot(dl, 0.1 * rl)), lambda e: loss_with(l0=l64 + 0.1 * e * rl)),
+                          ("q", float(np.dot(dq, 0.4 * rq)), lambda e: loss_with(q=q64 + 0.4 * e * rq))):
+        c = (f(eps) - f(-eps)) / (2 * eps)
+        fd[name] = {"adjoint": pred, "central_difference": c, "rel_err": abs(pred - c) / max(abs(c), 1e-300)}
+    log(f"finite differences {time.perf_counter() - t:.1f}s: " +
+        ", ".join(f"{k} {v['rel_err']:.2e}" for k, v in fd.items()))
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": T,
+        "warmup": args.warmup, "ms_per_step": t_adj / T * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32 trajectory, fp64 adjoint", "data": "synthetic",
+        "config": {"workload": WORKLOADS[4].format(n=n, T=T, K=K), "n_beads": n, "springs": s.m,
+                   "directed_edges": edges, "unique_pairs_initial": pairs_unique, "checkpoint_every": K,
+                   "seed": args.seed, "digest": s.digest()},
+        "roofline": roofline,
+        "kernels": kernels,
+        "adjoint": {"forward_ms": t_fwd * 1e3, "adjoint_ms": t_adj * 1e3, "adjoint_over_forward": t_adj / t_fwd,
+                    "loss": loss, "directional_fd": fd},
+    }
+    if not args.no_fp64 or not args.no_cpu:
+        from oracle import pyoracle as po
+        if not args.no_fp64:
+            t = time.perf_counter()
+            sm = workloads.generate(4, seed=args.seed, n=2000)
+            ym = workloads.fitting_target(sm, seed=args.seed)
+            g = SpringNetwork.from_system(sm).adjoint(T, ym, K)
+            o = po.Oracle(sm)
+            r = o.adjoint(T, ym.astype(np.float64))
+            o.close()
+            rel = lambda a, b: float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+            out["grad_err_vs_fp64"] = {"n_beads": sm.n, "steps": T, "loss_rel": abs(g[0] - r[0]) / r[0],
+                                       "dk": rel(g[1], r[1]), "dl0": rel(g[2], r[2]), "dq": rel(g[3], r[3]),
+                                       "measure": "max |g_gpu - g_fp64| / max |g_fp64|"}
+            log(f"fp64 gradient check {time.perf_counter() - t:.1f}s")
+        if not args.no_cpu:
+            o = po.Oracle(s)
+            o.set_state(x0, v0)
+            steps = 2
+            t = time.perf_counter()
+            o.adjoint(steps, y.astype(np.float64))
+            dt = time.perf_counter() - t
+            o.close()
+            out["cpu_baseline"] = {"value": n * steps / dt, "unit": "particle-steps/s",
+                                   "cores": int(po.oracle_lib().orc_num_threads()), "kind": "port",
+                                   "sample": f"FP64 oracle adjoint of {steps} steps of the same {n}-bead network, OpenMP"}
     print(json.dumps(out), flush=True)
 
 

@@ -20,6 +20,7 @@ FLAG_STATIC = 1
 E_SPRING, E_ELECTROSTATIC, E_STERIC, E_KINETIC, E_IMP, E_HYDROPHOBIC = range(6)
 NUM_ENERGIES = 6
 NS_STERIC, NS_ELECTROSTATIC, NS_HYDROPHOBIC = range(3)
+PROF_NUM = 5  # BS_PROF_REBUILD, _PAIR, _SPRING, _ADJ_PAIR, _ADJ_SPRING
 GRID_POTENTIAL, GRID_DENSITY = 0, 1
 
 # Exported symbols, one per declaration in include/bs_api.h.
@@ -30,7 +31,7 @@ EXPORTS = (
     "bs_last_nonfinite_particle", "bs_get_counters", "bs_profile", "bs_get_profile",
     "bs_get_cells", "bs_get_pairs", "bs_add_rigid_body", "bs_num_rigid_bodies", "bs_get_rigid_body",
     "bs_set_grid", "bs_set_burying", "bs_get_insertion_vector", "bs_set_systems", "bs_num_systems",
-    "bs_get_system_energies", "bs_autotune",
+    "bs_get_system_energies", "bs_autotune", "bs_adjoint",
 )
 PROF_REBUILD, PROF_PAIR, PROF_SPRING = range(3)
 
@@ -153,6 +154,7 @@ def bind(L):
     L.bs_num_systems.restype = ctypes.c_int32
     L.bs_get_system_energies.argtypes = [_vp, _f64p]
     L.bs_autotune.argtypes = [_vp, ctypes.c_int32, _i32p, _i32p]
+    L.bs_adjoint.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, _f32p, _f32p, _f64p, _f64p, _f64p, _f64p]
     return L
 
 

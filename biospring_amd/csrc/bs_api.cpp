@@ -146,8 +146,8 @@ struct bs_handle
         size_t a, b;
     };
     std::vector<Span> spans;
-    double prof_ms[BS_PROF_NUM] = {0, 0, 0};
-    int64_t prof_n[BS_PROF_NUM] = {0, 0, 0};
+    double prof_ms[BS_PROF_NUM] = {};
+    int64_t prof_n[BS_PROF_NUM] = {};
 };
 
 namespace
@@ -753,6 +753,17 @@ void prof_collect(bs_handle *h)
     h->evused = 0;
 }
 
+// Bounds the event pool: folds in what is recorded once it holds 4096 events.
+int prof_fold(bs_handle *h)
+{
+    if (h->prof && h->evused > 4096)
+    {
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        prof_collect(h);
+    }
+    return BS_OK;
+}
+
 int finish_call(bs_handle *h)
 {
     launch_energies(h->lists.empty() ? nullptr : h->epp, h->ese, h->seg_start, h->nseg, h->sys_seg,
@@ -831,6 +842,62 @@ int upload_ext(bs_handle *h, const float *ext3)
     HIPCHK(h, hipMemcpyAsync(h->fext, fe.data(), sizeof(float4) * h->n, hipMemcpyHostToDevice, h->stream));
     h->ext_uploaded = 1;
     return BS_OK;
+}
+
+// Device buffers of one call, released on every return path.
+struct Scratch
+{
+    std::vector<void *> ptrs;
+    Scratch() = default;
+    Scratch(const Scratch &) = delete;
+    Scratch &operator=(const Scratch &) = delete;
+    ~Scratch()
+    {
+        for (void *q : ptrs)
+            (void)hipFree(q);
+    }
+    template <typename T> T *get(size_t count, hipError_t &err)
+    {
+        T *q = dalloc<T>(count, err);
+        if (q)
+            ptrs.push_back(q);
+        return q;
+    }
+};
+
+// H w of the pair terms at the positions in pos[cur] (lists valid there),
+// into hw, and their dL/dq into dq.
+void launch_adj_pairs(bs_handle *h, const double4 *w, double4 *hw, double *dq)
+{
+    for (size_t li = 0; li < h->lists.size(); ++li)
+    {
+        const List &l = h->lists[li];
+        AdjPairArgs a{};
+        a.pos = h->pos[h->cur];
+        a.jprm = h->jprm;
+        a.cl = l.cl;
+        a.cln = l.cln;
+        a.w = w;
+        a.hw = hw;
+        a.dq = dq;
+        a.n = (int32_t)h->n;
+        a.ncl = h->ncl;
+        a.cap = l.cap;
+        a.accumulate = li > 0 ? 1 : 0;
+        a.rc = l.rc;
+        a.rc2lo = rc2(l.rc, 1.0 - 4e-6);
+        a.rc2hi = rc2(l.rc, 1.0 + 4e-6);
+        a.steric_scale = h->p.steric_gridscale;
+        a.kf = h->k.c_elec / (4.0 * h->k.pi * (double)h->p.electrostatic_dielectric) *
+               (double)h->p.electrostatic_scale;
+        a.i_mask = l.imask;
+        a.j_mask = l.jmask;
+        a.k = h->k;
+        const int32_t sm = (l.terms & TERM_STERIC) ? h->p.steric_mode : -1;
+        const int32_t em =
+            (l.terms & TERM_ELEC) ? (h->p.dielectric_model == BS_DIELECTRIC_DISTANCE ? 2 : 1) : 0;
+        launch_adj_pair(a, sm, em, h->pair_blocks, h->stream);
+    }
 }
 
 int do_setup(bs_handle *h)
@@ -1404,12 +1471,9 @@ int bs_step(bs_handle *h, int32_t nsteps, const float *ext_force3)
     }
     for (; s < nsteps; ++s)
     {
-        // Bound the event pool: fold in what is recorded every 1024 steps.
-        if (h->prof && h->evused > 4096)
-        {
-            HIPCHK(h, hipStreamSynchronize(h->stream));
-            prof_collect(h);
-        }
+        const int frc = prof_fold(h);
+        if (frc)
+            return frc;
         launch_step(h, s + 1 == nsteps);
     }
     h->steps_done += nsteps;
@@ -1978,6 +2042,262 @@ int bs_get_system_energies(bs_handle *h, double *out)
     if (h->last_call_forces_only)
         for (int32_t s = 0; s < nsys; ++s)
             out[6 * s + BS_E_KINETIC] = 0.0;
+    return BS_OK;
+}
+
+int bs_adjoint(bs_handle *h, int32_t nsteps, int32_t checkpoint_every, const float *target3, const float *ext_force3,
+               double *loss, double *dk, double *dl0, double *dq)
+{
+    if (!h)
+        return BS_ERR_ARG;
+    if (nsteps < 0)
+        return fail(h, BS_ERR_ARG, "bs_adjoint: negative step count");
+    if (!target3)
+        return fail(h, BS_ERR_ARG, "bs_adjoint: target positions required");
+    const bs_params &P = h->p;
+    if (P.imp_enable || P.hydrophobicity_enable)
+        return fail(h, BS_ERR_ARG, "bs_adjoint: IMPALA and the pseudo-hydrophobic term have no adjoint");
+    if (!h->bodies.empty())
+        return fail(h, BS_ERR_ARG, "bs_adjoint: rigid bodies have no adjoint");
+    if ((P.electrostatic_enable && P.potentialgrid_enable && h->grids[BS_GRID_POTENTIAL].set) ||
+        (P.densitygrid_enable && h->grids[BS_GRID_DENSITY].set))
+        return fail(h, BS_ERR_ARG, "bs_adjoint: field grids have no adjoint");
+    if (P.compensated_positions)
+        return fail(h, BS_ERR_ARG, "bs_adjoint: needs compensated_positions = 0");
+    int rc = ensure_ready(h);
+    if (rc)
+        return rc;
+    rc = upload_ext(h, ext_force3);
+    if (rc)
+        return rc;
+    const int64_t n = h->n, E = h->nedges;
+    const int32_t T = nsteps;
+    int32_t K = checkpoint_every > 0 ? checkpoint_every : (int32_t)std::ceil(std::sqrt((double)std::max(T, 1)));
+    K = std::max(1, std::min(K, std::max(T, 1)));
+    const int32_t ncp = (T + K - 1) / K;
+    const size_t s4 = sizeof(float4) * (size_t)n;
+    hipStream_t st = h->stream;
+    hipError_t e = hipSuccess;
+    Scratch S;
+    float4 *cp_pos, *cp_vel, *seg, *tgt, *fin_pos, *fin_vel;
+    int32_t *cp_cnt, *fin_cnt;
+    double4 *lx, *lv, *wb[2], *hwp = nullptr;
+    double *dke, *dle, *dqd, *lpart;
+#define SALLOC(ptr, T_, cnt)                                                                                     \
+    do                                                                                                            \
+    {                                                                                                             \
+        ptr = S.get<T_>((size_t)(cnt), e);                                                                        \
+        HIPCHK(h, e);                                                                                             \
+    } while (0)
+    SALLOC(cp_pos, float4, (int64_t)n * std::max(ncp, 1));
+    SALLOC(cp_vel, float4, (int64_t)n * std::max(ncp, 1));
+    SALLOC(cp_cnt, int32_t, 4 * std::max(ncp, 1));
+    SALLOC(seg, float4, (int64_t)n * K);
+    SALLOC(tgt, float4, n);
+    SALLOC(fin_pos, float4, n);
+    SALLOC(fin_vel, float4, n);
+    SALLOC(fin_cnt, int32_t, 4);
+    SALLOC(lx, double4, n);
+    SALLOC(lv, double4, n);
+    SALLOC(wb[0], double4, n);
+    SALLOC(wb[1], double4, n);
+    if (!h->lists.empty())
+        SALLOC(hwp, double4, n);
+    SALLOC(dke, double, E);
+    SALLOC(dle, double, E);
+    SALLOC(dqd, double, n);
+    const int32_t lblocks = (int32_t)((n + BLOCK - 1) / BLOCK);
+    SALLOC(lpart, double, lblocks);
+#undef SALLOC
+
+    // forward: T steps, state saved (and the lists rebuilt) every K steps so
+    // that recomputing a segment repeats it bit for bit
+    for (int32_t s = 0; s < T; ++s)
+    {
+        if (s % K == 0)
+        {
+            const size_t c = (size_t)(s / K);
+            HIPCHK(h, hipMemcpyAsync(cp_pos + c * n, h->pos[h->cur], s4, hipMemcpyDeviceToDevice, st));
+            HIPCHK(h, hipMemcpyAsync(cp_vel + c * n, h->vel, s4, hipMemcpyDeviceToDevice, st));
+            HIPCHK(h, hipMemcpyAsync(cp_cnt + 4 * c, h->counters, 4 * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
+            launch_fill_i32(h->status, 1, 1, nullptr, st);
+        }
+        rc = prof_fold(h);
+        if (rc)
+            return rc;
+        launch_step(h, s + 1 == T);
+    }
+    if (T > 0)
+    {
+        rc = finish_call(h);
+        if (rc)
+            return rc;
+        h->steps_done += T;
+        h->last_call_forces_only = false;
+    }
+    HIPCHK(h, hipMemcpyAsync(fin_pos, h->pos[h->cur], s4, hipMemcpyDeviceToDevice, st));
+    HIPCHK(h, hipMemcpyAsync(fin_vel, h->vel, s4, hipMemcpyDeviceToDevice, st));
+    HIPCHK(h, hipMemcpyAsync(fin_cnt, h->counters, 4 * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
+
+    // loss and the adjoint state at x_T
+    {
+        std::vector<float4> y((size_t)n);
+        for (int64_t t = 0; t < n; ++t)
+        {
+            const int64_t i = h->perm[(size_t)t];
+            y[(size_t)t] = make_float4(target3[3 * i], target3[3 * i + 1], target3[3 * i + 2], 0.0f);
+        }
+        HIPCHK(h, hipMemcpyAsync(tgt, y.data(), s4, hipMemcpyHostToDevice, st));
+        HIPCHK(h, hipStreamSynchronize(st));
+    }
+    const double dt = (double)P.timestep, gam = P.viscosity_enable ? (double)P.viscosity_value : 0.0;
+    launch_adj_init(h->pos[h->cur], tgt, h->jprm, h->iprm, (int32_t)n, dt, lx, lv, wb[0], lpart, lblocks, st);
+    HIPCHK(h, hipMemsetAsync(dke, 0, sizeof(double) * std::max<int64_t>(E, 1), st));
+    HIPCHK(h, hipMemsetAsync(dle, 0, sizeof(double) * std::max<int64_t>(E, 1), st));
+    HIPCHK(h, hipMemsetAsync(dqd, 0, sizeof(double) * n, st));
+    const double hs = 0.5 * std::max(P.list_skin, 0.0f);
+    const float half_skin2 = (float)(hs * hs * (1.0 - 1e-5));
+    int wc = 0;
+    for (int32_t c = ncp - 1; c >= 0; --c)
+    {
+        const int32_t s0 = c * K, s1 = std::min(T, s0 + K);
+        HIPCHK(h, hipMemcpyAsync(h->pos[h->cur], cp_pos + (size_t)c * n, s4, hipMemcpyDeviceToDevice, st));
+        HIPCHK(h, hipMemcpyAsync(h->vel, cp_vel + (size_t)c * n, s4, hipMemcpyDeviceToDevice, st));
+        HIPCHK(h, hipMemcpyAsync(h->counters, cp_cnt + 4 * (size_t)c, 4 * sizeof(int32_t), hipMemcpyDeviceToDevice,
+                                 st));
+        launch_fill_i32(h->status, 1, 1, nullptr, st);
+        for (int32_t s = s0; s < s1; ++s)
+        {
+            HIPCHK(h, hipMemcpyAsync(seg + (size_t)(s - s0) * n, h->pos[h->cur], s4, hipMemcpyDeviceToDevice, st));
+            if (s + 1 < s1)
+            {
+                rc = prof_fold(h);
+                if (rc)
+                    return rc;
+                launch_step(h, false);
+            }
+        }
+        for (int32_t s = s1 - 1; s >= s0; --s)
+        {
+            HIPCHK(h, hipMemcpyAsync(h->pos[h->cur], seg + (size_t)(s - s0) * n, s4, hipMemcpyDeviceToDevice, st));
+            rc = prof_fold(h);
+            if (rc)
+                return rc;
+            const size_t e0 = h->prof ? prof_mark(h) : SIZE_MAX;
+            launch_drift_check(h->pos[h->cur], h->xref, (int32_t)n, half_skin2, h->status, st);
+            rebuild_chain(h, true);
+            const size_t e1 = h->prof ? prof_mark(h) : SIZE_MAX;
+            if (hwp)
+                launch_adj_pairs(h, wb[wc], hwp, dqd);
+            const size_t e2 = h->prof ? prof_mark(h) : SIZE_MAX;
+            AdjSpringArgs a{};
+            a.pos = h->pos[h->cur];
+            a.jprm = h->jprm;
+            a.iprm = h->iprm;
+            a.sp_ptr = h->sp_ptr;
+            a.sp_j = h->sp_j;
+            a.sp_k = h->sp_k;
+            a.sp_l0 = h->sp_l0;
+            a.w = wb[wc];
+            a.hwp = hwp;
+            a.lx = lx;
+            a.lv = lv;
+            a.w_next = wb[wc ^ 1];
+            a.dk = dke;
+            a.dl0 = dle;
+            a.n = (int32_t)n;
+            a.spring_enable = P.spring_enable;
+            a.spring_scale = P.spring_scale;
+            a.dt = dt;
+            a.gamma = gam;
+            a.k = h->k;
+            launch_adj_spring(a, h->group, h->spring_blocks, st);
+            if (h->prof)
+            {
+                const size_t e3 = prof_mark(h);
+                prof_span(h, BS_PROF_REBUILD, e0, e1);
+                if (hwp)
+                    prof_span(h, BS_PROF_ADJ_PAIR, e1, e2);
+                prof_span(h, BS_PROF_ADJ_SPRING, e2, e3);
+            }
+            wc ^= 1;
+        }
+    }
+    // back to x_T (lists rebuilt at the next step)
+    HIPCHK(h, hipMemcpyAsync(h->pos[h->cur], fin_pos, s4, hipMemcpyDeviceToDevice, st));
+    HIPCHK(h, hipMemcpyAsync(h->vel, fin_vel, s4, hipMemcpyDeviceToDevice, st));
+    HIPCHK(h, hipMemcpyAsync(h->counters, fin_cnt, 4 * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
+    launch_fill_i32(h->status, 1, 1, nullptr, st);
+
+    std::vector<double> lp((size_t)lblocks), dqh((size_t)n), dkh, dlh;
+    int32_t stv[4];
+    HIPCHK(h, hipMemcpyAsync(lp.data(), lpart, sizeof(double) * lblocks, hipMemcpyDeviceToHost, st));
+    HIPCHK(h, hipMemcpyAsync(dqh.data(), dqd, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    HIPCHK(h, hipMemcpyAsync(stv, h->status, sizeof stv, hipMemcpyDeviceToHost, st));
+    std::vector<int32_t> hp, hj;
+    if (dk || dl0)
+    {
+        dkh.resize((size_t)E);
+        dlh.resize((size_t)E);
+        hp.resize((size_t)n + 1);
+        hj.resize((size_t)E);
+        if (E > 0)
+        {
+            HIPCHK(h, hipMemcpyAsync(dkh.data(), dke, sizeof(double) * E, hipMemcpyDeviceToHost, st));
+            HIPCHK(h, hipMemcpyAsync(dlh.data(), dle, sizeof(double) * E, hipMemcpyDeviceToHost, st));
+            HIPCHK(h, hipMemcpyAsync(hj.data(), h->sp_j, sizeof(int32_t) * E, hipMemcpyDeviceToHost, st));
+        }
+        HIPCHK(h, hipMemcpyAsync(hp.data(), h->sp_ptr, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToHost, st));
+    }
+    HIPCHK(h, hipStreamSynchronize(st));
+    prof_collect(h);
+    if (stv[2] != 0)
+        return fail(h, BS_ERR_STATE, "neighbour list capacity exceeded (max %d > cap); rerun bs_setup", stv[3]);
+    double L = 0.0;
+    for (double v : lp)
+        L += v;
+    if (loss)
+        *loss = L / (double)n;
+    if (dq)
+        for (int64_t t = 0; t < n; ++t)
+            dq[h->perm[(size_t)t]] = dqh[(size_t)t];
+    if (dk || dl0)
+    {
+        // a spring's two directed edges: row a holds j = b, row b holds j = a
+        auto edge = [&](int32_t a, int32_t b) -> int64_t {
+            int32_t lo = hp[(size_t)a], hi = hp[(size_t)a + 1];
+            while (lo < hi)
+            {
+                const int32_t mid = (lo + hi) >> 1;
+                if ((hj[(size_t)mid] & 0x7fffffff) < b)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            return (lo < hp[(size_t)a + 1] && (hj[(size_t)lo] & 0x7fffffff) == b) ? lo : -1;
+        };
+        const int64_t m = (int64_t)h->s1.size();
+        for (int64_t s = 0; s < m; ++s)
+        {
+            const int32_t a = h->iperm[h->s1[(size_t)s]], b = h->iperm[h->s2[(size_t)s]];
+            const int64_t ea = edge(a, b), eb = edge(b, a);
+            double gk = 0.0, gl = 0.0;
+            if (ea >= 0)
+            {
+                gk += dkh[(size_t)ea];
+                gl += dlh[(size_t)ea];
+            }
+            if (eb >= 0)
+            {
+                gk += dkh[(size_t)eb];
+                gl += dlh[(size_t)eb];
+            }
+            if (dk)
+                dk[s] = gk;
+            if (dl0)
+                dl0[s] = gl;
+        }
+    }
     return BS_OK;
 }
 

@@ -176,6 +176,47 @@ struct SpringArgs
     Consts k;
 };
 
+// Adjoint of the pair terms at one state: H w on i (each pair seen from both
+// sides) and dL/dq.  Positions FP32, everything else FP64.
+struct AdjPairArgs
+{
+    const float4 *pos;
+    const float4 *jprm;
+    const uint32_t *cl;
+    const int32_t *cln;
+    const double4 *w;     // [N] adjoint weights dt/m a (0 for static particles)
+    double4 *hw;          // [N] H w of this list (overwritten, or added with accumulate)
+    double *dq;           // [N] dL/dq accumulator
+    int32_t n, ncl, cap, accumulate;
+    float rc, rc2lo, rc2hi;
+    float steric_scale;
+    double kf;            // Coulomb force prefactor incl. scale and 1/eps
+    uint32_t i_mask, j_mask;
+    Consts k;
+};
+
+// Spring part of H w, the spring-parameter gradients and the reverse Euler
+// update of the adjoint state, per particle row.
+struct AdjSpringArgs
+{
+    const float4 *pos;
+    const float4 *jprm;
+    const float4 *iprm;
+    const int32_t *sp_ptr;
+    const int32_t *sp_j;
+    const float *sp_k;
+    const float *sp_l0;
+    const double4 *w;     // [N] weights of this step
+    const double4 *hwp;   // nullable: pair part of H w
+    double4 *lx, *lv;     // [N] adjoint of x and v (updated in place)
+    double4 *w_next;      // [N] weights of the step before
+    double *dk, *dl0;     // [E] gradients, on the owning edge of each spring
+    int32_t n, spring_enable;
+    float spring_scale;
+    double dt, gamma;     // gamma = 0 without viscosity
+    Consts k;
+};
+
 struct ListArgs
 {
     const float4 *pos;
@@ -227,6 +268,16 @@ void launch_finish_rebuild(const float4 *pos, float4 *xref, int32_t n, int32_t *
                            hipStream_t s);
 void launch_clear_rebuild(int32_t *status, int32_t *counters, hipStream_t s);
 void launch_rigid(const RigidArgs &a, hipStream_t s);
+void launch_adj_pair(const AdjPairArgs &a, int32_t steric_mode, int32_t elec_mode, int32_t nblocks, hipStream_t s);
+void launch_adj_spring(const AdjSpringArgs &a, int32_t group, int32_t nblocks, hipStream_t s);
+// lx = 2 (x - y) / n for dynamic particles, lv = 0, w = dt/m (dt lx); loss
+// partials |x - y|^2 per block (nblocks of BLOCK).
+void launch_adj_init(const float4 *pos, const float4 *target, const float4 *jprm, const float4 *iprm, int32_t n,
+                     double dt, double4 *lx, double4 *lv, double4 *w, double *loss_part, int32_t nblocks,
+                     hipStream_t s);
+// status[0] = 1 when any particle is more than half the list skin from xref.
+void launch_drift_check(const float4 *pos, const float4 *xref, int32_t n, float half_skin2, int32_t *status,
+                        hipStream_t s);
 void launch_centroid(const float4 *pos, int32_t n, double *partial, int32_t nblocks, double *out3, hipStream_t s);
 void launch_fill_i32(int32_t *p, int32_t v, int64_t n, const int32_t *gate, hipStream_t s);
 void launch_pairs_extract(const float4 *pos, const float4 *jprm, const uint32_t *cl, const int32_t *cln,

@@ -20,6 +20,8 @@
 //      U[-1,1]; springs < 10 A; membrane set by the caller.
 //   3  one system of a batch: workload 0's CA network with the spring cutoff
 //      drawn from U[9,14] A by the system's seed (the caller draws the sizes).
+//   4  fitting network: workload 1's beads and force field at 100,000 beads
+//      (same density), springs < 10 A.
 // Any workload accepts an explicit bead count (n > 0) at the same density.
 #include <algorithm>
 #include <array>
@@ -305,6 +307,7 @@ void bsg_params(int config, bs_params * p)
         p->list_skin = 1.0f;
         break;
     case 1:
+    case 4:
         p->steric_enable = 1;
         p->steric_mode = BS_STERIC_AMBER_12_6;
         p->steric_cutoff = 12.0f;
@@ -347,9 +350,10 @@ bsg_system * bsg_generate(int config, uint64_t seed, int64_t n)
         break;
     }
     case 1:
+    case 4:
     {
         if (n <= 0)
-            n = 250000;
+            n = config == 4 ? 100000 : 250000;
         const double L = 140.0 * std::cbrt((double)n / 250000.0);
         uniform_beads(*s, n, L, 1.2, rng);
         for (int64_t i = 0; i < n; ++i)
@@ -358,7 +362,7 @@ bsg_system * bsg_generate(int config, uint64_t seed, int64_t n)
             s->eps[i] = (float)rng.uniform(0.1, 1.0);
             s->charge[i] = (float)(0.4 * rng.normal());
         }
-        cutoff_springs(*s, 8.0, rng, 1, 0.5);
+        cutoff_springs(*s, config == 4 ? 10.0 : 8.0, rng, 1, 0.5);
         break;
     }
     case 2:

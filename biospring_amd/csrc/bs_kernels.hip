@@ -733,6 +733,142 @@ __global__ void __launch_bounds__(BLOCK) k_sys_energies(const double *seg_part, 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint of the step (no reference counterpart).  Reverse sweep of
+// v' = v + dt/m (F(x) - gamma v), x' = x + dt v' with the adjoint state
+// (lx, lv) of (x, v):
+//   a = lv + dt lx,  w = dt/m a,  lx += H(x) w,  lv = (1 - gamma dt/m) a,
+// H = dF/dx (symmetric).  One spring edge (i, j), d = x_j - x_i, r = |d|,
+// F_i = k (r - L0) C d / r = phi d:
+//   H w |_i = phi dw + (phi'/r) (d . dw) d,  dw = w_j - w_i,  phi' = (k C - phi) / r
+//   dL/dk += -(d . dw) (r - L0) C / r,  dL/dL0 += (d . dw) k C / r
+// accumulated on the edge of the row that owns the spring (a static partner,
+// or else the larger index is the partner), so each spring is read-modify-
+// written once per step.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(BLOCK) k_adj_init(const float4 *pos, const float4 *target, const float4 *jprm,
+                                                     const float4 *iprm, int32_t n, double dt, double4 *lx,
+                                                     double4 *lv, double4 *w, double *loss_part)
+{
+    __shared__ double sh[BLOCK / 64];
+    const int32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    double e2 = 0.0;
+    if (i < n)
+    {
+        const float4 x = pos[i], y = target[i];
+        const double ex = (double)x.x - (double)y.x, ey = (double)x.y - (double)y.y, ez = (double)x.z - (double)y.z;
+        e2 = ex * ex + ey * ey + ez * ez;
+        const bool dyn = !(fbits(jprm[i].w) & PF_STATIC);
+        const double m = iprm[i].x;
+        const double c = dyn ? 2.0 / (double)n : 0.0;
+        const double4 l = make_double4(c * ex, c * ey, c * ez, 0.0);
+        const double s = (dyn && m > 0.0) ? dt / m * dt : 0.0;
+        lx[i] = l;
+        lv[i] = make_double4(0.0, 0.0, 0.0, 0.0);
+        w[i] = make_double4(s * l.x, s * l.y, s * l.z, 0.0);
+    }
+    const double t = block_sum(e2, sh);
+    if (threadIdx.x == 0)
+        loss_part[blockIdx.x] = t;
+}
+
+__global__ void k_drift_check(const float4 *pos, const float4 *xref, int32_t n, float half_skin2, int32_t *status)
+{
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool far = false;
+    if (i < n)
+    {
+        const float4 p = pos[i], r = xref[i];
+        const float ex = p.x - r.x, ey = p.y - r.y, ez = p.z - r.z;
+        far = !(ex * ex + ey * ey + ez * ez <= half_skin2);
+    }
+    if (__ballot(far) != 0ull && (threadIdx.x & 63) == 0)
+        atomicOr(&status[0], 1);
+}
+
+template <int G>
+__global__ void __launch_bounds__(BLOCK) k_adj_spring(AdjSpringArgs a)
+{
+    const int32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t row = tid / G;
+    const int32_t lane = tid % G;
+    const bool active = row < a.n;
+    double hx = 0.0, hy = 0.0, hz = 0.0;
+    float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
+    uint32_t fli = PF_STATIC;
+    double4 wi = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (active)
+    {
+        pi = a.pos[row];
+        fli = fbits(a.jprm[row].w);
+        wi = a.w[row];
+    }
+    const bool dyn = active && !(fli & PF_STATIC);
+    if (dyn && a.spring_enable)
+    {
+        const double C = a.k.c_spring * (double)a.spring_scale;
+        const int32_t beg = a.sp_ptr[row], end = a.sp_ptr[row + 1];
+        for (int32_t e = beg + lane; e < end; e += G)
+        {
+            const int32_t jr = a.sp_j[e];
+            const int32_t j = jr & 0x7fffffff;
+            const double k = a.sp_k[e], l0 = a.sp_l0[e];
+            const float4 pj = a.pos[j];
+            const double4 wj = a.w[j];
+            const double dx = (double)pj.x - (double)pi.x, dy = (double)pj.y - (double)pi.y,
+                         dz = (double)pj.z - (double)pi.z;
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            if (!(r2 > 0.0))
+                continue;
+            const double r = sqrt(r2), ir = 1.0 / r;
+            const double phi = k * (r - l0) * C * ir, dphi = (k * C - phi) * ir;
+            const double wx = wj.x - wi.x, wy = wj.y - wi.y, wz = wj.z - wi.z;
+            const double dd = dx * wx + dy * wy + dz * wz;
+            const double g = dphi * ir * dd;
+            hx = fma(phi, wx, fma(g, dx, hx));
+            hy = fma(phi, wy, fma(g, dy, hy));
+            hz = fma(phi, wz, fma(g, dz, hz));
+            if (jr < 0 || j > row)
+            {
+                const double wu = dd * ir * C;
+                a.dk[e] -= wu * (r - l0);
+                a.dl0[e] += wu * k;
+            }
+        }
+    }
+    for (int off = G / 2; off > 0; off >>= 1)
+    {
+        hx += __shfl_xor(hx, off, G);
+        hy += __shfl_xor(hy, off, G);
+        hz += __shfl_xor(hz, off, G);
+    }
+    if (!active || lane != 0)
+        return;
+    if (!dyn)
+    {
+        a.w_next[row] = make_double4(0.0, 0.0, 0.0, 0.0);
+        return;
+    }
+    if (a.hwp)
+    {
+        const double4 hp = a.hwp[row];
+        hx += hp.x;
+        hy += hp.y;
+        hz += hp.z;
+    }
+    const double4 lx = a.lx[row], lv = a.lv[row];
+    const double m = a.iprm[row].x;
+    const double ax = fma(a.dt, lx.x, lv.x), ay = fma(a.dt, lx.y, lv.y), az = fma(a.dt, lx.z, lv.z);
+    const double4 nx = make_double4(lx.x + hx, lx.y + hy, lx.z + hz, 0.0);
+    const double damp = m > 0.0 ? 1.0 - a.gamma * a.dt / m : 1.0;
+    const double4 nv = make_double4(ax * damp, ay * damp, az * damp, 0.0);
+    const double s = m > 0.0 ? a.dt / m : 0.0;
+    a.lx[row] = nx;
+    a.lv[row] = nv;
+    a.w_next[row] = make_double4(s * fma(a.dt, nx.x, nv.x), s * fma(a.dt, nx.y, nv.y), s * fma(a.dt, nx.z, nv.z),
+                                 0.0);
+}
+
 __global__ void k_prestep(int32_t *status, int32_t *step, int32_t interval)
 {
     if (interval > 0 && *step > 0 && (*step % interval) == 0)
@@ -825,6 +961,35 @@ void launch_spring(const SpringArgs &a, int32_t group, int32_t nb, hipStream_t s
     case 32: spring_g<32>(a, nb, s); break;
     default: spring_g<16>(a, nb, s); break;
     }
+}
+
+template <int G> static void adj_spring_g(const AdjSpringArgs &a, int32_t nb, hipStream_t s)
+{
+    k_adj_spring<G><<<nb, BLOCK, 0, s>>>(a);
+}
+
+void launch_adj_spring(const AdjSpringArgs &a, int32_t group, int32_t nb, hipStream_t s)
+{
+    switch (group)
+    {
+    case 4: adj_spring_g<4>(a, nb, s); break;
+    case 8: adj_spring_g<8>(a, nb, s); break;
+    case 32: adj_spring_g<32>(a, nb, s); break;
+    default: adj_spring_g<16>(a, nb, s); break;
+    }
+}
+
+void launch_adj_init(const float4 *pos, const float4 *target, const float4 *jprm, const float4 *iprm, int32_t n,
+                     double dt, double4 *lx, double4 *lv, double4 *w, double *loss_part, int32_t nblocks,
+                     hipStream_t s)
+{
+    k_adj_init<<<nblocks, BLOCK, 0, s>>>(pos, target, jprm, iprm, n, dt, lx, lv, w, loss_part);
+}
+
+void launch_drift_check(const float4 *pos, const float4 *xref, int32_t n, float half_skin2, int32_t *status,
+                        hipStream_t s)
+{
+    k_drift_check<<<nblk(n), BLOCK, 0, s>>>(pos, xref, n, half_skin2, status);
 }
 
 void launch_energies(const double4 *epp, const double4 *eps, const int32_t *seg_start, int32_t nseg,

@@ -388,6 +388,148 @@ __global__ void __launch_bounds__(BLOCK) k_pair(PairArgs a)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint of the pair terms (no reference counterpart).  For F_i = phi(r) d,
+// d = x_j - x_i, the Jacobian applied to w is
+//   H w |_i = phi dw + (phi'/r) (d . dw) d,  dw = w_j - w_i,
+// with phi = fm / r and phi' = (fm' - phi) / r from the force module fm of
+// each term (energy/steric.hpp, electrostatic.hpp) and its r-derivative fm'.
+// dL/dq_i gathers q_j (kf / r^3) (d . dw) (constant eps; 2 kf / r^4 for
+// eps(r) = eps r); a static partner's share goes to it by an FP64 atomic.
+// Lanes, lists and the exact cutoff test are those of k_pair.
+// ---------------------------------------------------------------------------
+template <int SMODE, int EMODE>
+__device__ inline void adj_pair_term(bool masked, float4 pi, float4 ri, uint32_t fli, double4 wi, float4 pj,
+                                     float4 rj, double4 wj, int32_t j, const AdjPairArgs &a, double cs, double &hx,
+                                     double &hy, double &hz, double &dqi)
+{
+    const float fx = pj.x - pi.x, fy = pj.y - pi.y, fz = pj.z - pi.z;
+    const float r2f = fx * fx + fy * fy + fz * fz;
+    bool on = masked && r2f < a.rc2lo;
+    if (masked && r2f >= a.rc2lo && r2f < a.rc2hi)
+        on = exact_two_stage(pi, pj, a.rc);
+    const uint32_t flj = fbits(rj.w);
+    const double dx = (double)pj.x - (double)pi.x, dy = (double)pj.y - (double)pi.y, dz = (double)pj.z - (double)pi.z;
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    on = on && r2 > 0.0;
+    const double r2s = on ? r2 : 1.0;
+    const double r = sqrt(r2s), ir = 1.0 / r, ir2 = ir * ir;
+    double fm = 0.0, dfm = 0.0;
+    if (SMODE == 0)
+    {
+        const double sr = (double)ri.x + (double)rj.x;
+        if (r <= sr)
+        {
+            fm = (r - sr) * cs;
+            dfm = cs;
+        }
+    }
+    else if (SMODE > 0)
+    {
+        if (r2f >= 0.01f)
+        {
+            const double eij = (double)ri.y * (double)rj.y, R = (double)ri.x * (double)rj.x;
+            const double s2 = R * R * ir2, s6 = s2 * s2 * s2, s8 = s6 * s2, s12 = s6 * s6;
+            if (SMODE == 1)
+            {
+                fm = 12.0 * eij * (s6 - s12) * ir * cs;
+                dfm = 12.0 * eij * (-7.0 * s6 + 13.0 * s12) * ir2 * cs;
+            }
+            else if (SMODE == 2)
+            {
+                fm = 24.0 * eij * (s6 - s8) * ir * cs;
+                dfm = 24.0 * eij * (-7.0 * s6 + 9.0 * s8) * ir2 * cs;
+            }
+            else
+            {
+                fm = eij * (6.0 * s6 - 8.0 * s8) * ir * cs;
+                dfm = eij * (-42.0 * s6 + 72.0 * s8) * ir2 * cs;
+            }
+        }
+    }
+    fm *= (double)a.steric_scale;
+    dfm *= (double)a.steric_scale;
+    const double wx = wj.x - wi.x, wy = wj.y - wi.y, wz = wj.z - wi.z;
+    const double dd = dx * wx + dy * wy + dz * wz;
+    if (EMODE > 0)
+    {
+        const bool qon = (fli & PF_CHARGED) && (flj & PF_CHARGED) && r2f >= 1.0e-6f;
+        const double qq = qon ? (double)pi.w * (double)pj.w : 0.0;
+        const double kq = a.kf * qq;
+        const double ir3 = ir2 * ir;
+        // per unit q_i q_j, the Coulomb part of phi is -c (c = kf/r^3 or 2 kf/r^4)
+        const double c = EMODE == 1 ? a.kf * ir3 : 2.0 * a.kf * ir3 * ir;
+        fm += EMODE == 1 ? -kq * ir2 : -2.0 * kq * ir3;
+        dfm += EMODE == 1 ? 2.0 * kq * ir3 : 6.0 * kq * ir3 * ir;
+        if (on && qon)
+        {
+            dqi += c * dd * (double)pj.w;
+            if (flj & PF_STATIC)
+                atomicAdd(&a.dq[j], c * dd * (double)pi.w);
+        }
+    }
+    const double phi = fm * ir, dphi = (dfm - phi) * ir;
+    const double g = on ? dphi * ir * dd : 0.0;
+    const double ph = on ? phi : 0.0;
+    hx = fma(ph, wx, fma(g, dx, hx));
+    hy = fma(ph, wy, fma(g, dy, hy));
+    hz = fma(ph, wz, fma(g, dz, hz));
+}
+
+template <int SMODE, int EMODE>
+__global__ void __launch_bounds__(BLOCK) k_adj_pair(AdjPairArgs a)
+{
+    const int lane = threadIdx.x & 63;
+    const int32_t ci = blockIdx.x * WAVES + (threadIdx.x >> 6);
+    const int32_t il = lane & 7, jl = lane >> 3;
+    const int32_t i = ci * CLUSTER + il;
+    double hx = 0.0, hy = 0.0, hz = 0.0, dqi = 0.0;
+    const double cs = a.k.c_spring;
+    if (ci < a.ncl)
+    {
+        const bool iv = i < a.n;
+        const float4 pi = iv ? a.pos[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const float4 ri = iv ? a.jprm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const double4 wi = iv ? a.w[i] : make_double4(0.0, 0.0, 0.0, 0.0);
+        const uint32_t fli = fbits(ri.w);
+        const uint32_t ibit = 1u << (24 + il);
+        const int32_t cnt16 = (a.cln[ci] + 15) & ~15;
+        const uint32_t *L = a.cl + (size_t)ci * a.cap;
+        for (int32_t t = 0; t < cnt16; t += 16)
+        {
+            const uint32_t w0 = L[t + jl], w1 = L[t + 8 + jl];
+            const int32_t j0 = (int32_t)(w0 & CL_JMASK), j1 = (int32_t)(w1 & CL_JMASK);
+            const float4 pj0 = a.pos[j0], rj0 = a.jprm[j0], pj1 = a.pos[j1], rj1 = a.jprm[j1];
+            const double4 wj0 = a.w[j0], wj1 = a.w[j1];
+            adj_pair_term<SMODE, EMODE>((w0 & ibit) != 0u, pi, ri, fli, wi, pj0, rj0, wj0, j0, a, cs, hx, hy, hz,
+                                        dqi);
+            adj_pair_term<SMODE, EMODE>((w1 & ibit) != 0u, pi, ri, fli, wi, pj1, rj1, wj1, j1, a, cs, hx, hy, hz,
+                                        dqi);
+        }
+    }
+    for (int off = 8; off < 64; off <<= 1)
+    {
+        hx += __shfl_xor(hx, off, 64);
+        hy += __shfl_xor(hy, off, 64);
+        hz += __shfl_xor(hz, off, 64);
+        dqi += __shfl_xor(dqi, off, 64);
+    }
+    if (ci < a.ncl && jl == 0 && i < a.n)
+    {
+        double4 h = make_double4(hx, hy, hz, 0.0);
+        if (a.accumulate)
+        {
+            const double4 o = a.hw[i];
+            h.x += o.x;
+            h.y += o.y;
+            h.z += o.z;
+        }
+        a.hw[i] = h;
+        if (EMODE > 0)
+            a.dq[i] += dqi;
+    }
+}
+
 // Processed pairs of one search kind, as (min, max) original ids
 // (Particle.cpp:205-224: i dynamic, j static or later in id order).  One
 // counter update per wave keeps the atomic traffic low.
@@ -490,6 +632,28 @@ void launch_pair(const PairArgs &a, int32_t elec_mode, int32_t hydro, int32_t nb
     case 1: pair2<1>(a, elec_mode, hydro, nb, s); break;
     case 2: pair2<2>(a, elec_mode, hydro, nb, s); break;
     default: pair2<3>(a, elec_mode, hydro, nb, s); break;
+    }
+}
+
+template <int S> static void adj2(const AdjPairArgs &a, int32_t em, int32_t nb, hipStream_t s)
+{
+    if (em == 0)
+        k_adj_pair<S, 0><<<nb, BLOCK, 0, s>>>(a);
+    else if (em == 1)
+        k_adj_pair<S, 1><<<nb, BLOCK, 0, s>>>(a);
+    else
+        k_adj_pair<S, 2><<<nb, BLOCK, 0, s>>>(a);
+}
+
+void launch_adj_pair(const AdjPairArgs &a, int32_t steric_mode, int32_t elec_mode, int32_t nb, hipStream_t s)
+{
+    switch (steric_mode)
+    {
+    case -1: adj2<-1>(a, elec_mode, nb, s); break;
+    case 0: adj2<0>(a, elec_mode, nb, s); break;
+    case 1: adj2<1>(a, elec_mode, nb, s); break;
+    case 2: adj2<2>(a, elec_mode, nb, s); break;
+    default: adj2<3>(a, elec_mode, nb, s); break;
     }
 }
 

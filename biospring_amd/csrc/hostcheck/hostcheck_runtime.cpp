@@ -557,6 +557,116 @@ void launch_spring(const SpringArgs &a, int32_t group, int32_t nb, hipStream_t)
             a.eps[i] = make_double4(0, 0, 0, 0);
 }
 
+// Adjoint launchers: extents and list invariants as the kernels assume them.
+// The stand-ins compute the loss partials exactly, leave H w at zero and
+// count one unit of dk per reverse step on each owning edge (dynamic row,
+// static or larger partner), so that the host's edge-to-spring mapping can
+// be checked.
+void launch_adj_pair(const AdjPairArgs &a, int32_t, int32_t, int32_t nb, hipStream_t)
+{
+    need(a.pos, sizeof(float4) * a.n, "adj_pair: pos");
+    need(a.jprm, sizeof(float4) * a.n, "adj_pair: jprm");
+    need(a.cln, sizeof(int32_t) * a.ncl, "adj_pair: cln");
+    need(a.cl, sizeof(uint32_t) * (size_t)a.ncl * a.cap, "adj_pair: cl");
+    need(a.w, sizeof(double4) * a.n, "adj_pair: w");
+    need(a.hw, sizeof(double4) * a.n, "adj_pair: hw");
+    need(a.dq, sizeof(double) * a.n, "adj_pair: dq");
+    if ((int64_t)nb * (BLOCK / 64) < a.ncl)
+        hc_fail("adj_pair: grid too small");
+    for (int32_t ci = 0; ci < a.ncl; ++ci)
+    {
+        if (a.cln[ci] < 0 || a.cln[ci] > a.cap)
+            hc_fail("adj_pair: list length out of range");
+        const int32_t cnt16 = (a.cln[ci] + 15) & ~15;
+        if (cnt16 > a.cap)
+            hc_fail("adj_pair: padded list length exceeds the capacity");
+        for (int32_t t = 0; t < cnt16; ++t)
+            if ((int32_t)(a.cl[(size_t)ci * a.cap + t] & CL_JMASK) >= a.n)
+                hc_fail("adj_pair: neighbour index out of range");
+    }
+    if (!a.accumulate)
+        for (int32_t i = 0; i < a.n; ++i)
+            a.hw[i] = make_double4(0, 0, 0, 0);
+}
+
+void launch_adj_spring(const AdjSpringArgs &a, int32_t group, int32_t nb, hipStream_t)
+{
+    const int32_t n = a.n;
+    need(a.pos, sizeof(float4) * n, "adj_spring: pos");
+    need(a.jprm, sizeof(float4) * n, "adj_spring: jprm");
+    need(a.iprm, sizeof(float4) * n, "adj_spring: iprm");
+    need(a.sp_ptr, sizeof(int32_t) * (n + 1), "adj_spring: sp_ptr");
+    need(a.w, sizeof(double4) * n, "adj_spring: w");
+    need(a.lx, sizeof(double4) * n, "adj_spring: lx");
+    need(a.lv, sizeof(double4) * n, "adj_spring: lv");
+    need(a.w_next, sizeof(double4) * n, "adj_spring: w_next");
+    if (a.w_next == a.w)
+        hc_fail("adj_spring: w_next aliases w");
+    if (a.hwp)
+        need(a.hwp, sizeof(double4) * n, "adj_spring: hwp");
+    if ((int64_t)nb * BLOCK < (int64_t)n * group)
+        hc_fail("adj_spring: grid too small");
+    const int32_t ne = a.sp_ptr[n];
+    if (ne > 0)
+    {
+        need(a.sp_j, sizeof(int32_t) * ne, "adj_spring: sp_j");
+        need(a.sp_k, sizeof(float) * ne, "adj_spring: sp_k");
+        need(a.sp_l0, sizeof(float) * ne, "adj_spring: sp_l0");
+        need(a.dk, sizeof(double) * ne, "adj_spring: dk");
+        need(a.dl0, sizeof(double) * ne, "adj_spring: dl0");
+    }
+    for (int32_t i = 0; i < n; ++i)
+    {
+        const bool dyn = !(fbits(a.jprm[i].w) & PF_STATIC);
+        if (dyn && a.spring_enable)
+            for (int32_t e = a.sp_ptr[i]; e < a.sp_ptr[i + 1]; ++e)
+            {
+                if ((a.sp_j[e] & 0x7fffffff) >= n)
+                    hc_fail("adj_spring: edge index out of range");
+                if (a.sp_j[e] < 0 || a.sp_j[e] > i)
+                    a.dk[e] += 1.0;
+            }
+        a.w_next[i] = make_double4(0, 0, 0, 0);
+    }
+}
+
+void launch_adj_init(const float4 *pos, const float4 *target, const float4 *jprm, const float4 *iprm, int32_t n,
+                     double, double4 *lx, double4 *lv, double4 *w, double *loss_part, int32_t nblocks, hipStream_t)
+{
+    need(pos, sizeof(float4) * n, "adj_init: pos");
+    need(target, sizeof(float4) * n, "adj_init: target");
+    need(jprm, sizeof(float4) * n, "adj_init: jprm");
+    need(iprm, sizeof(float4) * n, "adj_init: iprm");
+    need(lx, sizeof(double4) * n, "adj_init: lx");
+    need(lv, sizeof(double4) * n, "adj_init: lv");
+    need(w, sizeof(double4) * n, "adj_init: w");
+    need(loss_part, sizeof(double) * nblocks, "adj_init: loss_part");
+    if ((int64_t)nblocks * BLOCK < n)
+        hc_fail("adj_init: grid too small");
+    for (int32_t b = 0; b < nblocks; ++b)
+    {
+        double t = 0.0;
+        for (int32_t i = b * BLOCK; i < std::min(n, (b + 1) * BLOCK); ++i)
+        {
+            const double ex = (double)pos[i].x - target[i].x, ey = (double)pos[i].y - target[i].y,
+                         ez = (double)pos[i].z - target[i].z;
+            t += ex * ex + ey * ey + ez * ez;
+        }
+        loss_part[b] = t;
+    }
+    for (int32_t i = 0; i < n; ++i)
+    {
+        lx[i] = lv[i] = w[i] = make_double4(0, 0, 0, 0);
+    }
+}
+
+void launch_drift_check(const float4 *pos, const float4 *xref, int32_t n, float, int32_t *status, hipStream_t)
+{
+    need(pos, sizeof(float4) * n, "drift_check: pos");
+    need(xref, sizeof(float4) * n, "drift_check: xref");
+    need(status, sizeof(int32_t) * 4, "drift_check: status");
+}
+
 void launch_energies(const double4 *epp, const double4 *eps, const int32_t *seg_start, int32_t nseg,
                      const int32_t *sys_seg, int32_t nsys, double *seg_part, double *out_sys, double *out_total,
                      hipStream_t)

@@ -128,16 +128,17 @@ class SpringNetwork:
                 "edges": int(c[4])}
 
     def profile(self, enable=True):
-        """Per-kernel-class HIP-event timing of run() (resets the totals)."""
+        """Per-kernel-class HIP-event timing of run() and adjoint() (resets the
+        totals)."""
         check(self._L.bs_profile(self._h, 1 if enable else 0), self._h, "bs_profile")
 
     def getProfile(self):
-        ms = np.zeros(3, np.float64)
-        n = np.zeros(3, np.int64)
+        ms = np.zeros(_abi.PROF_NUM, np.float64)
+        n = np.zeros(_abi.PROF_NUM, np.int64)
         check(self._L.bs_get_profile(self._h, ptr(ms, ctypes.c_double), ptr(n, ctypes.c_int64)), self._h,
               "bs_get_profile")
-        names = ("rebuild", "pair", "spring")
-        return {names[c]: {"ms": float(ms[c]), "launches": int(n[c])} for c in range(3)}
+        names = ("rebuild", "pair", "spring", "adj_pair", "adj_spring")
+        return {names[c]: {"ms": float(ms[c]), "launches": int(n[c])} for c in range(_abi.PROF_NUM)}
 
     def lastNonfiniteParticle(self):
         return int(self._L.bs_last_nonfinite_particle(self._h))
@@ -235,6 +236,26 @@ class SpringNetwork:
         g, gr = ctypes.c_int32(), ctypes.c_int32()
         check(self._L.bs_autotune(self._h, int(steps), ctypes.byref(g), ctypes.byref(gr)), self._h, "bs_autotune")
         return int(g.value), int(gr.value)
+
+    # -- gradients (no reference counterpart) --
+    def adjoint(self, nsteps, target, checkpoint_every=0):
+        """Runs nsteps steps (the state advances as with run()) and returns
+        (loss, dk, dl0, dq): the loss (1/n) sum |x_T - target|^2 and its FP64
+        gradients per spring (stiffness, rest length; bs_set_springs order)
+        and per particle charge.  The external force set by setForce applies."""
+        y = f32(target).reshape(-1)
+        if y.size != 3 * self.n:
+            raise ValueError("target must hold 3 coordinates per particle")
+        m = int(self._L.bs_num_springs(self._h))
+        dk = np.zeros(max(m, 1))
+        dl0 = np.zeros(max(m, 1))
+        dq = np.zeros(max(self.n, 1))
+        loss = ctypes.c_double()
+        c = ctypes
+        check(self._L.bs_adjoint(self._h, int(nsteps), int(checkpoint_every), ptr(y, c.c_float),
+                                 ptr(self._ext, c.c_float), c.byref(loss), ptr(dk, c.c_double),
+                                 ptr(dl0, c.c_double), ptr(dq, c.c_double)), self._h, "bs_adjoint")
+        return float(loss.value), dk[:m], dl0[:m], dq[:self.n]
 
     def close(self):
         if self._h:

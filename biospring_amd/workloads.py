@@ -3,7 +3,9 @@
 Workload 0: 2,000-bead CA elastic network (chain in a 30 A sphere, springs
 < 12 A, linear steric 4 A, Coulomb 16 A, 20% charged).  Workload 1: 250,000
 beads in a 140 A cube (springs < 8 A, Amber 12-6 + Coulomb with eps(r) = r,
-12 A cutoffs).  Workload 2: IMPALA CG beads.  See biospring_amd/csrc/bs_gen.cpp.
+12 A cutoffs).  Workload 2: IMPALA CG beads.  Workload 3: one system of a
+batch.  Workload 4: workload 1's beads at 100,000 with springs < 10 A (the
+fitting network).  See biospring_amd/csrc/bs_gen.cpp.
 """
 import ctypes
 import hashlib
@@ -140,3 +142,25 @@ def batch(nsys=512, seed=1, nmin=200, nmax=20000):
                  l0=cat("l0"), k=cat("k"))
     out.params = parts[0].params.copy()
     return out, off, parts
+
+
+def fitting_target(system, seed=1, modes=3, amplitude=5.0):
+    """Workload 4's target conformation: the positions displaced along
+    `modes` smooth low-frequency fields u_m(x) = a_m sin(k_m . x + phi_m)
+    (a_m a random unit vector, k_m = 2 pi / L times a nonzero vector of
+    -1/0/1 entries over the system's extent L, phi_m uniform), scaled so
+    that the largest displacement is `amplitude` A."""
+    rng = np.random.default_rng(seed)
+    x = system.x.reshape(-1, 3).astype(np.float64)
+    ext = np.maximum(x.max(0) - x.min(0), 1e-9)
+    u = np.zeros_like(x)
+    for _ in range(modes):
+        a = rng.normal(size=3)
+        a /= np.linalg.norm(a)
+        k = np.zeros(3)
+        while not k.any():
+            k = rng.integers(-1, 2, size=3).astype(np.float64)
+        ph = rng.uniform(0.0, 2.0 * np.pi)
+        u += np.sin((x - x.min(0)) @ (2.0 * np.pi * k / ext) + ph)[:, None] * a
+    u *= amplitude / max(np.linalg.norm(u, axis=1).max(), 1e-12)
+    return (x + u).astype(np.float32)

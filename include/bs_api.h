@@ -225,10 +225,12 @@ int bs_get_counters(const bs_handle *h, int64_t *out6);
 #define BS_PROF_REBUILD 0  /* neighbour-upkeep chain (box, binning, list build) */
 #define BS_PROF_PAIR    1  /* non-bonded pair kernels */
 #define BS_PROF_SPRING  2  /* spring gather + integrator (or force epilogue) */
-#define BS_PROF_NUM     3
+#define BS_PROF_ADJ_PAIR   3  /* bs_adjoint: pair part of H w and dL/dq */
+#define BS_PROF_ADJ_SPRING 4  /* bs_adjoint: spring part, parameter gradients, reverse update */
+#define BS_PROF_NUM     5
 
-/* Turns per-kernel-class event timing of bs_step on (1) or off (0) and resets
- * the totals.  Counterpart of the reference's wall-clock timeit::Profiler
+/* Turns per-kernel-class event timing of bs_step and bs_adjoint on (1) or off
+ * (0) and resets the totals.  Counterpart of the reference's wall-clock timeit::Profiler
  * (src/timeit.hpp:16-168; "Average framerate", SpringNetwork.cpp:305-308). */
 int bs_profile(bs_handle *h, int32_t enable);
 
@@ -335,6 +337,28 @@ int32_t bs_num_systems(const bs_handle *h);
 /* The six energies of every system after the last bs_step / bs_compute_forces,
  * out[6 * s + BS_E_*]. */
 int bs_get_system_energies(bs_handle *h, double *out);
+
+/* ---- adjoint ------------------------------------------------------------------- */
+
+/* Gradients of the trajectory loss
+ *   L = (1/n) sum_i |x_T,i - y_i|^2
+ * after nsteps steps from the current state (which advances, as bs_step) with
+ * respect to each spring's stiffness and rest length (dk, dl0: one value per
+ * spring kept by bs_set_springs, in its order) and each particle's charge
+ * (dq[n]).  target3 = y (3n floats, required); ext_force3 as in bs_step; every
+ * output is nullable.  The reverse sweep
+ *   a = lv + dt lx,  w = dt/m a,  lx += H(x_s) w,  lv = (1 - gamma dt/m) a
+ * runs in FP64 over the FP32 trajectory, H being the Jacobian of the forces
+ * (springs, the four steric modes, Coulomb with both dielectric models;
+ * viscosity enters through gamma), with the pair set of each state fixed as
+ * the forward step found it.  States are saved every checkpoint_every steps
+ * (<= 0: ceil(sqrt(nsteps))) and segments recomputed, so the work is about
+ * two forward passes plus the reverse sweep and the memory n * (32 bytes per
+ * checkpoint + 16 bytes per step of a segment).  BS_ERR_ARG with IMPALA, the
+ * pseudo-hydrophobic term, rigid bodies, field grids or compensated
+ * positions.  No reference counterpart (the reference has no gradients). */
+int bs_adjoint(bs_handle *h, int32_t nsteps, int32_t checkpoint_every, const float *target3,
+               const float *ext_force3, double *loss, double *dk, double *dl0, double *dq);
 
 /* ---- tuning ---------------------------------------------------------------------- */
 

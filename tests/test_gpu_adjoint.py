@@ -1,0 +1,148 @@
+"""Adjoint of the step on the GPU (bs_adjoint): gradients of the trajectory
+loss (1/n) sum |x_T - y|^2 with respect to spring stiffness, rest length and
+charge.
+
+The yardstick is the FP64 oracle's adjoint, itself pinned by central finite
+differences of the FP64 loss and by H w against the force derivative.  The
+GPU sweeps the FP32 trajectory in FP64, so its gradients differ from the
+oracle's by the float trajectory's own error, which each case reports.  Also:
+a directional central difference of the GPU's own loss, independence of the
+checkpoint interval, determinism, and that the state advances as a forward
+run does."""
+import sys
+
+import numpy as np
+import pytest
+
+from biospring_amd import _abi, workloads
+from biospring_amd.network import SpringNetwork
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+# max |g_gpu - g_oracle| / max |g_oracle| per gradient
+GRAD_RTOL = 2e-4
+
+
+def _log(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _case(name):
+    if name == "enm":  # linear steric + constant-dielectric Coulomb, static beads
+        s = workloads.generate(0, seed=11, n=500)
+        fl = s.flags.copy()
+        fl[::9] |= _abi.FLAG_STATIC
+        s.flags = fl
+        return s, s.params.copy(), 30
+    s = workloads.generate(1, seed=12, n=400)
+    p = s.params.copy()
+    if name == "amber_ddd":
+        return s, p, 20
+    p.dielectric_model = _abi.DIELECTRIC_CONSTANT
+    p.steric_mode = {"lewitt": _abi.STERIC_LEWITT_8_6, "zacharias": _abi.STERIC_ZACHARIAS_8_6}[name]
+    return s, p, 6 if name == "zacharias" else 12
+
+
+def _target(s):
+    x0 = s.x.reshape(-1, 3).astype(np.float64)
+    return (x0 + 0.5 * np.sin(x0 / 7.0)).astype(np.float32)
+
+
+def _rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+@pytest.mark.parametrize("name", ["amber_ddd", "enm", "lewitt", "zacharias"])
+def test_adjoint_matches_fp64_oracle(name):
+    s, p, T = _case(name)
+    y = _target(s)
+    _log(f"adjoint {name}: n={s.n} springs={s.m} T={T}")
+    net = SpringNetwork.from_system(s, params=p)
+    loss, dk, dl, dq = net.adjoint(T, y, checkpoint_every=7)
+    x_gpu, _ = net.getState()
+    o = po.Oracle(s, params=p)
+    lo, dko, dlo, dqo = o.adjoint(T, y.astype(np.float64))
+    xo, _ = o.state(po.F64)
+    traj = float(np.abs(x_gpu - xo).max())
+    errs = {"k": _rel(dk, dko), "l0": _rel(dl, dlo), "q": _rel(dq, dqo)}
+    _log(f"  loss {loss:.9g} vs {lo:.9g}; trajectory dev {traj:.2e} A; gradient errors {errs}")
+    assert abs(loss - lo) <= 1e-5 * lo
+    for g, e in errs.items():
+        assert e <= GRAD_RTOL, (g, e)
+    # uncharged particles have no charge gradient; static ones keep theirs
+    assert np.all(dq[s.charge == 0] == 0)
+
+
+def test_adjoint_directional_finite_difference():
+    """sum_s dL/dk_s k_s r_s against (L(k(1 + eps r)) - L(k(1 - eps r))) / 2 eps
+    of the GPU's own forward, and likewise for L0 and the charges."""
+    s = workloads.generate(1, seed=13, n=3000)
+    p = s.params.copy()
+    y = _target(s)
+    T, eps = 20, 1e-2
+    rng = np.random.default_rng(5)
+    rk, rl, rq = rng.normal(size=s.m), rng.normal(size=s.m), rng.normal(size=s.n) * (s.charge != 0)
+
+    def loss_of(k=None, l0=None, q=None):
+        t = workloads.System(**{f: getattr(s, f).copy() for f in workloads.System.__dataclass_fields__
+                                if f != "params"})
+        t.params = p
+        if k is not None:
+            t.k = k.astype(np.float32)
+        if l0 is not None:
+            t.l0 = l0.astype(np.float32)
+        if q is not None:
+            t.charge = q.astype(np.float32)
+        n = SpringNetwork.from_system(t, params=p)
+        n.run(T)
+        x, _ = n.getState()
+        return float(np.mean(np.sum((x.astype(np.float64) - y) ** 2, axis=1)))
+
+    net = SpringNetwork.from_system(s, params=p)
+    loss, dk, dl, dq = net.adjoint(T, y, checkpoint_every=5)
+    k64, l64, q64 = s.k.astype(np.float64), s.l0.astype(np.float64), s.charge.astype(np.float64)
+    checks = (
+        ("k", float(np.dot(dk, k64 * rk)), lambda e: loss_of(k=k64 * (1 + e * rk))),
+        ("l0", float(np.dot(dl, 0.1 * rl)), lambda e: loss_of(l0=l64 + 0.1 * e * rl)),
+        ("q", float(np.dot(dq, 0.4 * rq)), lambda e: loss_of(q=q64 + 0.4 * e * rq)),
+    )
+    for name, pred, f in checks:
+        fd = (f(eps) - f(-eps)) / (2 * eps)
+        _log(f"  directional d/d{name}: adjoint {pred:.6e} central difference {fd:.6e}")
+        assert abs(pred - fd) <= 1e-2 * abs(fd), (name, pred, fd)
+
+
+def test_adjoint_checkpoint_interval_and_determinism():
+    s = workloads.generate(1, seed=14, n=6000)
+    p = s.params.copy()
+    y = _target(s)
+    T = 24
+    x0 = s.x.reshape(-1, 3)
+    res = {}
+    for K in (24, 1, 5, 0, 5):
+        net = SpringNetwork.from_system(s, params=p)
+        r = net.adjoint(T, y, checkpoint_every=K)
+        if K in res:  # same interval again: bitwise the same
+            assert r[0] == res[K][0]
+            for a, b in zip(r[1:], res[K][1:]):
+                assert np.array_equal(a, b)
+        res[K] = r
+        x, v = net.getState()
+        assert np.isfinite(x).all() and not np.array_equal(x, x0)
+        net.run(3)  # the handle keeps stepping after the sweep
+        assert np.isfinite(net.getState()[0]).all()
+    ref = res[24]
+    for K, r in res.items():
+        assert abs(r[0] - ref[0]) <= 1e-6 * ref[0]
+        for g in range(1, 4):
+            assert _rel(r[g], ref[g]) <= 1e-5, (K, g, _rel(r[g], ref[g]))
+
+
+def test_adjoint_rejects_terms_without_adjoint():
+    s = workloads.generate(2, seed=3, n=2000)
+    p = workloads.impala_params(s, "flat")
+    net = SpringNetwork.from_system(s, params=p)
+    with pytest.raises(_abi.EngineError):
+        net.adjoint(3, s.x.reshape(-1, 3))

@@ -216,3 +216,59 @@ def test_batch_of_systems_sequence(hostcheck_lib):
         print("leak", L.hc_live_allocations() - base)
     """, hostcheck_lib)
     assert "leak 0" in out, out
+
+
+def test_adjoint_sequence(hostcheck_lib):
+    """Checkpointed reverse sweep: buffers, loss plumbing (the stand-in step
+    leaves positions unchanged) and the edge-to-spring map (the stand-in
+    counts one unit of dk per reverse step on the owning edge of each spring
+    with a dynamic end)."""
+    out = _run("""
+        s = workloads.generate(0, seed=4, n=400)
+        fl = s.flags.copy(); fl[::7] |= _abi.FLAG_STATIC; s.flags = fl
+        base = L.hc_live_allocations()
+        net = SpringNetwork.from_system(s, library=L)
+        net.setup()
+        x0, _ = net.getState()
+        y = x0 + np.random.default_rng(0).normal(0, 0.3, x0.shape).astype(np.float32)
+        ref = float(np.mean(np.sum((x0.astype(np.float64) - y) ** 2, axis=1)))
+        a, b, _, _ = s.spring_arrays()
+        dyn = (fl & _abi.FLAG_STATIC) == 0
+        ends = (dyn[a] | dyn[b]).astype(int)
+        for T, K in ((7, 3), (5, 0), (4, 4), (1, 9), (0, 0)):
+            loss, dk, dl0, dq = net.adjoint(T, y, K)
+            assert abs(loss - ref) <= 1e-12 * ref, (T, K, loss, ref)
+            assert dk.shape == (net_m := int(L.bs_num_springs(net._h)),) and dq.shape == (400,)
+            assert np.array_equal(dk, T * ends), (T, K)
+        p = s.params.copy(); p.imp_enable = 1
+        net.setParams(p)
+        try:
+            net.adjoint(3, y)
+            raise SystemExit("adjoint accepted IMPALA")
+        except _abi.EngineError as e:
+            assert e.code == _abi.BS_ERR_ARG
+        p.imp_enable = 0; p.compensated_positions = 1
+        net.setParams(p)
+        try:
+            net.adjoint(3, y)
+            raise SystemExit("adjoint accepted compensated positions")
+        except _abi.EngineError as e:
+            assert e.code == _abi.BS_ERR_ARG
+        p.compensated_positions = 0
+        net.setParams(p)
+        for bad in (-1,):
+            try:
+                net.adjoint(bad, y)
+                raise SystemExit("adjoint accepted a negative step count")
+            except _abi.EngineError as e:
+                assert e.code == _abi.BS_ERR_ARG
+        net.addRigidBody(np.arange(10, 30))
+        try:
+            net.adjoint(2, y)
+            raise SystemExit("adjoint accepted rigid bodies")
+        except _abi.EngineError as e:
+            assert e.code == _abi.BS_ERR_ARG
+        net.close()
+        print("leak", L.hc_live_allocations() - base)
+    """, hostcheck_lib)
+    assert "leak 0" in out, out
