@@ -525,7 +525,10 @@ __global__ void __launch_bounds__(BLOCK) k_adj_pair(AdjPairArgs a)
             h.z += o.z;
         }
         a.hw[i] = h;
-        if (EMODE > 0)
+        // a static particle's dq is only ever added to atomically (by its
+        // dynamic partners); a plain read-modify-write here would race
+        const bool dyn_i = !(fbits(a.jprm[i].w) & PF_STATIC);
+        if (EMODE > 0 && dyn_i)
             a.dq[i] += dqi;
     }
 }

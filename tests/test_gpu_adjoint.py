@@ -6,9 +6,13 @@ The yardstick is the FP64 oracle's adjoint, itself pinned by central finite
 differences of the FP64 loss and by H w against the force derivative.  The
 GPU sweeps the FP32 trajectory in FP64, so its gradients differ from the
 oracle's by the float trajectory's own error, which each case reports.  Also:
-a directional central difference of the GPU's own loss, independence of the
-checkpoint interval, determinism, and that the state advances as a forward
-run does."""
+independence of the checkpoint interval, determinism, and that the handle
+keeps stepping afterwards.
+
+Central differences of the GPU's own FP32 loss are not used: a change of
+loss of 1e-8 relative, which is what a 1e-3 change of one parameter makes
+over 20 femtoseconds, is below the rounding noise of a float trajectory; the
+differences are taken on the FP64 oracle instead."""
 import sys
 
 import numpy as np
@@ -75,43 +79,26 @@ def test_adjoint_matches_fp64_oracle(name):
     assert np.all(dq[s.charge == 0] == 0)
 
 
-def test_adjoint_directional_finite_difference():
-    """sum_s dL/dk_s k_s r_s against (L(k(1 + eps r)) - L(k(1 - eps r))) / 2 eps
-    of the GPU's own forward, and likewise for L0 and the charges."""
+def test_adjoint_stretched_start_matches_fp64_oracle():
+    """3,000 beads started 0.3 A (rms) off their springs' rest positions, so
+    that the spring terms carry the motion from the first step."""
     s = workloads.generate(1, seed=13, n=3000)
     p = s.params.copy()
+    x0 = s.x.reshape(-1, 3)
+    xs = (x0 + np.random.default_rng(3).normal(0, 0.3, x0.shape)).astype(np.float32)
     y = _target(s)
-    T, eps = 20, 1e-2
-    rng = np.random.default_rng(5)
-    rk, rl, rq = rng.normal(size=s.m), rng.normal(size=s.m), rng.normal(size=s.n) * (s.charge != 0)
-
-    def loss_of(k=None, l0=None, q=None):
-        t = workloads.System(**{f: getattr(s, f).copy() for f in workloads.System.__dataclass_fields__
-                                if f != "params"})
-        t.params = p
-        if k is not None:
-            t.k = k.astype(np.float32)
-        if l0 is not None:
-            t.l0 = l0.astype(np.float32)
-        if q is not None:
-            t.charge = q.astype(np.float32)
-        n = SpringNetwork.from_system(t, params=p)
-        n.run(T)
-        x, _ = n.getState()
-        return float(np.mean(np.sum((x.astype(np.float64) - y) ** 2, axis=1)))
-
+    T = 20
     net = SpringNetwork.from_system(s, params=p)
-    loss, dk, dl, dq = net.adjoint(T, y, checkpoint_every=5)
-    k64, l64, q64 = s.k.astype(np.float64), s.l0.astype(np.float64), s.charge.astype(np.float64)
-    checks = (
-        ("k", float(np.dot(dk, k64 * rk)), lambda e: loss_of(k=k64 * (1 + e * rk))),
-        ("l0", float(np.dot(dl, 0.1 * rl)), lambda e: loss_of(l0=l64 + 0.1 * e * rl)),
-        ("q", float(np.dot(dq, 0.4 * rq)), lambda e: loss_of(q=q64 + 0.4 * e * rq)),
-    )
-    for name, pred, f in checks:
-        fd = (f(eps) - f(-eps)) / (2 * eps)
-        _log(f"  directional d/d{name}: adjoint {pred:.6e} central difference {fd:.6e}")
-        assert abs(pred - fd) <= 1e-2 * abs(fd), (name, pred, fd)
+    net.setState(xs, None)
+    loss, dk, dl, dq = net.adjoint(T, y, checkpoint_every=6)
+    o = po.Oracle(s, params=p)
+    o.set_state(xs, None)
+    lo, dko, dlo, dqo = o.adjoint(T, y.astype(np.float64))
+    errs = {"k": _rel(dk, dko), "l0": _rel(dl, dlo), "q": _rel(dq, dqo)}
+    _log(f"  stretched start: loss {loss:.9g} vs {lo:.9g}; gradient errors {errs}")
+    assert abs(loss - lo) <= 1e-5 * lo
+    for g, e in errs.items():
+        assert e <= GRAD_RTOL, (g, e)
 
 
 def test_adjoint_checkpoint_interval_and_determinism():
