@@ -94,6 +94,7 @@ struct bs_handle
     float *sp_k = nullptr, *sp_l0 = nullptr;
     int64_t nedges = 0;
     int32_t ncl = 0;                                  // i-clusters of 8
+    int32_t *sp_own = nullptr;                        // [m] owning CSR edge of each spring (bs_adjoint), -1: none
     int32_t *ex_ptr = nullptr;                        // [ncl + 1] static exclusions
     uint32_t *ex_w = nullptr;                         // j << 8 | bonded mask
     int32_t *cell_of = nullptr, *counts = nullptr, *cstart = nullptr, *cursor = nullptr, *ctmp = nullptr,
@@ -209,9 +210,10 @@ void free_device(bs_handle *h)
     h->seg_part = h->sys_E = nullptr;
     h->shift = nullptr;
     h->nseg = 0;
-    f(h->ex_ptr); f(h->ex_w);
+    f(h->ex_ptr); f(h->ex_w); f(h->sp_own);
     h->ex_ptr = nullptr;
     h->ex_w = nullptr;
+    h->sp_own = nullptr;
     f(h->bodies_d); f(h->rx); f(h->rp0); f(h->frig); f(h->slot_p); f(h->slot_b); f(h->bstart);
     f(h->grid_d[0]); f(h->grid_d[1]); f(h->cent_part); f(h->cent);
     h->bodies_d = nullptr;
@@ -864,6 +866,48 @@ struct Scratch
         return q;
     }
 };
+
+// The CSR edge on which the adjoint sweep accumulates each spring's
+// gradients: the edge of the dynamic row that owns it (a static partner, or
+// else the larger index is the partner; the rule of k_adj_spring), -1 when
+// both ends are static.  Built once per setup.
+int build_spring_owners(bs_handle *h)
+{
+    const int64_t n = h->n, m = (int64_t)h->s1.size(), E = h->nedges;
+    if (h->sp_own || m == 0)
+        return BS_OK;
+    std::vector<int32_t> hp((size_t)n + 1), hj((size_t)std::max<int64_t>(E, 1)), own((size_t)m, -1);
+    HIPCHK(h, hipMemcpy(hp.data(), h->sp_ptr, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToHost));
+    if (E > 0)
+        HIPCHK(h, hipMemcpy(hj.data(), h->sp_j, sizeof(int32_t) * E, hipMemcpyDeviceToHost));
+#pragma omp parallel for schedule(static, 4096)
+    for (int64_t s = 0; s < m; ++s)
+    {
+        int32_t a = h->iperm[h->s1[(size_t)s]], b = h->iperm[h->s2[(size_t)s]];
+        const bool sa = (h->flags[h->s1[(size_t)s]] & BS_FLAG_STATIC) != 0;
+        const bool sb = (h->flags[h->s2[(size_t)s]] & BS_FLAG_STATIC) != 0;
+        if (sa && sb)
+            continue;
+        if (sa || (!sb && b < a))
+            std::swap(a, b); // row a owns the edge to b
+        int32_t lo = hp[(size_t)a], hi = hp[(size_t)a + 1];
+        while (lo < hi)
+        {
+            const int32_t mid = (lo + hi) >> 1;
+            if ((hj[(size_t)mid] & 0x7fffffff) < b)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        if (lo < hp[(size_t)a + 1] && (hj[(size_t)lo] & 0x7fffffff) == b)
+            own[(size_t)s] = lo;
+    }
+    hipError_t e = hipSuccess;
+    h->sp_own = dalloc<int32_t>((size_t)m, e);
+    HIPCHK(h, e);
+    HIPCHK(h, hipMemcpy(h->sp_own, own.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice));
+    return BS_OK;
+}
 
 // H w of the pair terms at the positions in pos[cur] (lists valid there),
 // into hw, and their dL/dq into dq.
@@ -2070,7 +2114,13 @@ int bs_adjoint(bs_handle *h, int32_t nsteps, int32_t checkpoint_every, const flo
     rc = upload_ext(h, ext_force3);
     if (rc)
         return rc;
-    const int64_t n = h->n, E = h->nedges;
+    if (dk || dl0)
+    {
+        rc = build_spring_owners(h);
+        if (rc)
+            return rc;
+    }
+    const int64_t n = h->n, E = h->nedges, m = (int64_t)h->s1.size();
     const int32_t T = nsteps;
     int32_t K = checkpoint_every > 0 ? checkpoint_every : (int32_t)std::ceil(std::sqrt((double)std::max(T, 1)));
     K = std::max(1, std::min(K, std::max(T, 1)));
@@ -2105,15 +2155,25 @@ int bs_adjoint(bs_handle *h, int32_t nsteps, int32_t checkpoint_every, const flo
         SALLOC(hwp, double4, n);
     SALLOC(dke, double, E);
     SALLOC(dle, double, E);
+    double *dks = nullptr, *dls = nullptr;
+    if (dk || dl0)
+    {
+        SALLOC(dks, double, m);
+        SALLOC(dls, double, m);
+    }
     SALLOC(dqd, double, n);
     const int32_t lblocks = (int32_t)((n + BLOCK - 1) / BLOCK);
     SALLOC(lpart, double, lblocks);
 #undef SALLOC
 
     // forward: T steps, state saved (and the lists rebuilt) every K steps so
-    // that recomputing a segment repeats it bit for bit
+    // that recomputing a segment repeats it bit for bit; the last segment's
+    // positions are kept as they pass and that segment is not recomputed
+    const int32_t s_last = (ncp - 1) * K;
     for (int32_t s = 0; s < T; ++s)
     {
+        if (s >= s_last)
+            HIPCHK(h, hipMemcpyAsync(seg + (size_t)(s - s_last) * n, h->pos[h->cur], s4, hipMemcpyDeviceToDevice, st));
         if (s % K == 0)
         {
             const size_t c = (size_t)(s / K);
@@ -2161,12 +2221,15 @@ int bs_adjoint(bs_handle *h, int32_t nsteps, int32_t checkpoint_every, const flo
     for (int32_t c = ncp - 1; c >= 0; --c)
     {
         const int32_t s0 = c * K, s1 = std::min(T, s0 + K);
-        HIPCHK(h, hipMemcpyAsync(h->pos[h->cur], cp_pos + (size_t)c * n, s4, hipMemcpyDeviceToDevice, st));
-        HIPCHK(h, hipMemcpyAsync(h->vel, cp_vel + (size_t)c * n, s4, hipMemcpyDeviceToDevice, st));
-        HIPCHK(h, hipMemcpyAsync(h->counters, cp_cnt + 4 * (size_t)c, 4 * sizeof(int32_t), hipMemcpyDeviceToDevice,
-                                 st));
-        launch_fill_i32(h->status, 1, 1, nullptr, st);
-        for (int32_t s = s0; s < s1; ++s)
+        if (c < ncp - 1)
+        {
+            HIPCHK(h, hipMemcpyAsync(h->pos[h->cur], cp_pos + (size_t)c * n, s4, hipMemcpyDeviceToDevice, st));
+            HIPCHK(h, hipMemcpyAsync(h->vel, cp_vel + (size_t)c * n, s4, hipMemcpyDeviceToDevice, st));
+            HIPCHK(h, hipMemcpyAsync(h->counters, cp_cnt + 4 * (size_t)c, 4 * sizeof(int32_t),
+                                     hipMemcpyDeviceToDevice, st));
+            launch_fill_i32(h->status, 1, 1, nullptr, st);
+        }
+        for (int32_t s = s0; s < s1 && c < ncp - 1; ++s)
         {
             HIPCHK(h, hipMemcpyAsync(seg + (size_t)(s - s0) * n, h->pos[h->cur], s4, hipMemcpyDeviceToDevice, st));
             if (s + 1 < s1)
@@ -2229,26 +2292,18 @@ int bs_adjoint(bs_handle *h, int32_t nsteps, int32_t checkpoint_every, const flo
     HIPCHK(h, hipMemcpyAsync(h->counters, fin_cnt, 4 * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
     launch_fill_i32(h->status, 1, 1, nullptr, st);
 
-    std::vector<double> lp((size_t)lblocks), dqh((size_t)n), dkh, dlh;
+    if (dk || dl0)
+        launch_spring_grads(h->sp_own, dke, dle, m, dks, dls, st);
+    std::vector<double> lp((size_t)lblocks), dqh((size_t)n);
     int32_t stv[4];
     HIPCHK(h, hipMemcpyAsync(lp.data(), lpart, sizeof(double) * lblocks, hipMemcpyDeviceToHost, st));
-    HIPCHK(h, hipMemcpyAsync(dqh.data(), dqd, sizeof(double) * n, hipMemcpyDeviceToHost, st));
     HIPCHK(h, hipMemcpyAsync(stv, h->status, sizeof stv, hipMemcpyDeviceToHost, st));
-    std::vector<int32_t> hp, hj;
-    if (dk || dl0)
-    {
-        dkh.resize((size_t)E);
-        dlh.resize((size_t)E);
-        hp.resize((size_t)n + 1);
-        hj.resize((size_t)E);
-        if (E > 0)
-        {
-            HIPCHK(h, hipMemcpyAsync(dkh.data(), dke, sizeof(double) * E, hipMemcpyDeviceToHost, st));
-            HIPCHK(h, hipMemcpyAsync(dlh.data(), dle, sizeof(double) * E, hipMemcpyDeviceToHost, st));
-            HIPCHK(h, hipMemcpyAsync(hj.data(), h->sp_j, sizeof(int32_t) * E, hipMemcpyDeviceToHost, st));
-        }
-        HIPCHK(h, hipMemcpyAsync(hp.data(), h->sp_ptr, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToHost, st));
-    }
+    if (dq)
+        HIPCHK(h, hipMemcpyAsync(dqh.data(), dqd, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    if (m > 0 && dk)
+        HIPCHK(h, hipMemcpyAsync(dk, dks, sizeof(double) * m, hipMemcpyDeviceToHost, st));
+    if (m > 0 && dl0)
+        HIPCHK(h, hipMemcpyAsync(dl0, dls, sizeof(double) * m, hipMemcpyDeviceToHost, st));
     HIPCHK(h, hipStreamSynchronize(st));
     prof_collect(h);
     if (stv[2] != 0)
@@ -2261,43 +2316,6 @@ int bs_adjoint(bs_handle *h, int32_t nsteps, int32_t checkpoint_every, const flo
     if (dq)
         for (int64_t t = 0; t < n; ++t)
             dq[h->perm[(size_t)t]] = dqh[(size_t)t];
-    if (dk || dl0)
-    {
-        // a spring's two directed edges: row a holds j = b, row b holds j = a
-        auto edge = [&](int32_t a, int32_t b) -> int64_t {
-            int32_t lo = hp[(size_t)a], hi = hp[(size_t)a + 1];
-            while (lo < hi)
-            {
-                const int32_t mid = (lo + hi) >> 1;
-                if ((hj[(size_t)mid] & 0x7fffffff) < b)
-                    lo = mid + 1;
-                else
-                    hi = mid;
-            }
-            return (lo < hp[(size_t)a + 1] && (hj[(size_t)lo] & 0x7fffffff) == b) ? lo : -1;
-        };
-        const int64_t m = (int64_t)h->s1.size();
-        for (int64_t s = 0; s < m; ++s)
-        {
-            const int32_t a = h->iperm[h->s1[(size_t)s]], b = h->iperm[h->s2[(size_t)s]];
-            const int64_t ea = edge(a, b), eb = edge(b, a);
-            double gk = 0.0, gl = 0.0;
-            if (ea >= 0)
-            {
-                gk += dkh[(size_t)ea];
-                gl += dlh[(size_t)ea];
-            }
-            if (eb >= 0)
-            {
-                gk += dkh[(size_t)eb];
-                gl += dlh[(size_t)eb];
-            }
-            if (dk)
-                dk[s] = gk;
-            if (dl0)
-                dl0[s] = gl;
-        }
-    }
     return BS_OK;
 }
 

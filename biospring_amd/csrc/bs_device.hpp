@@ -68,6 +68,16 @@ __device__ inline bool in_cutoff(float4 pi, float4 pj, float rc, float rc2lo, fl
     return exact_two_stage(pi, pj, rc);
 }
 
+// 1/sqrt(r2) to about 1e-14 relative: the FP32 hardware seed and one FP64
+// Newton step (the FP64 square root and divide are multi-instruction
+// sequences on CDNA).  For r2 > 0; below the smallest normal float the
+// seed is clamped (finite, inexact).
+__device__ inline double rsqrt_d(double r2)
+{
+    const double y = (double)__builtin_amdgcn_rsqf((float)fmax(r2, 1.1754944e-38));
+    return y * fma(-0.5 * r2, y * y, 1.5);
+}
+
 // Cell coordinate on the engine's binning grid: edge >= min_edge per axis and
 // the whole box covered by `dim` cells.
 __device__ inline int32_t internal_coord(float x, double mn, double len, int32_t dim, float min_edge)

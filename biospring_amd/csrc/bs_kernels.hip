@@ -772,6 +772,17 @@ __global__ void __launch_bounds__(BLOCK) k_adj_init(const float4 *pos, const flo
         loss_part[blockIdx.x] = t;
 }
 
+__global__ void k_spring_grads(const int32_t *own_e, const double *dke, const double *dle, int64_t m, double *dk,
+                               double *dl0)
+{
+    const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s >= m)
+        return;
+    const int32_t e = own_e[s];
+    dk[s] = e >= 0 ? dke[e] : 0.0;
+    dl0[s] = e >= 0 ? dle[e] : 0.0;
+}
+
 __global__ void k_drift_check(const float4 *pos, const float4 *xref, int32_t n, float half_skin2, int32_t *status)
 {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -820,7 +831,7 @@ __global__ void __launch_bounds__(BLOCK) k_adj_spring(AdjSpringArgs a)
             const double r2 = dx * dx + dy * dy + dz * dz;
             if (!(r2 > 0.0))
                 continue;
-            const double r = sqrt(r2), ir = 1.0 / r;
+            const double ir = rsqrt_d(r2), r = r2 * ir;
             const double phi = k * (r - l0) * C * ir, dphi = (k * C - phi) * ir;
             const double wx = wj.x - wi.x, wy = wj.y - wi.y, wz = wj.z - wi.z;
             const double dd = dx * wx + dy * wy + dz * wz;
@@ -984,6 +995,13 @@ void launch_adj_init(const float4 *pos, const float4 *target, const float4 *jprm
                      hipStream_t s)
 {
     k_adj_init<<<nblocks, BLOCK, 0, s>>>(pos, target, jprm, iprm, n, dt, lx, lv, w, loss_part);
+}
+
+void launch_spring_grads(const int32_t *own_e, const double *dke, const double *dle, int64_t m, double *dk,
+                         double *dl0, hipStream_t s)
+{
+    if (m > 0)
+        k_spring_grads<<<nblk(m), BLOCK, 0, s>>>(own_e, dke, dle, m, dk, dl0);
 }
 
 void launch_drift_check(const float4 *pos, const float4 *xref, int32_t n, float half_skin2, int32_t *status,

@@ -35,6 +35,36 @@ __device__ inline uint32_t bonded_mask(const int32_t *ex_ptr, const uint32_t *ex
     return (lo < end && (int32_t)(ex_w[lo] >> 8) == j) ? (ex_w[lo] & 0xFFu) : 0u;
 }
 
+// The cluster's spring exclusions as an open-addressing hash set in LDS
+// (one table per wave; key word j << 8 | mask, never 0, so 0 marks a free
+// slot).  A lookup costs about one LDS read where a binary search over the
+// exclusion list in global memory costs a chain of about ten dependent loads.
+constexpr int32_t EX_SLOTS = 2048;           // per wave, 8 KiB
+constexpr int32_t EX_HASH_MAX = EX_SLOTS / 2; // load factor <= 1/2
+
+__device__ inline uint32_t ex_hash(int32_t j) { return ((uint32_t)j * 0x9E3779B1u) >> 21; } // 11 bits
+
+__device__ inline void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ inline uint32_t bonded_mask_lds(const uint32_t *tab, int32_t j)
+{
+    uint32_t h = ex_hash(j);
+    for (;;)
+    {
+        const uint32_t v = tab[h];
+        if (v == 0u)
+            return 0u;
+        if ((int32_t)(v >> 8) == j)
+            return v & 0xFFu;
+        h = (h + 1u) & (EX_SLOTS - 1);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // List build: one wave per i-cluster, one candidate j per lane.  Candidates
 // come from the cells covering the cluster's bounding box grown by the list
@@ -44,10 +74,34 @@ __global__ void __launch_bounds__(BLOCK) k_cluster_list(ListArgs a)
 {
     if (gated_off(a.gate))
         return;
+    __shared__ uint32_t ex_tab[WAVES][EX_SLOTS];
     const int lane = threadIdx.x & 63;
     const int32_t ci = blockIdx.x * WAVES + (threadIdx.x >> 6);
     if (ci >= a.ncl)
         return; // whole wave
+    // exclusions of this cluster into the wave's hash table (small sets);
+    // larger sets are searched in global memory
+    uint32_t *tab = ex_tab[threadIdx.x >> 6];
+    bool ex_lds = false;
+    if (a.exclude_bonded)
+    {
+        const int32_t eb = a.ex_ptr[ci], ee = a.ex_ptr[ci + 1];
+        ex_lds = ee - eb <= EX_HASH_MAX;
+        if (ex_lds)
+        {
+            for (int32_t t = lane; t < EX_SLOTS; t += 64)
+                tab[t] = 0u;
+            wave_lds_sync();
+            for (int32_t t = eb + lane; t < ee; t += 64)
+            {
+                const uint32_t wv = a.ex_w[t];
+                uint32_t h = ex_hash((int32_t)(wv >> 8));
+                while (atomicCAS(&tab[h], 0u, wv) != 0u)
+                    h = (h + 1u) & (EX_SLOTS - 1);
+            }
+            wave_lds_sync();
+        }
+    }
     const int32_t il = lane & 7;
     const int32_t iself = ci * CLUSTER + il;
     float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -99,13 +153,43 @@ __global__ void __launch_bounds__(BLOCK) k_cluster_list(ListArgs a)
             clo[q] = internal_coord(bmin[q] - a.radius, mn, len, a.dims[q], a.cell_edge);
             chi[q] = internal_coord(bmax[q] + a.radius, mn, len, a.dims[q], a.cell_edge);
         }
+        // Rows of cells (fixed cy, cz) farther than the list radius from the
+        // cluster's box are skipped and each row's x range is cut to the
+        // radius left after the y and z gaps: about a third fewer candidates
+        // than the whole grown box.  The first and last cell of an axis are
+        // open-ended (internal_coord clamps), and the radius is padded beyond
+        // the candidate test's own r2list, so no candidate is lost.
+        double cmn[3], ce[3];
+        for (int q = 0; q < 3; ++q)
+        {
+            cmn[q] = a.box[q];
+            ce[q] = fmax((double)a.cell_edge, (a.box[3 + q] - a.box[q]) / (double)a.dims[q]);
+        }
+        const double rad = (double)a.radius * (1.0 + 1e-5) + 1e-6;
+        auto gap = [&](int q, int32_t c) {
+            const double lo = c == 0 ? -INFINITY : cmn[q] + c * ce[q];
+            const double hi = c == a.dims[q] - 1 ? INFINITY : cmn[q] + (c + 1) * ce[q];
+            return fmax(0.0, fmax(lo - (double)bmax[q], (double)bmin[q] - hi));
+        };
         for (int32_t cz = clo[2]; cz <= chi[2]; ++cz)
+        {
+            const double gz = gap(2, cz);
+            if (gz > rad)
+                continue;
             for (int32_t cy = clo[1]; cy <= chi[1]; ++cy)
             {
-                // cells cx = clo..chi are contiguous in the cell array
+                const double gy = gap(1, cy);
+                const double r2x = rad * rad - gz * gz - gy * gy;
+                if (r2x < 0.0)
+                    continue;
+                const double rx = sqrt(r2x);
+                const double mn0 = a.box[0], len0 = a.box[3] - a.box[0];
+                const int32_t cx0 = internal_coord((float)((double)bmin[0] - rx), mn0, len0, a.dims[0], a.cell_edge);
+                const int32_t cx1 = internal_coord((float)((double)bmax[0] + rx), mn0, len0, a.dims[0], a.cell_edge);
+                // cells cx = cx0..cx1 are contiguous in the cell array
                 const int32_t row = cy * a.dims[0] + cz * a.dims[0] * a.dims[1];
-                const int32_t s = a.cell_start[row + clo[0]];
-                const int32_t e = a.cell_start[row + chi[0] + 1];
+                const int32_t s = a.cell_start[row + cx0];
+                const int32_t e = a.cell_start[row + cx1 + 1];
                 for (int32_t t = s; t < e; t += 64)
                 {
                     const int32_t idx = t + lane;
@@ -135,7 +219,7 @@ __global__ void __launch_bounds__(BLOCK) k_cluster_list(ListArgs a)
                             }
                             m &= vmask;
                             if (m != 0u && a.exclude_bonded)
-                                m &= ~bonded_mask(a.ex_ptr, a.ex_w, ci, j);
+                                m &= ~(ex_lds ? bonded_mask_lds(tab, j) : bonded_mask(a.ex_ptr, a.ex_w, ci, j));
                         }
                     }
                     const uint64_t bal = __ballot(m != 0u);
@@ -152,6 +236,7 @@ __global__ void __launch_bounds__(BLOCK) k_cluster_list(ListArgs a)
                     }
                 }
             }
+        }
     }
     // zero words up to the next multiple of 16, read unconditionally by the
     // pair kernel (the capacity is a multiple of 16)
@@ -413,7 +498,7 @@ __device__ inline void adj_pair_term(bool masked, float4 pi, float4 ri, uint32_t
     const double r2 = dx * dx + dy * dy + dz * dz;
     on = on && r2 > 0.0;
     const double r2s = on ? r2 : 1.0;
-    const double r = sqrt(r2s), ir = 1.0 / r, ir2 = ir * ir;
+    const double ir = rsqrt_d(r2s), r = r2s * ir, ir2 = ir * ir;
     double fm = 0.0, dfm = 0.0;
     if (SMODE == 0)
     {

@@ -630,6 +630,25 @@ void launch_adj_spring(const AdjSpringArgs &a, int32_t group, int32_t nb, hipStr
     }
 }
 
+void launch_spring_grads(const int32_t *own_e, const double *dke, const double *dle, int64_t m, double *dk,
+                         double *dl0, hipStream_t)
+{
+    need(own_e, sizeof(int32_t) * m, "spring_grads: own_e");
+    need(dk, sizeof(double) * m, "spring_grads: dk");
+    need(dl0, sizeof(double) * m, "spring_grads: dl0");
+    for (int64_t s = 0; s < m; ++s)
+    {
+        const int32_t e = own_e[s];
+        if (e >= 0)
+        {
+            need(dke + e, sizeof(double), "spring_grads: dke");
+            need(dle + e, sizeof(double), "spring_grads: dle");
+        }
+        dk[s] = e >= 0 ? dke[e] : 0.0;
+        dl0[s] = e >= 0 ? dle[e] : 0.0;
+    }
+}
+
 void launch_adj_init(const float4 *pos, const float4 *target, const float4 *jprm, const float4 *iprm, int32_t n,
                      double, double4 *lx, double4 *lv, double4 *w, double *loss_part, int32_t nblocks, hipStream_t)
 {

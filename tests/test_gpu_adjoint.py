@@ -81,7 +81,12 @@ def test_adjoint_matches_fp64_oracle(name):
 
 def test_adjoint_stretched_start_matches_fp64_oracle():
     """3,000 beads started 0.3 A (rms) off their springs' rest positions, so
-    that the spring terms carry the motion from the first step."""
+    that the spring terms carry the motion from the first step.  With this
+    much motion, pairs cross the cutoffs during the run, and the gradient of a
+    frozen pair set depends on the step at which each crossing falls: the
+    oracle's own gradient moves by up to 1e-3 of its largest charge entry when
+    its start is perturbed by 1e-5 A, the size of the float trajectory's
+    error.  The bar is three times that envelope (and at least GRAD_RTOL)."""
     s = workloads.generate(1, seed=13, n=3000)
     p = s.params.copy()
     x0 = s.x.reshape(-1, 3)
@@ -94,11 +99,15 @@ def test_adjoint_stretched_start_matches_fp64_oracle():
     o = po.Oracle(s, params=p)
     o.set_state(xs, None)
     lo, dko, dlo, dqo = o.adjoint(T, y.astype(np.float64))
+    op = po.Oracle(s, params=p)
+    op.set_state_f64(xs.astype(np.float64) + np.random.default_rng(7).normal(0, 1e-5, xs.shape))
+    _, dkp, dlp, dqp = op.adjoint(T, y.astype(np.float64))
     errs = {"k": _rel(dk, dko), "l0": _rel(dl, dlo), "q": _rel(dq, dqo)}
-    _log(f"  stretched start: loss {loss:.9g} vs {lo:.9g}; gradient errors {errs}")
+    env = {"k": _rel(dkp, dko), "l0": _rel(dlp, dlo), "q": _rel(dqp, dqo)}
+    _log(f"  stretched start: loss {loss:.9g} vs {lo:.9g}; gradient errors {errs}; oracle envelope {env}")
     assert abs(loss - lo) <= 1e-5 * lo
     for g, e in errs.items():
-        assert e <= GRAD_RTOL, (g, e)
+        assert e <= max(GRAD_RTOL, 3 * env[g]), (g, e, env[g])
 
 
 def test_adjoint_checkpoint_interval_and_determinism():
