@@ -411,3 +411,32 @@ def test_adjoint_gradients_match_finite_differences():
         qm[i] -= h
         ref = (fd(q=qp) - fd(q=qm)) / (2 * h)
         assert abs(dq[i] - ref) <= 5e-4 * abs(ref) + 1e-14, (i, dq[i], ref)
+
+
+def test_adjoint_long_run_without_cutoff_crossings():
+    """100 steps with cutoffs beyond the box (no pair crosses a cutoff, so the
+    loss is smooth in the parameters): the adjoint equals the central
+    difference.  With 12 A cutoffs the loss jumps where pairs cross them and
+    differences over long runs pick the jumps up; the adjoint differentiates
+    the smooth part, the pair set of each step held fixed."""
+    s = workloads.generate(4, seed=3, n=300)
+    p = s.params.copy()
+    p.steric_cutoff = p.electrostatic_cutoff = 60.0
+    y = workloads.fitting_target(s, seed=3).astype(np.float64)
+    T, h = 100, 1e-3
+    loss, dk, dl, dq = po.Oracle(s, params=p).adjoint(T, y)
+
+    def fd(name, base, i):
+        lo = []
+        for sg in (1.0, -1.0):
+            a = base.astype(np.float64).copy()
+            a[i] += sg * h
+            o = po.Oracle(s, params=p)
+            o.set_arrays(**{name: a})
+            lo.append(o.loss(T, y))
+        return (lo[0] - lo[1]) / (2 * h)
+
+    for name, g, base in (("k", dk, s.k), ("l0", dl, s.l0), ("q", dq, s.charge)):
+        for i in np.argsort(-np.abs(g))[:2]:
+            ref = fd(name, base, int(i))
+            assert abs(g[i] - ref) <= 1e-3 * abs(ref), (name, int(i), g[i], ref)
