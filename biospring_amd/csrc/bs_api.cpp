@@ -33,6 +33,9 @@ struct List
     int32_t cap = 0;          // entries per i-cluster
     uint32_t *cl = nullptr;   // [ncl * cap] j | mask << 24
     int32_t *cln = nullptr;   // [ncl]
+    uint32_t *rows = nullptr; // row layout (BS_PAIR_ROWS): [row blocks * rcap * 64]
+    int32_t *rlen = nullptr;  // [row blocks * 64]
+    int32_t rcap = 0;
 };
 
 template <typename T> T *dalloc(size_t count, hipError_t &err)
@@ -226,6 +229,8 @@ void free_device(bs_handle *h)
     {
         f(l.cl);
         f(l.cln);
+        f(l.rows);
+        f(l.rlen);
     }
     h->lists.clear();
     h->vel = h->jprm = h->iprm = h->xref = h->fext = h->fout = nullptr;
@@ -265,7 +270,8 @@ int check_device(bs_handle *h)
     if (h->ncl != (int32_t)((h->n + CLUSTER - 1) / CLUSTER))
         return fail(h, BS_ERR_STATE, "internal: cluster count mismatch");
     for (const auto &l : h->lists)
-        if (!l.cl || !l.cln || l.cap <= 0 || l.ncells > h->cell_cap)
+        if (!l.cl || !l.cln || l.cap <= 0 || l.ncells > h->cell_cap ||
+            (h->p.pair_layout == BS_PAIR_ROWS && (!l.rows || !l.rlen || l.rcap < l.cap)))
             return fail(h, BS_ERR_STATE, "internal: neighbour list buffers missing");
     return BS_OK;
 }
@@ -567,6 +573,8 @@ void build_list(bs_handle *h, List &l, bool gated, int32_t cap_override)
     a.exclude_bonded = h->p.spring_enable ? 1 : 0;
     a.gate = g;
     launch_list_build(a, s);
+    if (l.rows && cap_override < 0)
+        launch_rows(l.cl, l.cln, h->ncl, l.cap, l.rows, l.rlen, l.rcap, g, s);
 }
 
 void rebuild_chain(bs_handle *h, bool gated)
@@ -600,6 +608,9 @@ void launch_pairs(bs_handle *h, bool energy)
         a.cln = l.cln;
         a.fnb = h->fnb;
         a.epp = h->epp;
+        a.rows = l.rows;
+        a.rlen = l.rlen;
+        a.rcap = l.rcap;
         a.n = n;
         a.ncl = h->ncl;
         a.cap = l.cap;
@@ -1268,6 +1279,13 @@ int do_setup(bs_handle *h)
         const int32_t mx = st[3];
         l.cap = (std::max<int32_t>(16, (int32_t)std::ceil(mx * 1.25) + 64) + 15) / 16 * 16;
         ALLOC(l.cl, uint32_t, (int64_t)h->ncl * l.cap);
+        if (h->p.pair_layout == BS_PAIR_ROWS)
+        {
+            const int64_t nrb = (n + 63) / 64;
+            l.rcap = l.cap; // a row is a subset of its cluster's list
+            ALLOC(l.rows, uint32_t, nrb * 64 * l.rcap);
+            ALLOC(l.rlen, int32_t, nrb * 64);
+        }
         launch_fill_i32(h->status + 2, 0, 1, nullptr, h->stream);
     }
     const int32_t st1[4] = {1, INT_MAX, 0, 0};

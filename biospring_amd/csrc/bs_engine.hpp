@@ -88,6 +88,9 @@ struct PairArgs
     const int32_t *cln;  // [ncl]
     double4 *fnb;       // non-bonded force, FP64
     double4 *epp;       // [N] per-particle steric, elec, hydro energy (energy steps)
+    const uint32_t *rows; // nullable: per-particle lists (row layout), else the cluster lists
+    const int32_t *rlen;  // [64 * row blocks] entries per particle
+    int32_t rcap;         // entries per particle row
     int32_t n, ncl, cap;
     int32_t accumulate; // add to fnb instead of overwriting
     int32_t energy;     // also write epp (the last step of a call only)
@@ -255,6 +258,12 @@ void launch_scatter(const int32_t *cell_of, int32_t n, const int32_t *start, int
 void launch_cell_sort(const int32_t *cell_of, int32_t n, const int32_t *start, const int32_t *tmp_idx,
                       const int32_t *key, int32_t *out_idx, const int32_t *gate, hipStream_t s);
 void launch_list_build(const ListArgs &a, hipStream_t s);
+// Row layout: the cluster lists expanded into one list per particle.  Blocks
+// of 64 consecutive particles; entry t of particle i is
+// rows[((i / 64) * rcap + t) * 64 + i % 64], in the order of the cluster list;
+// rlen[i] entries (rcap >= cap).
+void launch_rows(const uint32_t *cl, const int32_t *cln, int32_t ncl, int32_t cap, uint32_t *rows, int32_t *rlen,
+                 int32_t rcap, const int32_t *gate, hipStream_t s);
 void launch_pair(const PairArgs &a, int32_t elec_mode, int32_t hydro, int32_t nblocks, hipStream_t s);
 void launch_spring(const SpringArgs &a, int32_t group, int32_t nblocks, hipStream_t s);
 // Per-system energies from the per-particle ones: segments of contiguous

@@ -693,12 +693,164 @@ void launch_list_build(const ListArgs &a, hipStream_t s)
         k_cluster_list<<<cluster_blocks(a.ncl), BLOCK, 0, s>>>(a);
 }
 
+// ---------------------------------------------------------------------------
+// Row layout.  With springs excluded from the pair terms, the pairs a
+// particle sees lie in a shell (workload 1: 8 A to the 13 A list radius)
+// thinner than twice a cluster's size, so an entry of a cluster list is seen
+// by about 60% of the cluster's particles and the cluster kernel leaves the
+// other lanes idle.  The rows hold each particle's own entries, so every lane
+// evaluates a listed pair; the price is one gather of j's data per pair
+// instead of one per eight.
+//
+// k_rows: one wave per cluster; 64 list words at a time, one ballot per
+// cluster particle compacts the words it sees into its row, in the cluster
+// list's order.  The row kernel sums a particle's terms in that order; the
+// cluster kernel sums eight lane partials and then combines them, so the two
+// layouts agree to FP64 rounding, not bit for bit.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(BLOCK) k_rows(const uint32_t *cl, const int32_t *cln, int32_t ncl, int32_t cap,
+                                                uint32_t *rows, int32_t *rlen, int32_t rcap, const int32_t *gate)
+{
+    if (gated_off(gate))
+        return;
+    const int lane = threadIdx.x & 63;
+    const int32_t ci = blockIdx.x * WAVES + (threadIdx.x >> 6);
+    if (ci >= ncl)
+        return; // whole wave
+    const int32_t cnt = cln[ci];
+    const uint32_t *L = cl + (size_t)ci * cap;
+    const uint64_t below = (1ull << lane) - 1ull;
+    int32_t c[CLUSTER];
+#pragma unroll
+    for (int k = 0; k < CLUSTER; ++k)
+        c[k] = 0;
+    for (int32_t t0 = 0; t0 < cnt; t0 += 64)
+    {
+        const uint32_t w = t0 + lane < cnt ? L[t0 + lane] : 0u;
+        const uint32_t j = w & CL_JMASK;
+#pragma unroll
+        for (int k = 0; k < CLUSTER; ++k)
+        {
+            const bool on = (w >> (24 + k)) & 1u;
+            const uint64_t bal = __ballot(on);
+            if (on)
+            {
+                const int32_t i = ci * CLUSTER + k;
+                const int32_t t = c[k] + __popcll(bal & below);
+                rows[((size_t)(i >> 6) * rcap + t) * 64 + (i & 63)] = j;
+            }
+            c[k] += __popcll(bal);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < CLUSTER; ++k)
+        if (lane == k)
+            rlen[ci * CLUSTER + k] = c[k];
+}
+
+// Pair forces of 64 consecutive particles per wave, one particle per lane,
+// two row entries per lane per iteration; row words two iterations ahead and
+// j data one iteration ahead are in flight.  Entries at or past a row's end
+// (never written) are replaced by the particle itself and masked off.
+template <int SMODE, int EMODE, int HYDRO, bool ENERGY>
+__global__ void __launch_bounds__(BLOCK) k_pair_rows(PairArgs a)
+{
+    const int lane = threadIdx.x & 63;
+    const int32_t b = blockIdx.x * WAVES + (threadIdx.x >> 6);
+    const int32_t nrb = (a.n + 63) >> 6;
+    if (b >= nrb)
+        return; // whole wave
+    const int32_t i = b * 64 + lane;
+    const bool iv = i < a.n;
+    const int32_t self = iv ? i : 0;
+    const float4 pi = a.pos[self];
+    const float4 ri = a.jprm[self];
+    const uint32_t fli = fbits(ri.w);
+    const double qe = (double)pi.w * a.ke;
+    const double qf = (double)pi.w * a.kf;
+    const float cs = (float)a.k.c_spring;
+    const int32_t len = iv ? a.rlen[i] : 0;
+    int32_t mx = len;
+    for (int off = 32; off > 0; off >>= 1)
+        mx = max(mx, __shfl_xor(mx, off, 64));
+    double es = 0.0, ee = 0.0, eh = 0.0;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    if (mx > 0)
+    {
+        const uint32_t *R = a.rows + (size_t)b * a.rcap * 64 + lane;
+        const int32_t last = mx - 1;
+        auto word = [&](int32_t e) { return R[(size_t)min(e, last) * 64]; };
+        auto jof = [&](uint32_t w, int32_t e) { return e < len ? (int32_t)w : self; };
+        uint32_t w0 = word(0), w1 = word(1), n0 = word(2), n1 = word(3);
+        int32_t k0 = jof(w0, 0), k1 = jof(w1, 1);
+        float4 pj0 = a.pos[k0], rj0 = a.jprm[k0], pj1 = a.pos[k1], rj1 = a.jprm[k1];
+        for (int32_t e = 0; e < mx; e += 2)
+        {
+            const uint32_t m0 = word(e + 4), m1 = word(e + 5);
+            const int32_t q0 = jof(n0, e + 2), q1 = jof(n1, e + 3);
+            const float4 qj0 = a.pos[q0], sj0 = a.jprm[q0];
+            const float4 qj1 = a.pos[q1], sj1 = a.jprm[q1];
+            interact<SMODE, EMODE, HYDRO, ENERGY>(e < len, pi, ri, fli, pj0, rj0, a, cs, qe, qf, fx, fy, fz, es, ee,
+                                                  eh);
+            interact<SMODE, EMODE, HYDRO, ENERGY>(e + 1 < len, pi, ri, fli, pj1, rj1, a, cs, qe, qf, fx, fy, fz, es,
+                                                  ee, eh);
+            n0 = m0;
+            n1 = m1;
+            pj0 = qj0;
+            rj0 = sj0;
+            pj1 = qj1;
+            rj1 = sj1;
+        }
+        (void)w0;
+        (void)w1;
+    }
+    if (!iv)
+        return;
+    if (ENERGY)
+    {
+        double4 en = make_double4(es, ee, eh, 0.0);
+        if (a.accumulate)
+        {
+            const double4 o = a.epp[i];
+            en.x += o.x;
+            en.y += o.y;
+            en.z += o.z;
+        }
+        a.epp[i] = en;
+    }
+    double4 f = make_double4(fx, fy, fz, 0.0);
+    if (a.accumulate)
+    {
+        const double4 o = a.fnb[i];
+        f.x += o.x;
+        f.y += o.y;
+        f.z += o.z;
+    }
+    a.fnb[i] = f;
+}
+
 template <int S, int E, int H> static void pair3(const PairArgs &a, int32_t nb, hipStream_t s)
 {
+    if (a.rows)
+    {
+        const int32_t nrb = (a.n + 63) / 64, blocks = (nrb + WAVES - 1) / WAVES;
+        if (a.energy)
+            k_pair_rows<S, E, H, true><<<blocks, BLOCK, 0, s>>>(a);
+        else
+            k_pair_rows<S, E, H, false><<<blocks, BLOCK, 0, s>>>(a);
+        return;
+    }
     if (a.energy)
         k_pair<S, E, H, true><<<nb, BLOCK, 0, s>>>(a);
     else
         k_pair<S, E, H, false><<<nb, BLOCK, 0, s>>>(a);
+}
+
+void launch_rows(const uint32_t *cl, const int32_t *cln, int32_t ncl, int32_t cap, uint32_t *rows, int32_t *rlen,
+                 int32_t rcap, const int32_t *gate, hipStream_t s)
+{
+    if (ncl > 0)
+        k_rows<<<(ncl + WAVES - 1) / WAVES, BLOCK, 0, s>>>(cl, cln, ncl, cap, rows, rlen, rcap, gate);
 }
 
 template <int S> static void pair2(const PairArgs &a, int32_t em, int32_t h, int32_t nb, hipStream_t s)

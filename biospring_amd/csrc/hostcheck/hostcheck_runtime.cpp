@@ -461,8 +461,49 @@ void launch_list_build(const ListArgs &a, hipStream_t)
     }
 }
 
+void launch_rows(const uint32_t *cl, const int32_t *cln, int32_t ncl, int32_t cap, uint32_t *rows, int32_t *rlen,
+                 int32_t rcap, const int32_t *gate, hipStream_t)
+{
+    if (gated_off(gate))
+        return;
+    need(cln, sizeof(int32_t) * ncl, "rows: cln");
+    need(cl, sizeof(uint32_t) * (size_t)ncl * cap, "rows: cl");
+    const int64_t nrb = ((int64_t)ncl * CLUSTER + 63) / 64;
+    need(rlen, sizeof(int32_t) * nrb * 64, "rows: rlen");
+    need(rows, sizeof(uint32_t) * (size_t)nrb * rcap * 64, "rows: rows");
+    if (rcap < cap)
+        hc_fail("rows: row capacity below the cluster capacity");
+    for (int32_t ci = 0; ci < ncl; ++ci)
+        for (int k = 0; k < CLUSTER; ++k)
+        {
+            const int32_t i = ci * CLUSTER + k;
+            int32_t c = 0;
+            for (int32_t t = 0; t < cln[ci]; ++t)
+            {
+                const uint32_t w = cl[(size_t)ci * cap + t];
+                if ((w >> (24 + k)) & 1u)
+                    rows[((size_t)(i >> 6) * rcap + c++) * 64 + (i & 63)] = w & CL_JMASK;
+            }
+            rlen[i] = c;
+        }
+}
+
 void launch_pair(const PairArgs &a, int32_t, int32_t, int32_t nb, hipStream_t)
 {
+    if (a.rows)
+    {
+        const int64_t nrb = ((int64_t)a.n + 63) / 64;
+        need(a.rlen, sizeof(int32_t) * nrb * 64, "pair rows: rlen");
+        need(a.rows, sizeof(uint32_t) * (size_t)nrb * a.rcap * 64, "pair rows: rows");
+        for (int32_t i = 0; i < a.n; ++i)
+        {
+            if (a.rlen[i] < 0 || a.rlen[i] > a.rcap)
+                hc_fail("pair rows: row length out of range");
+            for (int32_t t = 0; t < a.rlen[i]; ++t)
+                if ((int32_t)a.rows[((size_t)(i >> 6) * a.rcap + t) * 64 + (i & 63)] >= a.n)
+                    hc_fail("pair rows: neighbour index out of range");
+        }
+    }
     need(a.pos, sizeof(float4) * a.n, "pair: pos");
     need(a.jprm, sizeof(float4) * a.n, "pair: jprm");
     need(a.cln, sizeof(int32_t) * a.ncl, "pair: cln");

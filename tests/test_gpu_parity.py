@@ -175,6 +175,28 @@ def test_determinism():
     assert np.array_equal(outs[0][2], outs[1][2])
 
 
+@pytest.mark.parametrize("name", ["enm2k", "lj20k"])
+def test_pair_layouts_agree(name):
+    """Row lists (the default) and cluster lists give the same forces to FP64
+    rounding (the sums associate differently) and both meet the FP64 bar."""
+    s = _system(name)
+    res = {}
+    for layout in (_abi.PAIR_ROWS, _abi.PAIR_CLUSTERS):
+        p = s.params.copy()
+        p.pair_layout = layout
+        p.viscosity_enable = 0
+        net = SpringNetwork.from_system(s, params=p)
+        net.run(5)
+        res[layout] = net.computeForces()
+    Fr, Er = res[_abi.PAIR_ROWS]
+    Fc, Ec = res[_abi.PAIR_CLUSTERS]
+    scale = np.abs(Fc).max()
+    _log(f"  {name}: rows vs clusters max |dF| / max|F| = {np.abs(Fr - Fc).max() / scale:.2e}")
+    assert np.abs(Fr - Fc).max() <= 1e-6 * scale
+    for slot in range(6):
+        assert abs(Er[slot] - Ec[slot]) <= 1e-12 * max(abs(Ec[slot]), 1e-3)
+
+
 def test_nonfinite_reported():
     s = _system("enm2k")
     net = SpringNetwork.from_system(s)
