@@ -922,7 +922,7 @@ int build_spring_owners(bs_handle *h)
 
 // H w of the pair terms at the positions in pos[cur] (lists valid there),
 // into hw, and their dL/dq into dq.
-void launch_adj_pairs(bs_handle *h, const double4 *w, double4 *hw, double *dq)
+void launch_adj_pairs(bs_handle *h, const float4 *w, double4 *hw, double *dq)
 {
     for (size_t li = 0; li < h->lists.size(); ++li)
     {
@@ -2149,7 +2149,8 @@ int bs_adjoint(bs_handle *h, int32_t nsteps, int32_t checkpoint_every, const flo
     Scratch S;
     float4 *cp_pos, *cp_vel, *seg, *tgt, *fin_pos, *fin_vel;
     int32_t *cp_cnt, *fin_cnt;
-    double4 *lx, *lv, *wb[2], *hwp = nullptr;
+    double4 *lx, *lv, *hwp = nullptr;
+    float4 *wb[2];
     double *dke, *dle, *dqd, *lpart;
 #define SALLOC(ptr, T_, cnt)                                                                                     \
     do                                                                                                            \
@@ -2167,8 +2168,8 @@ int bs_adjoint(bs_handle *h, int32_t nsteps, int32_t checkpoint_every, const flo
     SALLOC(fin_cnt, int32_t, 4);
     SALLOC(lx, double4, n);
     SALLOC(lv, double4, n);
-    SALLOC(wb[0], double4, n);
-    SALLOC(wb[1], double4, n);
+    SALLOC(wb[0], float4, n);
+    SALLOC(wb[1], float4, n);
     if (!h->lists.empty())
         SALLOC(hwp, double4, n);
     SALLOC(dke, double, E);

@@ -180,14 +180,14 @@ struct SpringArgs
 };
 
 // Adjoint of the pair terms at one state: H w on i (each pair seen from both
-// sides) and dL/dq.  Positions FP32, everything else FP64.
+// sides) and dL/dq.  Terms in FP32, sums in FP64.
 struct AdjPairArgs
 {
     const float4 *pos;
     const float4 *jprm;
     const uint32_t *cl;
     const int32_t *cln;
-    const double4 *w;     // [N] adjoint weights dt/m a (0 for static particles)
+    const float4 *w;      // [N] adjoint weights dt/m a, FP32 (0 for static particles)
     double4 *hw;          // [N] H w of this list (overwritten, or added with accumulate)
     double *dq;           // [N] dL/dq accumulator
     int32_t n, ncl, cap, accumulate;
@@ -209,10 +209,10 @@ struct AdjSpringArgs
     const int32_t *sp_j;
     const float *sp_k;
     const float *sp_l0;
-    const double4 *w;     // [N] weights of this step
+    const float4 *w;      // [N] weights of this step (FP32)
     const double4 *hwp;   // nullable: pair part of H w
     double4 *lx, *lv;     // [N] adjoint of x and v (updated in place)
-    double4 *w_next;      // [N] weights of the step before
+    float4 *w_next;       // [N] weights of the step before
     double *dk, *dl0;     // [E] gradients, on the owning edge of each spring
     int32_t n, spring_enable;
     float spring_scale;
@@ -282,7 +282,7 @@ void launch_adj_spring(const AdjSpringArgs &a, int32_t group, int32_t nblocks, h
 // lx = 2 (x - y) / n for dynamic particles, lv = 0, w = dt/m (dt lx); loss
 // partials |x - y|^2 per block (nblocks of BLOCK).
 void launch_adj_init(const float4 *pos, const float4 *target, const float4 *jprm, const float4 *iprm, int32_t n,
-                     double dt, double4 *lx, double4 *lv, double4 *w, double *loss_part, int32_t nblocks,
+                     double dt, double4 *lx, double4 *lv, float4 *w, double *loss_part, int32_t nblocks,
                      hipStream_t s);
 // Per-spring gradients from the per-edge accumulators: dk[s] = dke[own_e[s]]
 // (0 where own_e[s] < 0), likewise dl0.

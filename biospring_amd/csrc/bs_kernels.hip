@@ -748,7 +748,7 @@ __global__ void __launch_bounds__(BLOCK) k_sys_energies(const double *seg_part, 
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(BLOCK) k_adj_init(const float4 *pos, const float4 *target, const float4 *jprm,
                                                      const float4 *iprm, int32_t n, double dt, double4 *lx,
-                                                     double4 *lv, double4 *w, double *loss_part)
+                                                     double4 *lv, float4 *w, double *loss_part)
 {
     __shared__ double sh[BLOCK / 64];
     const int32_t i = blockIdx.x * BLOCK + threadIdx.x;
@@ -765,7 +765,7 @@ __global__ void __launch_bounds__(BLOCK) k_adj_init(const float4 *pos, const flo
         const double s = (dyn && m > 0.0) ? dt / m * dt : 0.0;
         lx[i] = l;
         lv[i] = make_double4(0.0, 0.0, 0.0, 0.0);
-        w[i] = make_double4(s * l.x, s * l.y, s * l.z, 0.0);
+        w[i] = make_float4((float)(s * l.x), (float)(s * l.y), (float)(s * l.z), 0.0f);
     }
     const double t = block_sum(e2, sh);
     if (threadIdx.x == 0)
@@ -807,12 +807,13 @@ __global__ void __launch_bounds__(BLOCK) k_adj_spring(AdjSpringArgs a)
     double hx = 0.0, hy = 0.0, hz = 0.0;
     float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
     uint32_t fli = PF_STATIC;
-    double4 wi = make_double4(0.0, 0.0, 0.0, 0.0);
+    double4 wi = make_double4(0.0, 0.0, 0.0, 0.0); // own weights, widened
     if (active)
     {
         pi = a.pos[row];
         fli = fbits(a.jprm[row].w);
-        wi = a.w[row];
+        const float4 wf = a.w[row];
+        wi = make_double4(wf.x, wf.y, wf.z, 0.0);
     }
     const bool dyn = active && !(fli & PF_STATIC);
     if (dyn && a.spring_enable)
@@ -825,7 +826,7 @@ __global__ void __launch_bounds__(BLOCK) k_adj_spring(AdjSpringArgs a)
             const int32_t j = jr & 0x7fffffff;
             const double k = a.sp_k[e], l0 = a.sp_l0[e];
             const float4 pj = a.pos[j];
-            const double4 wj = a.w[j];
+            const float4 wj = a.w[j];
             const double dx = (double)pj.x - (double)pi.x, dy = (double)pj.y - (double)pi.y,
                          dz = (double)pj.z - (double)pi.z;
             const double r2 = dx * dx + dy * dy + dz * dz;
@@ -833,7 +834,7 @@ __global__ void __launch_bounds__(BLOCK) k_adj_spring(AdjSpringArgs a)
                 continue;
             const double ir = rsqrt_d(r2), r = r2 * ir;
             const double phi = k * (r - l0) * C * ir, dphi = (k * C - phi) * ir;
-            const double wx = wj.x - wi.x, wy = wj.y - wi.y, wz = wj.z - wi.z;
+            const double wx = (double)wj.x - wi.x, wy = (double)wj.y - wi.y, wz = (double)wj.z - wi.z;
             const double dd = dx * wx + dy * wy + dz * wz;
             const double g = dphi * ir * dd;
             hx = fma(phi, wx, fma(g, dx, hx));
@@ -857,7 +858,7 @@ __global__ void __launch_bounds__(BLOCK) k_adj_spring(AdjSpringArgs a)
         return;
     if (!dyn)
     {
-        a.w_next[row] = make_double4(0.0, 0.0, 0.0, 0.0);
+        a.w_next[row] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         return;
     }
     if (a.hwp)
@@ -876,8 +877,8 @@ __global__ void __launch_bounds__(BLOCK) k_adj_spring(AdjSpringArgs a)
     const double s = m > 0.0 ? a.dt / m : 0.0;
     a.lx[row] = nx;
     a.lv[row] = nv;
-    a.w_next[row] = make_double4(s * fma(a.dt, nx.x, nv.x), s * fma(a.dt, nx.y, nv.y), s * fma(a.dt, nx.z, nv.z),
-                                 0.0);
+    a.w_next[row] = make_float4((float)(s * fma(a.dt, nx.x, nv.x)), (float)(s * fma(a.dt, nx.y, nv.y)),
+                                (float)(s * fma(a.dt, nx.z, nv.z)), 0.0f);
 }
 
 __global__ void k_prestep(int32_t *status, int32_t *step, int32_t interval)
@@ -991,7 +992,7 @@ void launch_adj_spring(const AdjSpringArgs &a, int32_t group, int32_t nb, hipStr
 }
 
 void launch_adj_init(const float4 *pos, const float4 *target, const float4 *jprm, const float4 *iprm, int32_t n,
-                     double dt, double4 *lx, double4 *lv, double4 *w, double *loss_part, int32_t nblocks,
+                     double dt, double4 *lx, double4 *lv, float4 *w, double *loss_part, int32_t nblocks,
                      hipStream_t s)
 {
     k_adj_init<<<nblocks, BLOCK, 0, s>>>(pos, target, jprm, iprm, n, dt, lx, lv, w, loss_part);

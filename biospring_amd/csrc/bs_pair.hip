@@ -481,84 +481,86 @@ __global__ void __launch_bounds__(BLOCK) k_pair(PairArgs a)
 // each term (energy/steric.hpp, electrostatic.hpp) and its r-derivative fm'.
 // dL/dq_i gathers q_j (kf / r^3) (d . dw) (constant eps; 2 kf / r^4 for
 // eps(r) = eps r); a static partner's share goes to it by an FP64 atomic.
-// Lanes, lists and the exact cutoff test are those of k_pair.
+// Lanes, lists, the exact cutoff test and the load pipeline are those of
+// k_pair.  The terms are formed in FP32 (the weights w are FP32; a gradient
+// needs about 1e-5, not the 1e-5-after-cancellation of a force) and summed
+// in FP64.
 // ---------------------------------------------------------------------------
 template <int SMODE, int EMODE>
-__device__ inline void adj_pair_term(bool masked, float4 pi, float4 ri, uint32_t fli, double4 wi, float4 pj,
-                                     float4 rj, double4 wj, int32_t j, const AdjPairArgs &a, double cs, double &hx,
-                                     double &hy, double &hz, double &dqi)
+__device__ inline void adj_pair_term(bool masked, float4 pi, float4 ri, uint32_t fli, float4 wi, float4 pj,
+                                     float4 rj, float4 wj, int32_t j, const AdjPairArgs &a, float cs, float kf,
+                                     double &hx, double &hy, double &hz, double &dqi)
 {
-    const float fx = pj.x - pi.x, fy = pj.y - pi.y, fz = pj.z - pi.z;
-    const float r2f = fx * fx + fy * fy + fz * fz;
+    const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+    const float r2f = dx * dx + dy * dy + dz * dz;
     bool on = masked && r2f < a.rc2lo;
     if (masked && r2f >= a.rc2lo && r2f < a.rc2hi)
         on = exact_two_stage(pi, pj, a.rc);
     const uint32_t flj = fbits(rj.w);
-    const double dx = (double)pj.x - (double)pi.x, dy = (double)pj.y - (double)pi.y, dz = (double)pj.z - (double)pi.z;
-    const double r2 = dx * dx + dy * dy + dz * dz;
-    on = on && r2 > 0.0;
-    const double r2s = on ? r2 : 1.0;
-    const double ir = rsqrt_d(r2s), r = r2s * ir, ir2 = ir * ir;
-    double fm = 0.0, dfm = 0.0;
+    on = on && r2f > 0.0f;
+    const float r2s = on ? r2f : 1.0f;
+    float ir = __builtin_amdgcn_rsqf(r2s);
+    ir = ir * (1.5f - 0.5f * r2s * ir * ir);
+    const float r = r2s * ir, ir2 = ir * ir;
+    float fm = 0.0f, dfm = 0.0f;
     if (SMODE == 0)
     {
-        const double sr = (double)ri.x + (double)rj.x;
-        if (r <= sr)
-        {
-            fm = (r - sr) * cs;
-            dfm = cs;
-        }
+        const float sr = ri.x + rj.x;
+        const bool ov = r <= sr;
+        fm = ov ? (r - sr) * cs : 0.0f;
+        dfm = ov ? cs : 0.0f;
     }
     else if (SMODE > 0)
     {
-        if (r2f >= 0.01f)
+        const float eij = ri.y * rj.y, R = ri.x * rj.x;
+        const float s2 = R * R * ir2, s6 = s2 * s2 * s2, s8 = s6 * s2;
+        float f0, d0;
+        if (SMODE == 1)
         {
-            const double eij = (double)ri.y * (double)rj.y, R = (double)ri.x * (double)rj.x;
-            const double s2 = R * R * ir2, s6 = s2 * s2 * s2, s8 = s6 * s2, s12 = s6 * s6;
-            if (SMODE == 1)
-            {
-                fm = 12.0 * eij * (s6 - s12) * ir * cs;
-                dfm = 12.0 * eij * (-7.0 * s6 + 13.0 * s12) * ir2 * cs;
-            }
-            else if (SMODE == 2)
-            {
-                fm = 24.0 * eij * (s6 - s8) * ir * cs;
-                dfm = 24.0 * eij * (-7.0 * s6 + 9.0 * s8) * ir2 * cs;
-            }
-            else
-            {
-                fm = eij * (6.0 * s6 - 8.0 * s8) * ir * cs;
-                dfm = eij * (-42.0 * s6 + 72.0 * s8) * ir2 * cs;
-            }
+            const float s12 = s6 * s6;
+            f0 = 12.0f * eij * (s6 - s12) * ir;
+            d0 = 12.0f * eij * (-7.0f * s6 + 13.0f * s12) * ir2;
         }
+        else if (SMODE == 2)
+        {
+            f0 = 24.0f * eij * (s6 - s8) * ir;
+            d0 = 24.0f * eij * (-7.0f * s6 + 9.0f * s8) * ir2;
+        }
+        else
+        {
+            f0 = eij * (6.0f * s6 - 8.0f * s8) * ir;
+            d0 = eij * (-42.0f * s6 + 72.0f * s8) * ir2;
+        }
+        const bool far = r2f >= 0.01f;
+        fm = far ? f0 * cs : 0.0f;
+        dfm = far ? d0 * cs : 0.0f;
     }
-    fm *= (double)a.steric_scale;
-    dfm *= (double)a.steric_scale;
-    const double wx = wj.x - wi.x, wy = wj.y - wi.y, wz = wj.z - wi.z;
-    const double dd = dx * wx + dy * wy + dz * wz;
+    fm *= a.steric_scale;
+    dfm *= a.steric_scale;
+    const float wx = wj.x - wi.x, wy = wj.y - wi.y, wz = wj.z - wi.z;
+    const float dd = dx * wx + dy * wy + dz * wz;
     if (EMODE > 0)
     {
         const bool qon = (fli & PF_CHARGED) && (flj & PF_CHARGED) && r2f >= 1.0e-6f;
-        const double qq = qon ? (double)pi.w * (double)pj.w : 0.0;
-        const double kq = a.kf * qq;
-        const double ir3 = ir2 * ir;
+        const float qq = qon ? pi.w * pj.w : 0.0f;
+        const float ir3 = ir2 * ir;
         // per unit q_i q_j, the Coulomb part of phi is -c (c = kf/r^3 or 2 kf/r^4)
-        const double c = EMODE == 1 ? a.kf * ir3 : 2.0 * a.kf * ir3 * ir;
-        fm += EMODE == 1 ? -kq * ir2 : -2.0 * kq * ir3;
-        dfm += EMODE == 1 ? 2.0 * kq * ir3 : 6.0 * kq * ir3 * ir;
+        const float c = EMODE == 1 ? kf * ir3 : 2.0f * kf * ir3 * ir;
+        fm -= c * qq * r;                                   // phi = -c qq
+        dfm += EMODE == 1 ? 2.0f * c * qq : 3.0f * c * qq;  // fm' = 2 kf qq / r^3 or 6 kf qq / r^4
         if (on && qon)
         {
-            dqi += c * dd * (double)pj.w;
+            dqi += (double)(c * dd) * (double)pj.w;
             if (flj & PF_STATIC)
-                atomicAdd(&a.dq[j], c * dd * (double)pi.w);
+                atomicAdd(&a.dq[j], (double)(c * dd) * (double)pi.w);
         }
     }
-    const double phi = fm * ir, dphi = (dfm - phi) * ir;
-    const double g = on ? dphi * ir * dd : 0.0;
-    const double ph = on ? phi : 0.0;
-    hx = fma(ph, wx, fma(g, dx, hx));
-    hy = fma(ph, wy, fma(g, dy, hy));
-    hz = fma(ph, wz, fma(g, dz, hz));
+    const float phi = fm * ir, dphi = (dfm - phi) * ir;
+    const float g = on ? dphi * ir * dd : 0.0f;
+    const float ph = on ? phi : 0.0f;
+    hx += (double)fmaf(ph, wx, g * dx);
+    hy += (double)fmaf(ph, wy, g * dy);
+    hz += (double)fmaf(ph, wz, g * dz);
 }
 
 template <int SMODE, int EMODE>
@@ -569,27 +571,55 @@ __global__ void __launch_bounds__(BLOCK) k_adj_pair(AdjPairArgs a)
     const int32_t il = lane & 7, jl = lane >> 3;
     const int32_t i = ci * CLUSTER + il;
     double hx = 0.0, hy = 0.0, hz = 0.0, dqi = 0.0;
-    const double cs = a.k.c_spring;
+    const float cs = (float)a.k.c_spring;
+    const float kf = (float)a.kf;
     if (ci < a.ncl)
     {
         const bool iv = i < a.n;
         const float4 pi = iv ? a.pos[i] : make_float4(0.f, 0.f, 0.f, 0.f);
         const float4 ri = iv ? a.jprm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-        const double4 wi = iv ? a.w[i] : make_double4(0.0, 0.0, 0.0, 0.0);
+        const float4 wi = iv ? a.w[i] : make_float4(0.f, 0.f, 0.f, 0.f);
         const uint32_t fli = fbits(ri.w);
         const uint32_t ibit = 1u << (24 + il);
         const int32_t cnt16 = (a.cln[ci] + 15) & ~15;
         const uint32_t *L = a.cl + (size_t)ci * a.cap;
+        const int32_t last = cnt16 > 16 ? cnt16 - 16 : 0;
+        uint32_t w0 = 0u, w1 = 0u, n0 = 0u, n1 = 0u;
+        if (cnt16 > 0)
+        {
+            const int32_t o = min(16, last);
+            w0 = L[jl];
+            w1 = L[8 + jl];
+            n0 = L[o + jl];
+            n1 = L[o + 8 + jl];
+        }
+        int32_t j0 = (int32_t)(w0 & CL_JMASK), j1 = (int32_t)(w1 & CL_JMASK);
+        float4 pj0 = a.pos[j0], rj0 = a.jprm[j0], wj0 = a.w[j0];
+        float4 pj1 = a.pos[j1], rj1 = a.jprm[j1], wj1 = a.w[j1];
         for (int32_t t = 0; t < cnt16; t += 16)
         {
-            const uint32_t w0 = L[t + jl], w1 = L[t + 8 + jl];
-            const int32_t j0 = (int32_t)(w0 & CL_JMASK), j1 = (int32_t)(w1 & CL_JMASK);
-            const float4 pj0 = a.pos[j0], rj0 = a.jprm[j0], pj1 = a.pos[j1], rj1 = a.jprm[j1];
-            const double4 wj0 = a.w[j0], wj1 = a.w[j1];
-            adj_pair_term<SMODE, EMODE>((w0 & ibit) != 0u, pi, ri, fli, wi, pj0, rj0, wj0, j0, a, cs, hx, hy, hz,
+            const int32_t o = min(t + 32, last);
+            const uint32_t m0 = L[o + jl];
+            const uint32_t m1 = L[o + 8 + jl];
+            const int32_t k0 = (int32_t)(n0 & CL_JMASK), k1 = (int32_t)(n1 & CL_JMASK);
+            const float4 qj0 = a.pos[k0], sj0 = a.jprm[k0], vj0 = a.w[k0];
+            const float4 qj1 = a.pos[k1], sj1 = a.jprm[k1], vj1 = a.w[k1];
+            adj_pair_term<SMODE, EMODE>((w0 & ibit) != 0u, pi, ri, fli, wi, pj0, rj0, wj0, j0, a, cs, kf, hx, hy, hz,
                                         dqi);
-            adj_pair_term<SMODE, EMODE>((w1 & ibit) != 0u, pi, ri, fli, wi, pj1, rj1, wj1, j1, a, cs, hx, hy, hz,
+            adj_pair_term<SMODE, EMODE>((w1 & ibit) != 0u, pi, ri, fli, wi, pj1, rj1, wj1, j1, a, cs, kf, hx, hy, hz,
                                         dqi);
+            w0 = n0;
+            w1 = n1;
+            n0 = m0;
+            n1 = m1;
+            j0 = k0;
+            j1 = k1;
+            pj0 = qj0;
+            rj0 = sj0;
+            wj0 = vj0;
+            pj1 = qj1;
+            rj1 = sj1;
+            wj1 = vj1;
         }
     }
     for (int off = 8; off < 64; off <<= 1)

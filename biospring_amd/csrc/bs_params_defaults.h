@@ -39,7 +39,7 @@ static inline void bs_params_fill_defaults(bs_params *p)
     p->densitygrid_enable = 0;
     p->densitygrid_scale = 1.0f;
     p->graph_steps = 0;
-    p->pair_layout = BS_PAIR_ROWS;
+    p->pair_layout = BS_PAIR_CLUSTERS;
 }
 
 #endif

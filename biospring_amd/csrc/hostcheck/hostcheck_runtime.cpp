@@ -609,7 +609,7 @@ void launch_adj_pair(const AdjPairArgs &a, int32_t, int32_t, int32_t nb, hipStre
     need(a.jprm, sizeof(float4) * a.n, "adj_pair: jprm");
     need(a.cln, sizeof(int32_t) * a.ncl, "adj_pair: cln");
     need(a.cl, sizeof(uint32_t) * (size_t)a.ncl * a.cap, "adj_pair: cl");
-    need(a.w, sizeof(double4) * a.n, "adj_pair: w");
+    need(a.w, sizeof(float4) * a.n, "adj_pair: w");
     need(a.hw, sizeof(double4) * a.n, "adj_pair: hw");
     need(a.dq, sizeof(double) * a.n, "adj_pair: dq");
     if ((int64_t)nb * (BLOCK / 64) < a.ncl)
@@ -637,11 +637,11 @@ void launch_adj_spring(const AdjSpringArgs &a, int32_t group, int32_t nb, hipStr
     need(a.jprm, sizeof(float4) * n, "adj_spring: jprm");
     need(a.iprm, sizeof(float4) * n, "adj_spring: iprm");
     need(a.sp_ptr, sizeof(int32_t) * (n + 1), "adj_spring: sp_ptr");
-    need(a.w, sizeof(double4) * n, "adj_spring: w");
+    need(a.w, sizeof(float4) * n, "adj_spring: w");
     need(a.lx, sizeof(double4) * n, "adj_spring: lx");
     need(a.lv, sizeof(double4) * n, "adj_spring: lv");
-    need(a.w_next, sizeof(double4) * n, "adj_spring: w_next");
-    if (a.w_next == a.w)
+    need(a.w_next, sizeof(float4) * n, "adj_spring: w_next");
+    if ((const float4 *)a.w_next == a.w)
         hc_fail("adj_spring: w_next aliases w");
     if (a.hwp)
         need(a.hwp, sizeof(double4) * n, "adj_spring: hwp");
@@ -667,7 +667,7 @@ void launch_adj_spring(const AdjSpringArgs &a, int32_t group, int32_t nb, hipStr
                 if (a.sp_j[e] < 0 || a.sp_j[e] > i)
                     a.dk[e] += 1.0;
             }
-        a.w_next[i] = make_double4(0, 0, 0, 0);
+        a.w_next[i] = make_float4(0, 0, 0, 0);
     }
 }
 
@@ -691,7 +691,7 @@ void launch_spring_grads(const int32_t *own_e, const double *dke, const double *
 }
 
 void launch_adj_init(const float4 *pos, const float4 *target, const float4 *jprm, const float4 *iprm, int32_t n,
-                     double, double4 *lx, double4 *lv, double4 *w, double *loss_part, int32_t nblocks, hipStream_t)
+                     double, double4 *lx, double4 *lv, float4 *w, double *loss_part, int32_t nblocks, hipStream_t)
 {
     need(pos, sizeof(float4) * n, "adj_init: pos");
     need(target, sizeof(float4) * n, "adj_init: target");
@@ -699,7 +699,7 @@ void launch_adj_init(const float4 *pos, const float4 *target, const float4 *jprm
     need(iprm, sizeof(float4) * n, "adj_init: iprm");
     need(lx, sizeof(double4) * n, "adj_init: lx");
     need(lv, sizeof(double4) * n, "adj_init: lv");
-    need(w, sizeof(double4) * n, "adj_init: w");
+    need(w, sizeof(float4) * n, "adj_init: w");
     need(loss_part, sizeof(double) * nblocks, "adj_init: loss_part");
     if ((int64_t)nblocks * BLOCK < n)
         hc_fail("adj_init: grid too small");
@@ -716,7 +716,8 @@ void launch_adj_init(const float4 *pos, const float4 *target, const float4 *jprm
     }
     for (int32_t i = 0; i < n; ++i)
     {
-        lx[i] = lv[i] = w[i] = make_double4(0, 0, 0, 0);
+        lx[i] = lv[i] = make_double4(0, 0, 0, 0);
+        w[i] = make_float4(0, 0, 0, 0);
     }
 }
 

@@ -42,8 +42,8 @@ extern "C" {
 #define BS_STERIC_LEWITT_8_6  2  /* "lennard-jones-8-6Lewitt" */
 #define BS_STERIC_ZACHARIAS_8_6 3 /* "lennard-jones-8-6Zacharias" */
 
-#define BS_PAIR_ROWS     0  /* bs_params.pair_layout */
-#define BS_PAIR_CLUSTERS 1
+#define BS_PAIR_CLUSTERS 0  /* bs_params.pair_layout */
+#define BS_PAIR_ROWS     1
 
 /* ---- dielectric models ------------------------------------------------- */
 #define BS_DIELECTRIC_CONSTANT 0  /* reference: electrostatic.hpp:16-49 */
@@ -130,12 +130,13 @@ typedef struct bs_params {
        Results are identical either way.  Default 0 */
     int32_t graph_steps;
 
-    /* Engine option: layout the pair kernel reads.  BS_PAIR_ROWS: the cluster
-       lists are expanded at each rebuild into one list per particle, so that
-       every lane evaluates a real pair (the default); BS_PAIR_CLUSTERS: the
-       kernel reads the cluster lists directly (8 particles x 8 entries per
-       wave, lanes of entries a particle does not see stay idle).  Results are
-       identical either way. */
+    /* Engine option: layout the pair kernel reads.  BS_PAIR_CLUSTERS (the
+       default): the cluster lists, 8 particles x 8 entries per wave, one load
+       of j's data serving 8 lanes; lanes of entries a particle does not see
+       stay idle.  BS_PAIR_ROWS: the cluster lists expanded at each rebuild
+       into one list per particle, every lane evaluating a listed pair at one
+       gather per pair (slower where gathers, not arithmetic, bound the
+       kernel).  Forces agree to FP64 rounding either way. */
     int32_t pair_layout;
 } bs_params;
 
