@@ -2132,7 +2132,7 @@ int bs_adjoint(bs_handle *h, int32_t nsteps, int32_t checkpoint_every, const flo
     rc = upload_ext(h, ext_force3);
     if (rc)
         return rc;
-    if (dk || dl0)
+    if ((dk || dl0) && !h->s1.empty())
     {
         rc = build_spring_owners(h);
         if (rc)
@@ -2151,7 +2151,8 @@ int bs_adjoint(bs_handle *h, int32_t nsteps, int32_t checkpoint_every, const flo
     int32_t *cp_cnt, *fin_cnt;
     double4 *lx, *lv, *hwp = nullptr;
     float4 *wb[2];
-    double *dke, *dle, *dqd, *lpart;
+    float *dke = nullptr, *dle = nullptr;
+    double *dqd, *lpart;
 #define SALLOC(ptr, T_, cnt)                                                                                     \
     do                                                                                                            \
     {                                                                                                             \
@@ -2172,11 +2173,14 @@ int bs_adjoint(bs_handle *h, int32_t nsteps, int32_t checkpoint_every, const flo
     SALLOC(wb[1], float4, n);
     if (!h->lists.empty())
         SALLOC(hwp, double4, n);
-    SALLOC(dke, double, E);
-    SALLOC(dle, double, E);
+    // spring gradients: FP32 sums per owning edge within a segment, flushed
+    // into FP64 per-spring totals at the end of each segment
+    const bool want_springs = (dk || dl0) && m > 0;
     double *dks = nullptr, *dls = nullptr;
-    if (dk || dl0)
+    if (want_springs)
     {
+        SALLOC(dke, float, E);
+        SALLOC(dle, float, E);
         SALLOC(dks, double, m);
         SALLOC(dls, double, m);
     }
@@ -2231,8 +2235,13 @@ int bs_adjoint(bs_handle *h, int32_t nsteps, int32_t checkpoint_every, const flo
     }
     const double dt = (double)P.timestep, gam = P.viscosity_enable ? (double)P.viscosity_value : 0.0;
     launch_adj_init(h->pos[h->cur], tgt, h->jprm, h->iprm, (int32_t)n, dt, lx, lv, wb[0], lpart, lblocks, st);
-    HIPCHK(h, hipMemsetAsync(dke, 0, sizeof(double) * std::max<int64_t>(E, 1), st));
-    HIPCHK(h, hipMemsetAsync(dle, 0, sizeof(double) * std::max<int64_t>(E, 1), st));
+    if (want_springs)
+    {
+        HIPCHK(h, hipMemsetAsync(dke, 0, sizeof(float) * std::max<int64_t>(E, 1), st));
+        HIPCHK(h, hipMemsetAsync(dle, 0, sizeof(float) * std::max<int64_t>(E, 1), st));
+        HIPCHK(h, hipMemsetAsync(dks, 0, sizeof(double) * m, st));
+        HIPCHK(h, hipMemsetAsync(dls, 0, sizeof(double) * m, st));
+    }
     HIPCHK(h, hipMemsetAsync(dqd, 0, sizeof(double) * n, st));
     const double hs = 0.5 * std::max(P.list_skin, 0.0f);
     const float half_skin2 = (float)(hs * hs * (1.0 - 1e-5));
@@ -2294,6 +2303,8 @@ int bs_adjoint(bs_handle *h, int32_t nsteps, int32_t checkpoint_every, const flo
             a.gamma = gam;
             a.k = h->k;
             launch_adj_spring(a, h->group, h->spring_blocks, st);
+            if (want_springs && s == s0)
+                launch_spring_grads(h->sp_own, dke, dle, m, dks, dls, st);
             if (h->prof)
             {
                 const size_t e3 = prof_mark(h);
@@ -2311,17 +2322,15 @@ int bs_adjoint(bs_handle *h, int32_t nsteps, int32_t checkpoint_every, const flo
     HIPCHK(h, hipMemcpyAsync(h->counters, fin_cnt, 4 * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
     launch_fill_i32(h->status, 1, 1, nullptr, st);
 
-    if (dk || dl0)
-        launch_spring_grads(h->sp_own, dke, dle, m, dks, dls, st);
     std::vector<double> lp((size_t)lblocks), dqh((size_t)n);
     int32_t stv[4];
     HIPCHK(h, hipMemcpyAsync(lp.data(), lpart, sizeof(double) * lblocks, hipMemcpyDeviceToHost, st));
     HIPCHK(h, hipMemcpyAsync(stv, h->status, sizeof stv, hipMemcpyDeviceToHost, st));
     if (dq)
         HIPCHK(h, hipMemcpyAsync(dqh.data(), dqd, sizeof(double) * n, hipMemcpyDeviceToHost, st));
-    if (m > 0 && dk)
+    if (want_springs && dk)
         HIPCHK(h, hipMemcpyAsync(dk, dks, sizeof(double) * m, hipMemcpyDeviceToHost, st));
-    if (m > 0 && dl0)
+    if (want_springs && dl0)
         HIPCHK(h, hipMemcpyAsync(dl0, dls, sizeof(double) * m, hipMemcpyDeviceToHost, st));
     HIPCHK(h, hipStreamSynchronize(st));
     prof_collect(h);

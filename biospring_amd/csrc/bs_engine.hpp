@@ -213,7 +213,7 @@ struct AdjSpringArgs
     const double4 *hwp;   // nullable: pair part of H w
     double4 *lx, *lv;     // [N] adjoint of x and v (updated in place)
     float4 *w_next;       // [N] weights of the step before
-    double *dk, *dl0;     // [E] gradients, on the owning edge of each spring
+    float *dk, *dl0;      // nullable: [E] this segment's gradient sums on the owning edge of each spring
     int32_t n, spring_enable;
     float spring_scale;
     double dt, gamma;     // gamma = 0 without viscosity
@@ -284,10 +284,11 @@ void launch_adj_spring(const AdjSpringArgs &a, int32_t group, int32_t nblocks, h
 void launch_adj_init(const float4 *pos, const float4 *target, const float4 *jprm, const float4 *iprm, int32_t n,
                      double dt, double4 *lx, double4 *lv, float4 *w, double *loss_part, int32_t nblocks,
                      hipStream_t s);
-// Per-spring gradients from the per-edge accumulators: dk[s] = dke[own_e[s]]
-// (0 where own_e[s] < 0), likewise dl0.
-void launch_spring_grads(const int32_t *own_e, const double *dke, const double *dle, int64_t m, double *dk,
-                         double *dl0, hipStream_t s);
+// Per-spring FP64 totals from the FP32 per-edge sums of a segment:
+// dk[s] += dke[own_e[s]] and the edge sum is cleared (own_e[s] < 0: none),
+// likewise dl0.
+void launch_spring_grads(const int32_t *own_e, float *dke, float *dle, int64_t m, double *dk, double *dl0,
+                         hipStream_t s);
 // status[0] = 1 when any particle is more than half the list skin from xref.
 void launch_drift_check(const float4 *pos, const float4 *xref, int32_t n, float half_skin2, int32_t *status,
                         hipStream_t s);

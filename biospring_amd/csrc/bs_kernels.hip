@@ -772,15 +772,20 @@ __global__ void __launch_bounds__(BLOCK) k_adj_init(const float4 *pos, const flo
         loss_part[blockIdx.x] = t;
 }
 
-__global__ void k_spring_grads(const int32_t *own_e, const double *dke, const double *dle, int64_t m, double *dk,
-                               double *dl0)
+// Segment flush: the FP32 per-edge sums of one segment (at most K steps) are
+// added to the FP64 per-spring totals and cleared.
+__global__ void k_spring_grads(const int32_t *own_e, float *dke, float *dle, int64_t m, double *dk, double *dl0)
 {
     const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (s >= m)
         return;
     const int32_t e = own_e[s];
-    dk[s] = e >= 0 ? dke[e] : 0.0;
-    dl0[s] = e >= 0 ? dle[e] : 0.0;
+    if (e < 0)
+        return;
+    dk[s] += (double)dke[e];
+    dl0[s] += (double)dle[e];
+    dke[e] = 0.0f;
+    dle[e] = 0.0f;
 }
 
 __global__ void k_drift_check(const float4 *pos, const float4 *xref, int32_t n, float half_skin2, int32_t *status)
@@ -820,31 +825,52 @@ __global__ void __launch_bounds__(BLOCK) k_adj_spring(AdjSpringArgs a)
     {
         const double C = a.k.c_spring * (double)a.spring_scale;
         const int32_t beg = a.sp_ptr[row], end = a.sp_ptr[row + 1];
-        for (int32_t e = beg + lane; e < end; e += G)
+        constexpr int U = 4; // edges in flight per lane
+        for (int32_t e0 = beg + lane; e0 < end; e0 += U * G)
         {
-            const int32_t jr = a.sp_j[e];
-            const int32_t j = jr & 0x7fffffff;
-            const double k = a.sp_k[e], l0 = a.sp_l0[e];
-            const float4 pj = a.pos[j];
-            const float4 wj = a.w[j];
-            const double dx = (double)pj.x - (double)pi.x, dy = (double)pj.y - (double)pi.y,
-                         dz = (double)pj.z - (double)pi.z;
-            const double r2 = dx * dx + dy * dy + dz * dz;
-            if (!(r2 > 0.0))
-                continue;
-            const double ir = rsqrt_d(r2), r = r2 * ir;
-            const double phi = k * (r - l0) * C * ir, dphi = (k * C - phi) * ir;
-            const double wx = (double)wj.x - wi.x, wy = (double)wj.y - wi.y, wz = (double)wj.z - wi.z;
-            const double dd = dx * wx + dy * wy + dz * wz;
-            const double g = dphi * ir * dd;
-            hx = fma(phi, wx, fma(g, dx, hx));
-            hy = fma(phi, wy, fma(g, dy, hy));
-            hz = fma(phi, wz, fma(g, dz, hz));
-            if (jr < 0 || j > row)
+            int32_t jr[U];
+            float kk[U], ll[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
             {
-                const double wu = dd * ir * C;
-                a.dk[e] -= wu * (r - l0);
-                a.dl0[e] += wu * k;
+                const int32_t e = e0 + u * G;
+                const bool ok = e < end;
+                jr[u] = ok ? a.sp_j[e] : row;
+                kk[u] = ok ? a.sp_k[e] : 0.0f;
+                ll[u] = ok ? a.sp_l0[e] : 0.0f;
+            }
+            float4 pj[U], wj[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+            {
+                pj[u] = a.pos[jr[u] & 0x7fffffff];
+                wj[u] = a.w[jr[u] & 0x7fffffff];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+            {
+                const int32_t e = e0 + u * G;
+                const int32_t j = jr[u] & 0x7fffffff;
+                const double dx = (double)pj[u].x - (double)pi.x, dy = (double)pj[u].y - (double)pi.y,
+                             dz = (double)pj[u].z - (double)pi.z;
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                if (!(e < end && r2 > 0.0))
+                    continue;
+                const double k = kk[u], l0 = ll[u];
+                const double ir = rsqrt_d(r2), r = r2 * ir;
+                const double phi = k * (r - l0) * C * ir, dphi = (k * C - phi) * ir;
+                const double wx = (double)wj[u].x - wi.x, wy = (double)wj[u].y - wi.y, wz = (double)wj[u].z - wi.z;
+                const double dd = dx * wx + dy * wy + dz * wz;
+                const double g = dphi * ir * dd;
+                hx = fma(phi, wx, fma(g, dx, hx));
+                hy = fma(phi, wy, fma(g, dy, hy));
+                hz = fma(phi, wz, fma(g, dz, hz));
+                if (a.dk && (jr[u] < 0 || j > row))
+                {
+                    const double wu = dd * ir * C;
+                    a.dk[e] -= (float)(wu * (r - l0));
+                    a.dl0[e] += (float)(wu * k);
+                }
             }
         }
     }
@@ -998,8 +1024,8 @@ void launch_adj_init(const float4 *pos, const float4 *target, const float4 *jprm
     k_adj_init<<<nblocks, BLOCK, 0, s>>>(pos, target, jprm, iprm, n, dt, lx, lv, w, loss_part);
 }
 
-void launch_spring_grads(const int32_t *own_e, const double *dke, const double *dle, int64_t m, double *dk,
-                         double *dl0, hipStream_t s)
+void launch_spring_grads(const int32_t *own_e, float *dke, float *dle, int64_t m, double *dk, double *dl0,
+                         hipStream_t s)
 {
     if (m > 0)
         k_spring_grads<<<nblk(m), BLOCK, 0, s>>>(own_e, dke, dle, m, dk, dl0);

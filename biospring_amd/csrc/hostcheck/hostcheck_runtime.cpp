@@ -653,8 +653,11 @@ void launch_adj_spring(const AdjSpringArgs &a, int32_t group, int32_t nb, hipStr
         need(a.sp_j, sizeof(int32_t) * ne, "adj_spring: sp_j");
         need(a.sp_k, sizeof(float) * ne, "adj_spring: sp_k");
         need(a.sp_l0, sizeof(float) * ne, "adj_spring: sp_l0");
-        need(a.dk, sizeof(double) * ne, "adj_spring: dk");
-        need(a.dl0, sizeof(double) * ne, "adj_spring: dl0");
+        if (a.dk)
+        {
+            need(a.dk, sizeof(float) * ne, "adj_spring: dk");
+            need(a.dl0, sizeof(float) * ne, "adj_spring: dl0");
+        }
     }
     for (int32_t i = 0; i < n; ++i)
     {
@@ -664,15 +667,15 @@ void launch_adj_spring(const AdjSpringArgs &a, int32_t group, int32_t nb, hipStr
             {
                 if ((a.sp_j[e] & 0x7fffffff) >= n)
                     hc_fail("adj_spring: edge index out of range");
-                if (a.sp_j[e] < 0 || a.sp_j[e] > i)
-                    a.dk[e] += 1.0;
+                if (a.dk && (a.sp_j[e] < 0 || a.sp_j[e] > i))
+                    a.dk[e] += 1.0f;
             }
         a.w_next[i] = make_float4(0, 0, 0, 0);
     }
 }
 
-void launch_spring_grads(const int32_t *own_e, const double *dke, const double *dle, int64_t m, double *dk,
-                         double *dl0, hipStream_t)
+void launch_spring_grads(const int32_t *own_e, float *dke, float *dle, int64_t m, double *dk, double *dl0,
+                         hipStream_t)
 {
     need(own_e, sizeof(int32_t) * m, "spring_grads: own_e");
     need(dk, sizeof(double) * m, "spring_grads: dk");
@@ -682,11 +685,12 @@ void launch_spring_grads(const int32_t *own_e, const double *dke, const double *
         const int32_t e = own_e[s];
         if (e >= 0)
         {
-            need(dke + e, sizeof(double), "spring_grads: dke");
-            need(dle + e, sizeof(double), "spring_grads: dle");
+            need(dke + e, sizeof(float), "spring_grads: dke");
+            need(dle + e, sizeof(float), "spring_grads: dle");
+            dk[s] += dke[e];
+            dl0[s] += dle[e];
+            dke[e] = dle[e] = 0.0f;
         }
-        dk[s] = e >= 0 ? dke[e] : 0.0;
-        dl0[s] = e >= 0 ? dle[e] : 0.0;
     }
 }
 
