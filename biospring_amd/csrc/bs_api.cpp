@@ -97,7 +97,11 @@ struct bs_handle
     float *sp_k = nullptr, *sp_l0 = nullptr;
     int64_t nedges = 0;
     int32_t ncl = 0;                                  // i-clusters of 8
-    int32_t *sp_own = nullptr;                        // [m] owning CSR edge of each spring (bs_adjoint), -1: none
+    // bs_adjoint: springs with a dynamic end in the order of their owning CSR
+    // edge ("slots"): slot_e[r] = that edge (ascending), slot_of[s] = slot of
+    // spring s or -1
+    int32_t *slot_e = nullptr, *slot_of = nullptr;
+    int64_t nslots = -1;
     int32_t *ex_ptr = nullptr;                        // [ncl + 1] static exclusions
     uint32_t *ex_w = nullptr;                         // j << 8 | bonded mask
     int32_t *cell_of = nullptr, *counts = nullptr, *cstart = nullptr, *cursor = nullptr, *ctmp = nullptr,
@@ -213,10 +217,11 @@ void free_device(bs_handle *h)
     h->seg_part = h->sys_E = nullptr;
     h->shift = nullptr;
     h->nseg = 0;
-    f(h->ex_ptr); f(h->ex_w); f(h->sp_own);
+    f(h->ex_ptr); f(h->ex_w); f(h->slot_e); f(h->slot_of);
     h->ex_ptr = nullptr;
     h->ex_w = nullptr;
-    h->sp_own = nullptr;
+    h->slot_e = h->slot_of = nullptr;
+    h->nslots = -1;
     f(h->bodies_d); f(h->rx); f(h->rp0); f(h->frig); f(h->slot_p); f(h->slot_b); f(h->bstart);
     f(h->grid_d[0]); f(h->grid_d[1]); f(h->cent_part); f(h->cent);
     h->bodies_d = nullptr;
@@ -885,7 +890,7 @@ struct Scratch
 int build_spring_owners(bs_handle *h)
 {
     const int64_t n = h->n, m = (int64_t)h->s1.size(), E = h->nedges;
-    if (h->sp_own || m == 0)
+    if (h->nslots >= 0 || m == 0)
         return BS_OK;
     std::vector<int32_t> hp((size_t)n + 1), hj((size_t)std::max<int64_t>(E, 1)), own((size_t)m, -1);
     HIPCHK(h, hipMemcpy(hp.data(), h->sp_ptr, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToHost));
@@ -913,10 +918,27 @@ int build_spring_owners(bs_handle *h)
         if (lo < hp[(size_t)a + 1] && (hj[(size_t)lo] & 0x7fffffff) == b)
             own[(size_t)s] = lo;
     }
+    std::vector<std::pair<int32_t, int32_t>> es;
+    es.reserve((size_t)m);
+    for (int64_t s = 0; s < m; ++s)
+        if (own[(size_t)s] >= 0)
+            es.push_back({own[(size_t)s], (int32_t)s});
+    std::sort(es.begin(), es.end());
+    std::vector<int32_t> se(es.size()), so((size_t)m, -1);
+    for (size_t r = 0; r < es.size(); ++r)
+    {
+        se[r] = es[r].first;
+        so[(size_t)es[r].second] = (int32_t)r;
+    }
     hipError_t e = hipSuccess;
-    h->sp_own = dalloc<int32_t>((size_t)m, e);
+    h->slot_e = dalloc<int32_t>(se.size(), e);
     HIPCHK(h, e);
-    HIPCHK(h, hipMemcpy(h->sp_own, own.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice));
+    h->slot_of = dalloc<int32_t>((size_t)m, e);
+    HIPCHK(h, e);
+    if (!se.empty())
+        HIPCHK(h, hipMemcpy(h->slot_e, se.data(), sizeof(int32_t) * se.size(), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->slot_of, so.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice));
+    h->nslots = (int64_t)se.size();
     return BS_OK;
 }
 
@@ -2173,17 +2195,22 @@ int bs_adjoint(bs_handle *h, int32_t nsteps, int32_t checkpoint_every, const flo
     SALLOC(wb[1], float4, n);
     if (!h->lists.empty())
         SALLOC(hwp, double4, n);
-    // spring gradients: FP32 sums per owning edge within a segment, flushed
-    // into FP64 per-spring totals at the end of each segment
+    // spring gradients: FP32 sums per owning edge over at most FLUSH steps,
+    // then added into FP64 totals per slot; per spring at the end
     const bool want_springs = (dk || dl0) && m > 0;
-    double *dks = nullptr, *dls = nullptr;
+    const int64_t ns = std::max<int64_t>(h->nslots, 0);
+    double *dks = nullptr, *dls = nullptr, *outk = nullptr, *outl = nullptr;
     if (want_springs)
     {
         SALLOC(dke, float, E);
         SALLOC(dle, float, E);
-        SALLOC(dks, double, m);
-        SALLOC(dls, double, m);
+        SALLOC(dks, double, ns);
+        SALLOC(dls, double, ns);
+        SALLOC(outk, double, m);
+        SALLOC(outl, double, m);
     }
+    constexpr int32_t FLUSH = 64;
+    int32_t since_flush = 0;
     SALLOC(dqd, double, n);
     const int32_t lblocks = (int32_t)((n + BLOCK - 1) / BLOCK);
     SALLOC(lpart, double, lblocks);
@@ -2239,8 +2266,8 @@ int bs_adjoint(bs_handle *h, int32_t nsteps, int32_t checkpoint_every, const flo
     {
         HIPCHK(h, hipMemsetAsync(dke, 0, sizeof(float) * std::max<int64_t>(E, 1), st));
         HIPCHK(h, hipMemsetAsync(dle, 0, sizeof(float) * std::max<int64_t>(E, 1), st));
-        HIPCHK(h, hipMemsetAsync(dks, 0, sizeof(double) * m, st));
-        HIPCHK(h, hipMemsetAsync(dls, 0, sizeof(double) * m, st));
+        HIPCHK(h, hipMemsetAsync(dks, 0, sizeof(double) * std::max<int64_t>(ns, 1), st));
+        HIPCHK(h, hipMemsetAsync(dls, 0, sizeof(double) * std::max<int64_t>(ns, 1), st));
     }
     HIPCHK(h, hipMemsetAsync(dqd, 0, sizeof(double) * n, st));
     const double hs = 0.5 * std::max(P.list_skin, 0.0f);
@@ -2303,8 +2330,11 @@ int bs_adjoint(bs_handle *h, int32_t nsteps, int32_t checkpoint_every, const flo
             a.gamma = gam;
             a.k = h->k;
             launch_adj_spring(a, h->group, h->spring_blocks, st);
-            if (want_springs && s == s0)
-                launch_spring_grads(h->sp_own, dke, dle, m, dks, dls, st);
+            if (want_springs && (++since_flush == FLUSH || s == 0))
+            {
+                launch_spring_grads(h->slot_e, dke, dle, ns, dks, dls, st);
+                since_flush = 0;
+            }
             if (h->prof)
             {
                 const size_t e3 = prof_mark(h);
@@ -2322,6 +2352,8 @@ int bs_adjoint(bs_handle *h, int32_t nsteps, int32_t checkpoint_every, const flo
     HIPCHK(h, hipMemcpyAsync(h->counters, fin_cnt, 4 * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
     launch_fill_i32(h->status, 1, 1, nullptr, st);
 
+    if (want_springs)
+        launch_spring_out(h->slot_of, m, dks, dls, outk, outl, st);
     std::vector<double> lp((size_t)lblocks), dqh((size_t)n);
     int32_t stv[4];
     HIPCHK(h, hipMemcpyAsync(lp.data(), lpart, sizeof(double) * lblocks, hipMemcpyDeviceToHost, st));
@@ -2329,9 +2361,9 @@ int bs_adjoint(bs_handle *h, int32_t nsteps, int32_t checkpoint_every, const flo
     if (dq)
         HIPCHK(h, hipMemcpyAsync(dqh.data(), dqd, sizeof(double) * n, hipMemcpyDeviceToHost, st));
     if (want_springs && dk)
-        HIPCHK(h, hipMemcpyAsync(dk, dks, sizeof(double) * m, hipMemcpyDeviceToHost, st));
+        HIPCHK(h, hipMemcpyAsync(dk, outk, sizeof(double) * m, hipMemcpyDeviceToHost, st));
     if (want_springs && dl0)
-        HIPCHK(h, hipMemcpyAsync(dl0, dls, sizeof(double) * m, hipMemcpyDeviceToHost, st));
+        HIPCHK(h, hipMemcpyAsync(dl0, outl, sizeof(double) * m, hipMemcpyDeviceToHost, st));
     HIPCHK(h, hipStreamSynchronize(st));
     prof_collect(h);
     if (stv[2] != 0)

@@ -284,11 +284,14 @@ void launch_adj_spring(const AdjSpringArgs &a, int32_t group, int32_t nblocks, h
 void launch_adj_init(const float4 *pos, const float4 *target, const float4 *jprm, const float4 *iprm, int32_t n,
                      double dt, double4 *lx, double4 *lv, float4 *w, double *loss_part, int32_t nblocks,
                      hipStream_t s);
-// Per-spring FP64 totals from the FP32 per-edge sums of a segment:
-// dk[s] += dke[own_e[s]] and the edge sum is cleared (own_e[s] < 0: none),
-// likewise dl0.
-void launch_spring_grads(const int32_t *own_e, float *dke, float *dle, int64_t m, double *dk, double *dl0,
+// Spring-gradient flush: for each slot r (springs in the order of their
+// owning CSR edge, slot_e ascending) dk[r] += dke[slot_e[r]] in FP64 and the
+// FP32 edge sum is cleared; likewise dl0.
+void launch_spring_grads(const int32_t *slot_e, float *dke, float *dle, int64_t nslots, double *dk, double *dl0,
                          hipStream_t s);
+// Per spring from per slot: out[s] = slot_of[s] >= 0 ? v[slot_of[s]] : 0.
+void launch_spring_out(const int32_t *slot_of, int64_t m, const double *dk, const double *dl0, double *outk,
+                       double *outl, hipStream_t s);
 // status[0] = 1 when any particle is more than half the list skin from xref.
 void launch_drift_check(const float4 *pos, const float4 *xref, int32_t n, float half_skin2, int32_t *status,
                         hipStream_t s);

@@ -772,20 +772,30 @@ __global__ void __launch_bounds__(BLOCK) k_adj_init(const float4 *pos, const flo
         loss_part[blockIdx.x] = t;
 }
 
-// Segment flush: the FP32 per-edge sums of one segment (at most K steps) are
-// added to the FP64 per-spring totals and cleared.
-__global__ void k_spring_grads(const int32_t *own_e, float *dke, float *dle, int64_t m, double *dk, double *dl0)
+// Flush: the FP32 per-edge sums of the last steps are added to the FP64
+// totals per slot and cleared.  Slots follow the edges, so every access
+// walks memory forward.
+__global__ void k_spring_grads(const int32_t *slot_e, float *dke, float *dle, int64_t ns, double *dk, double *dl0)
+{
+    const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (r >= ns)
+        return;
+    const int32_t e = slot_e[r];
+    dk[r] += (double)dke[e];
+    dl0[r] += (double)dle[e];
+    dke[e] = 0.0f;
+    dle[e] = 0.0f;
+}
+
+__global__ void k_spring_out(const int32_t *slot_of, int64_t m, const double *dk, const double *dl0, double *outk,
+                             double *outl)
 {
     const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (s >= m)
         return;
-    const int32_t e = own_e[s];
-    if (e < 0)
-        return;
-    dk[s] += (double)dke[e];
-    dl0[s] += (double)dle[e];
-    dke[e] = 0.0f;
-    dle[e] = 0.0f;
+    const int32_t r = slot_of[s];
+    outk[s] = r >= 0 ? dk[r] : 0.0;
+    outl[s] = r >= 0 ? dl0[r] : 0.0;
 }
 
 __global__ void k_drift_check(const float4 *pos, const float4 *xref, int32_t n, float half_skin2, int32_t *status)
@@ -1024,11 +1034,18 @@ void launch_adj_init(const float4 *pos, const float4 *target, const float4 *jprm
     k_adj_init<<<nblocks, BLOCK, 0, s>>>(pos, target, jprm, iprm, n, dt, lx, lv, w, loss_part);
 }
 
-void launch_spring_grads(const int32_t *own_e, float *dke, float *dle, int64_t m, double *dk, double *dl0,
+void launch_spring_grads(const int32_t *slot_e, float *dke, float *dle, int64_t ns, double *dk, double *dl0,
                          hipStream_t s)
 {
+    if (ns > 0)
+        k_spring_grads<<<nblk(ns), BLOCK, 0, s>>>(slot_e, dke, dle, ns, dk, dl0);
+}
+
+void launch_spring_out(const int32_t *slot_of, int64_t m, const double *dk, const double *dl0, double *outk,
+                       double *outl, hipStream_t s)
+{
     if (m > 0)
-        k_spring_grads<<<nblk(m), BLOCK, 0, s>>>(own_e, dke, dle, m, dk, dl0);
+        k_spring_out<<<nblk(m), BLOCK, 0, s>>>(slot_of, m, dk, dl0, outk, outl);
 }
 
 void launch_drift_check(const float4 *pos, const float4 *xref, int32_t n, float half_skin2, int32_t *status,

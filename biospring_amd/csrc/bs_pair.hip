@@ -122,11 +122,24 @@ __global__ void __launch_bounds__(BLOCK) k_cluster_list(ListArgs a)
     const bool vi = lane < CLUSTER && iself < a.n && !(fi & PF_STATIC) && (a.i_mask == 0u || (fi & a.i_mask));
     uint32_t todo = (uint32_t)(__ballot(vi) & 0xFFull);
     int32_t count = 0;
-    // a cluster that straddles two systems searches once per system
+    // A cluster searches once per group of its particles that share a system
+    // and lie within the list radius (per axis) of the group's first
+    // particle.  Compact clusters are one group; a loose one (sparse, clumpy
+    // systems, where 8 Morton-consecutive beads can be 30 A apart) would
+    // otherwise scan every cell of its whole grown box.  Groups' entries are
+    // appended in turn; a j seen by two groups appears twice with disjoint
+    // masks.
     while (todo != 0u)
     {
-        const uint32_t tag0 = __shfl(tag, __ffs(todo) - 1, 64);
-        const uint32_t vmask = (uint32_t)(__ballot(lane < CLUSTER && ((todo >> lane) & 1u) && tag == tag0) & 0xFFull);
+        const int f = __ffs(todo) - 1;
+        const uint32_t tag0 = __shfl(tag, f, 64);
+        const float fx = __shfl(pi.x, f, 64), fy = __shfl(pi.y, f, 64), fz = __shfl(pi.z, f, 64);
+        // the first particle always belongs to its group (also for a
+        // non-finite position, so that the loop ends)
+        const bool near = lane == f ||
+                          (fabsf(pi.x - fx) <= a.radius && fabsf(pi.y - fy) <= a.radius && fabsf(pi.z - fz) <= a.radius);
+        const uint32_t vmask =
+            (uint32_t)(__ballot(lane < CLUSTER && ((todo >> lane) & 1u) && tag == tag0 && near) & 0xFFull);
         todo &= ~vmask;
         float xi[CLUSTER], yi[CLUSTER], zi[CLUSTER];
         float bmin[3] = {INFINITY, INFINITY, INFINITY}, bmax[3] = {-INFINITY, -INFINITY, -INFINITY};

@@ -674,23 +674,41 @@ void launch_adj_spring(const AdjSpringArgs &a, int32_t group, int32_t nb, hipStr
     }
 }
 
-void launch_spring_grads(const int32_t *own_e, float *dke, float *dle, int64_t m, double *dk, double *dl0,
+void launch_spring_grads(const int32_t *slot_e, float *dke, float *dle, int64_t ns, double *dk, double *dl0,
                          hipStream_t)
 {
-    need(own_e, sizeof(int32_t) * m, "spring_grads: own_e");
-    need(dk, sizeof(double) * m, "spring_grads: dk");
-    need(dl0, sizeof(double) * m, "spring_grads: dl0");
+    need(slot_e, sizeof(int32_t) * ns, "spring_grads: slot_e");
+    need(dk, sizeof(double) * ns, "spring_grads: dk");
+    need(dl0, sizeof(double) * ns, "spring_grads: dl0");
+    for (int64_t r = 0; r < ns; ++r)
+    {
+        const int32_t e = slot_e[r];
+        if (r > 0 && e <= slot_e[r - 1])
+            hc_fail("spring_grads: slots not in edge order");
+        need(dke + e, sizeof(float), "spring_grads: dke");
+        need(dle + e, sizeof(float), "spring_grads: dle");
+        dk[r] += dke[e];
+        dl0[r] += dle[e];
+        dke[e] = dle[e] = 0.0f;
+    }
+}
+
+void launch_spring_out(const int32_t *slot_of, int64_t m, const double *dk, const double *dl0, double *outk,
+                       double *outl, hipStream_t)
+{
+    need(slot_of, sizeof(int32_t) * m, "spring_out: slot_of");
+    need(outk, sizeof(double) * m, "spring_out: outk");
+    need(outl, sizeof(double) * m, "spring_out: outl");
     for (int64_t s = 0; s < m; ++s)
     {
-        const int32_t e = own_e[s];
-        if (e >= 0)
+        const int32_t r = slot_of[s];
+        if (r >= 0)
         {
-            need(dke + e, sizeof(float), "spring_grads: dke");
-            need(dle + e, sizeof(float), "spring_grads: dle");
-            dk[s] += dke[e];
-            dl0[s] += dle[e];
-            dke[e] = dle[e] = 0.0f;
+            need(dk + r, sizeof(double), "spring_out: dk");
+            need(dl0 + r, sizeof(double), "spring_out: dl0");
         }
+        outk[s] = r >= 0 ? dk[r] : 0.0;
+        outl[s] = r >= 0 ? dl0[r] : 0.0;
     }
 }
 
