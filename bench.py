@@ -285,6 +285,8 @@ def main():
                 Eg = E if off is None else Es[int(np.searchsorted(off, sl.start))]
                 erel = max(erel, max(abs(Eg[k] - Ed[k]) / max(abs(Ed[k]), 1e-3) for k in range(6)))
                 o.close()
+            if args.workload == 1:
+                out["trajectory_err_vs_fp64"] = trajectory_check(args.seed)
             out["force_err_vs_fp64"] = {"per_particle_max": per_max, "global_rel_l2": (glob_num / glob_den) ** 0.5,
                                         "energy_rel_max": erel, "forces": "all terms but the viscous one",
                                         "state": f"after {args.warmup + args.steps + min(args.steps, 50)} steps",
@@ -317,6 +319,48 @@ def main():
             if REF_OVER_ORACLE:
                 out["cpu_baseline"]["reference_equivalent"] = done / dt / REF_OVER_ORACLE
     print(json.dumps(out), flush=True)
+
+
+def trajectory_check(seed, n=1000, marks=(1, 10, 100, 1000)):
+    """Position deviation from the FP64 oracle's trajectory after 1, 10, 100
+    and 1000 steps of a workload-1 network of n beads: the engine, the engine
+    with compensated positions, and the reference's own float path (the
+    float-faithful oracle) as the yardstick."""
+    from biospring_amd import workloads
+    from biospring_amd.network import SpringNetwork
+    from oracle import pyoracle as po
+
+    t = time.perf_counter()
+    s = workloads.generate(1, seed=seed, n=n)
+    runs = {}
+    for name, comp in (("gpu", 0), ("gpu_compensated", 1)):
+        p = s.params.copy()
+        p.compensated_positions = comp
+        net = SpringNetwork.from_system(s, params=p)
+        done, xs = 0, []
+        for m in marks:
+            net.run(m - done)
+            done = m
+            xs.append(net.getState()[0].astype(np.float64))
+        runs[name] = xs
+        net.close()
+    for name, mode in (("fp64", po.F64), ("reference_float_path", po.F32)):
+        o = po.Oracle(s)
+        done, xs = 0, []
+        for m in marks:
+            o.step(mode, m - done)
+            done = m
+            xs.append(o.state(mode)[0].astype(np.float64))
+        runs[name] = xs
+        o.close()
+    out = {"n_beads": n, "steps": list(marks)}
+    for name in ("gpu", "gpu_compensated", "reference_float_path"):
+        d = [np.linalg.norm(a - b, axis=1) for a, b in zip(runs[name], runs["fp64"])]
+        out[name] = {"rms_A": [float(np.sqrt(np.mean(v * v))) for v in d], "max_A": [float(v.max()) for v in d]}
+    log(f"trajectory check {time.perf_counter() - t:.1f}s: max dev at {marks[-1]} steps gpu "
+        f"{out['gpu']['max_A'][-1]:.2e}, compensated {out['gpu_compensated']['max_A'][-1]:.2e}, "
+        f"reference float path {out['reference_float_path']['max_A'][-1]:.2e} A")
+    return out
 
 
 def bench_adjoint(args):
