@@ -39,7 +39,7 @@ WORKLOADS = {
     2: "{n} CG beads, IMPALA curved double membrane (R 200 A, offsets 30 A), 4 rigid bodies, "
        "springs < 10 A, linear steric 4 A",
     3: "512 independent CA networks ({n} beads in all, 200-20,000 each, springs < 9-14 A), "
-       "linear steric 4 A + Coulomb 16 A, one launch sequence replayed from a HIP graph",
+       "linear steric 4 A + Coulomb 16 A, one launch sequence replayed from a HIP graph, autotuned launch configuration",
     4: "adjoint of {T} steps (checkpoints every {K}) of a {n}-bead network, Amber 12-6 + Coulomb eps(r)=r, "
        "12 A cutoffs, springs < 10 A; loss = MSD to a target displaced along 3 smooth modes (5 A)",
 }
@@ -133,7 +133,8 @@ def main():
                          "200-step trajectory loss on a 100k-bead network (--steps is the trajectory length)")
     ap.add_argument("--checkpoint", type=int, default=20, help="workload 4: checkpoint interval")
     ap.add_argument("--systems", type=int, default=512, help="workload 3: number of systems")
-    ap.add_argument("--autotune", action="store_true", help="pick the launch configuration first")
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick the launch configuration first (always done for workload 3)")
     ap.add_argument("--n", type=int, default=0, help="bead count (default: the workload's own)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-steps", type=int, default=8)
@@ -176,7 +177,7 @@ def main():
     net.setup()
     log(f"setup {time.perf_counter() - t:.1f}s")
     tuned = None
-    if args.autotune:
+    if args.autotune or args.workload == 3:
         t = time.perf_counter()
         tuned = net.autotune(10)
         log(f"autotune {tuned} in {time.perf_counter() - t:.1f}s")
@@ -241,7 +242,8 @@ def main():
                    "n_beads": n, "rigid_bodies": [len(g) for g in bodies],
                    "systems": 1 if off is None else len(off) - 1,
                    "graph_steps": int(s.params.graph_steps) if tuned is None else tuned[1],
-                   "spring_lanes": None if tuned is None else tuned[0], "springs": s.m, "directed_edges": edges, "unique_pairs": pairs_unique,
+                   "spring_lanes": None if tuned is None else tuned[0],
+                   "pair_layout": int(s.params.pair_layout) if tuned is None else tuned[2], "springs": s.m, "directed_edges": edges, "unique_pairs": pairs_unique,
                    "seed": args.seed, "digest": s.digest(),
                    "rebuilds_in_timed_region": cnt1["rebuilds"] - cnt0["rebuilds"],
                    "list_capacity": cnt1["list_cap"]},

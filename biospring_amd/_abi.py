@@ -16,7 +16,7 @@ GEN_PATH = os.path.join(_HERE, "libbsgen.so")
 BS_OK, BS_ERR_ARG, BS_ERR_NONFINITE, BS_ERR_HIP, BS_ERR_STATE, BS_ERR_NOGPU = range(6)
 STERIC_LINEAR, STERIC_AMBER_12_6, STERIC_LEWITT_8_6, STERIC_ZACHARIAS_8_6 = range(4)
 DIELECTRIC_CONSTANT, DIELECTRIC_DISTANCE = 0, 1
-PAIR_CLUSTERS, PAIR_ROWS = 0, 1
+PAIR_CLUSTERS, PAIR_ROWS, PAIR_QUEUES = 0, 1, 2
 FLAG_STATIC = 1
 E_SPRING, E_ELECTROSTATIC, E_STERIC, E_KINETIC, E_IMP, E_HYDROPHOBIC = range(6)
 NUM_ENERGIES = 6
@@ -155,7 +155,7 @@ def bind(L):
     L.bs_num_systems.argtypes = [_vp]
     L.bs_num_systems.restype = ctypes.c_int32
     L.bs_get_system_energies.argtypes = [_vp, _f64p]
-    L.bs_autotune.argtypes = [_vp, ctypes.c_int32, _i32p, _i32p]
+    L.bs_autotune.argtypes = [_vp, ctypes.c_int32, _i32p, _i32p, _i32p]
     L.bs_adjoint.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, _f32p, _f32p, _f64p, _f64p, _f64p, _f64p]
     return L
 

@@ -616,6 +616,7 @@ void launch_pairs(bs_handle *h, bool energy)
         a.rows = l.rows;
         a.rlen = l.rlen;
         a.rcap = l.rcap;
+        a.queues = h->p.pair_layout == BS_PAIR_QUEUES ? 1 : 0;
         a.n = n;
         a.ncl = h->ncl;
         a.cap = l.cap;
@@ -2379,7 +2380,7 @@ int bs_adjoint(bs_handle *h, int32_t nsteps, int32_t checkpoint_every, const flo
     return BS_OK;
 }
 
-int bs_autotune(bs_handle *h, int32_t steps, int32_t *group_out, int32_t *graph_out)
+int bs_autotune(bs_handle *h, int32_t steps, int32_t *group_out, int32_t *graph_out, int32_t *layout_out)
 {
     if (!h)
         return BS_ERR_ARG;
@@ -2393,16 +2394,32 @@ int bs_autotune(bs_handle *h, int32_t steps, int32_t *group_out, int32_t *graph_
     rc = bs_get_state(h, x.data(), v.data());
     if (rc)
         return rc;
-    static const int32_t groups[] = {4, 8, 16, 32};
-    static const int32_t graphs[] = {0, 16};
+    // first the pair layout (at the current lanes and graph setting), then
+    // lanes per spring row x graph replay with that layout
+    struct Trial
+    {
+        int32_t layout, group, graph;
+    };
+    std::vector<Trial> trials;
+    for (int32_t lay : {BS_PAIR_CLUSTERS, BS_PAIR_QUEUES})
+        trials.push_back({lay, h->group, h->p.graph_steps});
+    for (int32_t gs : {0, 16})
+        for (int32_t g : {4, 8, 16, 32})
+            trials.push_back({-1, g, gs});
     double best = INFINITY;
-    int32_t bg = h->group, bgr = h->p.graph_steps;
-    for (int32_t gs : graphs)
-        for (int32_t g : groups)
+    int32_t bg = h->group, bgr = h->p.graph_steps, blay = h->p.pair_layout == BS_PAIR_ROWS ? BS_PAIR_CLUSTERS
+                                                                                        : h->p.pair_layout;
+    for (size_t ti = 0; ti < trials.size(); ++ti)
+    {
+        if (ti == 2)
+            best = INFINITY; // second phase: compare among itself
+        const int32_t lay = trials[ti].layout >= 0 ? trials[ti].layout : blay;
+        const int32_t g = trials[ti].group, gs = trials[ti].graph;
         {
             bs_handle *t = nullptr;
             bs_params p = h->p;
             p.graph_steps = gs;
+            p.pair_layout = lay;
             if (bs_create(&p, &t) != BS_OK)
                 return fail(h, BS_ERR_HIP, "bs_autotune: could not create a trial handle");
             t->n = h->n;
@@ -2442,8 +2459,13 @@ int bs_autotune(bs_handle *h, int32_t steps, int32_t *group_out, int32_t *graph_
                     hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms < best)
                 {
                     best = ms;
-                    bg = g;
-                    bgr = gs;
+                    if (ti < 2)
+                        blay = lay;
+                    else
+                    {
+                        bg = g;
+                        bgr = gs;
+                    }
                 }
             }
             if (e0)
@@ -2454,15 +2476,20 @@ int bs_autotune(bs_handle *h, int32_t steps, int32_t *group_out, int32_t *graph_
             if (trc != BS_OK)
                 return fail(h, trc, "bs_autotune: trial failed");
         }
+    }
     h->group = bg;
     h->spring_blocks = (int32_t)((h->n * h->group + BLOCK - 1) / BLOCK);
-    if (h->p.graph_steps != bgr)
+    if (h->p.graph_steps != bgr || h->p.pair_layout != blay)
         drop_graph(h);
     h->p.graph_steps = bgr;
+    if (h->p.pair_layout != BS_PAIR_ROWS) // row buffers are sized at setup
+        h->p.pair_layout = blay;
     if (group_out)
         *group_out = bg;
     if (graph_out)
         *graph_out = bgr;
+    if (layout_out)
+        *layout_out = h->p.pair_layout;
     return BS_OK;
 }
 

@@ -91,6 +91,7 @@ struct PairArgs
     const uint32_t *rows; // nullable: per-particle lists (row layout), else the cluster lists
     const int32_t *rlen;  // [64 * row blocks] entries per particle
     int32_t rcap;         // entries per particle row
+    int32_t queues;       // cluster lists read through per-particle LDS queues (k_pair_q)
     int32_t n, ncl, cap;
     int32_t accumulate; // add to fnb instead of overwriting
     int32_t energy;     // also write epp (the last step of a call only)

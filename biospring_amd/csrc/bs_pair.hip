@@ -737,6 +737,117 @@ void launch_list_build(const ListArgs &a, hipStream_t s)
 }
 
 // ---------------------------------------------------------------------------
+// Queue layout (BS_PAIR_QUEUES): the cluster lists, read 64 words at a time.
+// Each word's j data is loaded once (one lane per word) into LDS, and each
+// of the 8 cluster particles gets an LDS queue of the words it sees (one
+// ballot per particle).  The 8 lanes of a particle then drain its queue, 8
+// entries per round, so the lanes evaluate listed pairs instead of idling
+// on entries their particle does not see (an entry is seen by about half of
+// a cluster's particles when springs empty the inner shell).  A chunk takes
+// as many rounds as its fullest queue.  Each lane sums a fixed subset of its
+// particle's entries in a fixed order; the 8 partial sums are combined as in
+// k_pair: deterministic.  The next chunk's words and j data are loaded
+// while the current one is evaluated.
+// ---------------------------------------------------------------------------
+template <int SMODE, int EMODE, int HYDRO, bool ENERGY>
+__global__ void __launch_bounds__(BLOCK) k_pair_q(PairArgs a)
+{
+    __shared__ float4 qpos[WAVES][64], qprm[WAVES][64];
+    __shared__ uint8_t qidx[WAVES][CLUSTER][64];
+    __shared__ int32_t qcnt[WAVES][CLUSTER];
+    const int wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int32_t ci = blockIdx.x * WAVES + wv;
+    if (ci >= a.ncl)
+        return; // whole wave
+    const int32_t il = lane & 7, sl = lane >> 3;
+    const int32_t i = ci * CLUSTER + il;
+    const bool iv = i < a.n;
+    const float4 pi = iv ? a.pos[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 ri = iv ? a.jprm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const uint32_t fli = fbits(ri.w);
+    const double qe = (double)pi.w * a.ke;
+    const double qf = (double)pi.w * a.kf;
+    const float cs = (float)a.k.c_spring;
+    const uint64_t below = (1ull << lane) - 1ull;
+    double es = 0.0, ee = 0.0, eh = 0.0;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    const int32_t cnt = a.cln[ci];
+    const uint32_t *L = a.cl + (size_t)ci * a.cap;
+    // chunk 0 in flight
+    uint32_t w = lane < cnt ? L[lane] : 0u;
+    float4 pj = a.pos[w & CL_JMASK], rj = a.jprm[w & CL_JMASK];
+    for (int32_t t0 = 0; t0 < cnt; t0 += 64)
+    {
+        qpos[wv][lane] = pj;
+        qprm[wv][lane] = rj;
+#pragma unroll
+        for (int k = 0; k < CLUSTER; ++k)
+        {
+            const bool on = (w >> (24 + k)) & 1u;
+            const uint64_t bal = __ballot(on);
+            if (on)
+                qidx[wv][k][__popcll(bal & below)] = (uint8_t)lane;
+            if (lane == k)
+                qcnt[wv][k] = __popcll(bal);
+        }
+        // next chunk in flight while this one is evaluated
+        const int32_t tn = t0 + 64 + lane;
+        w = tn < cnt ? L[tn] : 0u;
+        pj = a.pos[w & CL_JMASK];
+        rj = a.jprm[w & CL_JMASK];
+        wave_lds_sync();
+        const int32_t mine = qcnt[wv][il];
+        int32_t most = mine;
+        for (int off = 1; off < 8; off <<= 1)
+            most = max(most, __shfl_xor(most, off, 64));
+        for (int32_t e = sl; e < most; e += 8)
+        {
+            const bool valid = e < mine;
+            const int32_t src = valid ? qidx[wv][il][e] : 0;
+            const float4 qj = qpos[wv][src], sj = qprm[wv][src];
+            interact<SMODE, EMODE, HYDRO, ENERGY>(valid, pi, ri, fli, qj, sj, a, cs, qe, qf, fx, fy, fz, es, ee, eh);
+        }
+        wave_lds_sync();
+    }
+    for (int off = 8; off < 64; off <<= 1)
+    {
+        fx += __shfl_xor(fx, off, 64);
+        fy += __shfl_xor(fy, off, 64);
+        fz += __shfl_xor(fz, off, 64);
+        if (ENERGY)
+        {
+            es += __shfl_xor(es, off, 64);
+            ee += __shfl_xor(ee, off, 64);
+            eh += __shfl_xor(eh, off, 64);
+        }
+    }
+    if (sl != 0 || !iv)
+        return;
+    if (ENERGY)
+    {
+        double4 en = make_double4(es, ee, eh, 0.0);
+        if (a.accumulate)
+        {
+            const double4 o = a.epp[i];
+            en.x += o.x;
+            en.y += o.y;
+            en.z += o.z;
+        }
+        a.epp[i] = en;
+    }
+    double4 f = make_double4(fx, fy, fz, 0.0);
+    if (a.accumulate)
+    {
+        const double4 o = a.fnb[i];
+        f.x += o.x;
+        f.y += o.y;
+        f.z += o.z;
+    }
+    a.fnb[i] = f;
+}
+
+// ---------------------------------------------------------------------------
 // Row layout.  With springs excluded from the pair terms, the pairs a
 // particle sees lie in a shell (workload 1: 8 A to the 13 A list radius)
 // thinner than twice a cluster's size, so an entry of a cluster list is seen
@@ -874,6 +985,14 @@ __global__ void __launch_bounds__(BLOCK) k_pair_rows(PairArgs a)
 
 template <int S, int E, int H> static void pair3(const PairArgs &a, int32_t nb, hipStream_t s)
 {
+    if (a.queues)
+    {
+        if (a.energy)
+            k_pair_q<S, E, H, true><<<nb, BLOCK, 0, s>>>(a);
+        else
+            k_pair_q<S, E, H, false><<<nb, BLOCK, 0, s>>>(a);
+        return;
+    }
     if (a.rows)
     {
         const int32_t nrb = (a.n + 63) / 64, blocks = (nrb + WAVES - 1) / WAVES;

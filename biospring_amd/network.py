@@ -232,10 +232,11 @@ class SpringNetwork:
 
     def autotune(self, steps=20):
         """Picks the fastest launch configuration on a copy of the system;
-        returns (lanes per spring row, graph steps)."""
-        g, gr = ctypes.c_int32(), ctypes.c_int32()
-        check(self._L.bs_autotune(self._h, int(steps), ctypes.byref(g), ctypes.byref(gr)), self._h, "bs_autotune")
-        return int(g.value), int(gr.value)
+        returns (lanes per spring row, graph steps, pair layout)."""
+        g, gr, lay = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        check(self._L.bs_autotune(self._h, int(steps), ctypes.byref(g), ctypes.byref(gr), ctypes.byref(lay)),
+              self._h, "bs_autotune")
+        return int(g.value), int(gr.value), int(lay.value)
 
     # -- gradients (no reference counterpart) --
     def adjoint(self, nsteps, target, checkpoint_every=0):

@@ -44,6 +44,7 @@ extern "C" {
 
 #define BS_PAIR_CLUSTERS 0  /* bs_params.pair_layout */
 #define BS_PAIR_ROWS     1
+#define BS_PAIR_QUEUES   2
 
 /* ---- dielectric models ------------------------------------------------- */
 #define BS_DIELECTRIC_CONSTANT 0  /* reference: electrostatic.hpp:16-49 */
@@ -136,7 +137,10 @@ typedef struct bs_params {
        stay idle.  BS_PAIR_ROWS: the cluster lists expanded at each rebuild
        into one list per particle, every lane evaluating a listed pair at one
        gather per pair (slower where gathers, not arithmetic, bound the
-       kernel).  Forces agree to FP64 rounding either way. */
+       kernel).  BS_PAIR_QUEUES: the cluster lists read 64 words at a time,
+       each word's j data loaded once into LDS and queued for the particles
+       that see it, so that lanes evaluate listed pairs at the cluster
+       layout's load count.  Forces agree to FP64 rounding in every layout. */
     int32_t pair_layout;
 } bs_params;
 
@@ -375,10 +379,12 @@ int bs_adjoint(bs_handle *h, int32_t nsteps, int32_t checkpoint_every, const flo
 /* ---- tuning ---------------------------------------------------------------------- */
 
 /* Times `steps` steps of a copy of the current system for each launch
- * configuration (lanes per spring row 4/8/16/32; direct launches or a 16-step
- * graph) and keeps the fastest for h; h's state is not advanced.  The chosen
- * values go to *group_out / *graph_out (nullable).  No reference counterpart. */
-int bs_autotune(bs_handle *h, int32_t steps, int32_t *group_out, int32_t *graph_out);
+ * configuration and keeps the fastest for h; h's state is not advanced.
+ * First the pair layout (BS_PAIR_CLUSTERS or BS_PAIR_QUEUES; a handle set to
+ * BS_PAIR_ROWS keeps it), then lanes per spring row (4/8/16/32) x direct
+ * launches or a 16-step graph.  The chosen values go to *group_out,
+ * *graph_out, *layout_out (nullable).  No reference counterpart. */
+int bs_autotune(bs_handle *h, int32_t steps, int32_t *group_out, int32_t *graph_out, int32_t *layout_out);
 
 #ifdef __cplusplus
 }

@@ -68,3 +68,25 @@ def test_batch_graph_replay_matches_direct_launches():
         xb, vb = b.getState()
         assert np.array_equal(xa, xb) and np.array_equal(va, vb), n
         assert np.array_equal(a.getSystemEnergies(), b.getSystemEnergies()), n
+
+
+def test_autotune_picks_a_valid_configuration():
+    """bs_autotune (lanes per spring row, graph replay, pair layout) leaves the
+    state alone and the tuned handle steps to the same positions as an
+    untuned one (to FP64-summation rounding)."""
+    s, off, _ = workloads.batch(6, seed=8, nmax=1500)
+    a = SpringNetwork.from_system(s)
+    a.setSystems(off)
+    x0, v0 = a.getState()
+    g, gr, lay = a.autotune(5)
+    _log(f"autotune: lanes {g}, graph {gr}, layout {lay}")
+    assert g in (4, 8, 16, 32) and gr in (0, 16) and lay in (0, 2)
+    x1, v1 = a.getState()
+    assert np.array_equal(x0, x1) and np.array_equal(v0, v1)
+    b = SpringNetwork.from_system(s)
+    b.setSystems(off)
+    a.run(40)
+    b.run(40)
+    xa, _ = a.getState()
+    xb, _ = b.getState()
+    assert np.abs(xa - xb).max() <= 1e-4

@@ -177,24 +177,25 @@ def test_determinism():
 
 @pytest.mark.parametrize("name", ["enm2k", "lj20k"])
 def test_pair_layouts_agree(name):
-    """Cluster lists (the default) and row lists give the same forces to FP64
-    rounding (the sums associate differently)."""
+    """Cluster lists (the default), row lists and queued cluster lists give the
+    same forces to FP64 rounding (the sums associate differently)."""
     s = _system(name)
     res = {}
-    for layout in (_abi.PAIR_CLUSTERS, _abi.PAIR_ROWS):
+    for layout in (_abi.PAIR_CLUSTERS, _abi.PAIR_ROWS, _abi.PAIR_QUEUES):
         p = s.params.copy()
         p.pair_layout = layout
         p.viscosity_enable = 0
         net = SpringNetwork.from_system(s, params=p)
         net.run(5)
         res[layout] = net.computeForces()
-    Fr, Er = res[_abi.PAIR_ROWS]
     Fc, Ec = res[_abi.PAIR_CLUSTERS]
     scale = np.abs(Fc).max()
-    _log(f"  {name}: rows vs clusters max |dF| / max|F| = {np.abs(Fr - Fc).max() / scale:.2e}")
-    assert np.abs(Fr - Fc).max() <= 1e-6 * scale
-    for slot in range(6):
-        assert abs(Er[slot] - Ec[slot]) <= 1e-12 * max(abs(Ec[slot]), 1e-3)
+    for layout in (_abi.PAIR_ROWS, _abi.PAIR_QUEUES):
+        Fr, Er = res[layout]
+        _log(f"  {name}: layout {layout} vs clusters max |dF| / max|F| = {np.abs(Fr - Fc).max() / scale:.2e}")
+        assert np.abs(Fr - Fc).max() <= 1e-6 * scale
+        for slot in range(6):
+            assert abs(Er[slot] - Ec[slot]) <= 1e-12 * max(abs(Ec[slot]), 1e-3)
 
 
 def test_nonfinite_reported():
