@@ -97,6 +97,17 @@ struct bs_handle
     float *sp_k = nullptr, *sp_l0 = nullptr;
     int64_t nedges = 0;
     int32_t ncl = 0;                                  // i-clusters of 8
+    // bs_adjoint_forward's record: checkpoints, the last segment's positions
+    // and the state at x_T, valid while the state is unchanged
+    struct Tape
+    {
+        bool ready = false;
+        int32_t T = 0, K = 0, ncp = 0;
+        int ext = 0;
+        int64_t cap_cp = 0, cap_seg = 0;
+        float4 *cp_pos = nullptr, *cp_vel = nullptr, *seg = nullptr, *fin_pos = nullptr, *fin_vel = nullptr;
+        int32_t *cp_cnt = nullptr, *fin_cnt = nullptr;
+    } tape;
     // bs_adjoint: springs with a dynamic end in the order of their owning CSR
     // edge ("slots"): slot_e[r] = that edge (ascending), slot_of[s] = slot of
     // spring s or -1
@@ -194,9 +205,19 @@ void drop_graph(bs_handle *h)
     h->graph_len = 0;
 }
 
+void tape_free(bs_handle::Tape &tp)
+{
+    for (void *q : {(void *)tp.cp_pos, (void *)tp.cp_vel, (void *)tp.seg, (void *)tp.fin_pos, (void *)tp.fin_vel,
+                    (void *)tp.cp_cnt, (void *)tp.fin_cnt})
+        if (q)
+            (void)hipFree(q);
+    tp = bs_handle::Tape{};
+}
+
 void free_device(bs_handle *h)
 {
     drop_graph(h);
+    tape_free(h->tape);
     auto f = [](void *p) {
         if (p)
             (void)hipFree(p);
@@ -849,6 +870,8 @@ int capture_graph(bs_handle *h, int32_t G)
 
 int upload_ext(bs_handle *h, const float *ext3)
 {
+    if (ext3)
+        h->tape.ready = false; // a recorded forward pass used the force it had
     h->ext_uploaded = 0;
     if (!ext3)
         return BS_OK;
@@ -940,6 +963,286 @@ int build_spring_owners(bs_handle *h)
         HIPCHK(h, hipMemcpy(h->slot_e, se.data(), sizeof(int32_t) * se.size(), hipMemcpyHostToDevice));
     HIPCHK(h, hipMemcpy(h->slot_of, so.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice));
     h->nslots = (int64_t)se.size();
+    return BS_OK;
+}
+
+void launch_adj_pairs(bs_handle *h, const float4 *w, double4 *hw, double *dq);
+
+// Terms the adjoint covers; BS_ERR_ARG otherwise.
+int adjoint_check(bs_handle *h)
+{
+    const bs_params &P = h->p;
+    if (P.imp_enable || P.hydrophobicity_enable)
+        return fail(h, BS_ERR_ARG, "bs_adjoint: IMPALA and the pseudo-hydrophobic term have no adjoint");
+    if (!h->bodies.empty())
+        return fail(h, BS_ERR_ARG, "bs_adjoint: rigid bodies have no adjoint");
+    if ((P.electrostatic_enable && P.potentialgrid_enable && h->grids[BS_GRID_POTENTIAL].set) ||
+        (P.densitygrid_enable && h->grids[BS_GRID_DENSITY].set))
+        return fail(h, BS_ERR_ARG, "bs_adjoint: field grids have no adjoint");
+    if (P.compensated_positions)
+        return fail(h, BS_ERR_ARG, "bs_adjoint: needs compensated_positions = 0");
+    return BS_OK;
+}
+
+// Forward pass of the adjoint: T steps from the current state, which
+// advances as with bs_step.  The state is saved (and the lists rebuilt)
+// every K steps, so that recomputing a segment repeats it bit for bit, and
+// the last segment's positions are kept as they pass.  The record ("tape")
+// stays in the handle until the state changes.
+int adjoint_record(bs_handle *h, int32_t T, int32_t checkpoint_every)
+{
+    auto &tp = h->tape;
+    tp.ready = false;
+    const int64_t n = h->n;
+    int32_t K = checkpoint_every > 0 ? checkpoint_every : (int32_t)std::ceil(std::sqrt((double)std::max(T, 1)));
+    K = std::max(1, std::min(K, std::max(T, 1)));
+    const int32_t ncp = (T + K - 1) / K;
+    const size_t s4 = sizeof(float4) * (size_t)n;
+    hipStream_t st = h->stream;
+    hipError_t e = hipSuccess;
+    // buffers grow as needed and are reused
+    const int64_t need_cp = (int64_t)std::max(ncp, 1), need_seg = K;
+    if (need_cp > tp.cap_cp || need_seg > tp.cap_seg || !tp.fin_pos)
+    {
+        tape_free(tp);
+        tp.cp_pos = dalloc<float4>((size_t)(n * need_cp), e);
+        HIPCHK(h, e);
+        tp.cp_vel = dalloc<float4>((size_t)(n * need_cp), e);
+        HIPCHK(h, e);
+        tp.cp_cnt = dalloc<int32_t>((size_t)(4 * need_cp), e);
+        HIPCHK(h, e);
+        tp.seg = dalloc<float4>((size_t)(n * need_seg), e);
+        HIPCHK(h, e);
+        tp.fin_pos = dalloc<float4>((size_t)n, e);
+        HIPCHK(h, e);
+        tp.fin_vel = dalloc<float4>((size_t)n, e);
+        HIPCHK(h, e);
+        tp.fin_cnt = dalloc<int32_t>(4, e);
+        HIPCHK(h, e);
+        tp.cap_cp = need_cp;
+        tp.cap_seg = need_seg;
+    }
+    const int32_t s_last = (ncp - 1) * K;
+    for (int32_t s = 0; s < T; ++s)
+    {
+        if (s >= s_last)
+            HIPCHK(h, hipMemcpyAsync(tp.seg + (size_t)(s - s_last) * n, h->pos[h->cur], s4, hipMemcpyDeviceToDevice,
+                                     st));
+        if (s % K == 0)
+        {
+            const size_t c = (size_t)(s / K);
+            HIPCHK(h, hipMemcpyAsync(tp.cp_pos + c * n, h->pos[h->cur], s4, hipMemcpyDeviceToDevice, st));
+            HIPCHK(h, hipMemcpyAsync(tp.cp_vel + c * n, h->vel, s4, hipMemcpyDeviceToDevice, st));
+            HIPCHK(h, hipMemcpyAsync(tp.cp_cnt + 4 * c, h->counters, 4 * sizeof(int32_t), hipMemcpyDeviceToDevice,
+                                     st));
+            launch_fill_i32(h->status, 1, 1, nullptr, st);
+        }
+        const int rc = prof_fold(h);
+        if (rc)
+            return rc;
+        launch_step(h, s + 1 == T);
+    }
+    if (T > 0)
+    {
+        const int rc = finish_call(h);
+        if (rc)
+            return rc;
+        h->steps_done += T;
+        h->last_call_forces_only = false;
+    }
+    HIPCHK(h, hipMemcpyAsync(tp.fin_pos, h->pos[h->cur], s4, hipMemcpyDeviceToDevice, st));
+    HIPCHK(h, hipMemcpyAsync(tp.fin_vel, h->vel, s4, hipMemcpyDeviceToDevice, st));
+    HIPCHK(h, hipMemcpyAsync(tp.fin_cnt, h->counters, 4 * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
+    tp.T = T;
+    tp.K = K;
+    tp.ncp = ncp;
+    tp.ext = h->ext_uploaded;
+    tp.ready = true;
+    return BS_OK;
+}
+
+// Adjoint state of one sweep: lx, lv (FP64) and the weights w of the step
+// being reversed (FP32).
+struct AdjSeed
+{
+    double4 *lx = nullptr, *lv = nullptr;
+    float4 *w = nullptr;
+};
+
+int adjoint_alloc_seed(bs_handle *h, Scratch &S, AdjSeed &seed)
+{
+    hipError_t e = hipSuccess;
+    seed.lx = S.get<double4>((size_t)h->n, e);
+    HIPCHK(h, e);
+    seed.lv = S.get<double4>((size_t)h->n, e);
+    HIPCHK(h, e);
+    seed.w = S.get<float4>((size_t)h->n, e);
+    HIPCHK(h, e);
+    return BS_OK;
+}
+
+// Reverse sweep over the recorded forward pass from the seeded state at x_T:
+//   a = lv + dt lx,  w = dt/m a,  lx += H(x_s) w,  lv = (1 - gamma dt/m) a,
+// segment by segment (earlier segments recomputed from their checkpoints),
+// then the state goes back to x_T; the record stays usable.
+int adjoint_sweep(bs_handle *h, Scratch &S, AdjSeed &seed, double *dk, double *dl0, double *dq)
+{
+    const auto &tp = h->tape;
+    const bs_params &P = h->p;
+    const int64_t n = h->n, E = h->nedges, m = (int64_t)h->s1.size();
+    const int32_t T = tp.T, K = tp.K, ncp = tp.ncp;
+    const size_t s4 = sizeof(float4) * (size_t)n;
+    hipStream_t st = h->stream;
+    hipError_t e = hipSuccess;
+    int rc = BS_OK;
+    if ((dk || dl0) && m > 0)
+    {
+        rc = build_spring_owners(h);
+        if (rc)
+            return rc;
+    }
+    if (tp.ext != h->ext_uploaded)
+        return fail(h, BS_ERR_STATE, "bs_adjoint: external force changed since the forward pass");
+    double4 *hwp = nullptr;
+    float4 *wb[2] = {seed.w, nullptr};
+    float *dke = nullptr, *dle = nullptr;
+    double *dqd = nullptr;
+#define SALLOC(ptr, T_, cnt)                                                                                     \
+    do                                                                                                            \
+    {                                                                                                             \
+        ptr = S.get<T_>((size_t)(cnt), e);                                                                        \
+        HIPCHK(h, e);                                                                                             \
+    } while (0)
+    SALLOC(wb[1], float4, n);
+    if (!h->lists.empty())
+        SALLOC(hwp, double4, n);
+    // spring gradients: FP32 sums per owning edge over at most FLUSH steps,
+    // then added into FP64 totals per slot; per spring at the end
+    const bool want_springs = (dk || dl0) && m > 0;
+    const int64_t ns = std::max<int64_t>(h->nslots, 0);
+    double *dks = nullptr, *dls = nullptr, *outk = nullptr, *outl = nullptr;
+    if (want_springs)
+    {
+        SALLOC(dke, float, E);
+        SALLOC(dle, float, E);
+        SALLOC(dks, double, ns);
+        SALLOC(dls, double, ns);
+        SALLOC(outk, double, m);
+        SALLOC(outl, double, m);
+    }
+    SALLOC(dqd, double, n);
+#undef SALLOC
+    constexpr int32_t FLUSH = 64;
+    int32_t since_flush = 0;
+    const double dt = (double)P.timestep, gam = P.viscosity_enable ? (double)P.viscosity_value : 0.0;
+    if (want_springs)
+    {
+        HIPCHK(h, hipMemsetAsync(dke, 0, sizeof(float) * std::max<int64_t>(E, 1), st));
+        HIPCHK(h, hipMemsetAsync(dle, 0, sizeof(float) * std::max<int64_t>(E, 1), st));
+        HIPCHK(h, hipMemsetAsync(dks, 0, sizeof(double) * std::max<int64_t>(ns, 1), st));
+        HIPCHK(h, hipMemsetAsync(dls, 0, sizeof(double) * std::max<int64_t>(ns, 1), st));
+    }
+    HIPCHK(h, hipMemsetAsync(dqd, 0, sizeof(double) * n, st));
+    const double hs = 0.5 * std::max(P.list_skin, 0.0f);
+    const float half_skin2 = (float)(hs * hs * (1.0 - 1e-5));
+    int wc = 0;
+    for (int32_t c = ncp - 1; c >= 0; --c)
+    {
+        const int32_t s0 = c * K, s1 = std::min(T, s0 + K);
+        if (c < ncp - 1)
+        {
+            HIPCHK(h, hipMemcpyAsync(h->pos[h->cur], tp.cp_pos + (size_t)c * n, s4, hipMemcpyDeviceToDevice, st));
+            HIPCHK(h, hipMemcpyAsync(h->vel, tp.cp_vel + (size_t)c * n, s4, hipMemcpyDeviceToDevice, st));
+            HIPCHK(h, hipMemcpyAsync(h->counters, tp.cp_cnt + 4 * (size_t)c, 4 * sizeof(int32_t),
+                                     hipMemcpyDeviceToDevice, st));
+            launch_fill_i32(h->status, 1, 1, nullptr, st);
+        }
+        for (int32_t s = s0; s < s1 && c < ncp - 1; ++s)
+        {
+            HIPCHK(h, hipMemcpyAsync(tp.seg + (size_t)(s - s0) * n, h->pos[h->cur], s4, hipMemcpyDeviceToDevice, st));
+            if (s + 1 < s1)
+            {
+                rc = prof_fold(h);
+                if (rc)
+                    return rc;
+                launch_step(h, false);
+            }
+        }
+        for (int32_t s = s1 - 1; s >= s0; --s)
+        {
+            HIPCHK(h, hipMemcpyAsync(h->pos[h->cur], tp.seg + (size_t)(s - s0) * n, s4, hipMemcpyDeviceToDevice, st));
+            rc = prof_fold(h);
+            if (rc)
+                return rc;
+            const size_t e0 = h->prof ? prof_mark(h) : SIZE_MAX;
+            launch_drift_check(h->pos[h->cur], h->xref, (int32_t)n, half_skin2, h->status, st);
+            rebuild_chain(h, true);
+            const size_t e1 = h->prof ? prof_mark(h) : SIZE_MAX;
+            if (hwp)
+                launch_adj_pairs(h, wb[wc], hwp, dqd);
+            const size_t e2 = h->prof ? prof_mark(h) : SIZE_MAX;
+            AdjSpringArgs a{};
+            a.pos = h->pos[h->cur];
+            a.jprm = h->jprm;
+            a.iprm = h->iprm;
+            a.sp_ptr = h->sp_ptr;
+            a.sp_j = h->sp_j;
+            a.sp_k = h->sp_k;
+            a.sp_l0 = h->sp_l0;
+            a.w = wb[wc];
+            a.hwp = hwp;
+            a.lx = seed.lx;
+            a.lv = seed.lv;
+            a.w_next = wb[wc ^ 1];
+            a.dk = dke;
+            a.dl0 = dle;
+            a.n = (int32_t)n;
+            a.spring_enable = P.spring_enable;
+            a.spring_scale = P.spring_scale;
+            a.dt = dt;
+            a.gamma = gam;
+            a.k = h->k;
+            launch_adj_spring(a, h->group, h->spring_blocks, st);
+            if (want_springs && (++since_flush == FLUSH || s == 0))
+            {
+                launch_spring_grads(h->slot_e, dke, dle, ns, dks, dls, st);
+                since_flush = 0;
+            }
+            if (h->prof)
+            {
+                const size_t e3 = prof_mark(h);
+                prof_span(h, BS_PROF_REBUILD, e0, e1);
+                if (hwp)
+                    prof_span(h, BS_PROF_ADJ_PAIR, e1, e2);
+                prof_span(h, BS_PROF_ADJ_SPRING, e2, e3);
+            }
+            wc ^= 1;
+        }
+    }
+    // back to x_T (lists rebuilt at the next step)
+    HIPCHK(h, hipMemcpyAsync(h->pos[h->cur], tp.fin_pos, s4, hipMemcpyDeviceToDevice, st));
+    HIPCHK(h, hipMemcpyAsync(h->vel, tp.fin_vel, s4, hipMemcpyDeviceToDevice, st));
+    HIPCHK(h, hipMemcpyAsync(h->counters, tp.fin_cnt, 4 * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
+    launch_fill_i32(h->status, 1, 1, nullptr, st);
+
+    if (want_springs)
+        launch_spring_out(h->slot_of, m, dks, dls, outk, outl, st);
+    std::vector<double> dqh((size_t)n);
+    int32_t stv[4];
+    HIPCHK(h, hipMemcpyAsync(stv, h->status, sizeof stv, hipMemcpyDeviceToHost, st));
+    if (dq)
+        HIPCHK(h, hipMemcpyAsync(dqh.data(), dqd, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    if (want_springs && dk)
+        HIPCHK(h, hipMemcpyAsync(dk, outk, sizeof(double) * m, hipMemcpyDeviceToHost, st));
+    if (want_springs && dl0)
+        HIPCHK(h, hipMemcpyAsync(dl0, outl, sizeof(double) * m, hipMemcpyDeviceToHost, st));
+    HIPCHK(h, hipStreamSynchronize(st));
+    prof_collect(h);
+    if (stv[2] != 0)
+        return fail(h, BS_ERR_STATE, "neighbour list capacity exceeded (max %d > cap); rerun bs_setup", stv[3]);
+    if (dq)
+        for (int64_t t = 0; t < n; ++t)
+            dq[h->perm[(size_t)t]] = dqh[(size_t)t];
     return BS_OK;
 }
 
@@ -1535,6 +1838,7 @@ int bs_step(bs_handle *h, int32_t nsteps, const float *ext_force3)
     rc = upload_ext(h, ext_force3);
     if (rc)
         return rc;
+    h->tape.ready = false;
     int32_t s = 0;
     const int32_t G = (h->p.graph_steps + 1) / 2 * 2;
     if (G > 0 && !h->prof && nsteps > G)
@@ -1668,6 +1972,7 @@ int bs_set_state(bs_handle *h, const float *x3, const float *v3)
         std::memcpy(h->x.data(), x3, sizeof(float) * 3 * h->n);
     if (v3)
         std::memcpy(h->v.data(), v3, sizeof(float) * 3 * h->n);
+    h->tape.ready = false;
     if (!h->ready)
         return BS_OK;
     h->cur = 0;
@@ -2130,127 +2435,96 @@ int bs_get_system_energies(bs_handle *h, double *out)
     return BS_OK;
 }
 
-int bs_adjoint(bs_handle *h, int32_t nsteps, int32_t checkpoint_every, const float *target3, const float *ext_force3,
-               double *loss, double *dk, double *dl0, double *dq)
+int bs_adjoint_forward(bs_handle *h, int32_t nsteps, int32_t checkpoint_every, const float *ext_force3)
 {
     if (!h)
         return BS_ERR_ARG;
     if (nsteps < 0)
         return fail(h, BS_ERR_ARG, "bs_adjoint: negative step count");
-    if (!target3)
-        return fail(h, BS_ERR_ARG, "bs_adjoint: target positions required");
-    const bs_params &P = h->p;
-    if (P.imp_enable || P.hydrophobicity_enable)
-        return fail(h, BS_ERR_ARG, "bs_adjoint: IMPALA and the pseudo-hydrophobic term have no adjoint");
-    if (!h->bodies.empty())
-        return fail(h, BS_ERR_ARG, "bs_adjoint: rigid bodies have no adjoint");
-    if ((P.electrostatic_enable && P.potentialgrid_enable && h->grids[BS_GRID_POTENTIAL].set) ||
-        (P.densitygrid_enable && h->grids[BS_GRID_DENSITY].set))
-        return fail(h, BS_ERR_ARG, "bs_adjoint: field grids have no adjoint");
-    if (P.compensated_positions)
-        return fail(h, BS_ERR_ARG, "bs_adjoint: needs compensated_positions = 0");
-    int rc = ensure_ready(h);
+    int rc = adjoint_check(h);
+    if (rc)
+        return rc;
+    rc = ensure_ready(h);
     if (rc)
         return rc;
     rc = upload_ext(h, ext_force3);
     if (rc)
         return rc;
-    if ((dk || dl0) && !h->s1.empty())
-    {
-        rc = build_spring_owners(h);
-        if (rc)
-            return rc;
-    }
-    const int64_t n = h->n, E = h->nedges, m = (int64_t)h->s1.size();
-    const int32_t T = nsteps;
-    int32_t K = checkpoint_every > 0 ? checkpoint_every : (int32_t)std::ceil(std::sqrt((double)std::max(T, 1)));
-    K = std::max(1, std::min(K, std::max(T, 1)));
-    const int32_t ncp = (T + K - 1) / K;
-    const size_t s4 = sizeof(float4) * (size_t)n;
-    hipStream_t st = h->stream;
-    hipError_t e = hipSuccess;
-    Scratch S;
-    float4 *cp_pos, *cp_vel, *seg, *tgt, *fin_pos, *fin_vel;
-    int32_t *cp_cnt, *fin_cnt;
-    double4 *lx, *lv, *hwp = nullptr;
-    float4 *wb[2];
-    float *dke = nullptr, *dle = nullptr;
-    double *dqd, *lpart;
-#define SALLOC(ptr, T_, cnt)                                                                                     \
-    do                                                                                                            \
-    {                                                                                                             \
-        ptr = S.get<T_>((size_t)(cnt), e);                                                                        \
-        HIPCHK(h, e);                                                                                             \
-    } while (0)
-    SALLOC(cp_pos, float4, (int64_t)n * std::max(ncp, 1));
-    SALLOC(cp_vel, float4, (int64_t)n * std::max(ncp, 1));
-    SALLOC(cp_cnt, int32_t, 4 * std::max(ncp, 1));
-    SALLOC(seg, float4, (int64_t)n * K);
-    SALLOC(tgt, float4, n);
-    SALLOC(fin_pos, float4, n);
-    SALLOC(fin_vel, float4, n);
-    SALLOC(fin_cnt, int32_t, 4);
-    SALLOC(lx, double4, n);
-    SALLOC(lv, double4, n);
-    SALLOC(wb[0], float4, n);
-    SALLOC(wb[1], float4, n);
-    if (!h->lists.empty())
-        SALLOC(hwp, double4, n);
-    // spring gradients: FP32 sums per owning edge over at most FLUSH steps,
-    // then added into FP64 totals per slot; per spring at the end
-    const bool want_springs = (dk || dl0) && m > 0;
-    const int64_t ns = std::max<int64_t>(h->nslots, 0);
-    double *dks = nullptr, *dls = nullptr, *outk = nullptr, *outl = nullptr;
-    if (want_springs)
-    {
-        SALLOC(dke, float, E);
-        SALLOC(dle, float, E);
-        SALLOC(dks, double, ns);
-        SALLOC(dls, double, ns);
-        SALLOC(outk, double, m);
-        SALLOC(outl, double, m);
-    }
-    constexpr int32_t FLUSH = 64;
-    int32_t since_flush = 0;
-    SALLOC(dqd, double, n);
-    const int32_t lblocks = (int32_t)((n + BLOCK - 1) / BLOCK);
-    SALLOC(lpart, double, lblocks);
-#undef SALLOC
+    return adjoint_record(h, nsteps, checkpoint_every);
+}
 
-    // forward: T steps, state saved (and the lists rebuilt) every K steps so
-    // that recomputing a segment repeats it bit for bit; the last segment's
-    // positions are kept as they pass and that segment is not recomputed
-    const int32_t s_last = (ncp - 1) * K;
-    for (int32_t s = 0; s < T; ++s)
+int bs_adjoint_backward(bs_handle *h, const double *dl_dx3, const double *dl_dv3, double *dk, double *dl0,
+                        double *dq)
+{
+    if (!h)
+        return BS_ERR_ARG;
+    if (!h->tape.ready)
+        return fail(h, BS_ERR_STATE, "bs_adjoint_backward: no recorded forward pass at the current state");
+    if (!dl_dx3 && !dl_dv3)
+        return fail(h, BS_ERR_ARG, "bs_adjoint_backward: dL/dx or dL/dv required");
+    int rc = adjoint_check(h);
+    if (rc)
+        return rc;
+    rc = check_device(h);
+    if (rc)
+        return rc;
+    // seed in internal order: static particles carry no adjoint state
+    const int64_t n = h->n;
+    const double dt = (double)h->p.timestep;
+    std::vector<double4> lx((size_t)n), lv((size_t)n);
+    std::vector<float4> w((size_t)n);
+    for (int64_t t = 0; t < n; ++t)
     {
-        if (s >= s_last)
-            HIPCHK(h, hipMemcpyAsync(seg + (size_t)(s - s_last) * n, h->pos[h->cur], s4, hipMemcpyDeviceToDevice, st));
-        if (s % K == 0)
+        const int64_t i = h->perm[(size_t)t];
+        const bool dyn = !(h->flags[(size_t)i] & BS_FLAG_STATIC);
+        double x[3] = {0, 0, 0}, v[3] = {0, 0, 0};
+        for (int c = 0; c < 3; ++c)
         {
-            const size_t c = (size_t)(s / K);
-            HIPCHK(h, hipMemcpyAsync(cp_pos + c * n, h->pos[h->cur], s4, hipMemcpyDeviceToDevice, st));
-            HIPCHK(h, hipMemcpyAsync(cp_vel + c * n, h->vel, s4, hipMemcpyDeviceToDevice, st));
-            HIPCHK(h, hipMemcpyAsync(cp_cnt + 4 * c, h->counters, 4 * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
-            launch_fill_i32(h->status, 1, 1, nullptr, st);
+            if (dyn && dl_dx3)
+                x[c] = dl_dx3[3 * i + c];
+            if (dyn && dl_dv3)
+                v[c] = dl_dv3[3 * i + c];
         }
-        rc = prof_fold(h);
-        if (rc)
-            return rc;
-        launch_step(h, s + 1 == T);
+        lx[(size_t)t] = make_double4(x[0], x[1], x[2], 0.0);
+        lv[(size_t)t] = make_double4(v[0], v[1], v[2], 0.0);
+        const double m = h->mass[(size_t)i], s = (dyn && m > 0.0) ? dt / m : 0.0;
+        w[(size_t)t] = make_float4((float)(s * (v[0] + dt * x[0])), (float)(s * (v[1] + dt * x[1])),
+                                   (float)(s * (v[2] + dt * x[2])), 0.0f);
     }
-    if (T > 0)
-    {
-        rc = finish_call(h);
-        if (rc)
-            return rc;
-        h->steps_done += T;
-        h->last_call_forces_only = false;
-    }
-    HIPCHK(h, hipMemcpyAsync(fin_pos, h->pos[h->cur], s4, hipMemcpyDeviceToDevice, st));
-    HIPCHK(h, hipMemcpyAsync(fin_vel, h->vel, s4, hipMemcpyDeviceToDevice, st));
-    HIPCHK(h, hipMemcpyAsync(fin_cnt, h->counters, 4 * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
+    Scratch S;
+    AdjSeed seed{};
+    rc = adjoint_alloc_seed(h, S, seed);
+    if (rc)
+        return rc;
+    HIPCHK(h, hipMemcpyAsync(seed.lx, lx.data(), sizeof(double4) * n, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(seed.lv, lv.data(), sizeof(double4) * n, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(seed.w, w.data(), sizeof(float4) * n, hipMemcpyHostToDevice, h->stream));
+    return adjoint_sweep(h, S, seed, dk, dl0, dq);
+}
 
-    // loss and the adjoint state at x_T
+int bs_adjoint(bs_handle *h, int32_t nsteps, int32_t checkpoint_every, const float *target3, const float *ext_force3,
+               double *loss, double *dk, double *dl0, double *dq)
+{
+    if (!h)
+        return BS_ERR_ARG;
+    if (!target3)
+        return fail(h, BS_ERR_ARG, "bs_adjoint: target positions required");
+    int rc = bs_adjoint_forward(h, nsteps, checkpoint_every, ext_force3);
+    if (rc)
+        return rc;
+    // loss and seed lx = 2 (x_T - y) / n on the device
+    const int64_t n = h->n;
+    Scratch S;
+    AdjSeed seed{};
+    rc = adjoint_alloc_seed(h, S, seed);
+    if (rc)
+        return rc;
+    hipError_t e = hipSuccess;
+    float4 *tgt = S.get<float4>((size_t)n, e);
+    HIPCHK(h, e);
+    const int32_t lblocks = (int32_t)((n + BLOCK - 1) / BLOCK);
+    double *lpart = S.get<double>((size_t)lblocks, e);
+    HIPCHK(h, e);
     {
         std::vector<float4> y((size_t)n);
         for (int64_t t = 0; t < n; ++t)
@@ -2258,126 +2532,19 @@ int bs_adjoint(bs_handle *h, int32_t nsteps, int32_t checkpoint_every, const flo
             const int64_t i = h->perm[(size_t)t];
             y[(size_t)t] = make_float4(target3[3 * i], target3[3 * i + 1], target3[3 * i + 2], 0.0f);
         }
-        HIPCHK(h, hipMemcpyAsync(tgt, y.data(), s4, hipMemcpyHostToDevice, st));
-        HIPCHK(h, hipStreamSynchronize(st));
+        HIPCHK(h, hipMemcpyAsync(tgt, y.data(), sizeof(float4) * n, hipMemcpyHostToDevice, h->stream));
+        launch_adj_init(h->pos[h->cur], tgt, h->jprm, h->iprm, (int32_t)n, (double)h->p.timestep, seed.lx, seed.lv,
+                        seed.w, lpart, lblocks, h->stream);
+        std::vector<double> lp((size_t)lblocks);
+        HIPCHK(h, hipMemcpyAsync(lp.data(), lpart, sizeof(double) * lblocks, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        double L = 0.0;
+        for (double v : lp)
+            L += v;
+        if (loss)
+            *loss = L / (double)n;
     }
-    const double dt = (double)P.timestep, gam = P.viscosity_enable ? (double)P.viscosity_value : 0.0;
-    launch_adj_init(h->pos[h->cur], tgt, h->jprm, h->iprm, (int32_t)n, dt, lx, lv, wb[0], lpart, lblocks, st);
-    if (want_springs)
-    {
-        HIPCHK(h, hipMemsetAsync(dke, 0, sizeof(float) * std::max<int64_t>(E, 1), st));
-        HIPCHK(h, hipMemsetAsync(dle, 0, sizeof(float) * std::max<int64_t>(E, 1), st));
-        HIPCHK(h, hipMemsetAsync(dks, 0, sizeof(double) * std::max<int64_t>(ns, 1), st));
-        HIPCHK(h, hipMemsetAsync(dls, 0, sizeof(double) * std::max<int64_t>(ns, 1), st));
-    }
-    HIPCHK(h, hipMemsetAsync(dqd, 0, sizeof(double) * n, st));
-    const double hs = 0.5 * std::max(P.list_skin, 0.0f);
-    const float half_skin2 = (float)(hs * hs * (1.0 - 1e-5));
-    int wc = 0;
-    for (int32_t c = ncp - 1; c >= 0; --c)
-    {
-        const int32_t s0 = c * K, s1 = std::min(T, s0 + K);
-        if (c < ncp - 1)
-        {
-            HIPCHK(h, hipMemcpyAsync(h->pos[h->cur], cp_pos + (size_t)c * n, s4, hipMemcpyDeviceToDevice, st));
-            HIPCHK(h, hipMemcpyAsync(h->vel, cp_vel + (size_t)c * n, s4, hipMemcpyDeviceToDevice, st));
-            HIPCHK(h, hipMemcpyAsync(h->counters, cp_cnt + 4 * (size_t)c, 4 * sizeof(int32_t),
-                                     hipMemcpyDeviceToDevice, st));
-            launch_fill_i32(h->status, 1, 1, nullptr, st);
-        }
-        for (int32_t s = s0; s < s1 && c < ncp - 1; ++s)
-        {
-            HIPCHK(h, hipMemcpyAsync(seg + (size_t)(s - s0) * n, h->pos[h->cur], s4, hipMemcpyDeviceToDevice, st));
-            if (s + 1 < s1)
-            {
-                rc = prof_fold(h);
-                if (rc)
-                    return rc;
-                launch_step(h, false);
-            }
-        }
-        for (int32_t s = s1 - 1; s >= s0; --s)
-        {
-            HIPCHK(h, hipMemcpyAsync(h->pos[h->cur], seg + (size_t)(s - s0) * n, s4, hipMemcpyDeviceToDevice, st));
-            rc = prof_fold(h);
-            if (rc)
-                return rc;
-            const size_t e0 = h->prof ? prof_mark(h) : SIZE_MAX;
-            launch_drift_check(h->pos[h->cur], h->xref, (int32_t)n, half_skin2, h->status, st);
-            rebuild_chain(h, true);
-            const size_t e1 = h->prof ? prof_mark(h) : SIZE_MAX;
-            if (hwp)
-                launch_adj_pairs(h, wb[wc], hwp, dqd);
-            const size_t e2 = h->prof ? prof_mark(h) : SIZE_MAX;
-            AdjSpringArgs a{};
-            a.pos = h->pos[h->cur];
-            a.jprm = h->jprm;
-            a.iprm = h->iprm;
-            a.sp_ptr = h->sp_ptr;
-            a.sp_j = h->sp_j;
-            a.sp_k = h->sp_k;
-            a.sp_l0 = h->sp_l0;
-            a.w = wb[wc];
-            a.hwp = hwp;
-            a.lx = lx;
-            a.lv = lv;
-            a.w_next = wb[wc ^ 1];
-            a.dk = dke;
-            a.dl0 = dle;
-            a.n = (int32_t)n;
-            a.spring_enable = P.spring_enable;
-            a.spring_scale = P.spring_scale;
-            a.dt = dt;
-            a.gamma = gam;
-            a.k = h->k;
-            launch_adj_spring(a, h->group, h->spring_blocks, st);
-            if (want_springs && (++since_flush == FLUSH || s == 0))
-            {
-                launch_spring_grads(h->slot_e, dke, dle, ns, dks, dls, st);
-                since_flush = 0;
-            }
-            if (h->prof)
-            {
-                const size_t e3 = prof_mark(h);
-                prof_span(h, BS_PROF_REBUILD, e0, e1);
-                if (hwp)
-                    prof_span(h, BS_PROF_ADJ_PAIR, e1, e2);
-                prof_span(h, BS_PROF_ADJ_SPRING, e2, e3);
-            }
-            wc ^= 1;
-        }
-    }
-    // back to x_T (lists rebuilt at the next step)
-    HIPCHK(h, hipMemcpyAsync(h->pos[h->cur], fin_pos, s4, hipMemcpyDeviceToDevice, st));
-    HIPCHK(h, hipMemcpyAsync(h->vel, fin_vel, s4, hipMemcpyDeviceToDevice, st));
-    HIPCHK(h, hipMemcpyAsync(h->counters, fin_cnt, 4 * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
-    launch_fill_i32(h->status, 1, 1, nullptr, st);
-
-    if (want_springs)
-        launch_spring_out(h->slot_of, m, dks, dls, outk, outl, st);
-    std::vector<double> lp((size_t)lblocks), dqh((size_t)n);
-    int32_t stv[4];
-    HIPCHK(h, hipMemcpyAsync(lp.data(), lpart, sizeof(double) * lblocks, hipMemcpyDeviceToHost, st));
-    HIPCHK(h, hipMemcpyAsync(stv, h->status, sizeof stv, hipMemcpyDeviceToHost, st));
-    if (dq)
-        HIPCHK(h, hipMemcpyAsync(dqh.data(), dqd, sizeof(double) * n, hipMemcpyDeviceToHost, st));
-    if (want_springs && dk)
-        HIPCHK(h, hipMemcpyAsync(dk, outk, sizeof(double) * m, hipMemcpyDeviceToHost, st));
-    if (want_springs && dl0)
-        HIPCHK(h, hipMemcpyAsync(dl0, outl, sizeof(double) * m, hipMemcpyDeviceToHost, st));
-    HIPCHK(h, hipStreamSynchronize(st));
-    prof_collect(h);
-    if (stv[2] != 0)
-        return fail(h, BS_ERR_STATE, "neighbour list capacity exceeded (max %d > cap); rerun bs_setup", stv[3]);
-    double L = 0.0;
-    for (double v : lp)
-        L += v;
-    if (loss)
-        *loss = L / (double)n;
-    if (dq)
-        for (int64_t t = 0; t < n; ++t)
-            dq[h->perm[(size_t)t]] = dqh[(size_t)t];
-    return BS_OK;
+    return adjoint_sweep(h, S, seed, dk, dl0, dq);
 }
 
 int bs_autotune(bs_handle *h, int32_t steps, int32_t *group_out, int32_t *graph_out, int32_t *layout_out)
@@ -2477,6 +2644,8 @@ int bs_autotune(bs_handle *h, int32_t steps, int32_t *group_out, int32_t *graph_
                 return fail(h, trc, "bs_autotune: trial failed");
         }
     }
+    if (h->group != bg || h->p.pair_layout != blay)
+        h->tape.ready = false; // recomputation would not repeat the recorded pass bit for bit
     h->group = bg;
     h->spring_blocks = (int32_t)((h->n * h->group + BLOCK - 1) / BLOCK);
     if (h->p.graph_steps != bgr || h->p.pair_layout != blay)

@@ -258,6 +258,27 @@ class SpringNetwork:
                                  ptr(dl0, c.c_double), ptr(dq, c.c_double)), self._h, "bs_adjoint")
         return float(loss.value), dk[:m], dl0[:m], dq[:self.n]
 
+    def adjointForward(self, nsteps, checkpoint_every=0):
+        """Runs and records nsteps steps (the state advances) for
+        adjointBackward."""
+        check(self._L.bs_adjoint_forward(self._h, int(nsteps), int(checkpoint_every), ptr(self._ext, ctypes.c_float)),
+              self._h, "bs_adjoint_forward")
+
+    def adjointBackward(self, dl_dx=None, dl_dv=None):
+        """(dk, dl0, dq) of a loss of the recorded final state, given dL/dx_T
+        and/or dL/dv_T as (n, 3) arrays."""
+        c = ctypes
+        gx = None if dl_dx is None else np.ascontiguousarray(dl_dx, np.float64).reshape(-1)
+        gv = None if dl_dv is None else np.ascontiguousarray(dl_dv, np.float64).reshape(-1)
+        for g in (gx, gv):
+            if g is not None and g.size != 3 * self.n:
+                raise ValueError("gradients must hold 3 values per particle")
+        m = int(self._L.bs_num_springs(self._h))
+        dk, dl0, dq = np.zeros(max(m, 1)), np.zeros(max(m, 1)), np.zeros(max(self.n, 1))
+        check(self._L.bs_adjoint_backward(self._h, ptr(gx, c.c_double), ptr(gv, c.c_double), ptr(dk, c.c_double),
+                                          ptr(dl0, c.c_double), ptr(dq, c.c_double)), self._h, "bs_adjoint_backward")
+        return dk[:m], dl0[:m], dq[:self.n]
+
     def close(self):
         if self._h:
             self._L.bs_destroy(self._h)

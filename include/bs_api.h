@@ -376,6 +376,22 @@ int bs_get_system_energies(bs_handle *h, double *out);
 int bs_adjoint(bs_handle *h, int32_t nsteps, int32_t checkpoint_every, const float *target3,
                const float *ext_force3, double *loss, double *dk, double *dl0, double *dq);
 
+/* The adjoint in two calls, for any loss of the final state.
+ * bs_adjoint_forward runs nsteps steps from the current state (which
+ * advances, as bs_step) and records them: checkpoints every checkpoint_every
+ * steps (<= 0: ceil(sqrt(nsteps))) and the last segment.  The caller reads
+ * x_T, v_T (bs_get_state) and forms dL/dx_T, dL/dv_T.
+ * bs_adjoint_backward(dl_dx3, dl_dv3) (3n doubles each, either nullable, not
+ * both) runs the reverse sweep of bs_adjoint from that seed and writes dk,
+ * dl0, dq (nullable) as bs_adjoint does.  The state is back at x_T after it
+ * and the record stays valid, so several seeds can be swept; any call that
+ * changes the state, the parameters, the external force or the launch
+ * configuration drops it (BS_ERR_STATE on a later backward).
+ * No reference counterpart. */
+int bs_adjoint_forward(bs_handle *h, int32_t nsteps, int32_t checkpoint_every, const float *ext_force3);
+int bs_adjoint_backward(bs_handle *h, const double *dl_dx3, const double *dl_dv3, double *dk, double *dl0,
+                        double *dq);
+
 /* ---- tuning ---------------------------------------------------------------------- */
 
 /* Times `steps` steps of a copy of the current system for each launch

@@ -142,3 +142,54 @@ def test_adjoint_rejects_terms_without_adjoint():
     net = SpringNetwork.from_system(s, params=p)
     with pytest.raises(_abi.EngineError):
         net.adjoint(3, s.x.reshape(-1, 3))
+
+
+def test_two_phase_adjoint_matches_one_call_and_is_linear_in_the_seed():
+    """adjointForward + adjointBackward(dL/dx_T) reproduces bs_adjoint for the
+    MSD loss, the sweep is linear in the seed (dL/dx and dL/dv), and the
+    record serves several seeds."""
+    s = workloads.generate(1, seed=15, n=3000)
+    y = _target(s)
+    T = 16
+    a = SpringNetwork.from_system(s)
+    loss, dk, dl, dq = a.adjoint(T, y, checkpoint_every=5)
+    b = SpringNetwork.from_system(s)
+    b.adjointForward(T, 5)
+    x, _ = b.getState()
+    g = 2.0 * (x.astype(np.float64) - y) / s.n
+    dk2, dl2, dq2 = b.adjointBackward(g)
+    for u, v in ((dk, dk2), (dl, dl2), (dq, dq2)):
+        assert _rel(v, u) <= 1e-6
+    rng = np.random.default_rng(4)
+    g1, g2 = rng.normal(size=(s.n, 3)), rng.normal(size=(s.n, 3))
+    v1 = rng.normal(size=(s.n, 3))
+    r1 = b.adjointBackward(g1)
+    r2 = b.adjointBackward(g2)
+    r12 = b.adjointBackward(2.0 * g1 - 0.5 * g2)
+    rv = b.adjointBackward(None, v1)
+    rxv = b.adjointBackward(g1, v1)
+    for i in range(3):
+        assert _rel(r12[i], 2.0 * r1[i] - 0.5 * r2[i]) <= 1e-5
+        assert _rel(rxv[i], r1[i] + rv[i]) <= 1e-5
+    b.run(1)
+    with pytest.raises(_abi.EngineError):
+        b.adjointBackward(g1)
+
+
+def test_torch_autograd_wrapper_gives_the_adjoint_gradients():
+    torch = pytest.importorskip("torch")
+    from biospring_amd.autograd import final_positions
+
+    s = workloads.generate(1, seed=16, n=2000)
+    y = _target(s)
+    T = 12
+    ref = SpringNetwork.from_system(s).adjoint(T, y, checkpoint_every=4)
+    net = SpringNetwork.from_system(s)
+    k = torch.tensor(s.k.astype(np.float64), requires_grad=True)
+    q = torch.tensor(s.charge.astype(np.float64), requires_grad=True)
+    x = final_positions(net, s, k=k, q=q, nsteps=T, checkpoint_every=4)
+    loss = ((x - torch.from_numpy(y.astype(np.float64))) ** 2).sum(dim=1).mean()
+    loss.backward()
+    assert abs(loss.item() - ref[0]) <= 1e-9 * ref[0]
+    assert _rel(k.grad.numpy(), ref[1]) <= 1e-6
+    assert _rel(q.grad.numpy(), ref[3]) <= 1e-6

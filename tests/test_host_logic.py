@@ -240,6 +240,24 @@ def test_adjoint_sequence(hostcheck_lib):
             assert abs(loss - ref) <= 1e-12 * ref, (T, K, loss, ref)
             assert dk.shape == (net_m := int(L.bs_num_springs(net._h)),) and dq.shape == (400,)
             assert np.array_equal(dk, T * ends), (T, K)
+        # two-phase form: record once, sweep twice; state changes drop the record
+        # (bs_adjoint leaves one too)
+        net.run(1)
+        try:
+            net.adjointBackward(np.zeros((400, 3)))
+            raise SystemExit("backward without a record")
+        except _abi.EngineError as e:
+            assert e.code == _abi.BS_ERR_STATE
+        net.adjointForward(6, 4)
+        for seed in (np.ones((400, 3)), None):
+            dk2, dl2, dq2 = net.adjointBackward(seed, np.ones((400, 3)))
+            assert np.array_equal(dk2, 6 * ends)
+        net.run(1)
+        try:
+            net.adjointBackward(np.ones((400, 3)))
+            raise SystemExit("backward after a step")
+        except _abi.EngineError as e:
+            assert e.code == _abi.BS_ERR_STATE
         p = s.params.copy(); p.imp_enable = 1
         net.setParams(p)
         try:
