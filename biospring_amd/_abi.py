@@ -33,6 +33,7 @@ EXPORTS = (
     "bs_get_cells", "bs_get_pairs", "bs_add_rigid_body", "bs_num_rigid_bodies", "bs_get_rigid_body",
     "bs_set_grid", "bs_set_burying", "bs_get_insertion_vector", "bs_set_systems", "bs_num_systems",
     "bs_get_system_energies", "bs_autotune", "bs_adjoint", "bs_adjoint_forward", "bs_adjoint_backward",
+    "bs_springs_from_cutoff", "bs_get_springs",
 )
 PROF_REBUILD, PROF_PAIR, PROF_SPRING = range(3)
 
@@ -158,6 +159,8 @@ def bind(L):
     L.bs_autotune.argtypes = [_vp, ctypes.c_int32, _i32p, _i32p, _i32p]
     L.bs_adjoint.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, _f32p, _f32p, _f64p, _f64p, _f64p, _f64p]
     L.bs_adjoint_forward.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, _f32p]
+    L.bs_springs_from_cutoff.argtypes = [_vp, ctypes.c_double, ctypes.c_float, _i64p]
+    L.bs_get_springs.argtypes = [_vp, _u32p, _u32p, _f32p, _f32p]
     L.bs_adjoint_backward.argtypes = [_vp, _f64p, _f64p, _f64p, _f64p, _f64p]
     return L
 

@@ -2547,6 +2547,154 @@ int bs_adjoint(bs_handle *h, int32_t nsteps, int32_t checkpoint_every, const flo
     return adjoint_sweep(h, S, seed, dk, dl0, dq);
 }
 
+int bs_springs_from_cutoff(bs_handle *h, double cutoff, float stiffness, int64_t *count_out)
+{
+    if (!h)
+        return BS_ERR_ARG;
+    if (!h->have_particles)
+        return fail(h, BS_ERR_STATE, "bs_springs_from_cutoff: particles must be set first");
+    const float rcf = (float)cutoff; // NeighborSearch keeps the cutoff as a float
+    if (!(rcf >= 1e-6f) || !std::isfinite(rcf)) // nsearch.hpp:66-67 rejects cutoffs under 1e-6
+        return fail(h, BS_ERR_ARG, "bs_springs_from_cutoff: the cutoff must be finite and at least 1e-6");
+    const int64_t n = h->n;
+    if (n > (int64_t)CL_JMASK)
+        return fail(h, BS_ERR_ARG, "bs_springs_from_cutoff: more than %u particles", CL_JMASK);
+    hipStream_t st = h->stream;
+    hipError_t e = hipSuccess;
+    Scratch S;
+    std::vector<int32_t> rows((size_t)n + 1, 0);
+    std::vector<int32_t> jh;
+    std::vector<float> lh;
+    if (n > 1)
+    {
+        std::vector<float4> ph((size_t)n);
+        for (int64_t i = 0; i < n; ++i)
+        {
+            const float flag = __builtin_bit_cast(float, (uint32_t)((h->flags[(size_t)i] & BS_FLAG_STATIC) ? 1u : 0u));
+            ph[(size_t)i] = make_float4(h->x[3 * i], h->x[3 * i + 1], h->x[3 * i + 2], flag);
+        }
+        const int32_t nb = (int32_t)std::min<int64_t>(BLOCK, (n + BLOCK - 1) / BLOCK);
+        float4 *pos = S.get<float4>((size_t)n, e);
+        HIPCHK(h, e);
+        double *box_part = S.get<double>((size_t)(6 * nb), e);
+        HIPCHK(h, e);
+        double *box = S.get<double>(6, e);
+        HIPCHK(h, e);
+        HIPCHK(h, hipMemcpyAsync(pos, ph.data(), sizeof(float4) * n, hipMemcpyHostToDevice, st));
+        launch_box_reduce(pos, nullptr, (int32_t)n, box_part, box, nb, nullptr, st);
+        double bx[6];
+        HIPCHK(h, hipMemcpyAsync(bx, box, sizeof bx, hipMemcpyDeviceToHost, st));
+        HIPCHK(h, hipStreamSynchronize(st));
+        // cells of edge a little above the cutoff (so that rounding in the
+        // cell coordinate never puts a pair closer than the cutoff two cells
+        // apart), at most 2^24 of them
+        const float edge = rcf * (1.0f + 1e-5f);
+        int32_t dims[3];
+        int64_t ncells = 1;
+        for (int q = 0; q < 3; ++q)
+        {
+            const double len = bx[3 + q] - bx[q];
+            dims[q] = (int32_t)std::max(1.0, std::min(std::floor(len / (double)edge), 256.0));
+            ncells *= dims[q];
+        }
+        int32_t *counts = S.get<int32_t>((size_t)ncells, e);
+        HIPCHK(h, e);
+        int32_t *cstart = S.get<int32_t>((size_t)ncells + 1, e);
+        HIPCHK(h, e);
+        int32_t *cursor = S.get<int32_t>((size_t)ncells, e);
+        HIPCHK(h, e);
+        int32_t *cell_of = S.get<int32_t>((size_t)n, e);
+        HIPCHK(h, e);
+        int32_t *tmp = S.get<int32_t>((size_t)n, e);
+        HIPCHK(h, e);
+        int32_t *cidx = S.get<int32_t>((size_t)n, e);
+        HIPCHK(h, e);
+        int32_t *rcnt = S.get<int32_t>((size_t)n, e);
+        HIPCHK(h, e);
+        launch_fill_i32(counts, 0, ncells, nullptr, st);
+        launch_cell_ids(pos, nullptr, pos, 0u, (int32_t)n, box, dims[0], dims[1], dims[2], 0.0, edge, 0, cell_of, counts,
+                        nullptr, st);
+        launch_scan(counts, cstart, (int32_t)ncells, nullptr, st);
+        launch_fill_i32(cursor, 0, ncells, nullptr, st);
+        launch_scatter(cell_of, (int32_t)n, cstart, cursor, tmp, nullptr, st);
+        launch_cell_sort(cell_of, (int32_t)n, cstart, tmp, nullptr, cidx, nullptr, st);
+        TopoArgs a{};
+        a.pos = pos;
+        a.box = box;
+        a.cstart = cstart;
+        a.cidx = cidx;
+        a.counts = rcnt;
+        a.n = (int32_t)n;
+        for (int q = 0; q < 3; ++q)
+            a.dims[q] = dims[q];
+        a.edge = edge;
+        a.rc = (double)rcf;
+        launch_topo_springs(a, st);
+        HIPCHK(h, hipMemcpyAsync(rows.data() + 1, rcnt, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
+        HIPCHK(h, hipStreamSynchronize(st));
+        int64_t total = 0;
+        for (int64_t i = 0; i < n; ++i)
+        {
+            total += rows[(size_t)i + 1];
+            if (total > INT32_MAX)
+                return fail(h, BS_ERR_ARG, "bs_springs_from_cutoff: more than 2^31 springs");
+            rows[(size_t)i + 1] = (int32_t)total;
+        }
+        int32_t *offs = S.get<int32_t>((size_t)n, e);
+        HIPCHK(h, e);
+        int32_t *jout = S.get<int32_t>((size_t)std::max<int64_t>(total, 1), e);
+        HIPCHK(h, e);
+        float *lout = S.get<float>((size_t)std::max<int64_t>(total, 1), e);
+        HIPCHK(h, e);
+        HIPCHK(h, hipMemcpyAsync(offs, rows.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
+        a.offs = offs;
+        a.jout = jout;
+        a.l0out = lout;
+        launch_topo_springs(a, st);
+        jh.resize((size_t)total);
+        lh.resize((size_t)total);
+        if (total > 0)
+        {
+            HIPCHK(h, hipMemcpyAsync(jh.data(), jout, sizeof(int32_t) * total, hipMemcpyDeviceToHost, st));
+            HIPCHK(h, hipMemcpyAsync(lh.data(), lout, sizeof(float) * total, hipMemcpyDeviceToHost, st));
+        }
+        HIPCHK(h, hipStreamSynchronize(st));
+    }
+    const int64_t m = (int64_t)jh.size();
+    h->s1.resize((size_t)m);
+    h->s2.resize((size_t)m);
+    h->sl0.assign(lh.begin(), lh.end());
+    h->sk.assign((size_t)m, stiffness);
+#pragma omp parallel for schedule(static, 1024)
+    for (int64_t i = 0; i < n; ++i)
+        for (int32_t t = rows[(size_t)i]; t < rows[(size_t)i + 1]; ++t)
+        {
+            h->s1[(size_t)t] = (uint32_t)i;
+            h->s2[(size_t)t] = (uint32_t)jh[(size_t)t];
+        }
+    free_device(h);
+    h->have_springs = true;
+    if (count_out)
+        *count_out = m;
+    return BS_OK;
+}
+
+int bs_get_springs(const bs_handle *h, uint32_t *id1, uint32_t *id2, float *equilibrium, float *stiffness)
+{
+    if (!h)
+        return BS_ERR_ARG;
+    const size_t m = h->s1.size();
+    if (id1 && m)
+        std::memcpy(id1, h->s1.data(), sizeof(uint32_t) * m);
+    if (id2 && m)
+        std::memcpy(id2, h->s2.data(), sizeof(uint32_t) * m);
+    if (equilibrium && m)
+        std::memcpy(equilibrium, h->sl0.data(), sizeof(float) * m);
+    if (stiffness && m)
+        std::memcpy(stiffness, h->sk.data(), sizeof(float) * m);
+    return BS_OK;
+}
+
 int bs_autotune(bs_handle *h, int32_t steps, int32_t *group_out, int32_t *graph_out, int32_t *layout_out)
 {
     if (!h)

@@ -221,6 +221,24 @@ struct AdjSpringArgs
     Consts k;
 };
 
+// Springs from a cutoff: count pass (offs == nullptr: counts[i] = springs of
+// row i) or fill pass (row i at offs[i], sorted by j).
+struct TopoArgs
+{
+    const float4 *pos;        // original order; .w = static flag bits
+    const double *box;        // [6]
+    const int32_t *cstart;    // [ncells + 1]
+    const int32_t *cidx;      // members sorted by index within each cell
+    const int32_t *offs;      // nullable: fill pass
+    int32_t *counts;          // count pass: [n]
+    int32_t *jout;            // fill pass: [springs]
+    float *l0out;
+    int32_t n;
+    int32_t dims[3];
+    float edge;               // minimum cell edge (>= the cutoff)
+    double rc;                // the cutoff as the reference compares it: (double)(float)cutoff
+};
+
 struct ListArgs
 {
     const float4 *pos;
@@ -293,6 +311,7 @@ void launch_spring_grads(const int32_t *slot_e, float *dke, float *dle, int64_t 
 // Per spring from per slot: out[s] = slot_of[s] >= 0 ? v[slot_of[s]] : 0.
 void launch_spring_out(const int32_t *slot_of, int64_t m, const double *dk, const double *dl0, double *outk,
                        double *outl, hipStream_t s);
+void launch_topo_springs(const TopoArgs &a, hipStream_t s);
 // status[0] = 1 when any particle is more than half the list skin from xref.
 void launch_drift_check(const float4 *pos, const float4 *xref, int32_t n, float half_skin2, int32_t *status,
                         hipStream_t s);

@@ -917,6 +917,128 @@ __global__ void __launch_bounds__(BLOCK) k_adj_spring(AdjSpringArgs a)
                                 (float)(s * fma(a.dt, nx.z, nv.z)), 0.0f);
 }
 
+// ---------------------------------------------------------------------------
+// Springs from a cutoff (Topology::add_springs_from_cutoff,
+// src/topology/Topology.hpp:105-138): one spring for each pair i < j closer
+// than the cutoff, unless both particles are static, with rest length the
+// FP64 distance of the float positions (Spring.hpp:47-56) rounded to float.
+// The distance and its test are the reference's (measure::distance, unfused,
+// against the float cutoff: nsearch.hpp:230).  One thread per particle i
+// scans the 27 cells around it on a grid of edge >= cutoff; the count pass
+// sizes the rows, the fill pass writes each row and sorts it by j, so that
+// the springs come out ordered by (i, j).  pos.w carries the static flag.
+// ---------------------------------------------------------------------------
+__device__ __noinline__ bool topo_pair(float4 pi, float4 pj, double rc, double &d)
+{
+#pragma clang fp contract(off)
+    const double ax = (double)pj.x - (double)pi.x;
+    const double ay = (double)pj.y - (double)pi.y;
+    const double az = (double)pj.z - (double)pi.z;
+    d = sqrt(ax * ax + ay * ay + az * az);
+    return d < rc;
+}
+
+// Sorts a row (second indices j, all distinct, with their rest lengths) by
+// j: insertion sort for short rows, heapsort beyond, so that a crowded row
+// (many particles within the cutoff of one another) stays O(r log r).
+__device__ void topo_sort_row(int32_t *j, float *l, int32_t r)
+{
+    if (r <= 32)
+    {
+        for (int32_t u = 1; u < r; ++u)
+        {
+            const int32_t jv = j[u];
+            const float lv = l[u];
+            int32_t v = u - 1;
+            while (v >= 0 && j[v] > jv)
+            {
+                j[v + 1] = j[v];
+                l[v + 1] = l[v];
+                --v;
+            }
+            j[v + 1] = jv;
+            l[v + 1] = lv;
+        }
+        return;
+    }
+    auto sift = [&](int32_t root, int32_t end) {
+        const int32_t jv = j[root];
+        const float lv = l[root];
+        for (;;)
+        {
+            int32_t c = 2 * root + 1;
+            if (c >= end)
+                break;
+            if (c + 1 < end && j[c + 1] > j[c])
+                ++c;
+            if (jv >= j[c])
+                break;
+            j[root] = j[c];
+            l[root] = l[c];
+            root = c;
+        }
+        j[root] = jv;
+        l[root] = lv;
+    };
+    for (int32_t t = r / 2 - 1; t >= 0; --t)
+        sift(t, r);
+    for (int32_t e = r - 1; e > 0; --e)
+    {
+        const int32_t jt = j[e];
+        const float lt = l[e];
+        j[e] = j[0];
+        l[e] = l[0];
+        j[0] = jt;
+        l[0] = lt;
+        sift(0, e);
+    }
+}
+
+__global__ void k_topo_springs(TopoArgs a)
+{
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n)
+        return;
+    const float4 pi = a.pos[i];
+    const bool si = fbits(pi.w) != 0u;
+    int32_t c[3];
+    const float xs[3] = {pi.x, pi.y, pi.z};
+    for (int q = 0; q < 3; ++q)
+        c[q] = internal_coord(xs[q], a.box[q], a.box[3 + q] - a.box[q], a.dims[q], a.edge);
+    int32_t cnt = 0;
+    const int32_t base = a.offs ? a.offs[i] : 0;
+    for (int32_t z = max(c[2] - 1, 0); z <= min(c[2] + 1, a.dims[2] - 1); ++z)
+        for (int32_t y = max(c[1] - 1, 0); y <= min(c[1] + 1, a.dims[1] - 1); ++y)
+            for (int32_t x = max(c[0] - 1, 0); x <= min(c[0] + 1, a.dims[0] - 1); ++x)
+            {
+                const int32_t cell = x + y * a.dims[0] + z * a.dims[0] * a.dims[1];
+                for (int32_t t = a.cstart[cell]; t < a.cstart[cell + 1]; ++t)
+                {
+                    const int32_t j = a.cidx[t];
+                    if (j <= i)
+                        continue;
+                    const float4 pj = a.pos[j];
+                    if (si && fbits(pj.w) != 0u)
+                        continue;
+                    double d;
+                    if (!topo_pair(pi, pj, a.rc, d))
+                        continue;
+                    if (a.offs)
+                    {
+                        a.jout[base + cnt] = j;
+                        a.l0out[base + cnt] = (float)d;
+                    }
+                    ++cnt;
+                }
+            }
+    if (!a.offs)
+    {
+        a.counts[i] = cnt;
+        return;
+    }
+    topo_sort_row(a.jout + base, a.l0out + base, cnt);
+}
+
 __global__ void k_prestep(int32_t *status, int32_t *step, int32_t interval)
 {
     if (interval > 0 && *step > 0 && (*step % interval) == 0)
@@ -1046,6 +1168,12 @@ void launch_spring_out(const int32_t *slot_of, int64_t m, const double *dk, cons
 {
     if (m > 0)
         k_spring_out<<<nblk(m), BLOCK, 0, s>>>(slot_of, m, dk, dl0, outk, outl);
+}
+
+void launch_topo_springs(const TopoArgs &a, hipStream_t s)
+{
+    if (a.n > 0)
+        k_topo_springs<<<nblk(a.n), BLOCK, 0, s>>>(a);
 }
 
 void launch_drift_check(const float4 *pos, const float4 *xref, int32_t n, float half_skin2, int32_t *status,

@@ -743,6 +743,69 @@ void launch_adj_init(const float4 *pos, const float4 *target, const float4 *jprm
     }
 }
 
+// Springs from a cutoff, computed like the kernel (same grid walk and test).
+void launch_topo_springs(const TopoArgs &a, hipStream_t)
+{
+    const int32_t n = a.n;
+    need(a.pos, sizeof(float4) * n, "topo: pos");
+    need(a.box, sizeof(double) * 6, "topo: box");
+    const int64_t ncells = (int64_t)a.dims[0] * a.dims[1] * a.dims[2];
+    need(a.cstart, sizeof(int32_t) * (ncells + 1), "topo: cstart");
+    need(a.cidx, sizeof(int32_t) * n, "topo: cidx");
+    if (!a.offs)
+        need(a.counts, sizeof(int32_t) * n, "topo: counts");
+    else
+        need(a.offs, sizeof(int32_t) * n, "topo: offs");
+    for (int32_t i = 0; i < n; ++i)
+    {
+        const float4 pi = a.pos[i];
+        const bool si = fbits(pi.w) != 0u;
+        int32_t c[3];
+        const float xs[3] = {pi.x, pi.y, pi.z};
+        for (int q = 0; q < 3; ++q)
+            c[q] = internal_coord(xs[q], a.box[q], a.box[3 + q] - a.box[q], a.dims[q], a.edge);
+        std::vector<std::pair<int32_t, float>> row;
+        for (int32_t z = std::max(c[2] - 1, 0); z <= std::min(c[2] + 1, a.dims[2] - 1); ++z)
+            for (int32_t y = std::max(c[1] - 1, 0); y <= std::min(c[1] + 1, a.dims[1] - 1); ++y)
+                for (int32_t x = std::max(c[0] - 1, 0); x <= std::min(c[0] + 1, a.dims[0] - 1); ++x)
+                {
+                    const int32_t cell = x + y * a.dims[0] + z * a.dims[0] * a.dims[1];
+                    for (int32_t t = a.cstart[cell]; t < a.cstart[cell + 1]; ++t)
+                    {
+                        const int32_t j = a.cidx[t];
+                        if (j < 0 || j >= n)
+                            hc_fail("topo: member index out of range");
+                        if (j <= i)
+                            continue;
+                        const float4 pj = a.pos[j];
+                        if (si && fbits(pj.w) != 0u)
+                            continue;
+                        const double ax = (double)pj.x - (double)pi.x, ay = (double)pj.y - (double)pi.y,
+                                     az = (double)pj.z - (double)pi.z;
+                        volatile double s2 = ax * ax;
+                        s2 = s2 + ay * ay;
+                        s2 = s2 + az * az;
+                        const double d = std::sqrt((double)s2);
+                        if (d < a.rc)
+                            row.push_back({j, (float)d});
+                    }
+                }
+        if (!a.offs)
+        {
+            a.counts[i] = (int32_t)row.size();
+            continue;
+        }
+        std::sort(row.begin(), row.end(), [](auto &u, auto &v) { return u.first < v.first; });
+        need(a.jout + a.offs[i], sizeof(int32_t) * row.size(), "topo: jout");
+        need(a.l0out + a.offs[i], sizeof(float) * row.size(), "topo: l0out");
+        for (size_t t = 0; t < row.size(); ++t)
+        {
+            a.jout[a.offs[i] + t] = row[t].first;
+            a.l0out[a.offs[i] + t] = row[t].second;
+        }
+    }
+}
+
 void launch_drift_check(const float4 *pos, const float4 *xref, int32_t n, float, int32_t *status, hipStream_t)
 {
     need(pos, sizeof(float4) * n, "drift_check: pos");

@@ -47,6 +47,31 @@ class SpringNetwork:
         check(self._L.bs_set_springs(self._h, a.size, ptr(a, c.c_uint32), ptr(b, c.c_uint32),
                                      ptr(l0, c.c_float), ptr(k, c.c_float)), self._h, "bs_set_springs")
 
+    def springsFromCutoff(self, cutoff, stiffness=1.0):
+        """Topology::add_springs_from_cutoff on the GPU: a spring for every
+        pair closer than `cutoff` at the current positions (see
+        bs_springs_from_cutoff); returns the number of springs."""
+        m = ctypes.c_int64()
+        check(self._L.bs_springs_from_cutoff(self._h, float(cutoff), float(stiffness), ctypes.byref(m)), self._h,
+              "bs_springs_from_cutoff")
+        return int(m.value)
+
+    def getSprings(self):
+        """(id1, id2, equilibrium, stiffness) of the loaded springs."""
+        m = int(self._L.bs_num_springs(self._h))
+        a, b = np.zeros(max(m, 1), np.uint32), np.zeros(max(m, 1), np.uint32)
+        l0, k = np.zeros(max(m, 1), np.float32), np.zeros(max(m, 1), np.float32)
+        c = ctypes
+        check(self._L.bs_get_springs(self._h, ptr(a, c.c_uint32), ptr(b, c.c_uint32), ptr(l0, c.c_float),
+                                     ptr(k, c.c_float)), self._h, "bs_get_springs")
+        return a[:m], b[:m], l0[:m], k[:m]
+
+    def getNumberOfParticles(self):
+        return int(self._L.bs_num_particles(self._h))
+
+    def getNumberOfSprings(self):
+        return int(self._L.bs_num_springs(self._h))
+
     @classmethod
     def from_system(cls, system, params=None, library=None):
         net = cls(params if params is not None else system.params, library=library)

@@ -161,6 +161,23 @@ int bs_destroy(bs_handle *h);
 /* Message of the last failing call on h (static text if h is NULL). */
 const char *bs_last_error(const bs_handle *h);
 
+/* Replaces Topology::add_springs_from_cutoff (src/topology/Topology.hpp:105-138)
+ * followed by Topology::to_spring_network: one spring for each pair of the
+ * loaded particles (at their current positions) closer than `cutoff` (A),
+ * unless both are static, with rest length the FP64 distance of the float
+ * positions rounded to float (Spring.hpp:47-56, then the float of
+ * SpringNetwork::addSpring) and the given stiffness.  The pair test is the
+ * reference's: FP64 distance < the cutoff as a float (NeighborSearch keeps
+ * it as a float, nsearch.hpp:155-230).  Built on the GPU; replaces the
+ * springs of bs_set_springs.  Springs come ordered by (first, second)
+ * original id, first < second; the reference orders them by first id, then
+ * by its cell scan.  *count_out (nullable) = number of springs. */
+int bs_springs_from_cutoff(bs_handle *h, double cutoff, float stiffness, int64_t *count_out);
+
+/* The loaded springs (bs_set_springs or bs_springs_from_cutoff order);
+ * every pointer nullable, bs_num_springs entries each. */
+int bs_get_springs(const bs_handle *h, uint32_t *id1, uint32_t *id2, float *equilibrium, float *stiffness);
+
 /* Number of particles / springs currently loaded. */
 int64_t bs_num_particles(const bs_handle *h);
 int64_t bs_num_springs(const bs_handle *h);

@@ -2559,3 +2559,138 @@ int orc_step(orc *o, int mode, int nsteps, const double *ext3, double *E6)
 int64_t orc_last_nonfinite(const orc *o) { return o->nonfinite; }
 
 int orc_num_threads(void) { return max_threads(); }
+
+/* ---------------------------------------------------------------------------
+ * Spring network from a distance cutoff: topology::Topology::
+ * add_springs_from_cutoff (src/topology/Topology.hpp:105-138).  One spring per
+ * pair of particles closer than the cutoff, unless both are static
+ * (Topology.hpp:121-122).  The test is NeighborSearch's
+ * (src/nsearch.hpp:230): measure::distance (src/measure.hpp:162-168, the FP64
+ * distance of the float positions) < the cutoff held as a float
+ * (nsearch.hpp:57); the rest length is that distance (Spring.hpp:55-56),
+ * kept as a float by SpringNetwork::addSpring.  Each pair is found once, from
+ * its lower index, so the reference's SpringAlreadyExistsException path
+ * never arises here.
+ *
+ * Written as a plain grid of cells of edge >= the cutoff scanned over the 27
+ * cells around each particle; rows are sorted by the second index, so the
+ * output is ordered by (first, second).  Returns the number of springs and,
+ * when id1 is not NULL, writes the rows that fit in cap entries (all of them
+ * when cap >= that number); -1 for a cutoff the reference rejects
+ * (nsearch.hpp:66-67).
+ * ------------------------------------------------------------------------- */
+static int cmp_row_j(const void *a, const void *b)
+{
+    const uint32_t x = *(const uint32_t *)a, y = *(const uint32_t *)b;
+    return (x > y) - (x < y);
+}
+
+int64_t orc_springs_from_cutoff(int64_t n, const float *x3, const uint8_t *flags, double cutoff, int64_t cap,
+                                uint32_t *id1, uint32_t *id2, float *l0)
+{
+    const float rcf = (float)cutoff;
+    if (!(rcf >= 1e-6f) || !isfinite(rcf))
+        return -1;
+    if (n < 2)
+        return 0;
+    const double rc = (double)rcf;
+    double lo[3], hi[3];
+    for (int q = 0; q < 3; ++q)
+        lo[q] = hi[q] = x3[q];
+    for (int64_t i = 1; i < n; ++i)
+        for (int q = 0; q < 3; ++q)
+        {
+            const double v = x3[3 * i + q];
+            lo[q] = v < lo[q] ? v : lo[q];
+            hi[q] = v > hi[q] ? v : hi[q];
+        }
+    /* the cell edge is 1.001 x the cutoff: rounding of the cell coordinate
+     * can then never put two particles closer than the cutoff two cells apart */
+    const double edge = rc * 1.001;
+    int64_t dims[3], ncell = 1;
+    for (int q = 0; q < 3; ++q)
+    {
+        double d = floor((hi[q] - lo[q]) / edge);
+        dims[q] = d < 1.0 ? 1 : (d > 128.0 ? 128 : (int64_t)d);
+        ncell *= dims[q];
+    }
+    int64_t *cell = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
+    int64_t *start = (int64_t *)calloc((size_t)ncell + 1, sizeof(int64_t));
+    int64_t *member = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
+    for (int64_t i = 0; i < n; ++i)
+    {
+        int64_t c[3];
+        for (int q = 0; q < 3; ++q)
+        {
+            const double e = fmax(edge, (hi[q] - lo[q]) / (double)dims[q]);
+            const double u = ((double)x3[3 * i + q] - lo[q]) / e;
+            c[q] = u <= 0.0 ? 0 : (u >= (double)(dims[q] - 1) ? dims[q] - 1 : (int64_t)u);
+        }
+        cell[i] = c[0] + dims[0] * (c[1] + dims[1] * c[2]);
+        start[cell[i] + 1]++;
+    }
+    for (int64_t c = 0; c < ncell; ++c)
+        start[c + 1] += start[c];
+    {
+        int64_t *fill = (int64_t *)malloc(sizeof(int64_t) * (size_t)ncell);
+        memcpy(fill, start, sizeof(int64_t) * (size_t)ncell);
+        for (int64_t i = 0; i < n; ++i)
+            member[fill[cell[i]]++] = i;
+        free(fill);
+    }
+    struct
+    {
+        uint32_t j;
+        float l;
+    } *row = NULL;
+    int64_t rowcap = 0, m = 0;
+    for (int64_t i = 0; i < n; ++i)
+    {
+        const int si = flags && (flags[i] & BS_FLAG_STATIC);
+        const int64_t ci = cell[i];
+        const int64_t c0 = ci % dims[0], c1 = (ci / dims[0]) % dims[1], c2 = ci / (dims[0] * dims[1]);
+        int64_t r = 0;
+        for (int64_t z = c2 - 1; z <= c2 + 1; ++z)
+            for (int64_t y = c1 - 1; y <= c1 + 1; ++y)
+                for (int64_t x = c0 - 1; x <= c0 + 1; ++x)
+                {
+                    if (x < 0 || y < 0 || z < 0 || x >= dims[0] || y >= dims[1] || z >= dims[2])
+                        continue;
+                    const int64_t cc = x + dims[0] * (y + dims[1] * z);
+                    for (int64_t t = start[cc]; t < start[cc + 1]; ++t)
+                    {
+                        const int64_t j = member[t];
+                        if (j <= i || (si && flags && (flags[j] & BS_FLAG_STATIC)))
+                            continue;
+                        const double dx = (double)x3[3 * i] - (double)x3[3 * j];
+                        const double dy = (double)x3[3 * i + 1] - (double)x3[3 * j + 1];
+                        const double dz = (double)x3[3 * i + 2] - (double)x3[3 * j + 2];
+                        const double d = sqrt(dx * dx + dy * dy + dz * dz);
+                        if (!(d < rc))
+                            continue;
+                        if (r == rowcap)
+                        {
+                            rowcap = rowcap ? 2 * rowcap : 256;
+                            row = realloc(row, sizeof(*row) * (size_t)rowcap);
+                        }
+                        row[r].j = (uint32_t)j;
+                        row[r].l = (float)d;
+                        ++r;
+                    }
+                }
+        qsort(row, (size_t)r, sizeof(*row), cmp_row_j);
+        if (cap >= 0 && id1 && m + r <= cap)
+            for (int64_t t = 0; t < r; ++t)
+            {
+                id1[m + t] = (uint32_t)i;
+                id2[m + t] = row[t].j;
+                l0[m + t] = row[t].l;
+            }
+        m += r;
+    }
+    free(row);
+    free(member);
+    free(start);
+    free(cell);
+    return m;
+}

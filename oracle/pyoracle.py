@@ -82,6 +82,9 @@ def oracle_lib():
         L.orc_loss.restype = ctypes.c_double
         L.orc_set_params_arrays.argtypes = [_vp, _f32p, _f32p, _f32p]
         L.orc_hessian_vec.argtypes = [_vp, _f64p, _f64p]
+        L.orc_springs_from_cutoff.argtypes = [ctypes.c_int64, _f32p, _u8p, ctypes.c_double, ctypes.c_int64, _u32p,
+                                              _u32p, _f32p]
+        L.orc_springs_from_cutoff.restype = ctypes.c_int64
         _olib = L
     return _olib
 
@@ -117,6 +120,9 @@ def ref_lib():
         L.ref_set_grid.argtypes = [_vp, ctypes.c_int, _f64p, _i64p, _f64p, _f32p, ctypes.c_double]
         L.ref_set_burying.argtypes = [_vp, _f32p]
         L.ref_insertion.argtypes = [_vp, ctypes.c_int64, ctypes.c_int64, _f64p]
+        L.ref_springs_from_cutoff.argtypes = [ctypes.c_int64, _f32p, _u8p, ctypes.c_double, ctypes.c_double,
+                                              ctypes.c_int64, _u32p, _u32p, _f32p]
+        L.ref_springs_from_cutoff.restype = ctypes.c_int64
         _rlib = L
     return _rlib
 
@@ -367,3 +373,31 @@ def ref_pairs(system, kind, params=None):
     out = np.zeros(2 * max(cnt, 1), np.uint32)
     L.ref_pairs(*args, _p(out, c.c_uint32), cnt)
     return out[:2 * cnt].reshape(-1, 2)
+
+
+def _springs_call(fn, x, flags, cutoff, *extra):
+    x = np.ascontiguousarray(x, np.float32).reshape(-1)
+    n = x.size // 3
+    fl = None if flags is None else np.ascontiguousarray(flags, np.uint8)
+    c = ctypes
+    cnt = fn(n, _p(x, c.c_float), _p(fl, c.c_uint8), cutoff, *extra, 0, None, None, None)
+    if cnt < 0:
+        raise ValueError(f"cutoff {cutoff} rejected")
+    i1 = np.zeros(max(cnt, 1), np.uint32)
+    i2 = np.zeros(max(cnt, 1), np.uint32)
+    l0 = np.zeros(max(cnt, 1), np.float32)
+    fn(n, _p(x, c.c_float), _p(fl, c.c_uint8), cutoff, *extra, cnt, _p(i1, c.c_uint32), _p(i2, c.c_uint32),
+       _p(l0, c.c_float))
+    return i1[:cnt], i2[:cnt], l0[:cnt]
+
+
+def springs_from_cutoff(x, flags, cutoff):
+    """Oracle: springs of Topology::add_springs_from_cutoff ordered by
+    (first, second), first < second, with float rest lengths."""
+    return _springs_call(oracle_lib().orc_springs_from_cutoff, x, flags, cutoff)
+
+
+def ref_springs_from_cutoff(x, flags, cutoff, stiffness=1.0):
+    """The reference's own Topology::add_springs_from_cutoff, springs in its
+    order as (first, second, float(equilibrium))."""
+    return _springs_call(ref_lib().ref_springs_from_cutoff, x, flags, cutoff, stiffness)

@@ -23,18 +23,24 @@
 //    the DX reader does after parsing (src/IO/OpenDXReader.cpp:146-173).
 //  - ref_insertion: InsertionVector::computeVector/Angle/RollAngle
 //    (src/InsertionVector.cpp:18-78).
+//  - ref_springs_from_cutoff: topology::Topology::add_springs_from_cutoff
+//    (src/topology/Topology.hpp:105-138) on particles at the given positions,
+//    the springs read back as (first, second, float(equilibrium)) in the
+//    reference's own order.
 #include "SpringNetwork.h"
 #include "configuration/Configuration.hpp"
 #include "forcefield/ForceField.h"
 #include "InsertionVector.h"
 #include "grid/PotentialGrid.hpp"
 #include "rigidbody/RigidBodiesManager.h"
+#include "topology/Topology.hpp"
 
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <mutex>
+#include <unordered_map>
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
@@ -509,6 +515,38 @@ int64_t ref_pairs(const bs_params * p, int kind, int64_t n, const float * x3, co
             ij[2 * t + 1] = v[t].second;
         }
     return cnt;
+}
+
+// Springs of Topology::add_springs_from_cutoff: returns the count; with cap
+// >= count, writes id1/id2 (indices in the given order) and the rest lengths
+// as SpringNetwork::addSpring receives them (float).
+int64_t ref_springs_from_cutoff(int64_t n, const float * x3, const uint8_t * flags, double cutoff, double stiffness,
+                                int64_t cap, uint32_t * id1, uint32_t * id2, float * l0)
+{
+    using biospring::topology::Topology;
+    using biospring::topology::Particle;
+    std::vector<Particle> ps((size_t)n);
+    for (int64_t i = 0; i < n; ++i)
+    {
+        ps[(size_t)i].set_position(Vector3f(x3[3 * i], x3[3 * i + 1], x3[3 * i + 2]));
+        ps[(size_t)i].properties().set_static(flags && (flags[i] & BS_FLAG_STATIC));
+    }
+    Topology top;
+    top.add_particles(ps);
+    std::unordered_map<int64_t, uint32_t> index; // the collection copies particles with new ids
+    for (int64_t i = 0; i < n; ++i)
+        index[(int64_t)top.get_particle((size_t)i).unique_id()] = (uint32_t)i;
+    top.add_springs_from_cutoff(cutoff, stiffness);
+    const int64_t m = (int64_t)top.number_of_springs();
+    if (cap >= m)
+        for (int64_t s = 0; s < m; ++s)
+        {
+            const auto & sp = top.get_spring((size_t)s);
+            id1[s] = index.at((int64_t)sp.first().unique_id());
+            id2[s] = index.at((int64_t)sp.second().unique_id());
+            l0[s] = (float)sp.equilibrium();
+        }
+    return m;
 }
 
 } // extern "C"

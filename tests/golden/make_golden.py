@@ -226,6 +226,8 @@ def main():
     add_arrays("grid300", grid_origin=g_origin, grid_shape=g_shape, grid_cell=g_cell, grid_phi=phi,
                grid_density=dens, burying=bur, ivec_ids=np.array([3, 250], np.int64))
 
+    topology_cases()
+
     if ONLY and "scans" not in ONLY:
         return
     # two-particle scans of the reference regression tests
@@ -261,6 +263,68 @@ def main():
         scans[f"{tag}_F0"] = np.array(fs)
     np.savez_compressed(os.path.join(OUT, "scans.npz"), d=xs, **scans)
     print("scans written")
+
+
+def borderline_pairs(rng, rc, centers, inside):
+    """One pair of float points near each center whose FP64 distance lies
+    between the cutoff as a double and the cutoff as a float (inside=True:
+    the reference links them, since NeighborSearch compares with the float),
+    or just above the float cutoff (inside=False: not linked)."""
+    rc32 = float(np.float32(rc))
+    lo, hi = (rc, rc32) if inside else (rc32, rc32 + 2e-6)
+    out = []
+    while len(out) < len(centers):
+        a = (centers[len(out)] + rng.uniform(-1, 1, 3)).astype(np.float32)
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        b = (a.astype(np.float64) + (rc + 1e-7) * u).astype(np.float32)
+        d = a.astype(np.float64) - b.astype(np.float64)
+        d = np.sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2])
+        if lo < d < hi if inside else lo <= d < hi:
+            out.append((a, b))
+    return out
+
+
+def topology_cases():
+    """Springs of Topology::add_springs_from_cutoff (Topology.hpp:105-138).
+
+    topo_cube1000: the setup of the reference's own test
+    (tests/topology/test-Topology.cpp:149-170): 1,000 particles uniform in
+    [-100, 100]^3, cutoff 10 A.
+    topo_dense1200: 1,100 particles in a 50 A cube, every tenth static, cutoff
+    9.3 A (not a float), plus 40 pairs whose distance falls between 9.3 and
+    float(9.3) (linked), 20 pairs just above float(9.3) (not linked), two
+    coincident particles (linked with rest length 0) and a static pair closer
+    than the cutoff (not linked)."""
+    for name in ("topo_cube1000", "topo_dense1200"):
+        if ONLY and name not in ONLY:
+            continue
+        rng = np.random.default_rng(2024 if name == "topo_cube1000" else 2025)
+        if name == "topo_cube1000":
+            x = rng.uniform(-100, 100, (1000, 3)).astype(np.float32)
+            flags = np.zeros(1000, np.uint8)
+            rc = 10.0
+        else:
+            rc = 9.3
+            pts = [rng.uniform(-25, 25, (1100, 3)).astype(np.float32)]
+            fl = [np.where(np.arange(1100) % 10 == 0, _abi.FLAG_STATIC, 0).astype(np.uint8)]
+            # the special pairs sit apart from the cube (and from each other)
+            centers = [np.array([200.0 + 30.0 * k, 0.0, 0.0]) for k in range(60)]
+            for a, b in borderline_pairs(rng, rc, centers[:40], True) + borderline_pairs(rng, rc, centers[40:], False):
+                pts.append(np.stack([a, b]).astype(np.float32))
+                fl.append(np.zeros(2, np.uint8))
+            pts.append(np.array([[-200.0, 5.5, 1.25], [-200.0, 5.5, 1.25]], np.float32))
+            fl.append(np.zeros(2, np.uint8))
+            pts.append(np.array([[-200.0, -60.0, 0.0], [-197.0, -60.0, 0.0]], np.float32))
+            fl.append(np.full(2, _abi.FLAG_STATIC, np.uint8))
+            x = np.concatenate(pts)
+            flags = np.concatenate(fl)
+            perm = rng.permutation(len(x))  # special pairs spread over the index range
+            x, flags = x[perm], flags[perm]
+        i1, i2, l0 = po.ref_springs_from_cutoff(x, flags, rc)
+        path = os.path.join(OUT, f"{name}.npz")
+        np.savez_compressed(path, in_x=x, in_flags=flags, cutoff=np.float64(rc), ref_id1=i1, ref_id2=i2, ref_l0=l0)
+        print(f"{name}: n={len(x)} springs={len(i1)} -> {os.path.getsize(path) / 1024:.0f} KiB")
 
 
 if __name__ == "__main__":

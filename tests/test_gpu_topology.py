@@ -1,0 +1,125 @@
+"""Spring network from a distance cutoff on the GPU (bs_springs_from_cutoff,
+in place of Topology::add_springs_from_cutoff, src/topology/Topology.hpp:
+105-138).  The bar is bit-exact: the same (first, second) set as the
+reference's golden fixtures and as the oracle, and bit-identical float rest
+lengths.  At full size (250,000 beads, springs < 8 A) the GPU's set is also
+compared with the workload generator's own springs, built by the same rule on
+the host."""
+import sys
+
+import numpy as np
+import pytest
+
+from biospring_amd import _abi, workloads
+from biospring_amd.network import SpringNetwork
+from oracle import pyoracle as po
+from tests.test_topology import CASES, brute_force, load_case, same_springs
+
+pytestmark = pytest.mark.gpu
+
+
+def _log(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _gpu_springs(x, flags, cutoff, stiffness=1.0, params=None):
+    net = SpringNetwork(params if params is not None else workloads.params(1))
+    net.setParticles(x, flags=flags)
+    m = net.springsFromCutoff(cutoff, stiffness)
+    a, b, l0, k = net.getSprings()
+    assert m == len(a)
+    assert np.all(k == np.float32(stiffness))
+    return net, (a, b, l0)
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_matches_reference_fixture(name):
+    c = load_case(name)
+    _log(f"topology fixture {name}: n={len(c['in_flags'])}")
+    net, got = _gpu_springs(c["in_x"], c["in_flags"], float(c["cutoff"]), 0.7)
+    same_springs(got, (c["ref_id1"], c["ref_id2"], c["ref_l0"]))
+    key = got[0].astype(np.int64) * len(c["in_flags"]) + got[1]
+    assert np.all(np.diff(key) > 0)  # ordered by (first, second)
+    net.close()
+
+
+@pytest.mark.parametrize("seed,n,cutoff,span,static_frac", [(11, 20000, 7.1, 60.0, 0.0), (12, 20000, 9.3, 60.0, 0.2),
+                                                            (13, 8000, 12.0, 35.0, 0.5), (14, 3000, 3.0, 400.0, 0.1)])
+def test_matches_oracle_random(seed, n, cutoff, span, static_frac):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(-span, span, (n, 3)).astype(np.float32)
+    flags = np.where(rng.random(n) < static_frac, _abi.FLAG_STATIC, 0).astype(np.uint8)
+    _log(f"topology random: seed={seed} n={n} cutoff={cutoff}")
+    net, got = _gpu_springs(x, flags, cutoff)
+    same_springs(got, po.springs_from_cutoff(x, flags, cutoff))
+    net.close()
+
+
+def _clumps():
+    # 300 clumps of 10 particles spread over 6,000 A: more cells a side than
+    # the grid's 256, so cells grow beyond the cutoff
+    rng = np.random.default_rng(3)
+    c = rng.uniform(-3000, 3000, (300, 1, 3))
+    return (c + rng.uniform(-4, 4, (300, 10, 3))).reshape(-1, 3).astype(np.float32)
+
+
+def test_degenerate_layouts():
+    # a line of particles (zero extent on two axes), a lone particle, all
+    # particles at one point, and a sparse cloud over a box of 256+ cells a side
+    cases = [(np.stack([np.arange(500, dtype=np.float32) * 0.7, np.zeros(500, np.float32),
+                        np.zeros(500, np.float32)], 1), 2.0),
+             (np.array([[1.0, 2.0, 3.0]], np.float32), 5.0),
+             (np.full((40, 3), 3.25, np.float32), 1.0),
+             (_clumps(), 6.0)]
+    for x, rc in cases:
+        _log(f"topology degenerate: n={len(x)} cutoff={rc}")
+        net, got = _gpu_springs(x, None, rc)
+        same_springs(got, brute_force(x, None, rc))
+        net.close()
+    net = SpringNetwork(workloads.params(1))
+    net.setParticles(np.zeros((4, 3), np.float32))
+    for bad in (0.0, 5e-7, float("nan"), float("inf")):
+        with pytest.raises(_abi.EngineError):
+            net.springsFromCutoff(bad)
+    net.close()
+
+
+def test_full_size_matches_generator_and_oracle():
+    s = workloads.generate(1, seed=1)
+    _log(f"topology full size: n={s.n} springs={s.m}")
+    net, got = _gpu_springs(s.x, s.flags, 8.0)
+    a, b, l0 = got
+    assert np.array_equal(a, s.id1) and np.array_equal(b, s.id2)
+    assert np.array_equal(l0.view(np.uint32), s.l0.view(np.uint32))
+    same_springs(got, po.springs_from_cutoff(s.x, s.flags, 8.0))
+    net.close()
+
+
+def test_constructed_network_steps_like_the_oracle():
+    """Construction followed by the step: forces of the GPU-built network
+    agree with the FP64 oracle on the oracle-built one (1e-5), so the loaded
+    spring table is the one the step uses."""
+    s = workloads.generate(1, seed=21, n=3000)
+    p = s.params.copy()
+    p.viscosity_enable = 0
+    net = SpringNetwork(p)
+    net.setParticles(*s.particle_arrays())
+    m = net.springsFromCutoff(9.5, 0.8)
+    a, b, l0 = po.springs_from_cutoff(s.x, s.flags, 9.5)
+    assert m == len(a)
+    t = workloads.System(x=s.x, v=s.v, mass=s.mass, charge=s.charge, radius=s.radius, eps=s.eps, sasa=s.sasa,
+                         transfer=s.transfer, hydro=s.hydro, flags=s.flags, id1=a, id2=b, l0=l0,
+                         k=np.full(len(a), 0.8, np.float32))
+    t.params = p
+    F, E = net.computeForces()
+    o = po.Oracle(t, params=p)
+    Fd, Ed = o.compute_forces(po.F64)
+    o.close()
+    err = np.linalg.norm(F - Fd, axis=1).max() / np.linalg.norm(Fd, axis=1).max()
+    _log(f"  constructed network: springs {m}, force err {err:.2e}")
+    assert err <= 1e-5
+    assert abs(E[0] - Ed[0]) <= 1e-5 * max(abs(Ed[0]), 1e-3)
+    net.run(5)
+    assert np.isfinite(net.getState()[0]).all()
+    net.close()
