@@ -10,7 +10,10 @@ in HBM.
 
 Printed JSON (one line): throughput, the roofline of the dominant kernel (HIP
 events on the engine's stream over the timed region), the FP64-oracle force
-error at the final state, and the oracle's CPU time on a bounded sample.
+error at the final state, and the oracle's CPU time on a bounded sample.  The
+headline workload also times the spring network's construction from the 8 A
+cutoff on the GPU (network_construction), with the reference's own Topology
+build of a smaller network of the same density beside it.
 
 Two helper modes serve the HBM-traffic measurement of the kernels
 (rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE):
@@ -140,6 +143,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-construction", action="store_true",
+                    help="workload 1: skip timing the spring-network construction")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--pmc-summary", nargs=4, metavar=("FETCH", "FETCH_COPY", "WRITE", "WRITE_COPY"))
     args = ap.parse_args()
@@ -252,6 +257,9 @@ def main():
         "adjoint": None,
     }
 
+    if args.workload == 1 and not args.no_construction:
+        out["network_construction"] = bench_construction(s, cpu=not args.no_cpu)
+
     x, v = net.getState()
     # Conservative force: the viscous term -gamma v is an exact product in both
     # codes and is compared separately in the tests.
@@ -321,6 +329,67 @@ def main():
             if REF_OVER_ORACLE:
                 out["cpu_baseline"]["reference_equivalent"] = done / dt / REF_OVER_ORACLE
     print(json.dumps(out), flush=True)
+
+
+def bench_construction(s, cutoff=8.0, reps=3, cpu=True, cpu_sample=5000):
+    """Spring network from a cutoff (Topology::add_springs_from_cutoff) over
+    the workload's own positions: the GPU build (wall time of
+    bs_springs_from_cutoff, which ends with the table on the host, and HIP-event
+    times of its two passes), checked spring for spring against the
+    generator's host build; beside it the reference's own Topology on a
+    bounded sample of the same density."""
+    from biospring_amd import workloads
+    from biospring_amd.network import SpringNetwork
+    net = SpringNetwork(s.params)
+    net.setParticles(*s.particle_arrays())
+    net.springsFromCutoff(cutoff)  # first call: code objects, allocations
+    net.profile(True)
+    walls = []
+    for _ in range(reps):
+        net.setSprings([], [], [], [])
+        hip_sync()
+        t = time.perf_counter()
+        m = net.springsFromCutoff(cutoff)
+        walls.append(time.perf_counter() - t)
+    prof = net.getProfile()
+    net.profile(False)
+    a, b, l0, _ = net.getSprings()
+    same = bool(np.array_equal(a, s.id1) and np.array_equal(b, s.id2)
+                and np.array_equal(l0.view(np.uint32), s.l0.view(np.uint32)))
+    net.close()
+    n = s.n
+    count_ms = prof["topo_count"]["ms"] / max(prof["topo_count"]["launches"], 1)
+    fill_ms = prof["topo_fill"]["ms"] / max(prof["topo_fill"]["launches"], 1)
+    # count pass: positions in, counts out; fill pass: positions and row
+    # offsets in, (j, l0) out per spring
+    b_alg = 40 * n + 8 * m
+    kern_ms = count_ms + fill_ms
+    wall = float(np.median(walls))
+    out = {"cutoff": cutoff, "springs": m, "identical_to_host_build": same,
+           "wall_ms": wall * 1e3, "springs_per_s": m / wall,
+           "kernel_ms": {"grid_and_count": count_ms, "fill_and_sort": fill_ms},
+           "roofline": {"bound": "hbm", "achieved": b_alg / (kern_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": b_alg / (kern_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "alg_bytes": b_alg,
+                        "note": "27-cell candidate gathers are served from L2 and not counted"},
+           "wall_note": "wall time includes the download of the spring table to the host"}
+    if cpu:
+        from oracle import pyoracle as po
+        if po.ref_available():
+            smp = workloads.generate(1, seed=2, n=cpu_sample)
+            t = time.perf_counter()
+            r = po.ref_springs_from_cutoff(smp.x, smp.flags, cutoff)
+            dt = time.perf_counter() - t
+            out["cpu_baseline"] = {"value": len(r[0]) / dt, "unit": "springs/s", "cores": 1, "kind": "reference",
+                                   "sample": f"Topology::add_springs_from_cutoff on a {cpu_sample}-bead network of "
+                                             f"the same density ({len(r[0])} springs), compiled reference"}
+        else:
+            t = time.perf_counter()
+            r = po.springs_from_cutoff(s.x, s.flags, cutoff)
+            dt = time.perf_counter() - t
+            out["cpu_baseline"] = {"value": len(r[0]) / dt, "unit": "springs/s", "cores": 1, "kind": "port",
+                                   "sample": f"the oracle's cell-grid build on the same {n} beads"}
+    log(f"construction: {m} springs in {wall * 1e3:.1f} ms wall, kernels {kern_ms:.2f} ms, same={same}")
+    return out
 
 
 def trajectory_check(seed, n=1000, marks=(1, 10, 100, 1000)):

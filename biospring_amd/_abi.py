@@ -21,7 +21,7 @@ FLAG_STATIC = 1
 E_SPRING, E_ELECTROSTATIC, E_STERIC, E_KINETIC, E_IMP, E_HYDROPHOBIC = range(6)
 NUM_ENERGIES = 6
 NS_STERIC, NS_ELECTROSTATIC, NS_HYDROPHOBIC = range(3)
-PROF_NUM = 5  # BS_PROF_REBUILD, _PAIR, _SPRING, _ADJ_PAIR, _ADJ_SPRING
+PROF_NUM = 7  # BS_PROF_REBUILD, _PAIR, _SPRING, _ADJ_PAIR, _ADJ_SPRING, _TOPO_COUNT, _TOPO_FILL
 GRID_POTENTIAL, GRID_DENSITY = 0, 1
 
 # Exported symbols, one per declaration in include/bs_api.h.

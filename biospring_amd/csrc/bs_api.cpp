@@ -1777,11 +1777,14 @@ int bs_set_springs(bs_handle *h, int64_t m, const uint32_t *id1, const uint32_t 
     if (m < 0 || (m > 0 && (!id1 || !id2 || !equilibrium || !stiffness)))
         return fail(h, BS_ERR_ARG, "bs_set_springs: bad arrays");
     // SpringNetwork::addSpring: range and self checks, duplicates ignored.
+    // The table is built aside, so that a rejected call leaves h as it was.
     std::vector<std::vector<uint32_t>> seen((size_t)h->n);
-    h->s1.clear();
-    h->s2.clear();
-    h->sl0.clear();
-    h->sk.clear();
+    std::vector<uint32_t> s1, s2;
+    std::vector<float> l0v, kv;
+    s1.reserve((size_t)m);
+    s2.reserve((size_t)m);
+    l0v.reserve((size_t)m);
+    kv.reserve((size_t)m);
     for (int64_t s = 0; s < m; ++s)
     {
         const uint32_t a = id1[s], b = id2[s];
@@ -1795,12 +1798,19 @@ int bs_set_springs(bs_handle *h, int64_t m, const uint32_t *id1, const uint32_t 
             continue;
         sa.push_back(b);
         seen[b].push_back(a);
-        h->s1.push_back(a);
-        h->s2.push_back(b);
-        h->sl0.push_back(equilibrium[s]);
-        h->sk.push_back(stiffness[s]);
+        s1.push_back(a);
+        s2.push_back(b);
+        l0v.push_back(equilibrium[s]);
+        kv.push_back(stiffness[s]);
     }
-    free_device(h);
+    // the current (device) state stays: the step rebuilds around it
+    int rc = pull_and_release(h);
+    if (rc)
+        return rc;
+    h->s1.swap(s1);
+    h->s2.swap(s2);
+    h->sl0.swap(l0v);
+    h->sk.swap(kv);
     h->have_springs = true;
     return BS_OK;
 }
@@ -2559,12 +2569,18 @@ int bs_springs_from_cutoff(bs_handle *h, double cutoff, float stiffness, int64_t
     const int64_t n = h->n;
     if (n > (int64_t)CL_JMASK)
         return fail(h, BS_ERR_ARG, "bs_springs_from_cutoff: more than %u particles", CL_JMASK);
+    // the current positions (the device's, once the system has stepped)
+    int rc = pull_and_release(h);
+    if (rc)
+        return rc;
     hipStream_t st = h->stream;
     hipError_t e = hipSuccess;
     Scratch S;
     std::vector<int32_t> rows((size_t)n + 1, 0);
-    std::vector<int32_t> jh;
-    std::vector<float> lh;
+    int64_t m = 0; // springs added
+    // springs already loaded stay, with their own rest length and stiffness;
+    // their pairs get no second spring (SpringCollection.hpp:138-151,226-233)
+    const int64_t m0 = h->have_springs ? (int64_t)h->s1.size() : 0;
     if (n > 1)
     {
         std::vector<float4> ph((size_t)n);
@@ -2597,6 +2613,7 @@ int bs_springs_from_cutoff(bs_handle *h, double cutoff, float stiffness, int64_t
             dims[q] = (int32_t)std::max(1.0, std::min(std::floor(len / (double)edge), 256.0));
             ncells *= dims[q];
         }
+        const size_t e0 = h->prof ? prof_mark(h) : SIZE_MAX;
         int32_t *counts = S.get<int32_t>((size_t)ncells, e);
         HIPCHK(h, e);
         int32_t *cstart = S.get<int32_t>((size_t)ncells + 1, e);
@@ -2629,49 +2646,102 @@ int bs_springs_from_cutoff(bs_handle *h, double cutoff, float stiffness, int64_t
             a.dims[q] = dims[q];
         a.edge = edge;
         a.rc = (double)rcf;
+        // sqrt(r2) < rc  <=>  r2 < r2max, sqrt being monotonic and correctly
+        // rounded (here and on the device)
+        double r2max = a.rc * a.rc;
+        while (r2max > 0.0 && std::sqrt(r2max) >= a.rc)
+            r2max = std::nextafter(r2max, 0.0);
+        while (std::sqrt(r2max) < a.rc)
+            r2max = std::nextafter(r2max, INFINITY);
+        a.r2max = r2max;
+        if (m0 > 0)
+        {
+            // loaded springs as rows of their lower index, partners sorted
+            std::vector<int32_t> eo((size_t)n + 1, 0), ej((size_t)m0);
+            for (int64_t t = 0; t < m0; ++t)
+                ++eo[(size_t)std::min(h->s1[(size_t)t], h->s2[(size_t)t]) + 1];
+            for (int64_t i = 0; i < n; ++i)
+                eo[(size_t)i + 1] += eo[(size_t)i];
+            std::vector<int32_t> fill(eo.begin(), eo.end() - 1);
+            for (int64_t t = 0; t < m0; ++t)
+            {
+                const uint32_t u = h->s1[(size_t)t], v = h->s2[(size_t)t];
+                ej[(size_t)fill[std::min(u, v)]++] = (int32_t)std::max(u, v);
+            }
+            for (int64_t i = 0; i < n; ++i)
+                std::sort(ej.begin() + eo[(size_t)i], ej.begin() + eo[(size_t)i + 1]);
+            int32_t *eod = S.get<int32_t>((size_t)n + 1, e);
+            HIPCHK(h, e);
+            int32_t *ejd = S.get<int32_t>((size_t)m0, e);
+            HIPCHK(h, e);
+            HIPCHK(h, hipMemcpyAsync(eod, eo.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, st));
+            HIPCHK(h, hipMemcpyAsync(ejd, ej.data(), sizeof(int32_t) * m0, hipMemcpyHostToDevice, st));
+            // the host arrays must outlive the copies
+            HIPCHK(h, hipStreamSynchronize(st));
+            a.ex_off = eod;
+            a.ex_j = ejd;
+        }
         launch_topo_springs(a, st);
+        if (h->prof)
+            prof_span(h, BS_PROF_TOPO_COUNT, e0, prof_mark(h));
         HIPCHK(h, hipMemcpyAsync(rows.data() + 1, rcnt, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
         HIPCHK(h, hipStreamSynchronize(st));
         int64_t total = 0;
+        int32_t max_row = 0;
         for (int64_t i = 0; i < n; ++i)
         {
+            max_row = std::max(max_row, rows[(size_t)i + 1]);
             total += rows[(size_t)i + 1];
             if (total > INT32_MAX)
                 return fail(h, BS_ERR_ARG, "bs_springs_from_cutoff: more than 2^31 springs");
             rows[(size_t)i + 1] = (int32_t)total;
         }
-        int32_t *offs = S.get<int32_t>((size_t)n, e);
+        int32_t *offs = S.get<int32_t>((size_t)n + 1, e);
         HIPCHK(h, e);
         int32_t *jout = S.get<int32_t>((size_t)std::max<int64_t>(total, 1), e);
         HIPCHK(h, e);
         float *lout = S.get<float>((size_t)std::max<int64_t>(total, 1), e);
         HIPCHK(h, e);
-        HIPCHK(h, hipMemcpyAsync(offs, rows.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
+        HIPCHK(h, hipMemcpyAsync(offs, rows.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, st));
         a.offs = offs;
+        a.max_row = max_row;
         a.jout = jout;
         a.l0out = lout;
+        const size_t e2 = h->prof ? prof_mark(h) : SIZE_MAX;
         launch_topo_springs(a, st);
-        jh.resize((size_t)total);
-        lh.resize((size_t)total);
-        if (total > 0)
+        if (h->prof)
+            prof_span(h, BS_PROF_TOPO_FILL, e2, prof_mark(h));
+        // the new springs go straight into the tables, after the loaded ones
+        m = total;
+        h->s1.resize((size_t)(m0 + m));
+        h->s2.resize((size_t)(m0 + m));
+        h->sl0.resize((size_t)(m0 + m));
+        hipError_t ce = hipSuccess;
+        if (m > 0)
         {
-            HIPCHK(h, hipMemcpyAsync(jh.data(), jout, sizeof(int32_t) * total, hipMemcpyDeviceToHost, st));
-            HIPCHK(h, hipMemcpyAsync(lh.data(), lout, sizeof(float) * total, hipMemcpyDeviceToHost, st));
+            ce = hipMemcpyAsync(h->s2.data() + m0, jout, sizeof(int32_t) * m, hipMemcpyDeviceToHost, st);
+            if (ce == hipSuccess)
+                ce = hipMemcpyAsync(h->sl0.data() + m0, lout, sizeof(float) * m, hipMemcpyDeviceToHost, st);
         }
-        HIPCHK(h, hipStreamSynchronize(st));
-    }
-    const int64_t m = (int64_t)jh.size();
-    h->s1.resize((size_t)m);
-    h->s2.resize((size_t)m);
-    h->sl0.assign(lh.begin(), lh.end());
-    h->sk.assign((size_t)m, stiffness);
+        uint32_t *s1 = h->s1.data() + m0;
 #pragma omp parallel for schedule(static, 1024)
-    for (int64_t i = 0; i < n; ++i)
-        for (int32_t t = rows[(size_t)i]; t < rows[(size_t)i + 1]; ++t)
+        for (int64_t i = 0; i < n; ++i)
+            for (int32_t t = rows[(size_t)i]; t < rows[(size_t)i + 1]; ++t)
+                s1[t] = (uint32_t)i;
+        if (ce == hipSuccess)
+            ce = hipStreamSynchronize(st);
+        if (ce != hipSuccess)
         {
-            h->s1[(size_t)t] = (uint32_t)i;
-            h->s2[(size_t)t] = (uint32_t)jh[(size_t)t];
+            h->s1.resize((size_t)m0);
+            h->s2.resize((size_t)m0);
+            h->sl0.resize((size_t)m0);
+            HIPCHK(h, ce);
         }
+        if (h->prof)
+            prof_collect(h);
+    }
+    h->sk.resize((size_t)m0);
+    h->sk.resize((size_t)(m0 + m), stiffness);
     free_device(h);
     h->have_springs = true;
     if (count_out)

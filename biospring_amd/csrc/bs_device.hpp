@@ -10,6 +10,14 @@ __device__ inline uint32_t fbits(float f) { return __float_as_uint(f); }
 
 __device__ inline bool gated_off(const int32_t *gate) { return gate != nullptr && *gate == 0; }
 
+// Makes one wave's LDS writes visible to the other lanes of the same wave.
+__device__ inline void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ inline double wave_sum(double v)
 {
     for (int off = 32; off > 0; off >>= 1)

@@ -229,7 +229,7 @@ struct TopoArgs
     const double *box;        // [6]
     const int32_t *cstart;    // [ncells + 1]
     const int32_t *cidx;      // members sorted by index within each cell
-    const int32_t *offs;      // nullable: fill pass
+    const int32_t *offs;      // nullable: fill pass, row starts [n + 1]
     int32_t *counts;          // count pass: [n]
     int32_t *jout;            // fill pass: [springs]
     float *l0out;
@@ -237,6 +237,10 @@ struct TopoArgs
     int32_t dims[3];
     float edge;               // minimum cell edge (>= the cutoff)
     double rc;                // the cutoff as the reference compares it: (double)(float)cutoff
+    double r2max;             // smallest double whose correctly rounded sqrt is >= rc
+    const int32_t *ex_off;    // nullable: springs already loaded, as rows of
+    const int32_t *ex_j;      //   their lower index [n + 1], partners sorted
+    int32_t max_row;          // fill pass: the longest row
 };
 
 struct ListArgs

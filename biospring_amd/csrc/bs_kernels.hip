@@ -923,21 +923,13 @@ __global__ void __launch_bounds__(BLOCK) k_adj_spring(AdjSpringArgs a)
 // than the cutoff, unless both particles are static, with rest length the
 // FP64 distance of the float positions (Spring.hpp:47-56) rounded to float.
 // The distance and its test are the reference's (measure::distance, unfused,
-// against the float cutoff: nsearch.hpp:230).  One thread per particle i
+// against the float cutoff: nsearch.hpp:230).  One wave per particle i
 // scans the 27 cells around it on a grid of edge >= cutoff; the count pass
-// sizes the rows, the fill pass writes each row and sorts it by j, so that
-// the springs come out ordered by (i, j).  pos.w carries the static flag.
+// sizes the rows, the fill pass writes each row sorted by j, so that the
+// springs come out ordered by (i, j).  pos.w carries the static flag.
+// Pairs linked by a spring loaded before are skipped, as the reference's
+// add_spring throws for them (SpringCollection.hpp:139-160).
 // ---------------------------------------------------------------------------
-__device__ __noinline__ bool topo_pair(float4 pi, float4 pj, double rc, double &d)
-{
-#pragma clang fp contract(off)
-    const double ax = (double)pj.x - (double)pi.x;
-    const double ay = (double)pj.y - (double)pi.y;
-    const double az = (double)pj.z - (double)pi.z;
-    d = sqrt(ax * ax + ay * ay + az * az);
-    return d < rc;
-}
-
 // Sorts a row (second indices j, all distinct, with their rest lengths) by
 // j: insertion sort for short rows, heapsort beyond, so that a crowded row
 // (many particles within the cutoff of one another) stays O(r log r).
@@ -994,49 +986,161 @@ __device__ void topo_sort_row(int32_t *j, float *l, int32_t r)
     }
 }
 
-__global__ void k_topo_springs(TopoArgs a)
+// Whether i < j are already linked by a loaded spring (binary search in the
+// sorted partners of row i).
+__device__ inline bool topo_linked(const int32_t *off, const int32_t *js, int32_t i, int32_t j)
 {
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    int32_t lo = off[i], hi = off[i + 1];
+    while (lo < hi)
+    {
+        const int32_t mid = (lo + hi) >> 1;
+        if (js[mid] < j)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo < off[i + 1] && js[lo] == j;
+}
+
+// One wave per particle i.  Its candidates are the members of the 3 x 3
+// rows of cells around i (each row of 3 cells is one contiguous run of the
+// sorted members); the runs are laid end to end so that each pass of the
+// wave fills all 64 lanes.  The pair test is on the squared distance: r2 <
+// r2max, the smallest double whose correctly rounded square root reaches the
+// cutoff, is the same test as sqrt(r2) < cutoff, and the square root is
+// taken only for hits (their rest length).  A ballot compacts the hits; the
+// fill pass ranks the row by j in LDS (rank = number of hits with a smaller
+// j; the js of a row are distinct) and writes it sorted.  Rows longer than
+// TOPO_CAP are written as found and sorted by k_topo_sort_long.
+constexpr int32_t TOPO_WAVES = 4;
+constexpr int32_t TOPO_CAP = 512;
+
+__device__ inline double topo_r2(float4 pi, float4 pj)
+{
+#pragma clang fp contract(off)
+    const double ax = (double)pj.x - (double)pi.x;
+    const double ay = (double)pj.y - (double)pi.y;
+    const double az = (double)pj.z - (double)pi.z;
+    return ax * ax + ay * ay + az * az;
+}
+
+template <bool FILL>
+__global__ __launch_bounds__(64 * TOPO_WAVES) void k_topo_rows(TopoArgs a)
+{
+    __shared__ int32_t sj[FILL ? TOPO_WAVES : 1][FILL ? TOPO_CAP : 1];
+    __shared__ float sl[FILL ? TOPO_WAVES : 1][FILL ? TOPO_CAP : 1];
+    const int32_t lane = (int32_t)(threadIdx.x & 63u), w = (int32_t)(threadIdx.x >> 6);
+    const int32_t i = (int32_t)blockIdx.x * TOPO_WAVES + w;
     if (i >= a.n)
-        return;
+        return; // the whole wave: no block-wide barrier below
     const float4 pi = a.pos[i];
     const bool si = fbits(pi.w) != 0u;
     int32_t c[3];
     const float xs[3] = {pi.x, pi.y, pi.z};
     for (int q = 0; q < 3; ++q)
         c[q] = internal_coord(xs[q], a.box[q], a.box[3 + q] - a.box[q], a.dims[q], a.edge);
+    int32_t base = 0, len = 0;
+    if (FILL)
+    {
+        base = a.offs[i];
+        len = a.offs[i + 1] - base;
+    }
+    const bool in_lds = len <= TOPO_CAP;
+    const uint64_t below = (1ull << lane) - 1ull;
+    // the 9 runs: start in the member list and end in the concatenation
+    const int32_t x0 = max(c[0] - 1, 0), x1 = min(c[0] + 1, a.dims[0] - 1);
+    int32_t rs[9], re[9];
+    int32_t tot = 0;
+#pragma unroll
+    for (int q = 0; q < 9; ++q)
+    {
+        const int32_t y = c[1] - 1 + q % 3, z = c[2] - 1 + q / 3;
+        int32_t b = 0, e = 0;
+        if (y >= 0 && y < a.dims[1] && z >= 0 && z < a.dims[2])
+        {
+            const int32_t row = y * a.dims[0] + z * a.dims[0] * a.dims[1];
+            b = a.cstart[row + x0];
+            e = a.cstart[row + x1 + 1];
+        }
+        rs[q] = b - tot; // member index = rs[q] + position in the concatenation
+        tot += e - b;
+        re[q] = tot;
+    }
     int32_t cnt = 0;
-    const int32_t base = a.offs ? a.offs[i] : 0;
-    for (int32_t z = max(c[2] - 1, 0); z <= min(c[2] + 1, a.dims[2] - 1); ++z)
-        for (int32_t y = max(c[1] - 1, 0); y <= min(c[1] + 1, a.dims[1] - 1); ++y)
-            for (int32_t x = max(c[0] - 1, 0); x <= min(c[0] + 1, a.dims[0] - 1); ++x)
+    for (int32_t t0 = 0; t0 < tot; t0 += 64)
+    {
+        const int32_t t = t0 + lane;
+        bool hit = false;
+        int32_t j = 0;
+        double r2 = 0.0;
+        if (t < tot)
+        {
+            int32_t m = rs[8] + t;
+#pragma unroll
+            for (int q = 7; q >= 0; --q)
+                m = t < re[q] ? rs[q] + t : m;
+            j = a.cidx[m];
+            if (j > i)
             {
-                const int32_t cell = x + y * a.dims[0] + z * a.dims[0] * a.dims[1];
-                for (int32_t t = a.cstart[cell]; t < a.cstart[cell + 1]; ++t)
+                const float4 pj = a.pos[j];
+                if (!(si && fbits(pj.w) != 0u))
                 {
-                    const int32_t j = a.cidx[t];
-                    if (j <= i)
-                        continue;
-                    const float4 pj = a.pos[j];
-                    if (si && fbits(pj.w) != 0u)
-                        continue;
-                    double d;
-                    if (!topo_pair(pi, pj, a.rc, d))
-                        continue;
-                    if (a.offs)
-                    {
-                        a.jout[base + cnt] = j;
-                        a.l0out[base + cnt] = (float)d;
-                    }
-                    ++cnt;
+                    r2 = topo_r2(pi, pj);
+                    // SpringAlreadyExistsException: the loaded spring stays
+                    hit = r2 < a.r2max && !(a.ex_off && topo_linked(a.ex_off, a.ex_j, i, j));
                 }
             }
-    if (!a.offs)
+        }
+        const uint64_t bal = __ballot(hit);
+        if (FILL && hit)
+        {
+            const int32_t slot = cnt + __popcll(bal & below);
+            const float l0 = (float)sqrt(r2);
+            if (slot < len)
+            {
+                if (in_lds)
+                {
+                    sj[w][slot] = j;
+                    sl[w][slot] = l0;
+                }
+                else
+                {
+                    a.jout[base + slot] = j;
+                    a.l0out[base + slot] = l0;
+                }
+            }
+        }
+        cnt += __popcll(bal);
+    }
+    if (!FILL)
     {
-        a.counts[i] = cnt;
+        if (lane == 0)
+            a.counts[i] = cnt;
         return;
     }
-    topo_sort_row(a.jout + base, a.l0out + base, cnt);
+    if (!in_lds)
+        return;
+    wave_lds_sync();
+    const int32_t m = min(cnt, len);
+    for (int32_t e = lane; e < m; e += 64)
+    {
+        const int32_t jv = sj[w][e];
+        int32_t rank = 0;
+        for (int32_t k = 0; k < m; ++k)
+            rank += sj[w][k] < jv ? 1 : 0;
+        a.jout[base + rank] = jv;
+        a.l0out[base + rank] = sl[w][e];
+    }
+}
+
+__global__ void k_topo_sort_long(const int32_t *offs, int32_t n, int32_t *jout, float *l0out)
+{
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const int32_t b = offs[i], len = offs[i + 1] - b;
+    if (len > TOPO_CAP)
+        topo_sort_row(jout + b, l0out + b, len);
 }
 
 __global__ void k_prestep(int32_t *status, int32_t *step, int32_t interval)
@@ -1172,8 +1276,17 @@ void launch_spring_out(const int32_t *slot_of, int64_t m, const double *dk, cons
 
 void launch_topo_springs(const TopoArgs &a, hipStream_t s)
 {
-    if (a.n > 0)
-        k_topo_springs<<<nblk(a.n), BLOCK, 0, s>>>(a);
+    if (a.n <= 0)
+        return;
+    const int32_t blocks = (a.n + TOPO_WAVES - 1) / TOPO_WAVES;
+    if (!a.offs)
+    {
+        k_topo_rows<false><<<blocks, 64 * TOPO_WAVES, 0, s>>>(a);
+        return;
+    }
+    k_topo_rows<true><<<blocks, 64 * TOPO_WAVES, 0, s>>>(a);
+    if (a.max_row > TOPO_CAP)
+        k_topo_sort_long<<<nblk(a.n), BLOCK, 0, s>>>(a.offs, a.n, a.jout, a.l0out);
 }
 
 void launch_drift_check(const float4 *pos, const float4 *xref, int32_t n, float half_skin2, int32_t *status,

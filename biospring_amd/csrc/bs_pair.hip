@@ -44,13 +44,6 @@ constexpr int32_t EX_HASH_MAX = EX_SLOTS / 2; // load factor <= 1/2
 
 __device__ inline uint32_t ex_hash(int32_t j) { return ((uint32_t)j * 0x9E3779B1u) >> 21; } // 11 bits
 
-__device__ inline void wave_lds_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 __device__ inline uint32_t bonded_mask_lds(const uint32_t *tab, int32_t j)
 {
     uint32_t h = ex_hash(j);

@@ -752,10 +752,12 @@ void launch_topo_springs(const TopoArgs &a, hipStream_t)
     const int64_t ncells = (int64_t)a.dims[0] * a.dims[1] * a.dims[2];
     need(a.cstart, sizeof(int32_t) * (ncells + 1), "topo: cstart");
     need(a.cidx, sizeof(int32_t) * n, "topo: cidx");
+    if (!(std::sqrt(a.r2max) >= a.rc && std::sqrt(std::nextafter(a.r2max, 0.0)) < a.rc))
+        hc_fail("topo: r2max is not the squared-distance threshold of the cutoff");
     if (!a.offs)
         need(a.counts, sizeof(int32_t) * n, "topo: counts");
     else
-        need(a.offs, sizeof(int32_t) * n, "topo: offs");
+        need(a.offs, sizeof(int32_t) * (n + 1), "topo: offs");
     for (int32_t i = 0; i < n; ++i)
     {
         const float4 pi = a.pos[i];
@@ -786,8 +788,19 @@ void launch_topo_springs(const TopoArgs &a, hipStream_t)
                         s2 = s2 + ay * ay;
                         s2 = s2 + az * az;
                         const double d = std::sqrt((double)s2);
-                        if (d < a.rc)
-                            row.push_back({j, (float)d});
+                        if (!(d < a.rc))
+                            continue;
+                        if (a.ex_off)
+                        {
+                            need(a.ex_off + i, 8, "topo: ex_off[i..i+1]");
+                            const int32_t b = a.ex_off[i], e = a.ex_off[i + 1];
+                            need(a.ex_j + b, sizeof(int32_t) * (e - b), "topo: ex_j row");
+                            if (!std::is_sorted(a.ex_j + b, a.ex_j + e))
+                                hc_fail("topo: existing partners not sorted");
+                            if (std::binary_search(a.ex_j + b, a.ex_j + e, j))
+                                continue;
+                        }
+                        row.push_back({j, (float)d});
                     }
                 }
         if (!a.offs)
@@ -795,6 +808,10 @@ void launch_topo_springs(const TopoArgs &a, hipStream_t)
             a.counts[i] = (int32_t)row.size();
             continue;
         }
+        if ((int64_t)row.size() != (int64_t)a.offs[i + 1] - a.offs[i])
+            hc_fail("topo: fill pass row length differs from the count pass");
+        if ((int32_t)row.size() > a.max_row)
+            hc_fail("topo: row longer than max_row");
         std::sort(row.begin(), row.end(), [](auto &u, auto &v) { return u.first < v.first; });
         need(a.jout + a.offs[i], sizeof(int32_t) * row.size(), "topo: jout");
         need(a.l0out + a.offs[i], sizeof(float) * row.size(), "topo: l0out");

@@ -162,7 +162,7 @@ class SpringNetwork:
         n = np.zeros(_abi.PROF_NUM, np.int64)
         check(self._L.bs_get_profile(self._h, ptr(ms, ctypes.c_double), ptr(n, ctypes.c_int64)), self._h,
               "bs_get_profile")
-        names = ("rebuild", "pair", "spring", "adj_pair", "adj_spring")
+        names = ("rebuild", "pair", "spring", "adj_pair", "adj_spring", "topo_count", "topo_fill")
         return {names[c]: {"ms": float(ms[c]), "launches": int(n[c])} for c in range(_abi.PROF_NUM)}
 
     def lastNonfiniteParticle(self):

@@ -168,10 +168,15 @@ const char *bs_last_error(const bs_handle *h);
  * positions rounded to float (Spring.hpp:47-56, then the float of
  * SpringNetwork::addSpring) and the given stiffness.  The pair test is the
  * reference's: FP64 distance < the cutoff as a float (NeighborSearch keeps
- * it as a float, nsearch.hpp:155-230).  Built on the GPU; replaces the
- * springs of bs_set_springs.  Springs come ordered by (first, second)
- * original id, first < second; the reference orders them by first id, then
- * by its cell scan.  *count_out (nullable) = number of springs. */
+ * it as a float, nsearch.hpp:57,230); cutoffs under 1e-6 (nsearch.hpp:66-67),
+ * NaN or infinite -> BS_ERR_ARG.  Built on the GPU.  Springs already loaded
+ * (bs_set_springs, an earlier call) stay as they are, and their pairs get no
+ * new spring, as the reference's add_spring refuses an existing pair
+ * (SpringCollection.hpp:138-151,226-233); bs_set_springs with m = 0 clears
+ * them first.  New springs follow the loaded ones, ordered by (first,
+ * second) original id, first < second; the reference orders them by first
+ * id, then by its cell scan.  *count_out (nullable) = number of springs
+ * added. */
 int bs_springs_from_cutoff(bs_handle *h, double cutoff, float stiffness, int64_t *count_out);
 
 /* The loaded springs (bs_set_springs or bs_springs_from_cutoff order);
@@ -201,7 +206,9 @@ int bs_set_particles(bs_handle *h, int64_t n, const float *x3, const float *v3,
  * and stiffness[s] (kJ/mol/A^2).  Ids out of range or equal -> BS_ERR_ARG;
  * a repeated pair is ignored, like the reference's isInSpringNeighbors test.
  * The force acts along x[id2]-x[id1] exactly as Spring::computeForce
- * (Spring.cpp:20-34). */
+ * (Spring.cpp:20-34).  Replaces the loaded springs (m = 0 clears them); the
+ * current positions and velocities are kept.  A rejected call changes
+ * nothing. */
 int bs_set_springs(bs_handle *h, int64_t m, const uint32_t *id1, const uint32_t *id2,
                    const float *equilibrium, const float *stiffness);
 
@@ -260,7 +267,9 @@ int bs_get_counters(const bs_handle *h, int64_t *out6);
 #define BS_PROF_SPRING  2  /* spring gather + integrator (or force epilogue) */
 #define BS_PROF_ADJ_PAIR   3  /* bs_adjoint: pair part of H w and dL/dq */
 #define BS_PROF_ADJ_SPRING 4  /* bs_adjoint: spring part, parameter gradients, reverse update */
-#define BS_PROF_NUM     5
+#define BS_PROF_TOPO_COUNT 5  /* bs_springs_from_cutoff: binning and the count pass */
+#define BS_PROF_TOPO_FILL  6  /* bs_springs_from_cutoff: the fill pass (rows written and sorted) */
+#define BS_PROF_NUM     7
 
 /* Turns per-kernel-class event timing of bs_step and bs_adjoint on (1) or off
  * (0) and resets the totals.  Counterpart of the reference's wall-clock timeit::Profiler

@@ -120,8 +120,9 @@ def ref_lib():
         L.ref_set_grid.argtypes = [_vp, ctypes.c_int, _f64p, _i64p, _f64p, _f32p, ctypes.c_double]
         L.ref_set_burying.argtypes = [_vp, _f32p]
         L.ref_insertion.argtypes = [_vp, ctypes.c_int64, ctypes.c_int64, _f64p]
-        L.ref_springs_from_cutoff.argtypes = [ctypes.c_int64, _f32p, _u8p, ctypes.c_double, ctypes.c_double,
-                                              ctypes.c_int64, _u32p, _u32p, _f32p]
+        L.ref_springs_from_cutoff.argtypes = [ctypes.c_int64, _f32p, _u8p, ctypes.c_int64, _u32p, _u32p, _f32p, _f32p,
+                                              ctypes.c_double, ctypes.c_double, ctypes.c_int64, _u32p, _u32p, _f32p,
+                                              _f32p]
         L.ref_springs_from_cutoff.restype = ctypes.c_int64
         _rlib = L
     return _rlib
@@ -375,29 +376,42 @@ def ref_pairs(system, kind, params=None):
     return out[:2 * cnt].reshape(-1, 2)
 
 
-def _springs_call(fn, x, flags, cutoff, *extra):
+def springs_from_cutoff(x, flags, cutoff):
+    """Oracle: springs of Topology::add_springs_from_cutoff ordered by
+    (first, second), first < second, with float rest lengths."""
     x = np.ascontiguousarray(x, np.float32).reshape(-1)
     n = x.size // 3
     fl = None if flags is None else np.ascontiguousarray(flags, np.uint8)
     c = ctypes
-    cnt = fn(n, _p(x, c.c_float), _p(fl, c.c_uint8), cutoff, *extra, 0, None, None, None)
+    fn = oracle_lib().orc_springs_from_cutoff
+    cnt = fn(n, _p(x, c.c_float), _p(fl, c.c_uint8), cutoff, 0, None, None, None)
     if cnt < 0:
         raise ValueError(f"cutoff {cutoff} rejected")
-    i1 = np.zeros(max(cnt, 1), np.uint32)
-    i2 = np.zeros(max(cnt, 1), np.uint32)
+    i1, i2 = np.zeros(max(cnt, 1), np.uint32), np.zeros(max(cnt, 1), np.uint32)
     l0 = np.zeros(max(cnt, 1), np.float32)
-    fn(n, _p(x, c.c_float), _p(fl, c.c_uint8), cutoff, *extra, cnt, _p(i1, c.c_uint32), _p(i2, c.c_uint32),
-       _p(l0, c.c_float))
+    fn(n, _p(x, c.c_float), _p(fl, c.c_uint8), cutoff, cnt, _p(i1, c.c_uint32), _p(i2, c.c_uint32), _p(l0, c.c_float))
     return i1[:cnt], i2[:cnt], l0[:cnt]
 
 
-def springs_from_cutoff(x, flags, cutoff):
-    """Oracle: springs of Topology::add_springs_from_cutoff ordered by
-    (first, second), first < second, with float rest lengths."""
-    return _springs_call(oracle_lib().orc_springs_from_cutoff, x, flags, cutoff)
-
-
-def ref_springs_from_cutoff(x, flags, cutoff, stiffness=1.0):
-    """The reference's own Topology::add_springs_from_cutoff, springs in its
-    order as (first, second, float(equilibrium))."""
-    return _springs_call(ref_lib().ref_springs_from_cutoff, x, flags, cutoff, stiffness)
+def ref_springs_from_cutoff(x, flags, cutoff, stiffness=1.0, existing=None):
+    """The reference's own Topology::add_springs_from_cutoff, run after the
+    `existing` springs (id1, id2, l0, k) were added with add_spring; all
+    springs in its order as (first, second, float(equilibrium)), plus
+    float(stiffness) when `existing` is given."""
+    x = np.ascontiguousarray(x, np.float32).reshape(-1)
+    n = x.size // 3
+    fl = None if flags is None else np.ascontiguousarray(flags, np.uint8)
+    c = ctypes
+    ex = [None] * 4 if existing is None else [np.ascontiguousarray(a, t) for a, t in
+                                               zip(existing, (np.uint32, np.uint32, np.float32, np.float32))]
+    m0 = 0 if existing is None else len(ex[0])
+    head = [n, _p(x, c.c_float), _p(fl, c.c_uint8), m0, _p(ex[0], c.c_uint32), _p(ex[1], c.c_uint32),
+            _p(ex[2], c.c_float), _p(ex[3], c.c_float), cutoff, stiffness]
+    fn = ref_lib().ref_springs_from_cutoff
+    cnt = fn(*head, 0, None, None, None, None)
+    i1, i2 = np.zeros(max(cnt, 1), np.uint32), np.zeros(max(cnt, 1), np.uint32)
+    l0, k = np.zeros(max(cnt, 1), np.float32), np.zeros(max(cnt, 1), np.float32)
+    fn(*head, cnt, _p(i1, c.c_uint32), _p(i2, c.c_uint32), _p(l0, c.c_float), _p(k, c.c_float))
+    if existing is None:
+        return i1[:cnt], i2[:cnt], l0[:cnt]
+    return i1[:cnt], i2[:cnt], l0[:cnt], k[:cnt]

@@ -25,8 +25,9 @@
 //    (src/InsertionVector.cpp:18-78).
 //  - ref_springs_from_cutoff: topology::Topology::add_springs_from_cutoff
 //    (src/topology/Topology.hpp:105-138) on particles at the given positions,
-//    the springs read back as (first, second, float(equilibrium)) in the
-//    reference's own order.
+//    after the given springs were added with Topology::add_spring; all
+//    springs read back as (first, second, float(equilibrium),
+//    float(stiffness)) in the reference's own order.
 #include "SpringNetwork.h"
 #include "configuration/Configuration.hpp"
 #include "forcefield/ForceField.h"
@@ -520,8 +521,9 @@ int64_t ref_pairs(const bs_params * p, int kind, int64_t n, const float * x3, co
 // Springs of Topology::add_springs_from_cutoff: returns the count; with cap
 // >= count, writes id1/id2 (indices in the given order) and the rest lengths
 // as SpringNetwork::addSpring receives them (float).
-int64_t ref_springs_from_cutoff(int64_t n, const float * x3, const uint8_t * flags, double cutoff, double stiffness,
-                                int64_t cap, uint32_t * id1, uint32_t * id2, float * l0)
+int64_t ref_springs_from_cutoff(int64_t n, const float * x3, const uint8_t * flags, int64_t m0, const uint32_t * e1,
+                                const uint32_t * e2, const float * el0, const float * ek, double cutoff,
+                                double stiffness, int64_t cap, uint32_t * id1, uint32_t * id2, float * l0, float * k)
 {
     using biospring::topology::Topology;
     using biospring::topology::Particle;
@@ -536,6 +538,8 @@ int64_t ref_springs_from_cutoff(int64_t n, const float * x3, const uint8_t * fla
     std::unordered_map<int64_t, uint32_t> index; // the collection copies particles with new ids
     for (int64_t i = 0; i < n; ++i)
         index[(int64_t)top.get_particle((size_t)i).unique_id()] = (uint32_t)i;
+    for (int64_t s = 0; s < m0; ++s) // springs present before, e.g. from CONECT records
+        top.add_spring((size_t)e1[s], (size_t)e2[s], (double)el0[s], (double)ek[s]);
     top.add_springs_from_cutoff(cutoff, stiffness);
     const int64_t m = (int64_t)top.number_of_springs();
     if (cap >= m)
@@ -545,6 +549,7 @@ int64_t ref_springs_from_cutoff(int64_t n, const float * x3, const uint8_t * fla
             id1[s] = index.at((int64_t)sp.first().unique_id());
             id2[s] = index.at((int64_t)sp.second().unique_id());
             l0[s] = (float)sp.equilibrium();
+            k[s] = (float)sp.stiffness();
         }
     return m;
 }

@@ -326,6 +326,34 @@ def topology_cases():
         np.savez_compressed(path, in_x=x, in_flags=flags, cutoff=np.float64(rc), ref_id1=i1, ref_id2=i2, ref_l0=l0)
         print(f"{name}: n={len(x)} springs={len(i1)} -> {os.path.getsize(path) / 1024:.0f} KiB")
 
+    # springs present before the call (CONECT records read by pdb2spn,
+    # pdb2spn-cli.cpp:59-70): they keep their rest length and stiffness, and
+    # their pairs get no second spring (SpringCollection.hpp:138-151)
+    name = "topo_merge800"
+    if ONLY and name not in ONLY:
+        return
+    rng = np.random.default_rng(2026)
+    n, rc = 800, 6.5
+    x = rng.uniform(-20, 20, (n, 3)).astype(np.float32)
+    flags = np.where(rng.random(n) < 0.1, _abi.FLAG_STATIC, 0).astype(np.uint8)
+    xd = x.astype(np.float64)
+    dist = np.sqrt(((xd[:, None, :] - xd[None, :, :]) ** 2).sum(-1))
+    st = flags != 0
+    ii, jj = np.triu_indices(n, 1)
+    near = dist[ii, jj] < rc
+    pick = lambda mask, k: rng.choice(np.nonzero(mask)[0], k, replace=False)  # noqa: E731
+    sel = np.concatenate([pick(near & ~(st[ii] & st[jj]), 25), pick(~near, 20), pick(near & st[ii] & st[jj], 5)])
+    e1, e2 = ii[sel].astype(np.uint32), jj[sel].astype(np.uint32)
+    swap = rng.random(len(sel)) < 0.5  # either particle first
+    e1, e2 = np.where(swap, e2, e1), np.where(swap, e1, e2)
+    el0 = rng.uniform(2, 9, len(sel)).astype(np.float32)
+    ek = rng.uniform(0.5, 3, len(sel)).astype(np.float32)
+    i1, i2, l0, k = po.ref_springs_from_cutoff(x, flags, rc, 0.35, existing=(e1, e2, el0, ek))
+    path = os.path.join(OUT, f"{name}.npz")
+    np.savez_compressed(path, in_x=x, in_flags=flags, cutoff=np.float64(rc), stiffness=np.float64(0.35),
+                        pre_id1=e1, pre_id2=e2, pre_l0=el0, pre_k=ek, ref_id1=i1, ref_id2=i2, ref_l0=l0, ref_k=k)
+    print(f"{name}: n={n} springs={len(i1)} ({len(e1)} before) -> {os.path.getsize(path) / 1024:.0f} KiB")
+
 
 if __name__ == "__main__":
     main()

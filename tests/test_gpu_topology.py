@@ -13,7 +13,7 @@ import pytest
 from biospring_amd import _abi, workloads
 from biospring_amd.network import SpringNetwork
 from oracle import pyoracle as po
-from tests.test_topology import CASES, brute_force, load_case, same_springs
+from tests.test_topology import CASES, brute_force, check_merged, load_case, same_springs
 
 pytestmark = pytest.mark.gpu
 
@@ -44,6 +44,30 @@ def test_matches_reference_fixture(name):
     net.close()
 
 
+def test_keeps_loaded_springs_like_the_reference():
+    """Springs loaded before (bs_set_springs) stay first with their own rest
+    length and stiffness; their pairs get no second spring; positions are
+    the current ones after stepping."""
+    c = load_case("topo_merge800")
+    _log("topology merge fixture")
+    net = SpringNetwork(workloads.params(1))
+    net.setParticles(c["in_x"], flags=c["in_flags"])
+    net.setSprings(c["pre_id1"], c["pre_id2"], c["pre_l0"], c["pre_k"])
+    net.run(3)
+    net.setState(c["in_x"], None)
+    added = net.springsFromCutoff(float(c["cutoff"]), float(c["stiffness"]))
+    got = net.getSprings()
+    assert added == len(got[0]) - len(c["pre_id1"])
+    check_merged(got, c)
+    # after steps the construction sees the moved positions
+    net.run(5)
+    x, _ = net.getState()
+    net.setSprings([], [], [], [])
+    net.springsFromCutoff(float(c["cutoff"]))
+    same_springs(net.getSprings()[:3], po.springs_from_cutoff(x, c["in_flags"], float(c["cutoff"])))
+    net.close()
+
+
 @pytest.mark.parametrize("seed,n,cutoff,span,static_frac", [(11, 20000, 7.1, 60.0, 0.0), (12, 20000, 9.3, 60.0, 0.2),
                                                             (13, 8000, 12.0, 35.0, 0.5), (14, 3000, 3.0, 400.0, 0.1)])
 def test_matches_oracle_random(seed, n, cutoff, span, static_frac):
@@ -66,12 +90,15 @@ def _clumps():
 
 def test_degenerate_layouts():
     # a line of particles (zero extent on two axes), a lone particle, all
-    # particles at one point, and a sparse cloud over a box of 256+ cells a side
+    # particles at one point, clumps over a box of 256+ cells a side, and a
+    # crowd in which every pair is linked
     cases = [(np.stack([np.arange(500, dtype=np.float32) * 0.7, np.zeros(500, np.float32),
                         np.zeros(500, np.float32)], 1), 2.0),
              (np.array([[1.0, 2.0, 3.0]], np.float32), 5.0),
              (np.full((40, 3), 3.25, np.float32), 1.0),
-             (_clumps(), 6.0)]
+             (_clumps(), 6.0),
+             # rows of up to 1,499 springs: longer than a wave sorts in LDS
+             (np.random.default_rng(4).uniform(-1.5, 1.5, (1500, 3)).astype(np.float32), 10.0)]
     for x, rc in cases:
         _log(f"topology degenerate: n={len(x)} cutoff={rc}")
         net, got = _gpu_springs(x, None, rc)

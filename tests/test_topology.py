@@ -76,6 +76,42 @@ def test_oracle_matches_reference_fixture(name):
     assert np.all(np.diff(key) > 0)
 
 
+def merged_expectation(c, new):
+    """Springs after construction over loaded ones: the loaded springs in
+    their order, then the `new` (first, second, l0) springs whose pairs were
+    not loaded, with stiffness c["stiffness"]."""
+    pre = {(min(a, b), max(a, b)) for a, b in zip(c["pre_id1"].tolist(), c["pre_id2"].tolist())}
+    keep = np.array([(int(a), int(b)) not in pre for a, b in zip(new[0], new[1])], bool)
+    m = int(keep.sum())
+    return (np.concatenate([c["pre_id1"], new[0][keep]]), np.concatenate([c["pre_id2"], new[1][keep]]),
+            np.concatenate([c["pre_l0"], new[2][keep]]),
+            np.concatenate([c["pre_k"], np.full(m, np.float32(c["stiffness"]))]))
+
+
+def check_merged(got, c):
+    """got = (id1, id2, l0, k) of the engine: the loaded springs first, in
+    order and unchanged, then the new ones equal as a set to the reference's."""
+    m0 = len(c["pre_id1"])
+    assert len(got[0]) == len(c["ref_id1"])
+    for g, key in zip(got, ("id1", "id2", "l0", "k")):
+        assert np.array_equal(np.asarray(g)[:m0], c["pre_" + key]), key
+        assert np.array_equal(np.asarray(c["ref_" + key])[:m0], c["pre_" + key]), key
+    same_springs(tuple(np.asarray(g)[m0:] for g in got[:3]), (c["ref_id1"][m0:], c["ref_id2"][m0:], c["ref_l0"][m0:]))
+    assert np.all(np.asarray(got[3])[m0:] == np.float32(c["stiffness"]))
+    assert np.all(c["ref_k"][m0:] == np.float32(c["stiffness"]))
+
+
+def test_oracle_merge_fixture():
+    """Springs loaded before keep their rest length and stiffness (also
+    beyond the cutoff and between static particles); pairs they link get no
+    second spring."""
+    c = load_case("topo_merge800")
+    new = po.springs_from_cutoff(c["in_x"], c["in_flags"], float(c["cutoff"]))
+    exp = merged_expectation(c, new)
+    check_merged(exp, c)
+    assert len(new[0]) - (len(exp[0]) - len(c["pre_id1"])) == 25  # the loaded pairs within the cutoff
+
+
 @pytest.mark.parametrize("name", CASES)
 def test_fixture_matches_brute_force(name):
     """The reference's own test asserts that every pair closer than the
@@ -181,12 +217,29 @@ def test_host_construction_loads_the_fixture_springs(hostcheck_lib, tmp_path):
             net.springsFromCutoff(3.0)      # rebuilt over the moved positions
             net.run(1)
             net.close()
+        z = np.load({GOLDEN!r} + "/topo_merge800.npz")
+        net = SpringNetwork(workloads.params(1), library=L)
+        net.setParticles(z["in_x"], flags=z["in_flags"])
+        net.setSprings(z["pre_id1"], z["pre_id2"], z["pre_l0"], z["pre_k"])
+        net.run(1)                          # device state live: construction pulls it first
+        net.setState(z["in_x"], None)
+        added = net.springsFromCutoff(float(z["cutoff"]), float(z["stiffness"]))
+        got = net.getSprings()
+        assert added == len(got[0]) - len(z["pre_id1"])
+        for key, g in zip(("a", "b", "l0", "k"), got):
+            res["merge_" + key] = g
+        net.setSprings([], [], [], [])      # m = 0 clears the table
+        assert net.getNumberOfSprings() == 0
+        assert net.springsFromCutoff(float(z["cutoff"])) == len(got[0]) - 25
+        net.run(1)
+        net.close()
         assert L.hc_live_allocations() == base
         np.savez({str(out)!r}, **res)
         """)
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
     got = np.load(out)
+    check_merged(tuple(got["merge_" + k] for k in ("a", "b", "l0", "k")), load_case("topo_merge800"))
     for name in CASES:
         c = load_case(name)
         same_springs((got[name + "_a"], got[name + "_b"], got[name + "_l0"]),
