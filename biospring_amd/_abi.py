@@ -23,6 +23,7 @@ NUM_ENERGIES = 6
 NS_STERIC, NS_ELECTROSTATIC, NS_HYDROPHOBIC = range(3)
 PROF_NUM = 7  # BS_PROF_REBUILD, _PAIR, _SPRING, _ADJ_PAIR, _ADJ_SPRING, _TOPO_COUNT, _TOPO_FILL
 GRID_POTENTIAL, GRID_DENSITY = 0, 1
+TRAJ_PDB, TRAJ_XTC, TRAJ_CSV = 0, 1, 2
 
 # Exported symbols, one per declaration in include/bs_api.h.
 EXPORTS = (
@@ -33,7 +34,8 @@ EXPORTS = (
     "bs_get_cells", "bs_get_pairs", "bs_add_rigid_body", "bs_num_rigid_bodies", "bs_get_rigid_body",
     "bs_set_grid", "bs_set_burying", "bs_get_insertion_vector", "bs_set_systems", "bs_num_systems",
     "bs_get_system_energies", "bs_autotune", "bs_adjoint", "bs_adjoint_forward", "bs_adjoint_backward",
-    "bs_springs_from_cutoff", "bs_get_springs",
+    "bs_springs_from_cutoff", "bs_get_springs", "bs_add_trajectory", "bs_write_frame_now",
+    "bs_flush_trajectories", "bs_close_trajectories", "bs_set_particle_labels", "bs_set_insertion_vector",
 )
 PROF_REBUILD, PROF_PAIR, PROF_SPRING = range(3)
 
@@ -162,6 +164,13 @@ def bind(L):
     L.bs_springs_from_cutoff.argtypes = [_vp, ctypes.c_double, ctypes.c_float, _i64p]
     L.bs_get_springs.argtypes = [_vp, _u32p, _u32p, _f32p, _f32p]
     L.bs_adjoint_backward.argtypes = [_vp, _f64p, _f64p, _f64p, _f64p, _f64p]
+    L.bs_add_trajectory.argtypes = [_vp, ctypes.c_int32, ctypes.c_char_p, ctypes.c_int64]
+    L.bs_write_frame_now.argtypes = [_vp]
+    L.bs_flush_trajectories.argtypes = [_vp]
+    L.bs_close_trajectories.argtypes = [_vp]
+    _strs = ctypes.POINTER(ctypes.c_char_p)
+    L.bs_set_particle_labels.argtypes = [_vp, _strs, _strs, _strs, _i32p, _f32p, _f32p, _strs, _i32p]
+    L.bs_set_insertion_vector.argtypes = [_vp, ctypes.c_int64, ctypes.c_int64]
     return L
 
 

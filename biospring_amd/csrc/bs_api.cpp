@@ -4,6 +4,7 @@
 // CSR and the neighbour lists, and sequences the kernels of one step in the
 // order of SpringNetwork::computeStep (src/spn/SpringNetwork.cpp:251-268).
 #include "bs_engine.hpp"
+#include "bs_traj.hpp"
 #include "bs_params_defaults.h"
 
 #include <algorithm>
@@ -12,6 +13,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <numeric>
 #include <string>
@@ -167,6 +169,16 @@ struct bs_handle
     std::vector<Span> spans;
     double prof_ms[BS_PROF_NUM] = {};
     int64_t prof_n[BS_PROF_NUM] = {};
+
+    // trajectory output (bs_add_trajectory): writers, pinned frame slots, and
+    // the device buffer the positions are packed into
+    std::unique_ptr<bs::TrajectorySet> traj;
+    bs::TrajLabels labels;
+    int64_t ivec1 = -1, ivec2 = -1;
+    float *traj_stage = nullptr;
+    std::vector<float *> traj_x;
+    std::vector<double *> traj_e;
+    std::vector<hipEvent_t> traj_ev;
 };
 
 namespace
@@ -203,6 +215,31 @@ void drop_graph(bs_handle *h)
     h->graph_exec = nullptr;
     h->graph = nullptr;
     h->graph_len = 0;
+}
+
+constexpr int TRAJ_SLOTS = 3; // frames in flight between the stream and the writer thread
+
+// Writes out the queued frames, closes the files and frees the slots.
+void traj_release(bs_handle *h)
+{
+    if (h->traj)
+        h->traj->close();
+    h->traj.reset();
+    for (float *p : h->traj_x)
+        (void)hipHostFree(p);
+    for (double *p : h->traj_e)
+        (void)hipHostFree(p);
+    for (hipEvent_t e : h->traj_ev)
+        (void)hipEventDestroy(e);
+    h->traj_x.clear();
+    h->traj_e.clear();
+    h->traj_ev.clear();
+    if (h->traj_stage)
+    {
+        (void)hipStreamSynchronize(h->stream);
+        (void)hipFree(h->traj_stage);
+        h->traj_stage = nullptr;
+    }
 }
 
 void tape_free(bs_handle::Tape &tp)
@@ -804,10 +841,56 @@ int prof_fold(bs_handle *h)
     return BS_OK;
 }
 
-int finish_call(bs_handle *h)
+// The six energies of the last energy step, summed into energies_d.
+void launch_energy_sum(bs_handle *h)
 {
     launch_energies(h->lists.empty() ? nullptr : h->epp, h->ese, h->seg_start, h->nseg, h->sys_seg,
                     (int32_t)h->sys_off.size() - 1, h->seg_part, h->sys_E, h->energies_d, h->stream);
+}
+
+// Queues a trajectory frame of the current state at iteration nb: positions
+// packed into particle order and copied to a pinned slot, with the energies
+// of the last energy step (on the device when dev_energies, else the host
+// copy of the last call's), an event marking the copies; the writer thread
+// writes it when the event has passed.  all: to every writer.
+int traj_capture(bs_handle *h, int64_t nb, bool dev_energies, bool all)
+{
+    const int64_t n = h->n;
+    if (!h->traj_stage)
+        HIPCHK(h, hipMalloc((void **)&h->traj_stage, sizeof(float) * 3 * n));
+    const int s = h->traj->acquire();
+    hipError_t e = hipSuccess;
+    launch_pack_positions(h->pos[h->cur], h->p.compensated_positions ? h->plo[h->cur] : nullptr, h->perm_d,
+                          (int32_t)n, h->traj_stage, h->stream);
+    e = hipMemcpyAsync(h->traj_x[(size_t)s], h->traj_stage, sizeof(float) * 3 * n, hipMemcpyDeviceToHost,
+                       h->stream);
+    if (e == hipSuccess && dev_energies)
+    {
+        launch_energy_sum(h);
+        e = hipMemcpyAsync(h->traj_e[(size_t)s], h->energies_d, sizeof(double) * BS_NUM_ENERGIES,
+                           hipMemcpyDeviceToHost, h->stream);
+    }
+    else if (e == hipSuccess)
+        std::memcpy(h->traj_e[(size_t)s], h->energies, sizeof(double) * BS_NUM_ENERGIES);
+    if (e == hipSuccess)
+        e = hipEventRecord(h->traj_ev[(size_t)s], h->stream);
+    if (e != hipSuccess)
+    {
+        h->traj->release(s);
+        HIPCHK(h, e);
+    }
+    bs::TrajFrame f;
+    f.nbiter = nb;
+    f.x3 = h->traj_x[(size_t)s];
+    f.energies = h->traj_e[(size_t)s];
+    const hipEvent_t ev = h->traj_ev[(size_t)s];
+    h->traj->submit(s, f, [ev] { (void)hipEventSynchronize(ev); }, all);
+    return BS_OK;
+}
+
+int finish_call(bs_handle *h)
+{
+    launch_energy_sum(h);
     int32_t st[4];
     HIPCHK(h, hipMemcpyAsync(h->energies, h->energies_d, sizeof(h->energies), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipMemcpyAsync(st, h->status, sizeof st, hipMemcpyDeviceToHost, h->stream));
@@ -854,6 +937,10 @@ int capture_graph(bs_handle *h, int32_t G)
 {
     drop_graph(h);
     HIPCHK(h, hipStreamSynchronize(h->stream));
+    // the trajectory writer thread waits on events of this stream: let it
+    // finish first, as a wait on a capturing stream's event ends the capture
+    if (h->traj)
+        h->traj->drain();
     HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed));
     const int cur0 = h->cur;
     for (int32_t s = 0; s < G; ++s)
@@ -1707,6 +1794,7 @@ int bs_destroy(bs_handle *h)
         return BS_OK;
     if (h->stream)
         (void)hipStreamSynchronize(h->stream);
+    traj_release(h);
     for (hipEvent_t e : h->evpool)
         (void)hipEventDestroy(e);
     free_device(h);
@@ -1734,6 +1822,9 @@ int bs_set_particles(bs_handle *h, int64_t n, const float *x3, const float *v3, 
         return BS_ERR_ARG;
     if (n <= 0 || !x3)
         return fail(h, BS_ERR_ARG, "bs_set_particles: empty particle list");
+    traj_release(h); // writers follow one particle set
+    h->labels = bs::TrajLabels{};
+    h->ivec1 = h->ivec2 = -1;
     free_device(h);
     h->n = n;
     auto take = [n](std::vector<float> &dst, const float *src, int64_t per, float dflt) {
@@ -1849,14 +1940,27 @@ int bs_step(bs_handle *h, int32_t nsteps, const float *ext_force3)
     if (rc)
         return rc;
     h->tape.ready = false;
-    int32_t s = 0;
+    // Trajectory frames are taken at the start of the steps whose iteration
+    // count a writer's frequency divides (SpringNetwork::idleRun,
+    // SpringNetwork.cpp:344-362): the state then, with the energies of the
+    // step before, so that step computes its energies.
+    const bool tr = h->traj && !h->traj->empty();
     const int32_t G = (h->p.graph_steps + 1) / 2 * 2;
-    if (G > 0 && !h->prof && nsteps > G)
+    const bool graphs = G > 0 && !h->prof && nsteps > G;
+    int32_t s = 0;
+    while (s < nsteps)
     {
+        const int64_t nb = h->steps_done + s;
+        if (tr && h->traj->due(nb))
+        {
+            rc = traj_capture(h, nb, s > 0, false);
+            if (rc)
+                return rc;
+        }
         // replay a captured graph of G steps (from cur = 0 back to cur = 0)
-        for (; s < nsteps - 1 && h->cur != 0; ++s)
-            launch_step(h, false);
-        if (nsteps - 1 - s >= G)
+        // when no frame falls inside it or right after it; the last step stays
+        // outside, for the energies
+        if (graphs && h->cur == 0 && nsteps - 1 - s >= G && (!tr || h->traj->next_due(nb) > nb + G))
         {
             if (!h->graph_exec || h->graph_len != G || h->graph_ext != h->ext_uploaded)
             {
@@ -1864,16 +1968,15 @@ int bs_step(bs_handle *h, int32_t nsteps, const float *ext_force3)
                 if (rc)
                     return rc;
             }
-            for (; nsteps - 1 - s >= G; s += G)
-                HIPCHK(h, hipGraphLaunch(h->graph_exec, h->stream));
+            HIPCHK(h, hipGraphLaunch(h->graph_exec, h->stream));
+            s += G;
+            continue;
         }
-    }
-    for (; s < nsteps; ++s)
-    {
         const int frc = prof_fold(h);
         if (frc)
             return frc;
-        launch_step(h, s + 1 == nsteps);
+        launch_step(h, s + 1 == nsteps || (tr && h->traj->due(nb + 1)));
+        ++s;
     }
     h->steps_done += nsteps;
     h->last_call_forces_only = false;
@@ -2877,6 +2980,157 @@ int bs_autotune(bs_handle *h, int32_t steps, int32_t *group_out, int32_t *graph_
         *graph_out = bgr;
     if (layout_out)
         *layout_out = h->p.pair_layout;
+    return BS_OK;
+}
+
+// ---- trajectory output ------------------------------------------------------
+
+int bs_set_particle_labels(bs_handle *h, const char *const *name, const char *const *resname,
+                           const char *const *chain, const int32_t *resid, const float *occupancy,
+                           const float *tempfactor, const char *const *element, const int32_t *electron_charge)
+{
+    if (!h)
+        return BS_ERR_ARG;
+    if (!h->have_particles)
+        return fail(h, BS_ERR_STATE, "bs_set_particle_labels: particles must be set first");
+    const size_t n = (size_t)h->n;
+    auto strs = [n](std::vector<std::string> &dst, const char *const *src) {
+        dst.clear();
+        if (src)
+            for (size_t i = 0; i < n; ++i)
+                dst.emplace_back(src[i] ? src[i] : "");
+    };
+    bs::TrajLabels L;
+    strs(L.name, name);
+    strs(L.resname, resname);
+    strs(L.chain, chain);
+    strs(L.element, element);
+    if (resid)
+        L.resid.assign(resid, resid + n);
+    if (occupancy)
+        L.occupancy.assign(occupancy, occupancy + n);
+    if (tempfactor)
+        L.tempfactor.assign(tempfactor, tempfactor + n);
+    if (electron_charge)
+        L.echarge.assign(electron_charge, electron_charge + n);
+    h->labels = std::move(L);
+    return BS_OK;
+}
+
+int bs_set_insertion_vector(bs_handle *h, int64_t i1, int64_t i2)
+{
+    if (!h)
+        return BS_ERR_ARG;
+    if (i1 < 0 && i2 < 0)
+    {
+        h->ivec1 = h->ivec2 = -1;
+        return BS_OK;
+    }
+    if (i1 < 0 || i2 < 0 || i1 >= h->n || i2 >= h->n)
+        return fail(h, BS_ERR_ARG, "Insertion vector could not be initialized. One or both of the pair of "
+                                   "particles defining the insertion vector have an invalid identifier.");
+    h->ivec1 = i1;
+    h->ivec2 = i2;
+    return BS_OK;
+}
+
+int bs_add_trajectory(bs_handle *h, int32_t kind, const char *path, int64_t frequency)
+{
+    if (!h)
+        return BS_ERR_ARG;
+    if (!h->have_particles)
+        return fail(h, BS_ERR_STATE, "bs_add_trajectory: particles must be set first");
+    if (kind != BS_TRAJ_PDB && kind != BS_TRAJ_XTC && kind != BS_TRAJ_CSV)
+        return fail(h, BS_ERR_ARG, "bs_add_trajectory: unknown kind %d", kind);
+    if (!path || !*path)
+        return fail(h, BS_ERR_ARG, "bs_add_trajectory: empty path");
+    if (frequency < 1)
+        return fail(h, BS_ERR_ARG, "bs_add_trajectory: the frequency must be at least 1");
+    if (kind == BS_TRAJ_XTC && h->n > INT32_MAX / 3)
+        return fail(h, BS_ERR_ARG, "bs_add_trajectory: too many particles for XTC");
+    if (!h->traj)
+    {
+        h->traj.reset(new bs::TrajectorySet(TRAJ_SLOTS));
+        for (int k = 0; k < TRAJ_SLOTS; ++k)
+        {
+            float *x = nullptr;
+            double *e = nullptr;
+            hipEvent_t ev = nullptr;
+            hipError_t r = hipHostMalloc((void **)&x, sizeof(float) * 3 * h->n, hipHostMallocDefault);
+            if (r == hipSuccess)
+            {
+                h->traj_x.push_back(x);
+                r = hipHostMalloc((void **)&e, sizeof(double) * BS_NUM_ENERGIES, hipHostMallocDefault);
+            }
+            if (r == hipSuccess)
+            {
+                h->traj_e.push_back(e);
+                r = hipEventCreate(&ev);
+            }
+            if (r == hipSuccess)
+                h->traj_ev.push_back(ev);
+            if (r != hipSuccess)
+            {
+                traj_release(h);
+                HIPCHK(h, r);
+            }
+        }
+    }
+    auto ctx = std::make_shared<bs::TrajContext>();
+    ctx->n = h->n;
+    ctx->labels = h->labels;
+    ctx->echarge_default.resize((size_t)h->n);
+    for (size_t i = 0; i < (size_t)h->n; ++i)
+        ctx->echarge_default[i] = (int32_t)h->charge[i]; // ParticleProperty::setCharge
+    ctx->conect.reserve(h->s1.size());
+    for (size_t t = 0; t < h->s1.size(); ++t)
+        ctx->conect.emplace_back(h->s1[t], h->s2[t]);
+    ctx->spring = h->p.spring_enable != 0;
+    ctx->steric = h->p.steric_enable != 0;
+    ctx->elec = h->p.electrostatic_enable != 0;
+    ctx->imp = h->p.imp_enable != 0;
+    ctx->hydro = h->p.hydrophobicity_enable != 0;
+    ctx->ivec1 = h->ivec1;
+    ctx->ivec2 = h->ivec2;
+    std::string err;
+    auto w = bs::make_traj_writer(kind, path, frequency, ctx, err);
+    if (!w)
+        return fail(h, BS_ERR_ARG, "bs_add_trajectory: %s", err.c_str());
+    h->traj->add(std::move(w));
+    return BS_OK;
+}
+
+int bs_write_frame_now(bs_handle *h)
+{
+    if (!h)
+        return BS_ERR_ARG;
+    if (!h->traj || h->traj->empty())
+        return fail(h, BS_ERR_STATE, "bs_write_frame_now: no trajectory writer");
+    int rc = ensure_ready(h);
+    if (rc)
+        return rc;
+    return traj_capture(h, h->steps_done, false, true);
+}
+
+int bs_flush_trajectories(bs_handle *h)
+{
+    if (!h)
+        return BS_ERR_ARG;
+    if (h->traj && !h->traj->drain())
+        return fail(h, BS_ERR_ARG, "trajectory output: %s", h->traj->error().c_str());
+    return BS_OK;
+}
+
+int bs_close_trajectories(bs_handle *h)
+{
+    if (!h)
+        return BS_ERR_ARG;
+    std::string err;
+    if (h->traj && !h->traj->drain())
+        err = h->traj->error();
+    traj_release(h);
+    if (!err.empty())
+        return fail(h, BS_ERR_ARG, "trajectory output: %s", err.c_str());
     return BS_OK;
 }
 

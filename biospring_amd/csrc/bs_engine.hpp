@@ -316,6 +316,8 @@ void launch_spring_grads(const int32_t *slot_e, float *dke, float *dle, int64_t 
 void launch_spring_out(const int32_t *slot_of, int64_t m, const double *dk, const double *dl0, double *outk,
                        double *outl, hipStream_t s);
 void launch_topo_springs(const TopoArgs &a, hipStream_t s);
+void launch_pack_positions(const float4 *pos, const float4 *plo, const int32_t *perm, int32_t n, float *out3,
+                           hipStream_t s);
 // status[0] = 1 when any particle is more than half the list skin from xref.
 void launch_drift_check(const float4 *pos, const float4 *xref, int32_t n, float half_skin2, int32_t *status,
                         hipStream_t s);

@@ -1133,6 +1133,27 @@ __global__ __launch_bounds__(64 * TOPO_WAVES) void k_topo_rows(TopoArgs a)
     }
 }
 
+// Trajectory frames: positions back in particle order as float triples
+// (x[perm[t]] = pos[t]); with compensated positions, hi + lo rounded to float.
+__global__ void k_pack_positions(const float4 *pos, const float4 *plo, const int32_t *perm, int32_t n, float *out3)
+{
+    const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n)
+        return;
+    float4 p = pos[t];
+    if (plo)
+    {
+        const float4 l = plo[t];
+        p.x = (float)((double)p.x + (double)l.x);
+        p.y = (float)((double)p.y + (double)l.y);
+        p.z = (float)((double)p.z + (double)l.z);
+    }
+    const int64_t i = perm[t];
+    out3[3 * i] = p.x;
+    out3[3 * i + 1] = p.y;
+    out3[3 * i + 2] = p.z;
+}
+
 __global__ void k_topo_sort_long(const int32_t *offs, int32_t n, int32_t *jout, float *l0out)
 {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1287,6 +1308,13 @@ void launch_topo_springs(const TopoArgs &a, hipStream_t s)
     k_topo_rows<true><<<blocks, 64 * TOPO_WAVES, 0, s>>>(a);
     if (a.max_row > TOPO_CAP)
         k_topo_sort_long<<<nblk(a.n), BLOCK, 0, s>>>(a.offs, a.n, a.jout, a.l0out);
+}
+
+void launch_pack_positions(const float4 *pos, const float4 *plo, const int32_t *perm, int32_t n, float *out3,
+                           hipStream_t s)
+{
+    if (n > 0)
+        k_pack_positions<<<nblk(n), BLOCK, 0, s>>>(pos, plo, perm, n, out3);
 }
 
 void launch_drift_check(const float4 *pos, const float4 *xref, int32_t n, float half_skin2, int32_t *status,

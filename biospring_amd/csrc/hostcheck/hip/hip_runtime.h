@@ -36,6 +36,9 @@ void hc_fail(const char *what);
 
 hipError_t hipMalloc(void **p, size_t bytes);
 hipError_t hipFree(void *p);
+#define hipHostMallocDefault 0u
+hipError_t hipHostMalloc(void **p, size_t bytes, unsigned flags);
+hipError_t hipHostFree(void *p);
 hipError_t hipMemcpy(void *dst, const void *src, size_t bytes, hipMemcpyKind kind);
 hipError_t hipMemcpyAsync(void *dst, const void *src, size_t bytes, hipMemcpyKind kind, hipStream_t s);
 hipError_t hipMemsetAsync(void *dst, int v, size_t bytes, hipStream_t s);

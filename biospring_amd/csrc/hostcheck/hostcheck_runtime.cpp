@@ -86,6 +86,35 @@ hipError_t hipFree(void *p)
     return hipSuccess;
 }
 
+std::map<uintptr_t, size_t> &host_allocs()
+{
+    static std::map<uintptr_t, size_t> m;
+    return m;
+}
+
+hipError_t hipHostMalloc(void **p, size_t bytes, unsigned)
+{
+    *p = std::calloc(bytes ? bytes : 1, 1);
+    std::lock_guard<std::mutex> lk(mu());
+    host_allocs()[(uintptr_t)*p] = bytes;
+    return *p ? hipSuccess : hipErrorOutOfMemory;
+}
+
+hipError_t hipHostFree(void *p)
+{
+    if (!p)
+        return hipSuccess;
+    {
+        std::lock_guard<std::mutex> lk(mu());
+        auto it = host_allocs().find((uintptr_t)p);
+        if (it == host_allocs().end())
+            hc_fail("hipHostFree of a pointer that is not a live pinned allocation");
+        host_allocs().erase(it);
+    }
+    std::free(p);
+    return hipSuccess;
+}
+
 hipError_t hipMemcpy(void *dst, const void *src, size_t bytes, hipMemcpyKind kind)
 {
     if (kind == hipMemcpyHostToDevice)
@@ -155,7 +184,7 @@ const char *hipGetErrorString(hipError_t) { return "hostcheck error"; }
 extern "C" int64_t hc_live_allocations()
 {
     std::lock_guard<std::mutex> lk(mu());
-    return (int64_t)allocs().size();
+    return (int64_t)(allocs().size() + host_allocs().size());
 }
 
 // ---------------------------------------------------------------------------
@@ -820,6 +849,32 @@ void launch_topo_springs(const TopoArgs &a, hipStream_t)
             a.jout[a.offs[i] + t] = row[t].first;
             a.l0out[a.offs[i] + t] = row[t].second;
         }
+    }
+}
+
+void launch_pack_positions(const float4 *pos, const float4 *plo, const int32_t *perm, int32_t n, float *out3,
+                           hipStream_t)
+{
+    need(pos, sizeof(float4) * n, "pack: pos");
+    if (plo)
+        need(plo, sizeof(float4) * n, "pack: plo");
+    need(perm, sizeof(int32_t) * n, "pack: perm");
+    need(out3, sizeof(float) * 3 * n, "pack: out3");
+    for (int32_t t = 0; t < n; ++t)
+    {
+        float4 p = pos[t];
+        if (plo)
+        {
+            p.x = (float)((double)p.x + (double)plo[t].x);
+            p.y = (float)((double)p.y + (double)plo[t].y);
+            p.z = (float)((double)p.z + (double)plo[t].z);
+        }
+        const int32_t i = perm[t];
+        if (i < 0 || i >= n)
+            hc_fail("pack: permutation entry out of range");
+        out3[3 * i] = p.x;
+        out3[3 * i + 1] = p.y;
+        out3[3 * i + 2] = p.z;
     }
 }
 

@@ -6,6 +6,7 @@ energy getters.  Every call goes through libbiospring_amd.so; there is no CPU
 path.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -65,6 +66,45 @@ class SpringNetwork:
         check(self._L.bs_get_springs(self._h, ptr(a, c.c_uint32), ptr(b, c.c_uint32), ptr(l0, c.c_float),
                                      ptr(k, c.c_float)), self._h, "bs_get_springs")
         return a[:m], b[:m], l0[:m], k[:m]
+
+    # -- trajectory output (SpringNetwork::_setupTrajectories / writeNextStepNow)
+    def addTrajectory(self, kind, path, frequency=100):
+        """A PDB / XTC / CSV writer (_abi.TRAJ_*) taking a frame every
+        `frequency` iterations of run()."""
+        check(self._L.bs_add_trajectory(self._h, int(kind), os.fsencode(path), int(frequency)), self._h,
+              "bs_add_trajectory")
+
+    def writeNextStepNow(self):
+        check(self._L.bs_write_frame_now(self._h), self._h, "bs_write_frame_now")
+
+    def flushTrajectories(self):
+        check(self._L.bs_flush_trajectories(self._h), self._h, "bs_flush_trajectories")
+
+    def closeTrajectories(self):
+        check(self._L.bs_close_trajectories(self._h), self._h, "bs_close_trajectories")
+
+    def setParticleLabels(self, name=None, resname=None, chain=None, resid=None, occupancy=None,
+                          tempfactor=None, element=None, electron_charge=None):
+        """PDB fields per particle (lists of str, or arrays); None keeps the
+        reference's defaults."""
+        def strs(v):
+            if v is None:
+                return None
+            arr = (ctypes.c_char_p * self.n)(*[str(t).encode() for t in v])
+            keep.append(arr)
+            return ctypes.cast(arr, ctypes.POINTER(ctypes.c_char_p))
+        keep = []
+        ri = None if resid is None else np.ascontiguousarray(resid, np.int32)
+        oc = None if occupancy is None else f32(occupancy)
+        tf = None if tempfactor is None else f32(tempfactor)
+        ec = None if electron_charge is None else np.ascontiguousarray(electron_charge, np.int32)
+        c = ctypes
+        check(self._L.bs_set_particle_labels(self._h, strs(name), strs(resname), strs(chain), ptr(ri, c.c_int32),
+                                             ptr(oc, c.c_float), ptr(tf, c.c_float), strs(element),
+                                             ptr(ec, c.c_int32)), self._h, "bs_set_particle_labels")
+
+    def setInsertionVector(self, i1, i2):
+        check(self._L.bs_set_insertion_vector(self._h, int(i1), int(i2)), self._h, "bs_set_insertion_vector")
 
     def getNumberOfParticles(self):
         return int(self._L.bs_num_particles(self._h))

@@ -351,6 +351,59 @@ int bs_set_grid(bs_handle *h, int32_t kind, const double origin[3], const int64_
  * ParticleProperty.h:60-61; default 1).  NULL restores the default. */
 int bs_set_burying(bs_handle *h, const float *burying);
 
+/* ---- trajectory output ------------------------------------------------------- */
+
+#define BS_TRAJ_PDB 0  /* io::modern::PDBTrajectoryWriter (PDBTrajectoryWriter.cpp:10-33) */
+#define BS_TRAJ_XTC 1  /* io::modern::XTCTrajectoryWriter (XTCTrajectoryWriter.cpp:13-31) */
+#define BS_TRAJ_CSV 2  /* io::modern::CSVTrajectoryWriter (CSVTrajectoryWriter.cpp:11-60) */
+
+/* Replaces SpringNetwork::_setupTrajectories (SpringNetwork.cpp:768-778) for
+ * one writer: the pdbtrajectory / xtctrajectory / csvsampling settings
+ * (Setting.hpp:160-170) as kind, path and frequency (>= 1; the reference's
+ * default is 100).  bs_step then writes a frame at the start of every step
+ * whose iteration count (steps since bs_setup) the frequency divides, as
+ * TrajectoryManager::write_step from SpringNetwork::idleRun
+ * (TrajectoryManager.cpp:9-16, SpringNetwork.cpp:344-362): the positions at
+ * that point and the energies of the step before.  The positions are packed
+ * on the GPU and copied into pinned host slots; a writer thread formats and
+ * writes them while the stream goes on stepping, and graph replays go on
+ * between frames.  PDB: MODEL n / ATOM records (pdb_format.hpp:17-50) /
+ * CONECT records of the springs in the first model / ENDMDL.  XTC: positions
+ * in nm, step = time = the writer's frame count, unit box, precision 1000.
+ * CSV: header at once, then one row per frame; the energy columns follow the
+ * enabled terms of the current parameters, the insertion columns
+ * bs_set_insertion_vector.  Labels, springs, parameters and the insertion
+ * vector are taken as they are when the writer is added.  A file that cannot
+ * be opened -> BS_ERR_ARG.  bs_set_particles closes every writer. */
+int bs_add_trajectory(bs_handle *h, int32_t kind, const char *path, int64_t frequency);
+
+/* Replaces SpringNetwork::writeNextStepNow (SpringNetwork.cpp:408): a frame
+ * of the current state to every writer, whatever its frequency. */
+int bs_write_frame_now(bs_handle *h);
+
+/* Waits until every frame taken so far is written; BS_ERR_ARG if a write
+ * failed. */
+int bs_flush_trajectories(bs_handle *h);
+
+/* Flushes, closes the files and removes the writers (also done by bs_destroy). */
+int bs_close_trajectories(bs_handle *h);
+
+/* The PDB fields of spn::Particle (Particle.h:36-41: name, residue name,
+ * chain, residue id; ParticleProperty.h:20-21: occupancy, temperature factor,
+ * electron charge) and the element name, n entries each; any pointer may be
+ * NULL for the reference's defaults ("" or 0; the electron charge defaults to
+ * (int)charge, ParticleProperty.cpp:8-16).  Used by PDB writers added
+ * afterwards. */
+int bs_set_particle_labels(bs_handle *h, const char *const *name, const char *const *resname,
+                           const char *const *chain, const int32_t *resid, const float *occupancy,
+                           const float *tempfactor, const char *const *element, const int32_t *electron_charge);
+
+/* Replaces SpringNetwork::setInsertionVector (the ivector setting): CSV
+ * writers added afterwards carry the insertion angle and depth of particles
+ * i1, i2 (InsertionVector.cpp:18-40; depth = the centroid's z).  Both < 0
+ * turns the columns off.  Ids out of range -> BS_ERR_ARG. */
+int bs_set_insertion_vector(bs_handle *h, int64_t i1, int64_t i2);
+
 /* ---- insertion vector --------------------------------------------------------- */
 
 /* InsertionVector::computeVector / computeAngle / computeRollAngle

@@ -124,6 +124,10 @@ def ref_lib():
                                               ctypes.c_double, ctypes.c_double, ctypes.c_int64, _u32p, _u32p, _f32p,
                                               _f32p]
         L.ref_springs_from_cutoff.restype = ctypes.c_int64
+        L.ref_set_trajectories.argtypes = [ctypes.c_char_p, ctypes.c_int64, ctypes.c_char_p, ctypes.c_int64]
+        L.ref_xtc_write.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, _f32p, ctypes.c_float]
+        L.ref_xtc_read.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, _f32p,
+                                   ctypes.POINTER(ctypes.c_int), _f32p, _f32p, _f32p]
         _rlib = L
     return _rlib
 
@@ -415,3 +419,33 @@ def ref_springs_from_cutoff(x, flags, cutoff, stiffness=1.0, existing=None):
     if existing is None:
         return i1[:cnt], i2[:cnt], l0[:cnt]
     return i1[:cnt], i2[:cnt], l0[:cnt], k[:cnt]
+
+
+def ref_set_trajectories(pdb=None, pdb_frequency=100, csv=None, csv_frequency=100):
+    """pdbtrajectory / csvsampling settings for the next Reference(...)."""
+    enc = lambda p: None if p is None else os.fsencode(p)  # noqa: E731
+    ref_lib().ref_set_trajectories(enc(pdb), pdb_frequency, enc(csv), csv_frequency)
+
+
+def ref_xtc_write(path, frames_nm, precision=1000.0):
+    """frames (k, n, 3) in nm through the reference's xdrfile write_xtc."""
+    f = np.ascontiguousarray(frames_nm, np.float32)
+    rc = ref_lib().ref_xtc_write(os.fsencode(path), f.shape[0], f.shape[1], _p(f, ctypes.c_float), precision)
+    if rc != 0:
+        raise RuntimeError(f"write_xtc failed ({rc})")
+
+
+def ref_xtc_read(path, natoms, maxframes=1000):
+    """Frames of an XTC file through the reference's xdrfile read_xtc:
+    (x (k, n, 3) nm, steps, times, boxes (k, 3, 3), precisions)."""
+    x = np.zeros((maxframes, natoms, 3), np.float32)
+    st = np.zeros(maxframes, np.int32)
+    t = np.zeros(maxframes, np.float32)
+    box = np.zeros((maxframes, 3, 3), np.float32)
+    pr = np.zeros(maxframes, np.float32)
+    k = ref_lib().ref_xtc_read(os.fsencode(path), natoms, maxframes, _p(x, ctypes.c_float),
+                               st.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _p(t, ctypes.c_float),
+                               _p(box, ctypes.c_float), _p(pr, ctypes.c_float))
+    if k < 0:
+        raise RuntimeError(f"cannot open {path}")
+    return x[:k], st[:k], t[:k], box[:k], pr[:k]

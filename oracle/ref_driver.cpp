@@ -28,6 +28,13 @@
 //    after the given springs were added with Topology::add_spring; all
 //    springs read back as (first, second, float(equilibrium),
 //    float(stiffness)) in the reference's own order.
+//  - ref_set_trajectories: the pdbtrajectory / csvsampling settings applied
+//    to the next ref_create, so that the reference's own writers
+//    (io::modern::TrajectoryManager, SpringNetwork.cpp:768-778) run inside
+//    its computeStep.  Its XTC writer is not driven: XTCTrajectoryWriter
+//    never opens its XDR handle (XTCTrajectoryWriter.hpp:19-20, .cpp:13-31).
+//  - ref_xtc_write / ref_xtc_read: frames through the reference's vendored
+//    xdrfile write_xtc / read_xtc (xdrfile_xtc.c:89-119).
 #include "SpringNetwork.h"
 #include "configuration/Configuration.hpp"
 #include "forcefield/ForceField.h"
@@ -35,12 +42,15 @@
 #include "grid/PotentialGrid.hpp"
 #include "rigidbody/RigidBodiesManager.h"
 #include "topology/Topology.hpp"
+#include "xdrfile.h"
+#include "xdrfile_xtc.h"
 
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <mutex>
+#include <string>
 #include <unordered_map>
 #include <vector>
 #ifdef _OPENMP
@@ -66,9 +76,27 @@ const char * steric_mode_name(int m)
     }
 }
 
+struct TrajConf
+{
+    std::string pdb, csv;
+    size_t pdbf = 100, csvf = 100;
+} g_traj;
+
 biospring::configuration::Configuration make_config(const bs_params * p)
 {
     auto c = biospring::configuration::defaultConfiguration();
+    if (!g_traj.pdb.empty())
+    {
+        c.pdbtraj.enable = true;
+        c.pdbtraj.path = g_traj.pdb;
+        c.pdbtraj.frequency = g_traj.pdbf;
+    }
+    if (!g_traj.csv.empty())
+    {
+        c.csvsample.enable = true;
+        c.csvsample.path = g_traj.csv;
+        c.csvsample.frequency = g_traj.csvf;
+    }
     c.sim.nbsteps = -1;
     c.sim.timestep = p->timestep;
     c.sim.neighborskin = p->neighborskin;
@@ -552,6 +580,61 @@ int64_t ref_springs_from_cutoff(int64_t n, const float * x3, const uint8_t * fla
             k[s] = (float)sp.stiffness();
         }
     return m;
+}
+
+void ref_set_trajectories(const char * pdb, int64_t pdbf, const char * csv, int64_t csvf)
+{
+    g_traj.pdb = pdb ? pdb : "";
+    g_traj.csv = csv ? csv : "";
+    g_traj.pdbf = (size_t)pdbf;
+    g_traj.csvf = (size_t)csvf;
+}
+
+// Frames k = 0..nframes-1 with step = time = k and a unit box, as the
+// reference's XTC writer means to write them.
+int ref_xtc_write(const char * path, int nframes, int natoms, const float * x3nm, float prec)
+{
+    XDRFILE * xd = xdrfile_open(path, "w");
+    if (!xd)
+        return -1;
+    matrix box = {{1.0f, 0.0f, 0.0f}, {0.0f, 1.0f, 0.0f}, {0.0f, 0.0f, 1.0f}};
+    std::vector<float> buf((size_t)3 * natoms);
+    for (int k = 0; k < nframes; ++k)
+    {
+        std::memcpy(buf.data(), x3nm + (size_t)k * 3 * natoms, sizeof(float) * 3 * natoms);
+        if (write_xtc(xd, natoms, k, (float)k, box, reinterpret_cast<rvec *>(buf.data()), prec) != exdrOK)
+        {
+            xdrfile_close(xd);
+            return -2;
+        }
+    }
+    xdrfile_close(xd);
+    return 0;
+}
+
+// Reads up to maxframes frames; returns the number read.
+int ref_xtc_read(const char * path, int natoms, int maxframes, float * x3nm, int * steps, float * times,
+                 float * box9, float * precs)
+{
+    XDRFILE * xd = xdrfile_open(path, "r");
+    if (!xd)
+        return -1;
+    int k = 0;
+    for (; k < maxframes; ++k)
+    {
+        matrix box;
+        int step = 0;
+        float t = 0.0f, pr = 0.0f;
+        if (read_xtc(xd, natoms, &step, &t, box, reinterpret_cast<rvec *>(x3nm + (size_t)k * 3 * natoms), &pr) !=
+            exdrOK)
+            break;
+        steps[k] = step;
+        times[k] = t;
+        precs[k] = pr;
+        std::memcpy(box9 + 9 * k, box, sizeof box);
+    }
+    xdrfile_close(xd);
+    return k;
 }
 
 } // extern "C"

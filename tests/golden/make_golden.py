@@ -227,6 +227,7 @@ def main():
                grid_density=dens, burying=bur, ivec_ids=np.array([3, 250], np.int64))
 
     topology_cases()
+    trajectory_cases()
 
     if ONLY and "scans" not in ONLY:
         return
@@ -263,6 +264,64 @@ def main():
         scans[f"{tag}_F0"] = np.array(fs)
     np.savez_compressed(os.path.join(OUT, "scans.npz"), d=xs, **scans)
     print("scans written")
+
+
+def trajectory_cases():
+    """Trajectory output.
+
+    traj300: the reference's own PDB and CSV writers (PDBTrajectoryWriter,
+    CSVTrajectoryWriter through TrajectoryManager::write_step in
+    SpringNetwork::idleRun) over 7 computeStep calls of a 300-bead workload-1
+    network with constant-dielectric Coulomb, PDB every 2 iterations, CSV
+    every 3.
+    xtc_frames: XTC files written by the reference's vendored xdrfile
+    write_xtc (3 frames each, step = time = frame, unit box, precision 1000)
+    for point sets that reach each branch of its coordinate compression:
+    random, close pairs (the swapped-pair runs), a line (long runs), more
+    than 0xffffff integer units across (coordinates sent one by one), and
+    9 or fewer atoms (uncompressed)."""
+    import tempfile
+    if not ONLY or "traj300" in ONLY:
+        s = workloads.generate(1, seed=5, n=300)
+        s.params.dielectric_model = _abi.DIELECTRIC_CONSTANT  # the reference has no eps(r)
+        with tempfile.TemporaryDirectory() as d:
+            pdb, csv = os.path.join(d, "t.pdb"), os.path.join(d, "t.csv")
+            po.ref_set_trajectories(pdb=pdb, pdb_frequency=2, csv=csv, csv_frequency=3)
+            try:
+                r = po.Reference(s, s.params)
+                r.step(7)
+                x, v = r.state()
+                r.close()
+            finally:
+                po.ref_set_trajectories()
+            pdb_text = open(pdb, "rb").read()
+            csv_text = open(csv, "rb").read()
+        store("traj300", s, s.params, pdb=np.frombuffer(pdb_text, np.uint8), csv=np.frombuffer(csv_text, np.uint8),
+              steps=np.int64(7), pdb_frequency=np.int64(2), csv_frequency=np.int64(3), final_x=x)
+    if not ONLY or "xtc_frames" in ONLY:
+        rng = np.random.default_rng(77)
+        w = rng.uniform(-30, 30, (300, 1, 3))
+        sets = {
+            "random": rng.uniform(-50, 50, (1000, 3)),
+            "pairs": (w + rng.normal(0, 0.5, (300, 3, 3))).reshape(-1, 3),
+            "line": np.stack([np.arange(2000) * 0.37, np.zeros(2000), np.zeros(2000)], 1),
+            "wide": rng.uniform(-4e5, 4e5, (50, 3)),
+            "nine": rng.uniform(-5, 5, (9, 3)),
+            "ten": rng.uniform(-5, 5, (10, 3)),
+        }
+        out = {}
+        with tempfile.TemporaryDirectory() as d:
+            for name, xa in sets.items():
+                xa = xa.astype(np.float32)
+                frames = np.stack([xa, xa + np.float32(0.25), xa - np.float32(1.5)])
+                nm = (frames.astype(np.float64) / 10.0).astype(np.float32)
+                path = os.path.join(d, name + ".xtc")
+                po.ref_xtc_write(path, nm)
+                out[f"{name}__x"] = frames
+                out[f"{name}__xtc"] = np.frombuffer(open(path, "rb").read(), np.uint8)
+        path = os.path.join(OUT, "xtc_frames.npz")
+        np.savez_compressed(path, **out)
+        print(f"xtc_frames -> {os.path.getsize(path) / 1024:.0f} KiB")
 
 
 def borderline_pairs(rng, rc, centers, inside):
