@@ -145,6 +145,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-construction", action="store_true",
                     help="workload 1: skip timing the spring-network construction")
+    ap.add_argument("--no-trajectory", action="store_true",
+                    help="workload 1: skip timing the run with trajectory writers")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--pmc-summary", nargs=4, metavar=("FETCH", "FETCH_COPY", "WRITE", "WRITE_COPY"))
     args = ap.parse_args()
@@ -259,6 +261,8 @@ def main():
 
     if args.workload == 1 and not args.no_construction:
         out["network_construction"] = bench_construction(s, cpu=not args.no_cpu)
+    if args.workload == 1 and not args.no_trajectory:
+        out["trajectory_output"] = bench_trajectory(s, args.steps, out["ms_per_step"])
 
     x, v = net.getState()
     # Conservative force: the viscous term -gamma v is an exact product in both
@@ -390,6 +394,35 @@ def bench_construction(s, cutoff=8.0, reps=3, cpu=True, cpu_sample=5000):
                                    "sample": f"the oracle's cell-grid build on the same {n} beads"}
     log(f"construction: {m} springs in {wall * 1e3:.1f} ms wall, kernels {kern_ms:.2f} ms, same={same}")
     return out
+
+
+def bench_trajectory(s, steps, ms_plain, every=50):
+    """The same run with an XTC and a CSV writer taking a frame every `every`
+    iterations (files in a temporary directory), timed until the last frame
+    is on disk; beside the run without writers."""
+    import tempfile
+    from biospring_amd import _abi
+    from biospring_amd.network import SpringNetwork
+    net = SpringNetwork.from_system(s)
+    net.setup()
+    net.run(10)
+    with tempfile.TemporaryDirectory() as d:
+        net.addTrajectory(_abi.TRAJ_XTC, os.path.join(d, "t.xtc"), every)
+        net.addTrajectory(_abi.TRAJ_CSV, os.path.join(d, "t.csv"), every)
+        hip_sync()
+        t = time.perf_counter()
+        net.run(steps)
+        net.flushTrajectories()
+        dt = time.perf_counter() - t
+        xtc_bytes = os.path.getsize(os.path.join(d, "t.xtc"))
+        net.closeTrajectories()
+    net.close()
+    frames = len(range(10 + (-10) % every, 10 + steps, every))
+    ms = dt / steps * 1e3
+    log(f"trajectory: {frames} frames in {steps} steps, {ms:.3f} ms/step vs {ms_plain:.3f}")
+    return {"writers": ["xtc", "csv"], "every": every, "steps": steps, "frames": frames,
+            "ms_per_step": ms, "ms_per_step_without": ms_plain, "overhead": ms / ms_plain - 1.0,
+            "xtc_bytes_per_frame": xtc_bytes / max(frames, 1)}
 
 
 def trajectory_check(seed, n=1000, marks=(1, 10, 100, 1000)):

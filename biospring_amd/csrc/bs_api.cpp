@@ -4,6 +4,7 @@
 // CSR and the neighbour lists, and sequences the kernels of one step in the
 // order of SpringNetwork::computeStep (src/spn/SpringNetwork.cpp:251-268).
 #include "bs_engine.hpp"
+#include "bs_netcdf.hpp"
 #include "bs_traj.hpp"
 #include "bs_params_defaults.h"
 
@@ -3131,6 +3132,65 @@ int bs_close_trajectories(bs_handle *h)
     traj_release(h);
     if (!err.empty())
         return fail(h, BS_ERR_ARG, "trajectory output: %s", err.c_str());
+    return BS_OK;
+}
+
+// ---- topology files -----------------------------------------------------------
+
+int bs_load_netcdf(bs_handle *h, const char *path, char *warnings, int64_t warnings_len)
+{
+    if (!h)
+        return BS_ERR_ARG;
+    if (!path || !*path)
+        return fail(h, BS_ERR_ARG, "bs_load_netcdf: empty path");
+    bs::NcTopology t;
+    std::string err;
+    if (!bs::read_netcdf_topology(path, t, err))
+        return fail(h, BS_ERR_ARG, "%s", err.c_str());
+    std::vector<uint8_t> flags((size_t)t.n);
+    for (size_t i = 0; i < flags.size(); ++i)
+        flags[i] = t.is_static[i] ? BS_FLAG_STATIC : 0;
+    int rc = bs_set_particles(h, t.n, t.x.data(), nullptr, t.mass.data(), t.charge.data(), t.radius.data(),
+                              t.epsilon.data(), t.sasa.data(), t.transfer.data(), t.hydro.data(), flags.data());
+    if (rc)
+        return rc;
+    // to_spring_network copies the names, residue and chain (Topology.hpp:250-262)
+    h->labels.name = std::move(t.name);
+    h->labels.resname = std::move(t.resname);
+    h->labels.chain = std::move(t.chain);
+    h->labels.resid = std::move(t.resid);
+    rc = bs_set_springs(h, (int64_t)t.s1.size(), t.s1.data(), t.s2.data(), t.l0.data(), t.k.data());
+    if (rc)
+        return rc;
+    if (warnings && warnings_len > 0)
+    {
+        std::string all;
+        for (const auto &w : t.warnings)
+            all += w + "\n";
+        std::snprintf(warnings, (size_t)warnings_len, "%s", all.c_str());
+    }
+    return BS_OK;
+}
+
+int bs_get_particle_properties(const bs_handle *h, float *mass, float *charge, float *radius, float *epsilon,
+                               float *sasa, float *transfer, float *hydrophobicity, uint8_t *flags)
+{
+    if (!h)
+        return BS_ERR_ARG;
+    const size_t n = (size_t)h->n;
+    auto out = [n](float *dst, const std::vector<float> &src) {
+        if (dst && n)
+            std::memcpy(dst, src.data(), sizeof(float) * n);
+    };
+    out(mass, h->mass);
+    out(charge, h->charge);
+    out(radius, h->radius);
+    out(epsilon, h->eps);
+    out(sasa, h->sasa);
+    out(transfer, h->transfer);
+    out(hydrophobicity, h->hydro);
+    if (flags && n)
+        std::memcpy(flags, h->flags.data(), n);
     return BS_OK;
 }
 

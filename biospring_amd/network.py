@@ -67,6 +67,28 @@ class SpringNetwork:
                                      ptr(k, c.c_float)), self._h, "bs_get_springs")
         return a[:m], b[:m], l0[:m], k[:m]
 
+    # -- topology files (io::readTopology + Topology::to_spring_network)
+    def loadNetCDF(self, path):
+        """Particles, springs and PDB labels from a .nc topology; returns the
+        reader's warnings."""
+        buf = ctypes.create_string_buffer(1 << 16)
+        check(self._L.bs_load_netcdf(self._h, os.fsencode(path), buf, len(buf)), self._h, "bs_load_netcdf")
+        self.n = int(self._L.bs_num_particles(self._h))
+        return [w for w in buf.value.decode().split("\n") if w]
+
+    def getParticleProperties(self):
+        """dict of mass, charge, radius, epsilon, sasa, transfer, hydrophobicity, flags."""
+        n = int(self._L.bs_num_particles(self._h))
+        out = {k: np.zeros(max(n, 1), np.float32) for k in
+               ("mass", "charge", "radius", "epsilon", "sasa", "transfer", "hydrophobicity")}
+        out["flags"] = np.zeros(max(n, 1), np.uint8)
+        c = ctypes
+        check(self._L.bs_get_particle_properties(
+            self._h, *[ptr(out[k], c.c_float) for k in ("mass", "charge", "radius", "epsilon", "sasa", "transfer",
+                                                         "hydrophobicity")], ptr(out["flags"], c.c_uint8)),
+              self._h, "bs_get_particle_properties")
+        return {k: v[:n] for k, v in out.items()}
+
     # -- trajectory output (SpringNetwork::_setupTrajectories / writeNextStepNow)
     def addTrajectory(self, kind, path, frequency=100):
         """A PDB / XTC / CSV writer (_abi.TRAJ_*) taking a frame every

@@ -351,6 +351,30 @@ int bs_set_grid(bs_handle *h, int32_t kind, const double origin[3], const int64_
  * ParticleProperty.h:60-61; default 1).  NULL restores the default. */
 int bs_set_burying(bs_handle *h, const float *burying);
 
+/* ---- topology files ------------------------------------------------------------ */
+
+/* Replaces io::readTopology of a .nc file (NetCDFReader::read,
+ * src/IO/NetCDFReader.cpp:8-88) followed by Topology::to_spring_network
+ * (Topology.hpp:245-296): bs_set_particles and bs_set_springs from the
+ * file's particle_number / spring_number dimensions and its variables
+ * particleids, resids, resnames[4], particlenames[5], chainnames[4],
+ * dynamicstate (0 = static), coordinates[3], radii, mass (0 -> 1), epsilon,
+ * charges, optional surfaceaccessibility, hydrophobicityscale (the IMPALA
+ * transfer energy) and hydrophobicity, and springs[2], springsstiffness,
+ * springsequilibrium; names, residues and chains become the PDB labels.
+ * Values of any numeric type are converted as the netCDF library's getVar
+ * does.  Classic, 64-bit-offset and 64-bit-data files are read by the
+ * library itself; NetCDF-4 (HDF5) files -> BS_ERR_ARG.  A missing array,
+ * wrong shape, missing particle dimension, spring index out of range, self
+ * spring or repeated spring -> BS_ERR_ARG with the reference's message.
+ * warnings (nullable, warnings_len bytes): the reader's warning lines. */
+int bs_load_netcdf(bs_handle *h, const char *path, char *warnings, int64_t warnings_len);
+
+/* The per-particle properties as loaded (bs_set_particles order and
+ * defaults); every pointer nullable, bs_num_particles entries each. */
+int bs_get_particle_properties(const bs_handle *h, float *mass, float *charge, float *radius, float *epsilon,
+                               float *sasa, float *transfer, float *hydrophobicity, uint8_t *flags);
+
 /* ---- trajectory output ------------------------------------------------------- */
 
 #define BS_TRAJ_PDB 0  /* io::modern::PDBTrajectoryWriter (PDBTrajectoryWriter.cpp:10-33) */
