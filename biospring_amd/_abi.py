@@ -36,7 +36,7 @@ EXPORTS = (
     "bs_get_system_energies", "bs_autotune", "bs_adjoint", "bs_adjoint_forward", "bs_adjoint_backward",
     "bs_springs_from_cutoff", "bs_get_springs", "bs_add_trajectory", "bs_write_frame_now",
     "bs_flush_trajectories", "bs_close_trajectories", "bs_set_particle_labels", "bs_set_insertion_vector",
-    "bs_load_netcdf", "bs_get_particle_properties",
+    "bs_load_netcdf", "bs_get_particle_properties", "bs_set_sasa", "bs_set_membrane",
 )
 PROF_REBUILD, PROF_PAIR, PROF_SPRING = range(3)
 
@@ -174,6 +174,8 @@ def bind(L):
     L.bs_set_insertion_vector.argtypes = [_vp, ctypes.c_int64, ctypes.c_int64]
     L.bs_load_netcdf.argtypes = [_vp, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int64]
     L.bs_get_particle_properties.argtypes = [_vp] + [_f32p] * 7 + [_u8p]
+    L.bs_set_sasa.argtypes = [_vp, _f32p]
+    L.bs_set_membrane.argtypes = [_vp] + [ctypes.c_float] * 4
     return L
 
 

@@ -3194,4 +3194,55 @@ int bs_get_particle_properties(const bs_handle *h, float *mass, float *charge, f
     return BS_OK;
 }
 
+// ---- interactive updates (the interactor bridge) --------------------------------
+
+int bs_set_sasa(bs_handle *h, const float *sasa)
+{
+    if (!h || !sasa)
+        return BS_ERR_ARG;
+    if (!h->have_particles)
+        return fail(h, BS_ERR_STATE, "bs_set_sasa: no particles");
+    std::memcpy(h->sasa.data(), sasa, sizeof(float) * h->n);
+    h->tape.ready = false;
+    if (!h->ready)
+        return BS_OK;
+    // the per-particle IMPALA parameters in internal order; stream-ordered,
+    // so the steps already queued keep the old surfaces
+    std::vector<float4> ip((size_t)h->n);
+    for (int64_t t = 0; t < h->n; ++t)
+    {
+        const int64_t i = h->perm[(size_t)t];
+        ip[(size_t)t] = make_float4(h->mass[(size_t)i], h->sasa[(size_t)i], h->transfer[(size_t)i],
+                                    h->burying.empty() ? 1.0f : h->burying[(size_t)i]);
+    }
+    HIPCHK(h, hipMemcpyAsync(h->iprm, ip.data(), sizeof(float4) * h->n, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return BS_OK;
+}
+
+int bs_set_membrane(bs_handle *h, float upper_offset, float lower_offset, float upper_curvature,
+                    float lower_curvature)
+{
+    if (!h)
+        return BS_ERR_ARG;
+    if (!h->p.imp_enable)
+        return BS_OK; // CustomData applies these only with IMPALA on
+    float *dst[4] = {&h->p.imp_upper_offset, &h->p.imp_lower_offset, &h->p.imp_upper_curvature,
+                     &h->p.imp_lower_curvature};
+    const float val[4] = {upper_offset, lower_offset, upper_curvature, lower_curvature};
+    bool changed = false;
+    for (int k = 0; k < 4; ++k)
+        if (!std::isnan(val[k]) && *dst[k] != val[k])
+        {
+            *dst[k] = val[k];
+            changed = true;
+        }
+    if (changed)
+    {
+        drop_graph(h); // a captured graph holds the launch arguments
+        h->tape.ready = false;
+    }
+    return BS_OK;
+}
+
 } // extern "C"

@@ -67,6 +67,18 @@ class SpringNetwork:
                                      ptr(k, c.c_float)), self._h, "bs_get_springs")
         return a[:m], b[:m], l0[:m], k[:m]
 
+    # -- interactive updates (the interactor bridge)
+    def setSolventAccessibilitySurfaces(self, sasa):
+        """InteractorFreeSASA: per-particle SASA for IMPALA from the next step."""
+        a = f32(sasa)
+        check(self._L.bs_set_sasa(self._h, ptr(a, ctypes.c_float)), self._h, "bs_set_sasa")
+
+    def setMembrane(self, upper_offset=float("nan"), lower_offset=float("nan"), upper_curvature=float("nan"),
+                    lower_curvature=float("nan")):
+        """MDDriver custom data dmou / dmol / dmtc; NaN keeps a value."""
+        check(self._L.bs_set_membrane(self._h, upper_offset, lower_offset, upper_curvature, lower_curvature),
+              self._h, "bs_set_membrane")
+
     # -- topology files (io::readTopology + Topology::to_spring_network)
     def loadNetCDF(self, path):
         """Particles, springs and PDB labels from a .nc topology; returns the

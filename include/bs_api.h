@@ -351,6 +351,24 @@ int bs_set_grid(bs_handle *h, int32_t kind, const double origin[3], const int64_
  * ParticleProperty.h:60-61; default 1).  NULL restores the default. */
 int bs_set_burying(bs_handle *h, const float *burying);
 
+/* ---- interactive updates -------------------------------------------------------- */
+
+/* Replaces InteractorFreeSASA::syncParticleStateData
+ * (src/interactor/freesasa/InteractorFreeSASA.cpp:229-247): new solvent
+ * accessible surfaces for all particles (n floats), used by IMPALA from the
+ * next step on, without a re-setup.  External forces from an interactor
+ * (InteractorMDDriver::syncParticleStateData, InteractorMDDriver.cpp:309-321)
+ * are the ext_force3 argument of bs_step. */
+int bs_set_sasa(bs_handle *h, const float *sasa);
+
+/* Replaces the IMPALA custom data of the MDDriver bridge
+ * (src/interactor/mddriver/CustomData.cpp:61-108: "dmou", "dmol", "dmtc"):
+ * new double-membrane offsets and tube curvatures from the next step on,
+ * without a re-setup; a NaN keeps that value; ignored while IMPALA is off,
+ * as there. */
+int bs_set_membrane(bs_handle *h, float upper_offset, float lower_offset, float upper_curvature,
+                    float lower_curvature);
+
 /* ---- topology files ------------------------------------------------------------ */
 
 /* Replaces io::readTopology of a .nc file (NetCDFReader::read,

@@ -290,3 +290,29 @@ def test_adjoint_sequence(hostcheck_lib):
         print("leak", L.hc_live_allocations() - base)
     """, hostcheck_lib)
     assert "leak 0" in out, out
+
+
+def test_interactive_updates(hostcheck_lib):
+    _run("""
+        s = workloads.generate(2, seed=7, n=500)
+        p = workloads.impala_params(s, "double")
+        net = SpringNetwork(p, library=L)
+        try:
+            net.setSolventAccessibilitySurfaces(np.zeros(500, np.float32))
+            raise SystemExit("sasa accepted before particles")
+        except _abi.EngineError:
+            pass
+        net.setParticles(*s.particle_arrays())
+        net.setSprings(*s.spring_arrays())
+        net.setSolventAccessibilitySurfaces(s.sasa + 1)      # before setup: host copy only
+        assert np.array_equal(net.getParticleProperties()["sasa"], (s.sasa + 1).astype(np.float32))
+        net.run(3)
+        net.setSolventAccessibilitySurfaces(s.sasa + 2)      # after setup: device copy too
+        net.setMembrane(upper_offset=10.0)                   # NaN keeps the others
+        net.run(3)
+        q = s.params.copy(); q.imp_enable = 0
+        net.setParams(q)
+        net.setMembrane(1.0, 2.0, 3.0, 4.0)                  # ignored with IMPALA off
+        net.run(1)
+        net.close()
+        """, hostcheck_lib)
