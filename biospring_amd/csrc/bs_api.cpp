@@ -1082,7 +1082,19 @@ int adjoint_record(bs_handle *h, int32_t T, int32_t checkpoint_every)
     auto &tp = h->tape;
     tp.ready = false;
     const int64_t n = h->n;
-    int32_t K = checkpoint_every > 0 ? checkpoint_every : (int32_t)std::ceil(std::sqrt((double)std::max(T, 1)));
+    int32_t K = checkpoint_every;
+    if (K <= 0)
+    {
+        // automatic: the whole trajectory as one stored segment (no
+        // recomputation) when it takes at most half the free device memory;
+        // else segments of ceil(sqrt(T)) steps, the least memory
+        size_t mfree = 0, mtotal = 0;
+        const double whole = (double)n * (32.0 + 16.0 * (double)std::max(T, 1));
+        if (hipMemGetInfo(&mfree, &mtotal) == hipSuccess && whole <= 0.5 * (double)mfree)
+            K = std::max(T, 1);
+        else
+            K = (int32_t)std::ceil(std::sqrt((double)std::max(T, 1)));
+    }
     K = std::max(1, std::min(K, std::max(T, 1)));
     const int32_t ncp = (T + K - 1) / K;
     const size_t s4 = sizeof(float4) * (size_t)n;

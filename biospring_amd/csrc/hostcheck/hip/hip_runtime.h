@@ -46,6 +46,7 @@ hipError_t hipStreamSynchronize(hipStream_t s);
 hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned flags);
 hipError_t hipStreamDestroy(hipStream_t s);
 hipError_t hipGetDeviceCount(int *n);
+hipError_t hipMemGetInfo(size_t *free, size_t *total);
 hipError_t hipGetDevice(int *d);
 hipError_t hipGetDeviceProperties(hipDeviceProp_t *p, int d);
 hipError_t hipEventCreate(hipEvent_t *e);

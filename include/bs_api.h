@@ -489,9 +489,12 @@ int bs_get_system_energies(bs_handle *h, double *out);
  * (springs, the four steric modes, Coulomb with both dielectric models;
  * viscosity enters through gamma), with the pair set of each state fixed as
  * the forward step found it.  States are saved every checkpoint_every steps
- * (<= 0: ceil(sqrt(nsteps))) and segments recomputed, so the work is about
- * two forward passes plus the reverse sweep and the memory n * (32 bytes per
- * checkpoint + 16 bytes per step of a segment).  BS_ERR_ARG with IMPALA, the
+ * and segments recomputed, so the work is about two forward passes plus the
+ * reverse sweep and the memory n * (32 bytes per checkpoint + 16 bytes per
+ * step of a segment).  checkpoint_every <= 0 chooses: the whole trajectory as
+ * one segment, with no recomputation (one forward pass plus the sweep), when
+ * n * (32 + 16 nsteps) bytes fit in half the free device memory, else
+ * ceil(sqrt(nsteps)).  BS_ERR_ARG with IMPALA, the
  * pseudo-hydrophobic term, rigid bodies, field grids or compensated
  * positions.  No reference counterpart (the reference has no gradients). */
 int bs_adjoint(bs_handle *h, int32_t nsteps, int32_t checkpoint_every, const float *target3,
