@@ -48,6 +48,7 @@ WORKLOADS = {
 }
 HBM_PEAK_GBS = 8000.0       # MI355X HBM3E, GB/s
 FP32_PEAK_TFLOPS = 157.3    # MI355X FP32 vector/matrix, TFLOP/s
+FP64_PEAK_TFLOPS = 78.6     # MI355X FP64, TFLOP/s
 PAIR_FLOP = 40              # algorithmic flops per unique LJ+Coulomb pair
 ADJ_PAIR_FLOP = 140         # adjoint: flops per unique LJ+Coulomb pair (both sides, H w and dL/dq)
 
@@ -239,6 +240,19 @@ def main():
             if name in tt:
                 kernels[name]["traffic_bytes"] = tt[name]["bytes"]
         traffic = tt.get(dom, {}).get("bytes")
+    vfile = os.path.join(ROOT, "profiles", "pmc_valu.json")
+    if os.path.exists(vfile) and args.workload == 1 and n == 250000:
+        # executed VALU FLOPs per launch from the committed SQ_INSTS_VALU_* run,
+        # over this run's average launch time
+        with open(vfile) as fh:
+            vv = json.load(fh)
+        for name in ("pair", "spring_integrate"):
+            if name in vv:
+                ms = kernels[name]["avg_ms"]
+                e32, e64 = vv[name]["executed_fp32_flop"], vv[name]["executed_fp64_flop"]
+                kernels[name]["executed_valu"] = {"fp32_tflops": e32 / (ms * 1e-3) / 1e12,
+                                                  "fp64_tflops": e64 / (ms * 1e-3) / 1e12,
+                                                  "fp32_peak": FP32_PEAK_TFLOPS, "fp64_peak": FP64_PEAK_TFLOPS}
     d = kernels[dom]
     roofline = {"bound": d["bound"], "achieved": d["achieved"], "peak": d["peak"], "unit": d["unit"],
                 "frac": d["frac"], "traffic": traffic, "kernel": dom}
