@@ -2855,6 +2855,31 @@ int bs_springs_from_cutoff(bs_handle *h, double cutoff, float stiffness, int64_t
         }
         if (h->prof)
             prof_collect(h);
+        if (h->sys_off.size() > 2)
+        {
+            // independent systems (bs_set_systems): no spring joins two of
+            // them; rows are sorted by j > i, so a row keeps its j below the
+            // end of i's system
+            size_t w = (size_t)m0, sy = 0;
+            for (int64_t i = 0; i < n; ++i)
+            {
+                while (h->sys_off[sy + 1] <= i)
+                    ++sy;
+                const uint32_t end = (uint32_t)h->sys_off[sy + 1];
+                for (int32_t t = rows[(size_t)i]; t < rows[(size_t)i + 1]; ++t)
+                    if (h->s2[(size_t)(m0 + t)] < end)
+                    {
+                        h->s1[w] = h->s1[(size_t)(m0 + t)];
+                        h->s2[w] = h->s2[(size_t)(m0 + t)];
+                        h->sl0[w] = h->sl0[(size_t)(m0 + t)];
+                        ++w;
+                    }
+            }
+            m = (int64_t)w - m0;
+            h->s1.resize(w);
+            h->s2.resize(w);
+            h->sl0.resize(w);
+        }
     }
     h->sk.resize((size_t)m0);
     h->sk.resize((size_t)(m0 + m), stiffness);

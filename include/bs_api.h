@@ -175,8 +175,9 @@ const char *bs_last_error(const bs_handle *h);
  * (SpringCollection.hpp:138-151,226-233); bs_set_springs with m = 0 clears
  * them first.  New springs follow the loaded ones, ordered by (first,
  * second) original id, first < second; the reference orders them by first
- * id, then by its cell scan.  *count_out (nullable) = number of springs
- * added. */
+ * id, then by its cell scan.  With independent systems declared
+ * (bs_set_systems), no spring joins two systems.  *count_out (nullable) =
+ * number of springs added. */
 int bs_springs_from_cutoff(bs_handle *h, double cutoff, float stiffness, int64_t *count_out);
 
 /* The loaded springs (bs_set_springs or bs_springs_from_cutoff order);

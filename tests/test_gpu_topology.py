@@ -150,3 +150,22 @@ def test_constructed_network_steps_like_the_oracle():
     net.run(5)
     assert np.isfinite(net.getState()[0]).all()
     net.close()
+
+
+def test_systems_stay_apart():
+    rng = np.random.default_rng(10)
+    xa = rng.uniform(-30, 30, (6000, 3)).astype(np.float32)
+    xb = (xa + np.float32(0.25)).astype(np.float32)
+    x = np.concatenate([xa, xb])
+    net = SpringNetwork(workloads.params(1))
+    net.setParticles(x)
+    net.setSystems(np.array([0, 6000, 12000], np.int64))
+    _log("topology in two overlapping systems: n=12000")
+    net.springsFromCutoff(8.0)
+    a, b, l0, _ = net.getSprings()
+    assert np.all((a < 6000) == (b < 6000))
+    for lo, xs in ((0, xa), (6000, xb)):
+        sel = a >= lo if lo else a < 6000
+        same_springs((a[sel] - lo, b[sel] - lo, l0[sel]), po.springs_from_cutoff(xs, None, 8.0))
+    net.run(2)
+    net.close()

@@ -244,3 +244,39 @@ def test_host_construction_loads_the_fixture_springs(hostcheck_lib, tmp_path):
         c = load_case(name)
         same_springs((got[name + "_a"], got[name + "_b"], got[name + "_l0"]),
                      (c["ref_id1"], c["ref_id2"], c["ref_l0"]))
+
+
+def test_host_construction_keeps_systems_apart(hostcheck_lib, tmp_path):
+    """With independent systems declared, construction links particles of
+    one system only, even where two systems overlap in space; each system's
+    springs are the oracle's on that system alone."""
+    rng = np.random.default_rng(9)
+    xa = rng.uniform(-15, 15, (700, 3)).astype(np.float32)
+    xb = (xa + np.float32(0.5)).astype(np.float32)  # overlaps system A
+    x = np.concatenate([xa, xb])
+    np.save(tmp_path / "x.npy", x)
+    code = textwrap.dedent(f"""
+        import ctypes, sys
+        sys.path.insert(0, {ROOT!r})
+        import numpy as np
+        from biospring_amd import _abi, workloads
+        from biospring_amd.network import SpringNetwork
+        L = _abi.bind(ctypes.CDLL({hostcheck_lib!r}))
+        x = np.load({str(tmp_path / "x.npy")!r})
+        net = SpringNetwork(workloads.params(99), library=L)
+        net.setParticles(x)
+        net.setSystems(np.array([0, 700, 1400], np.int64))
+        net.springsFromCutoff(6.0)
+        a, b, l0, k = net.getSprings()
+        np.savez({str(tmp_path / "s.npz")!r}, a=a, b=b, l0=l0)
+        net.run(1)
+        net.close()
+        """)
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
+    z = np.load(tmp_path / "s.npz")
+    a, b, l0 = z["a"], z["b"], z["l0"]
+    assert np.all((a < 700) == (b < 700))
+    for lo, xs in ((0, xa), (700, xb)):
+        sel = a >= lo if lo else a < 700
+        same_springs((a[sel] - lo, b[sel] - lo, l0[sel]), po.springs_from_cutoff(xs, None, 6.0))
