@@ -3182,7 +3182,16 @@ int bs_load_netcdf(bs_handle *h, const char *path, char *warnings, int64_t warni
         return fail(h, BS_ERR_ARG, "bs_load_netcdf: empty path");
     bs::NcTopology t;
     std::string err;
-    if (!bs::read_netcdf_topology(path, t, err))
+    bool ok = false;
+    try
+    {
+        ok = bs::read_netcdf_topology(path, t, err);
+    }
+    catch (const std::exception &x) // memory for a damaged or huge file
+    {
+        err = std::string("cannot read ") + path + ": " + x.what();
+    }
+    if (!ok)
         return fail(h, BS_ERR_ARG, "%s", err.c_str());
     std::vector<uint8_t> flags((size_t)t.n);
     for (size_t i = 0; i < flags.size(); ++i)

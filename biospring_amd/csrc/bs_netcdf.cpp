@@ -183,10 +183,20 @@ class NcFile
     bool values(const Var &v, std::vector<double> &out)
     {
         const int ts = type_size(v.type);
+        // every element takes at least one byte of the file: larger shapes
+        // are a damaged header, not something to allocate
+        const std::string big = format("variable \"%s\": shape larger than the file", v.name.c_str());
         uint64_t per = 1; // elements per record (or in all, for a fixed variable)
         for (size_t k = is_record(v) ? 1 : 0; k < v.dimids.size(); ++k)
-            per *= dim_len(v, k);
+        {
+            const uint64_t d = dim_len(v, k);
+            if (d != 0 && per > buf.size() / d)
+                return fail(big);
+            per *= d;
+        }
         const uint64_t nrec = is_record(v) ? numrecs : 1;
+        if (nrec != 0 && per > buf.size() / nrec)
+            return fail(big);
         out.resize((size_t)(per * nrec));
         for (uint64_t r = 0; r < nrec; ++r)
         {

@@ -20,6 +20,7 @@
 #include <cstdarg>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
 
 namespace bs
 {
@@ -585,7 +586,17 @@ void TrajectorySet::run()
         for (auto &w : writers_)
             if (it.all || it.f.nbiter % w->frequency == 0)
             {
-                if (!w->write(it.f) && e.empty())
+                bool ok = false;
+                try
+                {
+                    ok = w->write(it.f);
+                }
+                catch (const std::exception &x)
+                {
+                    ok = false;
+                    e = e.empty() ? std::string("write failed: ") + x.what() : e;
+                }
+                if (!ok && e.empty())
                     e = "write failed: " + w->path;
                 ++w->frame;
             }
