@@ -37,6 +37,7 @@ EXPORTS = (
     "bs_springs_from_cutoff", "bs_get_springs", "bs_add_trajectory", "bs_write_frame_now",
     "bs_flush_trajectories", "bs_close_trajectories", "bs_set_particle_labels", "bs_set_insertion_vector",
     "bs_load_netcdf", "bs_get_particle_properties", "bs_set_sasa", "bs_set_membrane",
+    "bs_springs_between_groups_from_cutoff",
 )
 PROF_REBUILD, PROF_PAIR, PROF_SPRING = range(3)
 
@@ -164,6 +165,7 @@ def bind(L):
     L.bs_adjoint_forward.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, _f32p]
     L.bs_springs_from_cutoff.argtypes = [_vp, ctypes.c_double, ctypes.c_float, _i64p]
     L.bs_get_springs.argtypes = [_vp, _u32p, _u32p, _f32p, _f32p]
+    L.bs_springs_between_groups_from_cutoff.argtypes = [_vp, ctypes.c_double, ctypes.c_float, _i32p, _i64p]
     L.bs_adjoint_backward.argtypes = [_vp, _f64p, _f64p, _f64p, _f64p, _f64p]
     L.bs_add_trajectory.argtypes = [_vp, ctypes.c_int32, ctypes.c_char_p, ctypes.c_int64]
     L.bs_write_frame_now.argtypes = [_vp]

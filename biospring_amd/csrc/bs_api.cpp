@@ -2673,7 +2673,26 @@ int bs_adjoint(bs_handle *h, int32_t nsteps, int32_t checkpoint_every, const flo
     return adjoint_sweep(h, S, seed, dk, dl0, dq);
 }
 
+static int springs_from_cutoff(bs_handle *h, double cutoff, float stiffness, const int32_t *group,
+                               int64_t *count_out);
+
 int bs_springs_from_cutoff(bs_handle *h, double cutoff, float stiffness, int64_t *count_out)
+{
+    return springs_from_cutoff(h, cutoff, stiffness, nullptr, count_out);
+}
+
+int bs_springs_between_groups_from_cutoff(bs_handle *h, double cutoff, float stiffness, const int32_t *group,
+                                          int64_t *count_out)
+{
+    if (!h)
+        return BS_ERR_ARG;
+    if (!group)
+        return fail(h, BS_ERR_ARG, "bs_springs_between_groups_from_cutoff: no groups");
+    return springs_from_cutoff(h, cutoff, stiffness, group, count_out);
+}
+
+static int springs_from_cutoff(bs_handle *h, double cutoff, float stiffness, const int32_t *group,
+                               int64_t *count_out)
 {
     if (!h)
         return BS_ERR_ARG;
@@ -2762,6 +2781,13 @@ int bs_springs_from_cutoff(bs_handle *h, double cutoff, float stiffness, int64_t
             a.dims[q] = dims[q];
         a.edge = edge;
         a.rc = (double)rcf;
+        if (group)
+        {
+            int32_t *gd = S.get<int32_t>((size_t)n, e);
+            HIPCHK(h, e);
+            HIPCHK(h, hipMemcpyAsync(gd, group, sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
+            a.group = gd;
+        }
         // sqrt(r2) < rc  <=>  r2 < r2max, sqrt being monotonic and correctly
         // rounded (here and on the device)
         double r2max = a.rc * a.rc;

@@ -241,6 +241,7 @@ struct TopoArgs
     const int32_t *ex_off;    // nullable: springs already loaded, as rows of
     const int32_t *ex_j;      //   their lower index [n + 1], partners sorted
     int32_t max_row;          // fill pass: the longest row
+    const int32_t *group;     // nullable: only pairs of different groups
 };
 
 struct ListArgs

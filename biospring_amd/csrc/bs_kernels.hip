@@ -928,7 +928,9 @@ __global__ void __launch_bounds__(BLOCK) k_adj_spring(AdjSpringArgs a)
 // sizes the rows, the fill pass writes each row sorted by j, so that the
 // springs come out ordered by (i, j).  pos.w carries the static flag.
 // Pairs linked by a spring loaded before are skipped, as the reference's
-// add_spring throws for them (SpringCollection.hpp:139-160).
+// add_spring throws for them (SpringCollection.hpp:139-160).  With a group
+// per particle, only pairs of different groups are linked
+// (Topology::add_springs_between_topologies_from_cutoff, Topology.hpp:140-180).
 // ---------------------------------------------------------------------------
 // Sorts a row (second indices j, all distinct, with their rest lengths) by
 // j: insertion sort for short rows, heapsort beyond, so that a crowded row
@@ -1035,6 +1037,7 @@ __global__ __launch_bounds__(64 * TOPO_WAVES) void k_topo_rows(TopoArgs a)
         return; // the whole wave: no block-wide barrier below
     const float4 pi = a.pos[i];
     const bool si = fbits(pi.w) != 0u;
+    const int32_t gi = a.group ? a.group[i] : 0;
     int32_t c[3];
     const float xs[3] = {pi.x, pi.y, pi.z};
     for (int q = 0; q < 3; ++q)
@@ -1080,7 +1083,7 @@ __global__ __launch_bounds__(64 * TOPO_WAVES) void k_topo_rows(TopoArgs a)
             for (int q = 7; q >= 0; --q)
                 m = t < re[q] ? rs[q] + t : m;
             j = a.cidx[m];
-            if (j > i)
+            if (j > i && !(a.group && a.group[j] == gi)) // same topology: skipped when merging
             {
                 const float4 pj = a.pos[j];
                 if (!(si && fbits(pj.w) != 0u))

@@ -787,6 +787,8 @@ void launch_topo_springs(const TopoArgs &a, hipStream_t)
     const int64_t ncells = (int64_t)a.dims[0] * a.dims[1] * a.dims[2];
     need(a.cstart, sizeof(int32_t) * (ncells + 1), "topo: cstart");
     need(a.cidx, sizeof(int32_t) * n, "topo: cidx");
+    if (a.group)
+        need(a.group, sizeof(int32_t) * n, "topo: group");
     if (!(std::sqrt(a.r2max) >= a.rc && std::sqrt(std::nextafter(a.r2max, 0.0)) < a.rc))
         hc_fail("topo: r2max is not the squared-distance threshold of the cutoff");
     if (!a.offs)
@@ -813,6 +815,8 @@ void launch_topo_springs(const TopoArgs &a, hipStream_t)
                         if (j < 0 || j >= n)
                             hc_fail("topo: member index out of range");
                         if (j <= i)
+                            continue;
+                        if (a.group && a.group[j] == a.group[i])
                             continue;
                         const float4 pj = a.pos[j];
                         if (si && fbits(pj.w) != 0u)

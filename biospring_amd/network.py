@@ -57,6 +57,16 @@ class SpringNetwork:
               "bs_springs_from_cutoff")
         return int(m.value)
 
+    def springsBetweenGroupsFromCutoff(self, cutoff, group, stiffness=1.0):
+        """Topology::add_springs_between_topologies_from_cutoff: springs only
+        between particles of different groups (topology ids)."""
+        g = np.ascontiguousarray(group, np.int32)
+        m = ctypes.c_int64()
+        check(self._L.bs_springs_between_groups_from_cutoff(self._h, float(cutoff), float(stiffness),
+                                                            ptr(g, ctypes.c_int32), ctypes.byref(m)),
+              self._h, "bs_springs_between_groups_from_cutoff")
+        return int(m.value)
+
     def getSprings(self):
         """(id1, id2, equilibrium, stiffness) of the loaded springs."""
         m = int(self._L.bs_num_springs(self._h))

@@ -180,6 +180,14 @@ const char *bs_last_error(const bs_handle *h);
  * number of springs added. */
 int bs_springs_from_cutoff(bs_handle *h, double cutoff, float stiffness, int64_t *count_out);
 
+/* Replaces Topology::add_springs_between_topologies_from_cutoff
+ * (src/topology/Topology.hpp:140-180), the spring step of merging
+ * topologies (mergespn-cli.cpp:51): as bs_springs_from_cutoff, but only for
+ * pairs whose particles belong to different groups (group[i], n entries: the
+ * topology id of each particle). */
+int bs_springs_between_groups_from_cutoff(bs_handle *h, double cutoff, float stiffness, const int32_t *group,
+                                          int64_t *count_out);
+
 /* The loaded springs (bs_set_springs or bs_springs_from_cutoff order);
  * every pointer nullable, bs_num_springs entries each. */
 int bs_get_springs(const bs_handle *h, uint32_t *id1, uint32_t *id2, float *equilibrium, float *stiffness);

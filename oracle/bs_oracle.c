@@ -2574,7 +2574,10 @@ int orc_num_threads(void) { return max_threads(); }
  *
  * Written as a plain grid of cells of edge >= the cutoff scanned over the 27
  * cells around each particle; rows are sorted by the second index, so the
- * output is ordered by (first, second).  Returns the number of springs and,
+ * output is ordered by (first, second).  With groups (nullable), only pairs
+ * of different groups are linked: add_springs_between_topologies_from_cutoff
+ * (Topology.hpp:140-180), groups being the topology ids.  Returns the number
+ * of springs and,
  * when id1 is not NULL, writes the rows that fit in cap entries (all of them
  * when cap >= that number); -1 for a cutoff the reference rejects
  * (nsearch.hpp:66-67).
@@ -2585,8 +2588,8 @@ static int cmp_row_j(const void *a, const void *b)
     return (x > y) - (x < y);
 }
 
-int64_t orc_springs_from_cutoff(int64_t n, const float *x3, const uint8_t *flags, double cutoff, int64_t cap,
-                                uint32_t *id1, uint32_t *id2, float *l0)
+int64_t orc_springs_from_cutoff(int64_t n, const float *x3, const uint8_t *flags, const int32_t *groups,
+                                double cutoff, int64_t cap, uint32_t *id1, uint32_t *id2, float *l0)
 {
     const float rcf = (float)cutoff;
     if (!(rcf >= 1e-6f) || !isfinite(rcf))
@@ -2661,6 +2664,8 @@ int64_t orc_springs_from_cutoff(int64_t n, const float *x3, const uint8_t *flags
                     {
                         const int64_t j = member[t];
                         if (j <= i || (si && flags && (flags[j] & BS_FLAG_STATIC)))
+                            continue;
+                        if (groups && groups[i] == groups[j])
                             continue;
                         const double dx = (double)x3[3 * i] - (double)x3[3 * j];
                         const double dy = (double)x3[3 * i + 1] - (double)x3[3 * j + 1];

@@ -82,8 +82,8 @@ def oracle_lib():
         L.orc_loss.restype = ctypes.c_double
         L.orc_set_params_arrays.argtypes = [_vp, _f32p, _f32p, _f32p]
         L.orc_hessian_vec.argtypes = [_vp, _f64p, _f64p]
-        L.orc_springs_from_cutoff.argtypes = [ctypes.c_int64, _f32p, _u8p, ctypes.c_double, ctypes.c_int64, _u32p,
-                                              _u32p, _f32p]
+        L.orc_springs_from_cutoff.argtypes = [ctypes.c_int64, _f32p, _u8p, ctypes.POINTER(ctypes.c_int32),
+                                              ctypes.c_double, ctypes.c_int64, _u32p, _u32p, _f32p]
         L.orc_springs_from_cutoff.restype = ctypes.c_int64
         _olib = L
     return _olib
@@ -120,9 +120,9 @@ def ref_lib():
         L.ref_set_grid.argtypes = [_vp, ctypes.c_int, _f64p, _i64p, _f64p, _f32p, ctypes.c_double]
         L.ref_set_burying.argtypes = [_vp, _f32p]
         L.ref_insertion.argtypes = [_vp, ctypes.c_int64, ctypes.c_int64, _f64p]
-        L.ref_springs_from_cutoff.argtypes = [ctypes.c_int64, _f32p, _u8p, ctypes.c_int64, _u32p, _u32p, _f32p, _f32p,
-                                              ctypes.c_double, ctypes.c_double, ctypes.c_int64, _u32p, _u32p, _f32p,
-                                              _f32p]
+        L.ref_springs_from_cutoff.argtypes = [ctypes.c_int64, _f32p, _u8p, ctypes.POINTER(ctypes.c_int32),
+                                              ctypes.c_int64, _u32p, _u32p, _f32p, _f32p, ctypes.c_double,
+                                              ctypes.c_double, ctypes.c_int64, _u32p, _u32p, _f32p, _f32p]
         L.ref_springs_from_cutoff.restype = ctypes.c_int64
         L.ref_set_trajectories.argtypes = [ctypes.c_char_p, ctypes.c_int64, ctypes.c_char_p, ctypes.c_int64]
         L.ref_xtc_write.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, _f32p, ctypes.c_float]
@@ -380,24 +380,27 @@ def ref_pairs(system, kind, params=None):
     return out[:2 * cnt].reshape(-1, 2)
 
 
-def springs_from_cutoff(x, flags, cutoff):
-    """Oracle: springs of Topology::add_springs_from_cutoff ordered by
-    (first, second), first < second, with float rest lengths."""
+def springs_from_cutoff(x, flags, cutoff, groups=None):
+    """Oracle: springs of Topology::add_springs_from_cutoff (with groups,
+    add_springs_between_topologies_from_cutoff) ordered by (first, second),
+    first < second, with float rest lengths."""
     x = np.ascontiguousarray(x, np.float32).reshape(-1)
     n = x.size // 3
     fl = None if flags is None else np.ascontiguousarray(flags, np.uint8)
+    gr = None if groups is None else np.ascontiguousarray(groups, np.int32)
     c = ctypes
     fn = oracle_lib().orc_springs_from_cutoff
-    cnt = fn(n, _p(x, c.c_float), _p(fl, c.c_uint8), cutoff, 0, None, None, None)
+    head = [n, _p(x, c.c_float), _p(fl, c.c_uint8), _p(gr, c.c_int32), cutoff]
+    cnt = fn(*head, 0, None, None, None)
     if cnt < 0:
         raise ValueError(f"cutoff {cutoff} rejected")
     i1, i2 = np.zeros(max(cnt, 1), np.uint32), np.zeros(max(cnt, 1), np.uint32)
     l0 = np.zeros(max(cnt, 1), np.float32)
-    fn(n, _p(x, c.c_float), _p(fl, c.c_uint8), cutoff, cnt, _p(i1, c.c_uint32), _p(i2, c.c_uint32), _p(l0, c.c_float))
+    fn(*head, cnt, _p(i1, c.c_uint32), _p(i2, c.c_uint32), _p(l0, c.c_float))
     return i1[:cnt], i2[:cnt], l0[:cnt]
 
 
-def ref_springs_from_cutoff(x, flags, cutoff, stiffness=1.0, existing=None):
+def ref_springs_from_cutoff(x, flags, cutoff, stiffness=1.0, existing=None, groups=None):
     """The reference's own Topology::add_springs_from_cutoff, run after the
     `existing` springs (id1, id2, l0, k) were added with add_spring; all
     springs in its order as (first, second, float(equilibrium)), plus
@@ -409,8 +412,9 @@ def ref_springs_from_cutoff(x, flags, cutoff, stiffness=1.0, existing=None):
     ex = [None] * 4 if existing is None else [np.ascontiguousarray(a, t) for a, t in
                                                zip(existing, (np.uint32, np.uint32, np.float32, np.float32))]
     m0 = 0 if existing is None else len(ex[0])
-    head = [n, _p(x, c.c_float), _p(fl, c.c_uint8), m0, _p(ex[0], c.c_uint32), _p(ex[1], c.c_uint32),
-            _p(ex[2], c.c_float), _p(ex[3], c.c_float), cutoff, stiffness]
+    gr = None if groups is None else np.ascontiguousarray(groups, np.int32)
+    head = [n, _p(x, c.c_float), _p(fl, c.c_uint8), _p(gr, c.c_int32), m0, _p(ex[0], c.c_uint32),
+            _p(ex[1], c.c_uint32), _p(ex[2], c.c_float), _p(ex[3], c.c_float), cutoff, stiffness]
     fn = ref_lib().ref_springs_from_cutoff
     cnt = fn(*head, 0, None, None, None, None)
     i1, i2 = np.zeros(max(cnt, 1), np.uint32), np.zeros(max(cnt, 1), np.uint32)

@@ -24,8 +24,10 @@
 //  - ref_insertion: InsertionVector::computeVector/Angle/RollAngle
 //    (src/InsertionVector.cpp:18-78).
 //  - ref_springs_from_cutoff: topology::Topology::add_springs_from_cutoff
-//    (src/topology/Topology.hpp:105-138) on particles at the given positions,
-//    after the given springs were added with Topology::add_spring; all
+//    (src/topology/Topology.hpp:105-138), or with topology ids
+//    add_springs_between_topologies_from_cutoff (Topology.hpp:140-180), on
+//    particles at the given positions, after the given springs were added
+//    with Topology::add_spring; all
 //    springs read back as (first, second, float(equilibrium),
 //    float(stiffness)) in the reference's own order.
 //  - ref_set_trajectories: the pdbtrajectory / csvsampling settings applied
@@ -549,9 +551,10 @@ int64_t ref_pairs(const bs_params * p, int kind, int64_t n, const float * x3, co
 // Springs of Topology::add_springs_from_cutoff: returns the count; with cap
 // >= count, writes id1/id2 (indices in the given order) and the rest lengths
 // as SpringNetwork::addSpring receives them (float).
-int64_t ref_springs_from_cutoff(int64_t n, const float * x3, const uint8_t * flags, int64_t m0, const uint32_t * e1,
-                                const uint32_t * e2, const float * el0, const float * ek, double cutoff,
-                                double stiffness, int64_t cap, uint32_t * id1, uint32_t * id2, float * l0, float * k)
+int64_t ref_springs_from_cutoff(int64_t n, const float * x3, const uint8_t * flags, const int32_t * groups, int64_t m0,
+                                const uint32_t * e1, const uint32_t * e2, const float * el0, const float * ek,
+                                double cutoff, double stiffness, int64_t cap, uint32_t * id1, uint32_t * id2,
+                                float * l0, float * k)
 {
     using biospring::topology::Topology;
     using biospring::topology::Particle;
@@ -560,6 +563,8 @@ int64_t ref_springs_from_cutoff(int64_t n, const float * x3, const uint8_t * fla
     {
         ps[(size_t)i].set_position(Vector3f(x3[3 * i], x3[3 * i + 1], x3[3 * i + 2]));
         ps[(size_t)i].properties().set_static(flags && (flags[i] & BS_FLAG_STATIC));
+        if (groups)
+            ps[(size_t)i].properties().set_topology_id((size_t)groups[i]);
     }
     Topology top;
     top.add_particles(ps);
@@ -568,7 +573,10 @@ int64_t ref_springs_from_cutoff(int64_t n, const float * x3, const uint8_t * fla
         index[(int64_t)top.get_particle((size_t)i).unique_id()] = (uint32_t)i;
     for (int64_t s = 0; s < m0; ++s) // springs present before, e.g. from CONECT records
         top.add_spring((size_t)e1[s], (size_t)e2[s], (double)el0[s], (double)ek[s]);
-    top.add_springs_from_cutoff(cutoff, stiffness);
+    if (groups)
+        top.add_springs_between_topologies_from_cutoff(cutoff, stiffness);
+    else
+        top.add_springs_from_cutoff(cutoff, stiffness);
     const int64_t m = (int64_t)top.number_of_springs();
     if (cap >= m)
         for (int64_t s = 0; s < m; ++s)

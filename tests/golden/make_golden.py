@@ -385,6 +385,24 @@ def topology_cases():
         np.savez_compressed(path, in_x=x, in_flags=flags, cutoff=np.float64(rc), ref_id1=i1, ref_id2=i2, ref_l0=l0)
         print(f"{name}: n={len(x)} springs={len(i1)} -> {os.path.getsize(path) / 1024:.0f} KiB")
 
+    # two topologies being merged (mergespn-cli.cpp:51): springs only between
+    # particles of different topologies (Topology.hpp:140-180)
+    name = "topo_groups900"
+    if not ONLY or name in ONLY:
+        rng = np.random.default_rng(2027)
+        xa = rng.uniform(-15, 15, (450, 3))
+        xb = rng.uniform(-15, 15, (450, 3)) + np.array([12.0, 3.0, -4.0])
+        x = np.concatenate([xa, xb]).astype(np.float32)
+        groups = np.repeat(np.array([0, 1], np.int32), 450)
+        perm = rng.permutation(900)  # topologies interleaved in the particle order
+        x, groups = x[perm], groups[perm]
+        flags = np.where(rng.random(900) < 0.1, _abi.FLAG_STATIC, 0).astype(np.uint8)
+        i1, i2, l0 = po.ref_springs_from_cutoff(x, flags, 6.0, 1.0, groups=groups)
+        path = os.path.join(OUT, f"{name}.npz")
+        np.savez_compressed(path, in_x=x, in_flags=flags, in_groups=groups, cutoff=np.float64(6.0), ref_id1=i1,
+                            ref_id2=i2, ref_l0=l0)
+        print(f"{name}: n=900 springs={len(i1)} -> {os.path.getsize(path) / 1024:.0f} KiB")
+
     # springs present before the call (CONECT records read by pdb2spn,
     # pdb2spn-cli.cpp:59-70): they keep their rest length and stiffness, and
     # their pairs get no second spring (SpringCollection.hpp:138-151)

@@ -169,3 +169,22 @@ def test_systems_stay_apart():
         same_springs((a[sel] - lo, b[sel] - lo, l0[sel]), po.springs_from_cutoff(xs, None, 8.0))
     net.run(2)
     net.close()
+
+
+def test_between_groups_matches_reference_and_oracle():
+    c = load_case("topo_groups900")
+    _log("topology between groups: fixture")
+    net = SpringNetwork(workloads.params(1))
+    net.setParticles(c["in_x"], flags=c["in_flags"])
+    net.springsBetweenGroupsFromCutoff(float(c["cutoff"]), c["in_groups"])
+    same_springs(net.getSprings()[:3], (c["ref_id1"], c["ref_id2"], c["ref_l0"]))
+    net.close()
+    rng = np.random.default_rng(12)
+    x = rng.uniform(-40, 40, (20000, 3)).astype(np.float32)
+    g = rng.integers(0, 3, 20000).astype(np.int32)
+    _log("topology between groups: n=20000, 3 groups")
+    net = SpringNetwork(workloads.params(1))
+    net.setParticles(x)
+    net.springsBetweenGroupsFromCutoff(7.5, g)
+    same_springs(net.getSprings()[:3], po.springs_from_cutoff(x, None, 7.5, groups=g))
+    net.close()

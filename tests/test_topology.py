@@ -50,9 +50,10 @@ def same_springs(got, want):
     assert np.array_equal(g[2].view(np.uint32), w[2].view(np.uint32))
 
 
-def brute_force(x, flags, cutoff):
+def brute_force(x, flags, cutoff, groups=None):
     """All pairs i < j with the FP64 distance of the float positions below
-    the cutoff held as a float, static-static pairs left out."""
+    the cutoff held as a float, static-static pairs left out (and pairs of
+    one group, when groups are given)."""
     x = np.asarray(x, np.float32).reshape(-1, 3).astype(np.float64)
     n = len(x)
     ii, jj = np.triu_indices(n, 1)
@@ -62,6 +63,9 @@ def brute_force(x, flags, cutoff):
     if flags is not None:
         st = (np.asarray(flags) & _abi.FLAG_STATIC) != 0
         keep &= ~(st[ii] & st[jj])
+    if groups is not None:
+        g = np.asarray(groups)
+        keep &= g[ii] != g[jj]
     return ii[keep], jj[keep], dist[keep].astype(np.float32)
 
 
@@ -280,3 +284,40 @@ def test_host_construction_keeps_systems_apart(hostcheck_lib, tmp_path):
     for lo, xs in ((0, xa), (700, xb)):
         sel = a >= lo if lo else a < 700
         same_springs((a[sel] - lo, b[sel] - lo, l0[sel]), po.springs_from_cutoff(xs, None, 6.0))
+
+
+def test_groups_fixture_oracle_and_brute_force():
+    """Merging topologies (Topology::add_springs_between_topologies_from_cutoff):
+    the oracle and brute force give the reference's springs, all between the
+    two topologies."""
+    c = load_case("topo_groups900")
+    g = c["in_groups"]
+    ref = (c["ref_id1"], c["ref_id2"], c["ref_l0"])
+    assert np.all(g[ref[0]] != g[ref[1]])
+    same_springs(po.springs_from_cutoff(c["in_x"], c["in_flags"], float(c["cutoff"]), groups=g), ref)
+    same_springs(brute_force(c["in_x"], c["in_flags"], float(c["cutoff"]), groups=g), ref)
+
+
+def test_host_construction_between_groups(hostcheck_lib, tmp_path):
+    c = load_case("topo_groups900")
+    np.savez(tmp_path / "c.npz", **c)
+    code = textwrap.dedent(f"""
+        import ctypes, sys
+        sys.path.insert(0, {ROOT!r})
+        import numpy as np
+        from biospring_amd import _abi, workloads
+        from biospring_amd.network import SpringNetwork
+        L = _abi.bind(ctypes.CDLL({hostcheck_lib!r}))
+        z = np.load({str(tmp_path / "c.npz")!r})
+        net = SpringNetwork(workloads.params(99), library=L)
+        net.setParticles(z["in_x"], flags=z["in_flags"])
+        net.springsBetweenGroupsFromCutoff(float(z["cutoff"]), z["in_groups"], 0.5)
+        a, b, l0, k = net.getSprings()
+        assert np.all(k == np.float32(0.5))
+        np.savez({str(tmp_path / "s.npz")!r}, a=a, b=b, l0=l0)
+        net.close()
+        """)
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
+    z = np.load(tmp_path / "s.npz")
+    same_springs((z["a"], z["b"], z["l0"]), (c["ref_id1"], c["ref_id2"], c["ref_l0"]))
