@@ -37,7 +37,7 @@ EXPORTS = (
     "bs_springs_from_cutoff", "bs_get_springs", "bs_add_trajectory", "bs_write_frame_now",
     "bs_flush_trajectories", "bs_close_trajectories", "bs_set_particle_labels", "bs_set_insertion_vector",
     "bs_load_netcdf", "bs_get_particle_properties", "bs_set_sasa", "bs_set_membrane",
-    "bs_springs_between_groups_from_cutoff",
+    "bs_springs_between_groups_from_cutoff", "bs_save_netcdf",
 )
 PROF_REBUILD, PROF_PAIR, PROF_SPRING = range(3)
 
@@ -175,6 +175,7 @@ def bind(L):
     L.bs_set_particle_labels.argtypes = [_vp, _strs, _strs, _strs, _i32p, _f32p, _f32p, _strs, _i32p]
     L.bs_set_insertion_vector.argtypes = [_vp, ctypes.c_int64, ctypes.c_int64]
     L.bs_load_netcdf.argtypes = [_vp, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int64]
+    L.bs_save_netcdf.argtypes = [_vp, ctypes.c_char_p]
     L.bs_get_particle_properties.argtypes = [_vp] + [_f32p] * 7 + [_u8p]
     L.bs_set_sasa.argtypes = [_vp, _f32p]
     L.bs_set_membrane.argtypes = [_vp] + [ctypes.c_float] * 4

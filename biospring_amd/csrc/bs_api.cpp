@@ -3244,6 +3244,54 @@ int bs_load_netcdf(bs_handle *h, const char *path, char *warnings, int64_t warni
     return BS_OK;
 }
 
+int bs_save_netcdf(bs_handle *h, const char *path)
+{
+    if (!h)
+        return BS_ERR_ARG;
+    if (!path || !*path)
+        return fail(h, BS_ERR_ARG, "bs_save_netcdf: empty path");
+    if (!h->have_particles)
+        return fail(h, BS_ERR_STATE, "bs_save_netcdf: no particles");
+    bs::NcTopology t;
+    t.n = h->n;
+    t.x.resize((size_t)(3 * h->n));
+    std::vector<float> v((size_t)(3 * h->n));
+    int rc = bs_get_state(h, t.x.data(), v.data()); // the current positions
+    if (rc)
+        return rc;
+    t.mass = h->mass;
+    t.charge = h->charge;
+    t.radius = h->radius;
+    t.epsilon = h->eps;
+    t.sasa = h->sasa;
+    t.transfer = h->transfer;
+    t.hydro = h->hydro;
+    t.is_static.resize((size_t)h->n);
+    for (size_t i = 0; i < t.is_static.size(); ++i)
+        t.is_static[i] = (h->flags[i] & BS_FLAG_STATIC) ? 1 : 0;
+    t.name = h->labels.name;
+    t.resname = h->labels.resname;
+    t.chain = h->labels.chain;
+    t.resid = h->labels.resid;
+    t.s1 = h->s1;
+    t.s2 = h->s2;
+    t.l0 = h->sl0;
+    t.k = h->sk;
+    std::string err;
+    bool ok = false;
+    try
+    {
+        ok = bs::write_netcdf_topology(path, t, err);
+    }
+    catch (const std::exception &x)
+    {
+        err = x.what();
+    }
+    if (!ok)
+        return fail(h, BS_ERR_ARG, "bs_save_netcdf: %s", err.c_str());
+    return BS_OK;
+}
+
 int bs_get_particle_properties(const bs_handle *h, float *mass, float *charge, float *radius, float *epsilon,
                                float *sasa, float *transfer, float *hydrophobicity, uint8_t *flags)
 {

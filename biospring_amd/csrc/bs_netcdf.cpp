@@ -8,6 +8,8 @@
 // addParticlesToSpn / addSpringsToSpn (src/IO/NetCDFReader.cpp:8-280).
 #include "bs_netcdf.hpp"
 
+#include <algorithm>
+#include <cerrno>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -368,7 +370,201 @@ std::vector<std::string> names(const std::vector<char> &c, size_t n, size_t len)
     return out;
 }
 
+// ---------------------------------------------------------------------------
+// Writing (CDF-2: the classic layout with 64-bit variable offsets).
+// ---------------------------------------------------------------------------
+struct OutVar
+{
+    std::string name;
+    std::vector<int> dims;
+    int type;
+    std::vector<std::pair<std::string, std::string>> atts; // text attributes
+    std::vector<uint8_t> data;                              // big-endian, unpadded
+};
+
+void be32(std::vector<uint8_t> &o, uint32_t v)
+{
+    for (int s = 24; s >= 0; s -= 8)
+        o.push_back((uint8_t)(v >> s));
+}
+void be64(std::vector<uint8_t> &o, uint64_t v)
+{
+    be32(o, (uint32_t)(v >> 32));
+    be32(o, (uint32_t)v);
+}
+void pad4(std::vector<uint8_t> &o)
+{
+    while (o.size() % 4)
+        o.push_back(0);
+}
+void nc_name(std::vector<uint8_t> &o, const std::string &s)
+{
+    be32(o, (uint32_t)s.size());
+    o.insert(o.end(), s.begin(), s.end());
+    pad4(o);
+}
+template <class T> std::vector<uint8_t> be_values(const T *v, size_t n)
+{
+    std::vector<uint8_t> o;
+    o.reserve(n * sizeof(T));
+    for (size_t i = 0; i < n; ++i)
+    {
+        uint64_t u = 0;
+        std::memcpy(&u, &v[i], sizeof(T));
+        for (int b = (int)sizeof(T) - 1; b >= 0; --b)
+            o.push_back((uint8_t)(u >> (8 * b)));
+    }
+    return o;
+}
+std::vector<uint8_t> fixed_chars(const std::vector<std::string> &names, size_t n, size_t len)
+{
+    std::vector<uint8_t> o(n * len, 0);
+    for (size_t i = 0; i < n && i < names.size(); ++i)
+        std::memcpy(o.data() + i * len, names[i].data(), std::min(len, names[i].size())); // strncpy
+    return o;
+}
+
 } // namespace
+
+bool write_netcdf_topology(const std::string &path, const NcTopology &t, std::string &err)
+{
+    const size_t n = (size_t)t.n, m = t.s1.size();
+    std::vector<std::pair<std::string, uint64_t>> dims = {{"spatialdim", 3},
+                                                          {"particle_number", n},
+                                                          {"particlename_length", NAME_LEN},
+                                                          {"chainname_length", CHAIN_LEN},
+                                                          {"resname_length", RESNAME_LEN}};
+    enum
+    {
+        D_SPATIAL,
+        D_PNB,
+        D_PNAME,
+        D_CHAIN,
+        D_RESNAME,
+        D_SPRINGDIM,
+        D_SNB
+    };
+    std::vector<OutVar> vars;
+    auto add = [&](const char *nm, std::vector<int> d, int type, std::vector<std::pair<std::string, std::string>> atts,
+                   std::vector<uint8_t> data) { vars.push_back(OutVar{nm, std::move(d), type, std::move(atts), std::move(data)}); };
+    std::vector<int32_t> ids(n);
+    std::vector<int8_t> dyn8(n);
+    for (size_t i = 0; i < n; ++i)
+    {
+        ids[i] = (int32_t)i;
+        dyn8[i] = t.is_static[i] ? 0 : 1;
+    }
+    std::vector<int32_t> resid(t.resid);
+    resid.resize(n, 0);
+    add("coordinates", {D_PNB, D_SPATIAL}, NC_FLOAT, {{"units", "angstrom"}, {"long_name", "Particle coordinates"}},
+        be_values(t.x.data(), 3 * n));
+    add("particleids", {D_PNB}, NC_INT, {{"long_name", "Particle ids in source database"}}, be_values(ids.data(), n));
+    add("charges", {D_PNB}, NC_FLOAT, {{"units", "electron"}, {"long_name", "Particle charge id"}},
+        be_values(t.charge.data(), n));
+    add("radii", {D_PNB}, NC_FLOAT, {{"units", "A"}, {"long_name", "Particle radius"}}, be_values(t.radius.data(), n));
+    add("epsilon", {D_PNB}, NC_FLOAT, {{"units", "kJ.mol-1"}, {"long_name", "Particle epsilon for Lennard-Jones"}},
+        be_values(t.epsilon.data(), n));
+    add("mass", {D_PNB}, NC_FLOAT, {{"units", "Da"}, {"long_name", "Particle mass"}}, be_values(t.mass.data(), n));
+    add("surfaceaccessibility", {D_PNB}, NC_FLOAT,
+        {{"units", "A2 or percent"}, {"long_name", "Particle surface accessibility"}}, be_values(t.sasa.data(), n));
+    add("hydrophobicityscale", {D_PNB}, NC_FLOAT,
+        {{"units", "kJ.mol-1"}, {"long_name", "Particle hydrophobicity scale (transfer energy)"}},
+        be_values(t.transfer.data(), n));
+    add("hydrophobicity", {D_PNB}, NC_FLOAT, {{"long_name", "Particle hydrophobicity (pairwise hydrophobic force)"}},
+        be_values(t.hydro.data(), n));
+    add("resids", {D_PNB}, NC_INT, {{"long_name", "Particle residue id"}}, be_values(resid.data(), n));
+    add("dynamicstate", {D_PNB}, NC_BYTE, {{"long_name", "Particle dynamic state (static 0, dynamic 1)"}},
+        be_values(dyn8.data(), n));
+    add("particlenames", {D_PNB, D_PNAME}, NC_CHAR, {{"long_name", "Particle name"}}, fixed_chars(t.name, n, NAME_LEN));
+    add("resnames", {D_PNB, D_RESNAME}, NC_CHAR, {{"long_name", "Particle residue name"}},
+        fixed_chars(t.resname, n, RESNAME_LEN));
+    add("chainnames", {D_PNB, D_CHAIN}, NC_CHAR, {{"long_name", "Particle chain name"}},
+        fixed_chars(t.chain, n, CHAIN_LEN));
+    if (m > 0)
+    {
+        dims.emplace_back("springdim", 2);
+        dims.emplace_back("spring_number", m);
+        std::vector<int32_t> ij(2 * m), per(n, 0);
+        for (size_t s = 0; s < m; ++s)
+        {
+            ij[2 * s] = (int32_t)t.s1[s];
+            ij[2 * s + 1] = (int32_t)t.s2[s];
+            ++per[t.s1[s]];
+            ++per[t.s2[s]];
+        }
+        add("springs", {D_SNB, D_SPRINGDIM}, NC_INT,
+            {{"long_name", "Spring between particles referenced by 2 particle ids"}}, be_values(ij.data(), 2 * m));
+        add("springsstiffness", {D_SNB}, NC_FLOAT, {{"long_name", "Spring stiffness"}}, be_values(t.k.data(), m));
+        add("springsequilibrium", {D_SNB}, NC_FLOAT, {{"long_name", "Spring distance equilibrium"}},
+            be_values(t.l0.data(), m));
+        add("nbspringsperparticle", {D_PNB}, NC_INT, {{"long_name", "Number of springs per particle"}},
+            be_values(per.data(), n));
+    }
+    // header with placeholder offsets, then the data
+    auto header = [&](const std::vector<uint64_t> &begins) {
+        std::vector<uint8_t> h = {'C', 'D', 'F', 2};
+        be32(h, 0); // numrecs: no record dimension
+        be32(h, 0x0A);
+        be32(h, (uint32_t)dims.size());
+        for (const auto &d : dims)
+        {
+            nc_name(h, d.first);
+            be32(h, (uint32_t)d.second);
+        }
+        be32(h, 0); // no global attributes
+        be32(h, 0);
+        be32(h, 0x0B);
+        be32(h, (uint32_t)vars.size());
+        for (size_t v = 0; v < vars.size(); ++v)
+        {
+            const OutVar &x = vars[v];
+            nc_name(h, x.name);
+            be32(h, (uint32_t)x.dims.size());
+            for (int d : x.dims)
+                be32(h, (uint32_t)d);
+            be32(h, 0x0C);
+            be32(h, (uint32_t)x.atts.size());
+            for (const auto &a : x.atts)
+            {
+                nc_name(h, a.first);
+                be32(h, NC_CHAR);
+                be32(h, (uint32_t)a.second.size());
+                h.insert(h.end(), a.second.begin(), a.second.end());
+                pad4(h);
+            }
+            be32(h, (uint32_t)x.type);
+            be32(h, (uint32_t)std::min<uint64_t>((x.data.size() + 3) / 4 * 4, 0xffffffffu)); // vsize
+            be64(h, begins[v]);
+        }
+        return h;
+    };
+    std::vector<uint64_t> begins(vars.size(), 0);
+    uint64_t at = header(begins).size();
+    for (size_t v = 0; v < vars.size(); ++v)
+    {
+        begins[v] = at;
+        at += (vars[v].data.size() + 3) / 4 * 4;
+    }
+    const std::vector<uint8_t> h = header(begins);
+    FILE *f = std::fopen(path.c_str(), "wb");
+    if (!f)
+    {
+        err = "cannot open output file '" + path + "': " + std::strerror(errno);
+        return false;
+    }
+    bool ok = std::fwrite(h.data(), 1, h.size(), f) == h.size();
+    static const uint8_t zeros[4] = {0, 0, 0, 0};
+    for (const OutVar &x : vars)
+    {
+        ok = ok && std::fwrite(x.data.data(), 1, x.data.size(), f) == x.data.size();
+        const size_t padn = (4 - x.data.size() % 4) % 4;
+        ok = ok && std::fwrite(zeros, 1, padn, f) == padn;
+    }
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok)
+        err = "cannot write " + path;
+    return ok;
+}
 
 bool read_netcdf_topology(const std::string &path, NcTopology &t, std::string &err)
 {

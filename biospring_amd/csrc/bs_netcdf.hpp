@@ -32,4 +32,10 @@ struct NcTopology
 // where it has one).
 bool read_netcdf_topology(const std::string &path, NcTopology &out, std::string &err);
 
+// Writes the dimensions, variables and attributes of NetCDFWriter::writeBinary
+// (src/IO/NetCDFWriter.cpp:25-150) in the 64-bit-offset classic layout:
+// positions, ids (the particle index), properties, dynamic state, names cut
+// to their buffer lengths, springs and the springs per particle.
+bool write_netcdf_topology(const std::string &path, const NcTopology &t, std::string &err);
+
 } // namespace bs

@@ -98,6 +98,10 @@ class SpringNetwork:
         self.n = int(self._L.bs_num_particles(self._h))
         return [w for w in buf.value.decode().split("\n") if w]
 
+    def saveNetCDF(self, path):
+        """NetCDFWriter::writeBinary of the loaded network at its current positions."""
+        check(self._L.bs_save_netcdf(self._h, os.fsencode(path)), self._h, "bs_save_netcdf")
+
     def getParticleProperties(self):
         """dict of mass, charge, radius, epsilon, sasa, transfer, hydrophobicity, flags."""
         n = int(self._L.bs_num_particles(self._h))

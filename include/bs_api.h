@@ -397,6 +397,15 @@ int bs_set_membrane(bs_handle *h, float upper_offset, float lower_offset, float 
  * warnings (nullable, warnings_len bytes): the reader's warning lines. */
 int bs_load_netcdf(bs_handle *h, const char *path, char *warnings, int64_t warnings_len);
 
+/* Replaces NetCDFWriter::writeBinary (src/IO/NetCDFWriter.cpp:25-150) of
+ * the loaded network: the reference's dimensions, variables and attributes
+ * (coordinates at the current positions, particleids = the particle index,
+ * the properties, dynamicstate, the PDB labels cut to 5/4/4 characters,
+ * springs, springsstiffness, springsequilibrium, nbspringsperparticle),
+ * written in the 64-bit-offset classic layout, which bs_load_netcdf and the
+ * netCDF library read.  A file that cannot be written -> BS_ERR_ARG. */
+int bs_save_netcdf(bs_handle *h, const char *path);
+
 /* The per-particle properties as loaded (bs_set_particles order and
  * defaults); every pointer nullable, bs_num_particles entries each. */
 int bs_get_particle_properties(const bs_handle *h, float *mass, float *charge, float *radius, float *epsilon,

@@ -271,3 +271,40 @@ def test_reference_example_file(lib, tmp_path):
     check_loaded(net, expected(path))
     assert net.getNumberOfParticles() == 867 and net.getNumberOfSprings() == 3588
     net.close()
+
+
+def test_save_writes_the_reference_writer_layout(lib, tmp_path):
+    """bs_save_netcdf (NetCDFWriter::writeBinary): scipy reads back the
+    reference writer's dimensions, variables and attributes with the loaded
+    values, and bs_load_netcdf restores the same network."""
+    src = str(tmp_path / "in.nc")
+    write(src, *topology(seed=3))
+    net, _ = load(lib, src)
+    out = str(tmp_path / "out.nc")
+    net.saveNetCDF(out)
+    f = scipy_io.netcdf_file(out, "r", mmap=False)
+    assert f.version_byte == 2
+    assert dict(f.dimensions) == {"spatialdim": 3, "particle_number": 60, "particlename_length": 5,
+                                  "chainname_length": 4, "resname_length": 4, "springdim": 2, "spring_number": 90}
+    V = f.variables
+    assert list(V) == ["coordinates", "particleids", "charges", "radii", "epsilon", "mass", "surfaceaccessibility",
+                       "hydrophobicityscale", "hydrophobicity", "resids", "dynamicstate", "particlenames", "resnames",
+                       "chainnames", "springs", "springsstiffness", "springsequilibrium", "nbspringsperparticle"]
+    assert V["coordinates"].units == b"angstrom" and V["mass"].long_name == b"Particle mass"
+    assert np.array_equal(V["particleids"].data, np.arange(60))
+    e = expected(src)
+    assert np.array_equal(V["coordinates"].data, e["x"])
+    assert np.array_equal(V["mass"].data, e["mass"])  # the zero masses already became 1
+    assert np.array_equal(V["dynamicstate"].data == 0, e["static"])
+    assert np.array_equal(V["springs"].data, e["springs"])
+    per = np.bincount(e["springs"].reshape(-1), minlength=60)
+    assert np.array_equal(V["nbspringsperparticle"].data, per)
+    g = scipy_io.netcdf_file(src, "r", mmap=False)
+    for k in ("particlenames", "resnames", "chainnames", "resids"):
+        assert np.array_equal(V[k].data, g.variables[k].data), k
+    g.close()
+    f.close()
+    net.close()
+    back, _ = load(lib, out)
+    check_loaded(back, expected(out))
+    back.close()
