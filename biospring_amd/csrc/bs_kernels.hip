@@ -377,28 +377,48 @@ __global__ void __launch_bounds__(BLOCK) k_spring(SpringArgs a)
         const int32_t beg = a.sp_ptr[row], end = a.sp_ptr[row + 1];
         const double cs = a.k.c_spring * (double)a.spring_scale;
         constexpr int U = 4;
+        // the edge words of the next iteration load while this one gathers
+        // and computes (two dependent loads per edge: the word, then x_j)
+        int32_t jr[U];
+        float kk[U], ll[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+        {
+            const int32_t e = beg + lane + u * G;
+            const bool ok = e < end;
+            jr[u] = ok ? a.sp_j[e] : 0;
+            kk[u] = ok ? a.sp_k[e] : 0.0f;
+            ll[u] = ok ? a.sp_l0[e] : 0.0f;
+        }
         for (int32_t e0 = beg + lane; e0 < end; e0 += U * G)
         {
-            int32_t jr[U];
-            float kk[U], ll[U];
             float4 pj[U];
 #pragma unroll
             for (int u = 0; u < U; ++u)
-            {
-                const int32_t e = e0 + u * G;
-                const bool ok = e < end;
-                jr[u] = ok ? a.sp_j[e] : 0;
-                kk[u] = ok ? a.sp_k[e] : 0.0f;
-                ll[u] = ok ? a.sp_l0[e] : 0.0f;
-            }
+                pj[u] = a.pos[jr[u] & 0x7fffffff];
+            int32_t jn[U];
+            float kn[U], ln[U];
 #pragma unroll
             for (int u = 0; u < U; ++u)
-                pj[u] = a.pos[jr[u] & 0x7fffffff];
+            {
+                const int32_t e = e0 + (U + u) * G;
+                const bool ok = e < end;
+                jn[u] = ok ? a.sp_j[e] : 0;
+                kn[u] = ok ? a.sp_k[e] : 0.0f;
+                ln[u] = ok ? a.sp_l0[e] : 0.0f;
+            }
 #pragma unroll
             for (int u = 0; u < U; ++u)
                 if (e0 + u * G < end)
                     spring_edge<COMP>(pi, li, pj[u], a.plo, jr[u] & 0x7fffffff, kk[u], ll[u], jr[u] < 0, cs,
                                       a.spring_scale, a.energy != 0, fx, fy, fz, espr);
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+            {
+                jr[u] = jn[u];
+                kk[u] = kn[u];
+                ll[u] = ln[u];
+            }
         }
     }
     // reduce the force (and the spring energy) over the G lanes of the row
