@@ -856,25 +856,38 @@ __global__ void __launch_bounds__(BLOCK) k_adj_spring(AdjSpringArgs a)
         const double C = a.k.c_spring * (double)a.spring_scale;
         const int32_t beg = a.sp_ptr[row], end = a.sp_ptr[row + 1];
         constexpr int U = 4; // edges in flight per lane
+        // the edge words of the next iteration load while this one gathers
+        // and computes
+        int32_t jr[U];
+        float kk[U], ll[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+        {
+            const int32_t e = beg + lane + u * G;
+            const bool ok = e < end;
+            jr[u] = ok ? a.sp_j[e] : row;
+            kk[u] = ok ? a.sp_k[e] : 0.0f;
+            ll[u] = ok ? a.sp_l0[e] : 0.0f;
+        }
         for (int32_t e0 = beg + lane; e0 < end; e0 += U * G)
         {
-            int32_t jr[U];
-            float kk[U], ll[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u)
-            {
-                const int32_t e = e0 + u * G;
-                const bool ok = e < end;
-                jr[u] = ok ? a.sp_j[e] : row;
-                kk[u] = ok ? a.sp_k[e] : 0.0f;
-                ll[u] = ok ? a.sp_l0[e] : 0.0f;
-            }
+            int32_t jn[U];
+            float kn[U], ln[U];
             float4 pj[U], wj[U];
 #pragma unroll
             for (int u = 0; u < U; ++u)
             {
                 pj[u] = a.pos[jr[u] & 0x7fffffff];
                 wj[u] = a.w[jr[u] & 0x7fffffff];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+            {
+                const int32_t e = e0 + (U + u) * G;
+                const bool ok = e < end;
+                jn[u] = ok ? a.sp_j[e] : row;
+                kn[u] = ok ? a.sp_k[e] : 0.0f;
+                ln[u] = ok ? a.sp_l0[e] : 0.0f;
             }
 #pragma unroll
             for (int u = 0; u < U; ++u)
@@ -901,6 +914,13 @@ __global__ void __launch_bounds__(BLOCK) k_adj_spring(AdjSpringArgs a)
                     a.dk[e] -= (float)(wu * (r - l0));
                     a.dl0[e] += (float)(wu * k);
                 }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+            {
+                jr[u] = jn[u];
+                kk[u] = kn[u];
+                ll[u] = ln[u];
             }
         }
     }
