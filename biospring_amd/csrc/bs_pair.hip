@@ -275,26 +275,80 @@ __global__ void __launch_bounds__(BLOCK) k_cluster_list(ListArgs a)
 // half inside the cutoff at best, so branching would not skip work; it only
 // costs exec-mask bookkeeping and keeps loads from being issued early.  The
 // exact two-stage test runs for pairs in the guard band around rc^2 only.
+//
+// The kernel is bound by VALU issue, so the per-pair instruction count is
+// what is minimised: the force prefactors carry C_spring, the term scales
+// and i's own factor (PairI, formed once per lane); the pair predicates gate
+// FP32 values (one select each) instead of FP64 results; a charge that is
+// zero, or a pair gated off, zeroes the Coulomb term through q_j.
 // ---------------------------------------------------------------------------
+struct PairI
+{
+    float ks;  // steric force: C_spring x scale x {12 eps_i', 24 eps_i', eps_i} (Amber, Lewitt, Zacharias), C_spring x scale (linear)
+    float kh;  // hydrophobic force: scale x C_spring x h_i
+    double qe; // Coulomb energy: q_i ke
+    double qf; // Coulomb force: q_i kf, doubled for eps(r) = eps r
+};
+
+// An FP32 value kept as computed: a select of it stays one FP32 select
+// instead of being sunk past a later FP64 conversion (two selects).
+__device__ inline float keep(float x)
+{
+    asm("" : "+v"(x));
+    return x;
+}
+
+template <int SMODE, int EMODE>
+__device__ inline PairI pair_i(float4 pi, float4 ri, const PairArgs &a)
+{
+    PairI c;
+    const float cs = (float)a.k.c_spring;
+    const float m = SMODE == 1 ? 12.0f : SMODE == 2 ? 24.0f : 1.0f;
+    c.ks = SMODE == 0 ? cs * a.steric_scale : m * cs * a.steric_scale * ri.y;
+    c.kh = a.hydro_scale * cs * ri.z;
+    c.qe = (double)pi.w * a.ke;
+    c.qf = (double)pi.w * a.kf * (EMODE == 2 ? 2.0 : 1.0);
+    return c;
+}
+
 template <int SMODE, int EMODE, int HYDRO, bool ENERGY>
-__device__ inline void interact(bool masked, float4 pi, float4 ri, uint32_t fli, float4 pj, float4 rj,
-                                const PairArgs &a, float cs, double qe, double qf, double &fx, double &fy,
-                                double &fz, double &es, double &ee, double &eh)
+__device__ inline void interact(bool masked, float4 pi, float4 ri, float4 pj, float4 rj, const PairArgs &a,
+                                const PairI &c, double &fx, double &fy, double &fz, double &es, double &ee,
+                                double &eh)
 {
     const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
     const float r2 = dx * dx + dy * dy + dz * dz;
-    bool on = masked && r2 < a.rc2lo;
-    if (masked && r2 >= a.rc2lo && r2 < a.rc2hi) // rare: decide exactly
+    const bool lo = r2 < a.rc2lo;
+    bool on = masked && lo;
+    if (masked && !lo && r2 < a.rc2hi) // rare: decide exactly
         on = exact_two_stage(pi, pj, a.rc);
-    const uint32_t flj = fbits(rj.w);
-    const float wgt = (flj & PF_STATIC) ? 1.0f : 0.5f;
     const bool pos2 = r2 > 0.0f;
     const float r2s = pos2 ? r2 : 1.0f;
-    float inv = __builtin_amdgcn_rsqf(r2s);
-    inv = inv * (1.5f - 0.5f * r2s * inv * inv);
+    float inv;
+    double invd = 0.0;
+    if (EMODE > 0)
+    {
+        // 1/d to ~1e-13: the FP32 seed (its input kept at or above the
+        // smallest normal float, so it is finite) and one FP64 Newton step on
+        // the exact r^2 of the float components.  The FP32 1/d of the other
+        // terms is its rounding.
+        const double xd = dx, yd = dy, zd = dz;
+        const double r2d = fma(xd, xd, fma(yd, yd, zd * zd));
+        // r2 >= 0, so the larger bit pattern is the larger value (one integer max)
+        const float seed = __int_as_float(max(__float_as_int(r2), 0x00800000));
+        const double y = (double)__builtin_amdgcn_rsqf(seed);
+        invd = y * fma(-0.5 * r2d, y * y, 1.5);
+        inv = (float)invd;
+    }
+    else
+    {
+        inv = __builtin_amdgcn_rsqf(r2s);
+        inv = inv * (1.5f - 0.5f * r2s * inv * inv);
+    }
+    const uint32_t flj = fbits(rj.w);
+    const float wgt = (flj & PF_STATIC) ? 1.0f : 0.5f;
     const float inv2 = inv * inv;
-    float fscal = 0.0f; // force on i = (dx, dy, dz) * (fscal + fscd)
-    double fscd = 0.0;
+    float fs = 0.0f; // FP32 terms of the force scalar: force on i = (dx, dy, dz) * (fs + fsd)
     if (SMODE >= 0)
     {
         float en = 0.0f, f = 0.0f;
@@ -305,83 +359,82 @@ __device__ inline void interact(bool masked, float4 pi, float4 ri, uint32_t fli,
             const float dv = pos2 ? d - sr : -sr;
             const bool ov = !(dv > 0.0f);
             en = ov ? 0.5f * dv * dv : 0.0f;
-            f = (ov && pos2) ? dv * cs * inv : 0.0f; // -|dv| * C / d
+            f = (on && ov && pos2) ? dv * c.ks * inv : 0.0f; // -|dv| * C / d
         }
         else
         {
-            const float eij = ri.y * rj.y;
             const float R = ri.x * rj.x;
             const float s2 = R * R * inv2, s6 = s2 * s2 * s2;
-            float e0, f0;
+            float g, e0 = 0.0f;
             if (SMODE == 1)
-            { // Amber 12-6
-                const float s12 = s6 * s6;
-                e0 = eij * (s12 - 2.0f * s6);
-                f0 = 12.0f * eij * (s6 - s12) * inv2 * cs;
+            { // Amber 12-6: F = 12 eps (s6 - s12) / r^2, E = eps (s12 - 2 s6)
+                g = fmaf(-s6, s6, s6);
+                if (ENERGY)
+                    e0 = ri.y * rj.y * (s6 * s6 - 2.0f * s6);
             }
             else if (SMODE == 2)
-            { // Lewitt 8-6
+            { // Lewitt 8-6: F = 24 eps (s6 - s8) / r^2, E = eps (3 s8 - 4 s6)
                 const float s8 = s6 * s2;
-                e0 = eij * (3.0f * s8 - 4.0f * s6);
-                f0 = 24.0f * eij * (s6 - s8) * inv2 * cs;
+                g = s6 - s8;
+                if (ENERGY)
+                    e0 = ri.y * rj.y * (3.0f * s8 - 4.0f * s6);
             }
             else
             { // Zacharias 8-6: raw r, eps, product rules
                 const float s8 = s6 * s2;
-                e0 = eij * (s8 - s6);
-                f0 = eij * (6.0f * s6 - 8.0f * s8) * inv2 * cs;
+                g = 6.0f * s6 - 8.0f * s8;
+                if (ENERGY)
+                    e0 = ri.y * rj.y * (s8 - s6);
             }
             const bool far = r2 >= 0.01f; // 0.1 A floor (steric.hpp:62)
             en = far ? e0 : 0.0f;
-            f = far ? f0 : 0.0f;
+            f = (on && far) ? c.ks * rj.y * g * inv2 : 0.0f;
         }
         if (ENERGY)
             es += on ? (double)(wgt * a.steric_scale * en) : 0.0;
-        fscal += a.steric_scale * f;
+        fs += f;
     }
+    double fsd = 0.0;
     if (EMODE > 0)
     {
-        // Coulomb dominates the non-bonded force: its scalar is formed in
-        // FP64 from 1/d refined by one FP64 Newton step of the FP32 seed
-        const bool qon = (fli & PF_CHARGED) && (flj & PF_CHARGED) && r2 >= 1.0e-6f;
-        const double xd = dx, yd = dy, zd = dz;
-        const double r2d = qon ? fma(xd, xd, fma(yd, yd, zd * zd)) : 1.0;
-        const double y = inv;
-        const double invd = y * fma(-0.5 * r2d, y * y, 1.5);
-        const double qj = qon ? (double)pj.w : 0.0;
+        // Coulomb dominates the non-bonded force: its scalar is formed in FP64
+        const double qj = (double)keep((on && r2 >= 1.0e-6f) ? pj.w : 0.0f);
+        const double i2 = invd * invd;
         if (EMODE == 1)
         {
             if (ENERGY)
-                ee += on ? (double)wgt * qe * qj * invd : 0.0;
-            fscd = -(qf * qj * invd * invd * invd);
+                ee += (double)wgt * c.qe * qj * invd;
+            fsd = -(c.qf * qj) * (i2 * invd);
         }
         else
         { // eps(r) = eps * r
-            const double i2 = invd * invd;
             if (ENERGY)
-                ee += on ? (double)wgt * qe * qj * i2 : 0.0;
-            fscd = -(2.0 * qf * qj * i2 * i2);
+                ee += (double)wgt * c.qe * qj * i2;
+            fsd = -(c.qf * qj) * (i2 * i2);
         }
     }
     if (HYDRO)
     {
-        const bool hon = (fli & PF_HYDRO) && (flj & PF_HYDRO) && pos2;
-        const float d = r2s * inv;
-        const float hh = hon ? ri.z * rj.z * __expf(-d) : 0.0f;
+        // h_i h_j vanishes unless both are hydrophobic
+        const float e = __expf(-(r2s * inv));
         if (ENERGY)
+        {
+            const float hh = pos2 ? ri.z * rj.z * e : 0.0f;
             eh += on ? (double)(wgt * a.hydro_scale) * (-(double)hh * 6.02214169E23 * 1.0E-3) : 0.0;
-        fscal += a.hydro_scale * hh * cs * inv;
+        }
+        fs += (on && pos2) ? c.kh * rj.z * e * inv : 0.0f;
     }
-    const double fs = on ? (double)fscal + fscd : 0.0;
-    fx = fma((double)dx, fs, fx);
-    fy = fma((double)dy, fs, fy);
-    fz = fma((double)dz, fs, fz);
+    const double s = EMODE > 0 ? fsd + (double)keep(fs) : (double)fs;
+    fx = fma((double)dx, s, fx);
+    fy = fma((double)dy, s, fy);
+    fz = fma((double)dz, s, fz);
 }
 
-// Pair forces of one i-cluster per wave.  Two list words per lane per
-// iteration; while iteration t computes, the list words of iteration t+2 and
-// the j data of iteration t+1 are in flight (zero-padded lists: a zero word
-// is a masked-out entry pointing at particle 0).
+// Pair forces of one i-cluster per wave.  A block is 16 list words, two per
+// lane.  Blocks alternate between two register sets: while block t is
+// evaluated, the j data of block t+1 and the words of block t+2 are in
+// flight, and no register moves rotate the pipeline (zero-padded lists: a
+// zero word is a masked-out entry pointing at particle 0).
 template <int SMODE, int EMODE, int HYDRO, bool ENERGY>
 __global__ void __launch_bounds__(BLOCK) k_pair(PairArgs a)
 {
@@ -391,53 +444,58 @@ __global__ void __launch_bounds__(BLOCK) k_pair(PairArgs a)
     const int32_t i = ci * CLUSTER + il;
     double es = 0.0, ee = 0.0, eh = 0.0;
     double fx = 0.0, fy = 0.0, fz = 0.0;
-    const float cs = (float)a.k.c_spring;
     if (ci < a.ncl)
     {
         const bool iv = i < a.n;
         const float4 pi = iv ? a.pos[i] : make_float4(0.f, 0.f, 0.f, 0.f);
         const float4 ri = iv ? a.jprm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-        const uint32_t fli = fbits(ri.w);
-        const double qe = (double)pi.w * a.ke; // Coulomb prefactors with q_i folded in
-        const double qf = (double)pi.w * a.kf;
+        const PairI c = pair_i<SMODE, EMODE>(pi, ri, a);
         const uint32_t ibit = 1u << (24 + il);
         const int32_t cnt16 = (a.cln[ci] + 15) & ~15;
         const uint32_t *L = a.cl + (size_t)ci * a.cap;
-        // words of iteration t (w), t+1 (n); j data of iteration t (pj, rj).
         // Look-ahead reads past the end are clamped to the list's last 16
         // words (valid entries, never computed with), so every load is
         // unconditional and can be issued ahead of its use.
         const int32_t last = cnt16 > 16 ? cnt16 - 16 : 0;
-        uint32_t w0 = 0u, w1 = 0u, n0 = 0u, n1 = 0u;
+        uint32_t wa0 = 0u, wa1 = 0u, wb0 = 0u, wb1 = 0u; // words of the even (a) and odd (b) blocks
         if (cnt16 > 0)
         {
             const int32_t o = min(16, last);
-            w0 = L[jl];
-            w1 = L[8 + jl];
-            n0 = L[o + jl];
-            n1 = L[o + 8 + jl];
+            wa0 = L[jl];
+            wa1 = L[8 + jl];
+            wb0 = L[o + jl];
+            wb1 = L[o + 8 + jl];
         }
-        float4 pj0 = a.pos[w0 & CL_JMASK], rj0 = a.jprm[w0 & CL_JMASK];
-        float4 pj1 = a.pos[w1 & CL_JMASK], rj1 = a.jprm[w1 & CL_JMASK];
-        for (int32_t t = 0; t < cnt16; t += 16)
+        float4 pa0 = a.pos[wa0 & CL_JMASK], ra0 = a.jprm[wa0 & CL_JMASK];
+        float4 pa1 = a.pos[wa1 & CL_JMASK], ra1 = a.jprm[wa1 & CL_JMASK];
+        for (int32_t t = 0; t < cnt16; t += 32)
         {
-            const int32_t o = min(t + 32, last); // words of iteration t+2
-            const uint32_t m0 = L[o + jl];
-            const uint32_t m1 = L[o + 8 + jl];
-            const float4 qj0 = a.pos[n0 & CL_JMASK], sj0 = a.jprm[n0 & CL_JMASK];
-            const float4 qj1 = a.pos[n1 & CL_JMASK], sj1 = a.jprm[n1 & CL_JMASK];
-            interact<SMODE, EMODE, HYDRO, ENERGY>((w0 & ibit) != 0u, pi, ri, fli, pj0, rj0, a, cs, qe, qf, fx, fy,
-                                                  fz, es, ee, eh);
-            interact<SMODE, EMODE, HYDRO, ENERGY>((w1 & ibit) != 0u, pi, ri, fli, pj1, rj1, a, cs, qe, qf, fx, fy,
-                                                  fz, es, ee, eh);
-            w0 = n0;
-            w1 = n1;
-            n0 = m0;
-            n1 = m1;
-            pj0 = qj0;
-            rj0 = sj0;
-            pj1 = qj1;
-            rj1 = sj1;
+            // block t (set a); in flight: words of block t+2, j data of block t+1
+            const bool ma0 = (wa0 & ibit) != 0u, ma1 = (wa1 & ibit) != 0u;
+            {
+                const int32_t o = min(t + 32, last);
+                wa0 = L[o + jl];
+                wa1 = L[o + 8 + jl];
+            }
+            const float4 pb0 = a.pos[wb0 & CL_JMASK], rb0 = a.jprm[wb0 & CL_JMASK];
+            const float4 pb1 = a.pos[wb1 & CL_JMASK], rb1 = a.jprm[wb1 & CL_JMASK];
+            interact<SMODE, EMODE, HYDRO, ENERGY>(ma0, pi, ri, pa0, ra0, a, c, fx, fy, fz, es, ee, eh);
+            interact<SMODE, EMODE, HYDRO, ENERGY>(ma1, pi, ri, pa1, ra1, a, c, fx, fy, fz, es, ee, eh);
+            if (t + 16 >= cnt16)
+                break; // wave-uniform
+            // block t+1 (set b)
+            const bool mb0 = (wb0 & ibit) != 0u, mb1 = (wb1 & ibit) != 0u;
+            {
+                const int32_t o = min(t + 48, last);
+                wb0 = L[o + jl];
+                wb1 = L[o + 8 + jl];
+            }
+            pa0 = a.pos[wa0 & CL_JMASK];
+            ra0 = a.jprm[wa0 & CL_JMASK];
+            pa1 = a.pos[wa1 & CL_JMASK];
+            ra1 = a.jprm[wa1 & CL_JMASK];
+            interact<SMODE, EMODE, HYDRO, ENERGY>(mb0, pi, ri, pb0, rb0, a, c, fx, fy, fz, es, ee, eh);
+            interact<SMODE, EMODE, HYDRO, ENERGY>(mb1, pi, ri, pb1, rb1, a, c, fx, fy, fz, es, ee, eh);
         }
     }
     // sum over the 8 entry lanes of each i (lanes il, il+8, ..., il+56)
@@ -758,10 +816,7 @@ __global__ void __launch_bounds__(BLOCK) k_pair_q(PairArgs a)
     const bool iv = i < a.n;
     const float4 pi = iv ? a.pos[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     const float4 ri = iv ? a.jprm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-    const uint32_t fli = fbits(ri.w);
-    const double qe = (double)pi.w * a.ke;
-    const double qf = (double)pi.w * a.kf;
-    const float cs = (float)a.k.c_spring;
+    const PairI c = pair_i<SMODE, EMODE>(pi, ri, a);
     const uint64_t below = (1ull << lane) - 1ull;
     double es = 0.0, ee = 0.0, eh = 0.0;
     double fx = 0.0, fy = 0.0, fz = 0.0;
@@ -799,7 +854,7 @@ __global__ void __launch_bounds__(BLOCK) k_pair_q(PairArgs a)
             const bool valid = e < mine;
             const int32_t src = valid ? qidx[wv][il][e] : 0;
             const float4 qj = qpos[wv][src], sj = qprm[wv][src];
-            interact<SMODE, EMODE, HYDRO, ENERGY>(valid, pi, ri, fli, qj, sj, a, cs, qe, qf, fx, fy, fz, es, ee, eh);
+            interact<SMODE, EMODE, HYDRO, ENERGY>(valid, pi, ri, qj, sj, a, c, fx, fy, fz, es, ee, eh);
         }
         wave_lds_sync();
     }
@@ -912,10 +967,7 @@ __global__ void __launch_bounds__(BLOCK) k_pair_rows(PairArgs a)
     const int32_t self = iv ? i : 0;
     const float4 pi = a.pos[self];
     const float4 ri = a.jprm[self];
-    const uint32_t fli = fbits(ri.w);
-    const double qe = (double)pi.w * a.ke;
-    const double qf = (double)pi.w * a.kf;
-    const float cs = (float)a.k.c_spring;
+    const PairI c = pair_i<SMODE, EMODE>(pi, ri, a);
     const int32_t len = iv ? a.rlen[i] : 0;
     int32_t mx = len;
     for (int off = 32; off > 0; off >>= 1)
@@ -937,10 +989,8 @@ __global__ void __launch_bounds__(BLOCK) k_pair_rows(PairArgs a)
             const int32_t q0 = jof(n0, e + 2), q1 = jof(n1, e + 3);
             const float4 qj0 = a.pos[q0], sj0 = a.jprm[q0];
             const float4 qj1 = a.pos[q1], sj1 = a.jprm[q1];
-            interact<SMODE, EMODE, HYDRO, ENERGY>(e < len, pi, ri, fli, pj0, rj0, a, cs, qe, qf, fx, fy, fz, es, ee,
-                                                  eh);
-            interact<SMODE, EMODE, HYDRO, ENERGY>(e + 1 < len, pi, ri, fli, pj1, rj1, a, cs, qe, qf, fx, fy, fz, es,
-                                                  ee, eh);
+            interact<SMODE, EMODE, HYDRO, ENERGY>(e < len, pi, ri, pj0, rj0, a, c, fx, fy, fz, es, ee, eh);
+            interact<SMODE, EMODE, HYDRO, ENERGY>(e + 1 < len, pi, ri, pj1, rj1, a, c, fx, fy, fz, es, ee, eh);
             n0 = m0;
             n1 = m1;
             pj0 = qj0;
