@@ -324,30 +324,44 @@ __device__ inline void interact(bool masked, float4 pi, float4 ri, float4 pj, fl
         on = exact_two_stage(pi, pj, a.rc);
     const bool pos2 = r2 > 0.0f;
     const float r2s = pos2 ? r2 : 1.0f;
-    float inv;
-    double invd = 0.0;
+    // the distance itself is needed by the vacuum Coulomb term, the linear
+    // steric wall and the hydrophobic term; the others need 1/r^2 only
+    constexpr bool NEED_D = EMODE == 1 || SMODE == 0 || HYDRO != 0;
+    float inv = 0.0f, inv2;
+    double invd = 0.0, i2d = 0.0;
     if (EMODE > 0)
     {
-        // 1/d to ~1e-13: the FP32 seed (its input kept at or above the
-        // smallest normal float, so it is finite) and one FP64 Newton step on
-        // the exact r^2 of the float components.  The FP32 1/d of the other
-        // terms is its rounding.
+        // 1/d or 1/r^2 to ~1e-13: the FP32 seed (its input kept at or above
+        // the smallest normal float, so it is finite) and one FP64 Newton
+        // step on the exact r^2 of the float components.  The FP32 values of
+        // the other terms are its rounding.  r2 >= 0, so the larger bit
+        // pattern is the larger value (one integer max).
         const double xd = dx, yd = dy, zd = dz;
         const double r2d = fma(xd, xd, fma(yd, yd, zd * zd));
-        // r2 >= 0, so the larger bit pattern is the larger value (one integer max)
         const float seed = __int_as_float(max(__float_as_int(r2), 0x00800000));
-        const double y = (double)__builtin_amdgcn_rsqf(seed);
-        invd = y * fma(-0.5 * r2d, y * y, 1.5);
-        inv = (float)invd;
+        if (NEED_D)
+        {
+            const double y = (double)__builtin_amdgcn_rsqf(seed);
+            invd = y * fma(-0.5 * r2d, y * y, 1.5);
+            i2d = invd * invd;
+            inv = (float)invd;
+            inv2 = inv * inv;
+        }
+        else
+        {
+            const double y = (double)__builtin_amdgcn_rcpf(seed);
+            i2d = y * fma(-r2d, y, 2.0);
+            inv2 = (float)i2d;
+        }
     }
     else
     {
         inv = __builtin_amdgcn_rsqf(r2s);
         inv = inv * (1.5f - 0.5f * r2s * inv * inv);
+        inv2 = inv * inv;
     }
     const uint32_t flj = fbits(rj.w);
     const float wgt = (flj & PF_STATIC) ? 1.0f : 0.5f;
-    const float inv2 = inv * inv;
     float fs = 0.0f; // FP32 terms of the force scalar: force on i = (dx, dy, dz) * (fs + fsd)
     if (SMODE >= 0)
     {
@@ -399,18 +413,17 @@ __device__ inline void interact(bool masked, float4 pi, float4 ri, float4 pj, fl
     {
         // Coulomb dominates the non-bonded force: its scalar is formed in FP64
         const double qj = (double)keep((on && r2 >= 1.0e-6f) ? pj.w : 0.0f);
-        const double i2 = invd * invd;
         if (EMODE == 1)
         {
             if (ENERGY)
                 ee += (double)wgt * c.qe * qj * invd;
-            fsd = -(c.qf * qj) * (i2 * invd);
+            fsd = -(c.qf * qj) * (i2d * invd);
         }
         else
         { // eps(r) = eps * r
             if (ENERGY)
-                ee += (double)wgt * c.qe * qj * i2;
-            fsd = -(c.qf * qj) * (i2 * i2);
+                ee += (double)wgt * c.qe * qj * i2d;
+            fsd = -(c.qf * qj) * (i2d * i2d);
         }
     }
     if (HYDRO)
