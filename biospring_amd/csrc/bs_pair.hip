@@ -452,7 +452,9 @@ template <int SMODE, int EMODE, int HYDRO, bool ENERGY>
 __global__ void __launch_bounds__(BLOCK) k_pair(PairArgs a)
 {
     const int lane = threadIdx.x & 63;
-    const int32_t ci = blockIdx.x * WAVES + (threadIdx.x >> 6);
+    // wave-uniform, made visible to the compiler so that the list walk runs
+    // on scalar registers
+    const int32_t ci = __builtin_amdgcn_readfirstlane(blockIdx.x * WAVES + (threadIdx.x >> 6));
     const int32_t il = lane & 7, jl = lane >> 3;
     const int32_t i = ci * CLUSTER + il;
     double es = 0.0, ee = 0.0, eh = 0.0;
@@ -486,9 +488,9 @@ __global__ void __launch_bounds__(BLOCK) k_pair(PairArgs a)
             // block t (set a); in flight: words of block t+2, j data of block t+1
             const bool ma0 = (wa0 & ibit) != 0u, ma1 = (wa1 & ibit) != 0u;
             {
-                const int32_t o = min(t + 32, last);
-                wa0 = L[o + jl];
-                wa1 = L[o + 8 + jl];
+                const uint32_t *Lw = L + min(t + 32, last); // wave-uniform base, lane offset jl
+                wa0 = Lw[jl];
+                wa1 = Lw[8 + jl];
             }
             const float4 pb0 = a.pos[wb0 & CL_JMASK], rb0 = a.jprm[wb0 & CL_JMASK];
             const float4 pb1 = a.pos[wb1 & CL_JMASK], rb1 = a.jprm[wb1 & CL_JMASK];
@@ -499,9 +501,9 @@ __global__ void __launch_bounds__(BLOCK) k_pair(PairArgs a)
             // block t+1 (set b)
             const bool mb0 = (wb0 & ibit) != 0u, mb1 = (wb1 & ibit) != 0u;
             {
-                const int32_t o = min(t + 48, last);
-                wb0 = L[o + jl];
-                wb1 = L[o + 8 + jl];
+                const uint32_t *Lw = L + min(t + 48, last);
+                wb0 = Lw[jl];
+                wb1 = Lw[8 + jl];
             }
             pa0 = a.pos[wa0 & CL_JMASK];
             ra0 = a.jprm[wa0 & CL_JMASK];
