@@ -646,7 +646,7 @@ template <int SMODE, int EMODE>
 __global__ void __launch_bounds__(BLOCK) k_adj_pair(AdjPairArgs a)
 {
     const int lane = threadIdx.x & 63;
-    const int32_t ci = blockIdx.x * WAVES + (threadIdx.x >> 6);
+    const int32_t ci = __builtin_amdgcn_readfirstlane(blockIdx.x * WAVES + (threadIdx.x >> 6)); // wave-uniform
     const int32_t il = lane & 7, jl = lane >> 3;
     const int32_t i = ci * CLUSTER + il;
     double hx = 0.0, hy = 0.0, hz = 0.0, dqi = 0.0;
@@ -663,42 +663,50 @@ __global__ void __launch_bounds__(BLOCK) k_adj_pair(AdjPairArgs a)
         const int32_t cnt16 = (a.cln[ci] + 15) & ~15;
         const uint32_t *L = a.cl + (size_t)ci * a.cap;
         const int32_t last = cnt16 > 16 ? cnt16 - 16 : 0;
-        uint32_t w0 = 0u, w1 = 0u, n0 = 0u, n1 = 0u;
+        // blocks of 16 words alternate between two register sets, as in k_pair
+        uint32_t wa0 = 0u, wa1 = 0u, wb0 = 0u, wb1 = 0u;
         if (cnt16 > 0)
         {
             const int32_t o = min(16, last);
-            w0 = L[jl];
-            w1 = L[8 + jl];
-            n0 = L[o + jl];
-            n1 = L[o + 8 + jl];
+            wa0 = L[jl];
+            wa1 = L[8 + jl];
+            wb0 = L[o + jl];
+            wb1 = L[o + 8 + jl];
         }
-        int32_t j0 = (int32_t)(w0 & CL_JMASK), j1 = (int32_t)(w1 & CL_JMASK);
-        float4 pj0 = a.pos[j0], rj0 = a.jprm[j0], wj0 = a.w[j0];
-        float4 pj1 = a.pos[j1], rj1 = a.jprm[j1], wj1 = a.w[j1];
-        for (int32_t t = 0; t < cnt16; t += 16)
+        int32_t ja0 = (int32_t)(wa0 & CL_JMASK), ja1 = (int32_t)(wa1 & CL_JMASK);
+        float4 pa0 = a.pos[ja0], ra0 = a.jprm[ja0], va0 = a.w[ja0];
+        float4 pa1 = a.pos[ja1], ra1 = a.jprm[ja1], va1 = a.w[ja1];
+        for (int32_t t = 0; t < cnt16; t += 32)
         {
-            const int32_t o = min(t + 32, last);
-            const uint32_t m0 = L[o + jl];
-            const uint32_t m1 = L[o + 8 + jl];
-            const int32_t k0 = (int32_t)(n0 & CL_JMASK), k1 = (int32_t)(n1 & CL_JMASK);
-            const float4 qj0 = a.pos[k0], sj0 = a.jprm[k0], vj0 = a.w[k0];
-            const float4 qj1 = a.pos[k1], sj1 = a.jprm[k1], vj1 = a.w[k1];
-            adj_pair_term<SMODE, EMODE>((w0 & ibit) != 0u, pi, ri, fli, wi, pj0, rj0, wj0, j0, a, cs, kf, hx, hy, hz,
-                                        dqi);
-            adj_pair_term<SMODE, EMODE>((w1 & ibit) != 0u, pi, ri, fli, wi, pj1, rj1, wj1, j1, a, cs, kf, hx, hy, hz,
-                                        dqi);
-            w0 = n0;
-            w1 = n1;
-            n0 = m0;
-            n1 = m1;
-            j0 = k0;
-            j1 = k1;
-            pj0 = qj0;
-            rj0 = sj0;
-            wj0 = vj0;
-            pj1 = qj1;
-            rj1 = sj1;
-            wj1 = vj1;
+            const bool ma0 = (wa0 & ibit) != 0u, ma1 = (wa1 & ibit) != 0u;
+            {
+                const uint32_t *Lw = L + min(t + 32, last);
+                wa0 = Lw[jl];
+                wa1 = Lw[8 + jl];
+            }
+            const int32_t jb0 = (int32_t)(wb0 & CL_JMASK), jb1 = (int32_t)(wb1 & CL_JMASK);
+            const float4 pb0 = a.pos[jb0], rb0 = a.jprm[jb0], vb0 = a.w[jb0];
+            const float4 pb1 = a.pos[jb1], rb1 = a.jprm[jb1], vb1 = a.w[jb1];
+            adj_pair_term<SMODE, EMODE>(ma0, pi, ri, fli, wi, pa0, ra0, va0, ja0, a, cs, kf, hx, hy, hz, dqi);
+            adj_pair_term<SMODE, EMODE>(ma1, pi, ri, fli, wi, pa1, ra1, va1, ja1, a, cs, kf, hx, hy, hz, dqi);
+            if (t + 16 >= cnt16)
+                break; // wave-uniform
+            const bool mb0 = (wb0 & ibit) != 0u, mb1 = (wb1 & ibit) != 0u;
+            {
+                const uint32_t *Lw = L + min(t + 48, last);
+                wb0 = Lw[jl];
+                wb1 = Lw[8 + jl];
+            }
+            ja0 = (int32_t)(wa0 & CL_JMASK);
+            ja1 = (int32_t)(wa1 & CL_JMASK);
+            pa0 = a.pos[ja0];
+            ra0 = a.jprm[ja0];
+            va0 = a.w[ja0];
+            pa1 = a.pos[ja1];
+            ra1 = a.jprm[ja1];
+            va1 = a.w[ja1];
+            adj_pair_term<SMODE, EMODE>(mb0, pi, ri, fli, wi, pb0, rb0, vb0, jb0, a, cs, kf, hx, hy, hz, dqi);
+            adj_pair_term<SMODE, EMODE>(mb1, pi, ri, fli, wi, pb1, rb1, vb1, jb1, a, cs, kf, hx, hy, hz, dqi);
         }
     }
     for (int off = 8; off < 64; off <<= 1)
