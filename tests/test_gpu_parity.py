@@ -83,6 +83,37 @@ def test_forces_vs_fp64(name):
         assert err <= ENERGY_RTOL, (slot, E[slot], Ed[slot], err)
 
 
+def test_close_and_coincident_pairs():
+    """Charged beads placed on a non-bonded neighbour (r = 0), 1e-4 A from it
+    (below the Coulomb floor of 1e-3 A) and 0.05 A from it (below the steric
+    floor of 0.1 A): forces stay finite and match the FP64 oracle, and so do
+    the energies."""
+    s = workloads.generate(1, seed=2, n=3000)
+    x = s.x.reshape(-1, 3)
+    bonded = set(s.id2[s.id1 == 0].tolist()) | set(s.id1[s.id2 == 0].tolist())
+    order = np.argsort(-np.linalg.norm(x - x[0], axis=1))
+    movers = [int(j) for j in order if int(j) != 0 and int(j) not in bonded][:3]
+    s.charge[0] = 1.0
+    for j, off, q in zip(movers, (0.0, 1e-4, 0.05), (-1.0, 1.0, -1.0)):
+        x[j] = x[0] + np.float32([off, 0.0, 0.0])
+        s.charge[j] = q
+    _log("close pairs: movers", movers)
+    net = SpringNetwork.from_system(s)
+    F, E = net.computeForces()
+    net.close()
+    assert np.isfinite(F).all() and np.isfinite(E).all()
+    o = po.Oracle(s)
+    Fd, Ed = o.compute_forces(po.F64)
+    o.close()
+    dyn = (s.flags & 1) == 0
+    per, glob = force_errors(F, Fd, dyn)
+    _log("  force err per-particle max", per, "global", glob)
+    assert per <= FORCE_RTOL, per
+    for slot in range(6):
+        err = energy_error(E[slot], Ed[slot], floor=1e-3)
+        assert err <= ENERGY_RTOL, (slot, E[slot], Ed[slot], err)
+
+
 @pytest.mark.parametrize("name", ["enm2k", "lj3k_ddd", "lj20k"])
 def test_forces_after_steps(name):
     """Move the system with the GPU, then compare the conservative forces
