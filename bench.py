@@ -54,10 +54,59 @@ ADJ_PAIR_FLOP = 140         # adjoint: flops per unique LJ+Coulomb pair (both si
 
 
 COPY_BYTES = 1 << 30        # copy-calibration size
-# Reference (compiled from its sources, -O3 OpenMP) / float-faithful oracle
-# step time on the same 250k input with 8 OpenMP threads each, measured on an
-# 8-core x86-64 host: 9.64 s / 8.06 s per step.
-REF_OVER_ORACLE = 1.196
+
+
+def host_cpu():
+    """The host's CPU model and the threads this process may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0))}
+
+
+def cpu_threads():
+    """OpenMP threads for the CPU baselines: OMP_NUM_THREADS when set (the
+    host's share), else every CPU this process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return int(env) if env.isdigit() and int(env) > 0 else len(os.sched_getaffinity(0))
+
+
+def cpu_reference_baseline(seed, n_full, n_sample=50000, steps=2):
+    """The reference's own step (SpringNetwork::computeStep compiled from its
+    sources, -O3 OpenMP) on all host threads, over a bounded sample of the
+    headline workload: the same generator, density and force field at
+    n_sample beads, one untimed step (initRun, first neighbour build), then
+    `steps` timed steps.  The reference has a constant dielectric only, so it
+    runs the Coulomb term with eps(r) = eps; the pair count and loop structure
+    are those of the eps(r) = eps r workload."""
+    from biospring_amd import workloads
+    from oracle import pyoracle as po
+    thr = cpu_threads()
+    po.ref_lib().ref_set_threads(thr)
+    s = workloads.generate(1, seed=seed, n=n_sample)
+    t = time.perf_counter()
+    r = po.Reference(s)
+    t_create = time.perf_counter() - t
+    r.step(1)
+    t = time.perf_counter()
+    r.step(steps)
+    dt = time.perf_counter() - t
+    r.close()
+    log(f"reference CPU baseline: {n_sample} beads, {steps} steps in {dt:.1f}s on {thr} threads "
+        f"(network set-up {t_create:.1f}s)")
+    out = {"value": n_sample * steps / dt, "unit": "particle-steps/s", "cores": thr, "kind": "reference",
+           "sample": f"{steps} timed steps (after 1 untimed) of the reference's computeStep on a {n_sample}-bead "
+                     f"network of the same workload (density, springs < 8 A, Amber 12-6 + Coulomb at 12 A; "
+                     f"constant dielectric, the reference's only model), OpenMP on {thr} threads; the full "
+                     f"workload has {n_full} beads", "seconds_per_step": dt / steps}
+    out.update(host_cpu())
+    return out
 
 
 def hip_sync():
@@ -135,9 +184,10 @@ def main():
                          "membrane, 4 rigid bodies of 2,000 beads; 3: 512 independent CA networks of "
                          "200-20,000 beads stepped together from a HIP graph; 4: adjoint gradients of a "
                          "200-step trajectory loss on a 100k-bead network (--steps is the trajectory length)")
-    ap.add_argument("--checkpoint", type=int, default=0,
-                    help="workload 4: checkpoint interval (0: the engine's choice, the whole trajectory stored "
-                         "when it fits in half the free HBM)")
+    ap.add_argument("--checkpoint", type=int, default=20,
+                    help="adjoint: checkpoint interval (20, as config 4 states; 0: the engine's choice, the whole "
+                         "trajectory stored when it fits in half the free HBM)")
+    ap.add_argument("--no-adjoint", action="store_true", help="workload 1: skip the adjoint leg (config 4)")
     ap.add_argument("--systems", type=int, default=512, help="workload 3: number of systems")
     ap.add_argument("--autotune", action="store_true",
                     help="pick the launch configuration first (always done for workload 3)")
@@ -224,7 +274,7 @@ def main():
     kernels = {
         "spring_integrate": {"avg_ms": spring_ms, "bound": "hbm", "achieved": spring_gbs, "peak": HBM_PEAK_GBS,
                              "unit": "GB/s", "frac": spring_gbs / HBM_PEAK_GBS, "alg_bytes": b_alg},
-        "pair": {"avg_ms": pair_ms, "bound": "mfma", "achieved": pair_tflops, "peak": FP32_PEAK_TFLOPS,
+        "pair": {"avg_ms": pair_ms, "bound": "valu", "achieved": pair_tflops, "peak": FP32_PEAK_TFLOPS,
                  "unit": "TFLOP/s", "frac": pair_tflops / FP32_PEAK_TFLOPS, "alg_flops": f_alg,
                  "note": "compute-bound FP32/FP64 vector ALU work priced against the FP32 peak"},
         "rebuild_chain_avg_ms_per_step": rebuild_ms,
@@ -274,11 +324,16 @@ def main():
         "kernels": kernels,
         "adjoint": None,
     }
+    if args.workload == 1 and not args.no_adjoint:
+        # the adjoint half of the metric: config 4 as stated (100k beads, 200
+        # steps, checkpoints every 20 steps with recomputation)
+        out["adjoint"] = adjoint_leg(seed=args.seed, n=0, T=200, K=args.checkpoint or 20, warmup=5,
+                                     fp64=not args.no_fp64, cpu=not args.no_cpu)
 
     if args.workload == 1 and not args.no_construction:
         out["network_construction"] = bench_construction(s, cpu=not args.no_cpu)
     if args.workload == 1 and not args.no_trajectory:
-        out["trajectory_output"] = bench_trajectory(s, args.steps, out["ms_per_step"])
+        out["trajectory_output"] = bench_trajectory(s, args.steps, out["ms_per_step"], every=max(1, args.steps // 4))
 
     x, v = net.getState()
     # Conservative force: the viscous term -gamma v is an exact product in both
@@ -322,6 +377,8 @@ def main():
                                         "state": f"after {args.warmup + args.steps + min(args.steps, 50)} steps",
                                         "systems_checked": len(checks)}
             log(f"fp64 check {time.perf_counter() - t:.1f}s")
+        if not args.no_cpu and args.workload == 1 and po.ref_available():
+            out["cpu_baseline"] = cpu_reference_baseline(args.seed, n)
         if not args.no_cpu:
             # bounded sample: the oracle steps the same system (or successive
             # systems of the batch) from the current state
@@ -340,14 +397,17 @@ def main():
                 o.close()
                 if dt > 12.0:
                     break
-            out["cpu_baseline"] = {"value": done / dt, "unit": "particle-steps/s",
+            port = {"value": done / dt, "unit": "particle-steps/s",
                                    "cores": int(po.oracle_lib().orc_num_threads()), "kind": "port",
                                    "sample": (f"{args.cpu_steps} full step(s) of the same {n}-bead workload"
                                               if off is None else
                                               f"2 steps of each system in turn, {done // 2} beads in all")
                                              + ", float-faithful oracle, OpenMP"}
-            if REF_OVER_ORACLE:
-                out["cpu_baseline"]["reference_equivalent"] = done / dt / REF_OVER_ORACLE
+            port.update(host_cpu())
+            if "cpu_baseline" in out:
+                out["cpu_baseline"]["port"] = port
+            else:
+                out["cpu_baseline"] = port
     print(json.dumps(out), flush=True)
 
 
@@ -430,13 +490,16 @@ def bench_trajectory(s, steps, ms_plain, every=50):
         net.run(steps)
         net.flushTrajectories()
         dt = time.perf_counter() - t
-        xtc_bytes = os.path.getsize(os.path.join(d, "t.xtc"))
         net.closeTrajectories()
+        xtc_bytes = os.path.getsize(os.path.join(d, "t.xtc"))
+        with open(os.path.join(d, "t.csv")) as fh:
+            csv_rows = sum(1 for _ in fh) - 1
     net.close()
     frames = len(range(10 + (-10) % every, 10 + steps, every))
     ms = dt / steps * 1e3
     log(f"trajectory: {frames} frames in {steps} steps, {ms:.3f} ms/step vs {ms_plain:.3f}")
     return {"writers": ["xtc", "csv"], "every": every, "steps": steps, "frames": frames,
+            "csv_rows_written": csv_rows,
             "ms_per_step": ms, "ms_per_step_without": ms_plain, "overhead": ms / ms_plain - 1.0,
             "xtc_bytes_per_frame": xtc_bytes / max(frames, 1)}
 
@@ -483,26 +546,27 @@ def trajectory_check(seed, n=1000, marks=(1, 10, 100, 1000)):
     return out
 
 
-def bench_adjoint(args):
-    """Workload 4: the checkpointed adjoint (forward, recomputation and the
-    FP64 reverse sweep) timed against the plain forward of the same steps;
-    accuracy on a 2,000-bead network of the same kind against the FP64
-    oracle's gradient, and on 1,000 beads with cutoffs beyond the box against
-    central differences (h = 1e-3) of the FP64 loss."""
+def adjoint_leg(seed=1, n=0, T=200, K=20, warmup=5, fp64=True, cpu=True, check_n=1000, fd=True):
+    """The checkpointed adjoint of config 4 (forward, recomputation of every
+    segment and the FP64 reverse sweep), timed against the plain forward of
+    the same T steps.  Accuracy: the gradient of a check_n-bead network of the
+    same kind against the FP64 oracle's, and, on 1,000 beads with cutoffs
+    beyond the box, against central differences (h = 1e-3) of the FP64 loss.
+    CPU baseline: the FP64 oracle's forward + adjoint on a 10k-bead network of
+    the same kind."""
     from biospring_amd import _abi, workloads
     from biospring_amd.network import SpringNetwork
 
-    T, K = args.steps, args.checkpoint
     t = time.perf_counter()
-    s = workloads.generate(4, seed=args.seed, n=args.n)
-    y = workloads.fitting_target(s, seed=args.seed)
+    s = workloads.generate(4, seed=seed, n=n)
+    y = workloads.fitting_target(s, seed=seed)
     n = s.n
-    log(f"generated n={n} springs={s.m} in {time.perf_counter() - t:.1f}s digest={s.digest()}")
+    log(f"adjoint: generated n={n} springs={s.m} in {time.perf_counter() - t:.1f}s digest={s.digest()}")
     net = SpringNetwork.from_system(s)
     net.setup()
     x0, v0 = net.getState()
     pairs_unique = net.countPairs(_abi.NS_STERIC)
-    net.adjoint(min(max(args.warmup, 1), T), y, K)
+    net.adjoint(min(max(warmup, 1), T), y, K)
     net.setState(x0, v0)
     net.profile(True)
     net.adjoint(T, y, K)
@@ -520,8 +584,8 @@ def bench_adjoint(args):
 
     t_fwd, _ = timed(lambda: net.run(T))
     t_adj, (loss, dk, dl, dq) = timed(lambda: net.adjoint(T, y, K))
-    value = n * T / t_adj
-    log(f"forward {t_fwd * 1e3:.1f} ms, adjoint {t_adj * 1e3:.1f} ms, ratio {t_adj / t_fwd:.2f}")
+    net.close()
+    log(f"adjoint: forward {t_fwd * 1e3:.1f} ms, adjoint {t_adj * 1e3:.1f} ms, ratio {t_adj / t_fwd:.2f}")
 
     ap_ms = prof["adj_pair"]["ms"] / max(prof["adj_pair"]["launches"], 1)
     as_ms = prof["adj_spring"]["ms"] / max(prof["adj_spring"]["launches"], 1)
@@ -529,7 +593,7 @@ def bench_adjoint(args):
     owned = int(s.m)  # springs with a dynamic end: one read-modify-write of dk, dL0 each per step
     b_alg = 4 * (n + 1) + 12 * edges + 32 * owned + 280 * n
     kernels = {
-        "adj_pair": {"avg_ms": ap_ms, "bound": "mfma", "achieved": f_alg / (ap_ms * 1e-3) / 1e12,
+        "adj_pair": {"avg_ms": ap_ms, "bound": "valu", "achieved": f_alg / (ap_ms * 1e-3) / 1e12,
                      "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "alg_flops": f_alg,
                      "note": "FP32 terms with FP64 sums, priced against the FP32 peak (pairs at the initial state)"},
         "adj_spring": {"avg_ms": as_ms, "bound": "hbm", "achieved": b_alg / (as_ms * 1e-3) / 1e9,
@@ -544,51 +608,48 @@ def bench_adjoint(args):
     d = kernels[dom]
     roofline = {"bound": d["bound"], "achieved": d["achieved"], "peak": d["peak"], "unit": d["unit"],
                 "frac": d["frac"], "traffic": None, "kernel": dom}
-
     out = {
-        "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": T,
-        "warmup": args.warmup, "ms_per_step": t_adj / T * 1e3, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "fp32 trajectory, fp64 adjoint", "data": "synthetic",
-        "config": {"workload": WORKLOADS[4].format(n=n, T=T, ck=(f"checkpoints every {K}" if K > 0 else
-                                                             "the engine's checkpoint choice: the whole trajectory "
-                                                             "kept in HBM when it fits in half the free memory")),
-                   "n_beads": n, "springs": s.m,
-                   "directed_edges": edges, "unique_pairs_initial": pairs_unique, "checkpoint_every": K,
-                   "seed": args.seed, "digest": s.digest()},
-        "roofline": roofline,
-        "kernels": kernels,
-        "adjoint": {"forward_ms": t_fwd * 1e3, "adjoint_ms": t_adj * 1e3, "adjoint_over_forward": t_adj / t_fwd,
-                    "loss": loss},
+        "workload": WORKLOADS[4].format(n=n, T=T, ck=(f"checkpoints every {K} steps, segments recomputed"
+                                                      if K > 0 else
+                                                      "the engine's checkpoint choice: the whole trajectory "
+                                                      "kept in HBM when it fits in half the free memory")),
+        "n_beads": n, "steps": T, "checkpoint_every": K, "springs": s.m, "directed_edges": edges,
+        "unique_pairs_initial": pairs_unique, "seed": seed, "digest": s.digest(),
+        "value": n * T / t_adj, "unit": "particle-steps/s (forward + adjoint)",
+        "forward_ms": t_fwd * 1e3, "adjoint_ms": t_adj * 1e3, "adjoint_over_forward": t_adj / t_fwd,
+        "forward_particle_steps_per_s": n * T / t_fwd,
+        "loss": loss, "roofline": roofline, "kernels": kernels,
     }
-    if not args.no_fp64 or not args.no_cpu:
+    if fp64 or cpu:
         from oracle import pyoracle as po
-        if not args.no_fp64:
-            t = time.perf_counter()
-            sm = workloads.generate(4, seed=args.seed, n=2000)
-            ym = workloads.fitting_target(sm, seed=args.seed)
-            y64 = ym.astype(np.float64)
-            g = SpringNetwork.from_system(sm).adjoint(T, ym, K)
-            o = po.Oracle(sm)
-            r = o.adjoint(T, y64)
-            o.close()
-            rel = lambda a, b: float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
-            out["grad_err_vs_fp64"] = {"n_beads": sm.n, "steps": T, "loss_rel": abs(g[0] - r[0]) / r[0],
-                                       "dk": rel(g[1], r[1]), "dl0": rel(g[2], r[2]), "dq": rel(g[3], r[3]),
-                                       "measure": "max |g_gpu - g_fp64| / max |g_fp64|"}
+    if fp64:
+        t = time.perf_counter()
+        sm = workloads.generate(4, seed=seed, n=check_n)
+        ym = workloads.fitting_target(sm, seed=seed)
+        g = SpringNetwork.from_system(sm).adjoint(T, ym, K)
+        o = po.Oracle(sm)
+        r = o.adjoint(T, ym.astype(np.float64))
+        o.close()
+        rel = lambda a, b: float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+        out["grad_err_vs_fp64"] = {"n_beads": sm.n, "steps": T, "checkpoint_every": K,
+                                   "loss_rel": abs(g[0] - r[0]) / r[0],
+                                   "dk": rel(g[1], r[1]), "dl0": rel(g[2], r[2]), "dq": rel(g[3], r[3]),
+                                   "measure": "max |g_gpu - g_fp64| / max |g_fp64|"}
+        if fd:
             # Central differences (h = 1e-3) of the FP64 loss for the parameter
             # of largest gradient of each kind.  With 12 A cutoffs, pairs cross
             # them during 200 steps and the loss jumps there (the forces are
-            # cut, not smoothed), which a frozen-pair-set gradient does not see;
-            # the differences are therefore taken on 1,000 beads with cutoffs
-            # of 60 A, beyond the box, where no pair crosses and the loss is
-            # smooth.
-            sf = workloads.generate(4, seed=args.seed, n=1000)
-            yf = workloads.fitting_target(sf, seed=args.seed)
+            # cut, not smoothed), which a frozen-pair-set gradient does not
+            # see; the differences are therefore taken on 1,000 beads with
+            # cutoffs of 60 A, beyond the box, where no pair crosses and the
+            # loss is smooth.
+            sf = workloads.generate(4, seed=seed, n=1000)
+            yf = workloads.fitting_target(sf, seed=seed)
             pf = sf.params.copy()
             pf.steric_cutoff = pf.electrostatic_cutoff = 60.0
             yf64 = yf.astype(np.float64)
             gf = SpringNetwork.from_system(sf, params=pf).adjoint(T, yf, K)
-            fd = {}
+            fdr = {}
             for name, gi, base in (("k", 1, sf.k), ("l0", 2, sf.l0), ("q", 3, sf.charge)):
                 i = int(np.argmax(np.abs(gf[gi])))
                 lo = []
@@ -600,23 +661,44 @@ def bench_adjoint(args):
                     lo.append(oo.loss(T, yf64))
                     oo.close()
                 c = (lo[0] - lo[1]) / 2e-3
-                fd[name] = {"index": i, "central_difference": c, "gpu": float(gf[gi][i]),
-                            "gpu_rel_err": abs(gf[gi][i] - c) / abs(c)}
+                fdr[name] = {"index": i, "central_difference": c, "gpu": float(gf[gi][i]),
+                             "gpu_rel_err": abs(gf[gi][i] - c) / abs(c)}
             out["grad_err_vs_fp64"]["central_difference_h1e-3"] = {
-                "n_beads": sf.n, "cutoffs_A": 60.0, "steps": T, "params": fd}
-            log(f"fp64 gradient check {time.perf_counter() - t:.1f}s: " +
-                ", ".join(f"{k} {v['gpu_rel_err']:.2e}" for k, v in fd.items()))
-        if not args.no_cpu:
-            o = po.Oracle(s)
-            o.set_state(x0, v0)
-            steps = 2
-            t = time.perf_counter()
-            o.adjoint(steps, y.astype(np.float64))
-            dt = time.perf_counter() - t
-            o.close()
-            out["cpu_baseline"] = {"value": n * steps / dt, "unit": "particle-steps/s",
-                                   "cores": int(po.oracle_lib().orc_num_threads()), "kind": "port",
-                                   "sample": f"FP64 oracle adjoint of {steps} steps of the same {n}-bead network, OpenMP"}
+                "n_beads": sf.n, "cutoffs_A": 60.0, "steps": T, "params": fdr}
+        log(f"adjoint: fp64 gradient check {time.perf_counter() - t:.1f}s")
+    if cpu:
+        sc = workloads.generate(4, seed=seed, n=10000)
+        yc = workloads.fitting_target(sc, seed=seed)
+        o = po.Oracle(sc)
+        steps = 2
+        t = time.perf_counter()
+        o.adjoint(steps, yc.astype(np.float64))
+        dt = time.perf_counter() - t
+        o.close()
+        out["cpu_baseline"] = {"value": sc.n * steps / dt, "unit": "particle-steps/s (forward + adjoint)",
+                               "cores": int(po.oracle_lib().orc_num_threads()), "kind": "port",
+                               "sample": f"FP64 oracle forward + adjoint of {steps} steps of a {sc.n}-bead network "
+                                         f"of the same kind, OpenMP (the reference has no adjoint)"}
+    return out
+
+
+def bench_adjoint(args):
+    """Workload 4 alone: the adjoint leg as the bench line."""
+    leg = adjoint_leg(seed=args.seed, n=args.n, T=args.steps, K=args.checkpoint, warmup=args.warmup,
+                      fp64=not args.no_fp64, cpu=not args.no_cpu, check_n=2000)
+    out = {
+        "metric": METRIC, "value": leg["value"], "unit": "particle-steps/s", "n_gpus": 1, "steps": leg["steps"],
+        "warmup": args.warmup, "ms_per_step": leg["adjoint_ms"] / leg["steps"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32 trajectory, fp64 adjoint", "data": "synthetic",
+        "config": {k: leg[k] for k in ("workload", "n_beads", "springs", "directed_edges", "unique_pairs_initial",
+                                       "checkpoint_every", "seed", "digest")},
+        "roofline": leg["roofline"],
+        "kernels": leg["kernels"],
+        "adjoint": {k: leg[k] for k in ("forward_ms", "adjoint_ms", "adjoint_over_forward", "loss")},
+    }
+    for k in ("grad_err_vs_fp64", "cpu_baseline"):
+        if k in leg:
+            out[k] = leg[k]
     print(json.dumps(out), flush=True)
 
 

@@ -31,7 +31,7 @@ EXPORTS = (
     "bs_num_springs", "bs_set_particles", "bs_set_springs", "bs_set_params", "bs_setup",
     "bs_step", "bs_compute_forces", "bs_get_state", "bs_set_state", "bs_get_energies",
     "bs_last_nonfinite_particle", "bs_get_counters", "bs_profile", "bs_get_profile",
-    "bs_get_cells", "bs_get_pairs", "bs_add_rigid_body", "bs_num_rigid_bodies", "bs_get_rigid_body",
+    "bs_get_cells", "bs_get_pairs", "bs_add_rigid_body", "bs_num_rigid_bodies", "bs_get_rigid_body", "bs_get_probe",
     "bs_set_grid", "bs_set_burying", "bs_get_insertion_vector", "bs_set_systems", "bs_num_systems",
     "bs_get_system_energies", "bs_autotune", "bs_adjoint", "bs_adjoint_forward", "bs_adjoint_backward",
     "bs_springs_from_cutoff", "bs_get_springs", "bs_add_trajectory", "bs_write_frame_now",
@@ -79,6 +79,16 @@ class Params(ctypes.Structure):
         ("densitygrid_scale", ctypes.c_float),
         ("graph_steps", ctypes.c_int32),
         ("pair_layout", ctypes.c_int32),
+        ("probe_enable", ctypes.c_int32),
+        ("probe_enable_electrostatic", ctypes.c_int32),
+        ("probe_enable_steric", ctypes.c_int32),
+        ("probe_x", ctypes.c_float),
+        ("probe_y", ctypes.c_float),
+        ("probe_z", ctypes.c_float),
+        ("probe_mass", ctypes.c_float),
+        ("probe_epsilon", ctypes.c_float),
+        ("probe_radius", ctypes.c_float),
+        ("probe_charge", ctypes.c_float),
     ]
 
     def copy(self):
@@ -152,6 +162,7 @@ def bind(L):
     L.bs_add_rigid_body.argtypes = [_vp, ctypes.c_int64, _i64p]
     L.bs_num_rigid_bodies.argtypes = [_vp]
     L.bs_num_rigid_bodies.restype = ctypes.c_int32
+    L.bs_get_probe.argtypes = [_vp, _f32p, _f32p, _f32p]
     L.bs_get_rigid_body.argtypes = [_vp, ctypes.c_int32, _f64p]
     L.bs_set_grid.argtypes = [_vp, ctypes.c_int32, _f64p, _i64p, _f64p, _f32p]
     L.bs_set_burying.argtypes = [_vp, _f32p]

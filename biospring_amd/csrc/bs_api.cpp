@@ -150,13 +150,15 @@ struct bs_handle
     int32_t cent_blocks = 0;
     int32_t group = 16;
     double *energies_d = nullptr;
-    int32_t *status = nullptr;   // [0] rebuild, [1] nonfinite id, [2] overflow, [3] max count
+    int32_t *status = nullptr;   // [0] rebuild, [1] nonfinite id, [2] overflow, [3] max count, [5] stopped,
+                                 // [6] steps completed in this call (bs_device.hpp)
     int32_t *counters = nullptr; // [0] step counter, [1] rebuilds
     double energies[BS_NUM_ENERGIES] = {0, 0, 0, 0, 0, 0};
     int cur = 0;
     int64_t steps_done = 0;
     int64_t nonfinite = -1;
     int ext_uploaded = 0;
+    int32_t list_longest = 0; // longest neighbour list of the last overflowing build
 
     // event timing of bs_step (bs_profile)
     bool prof = false;
@@ -180,6 +182,18 @@ struct bs_handle
     std::vector<float *> traj_x;
     std::vector<double *> traj_e;
     std::vector<hipEvent_t> traj_ev;
+
+    // the probe particle (probe.*): its state between setups (placed at
+    // probe_x/y/z at rest by the first setup after bs_set_particles, as
+    // SpringNetwork::_setupProbe), the device copy and the spring kernel's
+    // block sums of its force and pair energies
+    struct ProbeHost
+    {
+        bool placed = false;
+        float x[3] = {0, 0, 0}, v[3] = {0, 0, 0}, f[3] = {0, 0, 0};
+    } probe;
+    ProbeDev *probe_d = nullptr;
+    double *probe_part = nullptr;
 };
 
 namespace
@@ -283,6 +297,9 @@ void free_device(bs_handle *h)
     h->nslots = -1;
     f(h->bodies_d); f(h->rx); f(h->rp0); f(h->frig); f(h->slot_p); f(h->slot_b); f(h->bstart);
     f(h->grid_d[0]); f(h->grid_d[1]); f(h->cent_part); f(h->cent);
+    f(h->probe_d); f(h->probe_part);
+    h->probe_d = nullptr;
+    h->probe_part = nullptr;
     h->bodies_d = nullptr;
     h->rx = h->rp0 = h->frig = nullptr;
     h->slot_p = h->slot_b = h->bstart = nullptr;
@@ -331,6 +348,8 @@ int check_device(bs_handle *h)
             return fail(h, BS_ERR_STATE, "internal: grid buffer missing");
     if (h->sys_off.size() > 2 && !h->shift)
         return fail(h, BS_ERR_STATE, "internal: system offsets missing");
+    if (h->p.probe_enable && (!h->probe_d || !h->probe_part))
+        return fail(h, BS_ERR_STATE, "internal: probe buffers missing");
     if (h->ncl != (int32_t)((h->n + CLUSTER - 1) / CLUSTER))
         return fail(h, BS_ERR_STATE, "internal: cluster count mismatch");
     for (const auto &l : h->lists)
@@ -681,6 +700,7 @@ void launch_pairs(bs_handle *h, bool energy)
         a.cap = l.cap;
         a.accumulate = li > 0 ? 1 : 0;
         a.energy = energy ? 1 : 0;
+        a.status = h->status;
         a.rc = l.rc;
         a.rc2lo = rc2(l.rc, 1.0 - 4e-6);
         a.rc2hi = rc2(l.rc, 1.0 + 4e-6);
@@ -717,6 +737,29 @@ GridView grid_view(const bs_handle *h, int kind, bool on, float scale)
     }
     g.scale = scale;
     return g;
+}
+
+ProbeArgs probe_args(const bs_handle *h)
+{
+    ProbeArgs a{};
+    if (!h->p.probe_enable)
+        return a;
+    const bs_params &P = h->p;
+    a.state = h->probe_d;
+    a.part = h->probe_part;
+    a.steric = P.probe_enable_steric;
+    a.elec = P.probe_enable_electrostatic;
+    a.steric_mode = P.steric_mode;
+    a.steric_scale = P.steric_gridscale;
+    const bool sq = P.steric_mode == BS_STERIC_AMBER_12_6 || P.steric_mode == BS_STERIC_LEWITT_8_6;
+    a.radius = sq ? std::sqrt(P.probe_radius) : P.probe_radius; // as jprm holds the particles'
+    a.eps = sq ? std::sqrt(P.probe_epsilon) : P.probe_epsilon;
+    a.mass = P.probe_mass;
+    a.ke = h->k.e_coul / P.electrostatic_dielectric * P.electrostatic_scale;
+    a.kf = h->k.c_elec / (4.0 * h->k.pi * P.electrostatic_dielectric) * P.electrostatic_scale;
+    a.ddd = P.dielectric_model == BS_DIELECTRIC_DISTANCE ? 1 : 0;
+    a.status = h->status;
+    return a;
 }
 
 void launch_springs(bs_handle *h, bool integrate, bool energy)
@@ -763,7 +806,19 @@ void launch_springs(bs_handle *h, bool integrate, bool energy)
                        h->p.potentialgrid_scale);
     a.gden = grid_view(h, BS_GRID_DENSITY, h->p.densitygrid_enable, h->p.densitygrid_scale);
     a.e_field = 1.3806504E-23 * 1.0E-3 * 6.02214169E23; // k_B, J -> kJ, N_A (constants.hpp:70-73)
+    a.probe = probe_args(h);
+    a.steps_done = h->status + 6;
     launch_spring(a, h->group, h->spring_blocks, h->stream);
+}
+
+// The probe's step after the spring kernel (integrate) or its force and pair
+// energies only (a force evaluation).
+void launch_probe_step(bs_handle *h, bool integrate)
+{
+    if (!h->p.probe_enable)
+        return;
+    launch_probe(probe_args(h), h->spring_blocks, h->p.timestep, h->p.viscosity_enable, h->p.viscosity_value,
+                 h->k.c_kinetic, integrate ? 1 : 0, h->stream);
 }
 
 void launch_bodies(bs_handle *h)
@@ -846,7 +901,8 @@ int prof_fold(bs_handle *h)
 void launch_energy_sum(bs_handle *h)
 {
     launch_energies(h->lists.empty() ? nullptr : h->epp, h->ese, h->seg_start, h->nseg, h->sys_seg,
-                    (int32_t)h->sys_off.size() - 1, h->seg_part, h->sys_E, h->energies_d, h->stream);
+                    (int32_t)h->sys_off.size() - 1, h->seg_part, h->sys_E, h->energies_d,
+                    h->p.probe_enable ? h->probe_d : nullptr, h->stream);
 }
 
 // Queues a trajectory frame of the current state at iteration nb: positions
@@ -873,6 +929,9 @@ int traj_capture(bs_handle *h, int64_t nb, bool dev_energies, bool all)
     }
     else if (e == hipSuccess)
         std::memcpy(h->traj_e[(size_t)s], h->energies, sizeof(double) * BS_NUM_ENERGIES);
+    int32_t *halt = reinterpret_cast<int32_t *>(h->traj_e[(size_t)s] + BS_NUM_ENERGIES);
+    if (e == hipSuccess) // whether the call had stopped before this frame
+        e = hipMemcpyAsync(halt, h->status, sizeof(int32_t) * 8, hipMemcpyDeviceToHost, h->stream);
     if (e == hipSuccess)
         e = hipEventRecord(h->traj_ev[(size_t)s], h->stream);
     if (e != hipSuccess)
@@ -884,26 +943,83 @@ int traj_capture(bs_handle *h, int64_t nb, bool dev_energies, bool all)
     f.nbiter = nb;
     f.x3 = h->traj_x[(size_t)s];
     f.energies = h->traj_e[(size_t)s];
+    f.status = halt;
     const hipEvent_t ev = h->traj_ev[(size_t)s];
-    h->traj->submit(s, f, [ev] { (void)hipEventSynchronize(ev); }, all);
+    h->traj->submit(
+        s, f,
+        [ev]() -> std::string {
+            const hipError_t r = hipEventSynchronize(ev);
+            return r == hipSuccess ? std::string() : std::string("frame copy failed: ") + hipGetErrorString(r);
+        },
+        all);
     return BS_OK;
 }
 
-int finish_call(bs_handle *h)
+// Ends a call's launches: sums the energies of its last energy step and reads
+// the status words.  completed (nullable): steps the call completed before it
+// stopped or ended.  overflow (nullable): set, instead of failing, when a list
+// overflowed, for the caller to lengthen the lists and resume.
+int finish_call(bs_handle *h, int64_t *completed = nullptr, bool *overflow = nullptr)
 {
     launch_energy_sum(h);
-    int32_t st[4];
+    int32_t st[8];
     HIPCHK(h, hipMemcpyAsync(h->energies, h->energies_d, sizeof(h->energies), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipMemcpyAsync(st, h->status, sizeof st, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     prof_collect(h);
+    if (completed)
+        *completed = st[6];
     if (st[2] != 0)
+    {
+        h->list_longest = st[3];
+        if (overflow)
+        {
+            *overflow = true;
+            return BS_OK;
+        }
         return fail(h, BS_ERR_STATE, "neighbour list capacity exceeded (max %d > cap); rerun bs_setup", st[3]);
+    }
     if (st[1] != INT_MAX)
     {
         h->nonfinite = st[1];
         return fail(h, BS_ERR_NONFINITE, "Found non-finite position for particle %d.", st[1]);
     }
+    return BS_OK;
+}
+
+// Longer neighbour lists after an overflow: every list shorter than the
+// longest list the failed build found, plus the setup's margin, is
+// reallocated; the stop is cleared and the next step rebuilds the lists.
+int regrow_lists(bs_handle *h)
+{
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    drop_graph(h);
+    h->tape.ready = false;
+    const int32_t need = (std::max<int32_t>(16, (int32_t)std::ceil(h->list_longest * 1.25) + 64) + 15) / 16 * 16;
+    hipError_t e;
+    for (auto &l : h->lists)
+    {
+        if (l.cap >= need)
+            continue;
+        (void)hipFree(l.cl);
+        l.cl = nullptr;
+        l.cap = need;
+        l.cl = dalloc<uint32_t>((size_t)h->ncl * l.cap, e);
+        HIPCHK(h, e);
+        if (l.rows)
+        {
+            (void)hipFree(l.rows);
+            l.rows = nullptr;
+            l.rcap = l.cap;
+            l.rows = dalloc<uint32_t>((size_t)((h->n + 63) / 64) * 64 * l.rcap, e);
+            HIPCHK(h, e);
+        }
+    }
+    int32_t st[8];
+    HIPCHK(h, hipMemcpy(st, h->status, sizeof st, hipMemcpyDeviceToHost));
+    st[0] = 1; // rebuild at the next step
+    st[2] = st[5] = st[6] = 0;
+    HIPCHK(h, hipMemcpy(h->status, st, sizeof st, hipMemcpyHostToDevice));
     return BS_OK;
 }
 
@@ -920,6 +1036,7 @@ void launch_step(bs_handle *h, bool energy)
     const size_t e2 = h->prof ? prof_mark(h) : SIZE_MAX;
     launch_springs(h, true, energy);
     launch_bodies(h);
+    launch_probe_step(h, true);
     if (h->prof)
     {
         const size_t e3 = prof_mark(h);
@@ -1069,6 +1186,8 @@ int adjoint_check(bs_handle *h)
         return fail(h, BS_ERR_ARG, "bs_adjoint: field grids have no adjoint");
     if (P.compensated_positions)
         return fail(h, BS_ERR_ARG, "bs_adjoint: needs compensated_positions = 0");
+    if (P.probe_enable)
+        return fail(h, BS_ERR_ARG, "bs_adjoint: the probe particle has no adjoint");
     return BS_OK;
 }
 
@@ -1415,7 +1534,7 @@ int do_setup(bs_handle *h)
     ALLOC(h->cell_of, int32_t, n);
     ALLOC(h->ctmp, int32_t, n);
     ALLOC(h->cidx, int32_t, n);
-    ALLOC(h->status, int32_t, 4);
+    ALLOC(h->status, int32_t, 8);
     ALLOC(h->counters, int32_t, 4);
     ALLOC(h->energies_d, double, BS_NUM_ENERGIES);
     h->box_blocks = (int32_t)std::min<int64_t>(BLOCK, (n + BLOCK - 1) / BLOCK); // one per final-reduce thread
@@ -1619,7 +1738,7 @@ int do_setup(bs_handle *h)
     }
 
     // status / counters
-    const int32_t st0[4] = {1, INT_MAX, 0, 0}, ct0[4] = {0, 0, 0, 0};
+    const int32_t st0[8] = {1, INT_MAX, 0, 0, 0, 0, 0, 0}, ct0[4] = {0, 0, 0, 0};
     HIPCHK(h, hipMemcpyAsync(h->status, st0, sizeof st0, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(h->counters, ct0, sizeof ct0, hipMemcpyHostToDevice, h->stream));
 
@@ -1629,6 +1748,31 @@ int do_setup(bs_handle *h)
     h->spring_blocks = (int32_t)((n * h->group + BLOCK - 1) / BLOCK);
     ALLOC(h->epp, double4, n);
     ALLOC(h->ese, double4, n);
+
+    // the probe particle: sized for the widest spring launch (32 lanes a row)
+    if (h->p.probe_enable)
+    {
+        if (h->sys_off.size() > 2)
+            return fail(h, BS_ERR_ARG, "bs_setup: the probe particle belongs to a single network, not to a batch of "
+                                       "systems");
+        if (!h->probe.placed)
+        { // SpringNetwork::_setupProbe (SpringNetwork.cpp:749-766): at probe.x/y/z, at rest
+            const float x[3] = {h->p.probe_x, h->p.probe_y, h->p.probe_z};
+            for (int c = 0; c < 3; ++c)
+            {
+                h->probe.x[c] = x[c];
+                h->probe.v[c] = h->probe.f[c] = 0.0f;
+            }
+            h->probe.placed = true;
+        }
+        ALLOC(h->probe_d, ProbeDev, 1);
+        ALLOC(h->probe_part, double, 8 * ((n * 32 + BLOCK - 1) / BLOCK));
+        ProbeDev pd{};
+        pd.x = make_float4(h->probe.x[0], h->probe.x[1], h->probe.x[2], h->p.probe_charge);
+        pd.v = make_float4(h->probe.v[0], h->probe.v[1], h->probe.v[2], 0.0f);
+        pd.f = make_double4(h->probe.f[0], h->probe.f[1], h->probe.f[2], 0.0);
+        HIPCHK(h, hipMemcpy(h->probe_d, &pd, sizeof pd, hipMemcpyHostToDevice));
+    }
 
     // static exclusions per i-cluster: the union of its 8 spring rows as
     // sorted (j, bonded-mask) words; springs never change during a run
@@ -1714,7 +1858,7 @@ int do_setup(bs_handle *h)
         }
         launch_fill_i32(h->status + 2, 0, 1, nullptr, h->stream);
     }
-    const int32_t st1[4] = {1, INT_MAX, 0, 0};
+    const int32_t st1[8] = {1, INT_MAX, 0, 0, 0, 0, 0, 0};
     HIPCHK(h, hipMemcpyAsync(h->status, st1, sizeof st1, hipMemcpyHostToDevice, h->stream));
     rebuild_chain(h, true);
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -1746,6 +1890,19 @@ int pull_and_release(bs_handle *h)
     int rc = bs_get_state(h, h->x.data(), h->v.data());
     if (rc)
         return rc;
+    if (h->probe_d)
+    {
+        ProbeDev pd;
+        HIPCHK(h, hipMemcpy(&pd, h->probe_d, sizeof pd, hipMemcpyDeviceToHost));
+        const float x[3] = {pd.x.x, pd.x.y, pd.x.z}, v[3] = {pd.v.x, pd.v.y, pd.v.z};
+        const double f[3] = {pd.f.x, pd.f.y, pd.f.z};
+        for (int c = 0; c < 3; ++c)
+        {
+            h->probe.x[c] = x[c];
+            h->probe.v[c] = v[c];
+            h->probe.f[c] = (float)f[c];
+        }
+    }
     if (!h->bodies.empty() && h->bodies_d)
     {
         std::vector<RigidBody> st(h->bodies.size());
@@ -1868,6 +2025,7 @@ int bs_set_particles(bs_handle *h, int64_t n, const float *x3, const float *v3, 
     h->have_particles = true;
     h->steps_done = 0;
     h->nonfinite = -1;
+    h->probe.placed = false; // the next setup places the probe anew
     return BS_OK;
 }
 
@@ -1959,12 +2117,16 @@ int bs_step(bs_handle *h, int32_t nsteps, const float *ext_force3)
     // step before, so that step computes its energies.
     const bool tr = h->traj && !h->traj->empty();
     const int32_t G = (h->p.graph_steps + 1) / 2 * 2;
-    const bool graphs = G > 0 && !h->prof && nsteps > G;
-    int32_t s = 0;
-    while (s < nsteps)
+    int64_t left = nsteps;
+    for (int pass = 0;; ++pass)
+    {
+    launch_fill_i32(h->status + 6, 0, 1, nullptr, h->stream); // steps completed in this pass
+    const bool graphs = G > 0 && !h->prof && left > G;
+    int64_t s = 0;
+    while (s < left)
     {
         const int64_t nb = h->steps_done + s;
-        if (tr && h->traj->due(nb))
+        if (tr && h->traj->due(nb) && !(pass > 0 && s == 0))
         {
             rc = traj_capture(h, nb, s > 0, false);
             if (rc)
@@ -1973,7 +2135,7 @@ int bs_step(bs_handle *h, int32_t nsteps, const float *ext_force3)
         // replay a captured graph of G steps (from cur = 0 back to cur = 0)
         // when no frame falls inside it or right after it; the last step stays
         // outside, for the energies
-        if (graphs && h->cur == 0 && nsteps - 1 - s >= G && (!tr || h->traj->next_due(nb) > nb + G))
+        if (graphs && h->cur == 0 && left - 1 - s >= G && (!tr || h->traj->next_due(nb) > nb + G))
         {
             if (!h->graph_exec || h->graph_len != G || h->graph_ext != h->ext_uploaded)
             {
@@ -1988,12 +2150,27 @@ int bs_step(bs_handle *h, int32_t nsteps, const float *ext_force3)
         const int frc = prof_fold(h);
         if (frc)
             return frc;
-        launch_step(h, s + 1 == nsteps || (tr && h->traj->due(nb + 1)));
+        launch_step(h, s + 1 == left || (tr && h->traj->due(nb + 1)));
         ++s;
     }
-    h->steps_done += nsteps;
+    int64_t done = 0;
+    bool overflow = false;
+    rc = finish_call(h, &done, &overflow);
+    h->steps_done += done;
     h->last_call_forces_only = false;
-    return finish_call(h);
+    if (rc || !overflow)
+        return rc;
+    // A list overflowed at the start of step `done` of this pass, which
+    // stopped there: lengthen the lists, rebuild at that state and go on.
+    // Frames taken after the stop were not written and are taken again.
+    if (pass >= 3)
+        return fail(h, BS_ERR_STATE, "neighbour list capacity exceeded (max %d) after lengthening the lists",
+                    h->list_longest);
+    rc = regrow_lists(h);
+    if (rc)
+        return rc;
+    left -= done;
+    }
 }
 
 int bs_compute_forces(bs_handle *h, const float *ext_force3, float *force3, double *energies)
@@ -2006,10 +2183,13 @@ int bs_compute_forces(bs_handle *h, const float *ext_force3, float *force3, doub
     rc = upload_ext(h, ext_force3);
     if (rc)
         return rc;
+    launch_fill_i32(h->status + 2, 0, 1, nullptr, h->stream);
+    launch_fill_i32(h->status + 5, 0, 1, nullptr, h->stream);
     rebuild_chain(h, true);
     if (!h->lists.empty())
         launch_pairs(h, true);
     launch_springs(h, false, true);
+    launch_probe_step(h, false);
     rc = finish_call(h);
     h->energies[BS_E_KINETIC] = 0.0;
     h->last_call_forces_only = true;
@@ -2105,8 +2285,10 @@ int bs_set_state(bs_handle *h, const float *x3, const float *v3)
     int rc = upload_positions(h);
     if (rc)
         return rc;
-    const int32_t one = 1;
-    HIPCHK(h, hipMemcpy(h->status, &one, sizeof one, hipMemcpyHostToDevice));
+    // a new state: rebuild the lists, clear a stop and a non-finite report
+    const int32_t st[8] = {1, INT_MAX, 0, 0, 0, 0, 0, 0};
+    HIPCHK(h, hipMemcpy(h->status, st, sizeof st, hipMemcpyHostToDevice));
+    h->nonfinite = -1;
     return BS_OK;
 }
 
@@ -2189,6 +2371,19 @@ int bs_get_cells(bs_handle *h, int32_t kind, int64_t *ncells_out, int64_t *nmemb
     double bx[6];
     HIPCHK(h, hipMemcpyAsync(bx, h->box, sizeof bx, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (h->p.probe_enable)
+    { // the probe is in the reference's container, so in its box, though never in a cell
+      // (SpringNetwork.cpp:763-764,816-824; nsearch.hpp:342-366)
+        ProbeDev pd;
+        HIPCHK(h, hipMemcpy(&pd, h->probe_d, sizeof pd, hipMemcpyDeviceToHost));
+        const double q[3] = {pd.x.x, pd.x.y, pd.x.z};
+        for (int c = 0; c < 3; ++c)
+        {
+            bx[c] = std::min(bx[c], q[c]);
+            bx[3 + c] = std::max(bx[3 + c], q[c]);
+        }
+        HIPCHK(h, hipMemcpy(h->box, bx, sizeof bx, hipMemcpyHostToDevice));
+    }
     const float cutoff = kind == BS_NS_ELECTROSTATIC ? h->p.electrostatic_cutoff
                          : kind == BS_NS_HYDROPHOBIC ? h->p.hydrophobicity_cutoff
                                                      : h->p.steric_cutoff;
@@ -2352,6 +2547,32 @@ int bs_add_rigid_body(bs_handle *h, int64_t n, const int64_t *ids)
     h->bodies.back().ids = v;
     for (int64_t id : v)
         h->body_of[(size_t)id] = b;
+    return BS_OK;
+}
+
+int bs_get_probe(bs_handle *h, float *x3, float *v3, float *f3)
+{
+    if (!h)
+        return BS_ERR_ARG;
+    if (!h->p.probe_enable)
+        return fail(h, BS_ERR_STATE, "bs_get_probe: the probe particle is not enabled");
+    int rc = ensure_ready(h);
+    if (rc)
+        return rc;
+    ProbeDev pd;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipMemcpy(&pd, h->probe_d, sizeof pd, hipMemcpyDeviceToHost));
+    const float x[3] = {pd.x.x, pd.x.y, pd.x.z}, v[3] = {pd.v.x, pd.v.y, pd.v.z};
+    const float f[3] = {(float)pd.f.x, (float)pd.f.y, (float)pd.f.z};
+    for (int c = 0; c < 3; ++c)
+    {
+        if (x3)
+            x3[c] = x[c];
+        if (v3)
+            v3[c] = v[c];
+        if (f3)
+            f3[c] = f[c];
+    }
     return BS_OK;
 }
 
@@ -3124,7 +3345,8 @@ int bs_add_trajectory(bs_handle *h, int32_t kind, const char *path, int64_t freq
             if (r == hipSuccess)
             {
                 h->traj_x.push_back(x);
-                r = hipHostMalloc((void **)&e, sizeof(double) * BS_NUM_ENERGIES, hipHostMallocDefault);
+                // the six energies, then the call's status words (8 int32)
+                r = hipHostMalloc((void **)&e, sizeof(double) * (BS_NUM_ENERGIES + 4), hipHostMallocDefault);
             }
             if (r == hipSuccess)
             {

@@ -8,7 +8,16 @@ namespace bs
 
 __device__ inline uint32_t fbits(float f) { return __float_as_uint(f); }
 
-__device__ inline bool gated_off(const int32_t *gate) { return gate != nullptr && *gate == 0; }
+// The status words a call's kernels share: [0] rebuild due, [1] smallest
+// particle id with a non-finite position (INT_MAX: none), [2] a neighbour list
+// overflowed, [3] longest list seen, [5] a non-finite position was seen before
+// this step began (k_prestep's snapshot of [1]), [6] steps completed in this
+// call.  Once a list overflowed or a
+// step produced a non-finite position, every later kernel of the call leaves
+// the state as it is: the call returns the state of that step.
+__device__ inline bool halted(const int32_t *status) { return status[2] != 0 || status[5] != 0; }
+
+__device__ inline bool gated_off(const int32_t *gate) { return gate != nullptr && (*gate == 0 || halted(gate)); }
 
 // Makes one wave's LDS writes visible to the other lanes of the same wave.
 __device__ inline void wave_lds_sync()

@@ -95,6 +95,7 @@ struct PairArgs
     int32_t n, ncl, cap;
     int32_t accumulate; // add to fnb instead of overwriting
     int32_t energy;     // also write epp (the last step of a call only)
+    const int32_t *status; // nullable: the call's status words (the kernel does nothing once halted)
     float rc, rc2lo, rc2hi;
     int32_t steric_mode;
     float steric_scale;
@@ -143,6 +144,31 @@ struct RigidArgs
     float half_skin2;
 };
 
+// The probe particle (SpringNetwork.cpp:156-182,749-766) on the device.
+struct ProbeDev
+{
+    float4 x;     // position, charge
+    float4 v;     // velocity
+    double4 f;    // force of the last evaluation (its viscosity included)
+    double e[4];  // steric, Coulomb, kinetic energy of the last evaluation
+};
+
+// The probe's interactions as the spring kernel's epilogue evaluates them:
+// one pair with every dynamic particle, no cutoff, no spring exclusion.
+struct ProbeArgs
+{
+    ProbeDev *state;       // nullptr: no probe
+    double *part;          // [spring blocks * 8] block sums: force on the probe (x, y, z), steric E, Coulomb E
+    int32_t steric, elec;  // probe terms on
+    int32_t steric_mode;
+    float steric_scale;
+    float radius, eps;     // probe radius and epsilon in jprm's form (square roots for Amber / Lewitt)
+    float mass;
+    double ke, kf;         // Coulomb energy / force prefactors incl. scale and 1/dielectric
+    int32_t ddd;           // eps(r) = eps r
+    const int32_t *status; // the call's status words
+};
+
 struct SpringArgs
 {
     const float4 *pos;
@@ -165,7 +191,8 @@ struct SpringArgs
     GridView gpot, gden; // electrostatic potential field, density field
     double e_field;      // potential * charge -> kJ/mol (electrostatic_field.hpp:15-23)
     double4 *eps;        // [N] per-particle spring, kinetic, imp, field energy (energy steps)
-    int32_t *status;     // [0] rebuild flag, [1] nonfinite min id, [2] overflow
+    int32_t *status;     // the call's status words (bs_device.hpp)
+    int32_t *steps_done; // integrate mode: incremented once per completed step
     int32_t n;
     int32_t spring_enable;
     float spring_scale;
@@ -177,6 +204,7 @@ struct SpringArgs
     float half_skin2;    // (list_skin/2)^2, conservative
     int32_t integrate;
     int32_t energy;      // also write eps (the last step of a call only)
+    ProbeArgs probe;
     Consts k;
 };
 
@@ -295,7 +323,11 @@ void launch_spring(const SpringArgs &a, int32_t group, int32_t nblocks, hipStrea
 // every system's segments in order; out_sys[nsys * 6], out_total[6].
 void launch_energies(const double4 *epp, const double4 *eps, const int32_t *seg_start, int32_t nseg,
                      const int32_t *sys_seg, int32_t nsys, double *seg_part, double *out_sys, double *out_total,
-                     hipStream_t s);
+                     const ProbeDev *probe, hipStream_t s);
+// The probe's force from the spring kernel's nparts block sums, its
+// viscosity and (integrate) its Euler step and kinetic energy.
+void launch_probe(const ProbeArgs &p, int32_t nparts, float dt, int32_t visc_enable, float visc, double c_kinetic,
+                  int32_t integrate, hipStream_t s);
 void launch_prestep(int32_t *status, int32_t *step_counter, int32_t interval, hipStream_t s);
 void launch_finish_rebuild(const float4 *pos, float4 *xref, int32_t n, int32_t *status, int32_t *counters,
                            hipStream_t s);

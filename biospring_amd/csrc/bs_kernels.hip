@@ -348,6 +348,74 @@ __device__ inline int64_t grid_cell(const GridView &g, float x, float y, float z
     return (q[0] * g.shape[1] + q[1]) * g.shape[2] + q[2];
 }
 
+// One particle's interactions with the probe (Particle::addStericProbeForce /
+// addElectrostaticProbeForce, Particle.cpp:434-465), in FP64: the force on
+// the particle goes to f; the pair energies (in full) to es, ee.  The
+// formulas are the pair kernel's (energy/steric.hpp, electrostatic.hpp) with
+// the probe as the first particle; no cutoff and no spring exclusion.
+__device__ inline void probe_pair(const ProbeArgs &p, const float4 q, float4 pi, float4 ri, double cs, double f[3],
+                                  double &es, double &ee)
+{
+    const double dx = (double)q.x - (double)pi.x, dy = (double)q.y - (double)pi.y, dz = (double)q.z - (double)pi.z;
+    const double r = sqrt(dx * dx + dy * dy + dz * dz);
+    const double inv = r > 1e-40 ? 1.0 / r : 0.0; // a zero vector stays zero (Vector3f::normalize)
+    double fm = 0.0;
+    if (p.steric)
+    {
+        double e = 0.0, m = 0.0;
+        if (p.steric_mode == 0)
+        { // linear: overlap of the raw radii
+            const double dv = r - ((double)p.radius + (double)ri.x);
+            if (!(dv > 0.0))
+            {
+                e = 0.5 * dv * dv;
+                m = -fabs(dv) * cs;
+            }
+        }
+        else if (r >= 0.1)
+        {
+            const bool prod = p.steric_mode == 3; // Zacharias: raw products; else square-root factors
+            const double eij = (double)p.eps * (double)ri.y;
+            const double R = (double)p.radius * (double)ri.x;
+            const double sr = R * inv, s2 = sr * sr, s6 = s2 * s2 * s2;
+            if (p.steric_mode == 1)
+            { // Amber 12-6
+                e = eij * s6 * s6 - 2.0 * eij * s6;
+                m = (-12.0 * eij * s6 * s6 + 12.0 * eij * s6) * inv * cs;
+            }
+            else if (p.steric_mode == 2)
+            { // Lewitt 8-6
+                e = 3.0 * eij * s6 * s2 - 4.0 * eij * s6;
+                m = (-24.0 * eij * s6 * s2 + 24.0 * eij * s6) * inv * cs;
+            }
+            else if (prod)
+            { // Zacharias 8-6
+                e = eij * s6 * s2 - eij * s6;
+                m = (-8.0 * eij * s6 * s2 + 6.0 * eij * s6) * inv * cs;
+            }
+        }
+        es += (double)p.steric_scale * e;
+        fm += (double)p.steric_scale * m;
+    }
+    if (p.elec && r >= 0.001)
+    {
+        const double qq = (double)q.w * (double)pi.w;
+        if (p.ddd)
+        { // eps(r) = eps r
+            ee += p.ke * qq * inv * inv;
+            fm += -2.0 * p.kf * qq * inv * inv * inv;
+        }
+        else
+        {
+            ee += p.ke * qq * inv;
+            fm += -p.kf * qq * inv * inv;
+        }
+    }
+    f[0] += dx * inv * fm;
+    f[1] += dy * inv * fm;
+    f[2] += dz * inv * fm;
+}
+
 // ---------------------------------------------------------------------------
 // Springs (CSR gather, G lanes per row, 4 edges in flight per lane) fused with
 // viscosity, external and IMPALA forces and the viscous semi-implicit Euler
@@ -359,8 +427,21 @@ __global__ void __launch_bounds__(BLOCK) k_spring(SpringArgs a)
     const int32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     const int32_t row = tid / G;
     const int32_t lane = tid % G;
+    if (halted(a.status))
+    { // a stopped call: the state is carried over to the other position buffer
+        if (INTEGRATE && row < a.n && lane == 0)
+        {
+            a.pos_out[row] = a.pos[row];
+            if (COMP)
+                a.plo_out[row] = a.plo[row];
+        }
+        return;
+    }
+    if (INTEGRATE && tid == 0 && a.steps_done)
+        atomicAdd(a.steps_done, 1);
     double espr = 0.0, eke = 0.0, eimp = 0.0, efld = 0.0;
     double fx = 0.0, fy = 0.0, fz = 0.0;
+    double pf[3] = {0.0, 0.0, 0.0}, pes = 0.0, pee = 0.0; // force on the probe, probe pair energies
     const bool active = row < a.n;
     float4 pi = make_float4(0.f, 0.f, 0.f, 0.f), li = make_float4(0.f, 0.f, 0.f, 0.f);
     uint32_t fli = PF_STATIC;
@@ -447,6 +528,17 @@ __global__ void __launch_bounds__(BLOCK) k_spring(SpringArgs a)
             sx += fe.x;
             sy += fe.y;
             sz += fe.z;
+        }
+        if (a.probe.state)
+        { // the probe: the particle's share here, the opposite on the probe
+            double f[3] = {0.0, 0.0, 0.0};
+            probe_pair(a.probe, a.probe.state->x, pi, a.jprm[row], a.k.c_spring, f, pes, pee);
+            sx += f[0];
+            sy += f[1];
+            sz += f[2];
+            pf[0] = -f[0];
+            pf[1] = -f[1];
+            pf[2] = -f[2];
         }
         const float4 ip = a.iprm[row];
         // field forces: nearest-lower-cell lookup, no interpolation
@@ -556,6 +648,66 @@ __global__ void __launch_bounds__(BLOCK) k_spring(SpringArgs a)
     }
     if (a.energy && active && lane == 0)
         a.eps[row] = make_double4(espr, eke, eimp, efld);
+    if (a.probe.state)
+    { // block sums of the probe's force and pair energies, in a fixed order
+        __shared__ double psh[BLOCK / 64];
+        const double v[5] = {pf[0], pf[1], pf[2], pes, pee};
+        for (int k = 0; k < 5; ++k)
+        {
+            const double t = block_sum(v[k], psh);
+            if (threadIdx.x == 0)
+                a.probe.part[8 * blockIdx.x + k] = t;
+        }
+    }
+}
+
+// The probe's step (SpringNetwork.cpp:156-182): its force is the sum of the
+// spring kernel's block sums (in block order), minus gamma v when viscosity
+// is on, rounded to float like a particle's; then the semi-implicit Euler
+// step and its kinetic energy (Particle.cpp:109-128).  One workgroup.
+__global__ void __launch_bounds__(BLOCK) k_probe(ProbeArgs p, int32_t nparts, float dt, int32_t visc_enable,
+                                                 float visc, double ck, int32_t integrate)
+{
+    if (halted(p.status))
+        return;
+    __shared__ double sh[BLOCK / 64];
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int32_t b = threadIdx.x; b < nparts; b += blockDim.x)
+        for (int k = 0; k < 5; ++k)
+            acc[k] += p.part[8 * b + k];
+    for (int k = 0; k < 5; ++k)
+        acc[k] = block_sum(acc[k], sh);
+    if (threadIdx.x != 0)
+        return;
+    ProbeDev &st = *p.state;
+    float4 v = st.v;
+    float fx = (float)acc[0], fy = (float)acc[1], fz = (float)acc[2];
+    if (visc_enable)
+    {
+        fx -= v.x * visc;
+        fy -= v.y * visc;
+        fz -= v.z * visc;
+    }
+    st.f = make_double4(fx, fy, fz, 0.0);
+    st.e[0] = acc[3];
+    st.e[1] = acc[4];
+    st.e[2] = 0.0;
+    if (!integrate)
+        return;
+    const float m = p.mass;
+    if (m > 0.0f)
+    {
+        v.x = v.x + (fx / m) * dt;
+        v.y = v.y + (fy / m) * dt;
+        v.z = v.z + (fz / m) * dt;
+    }
+    st.e[2] = 0.5 * (double)m * ((double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z) * ck;
+    float4 x = st.x;
+    x.x = x.x + v.x * dt;
+    x.y = x.y + v.y * dt;
+    x.z = x.z + v.z * dt;
+    st.x = x;
+    st.v = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -585,6 +737,8 @@ __device__ inline void qnormalize(double q[4])
 
 __global__ void __launch_bounds__(BLOCK) k_rigid_solve(RigidArgs a)
 {
+    if (halted(a.status))
+        return;
     __shared__ double sh[BLOCK / 64];
     const int32_t b = blockIdx.x;
     RigidBody &B = a.bodies[b];
@@ -635,7 +789,7 @@ __global__ void __launch_bounds__(BLOCK) k_rigid_solve(RigidArgs a)
 __global__ void k_rigid_place(RigidArgs a)
 {
     const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= a.nr)
+    if (s >= a.nr || halted(a.status))
         return;
     const int32_t p = a.slot_p[s];
     const RigidBody &B = a.bodies[a.slot_b[s]];
@@ -733,7 +887,8 @@ __global__ void __launch_bounds__(BLOCK) k_seg_energies(const double4 *epp, cons
 }
 
 __global__ void __launch_bounds__(BLOCK) k_sys_energies(const double *seg_part, const int32_t *sys_seg,
-                                                        int32_t nsys, double *out_sys, double *out_total)
+                                                        int32_t nsys, double *out_sys, double *out_total,
+                                                        const ProbeDev *probe)
 {
     for (int32_t s = threadIdx.x; s < nsys; s += blockDim.x)
         for (int k = 0; k < 6; ++k)
@@ -741,6 +896,10 @@ __global__ void __launch_bounds__(BLOCK) k_sys_energies(const double *seg_part, 
             double t = 0.0;
             for (int32_t g = sys_seg[s]; g < sys_seg[s + 1]; ++g)
                 t += seg_part[6 * g + k];
+            // the probe (a single network only): its pair energies and its
+            // kinetic energy (SpringNetwork.cpp:163-166,180)
+            if (probe && s == 0)
+                t += k == 2 ? probe->e[0] : k == 1 ? probe->e[1] : k == 3 ? probe->e[2] : 0.0;
             out_sys[6 * s + k] = t;
         }
     __syncthreads();
@@ -1209,6 +1368,13 @@ __global__ void k_topo_sort_long(const int32_t *offs, int32_t n, int32_t *jout, 
 
 __global__ void k_prestep(int32_t *status, int32_t *step, int32_t interval)
 {
+    if (halted(status))
+        return;
+    if (status[1] != 0x7fffffff)
+    { // the step before produced a non-finite position: stop here
+        status[5] = 1;
+        return;
+    }
     if (interval > 0 && *step > 0 && (*step % interval) == 0)
         status[0] = 1;
     *step += 1;
@@ -1216,7 +1382,7 @@ __global__ void k_prestep(int32_t *status, int32_t *step, int32_t interval)
 
 __global__ void k_copy_xref(const float4 *pos, float4 *xref, int32_t n, const int32_t *status)
 {
-    if (status[0] == 0)
+    if (status[0] == 0 || halted(status))
         return;
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n)
@@ -1225,7 +1391,7 @@ __global__ void k_copy_xref(const float4 *pos, float4 *xref, int32_t n, const in
 
 __global__ void k_clear_rebuild(int32_t *status, int32_t *counters)
 {
-    if (status[0] != 0)
+    if (status[0] != 0 && !halted(status))
     {
         status[0] = 0;
         counters[1] += 1;
@@ -1368,11 +1534,17 @@ void launch_drift_check(const float4 *pos, const float4 *xref, int32_t n, float 
 
 void launch_energies(const double4 *epp, const double4 *eps, const int32_t *seg_start, int32_t nseg,
                      const int32_t *sys_seg, int32_t nsys, double *seg_part, double *out_sys, double *out_total,
-                     hipStream_t s)
+                     const ProbeDev *probe, hipStream_t s)
 {
     if (nseg > 0)
         k_seg_energies<<<nseg, BLOCK, 0, s>>>(epp, eps, seg_start, seg_part);
-    k_sys_energies<<<1, BLOCK, 0, s>>>(seg_part, sys_seg, nsys, out_sys, out_total);
+    k_sys_energies<<<1, BLOCK, 0, s>>>(seg_part, sys_seg, nsys, out_sys, out_total, probe);
+}
+
+void launch_probe(const ProbeArgs &p, int32_t nparts, float dt, int32_t visc_enable, float visc, double c_kinetic,
+                  int32_t integrate, hipStream_t s)
+{
+    k_probe<<<1, BLOCK, 0, s>>>(p, nparts, dt, visc_enable, visc, c_kinetic, integrate);
 }
 
 void launch_prestep(int32_t *status, int32_t *step, int32_t interval, hipStream_t s)

@@ -151,13 +151,15 @@ class NcFile
             {
                 ++nrec;
                 only = &v;
-                recsize += v.vsize;
+                if (__builtin_add_overflow(recsize, v.vsize, &recsize))
+                    return fail("malformed NetCDF header (record size)");
             }
         if (nrec == 1 && type_size(only->type) < 4)
         {
             recsize = type_size(only->type);
             for (size_t k = 1; k < only->dimids.size(); ++k)
-                recsize *= dims[only->dimids[k]].second;
+                if (__builtin_mul_overflow(recsize, dims[only->dimids[k]].second, &recsize))
+                    return fail("malformed NetCDF header (record size)");
         }
         return true;
     }
@@ -199,12 +201,20 @@ class NcFile
         const uint64_t nrec = is_record(v) ? numrecs : 1;
         if (nrec != 0 && per > buf.size() / nrec)
             return fail(big);
+        // per * ts <= 8 * size (both factors bounded above) cannot wrap; the
+        // record offsets come from the file, so every product and sum is
+        // checked before it is used
+        const uint64_t size = buf.size(), bytes = per * (uint64_t)ts;
+        const std::string beyond = format("variable \"%s\": data beyond the end of the file", v.name.c_str());
+        if (bytes > size)
+            return fail(beyond);
         out.resize((size_t)(per * nrec));
         for (uint64_t r = 0; r < nrec; ++r)
         {
-            const uint64_t at = v.begin + r * (is_record(v) ? recsize : 0);
-            if (at + per * ts > buf.size())
-                return fail(format("variable \"%s\": data beyond the end of the file", v.name.c_str()));
+            uint64_t off = 0, at = 0;
+            if (__builtin_mul_overflow(r, is_record(v) ? recsize : 0, &off) ||
+                __builtin_add_overflow(v.begin, off, &at) || at > size || bytes > size - at)
+                return fail(beyond);
             const uint8_t *p = buf.data() + at;
             for (uint64_t e = 0; e < per; ++e, p += ts)
                 out[(size_t)(r * per + e)] = decode(v.type, p);
@@ -257,7 +267,7 @@ class NcFile
     std::string name()
     {
         const uint64_t len = count();
-        if (!ok_ || pos_ + len > buf.size())
+        if (!ok_ || len > buf.size() - pos_)
         {
             ok_ = false;
             return {};
@@ -279,8 +289,9 @@ class NcFile
             name();
             const int t = (int)u32();
             const uint64_t ne = count();
-            const uint64_t bytes = ne * (uint64_t)type_size(t);
-            if (type_size(t) == 0 || pos_ + bytes > buf.size())
+            uint64_t bytes = 0;
+            if (!ok_ || type_size(t) == 0 || __builtin_mul_overflow(ne, (uint64_t)type_size(t), &bytes) ||
+                bytes > buf.size() - pos_)
                 return fail("malformed NetCDF header (attribute)");
             pos_ += (size_t)((bytes + 3) / 4 * 4);
         }

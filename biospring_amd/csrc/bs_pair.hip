@@ -205,15 +205,17 @@ __global__ void __launch_bounds__(BLOCK) k_cluster_list(ListArgs a)
                     {
                         j = a.cell_idx[idx];
                         float4 pj = a.pos[j];
+                        bool same_sys = true;
                         if (a.shift)
-                        {
+                        { // independent systems never pair, however far they move from their setup places
                             const float4 o = a.shift[j];
                             pj.x += o.x;
                             pj.y += o.y;
                             pj.z += o.z;
+                            same_sys = fbits(o.w) == tag0;
                         }
                         const uint32_t fj = fbits(a.jprm[j].w);
-                        if (a.j_mask == 0u || (fj & a.j_mask))
+                        if (same_sys && (a.j_mask == 0u || (fj & a.j_mask)))
                         {
 #pragma unroll
                             for (int k = 0; k < CLUSTER; ++k)
@@ -451,6 +453,8 @@ __device__ inline void interact(bool masked, float4 pi, float4 ri, float4 pj, fl
 template <int SMODE, int EMODE, int HYDRO, bool ENERGY>
 __global__ void __launch_bounds__(BLOCK) k_pair(PairArgs a)
 {
+    if (a.status && halted(a.status))
+        return;
     const int lane = threadIdx.x & 63;
     // wave-uniform, made visible to the compiler so that the list walk runs
     // on scalar registers
@@ -826,6 +830,8 @@ void launch_list_build(const ListArgs &a, hipStream_t s)
 template <int SMODE, int EMODE, int HYDRO, bool ENERGY>
 __global__ void __launch_bounds__(BLOCK) k_pair_q(PairArgs a)
 {
+    if (a.status && halted(a.status))
+        return;
     __shared__ float4 qpos[WAVES][64], qprm[WAVES][64];
     __shared__ uint8_t qidx[WAVES][CLUSTER][64];
     __shared__ int32_t qcnt[WAVES][CLUSTER];
@@ -980,6 +986,8 @@ __global__ void __launch_bounds__(BLOCK) k_rows(const uint32_t *cl, const int32_
 template <int SMODE, int EMODE, int HYDRO, bool ENERGY>
 __global__ void __launch_bounds__(BLOCK) k_pair_rows(PairArgs a)
 {
+    if (a.status && halted(a.status))
+        return;
     const int lane = threadIdx.x & 63;
     const int32_t b = blockIdx.x * WAVES + (threadIdx.x >> 6);
     const int32_t nrb = (a.n + 63) >> 6;

@@ -40,6 +40,16 @@ static inline void bs_params_fill_defaults(bs_params *p)
     p->densitygrid_scale = 1.0f;
     p->graph_steps = 0;
     p->pair_layout = BS_PAIR_CLUSTERS;
+    p->probe_enable = 0;          /* probe.*, Configuration.hpp:186-195 */
+    p->probe_enable_electrostatic = 0;
+    p->probe_enable_steric = 0;
+    p->probe_x = 1.0f;
+    p->probe_y = 1.0f;
+    p->probe_z = 1.0f;
+    p->probe_mass = 1.0f;
+    p->probe_epsilon = 1.0f;
+    p->probe_radius = 1.0f;
+    p->probe_charge = 0.0f;
 }
 
 #endif

@@ -550,7 +550,7 @@ int TrajectorySet::acquire()
     return s;
 }
 
-void TrajectorySet::submit(int s, const TrajFrame &f, std::function<void()> wait, bool all)
+void TrajectorySet::submit(int s, const TrajFrame &f, std::function<std::string()> wait, bool all)
 {
     {
         std::lock_guard<std::mutex> lk(mu_);
@@ -573,7 +573,29 @@ void TrajectorySet::run()
             q_.pop_front();
             busy_ = true;
         }
-        it.wait();
+        std::string e;
+        try
+        {
+            e = it.wait();
+        }
+        catch (const std::exception &x)
+        {
+            e = std::string("frame copy failed: ") + x.what();
+        }
+        const bool stopped = it.f.status && (it.f.status[2] != 0 || it.f.status[5] != 0 ||
+                                             it.f.status[1] != INT32_MAX);
+        if (!e.empty() || stopped)
+        { // not written: a failed copy (reported), or a frame after the call stopped
+            {
+                std::lock_guard<std::mutex> lk(mu_);
+                if (!e.empty() && err_.empty())
+                    err_ = e;
+                free_[(size_t)it.slot] = true;
+                busy_ = false;
+            }
+            cv_.notify_all();
+            continue;
+        }
         const double t =
             std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
         // steps per second between the completions of consecutive frames
@@ -582,7 +604,6 @@ void TrajectorySet::run()
                              : 0.0;
         last_nb_ = it.f.nbiter;
         last_t_ = t;
-        std::string e;
         for (auto &w : writers_)
             if (it.all || it.f.nbiter % w->frequency == 0)
             {

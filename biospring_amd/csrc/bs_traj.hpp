@@ -59,6 +59,10 @@ struct TrajFrame
     const double *energies = nullptr; // [6]: spring, electrostatic, steric, kinetic, IMP, hydrophobic
     const float *x3 = nullptr;        // n x 3, particle order, A
     double framerate = 0.0;     // steps per second before this frame (0 for the first)
+    // nullable: the engine's status words when the frame was taken; a frame
+    // taken after its call stopped (an overflowed list, or a non-finite
+    // position in an earlier step: [2], [5], [1] != INT32_MAX) is not written
+    const int32_t *status = nullptr;
 };
 
 class TrajWriter
@@ -101,7 +105,9 @@ class TrajectorySet
     int acquire();
     // Queues frame f held in slot s; `wait` returns once the slot's copy has
     // landed.  all = true writes to every writer (writeNextStepNow).
-    void submit(int s, const TrajFrame &f, std::function<void()> wait, bool all);
+    // The callback returns an error message, empty on success; a frame whose
+    // copy failed is not written and the error is kept (error()).
+    void submit(int s, const TrajFrame &f, std::function<std::string()> wait, bool all);
     // Returns a slot without writing it (a frame whose copy failed).
     void release(int s);
     // Waits until every queued frame is written; false if a write failed.
@@ -115,7 +121,7 @@ class TrajectorySet
     {
         int slot;
         TrajFrame f;
-        std::function<void()> wait;
+        std::function<std::string()> wait;
         bool all;
     };
     void run();

@@ -215,12 +215,16 @@ hipError_t hipGraphDestroy(hipGraph_t) { return hipSuccess; }
 namespace bs
 {
 
+// The kernels' stop rule (bs_device.hpp): a list overflowed, or a step
+// before this one produced a non-finite position.
+static bool halted(const int32_t *status) { return status[2] != 0 || status[5] != 0; }
+
 static bool gated_off(const int32_t *gate)
 {
     if (!gate)
         return false;
-    need(gate, 4, "gate");
-    return *gate == 0;
+    need(gate, 32, "gate");
+    return *gate == 0 || halted(gate);
 }
 
 static uint32_t fbits(float f)
@@ -614,6 +618,11 @@ void launch_spring(const SpringArgs &a, int32_t group, int32_t nb, hipStream_t)
         need(a.fout, sizeof(float4) * n, "spring: fout");
     if ((int64_t)nb * BLOCK < (int64_t)n * group)
         hc_fail("spring: grid too small");
+    if (a.probe.state)
+    {
+        need(a.probe.state, sizeof(ProbeDev), "spring: probe state");
+        need(a.probe.part, sizeof(double) * 8 * nb, "spring: probe block sums");
+    }
     const int32_t ne = a.sp_ptr[n];
     if (ne > 0)
     {
@@ -624,6 +633,15 @@ void launch_spring(const SpringArgs &a, int32_t group, int32_t nb, hipStream_t)
     for (int32_t e = 0; e < ne; ++e)
         if ((a.sp_j[e] & 0x7fffffff) >= n)
             hc_fail("spring: edge index out of range");
+    need(a.status, 32, "spring: status");
+    if (a.integrate && halted(a.status))
+    { // a stopped call carries every position over
+        for (int32_t i = 0; i < n; ++i)
+            const_cast<float4 *>(a.pos_out)[i] = a.pos[i];
+        return;
+    }
+    if (a.integrate && a.steps_done)
+        *a.steps_done += 1;
     if (a.integrate)
         for (int32_t i = 0; i < n; ++i)
             if (!(fbits(a.jprm[i].w) & (PF_STATIC | PF_RIGID)))
@@ -897,8 +915,14 @@ void launch_drift_check(const float4 *pos, const float4 *xref, int32_t n, float,
 
 void launch_energies(const double4 *epp, const double4 *eps, const int32_t *seg_start, int32_t nseg,
                      const int32_t *sys_seg, int32_t nsys, double *seg_part, double *out_sys, double *out_total,
-                     hipStream_t)
+                     const ProbeDev *probe, hipStream_t)
 {
+    if (probe)
+    {
+        need(probe, sizeof(ProbeDev), "energies: probe");
+        if (nsys != 1)
+            hc_fail("energies: the probe with a batch of systems");
+    }
     need(seg_start, sizeof(int32_t) * (nseg + 1), "energies: seg_start");
     need(sys_seg, sizeof(int32_t) * (nsys + 1), "energies: sys_seg");
     need(seg_part, sizeof(double) * 6 * std::max(nseg, 1), "energies: seg_part");
@@ -917,6 +941,12 @@ void launch_energies(const double4 *epp, const double4 *eps, const int32_t *seg_
         out_sys[k] = 0.0;
     for (int k = 0; k < 6; ++k)
         out_total[k] = 0.0;
+}
+
+void launch_probe(const ProbeArgs &p, int32_t nparts, float, int32_t, float, double, int32_t, hipStream_t)
+{
+    need(p.state, sizeof(ProbeDev), "probe: state");
+    need(p.part, sizeof(double) * 8 * std::max(nparts, 1), "probe: block sums");
 }
 
 void launch_rigid(const RigidArgs &a, hipStream_t)
@@ -974,8 +1004,15 @@ void launch_centroid(const float4 *pos, int32_t n, double *partial, int32_t nblo
 
 void launch_prestep(int32_t *status, int32_t *step, int32_t interval, hipStream_t)
 {
-    need(status, 16, "prestep: status");
+    need(status, 32, "prestep: status");
     need(step, 4, "prestep: step");
+    if (halted(status))
+        return;
+    if (status[1] != 0x7fffffff)
+    {
+        status[5] = 1;
+        return;
+    }
     if (interval > 0 && *step > 0 && *step % interval == 0)
         status[0] = 1;
     *step += 1;
@@ -985,16 +1022,16 @@ void launch_finish_rebuild(const float4 *pos, float4 *xref, int32_t n, int32_t *
 {
     need(pos, sizeof(float4) * n, "xref: pos");
     need(xref, sizeof(float4) * n, "xref: xref");
-    need(status, 16, "xref: status");
-    if (status[0])
+    need(status, 32, "xref: status");
+    if (status[0] && !halted(status))
         std::memcpy(xref, pos, sizeof(float4) * n);
 }
 
 void launch_clear_rebuild(int32_t *status, int32_t *counters, hipStream_t)
 {
-    need(status, 16, "clear: status");
+    need(status, 32, "clear: status");
     need(counters, 8, "clear: counters");
-    if (status[0])
+    if (status[0] && !halted(status))
     {
         status[0] = 0;
         counters[1] += 1;

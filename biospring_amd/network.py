@@ -295,6 +295,14 @@ class SpringNetwork:
         check(self._L.bs_add_rigid_body(self._h, a.size, ptr(a, ctypes.c_int64)), self._h, "bs_add_rigid_body")
         return self.numRigidBodies()
 
+    def getProbeParticle(self):
+        """The probe particle (SpringNetwork::getProbeParticle): position,
+        velocity and the force of its last step, each (3,) float32."""
+        x, v, f = (np.zeros(3, np.float32) for _ in range(3))
+        check(self._L.bs_get_probe(self._h, ptr(x, ctypes.c_float), ptr(v, ctypes.c_float), ptr(f, ctypes.c_float)),
+              self._h, "bs_get_probe")
+        return x, v, f
+
     def numRigidBodies(self):
         return int(self._L.bs_num_rigid_bodies(self._h))
 

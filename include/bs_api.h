@@ -142,6 +142,26 @@ typedef struct bs_params {
        that see it, so that lanes evaluate listed pairs at the cluster
        layout's load count.  Forces agree to FP64 rounding in every layout. */
     int32_t pair_layout;
+
+    /* The probe particle (probe.*, Configuration.hpp:186-195): one extra
+       particle placed at setup (SpringNetwork::_setupProbe,
+       SpringNetwork.cpp:749-766) that interacts with every dynamic particle,
+       with no cutoff and no spring exclusion, through the steric and/or
+       Coulomb terms of the force field (Particle::addStericProbeForce /
+       addElectrostaticProbeForce, Particle.cpp:434-465), is never a
+       neighbour in the pair searches (SpringNetwork.cpp:816-824,
+       Particle.cpp:200,280,379), and is moved once per step by the viscous
+       Euler step (SpringNetwork.cpp:156-182).  Its state: bs_get_probe. */
+    int32_t probe_enable;               /* probe.enable, default 0 */
+    int32_t probe_enable_electrostatic; /* probe.enableelectrostatic, default 0 */
+    int32_t probe_enable_steric;        /* probe.enablesteric, default 0 */
+    float probe_x;                      /* probe.x (A), default 1 */
+    float probe_y;                      /* probe.y (A), default 1 */
+    float probe_z;                      /* probe.z (A), default 1 */
+    float probe_mass;                   /* probe.mass (Da), default 1 */
+    float probe_epsilon;                /* probe.epsilon (kJ/mol), default 1 */
+    float probe_radius;                 /* probe.radius (A), default 1 */
+    float probe_charge;                 /* probe.charge (e), default 0 */
 } bs_params;
 
 typedef struct bs_handle bs_handle;
@@ -313,6 +333,22 @@ int bs_get_cells(bs_handle *h, int32_t kind, int64_t *ncells, int64_t *nmembers,
  * distance != 0 for electrostatic/hydrophobic.  Written as (min,max) id pairs
  * sorted ascending.  *count receives the number; ij may be NULL to query it. */
 int bs_get_pairs(bs_handle *h, int32_t kind, uint32_t *ij, int64_t cap, int64_t *count);
+
+/* ---- the probe particle ------------------------------------------------------ */
+
+/* The probe particle (probe_enable): SpringNetwork::getProbeParticle
+ * (SpringNetwork.h:277).  Position, velocity and the force of the last
+ * evaluation (its pair forces and viscosity, SpringNetwork.cpp:158-177),
+ * 3 floats each, every pointer nullable.  The first setup after
+ * bs_set_particles places the probe at probe_x/y/z at rest
+ * (SpringNetwork::_setupProbe, SpringNetwork.cpp:749-766); every bs_step step
+ * then moves it once, after the particles' forces are taken
+ * (SpringNetwork.cpp:156-182).  bs_compute_forces includes the probe's pair
+ * forces on the particles and its pair energies, and leaves the probe in
+ * place (the reference's computeForces also moves it).  Not with a batch of
+ * systems (BS_ERR_ARG at setup) nor in the adjoint (BS_ERR_ARG).
+ * BS_ERR_STATE when the probe is not enabled. */
+int bs_get_probe(bs_handle *h, float *x3, float *v3, float *f3);
 
 /* ---- rigid bodies ------------------------------------------------------------- */
 

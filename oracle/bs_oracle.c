@@ -625,6 +625,11 @@ typedef struct orc {
     /* grids: [BS_GRID_POTENTIAL], [BS_GRID_DENSITY] (PotentialGrid.hpp) */
     struct ogrid *grid[2];
     float *burying;         /* ParticleProperty::_burying, default 1 */
+    /* the probe particle (SpringNetwork.cpp:156-182,749-766): float and
+     * double state and the force of the last evaluation; pdec is the float
+     * rounding of the double position (pair decisions, like xdec) */
+    float pxf[3], pvf[3], pff[3], pdec[3];
+    double pxd[3], pvd[3], pfd[3];
 } orc;
 
 /* A dense grid as PotentialGrid holds it after loading: per cell the scalar
@@ -687,6 +692,20 @@ static int is_bonded(const orc *o, int64_t i, uint32_t j)
     return lo < o->bptr[i + 1] && o->bidx[lo] == j;
 }
 
+/* SpringNetwork::_setupProbe (SpringNetwork.cpp:749-766): the probe starts at
+ * probe.x/y/z at rest. */
+static void probe_reset(orc *o)
+{
+    const float x[3] = {o->p.probe_x, o->p.probe_y, o->p.probe_z};
+    for (int c = 0; c < 3; ++c) {
+        o->pxf[c] = x[c];
+        o->pxd[c] = x[c];
+        o->pdec[c] = x[c];
+        o->pvf[c] = o->pff[c] = 0.0f;
+        o->pvd[c] = o->pfd[c] = 0.0;
+    }
+}
+
 orc *orc_create(const bs_params *p, int64_t n, const float *x3, const float *v3, const float *mass,
                 const float *charge, const float *radius, const float *eps, const float *sasa,
                 const float *transfer, const float *hydro, const uint8_t *flags, int64_t m,
@@ -695,6 +714,7 @@ orc *orc_create(const bs_params *p, int64_t n, const float *x3, const float *v3,
     orc *o = (orc *)calloc(1, sizeof(orc));
     o->p = *p;
     o->n = n;
+    probe_reset(o);
     o->xf = (float *)malloc(sizeof(float) * 3 * n);
     o->vf = (float *)calloc(3 * n, sizeof(float));
     o->ff = (float *)calloc(3 * n, sizeof(float));
@@ -843,7 +863,24 @@ void orc_destroy(orc *o)
 
 int64_t orc_num_springs(const orc *o) { return o->ns; }
 
-void orc_set_params(orc *o, const bs_params *p) { o->p = *p; }
+/* New parameters; a probe switched on here is placed as at setup. */
+void orc_set_params(orc *o, const bs_params *p)
+{
+    const int had_probe = o->p.probe_enable;
+    o->p = *p;
+    if (p->probe_enable && !had_probe)
+        probe_reset(o);
+}
+
+/* The probe's position, velocity and the force of the last evaluation. */
+void orc_probe_state(const orc *o, int mode, double *out9)
+{
+    for (int c = 0; c < 3; ++c) {
+        out9[c] = mode == ORC_F32 ? o->pxf[c] : o->pxd[c];
+        out9[3 + c] = mode == ORC_F32 ? o->pvf[c] : o->pvd[c];
+        out9[6 + c] = mode == ORC_F32 ? o->pff[c] : o->pfd[c];
+    }
+}
 
 void orc_set_state(orc *o, const float *x3, const float *v3)
 {
@@ -909,7 +946,9 @@ static inline int64_t cell_coord(double x, double mn, float r, int64_t na)
 static void build_cells(const orc *o, int kind, const float *pos, cells_t *cl)
 {
     const int64_t n = o->n;
-    /* Box(container) over every particle, box.hpp:37-69 */
+    /* Box(container) over every particle, box.hpp:37-69; the container holds
+     * the probe too (it is appended to the particles, SpringNetwork.cpp:763-764,
+     * and only left out of the cells, nsearch.hpp:363-365) */
     double mn[3], mx[3];
     for (int c = 0; c < 3; ++c)
         mn[c] = mx[c] = pos[c];
@@ -921,6 +960,15 @@ static void build_cells(const orc *o, int kind, const float *pos, cells_t *cl)
             if (v > mx[c])
                 mx[c] = v;
         }
+    if (o->p.probe_enable) {
+        const float *pp = pos == o->xf ? o->pxf : o->pdec;
+        for (int c = 0; c < 3; ++c) {
+            if ((double)pp[c] < mn[c])
+                mn[c] = pp[c];
+            if ((double)pp[c] > mx[c])
+                mx[c] = pp[c];
+        }
+    }
     cl->cutoff = kind_cutoff(o, kind);
     cl->radius = cl->cutoff + o->p.neighborskin;
     for (int c = 0; c < 3; ++c) {
@@ -1035,6 +1083,8 @@ static const float *decision_positions(orc *o, int mode)
         return o->xf;
     for (int64_t i = 0; i < 3 * o->n; ++i)
         o->xdec[i] = (float)o->xd[i];
+    for (int c = 0; c < 3; ++c)
+        o->pdec[c] = (float)o->pxd[c];
     return o->xdec;
 }
 
@@ -1280,6 +1330,123 @@ static inline void pair_eval_d(const orc *o, int kind, int64_t i, int64_t j, dou
     }
 }
 
+/* ---------------------------------------------------------------------------
+ * The probe particle, SpringNetwork.cpp:154-182.  After the pair terms and
+ * the per-particle energy sums: for every dynamic particle in order, its
+ * steric then its Coulomb interaction with the probe (Particle.cpp:434-465;
+ * no cutoff, no spring exclusion, the pair's full energy to the totals), the
+ * force on the particle and its opposite on the probe; then the probe's
+ * viscosity.  The probe's Euler step (probe_integrate_*) follows in a step.
+ * ------------------------------------------------------------------------- */
+static void probe_forces_f(orc *o, float *F, float *e_ster, float *e_elec)
+{
+    const bs_params *p = &o->p;
+    float *pf = o->pff;
+    pf[0] = pf[1] = pf[2] = 0.0f; /* resetForce */
+    for (int64_t t = 0; t < o->ndyn; ++t) {
+        const int64_t i = o->dyn[t];
+        for (int term = 0; term < 2; ++term) {
+            if (term == 0 ? !p->probe_enable_steric : !p->probe_enable_electrostatic)
+                continue;
+            float f[3] = {o->pxf[0] - o->xf[3 * i], o->pxf[1] - o->xf[3 * i + 1], o->pxf[2] - o->xf[3 * i + 2]};
+            const float d = norm_f(f[0], f[1], f[2]);
+            float e, fm;
+            if (term == 0) {
+                e = p->steric_gridscale *
+                    steric_energy_f(p->steric_mode, p->probe_radius, o->radius[i], p->probe_epsilon, o->eps[i], d);
+                fm = p->steric_gridscale *
+                     steric_fm_f(p->steric_mode, p->probe_radius, o->radius[i], p->probe_epsilon, o->eps[i], d);
+            } else {
+                e = p->electrostatic_scale * elec_energy_f(p->probe_charge, o->charge[i], d,
+                                                           p->electrostatic_dielectric, p->dielectric_model);
+                fm = p->electrostatic_scale * elec_fm_f(p->probe_charge, o->charge[i], d,
+                                                        p->electrostatic_dielectric, p->dielectric_model);
+            }
+            normalize_f(f);
+            for (int c = 0; c < 3; ++c) {
+                f[c] = f[c] * fm;
+                F[3 * i + c] = F[3 * i + c] + f[c];
+                pf[c] = pf[c] + -f[c];
+            }
+            if (term == 0)
+                *e_ster += e;
+            else
+                *e_elec += e;
+        }
+    }
+    if (p->viscosity_enable)
+        for (int c = 0; c < 3; ++c)
+            pf[c] = pf[c] - o->pvf[c] * p->viscosity_value;
+}
+
+static void probe_forces_d(orc *o, double *F, double *e_ster, double *e_elec)
+{
+    const bs_params *p = &o->p;
+    double *pf = o->pfd;
+    pf[0] = pf[1] = pf[2] = 0.0;
+    for (int64_t t = 0; t < o->ndyn; ++t) {
+        const int64_t i = o->dyn[t];
+        for (int term = 0; term < 2; ++term) {
+            if (term == 0 ? !p->probe_enable_steric : !p->probe_enable_electrostatic)
+                continue;
+            double f[3] = {o->pxd[0] - o->xd[3 * i], o->pxd[1] - o->xd[3 * i + 1], o->pxd[2] - o->xd[3 * i + 2]};
+            const double d = norm_d(f[0], f[1], f[2]);
+            double e, fm;
+            if (term == 0) {
+                e = p->steric_gridscale *
+                    steric_energy_d(p->steric_mode, p->probe_radius, o->radius[i], p->probe_epsilon, o->eps[i], d);
+                fm = p->steric_gridscale *
+                     steric_fm_d(p->steric_mode, p->probe_radius, o->radius[i], p->probe_epsilon, o->eps[i], d);
+            } else {
+                e = p->electrostatic_scale * elec_energy_d(p->probe_charge, o->charge[i], d,
+                                                           p->electrostatic_dielectric, p->dielectric_model);
+                fm = p->electrostatic_scale * elec_fm_d(p->probe_charge, o->charge[i], d,
+                                                        p->electrostatic_dielectric, p->dielectric_model);
+            }
+            normalize_d(f);
+            for (int c = 0; c < 3; ++c) {
+                F[3 * i + c] += f[c] * fm;
+                pf[c] -= f[c] * fm;
+            }
+            if (term == 0)
+                *e_ster += e;
+            else
+                *e_elec += e;
+        }
+    }
+    if (p->viscosity_enable)
+        for (int c = 0; c < 3; ++c)
+            pf[c] -= o->pvd[c] * (double)p->viscosity_value;
+}
+
+/* The probe's IntegrateEuler and its kinetic energy, added to the step's
+ * before the particles' (SpringNetwork.cpp:179-180, Particle.cpp:109-128). */
+static void probe_integrate_f(orc *o)
+{
+    const float dt = o->p.timestep, m = o->p.probe_mass;
+    float *x = o->pxf, *v = o->pvf, *f = o->pff;
+    if (m > 0.0f)
+        for (int c = 0; c < 3; ++c)
+            v[c] = v[c] + (f[c] / m) * dt;
+    const float vit = norm_f(v[0], v[1], v[2]);
+    const float ke = (float)(0.5f * m * (vit * vit) * c_kinetic());
+    for (int c = 0; c < 3; ++c)
+        x[c] = x[c] + v[c] * dt;
+    o->ef[BS_E_KINETIC] += ke;
+}
+
+static void probe_integrate_d(orc *o)
+{
+    const double dt = o->p.timestep, m = o->p.probe_mass;
+    double *x = o->pxd, *v = o->pvd, *f = o->pfd;
+    if (m > 0.0)
+        for (int c = 0; c < 3; ++c)
+            v[c] += f[c] / m * dt;
+    o->ed[BS_E_KINETIC] += 0.5 * m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) * c_kinetic();
+    for (int c = 0; c < 3; ++c)
+        x[c] += v[c] * dt;
+}
+
 /* computeSpringForces + computeParticleForces, float-faithful.  ext3 (nullable)
  * is added first (SpringNetwork::setForce in idleRun). */
 static void forces_f32(orc *o, const float *ext3)
@@ -1423,6 +1590,8 @@ static void forces_f32(orc *o, const float *ext3)
         e_imp += pe[4 * i + 2];
         e_hyd += pe[4 * i + 3];
     }
+    if (p->probe_enable)
+        probe_forces_f(o, F, &e_ster, &e_elec);
     o->ef[BS_E_ELECTROSTATIC] = e_elec;
     o->ef[BS_E_STERIC] = e_ster;
     o->ef[BS_E_IMP] = e_imp;
@@ -1560,6 +1729,8 @@ static void forces_f64(orc *o, const double *ext3)
         e_imp += pe[4 * i + 2];
         e_hyd += pe[4 * i + 3];
     }
+    if (p->probe_enable)
+        probe_forces_d(o, F, &e_ster, &e_elec);
     o->ed[BS_E_ELECTROSTATIC] = e_elec;
     o->ed[BS_E_STERIC] = e_ster;
     o->ed[BS_E_IMP] = e_imp;
@@ -2387,7 +2558,7 @@ static void adjoint_terms(orc *o, const double *w, double *hw, double *dk, doubl
 double orc_adjoint(orc *o, int nsteps, const double *y3, double *dk, double *dl0, double *dq)
 {
     const bs_params *p = &o->p;
-    if (p->imp_enable || p->hydrophobicity_enable || o->nbodies > 0 ||
+    if (p->imp_enable || p->hydrophobicity_enable || o->nbodies > 0 || p->probe_enable ||
         (p->electrostatic_enable && p->potentialgrid_enable && o->grid[BS_GRID_POTENTIAL]) ||
         (p->densitygrid_enable && o->grid[BS_GRID_DENSITY]))
         return NAN;
@@ -2451,6 +2622,8 @@ double orc_adjoint(orc *o, int nsteps, const double *y3, double *dk, double *dl0
  * advances), for finite differences. */
 double orc_loss(orc *o, int nsteps, const double *y3)
 {
+    if (o->p.probe_enable)
+        return NAN;
     for (int s = 0; s < nsteps; ++s) {
         forces_f64(o, NULL);
         integrate_f64(o);
@@ -2521,7 +2694,7 @@ void orc_compute_forces(orc *o, int mode, const double *ext3, double *F3, double
     }
 }
 
-/* nsteps of SpringNetwork::computeStep (without the probe).
+/* nsteps of SpringNetwork::computeStep.
  * Energies of the last step go to E6 (nullable). */
 int orc_step(orc *o, int mode, int nsteps, const double *ext3, double *E6)
 {
@@ -2535,6 +2708,8 @@ int orc_step(orc *o, int mode, int nsteps, const double *ext3, double *E6)
     for (int s = 0; s < nsteps && rc == BS_OK; ++s) {
         if (mode == ORC_F32) {
             forces_f32(o, ef);
+            if (o->p.probe_enable)
+                probe_integrate_f(o);
             if (o->p.rigidbody_enable) {
                 rigid_aggregate_f(o);
                 rigid_solve_f(o);
@@ -2542,6 +2717,8 @@ int orc_step(orc *o, int mode, int nsteps, const double *ext3, double *E6)
             rc = integrate_f32(o);
         } else {
             forces_f64(o, ext3);
+            if (o->p.probe_enable)
+                probe_integrate_d(o);
             if (o->p.rigidbody_enable) {
                 rigid_aggregate_d(o);
                 rigid_solve_d(o);

@@ -85,6 +85,7 @@ def oracle_lib():
         L.orc_springs_from_cutoff.argtypes = [ctypes.c_int64, _f32p, _u8p, ctypes.POINTER(ctypes.c_int32),
                                               ctypes.c_double, ctypes.c_int64, _u32p, _u32p, _f32p]
         L.orc_springs_from_cutoff.restype = ctypes.c_int64
+        L.orc_probe_state.argtypes = [_vp, ctypes.c_int, _f64p]
         _olib = L
     return _olib
 
@@ -124,6 +125,8 @@ def ref_lib():
                                               ctypes.c_int64, _u32p, _u32p, _f32p, _f32p, ctypes.c_double,
                                               ctypes.c_double, ctypes.c_int64, _u32p, _u32p, _f32p, _f32p]
         L.ref_springs_from_cutoff.restype = ctypes.c_int64
+        L.ref_get_probe.argtypes = [_vp, _f32p]
+        L.ref_get_probe.restype = ctypes.c_int
         L.ref_set_trajectories.argtypes = [ctypes.c_char_p, ctypes.c_int64, ctypes.c_char_p, ctypes.c_int64]
         L.ref_xtc_write.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, _f32p, ctypes.c_float]
         L.ref_xtc_read.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, _f32p,
@@ -191,6 +194,13 @@ class Oracle:
         e = None if ext is None else np.ascontiguousarray(ext, np.float64).reshape(-1)
         rc = self.L.orc_step(self.h, mode, nsteps, _p(e, ctypes.c_double), _p(E, ctypes.c_double))
         return rc, E
+
+    def probe(self, mode):
+        """The probe particle: position, velocity and the force of the last
+        evaluation, each (3,)."""
+        out = np.zeros(9)
+        self.L.orc_probe_state(self.h, mode, _p(out, ctypes.c_double))
+        return out[0:3], out[3:6], out[6:9]
 
     def pairs(self, kind, method=0, mode=F32):
         cnt = self.L.orc_pairs(self.h, kind, method, mode, None, 0)
@@ -323,6 +333,14 @@ class Reference:
         v = np.zeros(3 * self.n, np.float32)
         self.L.ref_get_state(self.h, _p(x, ctypes.c_float), _p(v, ctypes.c_float))
         return x.reshape(-1, 3), v.reshape(-1, 3)
+
+    def probe(self):
+        """The probe particle: position, velocity and the force of its last
+        step, as float32 (9,)."""
+        out = np.zeros(9, np.float32)
+        if self.L.ref_get_probe(self.h, _p(out, ctypes.c_float)) != 0:
+            raise RuntimeError("the probe is not enabled")
+        return out
 
     def add_rigid_body(self, ids):
         a = np.ascontiguousarray(ids, np.int64)

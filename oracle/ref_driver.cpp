@@ -120,6 +120,16 @@ biospring::configuration::Configuration make_config(const bs_params * p)
     c.hydrophobicity.cutoff = p->hydrophobicity_cutoff;
     c.hydrophobicity.scale = p->hydrophobicity_scale;
     c.rigidbody.enable = p->rigidbody_enable != 0;
+    c.probe.enable = p->probe_enable != 0;
+    c.probe.enableelectrostatic = p->probe_enable_electrostatic != 0;
+    c.probe.enablesteric = p->probe_enable_steric != 0;
+    c.probe.x = p->probe_x;
+    c.probe.y = p->probe_y;
+    c.probe.z = p->probe_z;
+    c.probe.mass = p->probe_mass;
+    c.probe.epsilon = p->probe_epsilon;
+    c.probe.radius = p->probe_radius;
+    c.probe.charge = p->probe_charge;
     return c;
 }
 
@@ -222,6 +232,7 @@ struct ExposedSearch : public biospring::nsearch::NeighborSearch<Container>
 struct RefSys
 {
     bs_params p;
+    int64_t n = 0; // particles given; the probe, when enabled, follows them
     std::unique_ptr<RefNet> net;
 };
 
@@ -295,6 +306,7 @@ void * ref_create(const bs_params * p, int64_t n, const float * x3, const float 
 {
     auto * s = new RefSys();
     s->p = *p;
+    s->n = n;
     s->net.reset(build_net(p, n, x3, v3, mass, charge, radius, eps, sasa, transfer, hydro, flags, m, id1, id2, l0, k));
     return s;
 }
@@ -356,7 +368,7 @@ void ref_set_grid(void * h, int kind, const double * origin, const int64_t * sha
 void ref_set_burying(void * h, const float * burying)
 {
     auto * s = static_cast<RefSys *>(h);
-    for (unsigned i = 0; i < s->net->getNumberOfParticles(); ++i)
+    for (unsigned i = 0; i < (unsigned)s->n; ++i)
         s->net->getParticle(i).setBurying(burying[i]);
 }
 
@@ -386,13 +398,13 @@ void ref_compute_forces(void * h, const float * ext3, float * F3, float * E6)
     RefNet & net = *s->net;
     net.idleRun();
     if (ext3)
-        for (unsigned i = 0; i < net.getNumberOfParticles(); ++i)
+        for (unsigned i = 0; i < (unsigned)s->n; ++i)
         {
             float f[3] = {ext3[3 * i], ext3[3 * i + 1], ext3[3 * i + 2]};
             net.setForce(i, f);
         }
     net.computeForces();
-    for (unsigned i = 0; i < net.getNumberOfParticles(); ++i)
+    for (unsigned i = 0; i < (unsigned)s->n; ++i)
     {
         const Vector3f f = net.getParticle(i).getForce();
         F3[3 * i] = f.getX();
@@ -415,7 +427,7 @@ int ref_step(void * h, int nsteps, const float * ext3, float * E6)
             // computeStep begins with idleRun, which resets energies and adds
             // interactor forces; setForce before it is equivalent because the
             // particle forces were reset at the end of the previous step.
-            for (unsigned i = 0; i < net.getNumberOfParticles(); ++i)
+            for (unsigned i = 0; i < (unsigned)s->n; ++i)
             {
                 float f[3] = {ext3[3 * i], ext3[3 * i + 1], ext3[3 * i + 2]};
                 net.setForce(i, f);
@@ -432,7 +444,7 @@ void ref_get_state(void * h, float * x3, float * v3)
 {
     auto * s = static_cast<RefSys *>(h);
     RefNet & net = *s->net;
-    for (unsigned i = 0; i < net.getNumberOfParticles(); ++i)
+    for (unsigned i = 0; i < (unsigned)s->n; ++i)
     {
         const Particle & q = net.getParticle(i);
         if (x3)
@@ -451,6 +463,26 @@ void ref_get_state(void * h, float * x3, float * v3)
     }
 }
 
+// The probe particle (SpringNetwork::getProbeParticle, SpringNetwork.h:277):
+// position, velocity, then the force of its last step (reset at the start of
+// the probe block and then accumulated, SpringNetwork.cpp:158-177).
+int ref_get_probe(void * h, float * out9)
+{
+    auto * s = static_cast<RefSys *>(h);
+    if (!s->net->isProbeEnabled())
+        return -1;
+    const Particle & q = s->net->getProbeParticle();
+    const Vector3f x = q.getPosition(), v = q.getVelocity(), f = q.getForce();
+    const Vector3f * vs[3] = {&x, &v, &f};
+    for (int k = 0; k < 3; ++k)
+    {
+        out9[3 * k] = vs[k]->getX();
+        out9[3 * k + 1] = vs[k]->getY();
+        out9[3 * k + 2] = vs[k]->getZ();
+    }
+    return 0;
+}
+
 // Cell list of search `kind` built by the reference NeighborSearch over the
 // network's current particles.  Returns the number of cells; cell_of[n]
 // (-1 for non-members), start[ncells+1], members[nmem], dims[3], box[6].
@@ -460,8 +492,11 @@ int64_t ref_cells(void * h, int kind, float cutoff, float skin, int64_t * cell_o
     auto * s = static_cast<RefSys *>(h);
     RefNet & net = *s->net;
     const Container & parts = net.getParticles();
+    // the probe, when enabled, is the last particle: in the box, not in the
+    // cells (SpringNetwork::_excludeProbeFromNeighborSearch)
+    const size_t np = (size_t)s->n;
     std::vector<size_t> inc;
-    for (size_t i = 0; i < parts.size(); ++i)
+    for (size_t i = 0; i < np; ++i)
     {
         const bool member = kind == BS_NS_ELECTROSTATIC ? parts[i].isCharged()
                             : kind == BS_NS_HYDROPHOBIC ? parts[i].isHydrophobic()
@@ -474,6 +509,8 @@ int64_t ref_cells(void * h, int kind, float cutoff, float skin, int64_t * cell_o
         ns.reset(new ExposedSearch(parts, cutoff, skin));
     else
         ns.reset(new ExposedSearch(parts, cutoff, inc, skin));
+    if (parts.size() > np)
+        ns->exclude_index(np);
     int64_t d[3];
     ns->dims(d);
     const int64_t ncells = d[0] * d[1] * d[2];
@@ -481,11 +518,11 @@ int64_t ref_cells(void * h, int kind, float cutoff, float skin, int64_t * cell_o
         std::memcpy(dims, d, sizeof d);
     if (box)
         ns->box(box);
-    std::vector<int64_t> member_flag(parts.size(), 0);
+    std::vector<int64_t> member_flag(np, 0);
     for (size_t i : inc)
         member_flag[i] = 1;
     if (cell_of)
-        for (size_t i = 0; i < parts.size(); ++i)
+        for (size_t i = 0; i < np; ++i)
             cell_of[i] = member_flag[i] ? ns->cell_of(i) : -1;
     int64_t w = 0;
     for (int64_t c = 0; c < ncells; ++c)
@@ -527,6 +564,10 @@ int64_t ref_pairs(const bs_params * p, int kind, int64_t n, const float * x3, co
     bs_params q = *p;
     q.viscosity_enable = 0;
     q.imp_enable = 0;
+    // the probe stays in the container (and so in the search box) but its
+    // own interactions, which are not neighbour pairs, are not recorded
+    q.probe_enable_steric = 0;
+    q.probe_enable_electrostatic = 0;
     std::unique_ptr<RefNet> net(build_net(&q, n, x3, nullptr, nullptr, enc_q.data(), nullptr, enc_eps.data(), nullptr,
                                           nullptr, enc_h.data(), flags, m, id1, id2, l0, k));
     auto * rec = new RecordingForceField();

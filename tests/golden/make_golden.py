@@ -226,6 +226,7 @@ def main():
     add_arrays("grid300", grid_origin=g_origin, grid_shape=g_shape, grid_cell=g_cell, grid_phi=phi,
                grid_density=dens, burying=bur, ivec_ids=np.array([3, 250], np.int64))
 
+    probe_cases()
     topology_cases()
     trajectory_cases()
 
@@ -264,6 +265,34 @@ def main():
         scans[f"{tag}_F0"] = np.array(fs)
     np.savez_compressed(os.path.join(OUT, "scans.npz"), d=xs, **scans)
     print("scans written")
+
+
+def probe_cases():
+    """The probe particle (SpringNetwork.cpp:154-182, Particle.cpp:434-465):
+    a charged probe beside the network, steric and Coulomb on; its position,
+    velocity and force are stored after each recorded step."""
+    probe_extra = lambda ref, st: {"probe": ref.probe()}  # noqa: E731
+    # CA network rules (linear steric, constant-dielectric Coulomb, viscosity),
+    # probe 3 A beyond the +x face, so that the linear wall (reach 4 A) acts on
+    # the outermost beads
+    pb = workloads.generate(0, seed=19, n=600)
+    pp = pb.params.copy()
+    xb = pb.x.reshape(-1, 3)
+    pp.probe_enable, pp.probe_enable_steric, pp.probe_enable_electrostatic = 1, 1, 1
+    pp.probe_x, pp.probe_y, pp.probe_z = float(xb[:, 0].max() + 3.0), float(xb[:, 1].mean()), float(xb[:, 2].mean())
+    pp.probe_mass, pp.probe_radius, pp.probe_epsilon, pp.probe_charge = 5.0, 3.0, 0.5, 0.3
+    ref_case("probe600", pb, pp, steps=(1, 5, 20), extra=probe_extra)
+    # Amber 12-6 + Coulomb network, probe 5 A beyond the +y face, steric only
+    # (the probe's charge is ignored), no viscosity
+    pa = workloads.generate(1, seed=20, n=800)
+    qa = pa.params.copy()
+    qa.dielectric_model = 0
+    qa.viscosity_enable = 0
+    xa = pa.x.reshape(-1, 3)
+    qa.probe_enable, qa.probe_enable_steric, qa.probe_enable_electrostatic = 1, 1, 0
+    qa.probe_x, qa.probe_y, qa.probe_z = float(xa[:, 0].mean()), float(xa[:, 1].max() + 5.0), float(xa[:, 2].mean())
+    qa.probe_mass, qa.probe_radius, qa.probe_epsilon, qa.probe_charge = 12.0, 3.5, 0.8, -1.0
+    ref_case("probe_amber800", pa, qa, steps=(1, 5), extra=probe_extra)
 
 
 def trajectory_cases():

@@ -43,6 +43,9 @@ def test_defaults_match_reference_configuration():
     assert p.electrostatic_cutoff == 16.0 and p.electrostatic_dielectric == 1.0
     assert p.viscosity_value == 1.0 and p.viscosity_enable == 0
     assert p.hydrophobicity_cutoff == 15.0 and p.imp_scale == 1.0
+    assert p.probe_enable == 0 and p.probe_enable_steric == 0 and p.probe_enable_electrostatic == 0
+    assert (p.probe_x, p.probe_y, p.probe_z) == (1.0, 1.0, 1.0)
+    assert (p.probe_mass, p.probe_epsilon, p.probe_radius, p.probe_charge) == (1.0, 1.0, 1.0, 0.0)
 
 
 def test_create_without_gpu_refuses():

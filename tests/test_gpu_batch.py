@@ -90,3 +90,39 @@ def test_autotune_picks_a_valid_configuration():
     xa, _ = a.getState()
     xb, _ = b.getState()
     assert np.abs(xa - xb).max() <= 1e-4
+
+
+def test_systems_never_pair_after_moving_onto_each_other():
+    """Independent systems stay independent however far they move from where
+    the setup placed them on the engine's binning grid: system 1 is
+    translated by exactly the lattice spacing between the systems' binning
+    sites, onto system 0's site; each system's forces and energies still
+    match its own FP64 oracle."""
+    s, off, parts = workloads.batch(3, seed=9, nmin=800, nmax=1500)
+    net = SpringNetwork.from_system(s)
+    net.setSystems(off)
+    net.setup()
+    x, v = net.getState()
+    p = s.params
+    ext = max(float(np.ptp(part.x.reshape(-1, 3).astype(np.float64), axis=0).max()) for part in parts)
+    radius = max(p.steric_cutoff if p.steric_enable else 0.0,
+                 p.electrostatic_cutoff if p.electrostatic_enable else 0.0) + max(p.list_skin, 0.0)
+    spacing = np.ceil(ext + 2.0 * radius + 20.0)
+    x = x.copy()
+    x[off[1]:off[2], 0] -= np.float32(spacing)
+    _log(f"moving system 1 by -{spacing} A in x")
+    net.setState(x, v)
+    F, E = net.computeForces()
+    Es = net.getSystemEnergies()
+    for k, part in enumerate(parts):
+        o = po.Oracle(part)
+        sl = slice(off[k], off[k + 1])
+        o.set_state(x[sl], v[sl])
+        Fd, Ed = o.compute_forces(po.F64)
+        o.close()
+        per, _ = force_errors(F[sl], Fd, (part.flags & 1) == 0)
+        _log(f"  system {k}: per-particle {per:.3g}")
+        assert per <= FORCE_RTOL, (k, per)
+        for slot in range(6):
+            assert energy_error(Es[k, slot], Ed[slot], floor=1e-3) <= ENERGY_RTOL, (k, slot, Es[k, slot], Ed[slot])
+    net.close()

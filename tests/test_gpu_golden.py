@@ -83,7 +83,7 @@ def test_golden_steric_variants():
 
 
 @pytest.mark.parametrize("name", ["enm600", "static800", "hydro600", "imp_curved", "rigid300", "rigidall200",
-                                  "grid300"])
+                                  "grid300", "probe600", "probe_amber800"])
 def test_golden_trajectory(name):
     """Positions after the fixture's step counts: the GPU is no further from
     the FP64 oracle than the reference's float trajectory is (plus 1e-5 of the
@@ -121,4 +121,12 @@ def test_golden_trajectory(name):
             eg, er = np.abs(ig - idd), np.abs(ir - idd)
             _log(f"    ivec gpu-f64 {eg.max():.3g} ref-f64 {er.max():.3g}")
             assert (eg <= 2 * er + 1e-4).all(), (name, st, ig, idd, ir)
+        if f"probe_{st}" in z:  # the probe: position, velocity, force of the step
+            pg = np.concatenate(net.getProbeParticle()).astype(np.float64)
+            pd = np.concatenate(o.probe(po.F64))
+            pr = z[f"probe_{st}"].astype(np.float64)
+            for sl in (slice(0, 3), slice(3, 6), slice(6, 9)):
+                eg, er = np.abs(pg[sl] - pd[sl]).max(), np.abs(pr[sl] - pd[sl]).max()
+                _log(f"    probe {sl}: gpu-f64 {eg:.3g} ref-f64 {er:.3g}")
+                assert eg <= 2 * er + 1e-5 * max(np.abs(pd[sl]).max(), 1e-9), (name, st, sl, pg, pd, pr)
         assert np.array_equal(net.getState()[0][(s.flags & 1) == 1], x0[(s.flags & 1) == 1].astype(np.float32))

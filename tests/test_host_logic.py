@@ -316,3 +316,30 @@ def test_interactive_updates(hostcheck_lib):
         net.run(1)
         net.close()
         """, hostcheck_lib)
+
+
+def test_overflow_lengthens_lists_and_resumes(hostcheck_lib):
+    """A neighbour list overflows mid-call (the beads are packed after the
+    lists were sized): the call stops at that step, the host lengthens the
+    lists and resumes, and every step of the call is completed; with the
+    CPU stand-in kernels positions stay put, so each rebuild sees the packed
+    state."""
+    out = _run('''
+        s = workloads.generate(1, seed=3, n=3000)
+        p = s.params.copy(); p.spring_enable = 0
+        net = SpringNetwork.from_system(s, params=p, library=L)
+        net.setup()
+        cap0 = net.counters()["list_cap"]
+        x = s.x.reshape(-1, 3).astype(np.float64)
+        c = x.mean(axis=0)
+        net.setState(((x - c) * 0.6 + c).astype(np.float32), np.zeros((s.n, 3), np.float32))
+        net.run(7)
+        cnt = net.counters()
+        assert cnt["steps"] == 7, cnt
+        assert cnt["list_cap"] > cap0 and cnt["max_list"] > cap0, (cap0, cnt)
+        net.run(3)
+        assert net.counters()["steps"] == 10
+        net.close()
+        print("ok", cap0, cnt["list_cap"])
+        ''', hostcheck_lib)
+    assert "ok" in out

@@ -308,3 +308,104 @@ def test_save_writes_the_reference_writer_layout(lib, tmp_path):
     back, _ = load(lib, out)
     check_loaded(back, expected(out))
     back.close()
+
+
+def _var_headers(raw):
+    """Offsets of each variable's vsize and begin fields in a classic
+    (version 1 or 2) header: {name: (vsize_offset, begin_offset, begin_size)}."""
+    import struct
+    ver = raw[3]
+    pos = 8
+
+    def u32():
+        nonlocal pos
+        v = struct.unpack(">I", raw[pos:pos + 4])[0]
+        pos += 4
+        return v
+
+    def name():
+        nonlocal pos
+        ln = u32()
+        s = raw[pos:pos + ln].decode()
+        pos += (ln + 3) // 4 * 4
+        return s
+
+    def atts():
+        nonlocal pos
+        tag, na = u32(), u32()
+        for _ in range(na):
+            name()
+            t, ne = u32(), u32()
+            pos += (ne * {1: 1, 2: 1, 3: 2, 4: 4, 5: 4, 6: 8}[t] + 3) // 4 * 4
+    tag, nd = u32(), u32()
+    for _ in range(nd):
+        name()
+        u32()
+    atts()
+    tag, nv = u32(), u32()
+    out = {}
+    for _ in range(nv):
+        nm = name()
+        for _ in range(u32()):
+            u32()
+        atts()
+        u32()
+        vs = pos
+        u32()
+        bsz = 8 if ver == 2 else 4
+        out[nm] = (vs, pos, bsz)
+        pos += bsz
+    return out
+
+
+@pytest.mark.parametrize("patch", ["begin_wraps", "begin_past_end", "vsize_wraps"])
+def test_hostile_offsets_are_refused(lib, tmp_path, patch):
+    """Offsets and sizes read from a damaged or hostile file are checked
+    without overflow: a begin near 2^64, a begin past the end and record
+    sizes whose sum wraps are refused with a message, not read."""
+    import struct
+    path = str(tmp_path / "t.nc")
+    dims, v = topology(n=20, m=10, record=patch == "vsize_wraps")
+    write(path, dims, v, version=2)
+    raw = bytearray(open(path, "rb").read())
+    hdr = _var_headers(bytes(raw))
+    if patch == "begin_wraps":
+        vs, b, sz = hdr["coordinates"]
+        raw[b:b + 8] = struct.pack(">Q", 2 ** 64 - 16)
+    elif patch == "begin_past_end":
+        vs, b, sz = hdr["coordinates"]
+        raw[b:b + 8] = struct.pack(">Q", len(raw) + 8)
+    else:  # record variables: two vsize values that sum past 2^32 * k wrap the total
+        recs = [k for k in ("coordinates", "charges", "radii", "mass") if k in hdr]
+        for k in recs[:2]:
+            vs, b, sz = hdr[k]
+            raw[vs:vs + 4] = struct.pack(">I", 0xFFFFFFF0)
+    open(path, "wb").write(bytes(raw))
+    net = SpringNetwork(workloads.params(99), library=lib)
+    with pytest.raises(_abi.EngineError) as ei:
+        net.loadNetCDF(path)
+    msg = str(ei.value)
+    assert "beyond the end of the file" in msg or "malformed" in msg or "larger than the file" in msg, msg
+    net.close()
+
+
+def test_hostile_cdf5_lengths_are_refused(lib, tmp_path):
+    """A 64-bit-data (CDF-5) header whose name length or attribute count is
+    near 2^64 is refused as malformed or truncated, without reading past the
+    buffer."""
+    import struct
+    for body in (
+        # dimension list with one name of length 2^64 - 2
+        struct.pack(">QIQQ", 0, 0x0A, 1, 2 ** 64 - 2) + b"ab\0\0",
+        # global attribute list: one attribute of 2^62 doubles
+        struct.pack(">QIQ", 0, 0, 0) + struct.pack(">IQQ", 0x0C, 1, 1) + b"a\0\0\0" +
+        struct.pack(">IQ", 6, 2 ** 62),
+    ):
+        path = str(tmp_path / "h.nc")
+        open(path, "wb").write(b"CDF\x05" + body + b"\0" * 16)
+        net = SpringNetwork(workloads.params(99), library=lib)
+        with pytest.raises(_abi.EngineError) as ei:
+            net.loadNetCDF(path)
+        msg = str(ei.value)
+        assert "malformed" in msg or "truncated" in msg or "not present" in msg, msg
+        net.close()
