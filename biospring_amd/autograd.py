@@ -13,6 +13,15 @@ import numpy as np
 import torch
 
 
+def _refuse_reload(net):
+    """Charges change through setParticles, which also clears the handle's
+    systems, rigid bodies, trajectory writers and step count: refuse rather
+    than lose them."""
+    if net.numSystems() > 1 or net.numRigidBodies() > 0:
+        raise ValueError("final_positions: q cannot be set on a network with independent systems or rigid bodies "
+                         "(setParticles would drop them); pass q=None")
+
+
 class _FinalPositions(torch.autograd.Function):
     @staticmethod
     def forward(ctx, k, l0, q, net, system, nsteps, checkpoint_every):
@@ -21,6 +30,7 @@ class _FinalPositions(torch.autograd.Function):
             net.setSprings(a, b, l0_0 if l0 is None else l0.detach().cpu().numpy().astype(np.float32),
                            k_0 if k is None else k.detach().cpu().numpy().astype(np.float32))
         if q is not None:
+            _refuse_reload(net)
             arrs = list(system.particle_arrays())
             x, v = net.getState()
             arrs[0], arrs[1], arrs[3] = x, v, q.detach().cpu().numpy().astype(np.float32)
@@ -29,7 +39,7 @@ class _FinalPositions(torch.autograd.Function):
                            (system.id1, system.id2,
                             system.l0 if l0 is None else l0.detach().cpu().numpy().astype(np.float32),
                             system.k if k is None else k.detach().cpu().numpy().astype(np.float32)))
-        net.adjointForward(nsteps, checkpoint_every)
+        ctx.tape = net.adjointForward(nsteps, checkpoint_every)
         x, _ = net.getState()
         ctx.net = net
         ctx.has = (k is not None, l0 is not None, q is not None)
@@ -37,6 +47,10 @@ class _FinalPositions(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, grad_x):
+        if ctx.net.tape != ctx.tape:
+            # the handle holds one record: a later forward replaced this one
+            raise RuntimeError("final_positions: the network recorded another forward pass since this one; "
+                               "call backward before the next forward, or use one network per pass")
         dk, dl0, dq = ctx.net.adjointBackward(grad_x.detach().cpu().numpy().astype(np.float64))
         out = [torch.from_numpy(g) if h else None for g, h in zip((dk, dl0, dq), ctx.has)]
         return out[0], out[1], out[2], None, None, None, None

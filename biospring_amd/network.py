@@ -381,9 +381,17 @@ class SpringNetwork:
 
     def adjointForward(self, nsteps, checkpoint_every=0):
         """Runs and records nsteps steps (the state advances) for
-        adjointBackward."""
+        adjointBackward.  Returns the record's generation number, which
+        changes with every recording (see tape)."""
         check(self._L.bs_adjoint_forward(self._h, int(nsteps), int(checkpoint_every), ptr(self._ext, ctypes.c_float)),
               self._h, "bs_adjoint_forward")
+        self._tape = getattr(self, "_tape", 0) + 1
+        return self._tape
+
+    @property
+    def tape(self):
+        """Generation number of the last adjointForward record (0: none)."""
+        return getattr(self, "_tape", 0)
 
     def adjointBackward(self, dl_dx=None, dl_dv=None):
         """(dk, dl0, dq) of a loss of the recorded final state, given dL/dx_T

@@ -193,3 +193,27 @@ def test_torch_autograd_wrapper_gives_the_adjoint_gradients():
     assert abs(loss.item() - ref[0]) <= 1e-9 * ref[0]
     assert _rel(k.grad.numpy(), ref[1]) <= 1e-6
     assert _rel(q.grad.numpy(), ref[3]) <= 1e-6
+
+
+def test_torch_wrapper_refuses_a_replaced_record():
+    """The handle holds one forward record: the backward of a pass whose
+    record a later forward replaced raises instead of returning the later
+    pass's gradients; charges are not reloaded on a network with rigid
+    bodies (setParticles would drop them)."""
+    torch = pytest.importorskip("torch")
+    from biospring_amd.autograd import final_positions
+
+    s = workloads.generate(1, seed=17, n=1000)
+    y = torch.from_numpy(_target(s).astype(np.float64))
+    net = SpringNetwork.from_system(s)
+    k = torch.tensor(s.k.astype(np.float64), requires_grad=True)
+    x1 = final_positions(net, s, k=k, nsteps=4)
+    x2 = final_positions(net, s, k=k, nsteps=4)
+    with pytest.raises(RuntimeError):
+        ((x1 - y) ** 2).sum().backward()
+    ((x2 - y) ** 2).sum().backward()
+    assert torch.isfinite(k.grad).all()
+    net.addRigidBody(np.arange(10))
+    q = torch.tensor(s.charge.astype(np.float64), requires_grad=True)
+    with pytest.raises(ValueError):
+        final_positions(net, s, q=q, nsteps=2)

@@ -62,15 +62,25 @@ def _packed(s, f=0.62):
     return (c + (x - c) * f).astype(np.float32)
 
 
+def _soft(s):
+    """Parameters under which packed beads stay put: the linear steric wall
+    (forces of order C_spring per A of overlap) at a 12 A cutoff, no springs
+    (packing would stretch them far beyond rest), no Coulomb."""
+    p = s.params.copy()
+    p.spring_enable = 0
+    p.electrostatic_enable = 0
+    p.steric_mode = _abi.STERIC_LINEAR
+    p.viscosity_enable = 1
+    return p
+
+
 def test_overflow_lengthens_lists_and_resumes():
     """The beads are packed to about 4x the density the lists were sized for:
     the next rebuild overflows; the call lengthens the lists, rebuilds and
     completes every step.  The result follows the FP64 oracle as closely as
     the reference's float path does."""
     s = workloads.generate(1, seed=3, n=4000)
-    p = s.params.copy()
-    p.spring_enable = 0  # packing would stretch the springs far beyond rest
-    p.viscosity_enable = 1
+    p = _soft(s)
     net = SpringNetwork.from_system(s, params=p)
     net.setup()
     cap0 = net.counters()["list_cap"]
@@ -100,8 +110,7 @@ def test_overflow_keeps_trajectory_frames():
     energy table has one row per due iteration, in order, as without an
     overflow."""
     s = workloads.generate(1, seed=4, n=3000)
-    p = s.params.copy()
-    p.spring_enable = 0
+    p = _soft(s)
     xp = _packed(s)
     v0 = np.zeros((s.n, 3), np.float32)
     rows = {}
@@ -118,8 +127,10 @@ def test_overflow_keeps_trajectory_frames():
             net.addTrajectory(_abi.TRAJ_CSV, path, 3)
             net.run(12)
             net.closeTrajectories()
-            with open(path) as fh:
-                rows[tag] = [ln.split(",")[0] for ln in fh.read().splitlines()[1:]]
+            with open(path) as fh:  # step, frame rate (timing: left out), energies
+                rows[tag] = [[c for k, c in enumerate(ln.split("\t")) if k != 1]
+                             for ln in fh.read().splitlines()[1:]]
             net.close()
     _log("frames", rows)
-    assert rows["overflow"] == rows["plain"] and len(rows["plain"]) == 4
+    assert [r[0] for r in rows["plain"]] == ["0", "3", "6", "9"]
+    assert rows["overflow"] == rows["plain"]
