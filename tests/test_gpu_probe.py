@@ -88,8 +88,12 @@ def test_probe_trajectory_vs_fp64(graph):
         assert eg <= 2 * er + 1e-5 * max(np.abs(pd[sl]).max(), 1e-9)
     moved = np.abs(pd[0:3] - np.array([p.probe_x, p.probe_y, p.probe_z])).max()
     assert moved > 1e3 * max(np.abs(pg[0:3] - pd[0:3]).max(), 1e-7), moved
+    # 50 steps apart the trajectories differ by ~2e-4 A, so the step energies
+    # are held to the float path's own distance from FP64 (or 1e-4)
     for slot in (_abi.E_STERIC, _abi.E_ELECTROSTATIC, _abi.E_KINETIC):
-        assert energy_error(E[slot], Ed[slot], floor=1e-3) <= 1e-4, (slot, E[slot], Ed[slot])
+        err, err_ref = energy_error(E[slot], Ed[slot], floor=1e-3), energy_error(Ef[slot], Ed[slot], floor=1e-3)
+        _log(f"  energy {slot}: gpu-f64 {err:.3g} f32-f64 {err_ref:.3g}")
+        assert err <= max(1e-4, 2 * err_ref), (slot, E[slot], Ed[slot], Ef[slot])
     o64.close()
     o32.close()
     if graph:
