@@ -16,6 +16,7 @@
 #include <cstring>
 #include <memory>
 #include <new>
+#include <type_traits>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -34,7 +35,8 @@ struct List
     int32_t dims[3] = {1, 1, 1};
     int32_t ncells = 1;
     int32_t cap = 0;          // entries per i-cluster
-    uint32_t *cl = nullptr;   // [ncl * cap] j | mask << 24
+    uint32_t *cl = nullptr;   // [ncl * cap] j | mask << 24 (dealt order for the queue layout)
+    uint32_t *clt = nullptr;  // [ncl * cap] the build's entries in search order, before dealing
     int32_t *cln = nullptr;   // [ncl]
     uint32_t *rows = nullptr; // row layout (BS_PAIR_ROWS): [row blocks * rcap * 64]
     int32_t *rlen = nullptr;  // [row blocks * 64]
@@ -106,6 +108,7 @@ struct bs_handle
     {
         bool ready = false;
         int32_t T = 0, K = 0, ncp = 0;
+        int32_t seg_of = -1; // the segment whose positions seg holds
         int ext = 0;
         int64_t cap_cp = 0, cap_seg = 0;
         float4 *cp_pos = nullptr, *cp_vel = nullptr, *seg = nullptr, *fin_pos = nullptr, *fin_vel = nullptr;
@@ -309,6 +312,7 @@ void free_device(bs_handle *h)
     for (auto &l : h->lists)
     {
         f(l.cl);
+        f(l.clt);
         f(l.cln);
         f(l.rows);
         f(l.rlen);
@@ -353,7 +357,7 @@ int check_device(bs_handle *h)
     if (h->ncl != (int32_t)((h->n + CLUSTER - 1) / CLUSTER))
         return fail(h, BS_ERR_STATE, "internal: cluster count mismatch");
     for (const auto &l : h->lists)
-        if (!l.cl || !l.cln || l.cap <= 0 || l.ncells > h->cell_cap ||
+        if (!l.cl || !l.clt || !l.cln || l.cap <= 0 || l.ncells > h->cell_cap ||
             (h->p.pair_layout == BS_PAIR_ROWS && (!l.rows || !l.rlen || l.rcap < l.cap)))
             return fail(h, BS_ERR_STATE, "internal: neighbour list buffers missing");
     return BS_OK;
@@ -641,6 +645,7 @@ void build_list(bs_handle *h, List &l, bool gated, int32_t cap_override)
     a.ex_ptr = h->ex_ptr;
     a.ex_w = h->ex_w;
     a.cl = l.cl;
+    a.clt = l.clt;
     a.cln = l.cln;
     a.status = h->status;
     a.n = n;
@@ -654,6 +659,7 @@ void build_list(bs_handle *h, List &l, bool gated, int32_t cap_override)
     a.i_mask = l.imask;
     a.j_mask = l.jmask;
     a.exclude_bonded = h->p.spring_enable ? 1 : 0;
+    a.deal = h->p.pair_layout == BS_PAIR_QUEUES ? 1 : 0;
     a.gate = g;
     launch_list_build(a, s);
     if (l.rows && cap_override < 0)
@@ -1002,9 +1008,12 @@ int regrow_lists(bs_handle *h)
         if (l.cap >= need)
             continue;
         (void)hipFree(l.cl);
-        l.cl = nullptr;
+        (void)hipFree(l.clt);
+        l.cl = l.clt = nullptr;
         l.cap = need;
         l.cl = dalloc<uint32_t>((size_t)h->ncl * l.cap, e);
+        HIPCHK(h, e);
+        l.clt = dalloc<uint32_t>((size_t)h->ncl * l.cap, e);
         HIPCHK(h, e);
         if (l.rows)
         {
@@ -1200,6 +1209,12 @@ int adjoint_record(bs_handle *h, int32_t T, int32_t checkpoint_every)
 {
     auto &tp = h->tape;
     tp.ready = false;
+    if (!h->s1.empty())
+    { // the backward calls find the spring gradient slots built
+        const int rc = build_spring_owners(h);
+        if (rc)
+            return rc;
+    }
     const int64_t n = h->n;
     int32_t K = checkpoint_every;
     if (K <= 0)
@@ -1275,108 +1290,125 @@ int adjoint_record(bs_handle *h, int32_t T, int32_t checkpoint_every)
     tp.T = T;
     tp.K = K;
     tp.ncp = ncp;
+    tp.seg_of = ncp - 1;
     tp.ext = h->ext_uploaded;
     tp.ready = true;
     return BS_OK;
 }
 
-// Adjoint state of one sweep: lx, lv (FP64) and the weights w of the step
-// being reversed (FP32).
-struct AdjSeed
+// Device buffers of one reverse sweep, laid out in 256-byte steps in one
+// block: the adjoint state lx, lv (FP64); the weights w of the step being
+// reversed and of the next (FP32); the pair terms' H w; the spring gradients
+// (FP32 per edge between flushes, FP64 per slot, per spring at the end);
+// dL/dq in internal and in particle order; the caller's seed as uploaded.
+struct AdjBufs
 {
-    double4 *lx = nullptr, *lv = nullptr;
-    float4 *w = nullptr;
+    double4 *lx = nullptr, *lv = nullptr, *hwp = nullptr;
+    float4 *w[2] = {nullptr, nullptr};
+    float *dke = nullptr, *dle = nullptr;
+    double *dks = nullptr, *dls = nullptr, *outk = nullptr, *outl = nullptr;
+    double *dqd = nullptr, *dqo = nullptr, *sx = nullptr, *sv = nullptr;
+    bool springs = false;
 };
 
-int adjoint_alloc_seed(bs_handle *h, Scratch &S, AdjSeed &seed)
+// Lays the sweep's buffers out from base (nullptr: sizes only) and returns
+// the bytes they take.  springs: spring gradients wanted (the slots must be
+// built); seed: room for a caller's dL/dx, dL/dv.
+size_t adj_carve(const bs_handle *h, bool springs, bool seed, char *base, AdjBufs &b)
 {
-    hipError_t e = hipSuccess;
-    seed.lx = S.get<double4>((size_t)h->n, e);
-    HIPCHK(h, e);
-    seed.lv = S.get<double4>((size_t)h->n, e);
-    HIPCHK(h, e);
-    seed.w = S.get<float4>((size_t)h->n, e);
-    HIPCHK(h, e);
-    return BS_OK;
+    const size_t n = (size_t)h->n, E = (size_t)std::max<int64_t>(h->nedges, 0),
+                 ns = (size_t)std::max<int64_t>(h->nslots, 0), m = h->s1.size();
+    size_t off = 0;
+    auto take = [&](auto *&ptr, size_t count) {
+        using T = std::remove_pointer_t<std::remove_reference_t<decltype(ptr)>>;
+        off = (off + 255) & ~(size_t)255;
+        ptr = base ? reinterpret_cast<T *>(base + off) : nullptr;
+        off += sizeof(T) * std::max<size_t>(count, 1);
+    };
+    b = AdjBufs{};
+    b.springs = springs;
+    take(b.lx, n);
+    take(b.lv, n);
+    take(b.w[0], n);
+    take(b.w[1], n);
+    if (!h->lists.empty())
+        take(b.hwp, n);
+    if (springs)
+    {
+        take(b.dke, E);
+        take(b.dle, E);
+        take(b.dks, ns);
+        take(b.dls, ns);
+        take(b.outk, m);
+        take(b.outl, m);
+    }
+    take(b.dqd, n);
+    take(b.dqo, n);
+    if (seed)
+    {
+        take(b.sx, 3 * n);
+        take(b.sv, 3 * n);
+    }
+    return off;
 }
 
-// Reverse sweep over the recorded forward pass from the seeded state at x_T:
+// Reverse sweep over the recorded forward pass from the seeded state at x_T
+// (b.lx, b.lv, b.w[0]):
 //   a = lv + dt lx,  w = dt/m a,  lx += H(x_s) w,  lv = (1 - gamma dt/m) a,
 // segment by segment (earlier segments recomputed from their checkpoints),
-// then the state goes back to x_T; the record stays usable.
-int adjoint_sweep(bs_handle *h, Scratch &S, AdjSeed &seed, double *dk, double *dl0, double *dq)
+// then the state goes back to x_T; the record stays usable.  Allocates
+// nothing: every buffer comes in b.
+int adjoint_sweep(bs_handle *h, AdjBufs &b, double *dk, double *dl0, double *dq)
 {
-    const auto &tp = h->tape;
+    auto &tp = h->tape;
     const bs_params &P = h->p;
     const int64_t n = h->n, E = h->nedges, m = (int64_t)h->s1.size();
     const int32_t T = tp.T, K = tp.K, ncp = tp.ncp;
     const size_t s4 = sizeof(float4) * (size_t)n;
     hipStream_t st = h->stream;
-    hipError_t e = hipSuccess;
     int rc = BS_OK;
-    if ((dk || dl0) && m > 0)
-    {
-        rc = build_spring_owners(h);
-        if (rc)
-            return rc;
-    }
     if (tp.ext != h->ext_uploaded)
         return fail(h, BS_ERR_STATE, "bs_adjoint: external force changed since the forward pass");
-    double4 *hwp = nullptr;
-    float4 *wb[2] = {seed.w, nullptr};
-    float *dke = nullptr, *dle = nullptr;
-    double *dqd = nullptr;
-#define SALLOC(ptr, T_, cnt)                                                                                     \
-    do                                                                                                            \
-    {                                                                                                             \
-        ptr = S.get<T_>((size_t)(cnt), e);                                                                        \
-        HIPCHK(h, e);                                                                                             \
-    } while (0)
-    SALLOC(wb[1], float4, n);
-    if (!h->lists.empty())
-        SALLOC(hwp, double4, n);
-    // spring gradients: FP32 sums per owning edge over at most FLUSH steps,
-    // then added into FP64 totals per slot; per spring at the end
-    const bool want_springs = (dk || dl0) && m > 0;
+    const bool want_springs = b.springs && m > 0;
+    if (want_springs && h->nslots < 0)
+        return fail(h, BS_ERR_STATE, "internal: spring gradient slots missing");
+    double4 *hwp = b.hwp;
+    float4 *wb[2] = {b.w[0], b.w[1]};
+    float *dke = b.dke, *dle = b.dle;
+    double *dks = b.dks, *dls = b.dls, *outk = b.outk, *outl = b.outl, *dqd = b.dqd;
     const int64_t ns = std::max<int64_t>(h->nslots, 0);
-    double *dks = nullptr, *dls = nullptr, *outk = nullptr, *outl = nullptr;
+    // cleared by a fill kernel (no memset nodes, should a caller capture)
     if (want_springs)
     {
-        SALLOC(dke, float, E);
-        SALLOC(dle, float, E);
-        SALLOC(dks, double, ns);
-        SALLOC(dls, double, ns);
-        SALLOC(outk, double, m);
-        SALLOC(outl, double, m);
+        launch_fill_i32(reinterpret_cast<int32_t *>(dke), 0, std::max<int64_t>(E, 1), nullptr, st);
+        launch_fill_i32(reinterpret_cast<int32_t *>(dle), 0, std::max<int64_t>(E, 1), nullptr, st);
+        launch_fill_i32(reinterpret_cast<int32_t *>(dks), 0, 2 * std::max<int64_t>(ns, 1), nullptr, st);
+        launch_fill_i32(reinterpret_cast<int32_t *>(dls), 0, 2 * std::max<int64_t>(ns, 1), nullptr, st);
     }
-    SALLOC(dqd, double, n);
-#undef SALLOC
+    launch_fill_i32(reinterpret_cast<int32_t *>(dqd), 0, 2 * n, nullptr, st);
     constexpr int32_t FLUSH = 64;
     int32_t since_flush = 0;
     const double dt = (double)P.timestep, gam = P.viscosity_enable ? (double)P.viscosity_value : 0.0;
-    if (want_springs)
-    {
-        HIPCHK(h, hipMemsetAsync(dke, 0, sizeof(float) * std::max<int64_t>(E, 1), st));
-        HIPCHK(h, hipMemsetAsync(dle, 0, sizeof(float) * std::max<int64_t>(E, 1), st));
-        HIPCHK(h, hipMemsetAsync(dks, 0, sizeof(double) * std::max<int64_t>(ns, 1), st));
-        HIPCHK(h, hipMemsetAsync(dls, 0, sizeof(double) * std::max<int64_t>(ns, 1), st));
-    }
-    HIPCHK(h, hipMemsetAsync(dqd, 0, sizeof(double) * n, st));
     const double hs = 0.5 * std::max(P.list_skin, 0.0f);
     const float half_skin2 = (float)(hs * hs * (1.0 - 1e-5));
     int wc = 0;
     for (int32_t c = ncp - 1; c >= 0; --c)
     {
         const int32_t s0 = c * K, s1 = std::min(T, s0 + K);
-        if (c < ncp - 1)
+        // a segment other than the one seg holds (every segment but the last
+        // on the first sweep; all of them on later sweeps of the record) is
+        // recomputed from its checkpoint, bit for bit as the forward pass
+        const bool recompute = tp.seg_of != c;
+        if (recompute)
         {
+            tp.seg_of = -1; // partly overwritten until the segment is complete
             HIPCHK(h, hipMemcpyAsync(h->pos[h->cur], tp.cp_pos + (size_t)c * n, s4, hipMemcpyDeviceToDevice, st));
             HIPCHK(h, hipMemcpyAsync(h->vel, tp.cp_vel + (size_t)c * n, s4, hipMemcpyDeviceToDevice, st));
             HIPCHK(h, hipMemcpyAsync(h->counters, tp.cp_cnt + 4 * (size_t)c, 4 * sizeof(int32_t),
                                      hipMemcpyDeviceToDevice, st));
             launch_fill_i32(h->status, 1, 1, nullptr, st);
         }
-        for (int32_t s = s0; s < s1 && c < ncp - 1; ++s)
+        for (int32_t s = s0; s < s1 && recompute; ++s)
         {
             HIPCHK(h, hipMemcpyAsync(tp.seg + (size_t)(s - s0) * n, h->pos[h->cur], s4, hipMemcpyDeviceToDevice, st));
             if (s + 1 < s1)
@@ -1387,6 +1419,12 @@ int adjoint_sweep(bs_handle *h, Scratch &S, AdjSeed &seed, double *dk, double *d
                 launch_step(h, false);
             }
         }
+        if (recompute)
+            tp.seg_of = c;
+        // lists rebuilt at each segment's first reversed state, so that the
+        // sweep does not depend on the lists a recomputation or an earlier
+        // call left: two sweeps of one record give the same bits
+        launch_fill_i32(h->status, 1, 1, nullptr, st);
         for (int32_t s = s1 - 1; s >= s0; --s)
         {
             HIPCHK(h, hipMemcpyAsync(h->pos[h->cur], tp.seg + (size_t)(s - s0) * n, s4, hipMemcpyDeviceToDevice, st));
@@ -1410,8 +1448,8 @@ int adjoint_sweep(bs_handle *h, Scratch &S, AdjSeed &seed, double *dk, double *d
             a.sp_l0 = h->sp_l0;
             a.w = wb[wc];
             a.hwp = hwp;
-            a.lx = seed.lx;
-            a.lv = seed.lv;
+            a.lx = b.lx;
+            a.lv = b.lv;
             a.w_next = wb[wc ^ 1];
             a.dk = dke;
             a.dl0 = dle;
@@ -1446,11 +1484,12 @@ int adjoint_sweep(bs_handle *h, Scratch &S, AdjSeed &seed, double *dk, double *d
 
     if (want_springs)
         launch_spring_out(h->slot_of, m, dks, dls, outk, outl, st);
-    std::vector<double> dqh((size_t)n);
+    if (dq)
+        launch_unperm_f64(dqd, h->perm_d, (int32_t)n, b.dqo, st);
     int32_t stv[4];
     HIPCHK(h, hipMemcpyAsync(stv, h->status, sizeof stv, hipMemcpyDeviceToHost, st));
     if (dq)
-        HIPCHK(h, hipMemcpyAsync(dqh.data(), dqd, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+        HIPCHK(h, hipMemcpyAsync(dq, b.dqo, sizeof(double) * n, hipMemcpyDeviceToHost, st));
     if (want_springs && dk)
         HIPCHK(h, hipMemcpyAsync(dk, outk, sizeof(double) * m, hipMemcpyDeviceToHost, st));
     if (want_springs && dl0)
@@ -1459,9 +1498,6 @@ int adjoint_sweep(bs_handle *h, Scratch &S, AdjSeed &seed, double *dk, double *d
     prof_collect(h);
     if (stv[2] != 0)
         return fail(h, BS_ERR_STATE, "neighbour list capacity exceeded (max %d > cap); rerun bs_setup", stv[3]);
-    if (dq)
-        for (int64_t t = 0; t < n; ++t)
-            dq[h->perm[(size_t)t]] = dqh[(size_t)t];
     return BS_OK;
 }
 
@@ -1849,6 +1885,7 @@ int do_setup(bs_handle *h)
         const int32_t mx = st[3];
         l.cap = (std::max<int32_t>(16, (int32_t)std::ceil(mx * 1.25) + 64) + 15) / 16 * 16;
         ALLOC(l.cl, uint32_t, (int64_t)h->ncl * l.cap);
+        ALLOC(l.clt, uint32_t, (int64_t)h->ncl * l.cap);
         if (h->p.pair_layout == BS_PAIR_ROWS)
         {
             const int64_t nrb = (n + 63) / 64;
@@ -2800,53 +2837,57 @@ int bs_adjoint_forward(bs_handle *h, int32_t nsteps, int32_t checkpoint_every, c
     return adjoint_record(h, nsteps, checkpoint_every);
 }
 
+// Checks shared by both forms of the backward call.
+static int adjoint_backward_check(bs_handle *h, const double *dl_dx3, const double *dl_dv3, const char *who)
+{
+    if (!h->tape.ready)
+        return fail(h, BS_ERR_STATE, "%s: no recorded forward pass at the current state", who);
+    if (!dl_dx3 && !dl_dv3)
+        return fail(h, BS_ERR_ARG, "%s: dL/dx or dL/dv required", who);
+    const int rc = adjoint_check(h);
+    if (rc)
+        return rc;
+    return check_device(h);
+}
+
+// The caller's seed (3n doubles each, particle order) uploaded into b.sx,
+// b.sv, turned into lx, lv, w on the device, then the sweep.
+static int adjoint_backward_on(bs_handle *h, AdjBufs &b, const double *dl_dx3, const double *dl_dv3, double *dk,
+                        double *dl0, double *dq)
+{
+    const int64_t n = h->n;
+    if (dl_dx3)
+        HIPCHK(h, hipMemcpyAsync(b.sx, dl_dx3, sizeof(double) * 3 * n, hipMemcpyHostToDevice, h->stream));
+    if (dl_dv3)
+        HIPCHK(h, hipMemcpyAsync(b.sv, dl_dv3, sizeof(double) * 3 * n, hipMemcpyHostToDevice, h->stream));
+    launch_adj_seed(dl_dx3 ? b.sx : nullptr, dl_dv3 ? b.sv : nullptr, h->perm_d, h->jprm, h->iprm, (int32_t)n,
+                    (double)h->p.timestep, b.lx, b.lv, b.w[0], h->stream);
+    return adjoint_sweep(h, b, dk, dl0, dq);
+}
+
 int bs_adjoint_backward(bs_handle *h, const double *dl_dx3, const double *dl_dv3, double *dk, double *dl0,
                         double *dq)
 {
     if (!h)
         return BS_ERR_ARG;
-    if (!h->tape.ready)
-        return fail(h, BS_ERR_STATE, "bs_adjoint_backward: no recorded forward pass at the current state");
-    if (!dl_dx3 && !dl_dv3)
-        return fail(h, BS_ERR_ARG, "bs_adjoint_backward: dL/dx or dL/dv required");
-    int rc = adjoint_check(h);
+    int rc = adjoint_backward_check(h, dl_dx3, dl_dv3, "bs_adjoint_backward");
     if (rc)
         return rc;
-    rc = check_device(h);
-    if (rc)
-        return rc;
-    // seed in internal order: static particles carry no adjoint state
-    const int64_t n = h->n;
-    const double dt = (double)h->p.timestep;
-    std::vector<double4> lx((size_t)n), lv((size_t)n);
-    std::vector<float4> w((size_t)n);
-    for (int64_t t = 0; t < n; ++t)
+    const bool springs = (dk || dl0) && !h->s1.empty();
+    if (springs)
     {
-        const int64_t i = h->perm[(size_t)t];
-        const bool dyn = !(h->flags[(size_t)i] & BS_FLAG_STATIC);
-        double x[3] = {0, 0, 0}, v[3] = {0, 0, 0};
-        for (int c = 0; c < 3; ++c)
-        {
-            if (dyn && dl_dx3)
-                x[c] = dl_dx3[3 * i + c];
-            if (dyn && dl_dv3)
-                v[c] = dl_dv3[3 * i + c];
-        }
-        lx[(size_t)t] = make_double4(x[0], x[1], x[2], 0.0);
-        lv[(size_t)t] = make_double4(v[0], v[1], v[2], 0.0);
-        const double m = h->mass[(size_t)i], s = (dyn && m > 0.0) ? dt / m : 0.0;
-        w[(size_t)t] = make_float4((float)(s * (v[0] + dt * x[0])), (float)(s * (v[1] + dt * x[1])),
-                                   (float)(s * (v[2] + dt * x[2])), 0.0f);
+        rc = build_spring_owners(h);
+        if (rc)
+            return rc;
     }
+    AdjBufs b;
+    const size_t bytes = adj_carve(h, springs, true, nullptr, b);
     Scratch S;
-    AdjSeed seed{};
-    rc = adjoint_alloc_seed(h, S, seed);
-    if (rc)
-        return rc;
-    HIPCHK(h, hipMemcpyAsync(seed.lx, lx.data(), sizeof(double4) * n, hipMemcpyHostToDevice, h->stream));
-    HIPCHK(h, hipMemcpyAsync(seed.lv, lv.data(), sizeof(double4) * n, hipMemcpyHostToDevice, h->stream));
-    HIPCHK(h, hipMemcpyAsync(seed.w, w.data(), sizeof(float4) * n, hipMemcpyHostToDevice, h->stream));
-    return adjoint_sweep(h, S, seed, dk, dl0, dq);
+    hipError_t e = hipSuccess;
+    char *base = S.get<char>(bytes, e);
+    HIPCHK(h, e);
+    adj_carve(h, springs, true, base, b);
+    return adjoint_backward_on(h, b, dl_dx3, dl_dv3, dk, dl0, dq);
 }
 
 int bs_adjoint(bs_handle *h, int32_t nsteps, int32_t checkpoint_every, const float *target3, const float *ext_force3,
@@ -2861,12 +2902,14 @@ int bs_adjoint(bs_handle *h, int32_t nsteps, int32_t checkpoint_every, const flo
         return rc;
     // loss and seed lx = 2 (x_T - y) / n on the device
     const int64_t n = h->n;
+    const bool springs = (dk || dl0) && !h->s1.empty();
+    AdjBufs b;
+    const size_t bytes = adj_carve(h, springs, false, nullptr, b);
     Scratch S;
-    AdjSeed seed{};
-    rc = adjoint_alloc_seed(h, S, seed);
-    if (rc)
-        return rc;
     hipError_t e = hipSuccess;
+    char *base = S.get<char>(bytes, e);
+    HIPCHK(h, e);
+    adj_carve(h, springs, false, base, b);
     float4 *tgt = S.get<float4>((size_t)n, e);
     HIPCHK(h, e);
     const int32_t lblocks = (int32_t)((n + BLOCK - 1) / BLOCK);
@@ -2880,8 +2923,8 @@ int bs_adjoint(bs_handle *h, int32_t nsteps, int32_t checkpoint_every, const flo
             y[(size_t)t] = make_float4(target3[3 * i], target3[3 * i + 1], target3[3 * i + 2], 0.0f);
         }
         HIPCHK(h, hipMemcpyAsync(tgt, y.data(), sizeof(float4) * n, hipMemcpyHostToDevice, h->stream));
-        launch_adj_init(h->pos[h->cur], tgt, h->jprm, h->iprm, (int32_t)n, (double)h->p.timestep, seed.lx, seed.lv,
-                        seed.w, lpart, lblocks, h->stream);
+        launch_adj_init(h->pos[h->cur], tgt, h->jprm, h->iprm, (int32_t)n, (double)h->p.timestep, b.lx, b.lv,
+                        b.w[0], lpart, lblocks, h->stream);
         std::vector<double> lp((size_t)lblocks);
         HIPCHK(h, hipMemcpyAsync(lp.data(), lpart, sizeof(double) * lblocks, hipMemcpyDeviceToHost, h->stream));
         HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -2891,7 +2934,7 @@ int bs_adjoint(bs_handle *h, int32_t nsteps, int32_t checkpoint_every, const flo
         if (loss)
             *loss = L / (double)n;
     }
-    return adjoint_sweep(h, S, seed, dk, dl0, dq);
+    return adjoint_sweep(h, b, dk, dl0, dq);
 }
 
 static int springs_from_cutoff(bs_handle *h, double cutoff, float stiffness, const int32_t *group,

@@ -79,6 +79,27 @@ inline Consts make_consts()
 constexpr int32_t CLUSTER = 8;
 constexpr uint32_t CL_JMASK = 0x00FFFFFFu;
 
+// A cluster's entries are stored in dealt order: its `kept` entries form
+// nch = ceil(kept / QCHUNK) chunks, and entry p of the search order goes to
+// chunk p mod nch.  Chunk c holds q + (c < r) entries from offset
+// c q + min(c, r) (q = kept / nch, r = kept mod nch), so the chunks tile the
+// list densely and each samples all of it: the spatial order of the search
+// (entries on one side of the cluster, then the other) would otherwise give
+// a chunk entries that only some of the cluster's particles see.
+constexpr int32_t QCHUNK = 128;
+struct Deal
+{
+    int32_t nch, q, r;
+};
+constexpr Deal deal_of(int32_t kept) // constexpr: callable on host and device
+{
+    Deal d{0, 0, 0};
+    d.nch = kept > 0 ? (kept + QCHUNK - 1) / QCHUNK : 0;
+    d.q = kept > 0 ? kept / d.nch : 0;
+    d.r = kept > 0 ? kept - d.q * d.nch : 0;
+    return d;
+}
+
 // Parameters of one pair kernel launch.
 struct PairArgs
 {
@@ -282,7 +303,8 @@ struct ListArgs
     const double *box;         // [6] current box (device)
     const int32_t *ex_ptr;     // [ncl + 1] static exclusions per i-cluster
     const uint32_t *ex_w;      // j << 8 | bonded mask, sorted by j
-    uint32_t *cl;              // [ncl * cap]
+    uint32_t *cl;              // [ncl * cap] search order, or dealt order when deal is set
+    uint32_t *clt;             // [ncl * cap] the entries in search order before dealing (deal only)
     int32_t *cln;              // [ncl]
     int32_t *status;           // [2] overflow, [3] max count
     int32_t n, ncl, cap;
@@ -292,6 +314,7 @@ struct ListArgs
     float r2list;
     uint32_t i_mask, j_mask;
     int32_t exclude_bonded;
+    int32_t deal;              // store the lists in dealt order (the queue layout reads chunks)
     const int32_t *gate;       // nullable: skip unless *gate != 0
 };
 
@@ -340,6 +363,12 @@ void launch_adj_spring(const AdjSpringArgs &a, int32_t group, int32_t nblocks, h
 void launch_adj_init(const float4 *pos, const float4 *target, const float4 *jprm, const float4 *iprm, int32_t n,
                      double dt, double4 *lx, double4 *lv, float4 *w, double *loss_part, int32_t nblocks,
                      hipStream_t s);
+// A caller's seed (dL/dx_T, dL/dv_T in particle order, either null) into lx,
+// lv and the first weights w, in internal order (perm: internal -> particle).
+void launch_adj_seed(const double *dx3, const double *dv3, const int32_t *perm, const float4 *jprm,
+                     const float4 *iprm, int32_t n, double dt, double4 *lx, double4 *lv, float4 *w, hipStream_t s);
+// out[perm[t]] = in[t] for t < n.
+void launch_unperm_f64(const double *in, const int32_t *perm, int32_t n, double *out, hipStream_t s);
 // Spring-gradient flush: for each slot r (springs in the order of their
 // owning CSR edge, slot_e ascending) dk[r] += dke[slot_e[r]] in FP64 and the
 // FP32 edge sum is cleared; likewise dl0.

@@ -951,6 +951,43 @@ __global__ void __launch_bounds__(BLOCK) k_adj_init(const float4 *pos, const flo
         loss_part[blockIdx.x] = t;
 }
 
+// A caller's seed dL/dx_T, dL/dv_T (3n doubles each in particle order, either
+// null) into the sweep's state in internal order: lx, lv, and the first
+// weights w = dt/m (lv + dt lx); static particles carry none.  Products and
+// sums are rounded one by one (no contraction), as on the host.
+__global__ void __launch_bounds__(BLOCK) k_adj_seed(const double *dx3, const double *dv3, const int32_t *perm,
+                                                     const float4 *jprm, const float4 *iprm, int32_t n, double dt,
+                                                     double4 *lx, double4 *lv, float4 *w)
+{
+#pragma clang fp contract(off)
+    const int32_t t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= n)
+        return;
+    const int64_t i = perm[t];
+    const bool dyn = !(fbits(jprm[t].w) & PF_STATIC);
+    double x[3] = {0.0, 0.0, 0.0}, v[3] = {0.0, 0.0, 0.0};
+    for (int c = 0; c < 3; ++c)
+    {
+        if (dyn && dx3)
+            x[c] = dx3[3 * i + c];
+        if (dyn && dv3)
+            v[c] = dv3[3 * i + c];
+    }
+    lx[t] = make_double4(x[0], x[1], x[2], 0.0);
+    lv[t] = make_double4(v[0], v[1], v[2], 0.0);
+    const double m = (double)iprm[t].x, s = (dyn && m > 0.0) ? dt / m : 0.0;
+    w[t] = make_float4((float)(s * (v[0] + dt * x[0])), (float)(s * (v[1] + dt * x[1])),
+                       (float)(s * (v[2] + dt * x[2])), 0.0f);
+}
+
+// out[perm[t]] = in[t]: internal order back to particle order.
+__global__ void k_unperm_f64(const double *in, const int32_t *perm, int32_t n, double *out)
+{
+    const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n)
+        out[perm[t]] = in[t];
+}
+
 // Flush: the FP32 per-edge sums of the last steps are added to the FP64
 // totals per slot and cleared.  Slots follow the edges, so every access
 // walks memory forward.
@@ -1488,6 +1525,19 @@ void launch_adj_init(const float4 *pos, const float4 *target, const float4 *jprm
                      hipStream_t s)
 {
     k_adj_init<<<nblocks, BLOCK, 0, s>>>(pos, target, jprm, iprm, n, dt, lx, lv, w, loss_part);
+}
+
+void launch_adj_seed(const double *dx3, const double *dv3, const int32_t *perm, const float4 *jprm,
+                     const float4 *iprm, int32_t n, double dt, double4 *lx, double4 *lv, float4 *w, hipStream_t s)
+{
+    if (n > 0)
+        k_adj_seed<<<nblk(n), BLOCK, 0, s>>>(dx3, dv3, perm, jprm, iprm, n, dt, lx, lv, w);
+}
+
+void launch_unperm_f64(const double *in, const int32_t *perm, int32_t n, double *out, hipStream_t s)
+{
+    if (n > 0)
+        k_unperm_f64<<<nblk(n), BLOCK, 0, s>>>(in, perm, n, out);
 }
 
 void launch_spring_grads(const int32_t *slot_e, float *dke, float *dle, int64_t ns, double *dk, double *dl0,

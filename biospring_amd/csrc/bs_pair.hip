@@ -238,12 +238,30 @@ __global__ void __launch_bounds__(BLOCK) k_cluster_list(ListArgs a)
                         {
                             const int32_t slot = count + off;
                             if (slot < a.cap)
-                                a.cl[(size_t)ci * a.cap + slot] = (uint32_t)j | (m << 24);
+                                (a.deal ? a.clt : a.cl)[(size_t)ci * a.cap + slot] = (uint32_t)j | (m << 24);
                         }
                         count += __popcll(bal);
                     }
                 }
             }
+        }
+    }
+    // Deal the entries into their chunks (dealt order: entry p to chunk
+    // p mod nch).  The wave reads back what its own lanes stored; a
+    // workgroup-scope release/acquire makes them visible (one CU).
+    const int32_t kept = count < a.cap ? count : a.cap;
+    if (a.deal && kept > 0)
+    {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        const Deal d = deal_of(kept);
+        const uint32_t *src = a.clt + (size_t)ci * a.cap;
+        uint32_t *dst = a.cl + (size_t)ci * a.cap;
+        for (int32_t p = lane; p < kept; p += 64)
+        {
+            const int32_t c = p % d.nch, t = p / d.nch;
+            dst[c * d.q + min(c, d.r) + t] = src[p];
         }
     }
     // zero words up to the next multiple of 16, read unconditionally by the
@@ -815,29 +833,35 @@ void launch_list_build(const ListArgs &a, hipStream_t s)
 }
 
 // ---------------------------------------------------------------------------
-// Queue layout (BS_PAIR_QUEUES): the cluster lists, read 64 words at a time.
-// Each word's j data is loaded once (one lane per word) into LDS, and each
-// of the 8 cluster particles gets an LDS queue of the words it sees (one
-// ballot per particle).  The 8 lanes of a particle then drain its queue, 8
-// entries per round, so the lanes evaluate listed pairs instead of idling
-// on entries their particle does not see (an entry is seen by about half of
-// a cluster's particles when springs empty the inner shell).  A chunk takes
-// as many rounds as its fullest queue.  Each lane sums a fixed subset of its
-// particle's entries in a fixed order; the 8 partial sums are combined as in
-// k_pair: deterministic.  The next chunk's words and j data are loaded
-// while the current one is evaluated.
+// Queue layout (BS_PAIR_QUEUES).  A wave takes its cluster's list one dealt
+// chunk (at most QCHUNK = 128 words) at a time.  Each word's j data is loaded
+// once (one lane per word) into LDS, and each of the 8 cluster particles gets
+// an LDS queue of the chunk's words it sees (one ballot per particle and 64
+// words).  The 8 lanes of a particle then drain its queue, 8 entries per
+// round, so the lanes evaluate listed pairs instead of idling on entries
+// their particle does not see (an entry is seen by about half of a
+// cluster's particles when springs empty the inner shell).  A chunk takes as
+// many rounds as its fullest queue; the dealt order gives every chunk a
+// sample of the whole list, so the queues of a chunk are nearly equal.  Each
+// lane sums a fixed subset of its particle's entries in a fixed order and
+// the 8 partial sums are combined as in k_pair: deterministic.  The next
+// chunk's words and j data are loaded while the current one is evaluated.
 // ---------------------------------------------------------------------------
+__device__ inline int32_t lanes_below(uint64_t b)
+{
+    return (int32_t)__builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
+}
+
 template <int SMODE, int EMODE, int HYDRO, bool ENERGY>
 __global__ void __launch_bounds__(BLOCK) k_pair_q(PairArgs a)
 {
     if (a.status && halted(a.status))
         return;
-    __shared__ float4 qpos[WAVES][64], qprm[WAVES][64];
-    __shared__ uint8_t qidx[WAVES][CLUSTER][64];
-    __shared__ int32_t qcnt[WAVES][CLUSTER];
+    __shared__ float4 qpos[WAVES][QCHUNK], qprm[WAVES][QCHUNK];
+    __shared__ uint8_t qidx[WAVES][CLUSTER][QCHUNK];
     const int wv = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
-    const int32_t ci = blockIdx.x * WAVES + wv;
+    const int32_t ci = __builtin_amdgcn_readfirstlane(blockIdx.x * WAVES + wv);
     if (ci >= a.ncl)
         return; // whole wave
     const int32_t il = lane & 7, sl = lane >> 3;
@@ -846,44 +870,66 @@ __global__ void __launch_bounds__(BLOCK) k_pair_q(PairArgs a)
     const float4 pi = iv ? a.pos[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     const float4 ri = iv ? a.jprm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     const PairI c = pair_i<SMODE, EMODE>(pi, ri, a);
-    const uint64_t below = (1ull << lane) - 1ull;
     double es = 0.0, ee = 0.0, eh = 0.0;
     double fx = 0.0, fy = 0.0, fz = 0.0;
-    const int32_t cnt = a.cln[ci];
+    const Deal d = deal_of(__builtin_amdgcn_readfirstlane(a.cln[ci]));
     const uint32_t *L = a.cl + (size_t)ci * a.cap;
-    // chunk 0 in flight
-    uint32_t w = lane < cnt ? L[lane] : 0u;
-    float4 pj = a.pos[w & CL_JMASK], rj = a.jprm[w & CL_JMASK];
-    for (int32_t t0 = 0; t0 < cnt; t0 += 64)
+    // chunk 0 in flight (a chunk holds at least one word)
+    int32_t cs = 0, cn = d.q + (d.r > 0 ? 1 : 0);
+    uint32_t w0 = 0u, w1 = 0u;
+    if (d.nch > 0)
     {
-        qpos[wv][lane] = pj;
-        qprm[wv][lane] = rj;
+        w0 = lane < cn ? L[lane] : 0u;
+        w1 = 64 + lane < cn ? L[64 + lane] : 0u;
+    }
+    float4 p0 = a.pos[w0 & CL_JMASK], r0 = a.jprm[w0 & CL_JMASK];
+    float4 p1 = a.pos[w1 & CL_JMASK], r1 = a.jprm[w1 & CL_JMASK];
+    for (int32_t ch = 0; ch < d.nch; ++ch)
+    {
+        qpos[wv][lane] = p0;
+        qprm[wv][lane] = r0;
+        qpos[wv][64 + lane] = p1;
+        qprm[wv][64 + lane] = r1;
+        int32_t cnt[CLUSTER];
 #pragma unroll
         for (int k = 0; k < CLUSTER; ++k)
         {
-            const bool on = (w >> (24 + k)) & 1u;
-            const uint64_t bal = __ballot(on);
-            if (on)
-                qidx[wv][k][__popcll(bal & below)] = (uint8_t)lane;
-            if (lane == k)
-                qcnt[wv][k] = __popcll(bal);
+            const bool s0 = (w0 >> (24 + k)) & 1u, s1 = (w1 >> (24 + k)) & 1u;
+            const uint64_t b0 = __ballot(s0), b1 = __ballot(s1);
+            const int32_t n0 = __popcll(b0);
+            if (s0)
+                qidx[wv][k][lanes_below(b0)] = (uint8_t)lane;
+            if (s1)
+                qidx[wv][k][n0 + lanes_below(b1)] = (uint8_t)(64 + lane);
+            cnt[k] = n0 + __popcll(b1);
         }
-        // next chunk in flight while this one is evaluated
-        const int32_t tn = t0 + 64 + lane;
-        w = tn < cnt ? L[tn] : 0u;
-        pj = a.pos[w & CL_JMASK];
-        rj = a.jprm[w & CL_JMASK];
-        wave_lds_sync();
-        const int32_t mine = qcnt[wv][il];
-        int32_t most = mine;
-        for (int off = 1; off < 8; off <<= 1)
-            most = max(most, __shfl_xor(most, off, 64));
-        for (int32_t e = sl; e < most; e += 8)
+        // the next chunk in flight while this one is evaluated
+        if (ch + 1 < d.nch)
         {
+            cs += cn;
+            cn = d.q + (ch + 1 < d.r ? 1 : 0);
+            w0 = lane < cn ? L[cs + lane] : 0u;
+            w1 = 64 + lane < cn ? L[cs + 64 + lane] : 0u;
+            p0 = a.pos[w0 & CL_JMASK];
+            r0 = a.jprm[w0 & CL_JMASK];
+            p1 = a.pos[w1 & CL_JMASK];
+            r1 = a.jprm[w1 & CL_JMASK];
+        }
+        wave_lds_sync();
+        int32_t mine = cnt[0], most = cnt[0];
+#pragma unroll
+        for (int k = 1; k < CLUSTER; ++k)
+        {
+            mine = il == k ? cnt[k] : mine;
+            most = max(most, cnt[k]);
+        }
+        for (int32_t e0 = 0; e0 < most; e0 += 8) // wave-uniform rounds
+        {
+            const int32_t e = e0 + sl;
             const bool valid = e < mine;
-            const int32_t src = valid ? qidx[wv][il][e] : 0;
-            const float4 qj = qpos[wv][src], sj = qprm[wv][src];
-            interact<SMODE, EMODE, HYDRO, ENERGY>(valid, pi, ri, qj, sj, a, c, fx, fy, fz, es, ee, eh);
+            const int32_t src = valid ? (int32_t)qidx[wv][il][e] : 0;
+            interact<SMODE, EMODE, HYDRO, ENERGY>(valid, pi, ri, qpos[wv][src], qprm[wv][src], a, c, fx, fy, fz, es,
+                                                  ee, eh);
         }
         wave_lds_sync();
     }

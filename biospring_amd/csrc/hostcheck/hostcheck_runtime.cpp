@@ -399,7 +399,11 @@ void launch_list_build(const ListArgs &a, hipStream_t)
     need(a.cln, sizeof(int32_t) * a.ncl, "list: cln");
     need(a.status, sizeof(int32_t) * 4, "list: status");
     if (a.cap > 0)
+    {
         need(a.cl, sizeof(uint32_t) * (size_t)a.ncl * a.cap, "list: cl");
+        if (a.deal)
+            need(a.clt, sizeof(uint32_t) * (size_t)a.ncl * a.cap, "list: clt");
+    }
     if ((int64_t)a.ncl * CLUSTER < a.n || (int64_t)(a.ncl - 1) * CLUSTER >= a.n)
         hc_fail("list: bad cluster count");
     if (a.exclude_bonded)
@@ -484,13 +488,27 @@ void launch_list_build(const ListArgs &a, hipStream_t)
                         if (!m)
                             continue;
                         if (count < a.cap)
-                            a.cl[(size_t)ci * a.cap + count] = (uint32_t)j | (m << 24);
+                            (a.deal ? a.clt : a.cl)[(size_t)ci * a.cap + count] = (uint32_t)j | (m << 24);
                         ++count;
                     }
                 }
         }
         if (a.cap % 16)
             hc_fail("list: capacity not a multiple of 16");
+        const int32_t kept = std::min(count, a.cap);
+        if (a.deal && kept > 0)
+        { // dealt order; every slot below kept written exactly once
+            const Deal d = deal_of(kept);
+            std::vector<char> hit((size_t)kept, 0);
+            for (int32_t p = 0; p < kept; ++p)
+            {
+                const int32_t c = p % d.nch, t = p / d.nch, q = c * d.q + std::min(c, d.r) + t;
+                if (q < 0 || q >= kept || hit[(size_t)q] || t >= d.q + (c < d.r ? 1 : 0))
+                    hc_fail("list: dealt position out of range or repeated");
+                hit[(size_t)q] = 1;
+                a.cl[(size_t)ci * a.cap + q] = a.clt[(size_t)ci * a.cap + p];
+            }
+        }
         for (int32_t slot = count; slot < std::min((count + 15) & ~15, a.cap); ++slot)
             a.cl[(size_t)ci * a.cap + slot] = 0u;
         a.cln[ci] = std::min(count, a.cap);
@@ -793,6 +811,45 @@ void launch_adj_init(const float4 *pos, const float4 *target, const float4 *jprm
     {
         lx[i] = lv[i] = make_double4(0, 0, 0, 0);
         w[i] = make_float4(0, 0, 0, 0);
+    }
+}
+
+void launch_adj_seed(const double *dx3, const double *dv3, const int32_t *perm, const float4 *jprm,
+                     const float4 *iprm, int32_t n, double, double4 *lx, double4 *lv, float4 *w, hipStream_t)
+{
+    if (n <= 0)
+        return;
+    if (dx3)
+        need(dx3, sizeof(double) * 3 * n, "adj_seed: dx3");
+    if (dv3)
+        need(dv3, sizeof(double) * 3 * n, "adj_seed: dv3");
+    need(perm, sizeof(int32_t) * n, "adj_seed: perm");
+    need(jprm, sizeof(float4) * n, "adj_seed: jprm");
+    need(iprm, sizeof(float4) * n, "adj_seed: iprm");
+    need(lx, sizeof(double4) * n, "adj_seed: lx");
+    need(lv, sizeof(double4) * n, "adj_seed: lv");
+    need(w, sizeof(float4) * n, "adj_seed: w");
+    for (int32_t t = 0; t < n; ++t)
+    {
+        if (perm[t] < 0 || perm[t] >= n)
+            hc_fail("adj_seed: perm out of range");
+        lx[t] = lv[t] = make_double4(0, 0, 0, 0);
+        w[t] = make_float4(0, 0, 0, 0);
+    }
+}
+
+void launch_unperm_f64(const double *in, const int32_t *perm, int32_t n, double *out, hipStream_t)
+{
+    if (n <= 0)
+        return;
+    need(in, sizeof(double) * n, "unperm: in");
+    need(perm, sizeof(int32_t) * n, "unperm: perm");
+    need(out, sizeof(double) * n, "unperm: out");
+    for (int32_t t = 0; t < n; ++t)
+    {
+        if (perm[t] < 0 || perm[t] >= n)
+            hc_fail("unperm: perm out of range");
+        out[perm[t]] = in[t];
     }
 }
 

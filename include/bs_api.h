@@ -137,10 +137,12 @@ typedef struct bs_params {
        stay idle.  BS_PAIR_ROWS: the cluster lists expanded at each rebuild
        into one list per particle, every lane evaluating a listed pair at one
        gather per pair (slower where gathers, not arithmetic, bound the
-       kernel).  BS_PAIR_QUEUES: the cluster lists read 64 words at a time,
-       each word's j data loaded once into LDS and queued for the particles
-       that see it, so that lanes evaluate listed pairs at the cluster
-       layout's load count.  Forces agree to FP64 rounding in every layout. */
+       kernel).  BS_PAIR_QUEUES: the cluster lists stored in dealt order
+       (chunks of at most 128 entries, each a sample of the whole list) and
+       read a chunk at a time, each entry's j data loaded once into LDS and
+       queued for the particles that see it, so that lanes evaluate listed
+       pairs at the cluster layout's load count.  Forces agree to FP64
+       rounding in every layout. */
     int32_t pair_layout;
 
     /* The probe particle (probe.*, Configuration.hpp:186-195): one extra

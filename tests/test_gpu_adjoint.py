@@ -171,6 +171,10 @@ def test_two_phase_adjoint_matches_one_call_and_is_linear_in_the_seed():
     for i in range(3):
         assert _rel(r12[i], 2.0 * r1[i] - 0.5 * r2[i]) <= 1e-5
         assert _rel(rxv[i], r1[i] + rv[i]) <= 1e-5
+    # a sweep does not depend on what earlier sweeps left (lists, buffers)
+    r1b = b.adjointBackward(g1)
+    for i in range(3):
+        assert np.array_equal(r1b[i], r1[i]), i
     b.run(1)
     with pytest.raises(_abi.EngineError):
         b.adjointBackward(g1)

@@ -74,7 +74,7 @@ def test_probe_trajectory_vs_fp64(graph):
     E = net.getEnergies()
     o64, o32 = po.Oracle(s, params=p), po.Oracle(s, params=p)
     _, Ed = o64.step(po.F64, 50)
-    o32.step(po.F32, 50)
+    _, Ef = o32.step(po.F32, 50)
     xd, _ = o64.state(po.F64)
     xf, _ = o32.state(po.F32)
     pd = np.concatenate(o64.probe(po.F64))
