@@ -34,6 +34,7 @@ EXPORTS = (
     "bs_get_cells", "bs_get_pairs", "bs_add_rigid_body", "bs_num_rigid_bodies", "bs_get_rigid_body", "bs_get_probe",
     "bs_set_grid", "bs_set_burying", "bs_get_insertion_vector", "bs_set_systems", "bs_num_systems",
     "bs_get_system_energies", "bs_autotune", "bs_adjoint", "bs_adjoint_forward", "bs_adjoint_backward",
+    "bs_adjoint_backward_noalloc",
     "bs_springs_from_cutoff", "bs_get_springs", "bs_add_trajectory", "bs_write_frame_now",
     "bs_flush_trajectories", "bs_close_trajectories", "bs_set_particle_labels", "bs_set_insertion_vector",
     "bs_load_netcdf", "bs_get_particle_properties", "bs_set_sasa", "bs_set_membrane",
@@ -178,6 +179,7 @@ def bind(L):
     L.bs_get_springs.argtypes = [_vp, _u32p, _u32p, _f32p, _f32p]
     L.bs_springs_between_groups_from_cutoff.argtypes = [_vp, ctypes.c_double, ctypes.c_float, _i32p, _i64p]
     L.bs_adjoint_backward.argtypes = [_vp, _f64p, _f64p, _f64p, _f64p, _f64p]
+    L.bs_adjoint_backward_noalloc.argtypes = [_vp, _f64p, _f64p, _f64p, _f64p, _f64p]
     L.bs_add_trajectory.argtypes = [_vp, ctypes.c_int32, ctypes.c_char_p, ctypes.c_int64]
     L.bs_write_frame_now.argtypes = [_vp]
     L.bs_flush_trajectories.argtypes = [_vp]

@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <type_traits>
 #include <numeric>
@@ -54,10 +55,46 @@ template <typename T> T *dalloc(size_t count, hipError_t &err)
 
 } // namespace
 
+// Device memory of bs_adjoint_backward_noalloc: a fixed array in the code
+// object, one copy on each GPU the library runs on, held until the process
+// ends.  In host code the array's name is a host-side stand-in, not a device
+// address: its address on the current device comes from hipGetSymbolAddress
+// (noalloc_setup, from bs_create) and only that address is used.  Every call
+// on one GPU shares that GPU's copy; calls that overlap on one GPU race.
+constexpr size_t NOALLOC_BYTES = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+namespace
+{
+
+constexpr int NOALLOC_MAXDEV = 64;
+std::mutex g_noalloc_mu;
+char *g_noalloc_ptr[NOALLOC_MAXDEV];
+hipError_t g_noalloc_err[NOALLOC_MAXDEV];
+bool g_noalloc_done[NOALLOC_MAXDEV];
+
+// Once per device, with that device current: the address of its copy.
+void noalloc_setup(int dev)
+{
+    if (dev < 0 || dev >= NOALLOC_MAXDEV)
+        return;
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (g_noalloc_done[dev])
+        return;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+    g_noalloc_ptr[dev] = e == hipSuccess ? static_cast<char *>(ptr) : nullptr;
+    g_noalloc_err[dev] = e;
+    g_noalloc_done[dev] = true;
+}
+
+} // namespace
+
 struct bs_handle
 {
     bs_params p{};
     std::string err;
+    int device = 0; // current device at bs_create
     hipStream_t stream = nullptr;
     Consts k{};
 
@@ -1991,6 +2028,8 @@ int bs_create(const bs_params *p, bs_handle **out)
         delete h;
         return BS_ERR_HIP;
     }
+    h->device = dev;
+    noalloc_setup(dev);
     *out = h;
     return BS_OK;
 }
@@ -2837,6 +2876,23 @@ int bs_adjoint_forward(bs_handle *h, int32_t nsteps, int32_t checkpoint_every, c
     return adjoint_record(h, nsteps, checkpoint_every);
 }
 
+// This handle's device's copy of the fixed scratch array.
+static int noalloc_base(bs_handle *h, char **base)
+{
+    *base = nullptr;
+    const int dev = h->device;
+    if (dev < 0 || dev >= NOALLOC_MAXDEV)
+        return fail(h, BS_ERR_STATE, "no fixed device scratch on device %d (limit %d devices)", dev, NOALLOC_MAXDEV);
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_done[dev])
+        return fail(h, BS_ERR_STATE, "internal: device scratch not set up on device %d", dev);
+    if (!g_noalloc_ptr[dev])
+        return fail(h, BS_ERR_HIP, "hipGetSymbolAddress of the device scratch failed on device %d: %s", dev,
+                    hipGetErrorString(g_noalloc_err[dev]));
+    *base = g_noalloc_ptr[dev];
+    return BS_OK;
+}
+
 // Checks shared by both forms of the backward call.
 static int adjoint_backward_check(bs_handle *h, const double *dl_dx3, const double *dl_dv3, const char *who)
 {
@@ -2886,6 +2942,35 @@ int bs_adjoint_backward(bs_handle *h, const double *dl_dx3, const double *dl_dv3
     hipError_t e = hipSuccess;
     char *base = S.get<char>(bytes, e);
     HIPCHK(h, e);
+    adj_carve(h, springs, true, base, b);
+    return adjoint_backward_on(h, b, dl_dx3, dl_dv3, dk, dl0, dq);
+}
+
+int bs_adjoint_backward_noalloc(bs_handle *h, const double *dl_dx3, const double *dl_dv3, double *dk, double *dl0,
+                                double *dq)
+{
+    if (!h)
+        return BS_ERR_ARG;
+    const char *who = "bs_adjoint_backward_noalloc";
+    int rc = adjoint_backward_check(h, dl_dx3, dl_dv3, who);
+    if (rc)
+        return rc;
+    if (h->prof)
+        return fail(h, BS_ERR_STATE, "%s: profiling creates events; turn it off first", who);
+    const bool springs = (dk || dl0) && !h->s1.empty();
+    if (springs && h->nslots < 0)
+        return fail(h, BS_ERR_STATE, "%s: spring gradient slots missing; run bs_adjoint_forward first", who);
+    char *base = nullptr;
+    rc = noalloc_base(h, &base);
+    if (rc)
+        return rc;
+    AdjBufs b;
+    const size_t bytes = adj_carve(h, springs, true, nullptr, b);
+    if (bytes > NOALLOC_BYTES)
+        return fail(h, BS_ERR_ARG,
+                    "%s: this system needs %zu bytes of device scratch, more than the fixed %zu (%lld particles, "
+                    "%lld springs); use bs_adjoint_backward",
+                    who, bytes, NOALLOC_BYTES, (long long)h->n, (long long)h->s1.size());
     adj_carve(h, springs, true, base, b);
     return adjoint_backward_on(h, b, dl_dx3, dl_dv3, dk, dl0, dq);
 }

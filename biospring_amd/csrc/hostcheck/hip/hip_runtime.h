@@ -55,6 +55,13 @@ hipError_t hipEventElapsedTime(float *ms, hipEvent_t a, hipEvent_t b);
 hipError_t hipEventDestroy(hipEvent_t e);
 hipError_t hipEventSynchronize(hipEvent_t e);
 const char *hipGetErrorString(hipError_t e);
+// Device-side file-scope arrays are host arrays here; HIP_SYMBOL registers
+// one as a live device allocation so launches into it pass the extent checks.
+#define __device__
+#define __align__(x) __attribute__((aligned(x)))
+const void *hc_symbol(const void *p, size_t bytes);
+#define HIP_SYMBOL(x) hc_symbol(&(x), sizeof(x))
+hipError_t hipGetSymbolAddress(void **p, const void *symbol);
 // Graphs: capture is not emulated (the launchers here run at once); the
 // calls report "not supported", which the engine turns into an error.
 typedef struct ihipGraph_t *hipGraph_t;

@@ -31,17 +31,42 @@ std::mutex &mu()
     static std::mutex m;
     return m;
 }
+// file-scope device arrays, registered by HIP_SYMBOL
+std::map<uintptr_t, size_t> &symbols()
+{
+    static std::map<uintptr_t, size_t> m;
+    return m;
+}
+int64_t g_malloc_calls = 0;
+
+bool inside(const std::map<uintptr_t, size_t> &m, uintptr_t a, size_t bytes)
+{
+    auto it = m.upper_bound(a);
+    if (it == m.begin())
+        return false;
+    --it;
+    return a >= it->first && a + bytes <= it->first + it->second;
+}
 } // namespace
 
 bool hc_valid(const void *p, size_t bytes)
 {
     std::lock_guard<std::mutex> lk(mu());
     const uintptr_t a = (uintptr_t)p;
-    auto it = allocs().upper_bound(a);
-    if (it == allocs().begin())
-        return false;
-    --it;
-    return a >= it->first && a + bytes <= it->first + it->second;
+    return inside(allocs(), a, bytes) || inside(symbols(), a, bytes);
+}
+
+const void *hc_symbol(const void *p, size_t bytes)
+{
+    std::lock_guard<std::mutex> lk(mu());
+    symbols()[(uintptr_t)p] = bytes;
+    return p;
+}
+
+hipError_t hipGetSymbolAddress(void **p, const void *symbol)
+{
+    *p = const_cast<void *>(symbol);
+    return hipSuccess;
 }
 
 void hc_fail(const char *what)
@@ -67,6 +92,7 @@ hipError_t hipMalloc(void **p, size_t bytes)
 {
     *p = std::calloc(bytes ? bytes : 1, 1);
     std::lock_guard<std::mutex> lk(mu());
+    ++g_malloc_calls;
     allocs()[(uintptr_t)*p] = bytes;
     return *p ? hipSuccess : hipErrorOutOfMemory;
 }
@@ -191,6 +217,13 @@ extern "C" int64_t hc_live_allocations()
 {
     std::lock_guard<std::mutex> lk(mu());
     return (int64_t)(allocs().size() + host_allocs().size());
+}
+
+// hipMalloc calls so far (a call that allocates nothing leaves it unchanged).
+extern "C" int64_t hc_malloc_calls()
+{
+    std::lock_guard<std::mutex> lk(mu());
+    return g_malloc_calls;
 }
 
 // ---------------------------------------------------------------------------

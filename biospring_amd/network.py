@@ -393,9 +393,11 @@ class SpringNetwork:
         """Generation number of the last adjointForward record (0: none)."""
         return getattr(self, "_tape", 0)
 
-    def adjointBackward(self, dl_dx=None, dl_dv=None):
+    def adjointBackward(self, dl_dx=None, dl_dv=None, noalloc=False):
         """(dk, dl0, dq) of a loss of the recorded final state, given dL/dx_T
-        and/or dL/dv_T as (n, 3) arrays."""
+        and/or dL/dv_T as (n, 3) arrays.  noalloc: the variant that takes its
+        device buffers from the library's fixed array instead of allocating
+        (bs_adjoint_backward_noalloc; systems up to its size)."""
         c = ctypes
         gx = None if dl_dx is None else np.ascontiguousarray(dl_dx, np.float64).reshape(-1)
         gv = None if dl_dv is None else np.ascontiguousarray(dl_dv, np.float64).reshape(-1)
@@ -404,8 +406,9 @@ class SpringNetwork:
                 raise ValueError("gradients must hold 3 values per particle")
         m = int(self._L.bs_num_springs(self._h))
         dk, dl0, dq = np.zeros(max(m, 1)), np.zeros(max(m, 1)), np.zeros(max(self.n, 1))
-        check(self._L.bs_adjoint_backward(self._h, ptr(gx, c.c_double), ptr(gv, c.c_double), ptr(dk, c.c_double),
-                                          ptr(dl0, c.c_double), ptr(dq, c.c_double)), self._h, "bs_adjoint_backward")
+        name = "bs_adjoint_backward_noalloc" if noalloc else "bs_adjoint_backward"
+        check(getattr(self._L, name)(self._h, ptr(gx, c.c_double), ptr(gv, c.c_double), ptr(dk, c.c_double),
+                                     ptr(dl0, c.c_double), ptr(dq, c.c_double)), self._h, name)
         return dk[:m], dl0[:m], dq[:self.n]
 
     def close(self):

@@ -572,6 +572,18 @@ int bs_adjoint_forward(bs_handle *h, int32_t nsteps, int32_t checkpoint_every, c
 int bs_adjoint_backward(bs_handle *h, const double *dl_dx3, const double *dl_dv3, double *dk, double *dl0,
                         double *dq);
 
+/* bs_adjoint_backward that allocates nothing: the same arguments, results
+ * and errors, with the sweep's device buffers (about 192 bytes per particle
+ * plus 48 per spring when dk or dl0 is wanted) taken from a fixed 64 MiB
+ * array that the library holds on each GPU until the process ends.  Larger
+ * systems are refused with BS_ERR_ARG and a message giving the size needed.
+ * Calls on one GPU share that array: calls that overlap on one GPU race.
+ * BS_ERR_STATE while profiling is on (bs_profile creates events).
+ * No counterpart in the reference; it is for callers that cannot allocate
+ * device memory (fitting loops under a no-allocation rule, serving loops). */
+int bs_adjoint_backward_noalloc(bs_handle *h, const double *dl_dx3, const double *dl_dv3, double *dk,
+                                double *dl0, double *dq);
+
 /* ---- tuning ---------------------------------------------------------------------- */
 
 /* Times `steps` steps of a copy of the current system for each launch

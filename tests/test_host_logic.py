@@ -292,6 +292,66 @@ def test_adjoint_sequence(hostcheck_lib):
     assert "leak 0" in out, out
 
 
+def test_noalloc_backward_allocates_nothing(hostcheck_lib):
+    """bs_adjoint_backward_noalloc: no hipMalloc during the call (the
+    allocating form makes some), every buffer of the sweep inside the fixed
+    array (the stand-in launchers check extents against it), the same
+    results as the allocating form, and a system too large for the array
+    refused with BS_ERR_ARG and the size it needs."""
+    out = _run("""
+        L.hc_malloc_calls.restype = ctypes.c_int64
+        s = workloads.generate(0, seed=4, n=400)
+        fl = s.flags.copy(); fl[::7] |= _abi.FLAG_STATIC; s.flags = fl
+        net = SpringNetwork.from_system(s, library=L)
+        net.setup()
+        net.adjointForward(5, 2)
+        g = np.random.default_rng(1).normal(size=(400, 3))
+        c0 = L.hc_malloc_calls()
+        want = net.adjointBackward(g, g)
+        c1 = L.hc_malloc_calls()
+        got = net.adjointBackward(g, g, noalloc=True)
+        c2 = L.hc_malloc_calls()
+        assert c1 > c0 and c2 == c1, (c0, c1, c2)
+        for a, b in zip(want, got):
+            assert np.array_equal(a, b)
+        for seeds in ((None, g), (g, None)):
+            net.adjointBackward(*seeds, noalloc=True)
+        assert L.hc_malloc_calls() == c2
+        try:
+            net.adjointBackward(None, None, noalloc=True)
+            raise SystemExit("accepted no seed")
+        except _abi.EngineError as e:
+            assert e.code == _abi.BS_ERR_ARG
+        net.profile(True)
+        try:
+            net.adjointBackward(g, noalloc=True)
+            raise SystemExit("accepted profiling")
+        except _abi.EngineError as e:
+            assert e.code == _abi.BS_ERR_STATE
+        net.profile(False)
+        net.run(1)
+        try:
+            net.adjointBackward(g, noalloc=True)
+            raise SystemExit("backward without a record")
+        except _abi.EngineError as e:
+            assert e.code == _abi.BS_ERR_STATE
+        net.close()
+        big = workloads.generate(1, seed=2, n=20000)
+        b = SpringNetwork.from_system(big, library=L)
+        b.adjointForward(1, 1)
+        try:
+            b.adjointBackward(np.ones((20000, 3)), noalloc=True)
+            raise SystemExit("accepted a system beyond the fixed array")
+        except _abi.EngineError as e:
+            assert e.code == _abi.BS_ERR_ARG and "bytes of device scratch" in str(e), str(e)
+        dk, _, _ = b.adjointBackward(np.ones((20000, 3)), noalloc=False)
+        assert dk.shape == (big.m,)
+        b.close()
+        print("noalloc ok")
+    """, hostcheck_lib)
+    assert "noalloc ok" in out, out
+
+
 def test_interactive_updates(hostcheck_lib):
     _run("""
         s = workloads.generate(2, seed=7, n=500)
