@@ -584,8 +584,23 @@ def adjoint_leg(seed=1, n=0, T=200, K=20, warmup=5, fp64=True, cpu=True, check_n
 
     t_fwd, _ = timed(lambda: net.run(T))
     t_adj, (loss, dk, dl, dq) = timed(lambda: net.adjoint(T, y, K))
+    # the same in two calls: the recorded forward, then the backward sweep
+    # (segment recomputation, reverse steps, gradients to the host) from the
+    # MSD loss's seed formed on the host, as a caller with its own loss does
+    t_rec, _ = timed(lambda: net.adjointForward(T, K))
+    xT, _ = net.getState()
+    seed = 2.0 * (xT.astype(np.float64) - y.astype(np.float64)) / n
+    hip_sync()
+    t0 = time.perf_counter()
+    dk2, dl2, dq2 = net.adjointBackward(seed)
+    hip_sync()
+    t_bwd = time.perf_counter() - t0
+    two_call_rel = max(float(np.abs(u - v).max() / max(np.abs(v).max(), 1e-300))
+                       for u, v in ((dk2, dk), (dl2, dl), (dq2, dq)))
     net.close()
-    log(f"adjoint: forward {t_fwd * 1e3:.1f} ms, adjoint {t_adj * 1e3:.1f} ms, ratio {t_adj / t_fwd:.2f}")
+    log(f"adjoint: forward {t_fwd * 1e3:.1f} ms; bs_adjoint {t_adj * 1e3:.1f} ms ({t_adj / t_fwd:.2f}x); "
+        f"recorded forward {t_rec * 1e3:.1f} ms + backward {t_bwd * 1e3:.1f} ms ({t_bwd / t_fwd:.2f}x); "
+        f"two-call gradients vs one call {two_call_rel:.1e}")
 
     ap_ms = prof["adj_pair"]["ms"] / max(prof["adj_pair"]["launches"], 1)
     as_ms = prof["adj_spring"]["ms"] / max(prof["adj_spring"]["launches"], 1)
@@ -616,7 +631,15 @@ def adjoint_leg(seed=1, n=0, T=200, K=20, warmup=5, fp64=True, cpu=True, check_n
         "n_beads": n, "steps": T, "checkpoint_every": K, "springs": s.m, "directed_edges": edges,
         "unique_pairs_initial": pairs_unique, "seed": seed, "digest": s.digest(),
         "value": n * T / t_adj, "unit": "particle-steps/s (forward + adjoint)",
-        "forward_ms": t_fwd * 1e3, "adjoint_ms": t_adj * 1e3, "adjoint_over_forward": t_adj / t_fwd,
+        "forward_ms": t_fwd * 1e3,
+        "adjoint_ms": t_adj * 1e3, "adjoint_call_over_forward": t_adj / t_fwd,
+        "recorded_forward_ms": t_rec * 1e3, "backward_ms": t_bwd * 1e3,
+        "adjoint_over_forward": t_bwd / t_fwd,
+        "adjoint_over_forward_note": ("backward sweep (recomputation of the checkpointed segments, the reverse "
+                                      "steps and the gradients' download) over the plain forward of the same "
+                                      "steps; adjoint_call_over_forward is the whole bs_adjoint call, its own "
+                                      "recorded forward included"),
+        "two_call_gradients_rel_diff": two_call_rel,
         "forward_particle_steps_per_s": n * T / t_fwd,
         "loss": loss, "roofline": roofline, "kernels": kernels,
     }
@@ -694,7 +717,8 @@ def bench_adjoint(args):
                                        "checkpoint_every", "seed", "digest")},
         "roofline": leg["roofline"],
         "kernels": leg["kernels"],
-        "adjoint": {k: leg[k] for k in ("forward_ms", "adjoint_ms", "adjoint_over_forward", "loss")},
+        "adjoint": {k: leg[k] for k in ("forward_ms", "adjoint_ms", "adjoint_call_over_forward", "recorded_forward_ms",
+                                        "backward_ms", "adjoint_over_forward", "loss")},
     }
     for k in ("grad_err_vs_fp64", "cpu_baseline"):
         if k in leg:

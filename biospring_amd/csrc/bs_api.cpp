@@ -1557,6 +1557,7 @@ void launch_adj_pairs(bs_handle *h, const float4 *w, double4 *hw, double *dq)
         a.ncl = h->ncl;
         a.cap = l.cap;
         a.accumulate = li > 0 ? 1 : 0;
+        a.queues = h->p.pair_layout == BS_PAIR_QUEUES ? 1 : 0;
         a.rc = l.rc;
         a.rc2lo = rc2(l.rc, 1.0 - 4e-6);
         a.rc2hi = rc2(l.rc, 1.0 + 4e-6);
@@ -3385,8 +3386,11 @@ int bs_autotune(bs_handle *h, int32_t steps, int32_t *group_out, int32_t *graph_
     if (h->p.graph_steps != bgr || h->p.pair_layout != blay)
         drop_graph(h);
     h->p.graph_steps = bgr;
-    if (h->p.pair_layout != BS_PAIR_ROWS) // row buffers are sized at setup
+    if (h->p.pair_layout != BS_PAIR_ROWS && h->p.pair_layout != blay)
+    { // row buffers are sized at setup; the other two read lists in their own order
         h->p.pair_layout = blay;
+        launch_fill_i32(h->status, 1, 1, nullptr, h->stream);
+    }
     if (group_out)
         *group_out = bg;
     if (graph_out)

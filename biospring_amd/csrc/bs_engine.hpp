@@ -241,6 +241,7 @@ struct AdjPairArgs
     double4 *hw;          // [N] H w of this list (overwritten, or added with accumulate)
     double *dq;           // [N] dL/dq accumulator
     int32_t n, ncl, cap, accumulate;
+    int32_t queues;       // the lists in dealt order, read through per-particle LDS queues (k_adj_pair_q)
     float rc, rc2lo, rc2hi;
     float steric_scale;
     double kf;            // Coulomb force prefactor incl. scale and 1/eps

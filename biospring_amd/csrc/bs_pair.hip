@@ -1157,8 +1157,120 @@ void launch_pair(const PairArgs &a, int32_t elec_mode, int32_t hydro, int32_t nb
     }
 }
 
+// ---------------------------------------------------------------------------
+// The adjoint's pair terms in the queue layout (dealt lists, as k_pair_q):
+// each chunk's j data (position, parameters, adjoint weight; the index j in
+// the weight's unused fourth word, for a static partner's dq) is staged in
+// LDS once, queued per cluster particle, and drained 8 entries per round.
+// The terms and their FP64 sums are k_adj_pair's.
+// ---------------------------------------------------------------------------
+constexpr int32_t ACHUNK = 64; // entries per chunk: three float4 per entry in LDS
+
+template <int SMODE, int EMODE>
+__global__ void __launch_bounds__(BLOCK) k_adj_pair_q(AdjPairArgs a)
+{
+    __shared__ float4 qpos[WAVES][ACHUNK], qprm[WAVES][ACHUNK], qw[WAVES][ACHUNK];
+    __shared__ uint8_t qidx[WAVES][CLUSTER][ACHUNK];
+    const int wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int32_t ci = __builtin_amdgcn_readfirstlane(blockIdx.x * WAVES + wv);
+    if (ci >= a.ncl)
+        return; // whole wave
+    const int32_t il = lane & 7, sl = lane >> 3;
+    const int32_t i = ci * CLUSTER + il;
+    const bool iv = i < a.n;
+    const float4 pi = iv ? a.pos[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 ri = iv ? a.jprm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 wi = iv ? a.w[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const uint32_t fli = fbits(ri.w);
+    const float cs = (float)a.k.c_spring;
+    const float kf = (float)a.kf;
+    double hx = 0.0, hy = 0.0, hz = 0.0, dqi = 0.0;
+    const int32_t cnt = __builtin_amdgcn_readfirstlane(a.cln[ci]);
+    const uint32_t *L = a.cl + (size_t)ci * a.cap;
+    // 64 entries at a time in list order (the dealt order spreads each
+    // 64-entry stretch over the whole list too); the next stretch in flight
+    uint32_t w = lane < cnt ? L[lane] : 0u;
+    uint32_t j = w & CL_JMASK;
+    float4 pj = a.pos[j], rj = a.jprm[j], vj = a.w[j];
+    for (int32_t t0 = 0; t0 < cnt; t0 += ACHUNK)
+    {
+        qpos[wv][lane] = pj;
+        qprm[wv][lane] = rj;
+        qw[wv][lane] = make_float4(vj.x, vj.y, vj.z, __uint_as_float(j));
+        int32_t nq[CLUSTER];
+#pragma unroll
+        for (int k = 0; k < CLUSTER; ++k)
+        {
+            const bool on = (w >> (24 + k)) & 1u;
+            const uint64_t b = __ballot(on);
+            if (on)
+                qidx[wv][k][lanes_below(b)] = (uint8_t)lane;
+            nq[k] = __popcll(b);
+        }
+        const int32_t tn = t0 + ACHUNK + lane;
+        w = tn < cnt ? L[tn] : 0u;
+        j = w & CL_JMASK;
+        pj = a.pos[j];
+        rj = a.jprm[j];
+        vj = a.w[j];
+        wave_lds_sync();
+        int32_t mine = nq[0], most = nq[0];
+#pragma unroll
+        for (int k = 1; k < CLUSTER; ++k)
+        {
+            mine = il == k ? nq[k] : mine;
+            most = max(most, nq[k]);
+        }
+        for (int32_t e0 = 0; e0 < most; e0 += 8) // wave-uniform rounds
+        {
+            const int32_t e = e0 + sl;
+            const bool valid = e < mine;
+            const int32_t src = valid ? (int32_t)qidx[wv][il][e] : 0;
+            const float4 sw = qw[wv][src];
+            adj_pair_term<SMODE, EMODE>(valid, pi, ri, fli, wi, qpos[wv][src], qprm[wv][src],
+                                        make_float4(sw.x, sw.y, sw.z, 0.0f), (int32_t)__float_as_uint(sw.w), a, cs,
+                                        kf, hx, hy, hz, dqi);
+        }
+        wave_lds_sync();
+    }
+    for (int off = 8; off < 64; off <<= 1)
+    {
+        hx += __shfl_xor(hx, off, 64);
+        hy += __shfl_xor(hy, off, 64);
+        hz += __shfl_xor(hz, off, 64);
+        dqi += __shfl_xor(dqi, off, 64);
+    }
+    if (sl == 0 && iv)
+    {
+        double4 h = make_double4(hx, hy, hz, 0.0);
+        if (a.accumulate)
+        {
+            const double4 o = a.hw[i];
+            h.x += o.x;
+            h.y += o.y;
+            h.z += o.z;
+        }
+        a.hw[i] = h;
+        // a static particle's dq is only ever added to atomically (by its
+        // dynamic partners); a plain read-modify-write here would race
+        if (EMODE > 0 && !(fli & PF_STATIC))
+            a.dq[i] += dqi;
+    }
+}
+
 template <int S> static void adj2(const AdjPairArgs &a, int32_t em, int32_t nb, hipStream_t s)
 {
+    if (a.queues)
+    {
+        if (em == 0)
+            k_adj_pair_q<S, 0><<<nb, BLOCK, 0, s>>>(a);
+        else if (em == 1)
+            k_adj_pair_q<S, 1><<<nb, BLOCK, 0, s>>>(a);
+        else
+            k_adj_pair_q<S, 2><<<nb, BLOCK, 0, s>>>(a);
+        return;
+    }
     if (em == 0)
         k_adj_pair<S, 0><<<nb, BLOCK, 0, s>>>(a);
     else if (em == 1)

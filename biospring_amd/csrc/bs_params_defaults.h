@@ -39,7 +39,7 @@ static inline void bs_params_fill_defaults(bs_params *p)
     p->densitygrid_enable = 0;
     p->densitygrid_scale = 1.0f;
     p->graph_steps = 0;
-    p->pair_layout = BS_PAIR_CLUSTERS;
+    p->pair_layout = BS_PAIR_QUEUES;
     p->probe_enable = 0;          /* probe.*, Configuration.hpp:186-195 */
     p->probe_enable_electrostatic = 0;
     p->probe_enable_steric = 0;

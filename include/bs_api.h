@@ -131,18 +131,18 @@ typedef struct bs_params {
        Results are identical either way.  Default 0 */
     int32_t graph_steps;
 
-    /* Engine option: layout the pair kernel reads.  BS_PAIR_CLUSTERS (the
-       default): the cluster lists, 8 particles x 8 entries per wave, one load
-       of j's data serving 8 lanes; lanes of entries a particle does not see
-       stay idle.  BS_PAIR_ROWS: the cluster lists expanded at each rebuild
-       into one list per particle, every lane evaluating a listed pair at one
-       gather per pair (slower where gathers, not arithmetic, bound the
-       kernel).  BS_PAIR_QUEUES: the cluster lists stored in dealt order
-       (chunks of at most 128 entries, each a sample of the whole list) and
-       read a chunk at a time, each entry's j data loaded once into LDS and
-       queued for the particles that see it, so that lanes evaluate listed
-       pairs at the cluster layout's load count.  Forces agree to FP64
-       rounding in every layout. */
+    /* Engine option: layout the pair kernels read.  BS_PAIR_QUEUES (the
+       default): the cluster lists stored in dealt order (chunks of at most
+       128 entries, each a sample of the whole list) and read a chunk at a
+       time, each entry's j data loaded once into LDS and queued for the
+       particles that see it, so that lanes evaluate listed pairs at the
+       cluster layout's load count.  BS_PAIR_CLUSTERS: the cluster lists in
+       search order, 8 particles x 8 entries per wave, one load of j's data
+       serving 8 lanes; lanes of entries a particle does not see stay idle.
+       BS_PAIR_ROWS: the cluster lists expanded at each rebuild into one list
+       per particle, every lane evaluating a listed pair at one gather per
+       pair (slower where gathers, not arithmetic, bound the kernel).  Forces
+       agree to FP64 rounding in every layout. */
     int32_t pair_layout;
 
     /* The probe particle (probe.*, Configuration.hpp:186-195): one extra

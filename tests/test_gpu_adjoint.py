@@ -58,9 +58,13 @@ def _rel(a, b):
     return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
 
 
-@pytest.mark.parametrize("name", ["amber_ddd", "enm", "lewitt", "zacharias"])
-def test_adjoint_matches_fp64_oracle(name):
+@pytest.mark.parametrize("name,layout", [("amber_ddd", _abi.PAIR_QUEUES), ("enm", _abi.PAIR_QUEUES),
+                                         ("lewitt", _abi.PAIR_QUEUES), ("zacharias", _abi.PAIR_QUEUES),
+                                         ("amber_ddd", _abi.PAIR_CLUSTERS), ("enm", _abi.PAIR_CLUSTERS)])
+def test_adjoint_matches_fp64_oracle(name, layout):
+    """Both pair layouts' adjoint kernels (queued and cluster lists)."""
     s, p, T = _case(name)
+    p.pair_layout = layout
     y = _target(s)
     _log(f"adjoint {name}: n={s.n} springs={s.m} T={T}")
     net = SpringNetwork.from_system(s, params=p)

@@ -91,13 +91,16 @@ def test_small_and_degenerate_systems(hostcheck_lib):
 def test_cluster_lists_cover_reference_pairs(hostcheck_lib):
     """The host-built exclusion lists and the cluster search ranges, run
     through the checker's CPU emulation of the list build, reproduce the
-    oracle's processed pair sets exactly (also after motion and a rebuild)."""
+    oracle's processed pair sets exactly (also after motion and a rebuild),
+    in search order and in the queue layout's dealt order, whose chunk
+    queues the checker's pair launcher verifies entry by entry."""
     _run("""
         from oracle import pyoracle as po
-        for cfg, n, seed in ((0, 0, 2), (1, 3000, 4), (1, 12000, 5)):
+        for cfg, n, seed, lay in ((0, 0, 2, 0), (1, 3000, 4, 0), (1, 12000, 5, 0), (0, 0, 2, 2), (1, 12000, 5, 2)):
             s = workloads.generate(cfg, seed=seed, n=n)
-            net = SpringNetwork.from_system(s, library=L)
-            net.setup()
+            p = s.params.copy(); p.pair_layout = lay
+            net = SpringNetwork.from_system(s, params=p, library=L)
+            net.setup(); net.run(2)
             o = po.Oracle(s)
             for kind in (0, 1):
                 assert np.array_equal(net.getPairs(kind), o.pairs(kind)), (cfg, n, kind)
