@@ -21,7 +21,10 @@ Two helper modes serve the HBM-traffic measurement of the kernels
                         that converts the counters' KiB into bytes;
   --pmc-summary F C W D reads the four counter runs (FETCH bench/copy dirs,
                         WRITE bench/copy dirs) and writes
-                        profiles/pmc_traffic.json, which later bench lines cite.
+                        profiles/pmc_traffic.json, which later bench lines cite;
+  --pmc-valu-summary D C  reads one run of the eight SQ_INSTS_VALU_* type
+                        counters (dir D, made by command C) and writes
+                        profiles/pmc_valu.json (executed FP32/FP64 FLOP).
 """
 import argparse
 import ctypes
@@ -155,9 +158,10 @@ def pmc_summary(fetch_dir, fetch_copy_dir, write_dir, write_copy_dir):
     sw = COPY_BYTES / (cw * 1024.0)
     fm = _counter_means(fetch_dir, "FETCH_SIZE")
     wm = _counter_means(write_dir, "WRITE_SIZE")
-    for name, key in (("pair", "k_pair<"), ("spring_integrate", "k_spring<16, true"),
-                      ("cluster_list", "k_cluster_list")):
-        f, w = pick(fm, key), pick(wm, key)
+    for name, keys in (("pair", ("k_pair_q<1, 2, 0, false>", "k_pair<1, 2, 0, false>")),
+                       ("spring_integrate", ("k_spring<16, true, false",)), ("cluster_list", ("k_cluster_list",))):
+        f = next((v for v in (pick(fm, k) for k in keys) if v is not None), None)
+        w = next((v for v in (pick(wm, k) for k in keys) if v is not None), None)
         if f is None or w is None:
             continue
         out[name] = {"fetch_bytes": f * 1024.0 * sf, "write_bytes": w * 1024.0 * sw,
@@ -165,6 +169,36 @@ def pmc_summary(fetch_dir, fetch_copy_dir, write_dir, write_copy_dir):
     out["calibration"] = {"copy_bytes": COPY_BYTES, "copy_fetch_kib": cf, "copy_write_kib": cw,
                           "fetch_scale": sf, "write_scale": sw}
     path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    with open(path, "w") as fh:
+        json.dump(out, fh, indent=1)
+    print(json.dumps(out))
+
+
+VALU_COUNTERS = ("SQ_INSTS_VALU_ADD_F32", "SQ_INSTS_VALU_MUL_F32", "SQ_INSTS_VALU_FMA_F32",
+                 "SQ_INSTS_VALU_TRANS_F32", "SQ_INSTS_VALU_ADD_F64", "SQ_INSTS_VALU_MUL_F64",
+                 "SQ_INSTS_VALU_FMA_F64", "SQ_INSTS_VALU_TRANS_F64")
+
+
+def pmc_valu_summary(valu_dir, command):
+    """profiles/pmc_valu.json from one rocprofv3 --pmc run of the eight
+    SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_{F32,F64} counters: executed FLOP per
+    launch = 64 lanes x (ADD + MUL + 2 FMA + TRANS) wave instructions."""
+    out = {"command": command, "flop_rule": "64 lanes x (ADD + MUL + 2 FMA + TRANS) wave instructions, per type"}
+    means = {c: _counter_means(valu_dir, c) for c in VALU_COUNTERS}
+    for name, keys in (("pair", ("k_pair_q<1, 2, 0, false>", "k_pair<1, 2, 0, false>")),
+                       ("spring_integrate", ("k_spring<16, true, false",))):
+        row = {}
+        for c in VALU_COUNTERS:
+            cand = [v for k, v in means[c].items() if any(key in k for key in keys)]
+            if cand:
+                row[c] = cand[0]
+        if len(row) != len(VALU_COUNTERS):
+            continue
+        flop = lambda t: 64.0 * (row[f"SQ_INSTS_VALU_ADD_{t}"] + row[f"SQ_INSTS_VALU_MUL_{t}"] +
+                                 2.0 * row[f"SQ_INSTS_VALU_FMA_{t}"] + row[f"SQ_INSTS_VALU_TRANS_{t}"])
+        out[name] = {"counters_per_dispatch": row, "executed_fp32_flop": flop("F32"),
+                     "executed_fp64_flop": flop("F64")}
+    path = os.path.join(ROOT, "profiles", "pmc_valu.json")
     with open(path, "w") as fh:
         json.dump(out, fh, indent=1)
     print(json.dumps(out))
@@ -202,11 +236,14 @@ def main():
                     help="workload 1: skip timing the run with trajectory writers")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--pmc-summary", nargs=4, metavar=("FETCH", "FETCH_COPY", "WRITE", "WRITE_COPY"))
+    ap.add_argument("--pmc-valu-summary", nargs=2, metavar=("VALU_DIR", "COMMAND"))
     args = ap.parse_args()
     if args.copy_calibration:
         return copy_calibration()
     if args.pmc_summary:
         return pmc_summary(*args.pmc_summary)
+    if args.pmc_valu_summary:
+        return pmc_valu_summary(*args.pmc_valu_summary)
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is a design note only",
                           "n_gpus": args.gpus}))
@@ -589,10 +626,10 @@ def adjoint_leg(seed=1, n=0, T=200, K=20, warmup=5, fp64=True, cpu=True, check_n
     # MSD loss's seed formed on the host, as a caller with its own loss does
     t_rec, _ = timed(lambda: net.adjointForward(T, K))
     xT, _ = net.getState()
-    seed = 2.0 * (xT.astype(np.float64) - y.astype(np.float64)) / n
+    g_seed = 2.0 * (xT.astype(np.float64) - y.astype(np.float64)) / n
     hip_sync()
     t0 = time.perf_counter()
-    dk2, dl2, dq2 = net.adjointBackward(seed)
+    dk2, dl2, dq2 = net.adjointBackward(g_seed)
     hip_sync()
     t_bwd = time.perf_counter() - t0
     two_call_rel = max(float(np.abs(u - v).max() / max(np.abs(v).max(), 1e-300))

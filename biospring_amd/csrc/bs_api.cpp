@@ -272,7 +272,7 @@ void drop_graph(bs_handle *h)
     h->graph_len = 0;
 }
 
-constexpr int TRAJ_SLOTS = 3; // frames in flight between the stream and the writer thread
+constexpr int TRAJ_SLOTS = 8; // frames in flight between the stream and the file writers
 
 // Writes out the queued frames, closes the files and frees the slots.
 void traj_release(bs_handle *h)

@@ -315,7 +315,7 @@ class FileWriter : public TrajWriter
         if (f)
             std::fclose(f);
     }
-    bool emit(const std::string &s)
+    bool emit(const std::string &s) override
     {
         if (!s.empty() && std::fwrite(s.data(), 1, s.size(), f) != s.size())
             return false;
@@ -343,14 +343,14 @@ template <class T> const T &at_or(const std::vector<T> &v, int64_t i, const T &d
 class PdbWriter : public FileWriter
 {
   public:
-    bool write(const TrajFrame &fr) override
+    bool encode(const TrajFrame &fr, int64_t index, std::string &s) override
     {
         const TrajContext &c = *ctx;
         const TrajLabels &L = c.labels;
         static const std::string empty;
-        std::string s;
+        s.clear();
         s.reserve((size_t)(c.n + 2) * 81);
-        s += "MODEL    " + std::to_string(frame) + "\n";
+        s += "MODEL    " + std::to_string(index) + "\n";
         for (int64_t i = 0; i < c.n; ++i)
         {
             const std::string &nm = at_or(L.name, i, empty);
@@ -371,18 +371,18 @@ class PdbWriter : public FileWriter
                     (double)fr.x3[3 * i + 2], (double)at_or(L.occupancy, i, 0.0f),
                     (double)at_or(L.tempfactor, i, 0.0f), at_or(L.element, i, empty).c_str(), charge);
         }
-        if (frame == 0)
+        if (index == 0)
             for (const auto &sp : c.conect)
                 appendf(s, "CONECT%5d%5d\n", (int)sp.first + 1, (int)sp.second + 1);
         s += "ENDMDL\n";
-        return emit(s);
+        return true;
     }
 };
 
 class XtcWriter : public FileWriter
 {
   public:
-    bool write(const TrajFrame &fr) override
+    bool encode(const TrajFrame &fr, int64_t index, std::string &s) override
     {
         const int64_t n = ctx->n;
         std::vector<float> nm((size_t)(3 * n));
@@ -390,8 +390,9 @@ class XtcWriter : public FileWriter
             nm[(size_t)k] = (float)((double)fr.x3[k] / 10.0);
         const float box[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
         std::vector<uint8_t> out;
-        xtc_encode_frame(out, (int32_t)n, (int32_t)frame, (float)frame, box, nm.data(), 1000.0f);
-        return std::fwrite(out.data(), 1, out.size(), f) == out.size() && std::fflush(f) == 0;
+        xtc_encode_frame(out, (int32_t)n, (int32_t)index, (float)index, box, nm.data(), 1000.0f);
+        s.assign(reinterpret_cast<const char *>(out.data()), out.size());
+        return true;
     }
 };
 
@@ -419,10 +420,12 @@ class CsvWriter : public FileWriter
         s += "\n";
         return emit(s);
     }
-    bool write(const TrajFrame &fr) override
+    // a row holds the frame rate, known on the ordered path only
+    bool ordered() const override { return true; }
+    bool encode(const TrajFrame &fr, int64_t, std::string &s) override
     {
         const TrajContext &c = *ctx;
-        std::string s = std::to_string(fr.nbiter) + "\t";
+        s = std::to_string(fr.nbiter) + "\t";
         put_g(s, (float)fr.framerate);
         s += "\t";
         put_g(s, (float)fr.energies[3]);
@@ -454,7 +457,7 @@ class CsvWriter : public FileWriter
             put_g(s, (float)cz);
         }
         s += "\n";
-        return emit(s);
+        return true;
     }
 };
 
@@ -513,7 +516,6 @@ void TrajectorySet::add(std::unique_ptr<TrajWriter> w)
     std::lock_guard<std::mutex> lk(mu_);
     writers_.push_back(std::move(w));
 }
-
 void TrajectorySet::release(int s)
 {
     {
@@ -522,7 +524,6 @@ void TrajectorySet::release(int s)
     }
     cv_.notify_all();
 }
-
 bool TrajectorySet::due(int64_t nb) const
 {
     for (const auto &w : writers_)
@@ -530,7 +531,6 @@ bool TrajectorySet::due(int64_t nb) const
             return true;
     return false;
 }
-
 int64_t TrajectorySet::next_due(int64_t nb) const
 {
     int64_t best = INT64_MAX;
@@ -538,79 +538,132 @@ int64_t TrajectorySet::next_due(int64_t nb) const
         best = std::min(best, (nb / w->frequency + 1) * w->frequency);
     return best;
 }
-
 int TrajectorySet::acquire()
 {
     std::unique_lock<std::mutex> lk(mu_);
     if (!th_.joinable())
+    {
         th_ = std::thread([this] { run(); });
+        const int hw = (int)std::thread::hardware_concurrency();
+        const int ne = std::max(1, std::min((int)free_.size(), hw / 2));
+        for (int k = 0; k < ne; ++k)
+            enc_.emplace_back([this] { encode_loop(); });
+    }
     cv_.wait(lk, [&] { return std::find(free_.begin(), free_.end(), true) != free_.end(); });
     const int s = (int)(std::find(free_.begin(), free_.end(), true) - free_.begin());
     free_[(size_t)s] = false;
     return s;
 }
-
 void TrajectorySet::submit(int s, const TrajFrame &f, std::function<std::string()> wait, bool all)
 {
+    auto job = std::make_shared<Job>();
+    job->slot = s;
+    job->f = f;
+    job->wait = std::move(wait);
     {
         std::lock_guard<std::mutex> lk(mu_);
-        q_.push_back(Item{s, f, std::move(wait), all});
+        // each due writer's frame number, in submission order
+        for (size_t k = 0; k < writers_.size(); ++k)
+            if (all || f.nbiter % writers_[k]->frequency == 0)
+            {
+                job->writer.push_back((int)k);
+                job->index.push_back(writers_[k]->frame++);
+            }
+        job->bytes.resize(job->writer.size());
+        q_.push_back(job);
+        todo_.push_back(job);
     }
     cv_.notify_all();
 }
 
-void TrajectorySet::run()
+// Encoder threads: wait for a frame's copy, then format it for every due
+// writer that does not need the ordered path, frames in parallel.
+void TrajectorySet::encode_loop()
 {
     for (;;)
     {
-        Item it;
+        std::shared_ptr<Job> job;
         {
             std::unique_lock<std::mutex> lk(mu_);
-            cv_.wait(lk, [&] { return stop_ || !q_.empty(); });
-            if (q_.empty())
+            cv_.wait(lk, [&] { return stop_ || !todo_.empty(); });
+            if (todo_.empty())
                 return;
-            it = std::move(q_.front());
-            q_.pop_front();
-            busy_ = true;
+            job = todo_.front();
+            todo_.pop_front();
         }
         std::string e;
         try
         {
-            e = it.wait();
+            e = job->wait();
         }
         catch (const std::exception &x)
         {
             e = std::string("frame copy failed: ") + x.what();
         }
-        const bool stopped = it.f.status && (it.f.status[2] != 0 || it.f.status[5] != 0 ||
-                                             it.f.status[1] != INT32_MAX);
-        if (!e.empty() || stopped)
-        { // not written: a failed copy (reported), or a frame after the call stopped
+        const int32_t *st = job->f.status;
+        const bool stopped = st && (st[2] != 0 || st[5] != 0 || st[1] != INT32_MAX);
+        if (e.empty() && !stopped)
+            for (size_t j = 0; j < job->writer.size(); ++j)
             {
-                std::lock_guard<std::mutex> lk(mu_);
-                if (!e.empty() && err_.empty())
-                    err_ = e;
-                free_[(size_t)it.slot] = true;
-                busy_ = false;
+                TrajWriter &w = *writers_[(size_t)job->writer[j]];
+                if (w.ordered())
+                    continue;
+                try
+                {
+                    if (!w.encode(job->f, job->index[j], job->bytes[j]))
+                        e = "write failed: " + w.path;
+                }
+                catch (const std::exception &x)
+                {
+                    e = std::string("write failed: ") + x.what();
+                }
+                if (!e.empty())
+                    break;
             }
-            cv_.notify_all();
-            continue;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            job->err = e;
+            job->skip = !e.empty() || stopped;
+            job->encoded = true;
         }
-        const double t =
-            std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
-        // steps per second between the completions of consecutive frames
-        it.f.framerate = (last_nb_ >= 0 && t > last_t_ && it.f.nbiter > last_nb_)
-                             ? (double)(it.f.nbiter - last_nb_) / (t - last_t_)
-                             : 0.0;
-        last_nb_ = it.f.nbiter;
-        last_t_ = t;
-        for (auto &w : writers_)
-            if (it.all || it.f.nbiter % w->frequency == 0)
+        cv_.notify_all();
+    }
+}
+
+// The ordered path: frames in submission order, each once its encoding is
+// done; the ordered writers' rows, then every payload appended to its file.
+void TrajectorySet::run()
+{
+    for (;;)
+    {
+        std::shared_ptr<Job> job;
+        {
+            std::unique_lock<std::mutex> lk(mu_);
+            cv_.wait(lk, [&] { return (stop_ && q_.empty()) || (!q_.empty() && q_.front()->encoded); });
+            if (q_.empty())
+                return;
+            job = q_.front();
+            q_.pop_front();
+            busy_ = true;
+        }
+        std::string e = job->err;
+        if (!job->skip)
+        {
+            const double t =
+                std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+            // steps per second between the completions of consecutive frames
+            job->f.framerate = (last_nb_ >= 0 && t > last_t_ && job->f.nbiter > last_nb_)
+                                   ? (double)(job->f.nbiter - last_nb_) / (t - last_t_)
+                                   : 0.0;
+            last_nb_ = job->f.nbiter;
+            last_t_ = t;
+            for (size_t j = 0; j < job->writer.size(); ++j)
             {
+                TrajWriter &w = *writers_[(size_t)job->writer[j]];
                 bool ok = false;
                 try
                 {
-                    ok = w->write(it.f);
+                    ok = (!w.ordered() || w.encode(job->f, job->index[j], job->bytes[j])) && w.emit(job->bytes[j]);
                 }
                 catch (const std::exception &x)
                 {
@@ -618,27 +671,25 @@ void TrajectorySet::run()
                     e = e.empty() ? std::string("write failed: ") + x.what() : e;
                 }
                 if (!ok && e.empty())
-                    e = "write failed: " + w->path;
-                ++w->frame;
+                    e = "write failed: " + w.path;
             }
+        }
         {
             std::lock_guard<std::mutex> lk(mu_);
             if (!e.empty() && err_.empty())
                 err_ = e;
-            free_[(size_t)it.slot] = true;
+            free_[(size_t)job->slot] = true;
             busy_ = false;
         }
         cv_.notify_all();
     }
 }
-
 bool TrajectorySet::drain()
 {
     std::unique_lock<std::mutex> lk(mu_);
     cv_.wait(lk, [&] { return q_.empty() && !busy_; });
     return err_.empty();
 }
-
 void TrajectorySet::close()
 {
     if (th_.joinable())
@@ -650,6 +701,9 @@ void TrajectorySet::close()
         }
         cv_.notify_all();
         th_.join();
+        for (auto &t : enc_)
+            t.join();
+        enc_.clear();
     }
     writers_.clear();
 }

@@ -3,9 +3,12 @@
 // (PDBTrajectoryWriter.cpp:10-33, pdb_format.hpp:17-58), XTC frames
 // (XTCTrajectoryWriter.cpp:13-31 over the xdrfile XTC format) and the CSV
 // energy table (CSVTrajectoryWriter.cpp:11-60) -- fed from frames that the
-// engine copies out of HBM into pinned host slots at step boundaries.  A
-// writer thread formats and writes each frame once its copy has landed, so the
-// stream keeps stepping while files are written.
+// engine copies out of HBM into pinned host slots at step boundaries.
+// Encoder threads format frames in parallel once their copies have landed
+// (an XTC frame of 250k beads takes milliseconds to compress); one writer
+// thread appends them to the files in frame order, so the stream keeps
+// stepping while files are written and the files hold the same bytes as a
+// sequential writer's.
 //
 // Host code only: no HIP types here.  The engine hands each frame over with a
 // `wait` callback (an event synchronisation) and gets the slot back through
@@ -69,10 +72,16 @@ class TrajWriter
 {
   public:
     virtual ~TrajWriter() = default;
-    // Returns false after an I/O error (the file stays as written so far).
-    virtual bool write(const TrajFrame &f) = 0;
+    // The bytes of frame f as this writer's frame number `index`; false on a
+    // formatting failure.  Called from any thread unless ordered().
+    virtual bool encode(const TrajFrame &f, int64_t index, std::string &out) = 0;
+    // Appends bytes to the file; false after an I/O error (the file stays as
+    // written so far).  Called in frame order.
+    virtual bool emit(const std::string &bytes) = 0;
+    // encode needs the frame order (the CSV rows' frame rate).
+    virtual bool ordered() const { return false; }
     int64_t frequency = 1;
-    int64_t frame = 0; // frames written by this writer
+    int64_t frame = 0; // frames given to this writer
     std::string path;
 };
 
@@ -117,20 +126,27 @@ class TrajectorySet
     const std::string &error() const { return err_; }
 
   private:
-    struct Item
+    struct Job
     {
-        int slot;
+        int slot = -1;
         TrajFrame f;
         std::function<std::string()> wait;
-        bool all;
+        std::vector<int> writer;        // the due writers
+        std::vector<int64_t> index;     // their frame numbers
+        std::vector<std::string> bytes; // their payloads
+        std::string err;
+        bool skip = false, encoded = false;
     };
     void run();
+    void encode_loop();
     std::vector<std::unique_ptr<TrajWriter>> writers_;
     std::vector<bool> free_;
-    std::deque<Item> q_;
+    std::deque<std::shared_ptr<Job>> q_;    // submission order, until written
+    std::deque<std::shared_ptr<Job>> todo_; // waiting for an encoder
     std::mutex mu_;
     std::condition_variable cv_;
     std::thread th_;
+    std::vector<std::thread> enc_;
     bool stop_ = false, busy_ = false;
     std::string err_;
     int64_t last_nb_ = -1;
