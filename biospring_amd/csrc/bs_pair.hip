@@ -177,73 +177,116 @@ __global__ void __launch_bounds__(BLOCK) k_cluster_list(ListArgs a)
             const double hi = c == a.dims[q] - 1 ? INFINITY : cmn[q] + (c + 1) * ce[q];
             return fmax(0.0, fmax(lo - (double)bmax[q], (double)bmin[q] - hi));
         };
-        for (int32_t cz = clo[2]; cz <= chi[2]; ++cz)
-        {
-            const double gz = gap(2, cz);
-            if (gz > rad)
-                continue;
-            for (int32_t cy = clo[1]; cy <= chi[1]; ++cy)
-            {
-                const double gy = gap(1, cy);
-                const double r2x = rad * rad - gz * gz - gy * gy;
+        // The candidates form a stream of batches of at most 64 (a row's
+        // cells cx0..cx1 are contiguous in the cell array), generated in row
+        // order.  A three-stage pipeline runs over it: while batch b is
+        // tested, batch b+1's positions and batch b+2's particle indices are
+        // in flight, so a batch never waits for its own loads.
+        int32_t gz = clo[2], gy = clo[1], rt = 0, re = 0; // generator: next row, current row's rest
+        auto next_batch = [&](int32_t &bt, int32_t &be) {
+            while (rt >= re && gz <= chi[2])
+            { // the next row within the radius
+                const int32_t cz = gz, cy = gy;
+                if (++gy > chi[1])
+                {
+                    gy = clo[1];
+                    ++gz;
+                }
+                const double gzv = gap(2, cz);
+                if (gzv > rad)
+                    continue;
+                const double gyv = gap(1, cy);
+                const double r2x = rad * rad - gzv * gzv - gyv * gyv;
                 if (r2x < 0.0)
                     continue;
                 const double rx = sqrt(r2x);
                 const double mn0 = a.box[0], len0 = a.box[3] - a.box[0];
                 const int32_t cx0 = internal_coord((float)((double)bmin[0] - rx), mn0, len0, a.dims[0], a.cell_edge);
                 const int32_t cx1 = internal_coord((float)((double)bmax[0] + rx), mn0, len0, a.dims[0], a.cell_edge);
-                // cells cx = cx0..cx1 are contiguous in the cell array
                 const int32_t row = cy * a.dims[0] + cz * a.dims[0] * a.dims[1];
-                const int32_t s = a.cell_start[row + cx0];
-                const int32_t e = a.cell_start[row + cx1 + 1];
-                for (int32_t t = s; t < e; t += 64)
+                rt = a.cell_start[row + cx0];
+                re = a.cell_start[row + cx1 + 1];
+            }
+            if (rt < re)
+            {
+                bt = rt;
+                be = re;
+                rt += 64;
+            }
+            else
+                bt = be = 0; // exhausted
+        };
+        // Index loads are clamped into the cell array (n entries); a lane
+        // past its batch's end gathers particle 0 and tests nothing.
+        int32_t t0, e0, t1, e1, t2, e2;
+        next_batch(t0, e0);
+        next_batch(t1, e1);
+        const int32_t last = a.n > 0 ? a.n - 1 : 0;
+        int32_t j0 = a.cell_idx[min(t0 + lane, last)];
+        int32_t j1 = a.cell_idx[min(t1 + lane, last)];
+        j0 = t0 + lane < e0 ? j0 : 0;
+        float4 p0 = a.pos[j0];
+        float4 o0 = a.shift ? a.shift[j0] : make_float4(0.f, 0.f, 0.f, 0.f);
+        uint32_t f0 = fbits(a.jprm[j0].w);
+        while (t0 < e0) // wave-uniform
+        {
+            next_batch(t2, e2);
+            // batch b+1's positions and b+2's indices in flight
+            j1 = t1 + lane < e1 ? j1 : 0;
+            const float4 p1 = a.pos[j1];
+            const float4 o1 = a.shift ? a.shift[j1] : make_float4(0.f, 0.f, 0.f, 0.f);
+            const uint32_t f1 = fbits(a.jprm[j1].w);
+            const int32_t j2 = a.cell_idx[min(t2 + lane, last)];
+            // batch b
+            uint32_t m = 0u;
+            const int32_t j = j0;
+            if (t0 + lane < e0)
+            {
+                float4 pj = p0;
+                bool same_sys = true;
+                if (a.shift)
+                { // independent systems never pair, however far they move from their setup places
+                    pj.x += o0.x;
+                    pj.y += o0.y;
+                    pj.z += o0.z;
+                    same_sys = fbits(o0.w) == tag0;
+                }
+                if (same_sys && (a.j_mask == 0u || (f0 & a.j_mask)))
                 {
-                    const int32_t idx = t + lane;
-                    uint32_t m = 0u;
-                    int32_t j = 0;
-                    if (idx < e)
-                    {
-                        j = a.cell_idx[idx];
-                        float4 pj = a.pos[j];
-                        bool same_sys = true;
-                        if (a.shift)
-                        { // independent systems never pair, however far they move from their setup places
-                            const float4 o = a.shift[j];
-                            pj.x += o.x;
-                            pj.y += o.y;
-                            pj.z += o.z;
-                            same_sys = fbits(o.w) == tag0;
-                        }
-                        const uint32_t fj = fbits(a.jprm[j].w);
-                        if (same_sys && (a.j_mask == 0u || (fj & a.j_mask)))
-                        {
 #pragma unroll
-                            for (int k = 0; k < CLUSTER; ++k)
-                            {
-                                const float dx = pj.x - xi[k], dy = pj.y - yi[k], dz = pj.z - zi[k];
-                                const float r2 = dx * dx + dy * dy + dz * dz;
-                                if (r2 < a.r2list && j != ci * CLUSTER + k)
-                                    m |= 1u << k;
-                            }
-                            m &= vmask;
-                            if (m != 0u && a.exclude_bonded)
-                                m &= ~(ex_lds ? bonded_mask_lds(tab, j) : bonded_mask(a.ex_ptr, a.ex_w, ci, j));
-                        }
-                    }
-                    const uint64_t bal = __ballot(m != 0u);
-                    if (bal != 0ull)
+                    for (int k = 0; k < CLUSTER; ++k)
                     {
-                        const int32_t off = __popcll(bal & ((1ull << lane) - 1ull));
-                        if (m != 0u)
-                        {
-                            const int32_t slot = count + off;
-                            if (slot < a.cap)
-                                (a.deal ? a.clt : a.cl)[(size_t)ci * a.cap + slot] = (uint32_t)j | (m << 24);
-                        }
-                        count += __popcll(bal);
+                        const float dx = pj.x - xi[k], dy = pj.y - yi[k], dz = pj.z - zi[k];
+                        const float r2 = dx * dx + dy * dy + dz * dz;
+                        if (r2 < a.r2list && j != ci * CLUSTER + k)
+                            m |= 1u << k;
                     }
+                    m &= vmask;
+                    if (m != 0u && a.exclude_bonded)
+                        m &= ~(ex_lds ? bonded_mask_lds(tab, j) : bonded_mask(a.ex_ptr, a.ex_w, ci, j));
                 }
             }
+            const uint64_t bal = __ballot(m != 0u);
+            if (bal != 0ull)
+            {
+                const int32_t off = __popcll(bal & ((1ull << lane) - 1ull));
+                if (m != 0u)
+                {
+                    const int32_t slot = count + off;
+                    if (slot < a.cap)
+                        (a.deal ? a.clt : a.cl)[(size_t)ci * a.cap + slot] = (uint32_t)j | (m << 24);
+                }
+                count += __popcll(bal);
+            }
+            t0 = t1;
+            e0 = e1;
+            t1 = t2;
+            e1 = e2;
+            j0 = j1;
+            j1 = j2;
+            p0 = p1;
+            o0 = o1;
+            f0 = f1;
         }
     }
     // Deal the entries into their chunks (dealt order: entry p to chunk
