@@ -158,7 +158,7 @@ struct bs_handle
     int64_t nslots = -1;
     int32_t *ex_ptr = nullptr;                        // [ncl + 1] static exclusions
     uint32_t *ex_w = nullptr;                         // j << 8 | bonded mask
-    int32_t *cell_of = nullptr, *counts = nullptr, *cstart = nullptr, *cursor = nullptr, *ctmp = nullptr,
+    int32_t *cell_of = nullptr, *counts = nullptr, *cstart = nullptr, *ctmp = nullptr,
             *cidx = nullptr;
     int32_t cell_cap = 0;
     double *box_part = nullptr, *box = nullptr;
@@ -322,7 +322,7 @@ void free_device(bs_handle *h)
     }
     f(h->vel); f(h->fnb); f(h->jprm); f(h->iprm); f(h->xref); f(h->fext); f(h->fout);
     f(h->sp_ptr); f(h->sp_j); f(h->perm_d); f(h->sp_k); f(h->sp_l0);
-    f(h->cell_of); f(h->counts); f(h->cstart); f(h->cursor); f(h->ctmp); f(h->cidx);
+    f(h->cell_of); f(h->counts); f(h->cstart); f(h->ctmp); f(h->cidx);
     f(h->box_part); f(h->box); f(h->energies_d); f(h->status); f(h->counters);
     f(h->epp); f(h->ese); f(h->seg_start); f(h->sys_seg); f(h->seg_part); f(h->sys_E); f(h->shift);
     h->epp = h->ese = nullptr;
@@ -359,7 +359,7 @@ void free_device(bs_handle *h)
     h->fnb = nullptr;
     h->sp_ptr = h->sp_j = h->perm_d = nullptr;
     h->sp_k = h->sp_l0 = nullptr;
-    h->cell_of = h->counts = h->cstart = h->cursor = h->ctmp = h->cidx = nullptr;
+    h->cell_of = h->counts = h->cstart = h->ctmp = h->cidx = nullptr;
     h->box_part = h->box = h->energies_d = nullptr;
     h->status = h->counters = nullptr;
     h->cell_cap = 0;
@@ -373,7 +373,7 @@ int check_device(bs_handle *h)
 {
     const void *need[] = {h->pos[0], h->pos[1], h->vel, h->fnb, h->jprm, h->iprm, h->xref, h->fext, h->fout,
                           h->sp_ptr, h->sp_j, h->sp_k, h->sp_l0, h->perm_d, h->cell_of, h->ctmp, h->cidx,
-                          h->counts, h->cstart, h->cursor, h->box, h->box_part, h->epp, h->ese, h->seg_start,
+                          h->counts, h->cstart, h->box, h->box_part, h->epp, h->ese, h->seg_start,
                           h->sys_seg, h->seg_part, h->sys_E, h->energies_d, h->status, h->counters, h->ex_ptr,
                           h->ex_w};
     for (const void *q : need)
@@ -645,13 +645,13 @@ int ensure_cells(bs_handle *h, int32_t ncells)
     };
     f(h->counts);
     f(h->cstart);
-    f(h->cursor);
     h->counts = dalloc<int32_t>((size_t)ncells, e);
     HIPCHK(h, e);
     h->cstart = dalloc<int32_t>((size_t)ncells + 1, e);
     HIPCHK(h, e);
-    h->cursor = dalloc<int32_t>((size_t)ncells, e);
-    HIPCHK(h, e);
+    // every binning counts its cells back down to zero (launch_scatter's
+    // drain), so the counts are zeroed only here
+    launch_fill_i32(h->counts, 0, ncells, nullptr, h->stream);
     h->cell_cap = ncells;
     return BS_OK;
 }
@@ -665,12 +665,13 @@ void build_list(bs_handle *h, List &l, bool gated, int32_t cap_override)
     const int32_t n = (int32_t)h->n;
     const int32_t *g = gate_of(h, gated);
     hipStream_t s = h->stream;
-    launch_fill_i32(h->counts, 0, l.ncells, g, s);
-    launch_cell_ids(h->pos[h->cur], h->shift, h->jprm, l.jmask, n, h->box, l.dims[0], l.dims[1], l.dims[2], 0.0,
+    // the box from the partials of launch_box_partial; the first list also
+    // takes the reference positions of the drift test
+    launch_cell_ids(h->pos[h->cur], h->shift, h->jprm, l.jmask, n, h->box, h->box_part, h->box_blocks,
+                    &l == &h->lists.front() ? h->xref : nullptr, l.dims[0], l.dims[1], l.dims[2], 0.0,
                     0.5f * l.radius, 0, h->cell_of, h->counts, g, s);
     launch_scan(h->counts, h->cstart, l.ncells, g, s);
-    launch_fill_i32(h->cursor, 0, l.ncells, g, s);
-    launch_scatter(h->cell_of, n, h->cstart, h->cursor, h->ctmp, g, s);
+    launch_scatter(h->cell_of, n, h->cstart, h->counts, 1, h->ctmp, g, s);
     launch_cell_sort(h->cell_of, n, h->cstart, h->ctmp, nullptr, h->cidx, g, s);
     ListArgs a{};
     a.pos = h->pos[h->cur];
@@ -711,10 +712,9 @@ void rebuild_chain(bs_handle *h, bool gated)
         return;
     }
     const int32_t *g = gate_of(h, gated);
-    launch_box_reduce(h->pos[h->cur], h->shift, (int32_t)h->n, h->box_part, h->box, h->box_blocks, g, h->stream);
+    launch_box_partial(h->pos[h->cur], h->shift, (int32_t)h->n, h->box_part, h->box_blocks, g, h->stream);
     for (auto &l : h->lists)
         build_list(h, l, gated, -1);
-    launch_finish_rebuild(h->pos[h->cur], h->xref, (int32_t)h->n, h->status, h->counters, h->stream);
     launch_clear_rebuild(h->status, h->counters, h->stream);
 }
 
@@ -2476,12 +2476,10 @@ int bs_get_cells(bs_handle *h, int32_t kind, int64_t *ncells_out, int64_t *nmemb
         return rc;
     const uint32_t mask = kind == BS_NS_ELECTROSTATIC ? PF_CHARGED : kind == BS_NS_HYDROPHOBIC ? PF_HYDRO : 0u;
     hipStream_t s = h->stream;
-    launch_fill_i32(h->counts, 0, ncells, nullptr, s);
-    launch_cell_ids(h->pos[h->cur], nullptr, h->jprm, mask, n, h->box, (int32_t)d[0], (int32_t)d[1], (int32_t)d[2], (double)r,
-                    r, 1, h->cell_of, h->counts, nullptr, s);
+    launch_cell_ids(h->pos[h->cur], nullptr, h->jprm, mask, n, h->box, nullptr, 0, nullptr, (int32_t)d[0],
+                    (int32_t)d[1], (int32_t)d[2], (double)r, r, 1, h->cell_of, h->counts, nullptr, s);
     launch_scan(h->counts, h->cstart, (int32_t)ncells, nullptr, s);
-    launch_fill_i32(h->cursor, 0, ncells, nullptr, s);
-    launch_scatter(h->cell_of, n, h->cstart, h->cursor, h->ctmp, nullptr, s);
+    launch_scatter(h->cell_of, n, h->cstart, h->counts, 1, h->ctmp, nullptr, s);
     launch_cell_sort(h->cell_of, n, h->cstart, h->ctmp, h->perm_d, h->cidx, nullptr, s);
     std::vector<int32_t> cof((size_t)n), cst((size_t)ncells + 1), cid((size_t)n);
     HIPCHK(h, hipMemcpyAsync(cof.data(), h->cell_of, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
@@ -3114,11 +3112,11 @@ static int springs_from_cutoff(bs_handle *h, double cutoff, float stiffness, con
         int32_t *rcnt = S.get<int32_t>((size_t)n, e);
         HIPCHK(h, e);
         launch_fill_i32(counts, 0, ncells, nullptr, st);
-        launch_cell_ids(pos, nullptr, pos, 0u, (int32_t)n, box, dims[0], dims[1], dims[2], 0.0, edge, 0, cell_of, counts,
-                        nullptr, st);
+        launch_cell_ids(pos, nullptr, pos, 0u, (int32_t)n, box, nullptr, 0, nullptr, dims[0], dims[1], dims[2], 0.0, edge,
+                        0, cell_of, counts, nullptr, st);
         launch_scan(counts, cstart, (int32_t)ncells, nullptr, st);
         launch_fill_i32(cursor, 0, ncells, nullptr, st);
-        launch_scatter(cell_of, (int32_t)n, cstart, cursor, tmp, nullptr, st);
+        launch_scatter(cell_of, (int32_t)n, cstart, cursor, 0, tmp, nullptr, st);
         launch_cell_sort(cell_of, (int32_t)n, cstart, tmp, nullptr, cidx, nullptr, st);
         TopoArgs a{};
         a.pos = pos;

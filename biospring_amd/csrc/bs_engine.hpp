@@ -324,13 +324,21 @@ struct ListArgs
 // keeps independent systems of a batch apart on the engine's grid.
 void launch_box_reduce(const float4 *pos, const float4 *shift, int32_t n, double *partial, double *box,
                        int32_t nblocks, const int32_t *gate, hipStream_t s);
+// The first half of launch_box_reduce: nblocks (<= BLOCK) block partials,
+// reduced by launch_cell_ids (box_part).
+void launch_box_partial(const float4 *pos, const float4 *shift, int32_t n, double *partial, int32_t nblocks,
+                        const int32_t *gate, hipStream_t s);
+// box_part (nullable): the box is first reduced from box_nb partials and
+// stored in box; otherwise box is read.  xref (nullable): positions copied
+// there as well.
 void launch_cell_ids(const float4 *pos, const float4 *shift, const float4 *jprm, uint32_t mask, int32_t n,
-                     const double *box, int32_t dims0, int32_t dims1, int32_t dims2, double edge_exact,
-                     float min_edge, int32_t exact_mode, int32_t *cell_of, int32_t *counts, const int32_t *gate,
-                     hipStream_t s);
+                     double *box, const double *box_part, int32_t box_nb, float4 *xref, int32_t dims0,
+                     int32_t dims1, int32_t dims2, double edge_exact, float min_edge, int32_t exact_mode,
+                     int32_t *cell_of, int32_t *counts, const int32_t *gate, hipStream_t s);
 void launch_scan(const int32_t *counts, int32_t *start, int32_t ncells, const int32_t *gate, hipStream_t s);
-void launch_scatter(const int32_t *cell_of, int32_t n, const int32_t *start, int32_t *cursor, int32_t *tmp_idx,
-                    const int32_t *gate, hipStream_t s);
+// drain: cursor is the counts array, left at zero; otherwise a zeroed cursor.
+void launch_scatter(const int32_t *cell_of, int32_t n, const int32_t *start, int32_t *cursor, int32_t drain,
+                    int32_t *tmp_idx, const int32_t *gate, hipStream_t s);
 void launch_cell_sort(const int32_t *cell_of, int32_t n, const int32_t *start, const int32_t *tmp_idx,
                       const int32_t *key, int32_t *out_idx, const int32_t *gate, hipStream_t s);
 void launch_list_build(const ListArgs &a, hipStream_t s);
@@ -353,8 +361,6 @@ void launch_energies(const double4 *epp, const double4 *eps, const int32_t *seg_
 void launch_probe(const ProbeArgs &p, int32_t nparts, float dt, int32_t visc_enable, float visc, double c_kinetic,
                   int32_t integrate, hipStream_t s);
 void launch_prestep(int32_t *status, int32_t *step_counter, int32_t interval, hipStream_t s);
-void launch_finish_rebuild(const float4 *pos, float4 *xref, int32_t n, int32_t *status, int32_t *counters,
-                           hipStream_t s);
 void launch_clear_rebuild(int32_t *status, int32_t *counters, hipStream_t s);
 void launch_rigid(const RigidArgs &a, hipStream_t s);
 void launch_adj_pair(const AdjPairArgs &a, int32_t steric_mode, int32_t elec_mode, int32_t nblocks, hipStream_t s);

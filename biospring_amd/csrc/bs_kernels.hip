@@ -71,12 +71,10 @@ __global__ void k_box_partial(const float4 *pos, const float4 *shift, int32_t n,
     }
 }
 
-// Combines up to BLOCK block partials (one per thread) with wave shuffles.
-__global__ void __launch_bounds__(BLOCK) k_box_final(const double *partial, int32_t nb, double *box,
-                                                     const int32_t *gate)
+// Combines up to BLOCK block partials (one per thread) with wave shuffles
+// into out[6]; every thread of the (BLOCK-wide) block takes part.
+__device__ void block_box_final(const double *partial, int32_t nb, double *out)
 {
-    if (gated_off(gate))
-        return;
     __shared__ double sh[6][BLOCK / 64];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     double v[6];
@@ -100,8 +98,16 @@ __global__ void __launch_bounds__(BLOCK) k_box_final(const double *partial, int3
         double r = sh[t][0];
         for (int w = 1; w < BLOCK / 64; ++w)
             r = t < 3 ? fmin(r, sh[t][w]) : fmax(r, sh[t][w]);
-        box[t] = r;
+        out[t] = r;
     }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_box_final(const double *partial, int32_t nb, double *box,
+                                                     const int32_t *gate)
+{
+    if (gated_off(gate))
+        return;
+    block_box_final(partial, nb, box);
 }
 
 // ---------------------------------------------------------------------------
@@ -109,16 +115,38 @@ __global__ void __launch_bounds__(BLOCK) k_box_final(const double *partial, int3
 // exact_mode 1: nsearch.hpp:277-293 (size_t((x - min)/edge), clamped), FP64.
 // exact_mode 0: the engine's own grid, edge >= min_edge on every axis.
 // ---------------------------------------------------------------------------
-__global__ void k_cell_ids(const float4 *pos, const float4 *shift, const float4 *jprm, uint32_t mask, int32_t n,
-                           const double *box,
-                           int32_t d0, int32_t d1, int32_t d2, double edge_exact, float min_edge, int32_t exact_mode,
-                           int32_t *cell_of, int32_t *counts, const int32_t *gate)
+//
+// box_part (optional): the block partials of k_box_partial; each workgroup
+// then reduces them itself (the first one also stores the box for later
+// kernels), which saves the launch of k_box_final.  xref (optional): the
+// positions are also copied there (the list's reference positions).
+__global__ void __launch_bounds__(BLOCK) k_cell_ids(const float4 *pos, const float4 *shift, const float4 *jprm,
+                                                    uint32_t mask, int32_t n, double *box, const double *box_part,
+                                                    int32_t box_nb, float4 *xref, int32_t d0, int32_t d1, int32_t d2,
+                                                    double edge_exact, float min_edge, int32_t exact_mode,
+                                                    int32_t *cell_of, int32_t *counts, const int32_t *gate)
 {
     if (gated_off(gate))
         return;
+    __shared__ double sbox[6];
+    if (box_part)
+    {
+        block_box_final(box_part, box_nb, sbox);
+        __syncthreads();
+        if (blockIdx.x == 0 && threadIdx.x < 6)
+            box[threadIdx.x] = sbox[threadIdx.x];
+    }
+    else
+    {
+        if (threadIdx.x < 6)
+            sbox[threadIdx.x] = box[threadIdx.x];
+        __syncthreads();
+    }
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
+    if (xref)
+        xref[i] = pos[i];
     const uint32_t fl = fbits(jprm[i].w);
     if (mask != 0u && (fl & mask) == 0u)
     {
@@ -140,13 +168,13 @@ __global__ void k_cell_ids(const float4 *pos, const float4 *shift, const float4 
     {
         if (exact_mode)
         {
-            const double q = ((double)xs[a] - box[a]) / edge_exact;
+            const double q = ((double)xs[a] - sbox[a]) / edge_exact;
             const uint64_t u = q > 0.0 ? (uint64_t)q : 0u;
             c[a] = (int32_t)(u < (uint64_t)(dims[a] - 1) ? u : (uint64_t)(dims[a] - 1));
         }
         else
         {
-            c[a] = internal_coord(xs[a], box[a], box[3 + a] - box[a], dims[a], min_edge);
+            c[a] = internal_coord(xs[a], sbox[a], sbox[3 + a] - sbox[a], dims[a], min_edge);
         }
     }
     const int32_t id = c[0] + c[1] * d0 + c[2] * d0 * d1;
@@ -206,8 +234,12 @@ __global__ void __launch_bounds__(1024) k_scan(const int32_t *counts, int32_t *s
         start[n] = carry;
 }
 
-__global__ void k_scatter(const int32_t *cell_of, int32_t n, const int32_t *start, int32_t *cursor, int32_t *tmp,
-                          const int32_t *gate)
+// drain: cursor holds the cell counts, which are counted back down to zero
+// (so that the next binning starts from zeroed counts without a fill);
+// otherwise cursor starts at zero and counts up.  The order inside a cell
+// is arbitrary either way (k_cell_sort fixes it).
+__global__ void k_scatter(const int32_t *cell_of, int32_t n, const int32_t *start, int32_t *cursor, int32_t drain,
+                          int32_t *tmp, const int32_t *gate)
 {
     if (gated_off(gate))
         return;
@@ -217,7 +249,8 @@ __global__ void k_scatter(const int32_t *cell_of, int32_t n, const int32_t *star
     const int32_t c = cell_of[i];
     if (c < 0)
         return;
-    tmp[start[c] + atomicAdd(&cursor[c], 1)] = i;
+    const int32_t slot = drain ? atomicSub(&cursor[c], 1) - 1 : atomicAdd(&cursor[c], 1);
+    tmp[start[c] + slot] = i;
 }
 
 // Places each member at its rank by key within its cell (keys are distinct).
@@ -417,7 +450,7 @@ __device__ inline void probe_pair(const ProbeArgs &p, const float4 q, float4 pi,
 }
 
 // ---------------------------------------------------------------------------
-// Springs (CSR gather, G lanes per row, 4 edges in flight per lane) fused with
+// Springs (CSR gather, G lanes per row, 8 edges in flight per lane) fused with
 // viscosity, external and IMPALA forces and the viscous semi-implicit Euler
 // step (SpringNetwork.cpp:63-90,202-241; Spring.cpp:20-34; Particle.cpp:80-128).
 // ---------------------------------------------------------------------------
@@ -457,7 +490,7 @@ __global__ void __launch_bounds__(BLOCK) k_spring(SpringArgs a)
     {
         const int32_t beg = a.sp_ptr[row], end = a.sp_ptr[row + 1];
         const double cs = a.k.c_spring * (double)a.spring_scale;
-        constexpr int U = 4;
+        constexpr int U = 8;
         // the edge words of the next iteration load while this one gathers
         // and computes (two dependent loads per edge: the word, then x_j)
         int32_t jr[U];
@@ -1417,15 +1450,6 @@ __global__ void k_prestep(int32_t *status, int32_t *step, int32_t interval)
     *step += 1;
 }
 
-__global__ void k_copy_xref(const float4 *pos, float4 *xref, int32_t n, const int32_t *status)
-{
-    if (status[0] == 0 || halted(status))
-        return;
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n)
-        xref[i] = pos[i];
-}
-
 __global__ void k_clear_rebuild(int32_t *status, int32_t *counters)
 {
     if (status[0] != 0 && !halted(status))
@@ -1456,13 +1480,20 @@ void launch_box_reduce(const float4 *pos, const float4 *shift, int32_t n, double
     k_box_final<<<1, BLOCK, 0, s>>>(partial, nblocks, box, gate);
 }
 
+void launch_box_partial(const float4 *pos, const float4 *shift, int32_t n, double *partial, int32_t nblocks,
+                        const int32_t *gate, hipStream_t s)
+{
+    k_box_partial<<<nblocks, BLOCK, 0, s>>>(pos, shift, n, partial, gate);
+}
+
 void launch_cell_ids(const float4 *pos, const float4 *shift, const float4 *jprm, uint32_t mask, int32_t n,
-                     const double *box, int32_t d0, int32_t d1, int32_t d2, double edge_exact, float min_edge,
-                     int32_t exact_mode, int32_t *cell_of, int32_t *counts, const int32_t *gate, hipStream_t s)
+                     double *box, const double *box_part, int32_t box_nb, float4 *xref, int32_t d0, int32_t d1,
+                     int32_t d2, double edge_exact, float min_edge, int32_t exact_mode, int32_t *cell_of,
+                     int32_t *counts, const int32_t *gate, hipStream_t s)
 {
     if (n > 0)
-        k_cell_ids<<<nblk(n), BLOCK, 0, s>>>(pos, shift, jprm, mask, n, box, d0, d1, d2, edge_exact, min_edge,
-                                              exact_mode, cell_of, counts, gate);
+        k_cell_ids<<<nblk(n), BLOCK, 0, s>>>(pos, shift, jprm, mask, n, box, box_part, box_nb, xref, d0, d1, d2,
+                                              edge_exact, min_edge, exact_mode, cell_of, counts, gate);
 }
 
 void launch_scan(const int32_t *counts, int32_t *start, int32_t ncells, const int32_t *gate, hipStream_t s)
@@ -1470,11 +1501,11 @@ void launch_scan(const int32_t *counts, int32_t *start, int32_t ncells, const in
     k_scan<<<1, 1024, 0, s>>>(counts, start, ncells, gate);
 }
 
-void launch_scatter(const int32_t *cell_of, int32_t n, const int32_t *start, int32_t *cursor, int32_t *tmp,
-                    const int32_t *gate, hipStream_t s)
+void launch_scatter(const int32_t *cell_of, int32_t n, const int32_t *start, int32_t *cursor, int32_t drain,
+                    int32_t *tmp, const int32_t *gate, hipStream_t s)
 {
     if (n > 0)
-        k_scatter<<<nblk(n), BLOCK, 0, s>>>(cell_of, n, start, cursor, tmp, gate);
+        k_scatter<<<nblk(n), BLOCK, 0, s>>>(cell_of, n, start, cursor, drain, tmp, gate);
 }
 
 void launch_cell_sort(const int32_t *cell_of, int32_t n, const int32_t *start, const int32_t *tmp,
@@ -1600,14 +1631,6 @@ void launch_probe(const ProbeArgs &p, int32_t nparts, float dt, int32_t visc_ena
 void launch_prestep(int32_t *status, int32_t *step, int32_t interval, hipStream_t s)
 {
     k_prestep<<<1, 1, 0, s>>>(status, step, interval);
-}
-
-void launch_finish_rebuild(const float4 *pos, float4 *xref, int32_t n, int32_t *status, int32_t *counters,
-                           hipStream_t s)
-{
-    if (n > 0)
-        k_copy_xref<<<nblk(n), BLOCK, 0, s>>>(pos, xref, n, status);
-    (void)counters;
 }
 
 void launch_clear_rebuild(int32_t *status, int32_t *counters, hipStream_t s)

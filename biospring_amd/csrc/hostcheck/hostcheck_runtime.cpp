@@ -308,6 +308,25 @@ void launch_box_reduce(const float4 *pos, const float4 *shift, int32_t n, double
     }
 }
 
+// One partial per block, as the device writes them: the whole box in
+// partial 0, empty boxes in the others (their min/max change nothing).
+void launch_box_partial(const float4 *pos, const float4 *shift, int32_t n, double *partial, int32_t nblocks,
+                        const int32_t *gate, hipStream_t s)
+{
+    need(partial, sizeof(double) * 6 * nblocks, "box_partial: partial");
+    if (nblocks < 1 || nblocks > 256)
+        hc_fail("box_partial: block count outside 1..BLOCK");
+    if (gated_off(gate))
+        return;
+    launch_box_reduce(pos, shift, n, partial, partial, 1, nullptr, s);
+    for (int32_t b = 1; b < nblocks; ++b)
+        for (int c = 0; c < 3; ++c)
+        {
+            partial[6 * b + c] = INFINITY;
+            partial[6 * b + 3 + c] = -INFINITY;
+        }
+}
+
 static int32_t internal_coord(float x, double mn, double len, int32_t dim, float min_edge)
 {
     const double e = std::max((double)min_edge, len / (double)dim);
@@ -317,8 +336,9 @@ static int32_t internal_coord(float x, double mn, double len, int32_t dim, float
 }
 
 void launch_cell_ids(const float4 *pos, const float4 *shift, const float4 *jprm, uint32_t mask, int32_t n,
-                     const double *box, int32_t d0, int32_t d1, int32_t d2, double edge_exact, float min_edge,
-                     int32_t exact_mode, int32_t *cell_of, int32_t *counts, const int32_t *gate, hipStream_t)
+                     double *box, const double *box_part, int32_t box_nb, float4 *xref, int32_t d0, int32_t d1,
+                     int32_t d2, double edge_exact, float min_edge, int32_t exact_mode, int32_t *cell_of,
+                     int32_t *counts, const int32_t *gate, hipStream_t)
 {
     const int64_t nc = (int64_t)d0 * d1 * d2;
     need(pos, sizeof(float4) * n, "cell_ids: pos");
@@ -328,8 +348,29 @@ void launch_cell_ids(const float4 *pos, const float4 *shift, const float4 *jprm,
     need(box, sizeof(double) * 6, "cell_ids: box");
     need(cell_of, sizeof(int32_t) * n, "cell_ids: cell_of");
     need(counts, sizeof(int32_t) * nc, "cell_ids: counts");
+    if (box_part)
+    {
+        need(box_part, sizeof(double) * 6 * box_nb, "cell_ids: box partials");
+        if (box_nb < 1 || box_nb > 256)
+            hc_fail("cell_ids: partial count outside 1..BLOCK");
+    }
+    if (xref)
+        need(xref, sizeof(float4) * n, "cell_ids: xref");
     if (gated_off(gate))
         return;
+    if (box_part)
+        for (int c = 0; c < 3; ++c)
+        {
+            box[c] = INFINITY;
+            box[3 + c] = -INFINITY;
+            for (int32_t b = 0; b < box_nb; ++b)
+            {
+                box[c] = std::min(box[c], box_part[6 * b + c]);
+                box[3 + c] = std::max(box[3 + c], box_part[6 * b + 3 + c]);
+            }
+        }
+    if (xref && n > 0)
+        std::memcpy(xref, pos, sizeof(float4) * n);
     const int32_t dims[3] = {d0, d1, d2};
     for (int32_t i = 0; i < n; ++i)
     {
@@ -374,8 +415,8 @@ void launch_scan(const int32_t *counts, int32_t *start, int32_t ncells, const in
     start[ncells] = run;
 }
 
-void launch_scatter(const int32_t *cell_of, int32_t n, const int32_t *start, int32_t *cursor, int32_t *tmp,
-                    const int32_t *gate, hipStream_t)
+void launch_scatter(const int32_t *cell_of, int32_t n, const int32_t *start, int32_t *cursor, int32_t drain,
+                    int32_t *tmp, const int32_t *gate, hipStream_t)
 {
     need(cell_of, sizeof(int32_t) * n, "scatter: cell_of");
     need(tmp, sizeof(int32_t) * n, "scatter: tmp");
@@ -388,8 +429,8 @@ void launch_scatter(const int32_t *cell_of, int32_t n, const int32_t *start, int
             continue;
         need(start + c, 8, "scatter: start[c..c+1]");
         need(cursor + c, 4, "scatter: cursor[c]");
-        const int32_t slot = start[c] + cursor[c]++;
-        if (slot < 0 || slot >= n)
+        const int32_t slot = start[c] + (drain ? --cursor[c] : cursor[c]++);
+        if (slot < 0 || slot >= n || (drain && cursor[c] < 0))
             hc_fail("scatter: slot out of range");
         tmp[slot] = i;
     }
@@ -1106,15 +1147,6 @@ void launch_prestep(int32_t *status, int32_t *step, int32_t interval, hipStream_
     if (interval > 0 && *step > 0 && *step % interval == 0)
         status[0] = 1;
     *step += 1;
-}
-
-void launch_finish_rebuild(const float4 *pos, float4 *xref, int32_t n, int32_t *status, int32_t *, hipStream_t)
-{
-    need(pos, sizeof(float4) * n, "xref: pos");
-    need(xref, sizeof(float4) * n, "xref: xref");
-    need(status, 32, "xref: status");
-    if (status[0] && !halted(status))
-        std::memcpy(xref, pos, sizeof(float4) * n);
 }
 
 void launch_clear_rebuild(int32_t *status, int32_t *counters, hipStream_t)
