@@ -659,7 +659,7 @@ int ensure_cells(bs_handle *h, int32_t ncells)
 const int32_t *gate_of(bs_handle *h, bool gated) { return gated ? h->status : nullptr; }
 
 // Engine cell grid + Verlet list of list l.  gated: skipped on the device
-// unless the rebuild flag is set.
+// unless the rebuild gate (status[0]) is set.
 void build_list(bs_handle *h, List &l, bool gated, int32_t cap_override)
 {
     const int32_t n = (int32_t)h->n;
@@ -704,18 +704,18 @@ void build_list(bs_handle *h, List &l, bool gated, int32_t cap_override)
         launch_rows(l.cl, l.cln, h->ncl, l.cap, l.rows, l.rlen, l.rcap, g, s);
 }
 
-void rebuild_chain(bs_handle *h, bool gated)
+// The gated list rebuild.  prepped: the gate was just set by a step's
+// prestep; otherwise it is set here from the drift flag alone.
+void rebuild_chain(bs_handle *h, bool gated, bool prepped = false)
 {
+    if (!prepped)
+        launch_prestep(h->status, h->counters, 0, 0, h->stream);
     if (h->lists.empty())
-    {
-        launch_clear_rebuild(h->status, h->counters, h->stream);
         return;
-    }
     const int32_t *g = gate_of(h, gated);
     launch_box_partial(h->pos[h->cur], h->shift, (int32_t)h->n, h->box_part, h->box_blocks, g, h->stream);
     for (auto &l : h->lists)
         build_list(h, l, gated, -1);
-    launch_clear_rebuild(h->status, h->counters, h->stream);
 }
 
 float rc2(float rc, double f) { return (float)((double)rc * rc * f); }
@@ -1063,7 +1063,7 @@ int regrow_lists(bs_handle *h)
     }
     int32_t st[8];
     HIPCHK(h, hipMemcpy(st, h->status, sizeof st, hipMemcpyDeviceToHost));
-    st[0] = 1; // rebuild at the next step
+    st[7] = 1; // rebuild at the next step
     st[2] = st[5] = st[6] = 0;
     HIPCHK(h, hipMemcpy(h->status, st, sizeof st, hipMemcpyHostToDevice));
     return BS_OK;
@@ -1074,8 +1074,8 @@ int regrow_lists(bs_handle *h)
 void launch_step(bs_handle *h, bool energy)
 {
     const size_t e0 = h->prof ? prof_mark(h) : SIZE_MAX;
-    launch_prestep(h->status, h->counters, h->p.rebuild_interval, h->stream);
-    rebuild_chain(h, true);
+    launch_prestep(h->status, h->counters, h->p.rebuild_interval, 1, h->stream);
+    rebuild_chain(h, true, true);
     const size_t e1 = h->prof ? prof_mark(h) : SIZE_MAX;
     if (!h->lists.empty())
         launch_pairs(h, energy);
@@ -1306,7 +1306,7 @@ int adjoint_record(bs_handle *h, int32_t T, int32_t checkpoint_every)
             HIPCHK(h, hipMemcpyAsync(tp.cp_vel + c * n, h->vel, s4, hipMemcpyDeviceToDevice, st));
             HIPCHK(h, hipMemcpyAsync(tp.cp_cnt + 4 * c, h->counters, 4 * sizeof(int32_t), hipMemcpyDeviceToDevice,
                                      st));
-            launch_fill_i32(h->status, 1, 1, nullptr, st);
+            launch_fill_i32(h->status + 7, 1, 1, nullptr, st);
         }
         const int rc = prof_fold(h);
         if (rc)
@@ -1443,7 +1443,7 @@ int adjoint_sweep(bs_handle *h, AdjBufs &b, double *dk, double *dl0, double *dq)
             HIPCHK(h, hipMemcpyAsync(h->vel, tp.cp_vel + (size_t)c * n, s4, hipMemcpyDeviceToDevice, st));
             HIPCHK(h, hipMemcpyAsync(h->counters, tp.cp_cnt + 4 * (size_t)c, 4 * sizeof(int32_t),
                                      hipMemcpyDeviceToDevice, st));
-            launch_fill_i32(h->status, 1, 1, nullptr, st);
+            launch_fill_i32(h->status + 7, 1, 1, nullptr, st);
         }
         for (int32_t s = s0; s < s1 && recompute; ++s)
         {
@@ -1461,7 +1461,7 @@ int adjoint_sweep(bs_handle *h, AdjBufs &b, double *dk, double *dl0, double *dq)
         // lists rebuilt at each segment's first reversed state, so that the
         // sweep does not depend on the lists a recomputation or an earlier
         // call left: two sweeps of one record give the same bits
-        launch_fill_i32(h->status, 1, 1, nullptr, st);
+        launch_fill_i32(h->status + 7, 1, 1, nullptr, st);
         for (int32_t s = s1 - 1; s >= s0; --s)
         {
             HIPCHK(h, hipMemcpyAsync(h->pos[h->cur], tp.seg + (size_t)(s - s0) * n, s4, hipMemcpyDeviceToDevice, st));
@@ -1517,7 +1517,7 @@ int adjoint_sweep(bs_handle *h, AdjBufs &b, double *dk, double *dl0, double *dq)
     HIPCHK(h, hipMemcpyAsync(h->pos[h->cur], tp.fin_pos, s4, hipMemcpyDeviceToDevice, st));
     HIPCHK(h, hipMemcpyAsync(h->vel, tp.fin_vel, s4, hipMemcpyDeviceToDevice, st));
     HIPCHK(h, hipMemcpyAsync(h->counters, tp.fin_cnt, 4 * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
-    launch_fill_i32(h->status, 1, 1, nullptr, st);
+    launch_fill_i32(h->status + 7, 1, 1, nullptr, st);
 
     if (want_springs)
         launch_spring_out(h->slot_of, m, dks, dls, outk, outl, st);
@@ -1812,7 +1812,7 @@ int do_setup(bs_handle *h)
     }
 
     // status / counters
-    const int32_t st0[8] = {1, INT_MAX, 0, 0, 0, 0, 0, 0}, ct0[4] = {0, 0, 0, 0};
+    const int32_t st0[8] = {0, INT_MAX, 0, 0, 0, 0, 0, 1}, ct0[4] = {0, 0, 0, 0};
     HIPCHK(h, hipMemcpyAsync(h->status, st0, sizeof st0, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(h->counters, ct0, sizeof ct0, hipMemcpyHostToDevice, h->stream));
 
@@ -1933,7 +1933,7 @@ int do_setup(bs_handle *h)
         }
         launch_fill_i32(h->status + 2, 0, 1, nullptr, h->stream);
     }
-    const int32_t st1[8] = {1, INT_MAX, 0, 0, 0, 0, 0, 0};
+    const int32_t st1[8] = {0, INT_MAX, 0, 0, 0, 0, 0, 1};
     HIPCHK(h, hipMemcpyAsync(h->status, st1, sizeof st1, hipMemcpyHostToDevice, h->stream));
     rebuild_chain(h, true);
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -2363,7 +2363,7 @@ int bs_set_state(bs_handle *h, const float *x3, const float *v3)
     if (rc)
         return rc;
     // a new state: rebuild the lists, clear a stop and a non-finite report
-    const int32_t st[8] = {1, INT_MAX, 0, 0, 0, 0, 0, 0};
+    const int32_t st[8] = {0, INT_MAX, 0, 0, 0, 0, 0, 1};
     HIPCHK(h, hipMemcpy(h->status, st, sizeof st, hipMemcpyHostToDevice));
     h->nonfinite = -1;
     return BS_OK;
@@ -3387,7 +3387,7 @@ int bs_autotune(bs_handle *h, int32_t steps, int32_t *group_out, int32_t *graph_
     if (h->p.pair_layout != BS_PAIR_ROWS && h->p.pair_layout != blay)
     { // row buffers are sized at setup; the other two read lists in their own order
         h->p.pair_layout = blay;
-        launch_fill_i32(h->status, 1, 1, nullptr, h->stream);
+        launch_fill_i32(h->status + 7, 1, 1, nullptr, h->stream);
     }
     if (group_out)
         *group_out = bg;

@@ -360,8 +360,13 @@ void launch_energies(const double4 *epp, const double4 *eps, const int32_t *seg_
 // viscosity and (integrate) its Euler step and kinetic energy.
 void launch_probe(const ProbeArgs &p, int32_t nparts, float dt, int32_t visc_enable, float visc, double c_kinetic,
                   int32_t integrate, hipStream_t s);
-void launch_prestep(int32_t *status, int32_t *step_counter, int32_t interval, hipStream_t s);
-void launch_clear_rebuild(int32_t *status, int32_t *counters, hipStream_t s);
+// Status words: [0] rebuild gate of the current step's list kernels, [1]
+// lowest particle with a non-finite position (INT_MAX: none), [2] list
+// overflow, [3] longest list, [5] stop, [6] steps completed in the pass, [7]
+// drift flag (a rebuild at the next gate).  launch_prestep sets the gate from
+// the drift flag and, at the start of a step (advance), from the rebuild
+// interval; counters[0] counts steps, counters[1] rebuilds.
+void launch_prestep(int32_t *status, int32_t *counters, int32_t interval, int32_t advance, hipStream_t s);
 void launch_rigid(const RigidArgs &a, hipStream_t s);
 void launch_adj_pair(const AdjPairArgs &a, int32_t steric_mode, int32_t elec_mode, int32_t nblocks, hipStream_t s);
 void launch_adj_spring(const AdjSpringArgs &a, int32_t group, int32_t nblocks, hipStream_t s);
@@ -387,7 +392,7 @@ void launch_spring_out(const int32_t *slot_of, int64_t m, const double *dk, cons
 void launch_topo_springs(const TopoArgs &a, hipStream_t s);
 void launch_pack_positions(const float4 *pos, const float4 *plo, const int32_t *perm, int32_t n, float *out3,
                            hipStream_t s);
-// status[0] = 1 when any particle is more than half the list skin from xref.
+// status[7] (the drift flag) = 1 when any particle is more than half the list skin from xref.
 void launch_drift_check(const float4 *pos, const float4 *xref, int32_t n, float half_skin2, int32_t *status,
                         hipStream_t s);
 void launch_centroid(const float4 *pos, int32_t n, double *partial, int32_t nblocks, double *out3, hipStream_t s);

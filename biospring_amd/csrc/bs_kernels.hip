@@ -674,7 +674,7 @@ __global__ void __launch_bounds__(BLOCK) k_spring(SpringArgs a)
             const float4 xr = a.xref[row];
             const float ex = xn.x - xr.x, ey = xn.y - xr.y, ez = xn.z - xr.z;
             if (!(ex * ex + ey * ey + ez * ez <= a.half_skin2))
-                atomicOr(&a.status[0], 1);
+                atomicOr(&a.status[7], 1);
             a.vel[row] = v;
             a.pos_out[row] = xn;
         }
@@ -849,7 +849,7 @@ __global__ void k_rigid_place(RigidArgs a)
     const float4 xr = a.xref[p];
     const float ex = xn.x - xr.x, ey = xn.y - xr.y, ez = xn.z - xr.z;
     if (!(ex * ex + ey * ey + ez * ez <= a.half_skin2))
-        atomicOr(&a.status[0], 1);
+        atomicOr(&a.status[7], 1);
 }
 
 // Sum of positions in FP64, block partials then one fixed-order final sum.
@@ -1058,7 +1058,7 @@ __global__ void k_drift_check(const float4 *pos, const float4 *xref, int32_t n, 
         far = !(ex * ex + ey * ey + ez * ez <= half_skin2);
     }
     if (__ballot(far) != 0ull && (threadIdx.x & 63) == 0)
-        atomicOr(&status[0], 1);
+        atomicOr(&status[7], 1);
 }
 
 template <int G>
@@ -1436,27 +1436,29 @@ __global__ void k_topo_sort_long(const int32_t *offs, int32_t n, int32_t *jout, 
         topo_sort_row(jout + b, l0out + b, len);
 }
 
-__global__ void k_prestep(int32_t *status, int32_t *step, int32_t interval)
+// The rebuild gate of the kernels that follow: status[0] = a drift flag
+// (status[7], raised by the integrator or by the host) or, with advance, a
+// rebuild interval that falls due; the drift flag is consumed and the
+// rebuild counted (counters[1]).  advance: the start of a step, which also
+// stops the call after a non-finite position and counts the step
+// (counters[0]).
+__global__ void k_prestep(int32_t *status, int32_t *counters, int32_t interval, int32_t advance)
 {
     if (halted(status))
         return;
-    if (status[1] != 0x7fffffff)
+    if (advance && status[1] != 0x7fffffff)
     { // the step before produced a non-finite position: stop here
         status[5] = 1;
         return;
     }
-    if (interval > 0 && *step > 0 && (*step % interval) == 0)
-        status[0] = 1;
-    *step += 1;
-}
-
-__global__ void k_clear_rebuild(int32_t *status, int32_t *counters)
-{
-    if (status[0] != 0 && !halted(status))
-    {
-        status[0] = 0;
-        counters[1] += 1;
-    }
+    const int32_t step = counters[0];
+    const bool due = advance && interval > 0 && step > 0 && (step % interval) == 0;
+    const int32_t r = (status[7] != 0 || due) ? 1 : 0;
+    status[7] = 0;
+    status[0] = r;
+    counters[1] += r;
+    if (advance)
+        counters[0] = step + 1;
 }
 
 __global__ void k_fill_i32(int32_t *p, int32_t v, int64_t n, const int32_t *gate)
@@ -1628,14 +1630,9 @@ void launch_probe(const ProbeArgs &p, int32_t nparts, float dt, int32_t visc_ena
     k_probe<<<1, BLOCK, 0, s>>>(p, nparts, dt, visc_enable, visc, c_kinetic, integrate);
 }
 
-void launch_prestep(int32_t *status, int32_t *step, int32_t interval, hipStream_t s)
+void launch_prestep(int32_t *status, int32_t *counters, int32_t interval, int32_t advance, hipStream_t s)
 {
-    k_prestep<<<1, 1, 0, s>>>(status, step, interval);
-}
-
-void launch_clear_rebuild(int32_t *status, int32_t *counters, hipStream_t s)
-{
-    k_clear_rebuild<<<1, 1, 0, s>>>(status, counters);
+    k_prestep<<<1, 1, 0, s>>>(status, counters, interval, advance);
 }
 
 void launch_rigid(const RigidArgs &a, hipStream_t s)

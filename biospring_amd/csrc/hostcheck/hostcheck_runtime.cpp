@@ -683,7 +683,7 @@ void launch_spring(const SpringArgs &a, int32_t group, int32_t nb, hipStream_t)
     need(a.xref, sizeof(float4) * n, "spring: xref");
     need(a.sp_ptr, sizeof(int32_t) * (n + 1), "spring: sp_ptr");
     need(a.perm, sizeof(int32_t) * n, "spring: perm");
-    need(a.status, sizeof(int32_t) * 4, "spring: status");
+    need(a.status, sizeof(int32_t) * 8, "spring: status");
     if (a.energy)
         need(a.eps, sizeof(double4) * n, "spring: eps");
     for (const GridView *g : {&a.gpot, &a.gden})
@@ -1041,7 +1041,7 @@ void launch_drift_check(const float4 *pos, const float4 *xref, int32_t n, float,
 {
     need(pos, sizeof(float4) * n, "drift_check: pos");
     need(xref, sizeof(float4) * n, "drift_check: xref");
-    need(status, sizeof(int32_t) * 4, "drift_check: status");
+    need(status, sizeof(int32_t) * 8, "drift_check: status");
 }
 
 void launch_energies(const double4 *epp, const double4 *eps, const int32_t *seg_start, int32_t nseg,
@@ -1097,7 +1097,7 @@ void launch_rigid(const RigidArgs &a, hipStream_t)
     need(a.slot_b, sizeof(int32_t) * a.nr, "rigid: slot_b");
     need(a.p0, sizeof(double4) * a.nr, "rigid: p0");
     need(a.rx, sizeof(double4) * a.nr, "rigid: rx");
-    need(a.status, sizeof(int32_t) * 4, "rigid: status");
+    need(a.status, sizeof(int32_t) * 8, "rigid: status");
     int32_t n = 0;
     for (int32_t t = 0; t < a.nr; ++t)
     {
@@ -1133,31 +1133,24 @@ void launch_centroid(const float4 *pos, int32_t n, double *partial, int32_t nblo
         out3[k] = n > 0 ? c[k] / n : 0.0;
 }
 
-void launch_prestep(int32_t *status, int32_t *step, int32_t interval, hipStream_t)
+void launch_prestep(int32_t *status, int32_t *counters, int32_t interval, int32_t advance, hipStream_t)
 {
     need(status, 32, "prestep: status");
-    need(step, 4, "prestep: step");
+    need(counters, 8, "prestep: counters");
     if (halted(status))
         return;
-    if (status[1] != 0x7fffffff)
+    if (advance && status[1] != 0x7fffffff)
     {
         status[5] = 1;
         return;
     }
-    if (interval > 0 && *step > 0 && *step % interval == 0)
-        status[0] = 1;
-    *step += 1;
-}
-
-void launch_clear_rebuild(int32_t *status, int32_t *counters, hipStream_t)
-{
-    need(status, 32, "clear: status");
-    need(counters, 8, "clear: counters");
-    if (status[0] && !halted(status))
-    {
-        status[0] = 0;
-        counters[1] += 1;
-    }
+    const bool due = advance && interval > 0 && counters[0] > 0 && counters[0] % interval == 0;
+    const int32_t r = (status[7] != 0 || due) ? 1 : 0;
+    status[7] = 0;
+    status[0] = r;
+    counters[1] += r;
+    if (advance)
+        counters[0] += 1;
 }
 
 void launch_fill_i32(int32_t *p, int32_t v, int64_t n, const int32_t *gate, hipStream_t)
