@@ -152,9 +152,10 @@ struct bs_handle
         int32_t *cp_cnt = nullptr, *fin_cnt = nullptr;
     } tape;
     // bs_adjoint: springs with a dynamic end in the order of their owning CSR
-    // edge ("slots"): slot_e[r] = that edge (ascending), slot_of[s] = slot of
-    // spring s or -1
-    int32_t *slot_e = nullptr, *slot_of = nullptr;
+    // edge ("slots"; the owning edge is in the dynamic end's row, or in the
+    // lower index's when both ends move): slot_ptr[i] = first slot of row i,
+    // slot_of[s] = slot of spring s or -1
+    int32_t *slot_ptr = nullptr, *slot_of = nullptr;
     int64_t nslots = -1;
     int32_t *ex_ptr = nullptr;                        // [ncl + 1] static exclusions
     uint32_t *ex_w = nullptr;                         // j << 8 | bonded mask
@@ -330,10 +331,10 @@ void free_device(bs_handle *h)
     h->seg_part = h->sys_E = nullptr;
     h->shift = nullptr;
     h->nseg = 0;
-    f(h->ex_ptr); f(h->ex_w); f(h->slot_e); f(h->slot_of);
+    f(h->ex_ptr); f(h->ex_w); f(h->slot_ptr); f(h->slot_of);
     h->ex_ptr = nullptr;
     h->ex_w = nullptr;
-    h->slot_e = h->slot_of = nullptr;
+    h->slot_ptr = h->slot_of = nullptr;
     h->nslots = -1;
     f(h->bodies_d); f(h->rx); f(h->rp0); f(h->frig); f(h->slot_p); f(h->slot_b); f(h->bstart);
     f(h->grid_d[0]); f(h->grid_d[1]); f(h->cent_part); f(h->cent);
@@ -1171,13 +1172,28 @@ int build_spring_owners(bs_handle *h)
     HIPCHK(h, hipMemcpy(hp.data(), h->sp_ptr, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToHost));
     if (E > 0)
         HIPCHK(h, hipMemcpy(hj.data(), h->sp_j, sizeof(int32_t) * E, hipMemcpyDeviceToHost));
+    // a slot per owned edge: in a moving row, the edges to static partners
+    // and to partners of higher index, numbered in edge order
+    std::vector<int32_t> ord((size_t)std::max<int64_t>(E, 1), -1), sp((size_t)n + 1, 0);
+    for (int64_t i = 0; i < n; ++i)
+    {
+        int32_t c = sp[(size_t)i];
+        if (!(h->flags[(size_t)h->perm[(size_t)i]] & BS_FLAG_STATIC))
+            for (int32_t t = hp[(size_t)i]; t < hp[(size_t)i + 1]; ++t)
+                if (hj[(size_t)t] < 0 || hj[(size_t)t] > (int32_t)i)
+                    ord[(size_t)t] = c++;
+        sp[(size_t)i + 1] = c;
+    }
+    const int64_t ns = sp[(size_t)n];
+    // each spring with a moving end finds its edge in the owning row (the
+    // table holds no repeated pair: every way in drops repeats)
 #pragma omp parallel for schedule(static, 4096)
     for (int64_t s = 0; s < m; ++s)
     {
         int32_t a = h->iperm[h->s1[(size_t)s]], b = h->iperm[h->s2[(size_t)s]];
         const bool sa = (h->flags[h->s1[(size_t)s]] & BS_FLAG_STATIC) != 0;
         const bool sb = (h->flags[h->s2[(size_t)s]] & BS_FLAG_STATIC) != 0;
-        if (sa && sb)
+        if ((sa && sb) || a == b)
             continue;
         if (sa || (!sb && b < a))
             std::swap(a, b); // row a owns the edge to b
@@ -1193,27 +1209,27 @@ int build_spring_owners(bs_handle *h)
         if (lo < hp[(size_t)a + 1] && (hj[(size_t)lo] & 0x7fffffff) == b)
             own[(size_t)s] = lo;
     }
-    std::vector<std::pair<int32_t, int32_t>> es;
-    es.reserve((size_t)m);
+    std::vector<int32_t> so((size_t)m, -1);
+    std::vector<char> used((size_t)std::max<int64_t>(ns, 1), 0);
     for (int64_t s = 0; s < m; ++s)
-        if (own[(size_t)s] >= 0)
-            es.push_back({own[(size_t)s], (int32_t)s});
-    std::sort(es.begin(), es.end());
-    std::vector<int32_t> se(es.size()), so((size_t)m, -1);
-    for (size_t r = 0; r < es.size(); ++r)
     {
-        se[r] = es[r].first;
-        so[(size_t)es[r].second] = (int32_t)r;
+        const int32_t e = own[(size_t)s];
+        if (e < 0)
+            continue;
+        const int32_t r = ord[(size_t)e];
+        if (r < 0 || used[(size_t)r])
+            return fail(h, BS_ERR_STATE, "internal: spring %lld has no free slot", (long long)s);
+        used[(size_t)r] = 1;
+        so[(size_t)s] = r;
     }
     hipError_t e = hipSuccess;
-    h->slot_e = dalloc<int32_t>(se.size(), e);
+    h->slot_ptr = dalloc<int32_t>((size_t)n + 1, e);
     HIPCHK(h, e);
     h->slot_of = dalloc<int32_t>((size_t)m, e);
     HIPCHK(h, e);
-    if (!se.empty())
-        HIPCHK(h, hipMemcpy(h->slot_e, se.data(), sizeof(int32_t) * se.size(), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->slot_ptr, sp.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice));
     HIPCHK(h, hipMemcpy(h->slot_of, so.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice));
-    h->nslots = (int64_t)se.size();
+    h->nslots = ns;
     return BS_OK;
 }
 
@@ -1353,8 +1369,7 @@ struct AdjBufs
 // built); seed: room for a caller's dL/dx, dL/dv.
 size_t adj_carve(const bs_handle *h, bool springs, bool seed, char *base, AdjBufs &b)
 {
-    const size_t n = (size_t)h->n, E = (size_t)std::max<int64_t>(h->nedges, 0),
-                 ns = (size_t)std::max<int64_t>(h->nslots, 0), m = h->s1.size();
+    const size_t n = (size_t)h->n, ns = (size_t)std::max<int64_t>(h->nslots, 0), m = h->s1.size();
     size_t off = 0;
     auto take = [&](auto *&ptr, size_t count) {
         using T = std::remove_pointer_t<std::remove_reference_t<decltype(ptr)>>;
@@ -1372,8 +1387,8 @@ size_t adj_carve(const bs_handle *h, bool springs, bool seed, char *base, AdjBuf
         take(b.hwp, n);
     if (springs)
     {
-        take(b.dke, E);
-        take(b.dle, E);
+        take(b.dke, ns);
+        take(b.dle, ns);
         take(b.dks, ns);
         take(b.dls, ns);
         take(b.outk, m);
@@ -1399,7 +1414,7 @@ int adjoint_sweep(bs_handle *h, AdjBufs &b, double *dk, double *dl0, double *dq)
 {
     auto &tp = h->tape;
     const bs_params &P = h->p;
-    const int64_t n = h->n, E = h->nedges, m = (int64_t)h->s1.size();
+    const int64_t n = h->n, m = (int64_t)h->s1.size();
     const int32_t T = tp.T, K = tp.K, ncp = tp.ncp;
     const size_t s4 = sizeof(float4) * (size_t)n;
     hipStream_t st = h->stream;
@@ -1417,8 +1432,8 @@ int adjoint_sweep(bs_handle *h, AdjBufs &b, double *dk, double *dl0, double *dq)
     // cleared by a fill kernel (no memset nodes, should a caller capture)
     if (want_springs)
     {
-        launch_fill_i32(reinterpret_cast<int32_t *>(dke), 0, std::max<int64_t>(E, 1), nullptr, st);
-        launch_fill_i32(reinterpret_cast<int32_t *>(dle), 0, std::max<int64_t>(E, 1), nullptr, st);
+        launch_fill_i32(reinterpret_cast<int32_t *>(dke), 0, std::max<int64_t>(ns, 1), nullptr, st);
+        launch_fill_i32(reinterpret_cast<int32_t *>(dle), 0, std::max<int64_t>(ns, 1), nullptr, st);
         launch_fill_i32(reinterpret_cast<int32_t *>(dks), 0, 2 * std::max<int64_t>(ns, 1), nullptr, st);
         launch_fill_i32(reinterpret_cast<int32_t *>(dls), 0, 2 * std::max<int64_t>(ns, 1), nullptr, st);
     }
@@ -1488,8 +1503,9 @@ int adjoint_sweep(bs_handle *h, AdjBufs &b, double *dk, double *dl0, double *dq)
             a.lx = b.lx;
             a.lv = b.lv;
             a.w_next = wb[wc ^ 1];
-            a.dk = dke;
-            a.dl0 = dle;
+            a.dk = want_springs ? dke : nullptr;
+            a.dl0 = want_springs ? dle : nullptr;
+            a.slot_ptr = h->slot_ptr;
             a.n = (int32_t)n;
             a.spring_enable = P.spring_enable;
             a.spring_scale = P.spring_scale;
@@ -1499,7 +1515,7 @@ int adjoint_sweep(bs_handle *h, AdjBufs &b, double *dk, double *dl0, double *dq)
             launch_adj_spring(a, h->group, h->spring_blocks, st);
             if (want_springs && (++since_flush == FLUSH || s == 0))
             {
-                launch_spring_grads(h->slot_e, dke, dle, ns, dks, dls, st);
+                launch_spring_grads(dke, dle, ns, dks, dls, st);
                 since_flush = 0;
             }
             if (h->prof)

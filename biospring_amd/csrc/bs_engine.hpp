@@ -264,7 +264,9 @@ struct AdjSpringArgs
     const double4 *hwp;   // nullable: pair part of H w
     double4 *lx, *lv;     // [N] adjoint of x and v (updated in place)
     float4 *w_next;       // [N] weights of the step before
-    float *dk, *dl0;      // nullable: [E] this segment's gradient sums on the owning edge of each spring
+    float *dk, *dl0;      // nullable: [slots] gradient sums since the last flush, per spring slot
+    const int32_t *slot_ptr; // [N + 1] first slot of each row: a dynamic row's slots are its owned edges
+                             // (partner static or of higher index), numbered in edge order
     int32_t n, spring_enable;
     float spring_scale;
     double dt, gamma;     // gamma = 0 without viscosity
@@ -382,10 +384,9 @@ void launch_adj_seed(const double *dx3, const double *dv3, const int32_t *perm, 
 // out[perm[t]] = in[t] for t < n.
 void launch_unperm_f64(const double *in, const int32_t *perm, int32_t n, double *out, hipStream_t s);
 // Spring-gradient flush: for each slot r (springs in the order of their
-// owning CSR edge, slot_e ascending) dk[r] += dke[slot_e[r]] in FP64 and the
-// FP32 edge sum is cleared; likewise dl0.
-void launch_spring_grads(const int32_t *slot_e, float *dke, float *dle, int64_t nslots, double *dk, double *dl0,
-                         hipStream_t s);
+// owning CSR edge) dk[r] += dke[r] in FP64 and the FP32 sum is cleared;
+// likewise dl0.
+void launch_spring_grads(float *dke, float *dle, int64_t nslots, double *dk, double *dl0, hipStream_t s);
 // Per spring from per slot: out[s] = slot_of[s] >= 0 ? v[slot_of[s]] : 0.
 void launch_spring_out(const int32_t *slot_of, int64_t m, const double *dk, const double *dl0, double *outk,
                        double *outl, hipStream_t s);

@@ -1021,19 +1021,17 @@ __global__ void k_unperm_f64(const double *in, const int32_t *perm, int32_t n, d
         out[perm[t]] = in[t];
 }
 
-// Flush: the FP32 per-edge sums of the last steps are added to the FP64
-// totals per slot and cleared.  Slots follow the edges, so every access
-// walks memory forward.
-__global__ void k_spring_grads(const int32_t *slot_e, float *dke, float *dle, int64_t ns, double *dk, double *dl0)
+// Flush: the FP32 per-slot sums of the last steps are added to the FP64
+// totals and cleared.
+__global__ void k_spring_grads(float *dke, float *dle, int64_t ns, double *dk, double *dl0)
 {
     const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (r >= ns)
         return;
-    const int32_t e = slot_e[r];
-    dk[r] += (double)dke[e];
-    dl0[r] += (double)dle[e];
-    dke[e] = 0.0f;
-    dle[e] = 0.0f;
+    dk[r] += (double)dke[r];
+    dl0[r] += (double)dle[r];
+    dke[r] = 0.0f;
+    dle[r] = 0.0f;
 }
 
 __global__ void k_spring_out(const int32_t *slot_of, int64_t m, const double *dk, const double *dl0, double *outk,
@@ -1084,6 +1082,11 @@ __global__ void __launch_bounds__(BLOCK) k_adj_spring(AdjSpringArgs a)
     {
         const double C = a.k.c_spring * (double)a.spring_scale;
         const int32_t beg = a.sp_ptr[row], end = a.sp_ptr[row + 1];
+        // the row's next slot: owned edges take consecutive slots in edge
+        // order, counted over the row's G lanes with a ballot per batch
+        int32_t next_slot = a.dk ? a.slot_ptr[row] : 0;
+        const int32_t gbase = (int32_t)(threadIdx.x & 63u) - lane; // first lane of the row in the wave
+        const uint64_t gmask = ((1ull << G) - 1ull) << gbase; // G <= 32
         constexpr int U = 4; // edges in flight per lane
         // the edge words of the next iteration load while this one gathers
         // and computes
@@ -1123,6 +1126,10 @@ __global__ void __launch_bounds__(BLOCK) k_adj_spring(AdjSpringArgs a)
             {
                 const int32_t e = e0 + u * G;
                 const int32_t j = jr[u] & 0x7fffffff;
+                const bool owned = a.dk && e < end && (jr[u] < 0 || j > row);
+                const uint64_t ob = __ballot(owned) & gmask;
+                const int32_t slot = next_slot + (int32_t)__popcll(ob & ((1ull << (threadIdx.x & 63u)) - 1ull));
+                next_slot += (int32_t)__popcll(ob);
                 const double dx = (double)pj[u].x - (double)pi.x, dy = (double)pj[u].y - (double)pi.y,
                              dz = (double)pj[u].z - (double)pi.z;
                 const double r2 = dx * dx + dy * dy + dz * dz;
@@ -1137,11 +1144,11 @@ __global__ void __launch_bounds__(BLOCK) k_adj_spring(AdjSpringArgs a)
                 hx = fma(phi, wx, fma(g, dx, hx));
                 hy = fma(phi, wy, fma(g, dy, hy));
                 hz = fma(phi, wz, fma(g, dz, hz));
-                if (a.dk && (jr[u] < 0 || j > row))
+                if (owned)
                 {
                     const double wu = dd * ir * C;
-                    a.dk[e] -= (float)(wu * (r - l0));
-                    a.dl0[e] += (float)(wu * k);
+                    a.dk[slot] -= (float)(wu * (r - l0));
+                    a.dl0[slot] += (float)(wu * k);
                 }
             }
 #pragma unroll
@@ -1573,11 +1580,10 @@ void launch_unperm_f64(const double *in, const int32_t *perm, int32_t n, double 
         k_unperm_f64<<<nblk(n), BLOCK, 0, s>>>(in, perm, n, out);
 }
 
-void launch_spring_grads(const int32_t *slot_e, float *dke, float *dle, int64_t ns, double *dk, double *dl0,
-                         hipStream_t s)
+void launch_spring_grads(float *dke, float *dle, int64_t ns, double *dk, double *dl0, hipStream_t s)
 {
     if (ns > 0)
-        k_spring_grads<<<nblk(ns), BLOCK, 0, s>>>(slot_e, dke, dle, ns, dk, dl0);
+        k_spring_grads<<<nblk(ns), BLOCK, 0, s>>>(dke, dle, ns, dk, dl0);
 }
 
 void launch_spring_out(const int32_t *slot_of, int64_t m, const double *dk, const double *dl0, double *outk,

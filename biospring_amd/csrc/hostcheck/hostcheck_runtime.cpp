@@ -798,43 +798,46 @@ void launch_adj_spring(const AdjSpringArgs &a, int32_t group, int32_t nb, hipStr
         need(a.sp_j, sizeof(int32_t) * ne, "adj_spring: sp_j");
         need(a.sp_k, sizeof(float) * ne, "adj_spring: sp_k");
         need(a.sp_l0, sizeof(float) * ne, "adj_spring: sp_l0");
-        if (a.dk)
-        {
-            need(a.dk, sizeof(float) * ne, "adj_spring: dk");
-            need(a.dl0, sizeof(float) * ne, "adj_spring: dl0");
-        }
+    }
+    if (a.dk)
+    {
+        need(a.slot_ptr, sizeof(int32_t) * (n + 1), "adj_spring: slot_ptr");
+        need(a.dk, sizeof(float) * a.slot_ptr[n], "adj_spring: dk");
+        need(a.dl0, sizeof(float) * a.slot_ptr[n], "adj_spring: dl0");
     }
     for (int32_t i = 0; i < n; ++i)
     {
         const bool dyn = !(fbits(a.jprm[i].w) & PF_STATIC);
+        int32_t slot = a.dk ? a.slot_ptr[i] : 0;
         if (dyn && a.spring_enable)
             for (int32_t e = a.sp_ptr[i]; e < a.sp_ptr[i + 1]; ++e)
             {
                 if ((a.sp_j[e] & 0x7fffffff) >= n)
                     hc_fail("adj_spring: edge index out of range");
                 if (a.dk && (a.sp_j[e] < 0 || a.sp_j[e] > i))
-                    a.dk[e] += 1.0f;
+                {
+                    if (slot >= a.slot_ptr[i + 1])
+                        hc_fail("adj_spring: more owned edges than slots in a row");
+                    a.dk[slot++] += 1.0f;
+                }
             }
+        if (a.dk && dyn && a.spring_enable && slot != a.slot_ptr[i + 1])
+            hc_fail("adj_spring: fewer owned edges than slots in a row");
         a.w_next[i] = make_float4(0, 0, 0, 0);
     }
 }
 
-void launch_spring_grads(const int32_t *slot_e, float *dke, float *dle, int64_t ns, double *dk, double *dl0,
-                         hipStream_t)
+void launch_spring_grads(float *dke, float *dle, int64_t ns, double *dk, double *dl0, hipStream_t)
 {
-    need(slot_e, sizeof(int32_t) * ns, "spring_grads: slot_e");
+    need(dke, sizeof(float) * ns, "spring_grads: dke");
+    need(dle, sizeof(float) * ns, "spring_grads: dle");
     need(dk, sizeof(double) * ns, "spring_grads: dk");
     need(dl0, sizeof(double) * ns, "spring_grads: dl0");
     for (int64_t r = 0; r < ns; ++r)
     {
-        const int32_t e = slot_e[r];
-        if (r > 0 && e <= slot_e[r - 1])
-            hc_fail("spring_grads: slots not in edge order");
-        need(dke + e, sizeof(float), "spring_grads: dke");
-        need(dle + e, sizeof(float), "spring_grads: dle");
-        dk[r] += dke[e];
-        dl0[r] += dle[e];
-        dke[e] = dle[e] = 0.0f;
+        dk[r] += dke[r];
+        dl0[r] += dle[r];
+        dke[r] = dle[r] = 0.0f;
     }
 }
 
