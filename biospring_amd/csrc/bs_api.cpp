@@ -1473,10 +1473,14 @@ int adjoint_sweep(bs_handle *h, AdjBufs &b, double *dk, double *dl0, double *dq)
         }
         if (recompute)
             tp.seg_of = c;
-        // lists rebuilt at each segment's first reversed state, so that the
-        // sweep does not depend on the lists a recomputation or an earlier
-        // call left: two sweeps of one record give the same bits
-        launch_fill_i32(h->status + 7, 1, 1, nullptr, st);
+        // The reversed steps start from the lists the recomputation left
+        // (rebuilt from its checkpoint like the forward pass, so the same
+        // on every sweep).  The last segment is the one a first sweep takes
+        // from the record, with whatever lists the record's last step
+        // left: its lists are rebuilt at its first reversed state, on every
+        // sweep, so that two sweeps of one record give the same bits.
+        if (!recompute || c == ncp - 1)
+            launch_fill_i32(h->status + 7, 1, 1, nullptr, st);
         for (int32_t s = s1 - 1; s >= s0; --s)
         {
             HIPCHK(h, hipMemcpyAsync(h->pos[h->cur], tp.seg + (size_t)(s - s0) * n, s4, hipMemcpyDeviceToDevice, st));

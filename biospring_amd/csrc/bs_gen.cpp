@@ -314,7 +314,7 @@ void bsg_params(int config, bs_params * p)
         p->electrostatic_enable = 1;
         p->electrostatic_cutoff = 12.0f;
         p->dielectric_model = BS_DIELECTRIC_DISTANCE;
-        p->list_skin = 0.5f; // beads move < 0.25 A in 10 steps: no rebuilds beyond the interval
+        p->list_skin = 0.25f; // beads move < 0.125 A in 10 steps: no rebuilds beyond the interval
         p->rebuild_interval = 10;
         break;
     case 2:

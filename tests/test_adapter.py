@@ -17,6 +17,7 @@ import os
 import subprocess
 
 import pytest
+from conftest import make_locked
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 REF = "/root/reference/src"
@@ -29,7 +30,7 @@ REF_OBJ = os.path.join(ROOT, "oracle", "_ref", "obj")
 def driver(tmp_path_factory):
     if not os.path.isdir(REF) or not glob.glob(os.path.join(REF_OBJ, "*.o")):
         pytest.skip("the reference sources and their objects (make -C oracle ref) are not here")
-    subprocess.run(["make", "-C", CSRC, "hostcheck"], check=True, stdout=subprocess.DEVNULL)
+    make_locked("-C", CSRC, "hostcheck")
     exe = str(tmp_path_factory.mktemp("adapter") / "adapter_check")
     cmd = ["g++", "-std=c++20", "-O1", "-fopenmp", "-DOPENMP_SUPPORT", "-w",
            f"-I{REF}", f"-I{REF}/spn", f"-I{REF}/IO/xdrfile/src", f"-I{ROOT}/include",

@@ -11,6 +11,7 @@ import sys
 import textwrap
 
 import pytest
+from conftest import make_locked
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CSRC = os.path.join(ROOT, "biospring_amd", "csrc")
@@ -19,7 +20,7 @@ HOSTCHECK = os.path.join(CSRC, "_build", "libbs_hostcheck.so")
 
 @pytest.fixture(scope="module")
 def hostcheck_lib():
-    subprocess.run(["make", "-C", CSRC, "hostcheck"], check=True, stdout=subprocess.DEVNULL)
+    make_locked("-C", CSRC, "hostcheck")
     assert os.path.exists(HOSTCHECK)
     return HOSTCHECK
 

@@ -20,6 +20,7 @@ scipy_io = pytest.importorskip("scipy.io")
 
 from biospring_amd import _abi, workloads  # noqa: E402
 from biospring_amd.network import SpringNetwork  # noqa: E402
+from conftest import make_locked
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CSRC = os.path.join(ROOT, "biospring_amd", "csrc")
@@ -29,7 +30,7 @@ EXAMPLE = "/root/reference/scripts/outdated/articulatedrigidbody/example/1IYT.nc
 
 @pytest.fixture(scope="module")
 def lib():
-    subprocess.run(["make", "-C", CSRC, "hostcheck"], check=True, stdout=subprocess.DEVNULL)
+    make_locked("-C", CSRC, "hostcheck")
     return _abi.bind(ctypes.CDLL(HOSTCHECK))
 
 

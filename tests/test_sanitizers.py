@@ -16,6 +16,7 @@ import sys
 import textwrap
 
 import pytest
+from conftest import make_locked
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CSRC = os.path.join(ROOT, "biospring_amd", "csrc")
@@ -34,8 +35,8 @@ def asan():
     lib = _libasan()
     if lib is None:
         pytest.skip("libasan is not installed")
-    subprocess.run(["make", "-s", "-C", CSRC, "hostcheck-san"], check=True)
-    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "san"], check=True)
+    make_locked("-s", "-C", CSRC, "hostcheck-san")
+    make_locked("-s", "-C", os.path.join(ROOT, "oracle"), "san")
     return lib
 
 
