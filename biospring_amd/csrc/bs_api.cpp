@@ -102,8 +102,15 @@ struct bs_handle
     int64_t n = 0;
     std::vector<float> x, v, mass, charge, radius, eps, sasa, transfer, hydro;
     std::vector<uint8_t> flags;
+    // The spring table (original ids) in host memory (sp_host) and/or on the
+    // device (sp_dev): construction leaves it on the device and setup reads
+    // it there; the host copy is filled on demand (springs_host).
     std::vector<uint32_t> s1, s2;
     std::vector<float> sl0, sk;
+    uint32_t *dsa = nullptr, *dsb = nullptr;
+    float *dsl = nullptr, *dsk = nullptr;
+    int64_t dsp_cap = 0, nsp = 0; // device capacity, springs
+    bool sp_host = true, sp_dev = false;
     bool have_particles = false, have_springs = false, ready = false;
 
     // internal order
@@ -1159,15 +1166,115 @@ struct Scratch
     }
 };
 
+// The host copy of the spring table, downloaded when the device holds the
+// only copy (after a construction).
+int springs_host(bs_handle *h)
+{
+    if (h->sp_host)
+        return BS_OK;
+    const size_t m = (size_t)h->nsp;
+    h->s1.resize(m);
+    h->s2.resize(m);
+    h->sl0.resize(m);
+    h->sk.resize(m);
+    if (m)
+    {
+        HIPCHK(h, hipMemcpy(h->s1.data(), h->dsa, sizeof(uint32_t) * m, hipMemcpyDeviceToHost));
+        HIPCHK(h, hipMemcpy(h->s2.data(), h->dsb, sizeof(uint32_t) * m, hipMemcpyDeviceToHost));
+        HIPCHK(h, hipMemcpy(h->sl0.data(), h->dsl, sizeof(float) * m, hipMemcpyDeviceToHost));
+        HIPCHK(h, hipMemcpy(h->sk.data(), h->dsk, sizeof(float) * m, hipMemcpyDeviceToHost));
+    }
+    h->sp_host = true;
+    return BS_OK;
+}
+
+// Device table of capacity >= cap whose first keep springs are the current
+// table's (device copy, or uploaded from the host copy).
+int springs_device_reserve(bs_handle *h, int64_t cap, int64_t keep)
+{
+    if (keep > 0 && !h->sp_dev && !h->sp_host)
+        return fail(h, BS_ERR_STATE, "internal: no copy of the spring table");
+    hipError_t e = hipSuccess;
+    if (cap > h->dsp_cap)
+    {
+        uint32_t *a = dalloc<uint32_t>((size_t)cap, e), *b = nullptr;
+        float *l = nullptr, *k = nullptr;
+        if (e == hipSuccess)
+            b = dalloc<uint32_t>((size_t)cap, e);
+        if (e == hipSuccess)
+            l = dalloc<float>((size_t)cap, e);
+        if (e == hipSuccess)
+            k = dalloc<float>((size_t)cap, e);
+        if (e == hipSuccess && keep > 0 && h->sp_dev)
+        {
+            const size_t kb = (size_t)keep;
+            e = hipMemcpy(a, h->dsa, sizeof(uint32_t) * kb, hipMemcpyDeviceToDevice);
+            if (e == hipSuccess)
+                e = hipMemcpy(b, h->dsb, sizeof(uint32_t) * kb, hipMemcpyDeviceToDevice);
+            if (e == hipSuccess)
+                e = hipMemcpy(l, h->dsl, sizeof(float) * kb, hipMemcpyDeviceToDevice);
+            if (e == hipSuccess)
+                e = hipMemcpy(k, h->dsk, sizeof(float) * kb, hipMemcpyDeviceToDevice);
+        }
+        if (e != hipSuccess)
+        {
+            for (void *q : {(void *)a, (void *)b, (void *)l, (void *)k})
+                if (q)
+                    (void)hipFree(q);
+            HIPCHK(h, e);
+        }
+        for (void *q : {(void *)h->dsa, (void *)h->dsb, (void *)h->dsl, (void *)h->dsk})
+            if (q)
+                (void)hipFree(q);
+        h->dsa = a;
+        h->dsb = b;
+        h->dsl = l;
+        h->dsk = k;
+        h->dsp_cap = cap;
+    }
+    if (keep > 0 && !h->sp_dev)
+    {
+        const size_t kb = (size_t)keep;
+        HIPCHK(h, hipMemcpy(h->dsa, h->s1.data(), sizeof(uint32_t) * kb, hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemcpy(h->dsb, h->s2.data(), sizeof(uint32_t) * kb, hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemcpy(h->dsl, h->sl0.data(), sizeof(float) * kb, hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemcpy(h->dsk, h->sk.data(), sizeof(float) * kb, hipMemcpyHostToDevice));
+    }
+    return BS_OK;
+}
+
+// The device copy of the spring table, uploaded when only the host has it.
+int springs_device(bs_handle *h)
+{
+    if (h->sp_dev)
+        return BS_OK;
+    const int rc = springs_device_reserve(h, std::max<int64_t>(h->nsp, 1), h->nsp);
+    if (rc)
+        return rc;
+    h->sp_dev = true;
+    return BS_OK;
+}
+
+// The host copy was replaced: the device copy is stale.
+void springs_host_changed(bs_handle *h)
+{
+    h->nsp = (int64_t)h->s1.size();
+    h->sp_host = true;
+    h->sp_dev = false;
+}
+
 // The CSR edge on which the adjoint sweep accumulates each spring's
 // gradients: the edge of the dynamic row that owns it (a static partner, or
 // else the larger index is the partner; the rule of k_adj_spring), -1 when
 // both ends are static.  Built once per setup.
 int build_spring_owners(bs_handle *h)
 {
-    const int64_t n = h->n, m = (int64_t)h->s1.size(), E = h->nedges;
+    const int64_t n = h->n, m = h->nsp, E = h->nedges;
     if (h->nslots >= 0 || m == 0)
         return BS_OK;
+    const int hrc = springs_host(h);
+    if (hrc)
+        return hrc;
     std::vector<int32_t> hp((size_t)n + 1), hj((size_t)std::max<int64_t>(E, 1)), own((size_t)m, -1);
     HIPCHK(h, hipMemcpy(hp.data(), h->sp_ptr, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToHost));
     if (E > 0)
@@ -1262,7 +1369,7 @@ int adjoint_record(bs_handle *h, int32_t T, int32_t checkpoint_every)
 {
     auto &tp = h->tape;
     tp.ready = false;
-    if (!h->s1.empty())
+    if (h->nsp > 0)
     { // the backward calls find the spring gradient slots built
         const int rc = build_spring_owners(h);
         if (rc)
@@ -1369,7 +1476,7 @@ struct AdjBufs
 // built); seed: room for a caller's dL/dx, dL/dv.
 size_t adj_carve(const bs_handle *h, bool springs, bool seed, char *base, AdjBufs &b)
 {
-    const size_t n = (size_t)h->n, ns = (size_t)std::max<int64_t>(h->nslots, 0), m = h->s1.size();
+    const size_t n = (size_t)h->n, ns = (size_t)std::max<int64_t>(h->nslots, 0), m = (size_t)h->nsp;
     size_t off = 0;
     auto take = [&](auto *&ptr, size_t count) {
         using T = std::remove_pointer_t<std::remove_reference_t<decltype(ptr)>>;
@@ -1414,7 +1521,7 @@ int adjoint_sweep(bs_handle *h, AdjBufs &b, double *dk, double *dl0, double *dq)
 {
     auto &tp = h->tape;
     const bs_params &P = h->p;
-    const int64_t n = h->n, m = (int64_t)h->s1.size();
+    const int64_t n = h->n, m = h->nsp;
     const int32_t T = tp.T, K = tp.K, ncp = tp.ncp;
     const size_t s4 = sizeof(float4) * (size_t)n;
     hipStream_t st = h->stream;
@@ -1649,87 +1756,108 @@ int do_setup(bs_handle *h)
     HIPCHK(h, hipMemcpyAsync(h->perm_d, h->perm.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
 
-    // spring CSR, both directions, rows sorted by j (internal ids)
-    const int64_t m = (int64_t)h->s1.size();
-    std::vector<int32_t> ptr((size_t)n + 1, 0);
-    for (int64_t s = 0; s < m; ++s)
-    {
-        ptr[(size_t)h->iperm[h->s1[(size_t)s]] + 1]++;
-        ptr[(size_t)h->iperm[h->s2[(size_t)s]] + 1]++;
-    }
-    for (int64_t i = 0; i < n; ++i)
-        ptr[(size_t)i + 1] += ptr[(size_t)i];
-    h->nedges = ptr[(size_t)n];
-    struct Edge
-    {
-        int32_t j;
-        float k, l0;
-    };
-    std::vector<Edge> edges((size_t)h->nedges);
-    {
-        std::vector<int32_t> fill(ptr.begin(), ptr.end() - 1);
-        for (int64_t s = 0; s < m; ++s)
-        {
-            const int32_t a = h->iperm[h->s1[(size_t)s]], b = h->iperm[h->s2[(size_t)s]];
-            edges[(size_t)fill[(size_t)a]++] = {b, h->sk[(size_t)s], h->sl0[(size_t)s]};
-            edges[(size_t)fill[(size_t)b]++] = {a, h->sk[(size_t)s], h->sl0[(size_t)s]};
-        }
-    }
-    std::vector<int32_t> ej((size_t)h->nedges);
-    std::vector<float> ek((size_t)h->nedges), el((size_t)h->nedges);
-#pragma omp parallel for schedule(dynamic, 1024)
-    for (int64_t i = 0; i < n; ++i)
-    {
-        std::sort(edges.begin() + ptr[(size_t)i], edges.begin() + ptr[(size_t)i + 1],
-                  [](const Edge &a, const Edge &b) { return a.j < b.j; });
-        for (int32_t t = ptr[(size_t)i]; t < ptr[(size_t)i + 1]; ++t)
-        {
-            const int32_t j = edges[(size_t)t].j;
-            const bool js = (h->flags[(size_t)h->perm[(size_t)j]] & BS_FLAG_STATIC) != 0;
-            ej[(size_t)t] = js ? (int32_t)((uint32_t)j | 0x80000000u) : j;
-            ek[(size_t)t] = edges[(size_t)t].k;
-            el[(size_t)t] = edges[(size_t)t].l0;
-        }
-    }
-    // The force CSR leaves out springs between two rigid particles
-    // (RigidBody::UpdateSpringsState, RigidBody.cpp:42-53; Spring.cpp:24);
-    // the exclusions below keep every spring.
-    std::vector<int32_t> fptr, fej;
-    std::vector<float> fek, fel;
+    // Spring CSR on the device from the device copy of the spring table:
+    // both directions, rows sorted by partner (internal ids).  The full rows
+    // give each i-cluster's exclusions; the force rows leave out springs
+    // between two rigid particles (RigidBody::UpdateSpringsState,
+    // RigidBody.cpp:42-53; Spring.cpp:24) while the exclusions keep every
+    // spring.
+    int rc = springs_device(h);
+    if (rc)
+        return rc;
+    const int64_t m = h->nsp;
+    if (m > (int64_t)(INT32_MAX / 2))
+        return fail(h, BS_ERR_ARG, "bs_setup: more than 2^30 springs");
+    hipStream_t st = h->stream;
+    Scratch S;
+    int32_t *iperm_d = S.get<int32_t>((size_t)n, e);
+    HIPCHK(h, e);
+    int32_t *cnt = S.get<int32_t>((size_t)n, e);
+    HIPCHK(h, e);
+    HIPCHK(h, hipMemcpyAsync(iperm_d, h->iperm.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
     const bool any_rigid = !h->bodies.empty();
+    int32_t *fptr = nullptr; // full rows
     if (any_rigid)
     {
-        fptr.assign((size_t)n + 1, 0);
-        for (int64_t i = 0; i < n; ++i)
-        {
-            const bool ri = h->body_of[(size_t)h->perm[(size_t)i]] >= 0;
-            for (int32_t t = ptr[(size_t)i]; t < ptr[(size_t)i + 1]; ++t)
-            {
-                const int32_t j = ej[(size_t)t] & 0x7fffffff;
-                if (ri && h->body_of[(size_t)h->perm[(size_t)j]] >= 0)
-                    continue;
-                fej.push_back(ej[(size_t)t]);
-                fek.push_back(ek[(size_t)t]);
-                fel.push_back(el[(size_t)t]);
-            }
-            fptr[(size_t)i + 1] = (int32_t)fej.size();
-        }
+        fptr = S.get<int32_t>((size_t)n + 1, e);
+        HIPCHK(h, e);
     }
-    const std::vector<int32_t> &up_ptr = any_rigid ? fptr : ptr, &up_j = any_rigid ? fej : ej;
-    const std::vector<float> &up_k = any_rigid ? fek : ek, &up_l = any_rigid ? fel : el;
-    h->nedges = up_ptr[(size_t)n];
-    ALLOC(h->sp_ptr, int32_t, n + 1);
-    ALLOC(h->sp_j, int32_t, h->nedges);
-    ALLOC(h->sp_k, float, h->nedges);
-    ALLOC(h->sp_l0, float, h->nedges);
-    HIPCHK(h, hipMemcpyAsync(h->sp_ptr, up_ptr.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, h->stream));
-    if (h->nedges)
+    else
     {
-        HIPCHK(h, hipMemcpyAsync(h->sp_j, up_j.data(), sizeof(int32_t) * h->nedges, hipMemcpyHostToDevice, h->stream));
-        HIPCHK(h, hipMemcpyAsync(h->sp_k, up_k.data(), sizeof(float) * h->nedges, hipMemcpyHostToDevice, h->stream));
-        HIPCHK(h, hipMemcpyAsync(h->sp_l0, up_l.data(), sizeof(float) * h->nedges, hipMemcpyHostToDevice, h->stream));
+        ALLOC(h->sp_ptr, int32_t, n + 1);
+        fptr = h->sp_ptr;
     }
-    int rc = upload_positions(h);
+    launch_fill_i32(cnt, 0, n, nullptr, st);
+    launch_edge_count(h->dsa, h->dsb, m, iperm_d, cnt, st);
+    launch_scan(cnt, fptr, (int32_t)n, nullptr, st);
+    int32_t efull = 0;
+    HIPCHK(h, hipMemcpyAsync(&efull, fptr + n, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(h, hipStreamSynchronize(st));
+    uint64_t *key = S.get<uint64_t>((size_t)efull, e);
+    HIPCHK(h, e);
+    int32_t *fej = nullptr;
+    float *fek = nullptr, *fel = nullptr;
+    if (any_rigid)
+    {
+        fej = S.get<int32_t>((size_t)efull, e);
+        HIPCHK(h, e);
+        fek = S.get<float>((size_t)efull, e);
+        HIPCHK(h, e);
+        fel = S.get<float>((size_t)efull, e);
+        HIPCHK(h, e);
+    }
+    else
+    {
+        ALLOC(h->sp_j, int32_t, efull);
+        ALLOC(h->sp_k, float, efull);
+        ALLOC(h->sp_l0, float, efull);
+        fej = h->sp_j;
+        fek = h->sp_k;
+        fel = h->sp_l0;
+    }
+    launch_fill_i32(cnt, 0, n, nullptr, st);
+    launch_edge_fill(h->dsa, h->dsb, m, iperm_d, fptr, cnt, key, st);
+    launch_edge_sort(fptr, (int32_t)n, key, h->jprm, h->dsl, h->dsk, fej, fek, fel, st);
+    // static exclusions per i-cluster: the union of its 8 rows as sorted
+    // (j, bonded-mask) words; springs never change during a run
+    {
+        const int32_t ncl = (int32_t)((n + CLUSTER - 1) / CLUSTER);
+        uint32_t *merged = S.get<uint32_t>((size_t)efull, e);
+        HIPCHK(h, e);
+        int32_t *ucnt = S.get<int32_t>((size_t)ncl, e);
+        HIPCHK(h, e);
+        launch_ex_merge(fptr, fej, (int32_t)n, ncl, merged, ucnt, st);
+        ALLOC(h->ex_ptr, int32_t, ncl + 1);
+        launch_scan(ucnt, h->ex_ptr, ncl, nullptr, st);
+        int32_t nex = 0;
+        HIPCHK(h, hipMemcpyAsync(&nex, h->ex_ptr + ncl, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        HIPCHK(h, hipStreamSynchronize(st));
+        ALLOC(h->ex_w, uint32_t, nex);
+        launch_ex_fill(fptr, (int32_t)n, ncl, merged, h->ex_ptr, h->ex_w, st);
+    }
+    h->nedges = efull;
+    if (any_rigid)
+    {
+        std::vector<uint8_t> rg((size_t)n);
+        for (int64_t t = 0; t < n; ++t)
+            rg[(size_t)t] = h->body_of[(size_t)h->perm[(size_t)t]] >= 0 ? 1 : 0;
+        uint8_t *rg_d = S.get<uint8_t>((size_t)n, e);
+        HIPCHK(h, e);
+        HIPCHK(h, hipMemcpyAsync(rg_d, rg.data(), (size_t)n, hipMemcpyHostToDevice, st));
+        launch_edge_keep(fptr, (int32_t)n, fej, fek, fel, rg_d, nullptr, cnt, nullptr, nullptr, nullptr, st);
+        ALLOC(h->sp_ptr, int32_t, n + 1);
+        launch_scan(cnt, h->sp_ptr, (int32_t)n, nullptr, st);
+        int32_t ekeep = 0;
+        HIPCHK(h, hipMemcpyAsync(&ekeep, h->sp_ptr + n, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        HIPCHK(h, hipStreamSynchronize(st)); // also: rg outlives its copy
+        ALLOC(h->sp_j, int32_t, ekeep);
+        ALLOC(h->sp_k, float, ekeep);
+        ALLOC(h->sp_l0, float, ekeep);
+        launch_edge_keep(fptr, (int32_t)n, fej, fek, fel, rg_d, h->sp_ptr, nullptr, h->sp_j, h->sp_k, h->sp_l0, st);
+        h->nedges = ekeep;
+    }
+    HIPCHK(h, hipStreamSynchronize(st)); // the scratch is freed on return
+    rc = upload_positions(h);
     if (rc)
         return rc;
 
@@ -1866,46 +1994,6 @@ int do_setup(bs_handle *h)
         pd.v = make_float4(h->probe.v[0], h->probe.v[1], h->probe.v[2], 0.0f);
         pd.f = make_double4(h->probe.f[0], h->probe.f[1], h->probe.f[2], 0.0);
         HIPCHK(h, hipMemcpy(h->probe_d, &pd, sizeof pd, hipMemcpyHostToDevice));
-    }
-
-    // static exclusions per i-cluster: the union of its 8 spring rows as
-    // sorted (j, bonded-mask) words; springs never change during a run
-    {
-        std::vector<std::vector<uint32_t>> ex((size_t)h->ncl);
-#pragma omp parallel for schedule(dynamic, 64)
-        for (int32_t c = 0; c < h->ncl; ++c)
-        {
-            std::vector<uint32_t> &w = ex[(size_t)c];
-            for (int32_t k = 0; k < CLUSTER; ++k)
-            {
-                const int64_t i = (int64_t)c * CLUSTER + k;
-                if (i >= n)
-                    break;
-                for (int32_t t = ptr[(size_t)i]; t < ptr[(size_t)i + 1]; ++t)
-                    w.push_back((uint32_t)(ej[(size_t)t] & 0x7fffffff) << 8 | (1u << k));
-            }
-            std::sort(w.begin(), w.end());
-            size_t o = 0;
-            for (size_t t = 0; t < w.size(); ++t)
-            {
-                if (o > 0 && (w[o - 1] >> 8) == (w[t] >> 8))
-                    w[o - 1] |= w[t] & 0xFFu;
-                else
-                    w[o++] = w[t];
-            }
-            w.resize(o);
-        }
-        std::vector<int32_t> xp((size_t)h->ncl + 1, 0);
-        for (int32_t c = 0; c < h->ncl; ++c)
-            xp[(size_t)c + 1] = xp[(size_t)c] + (int32_t)ex[(size_t)c].size();
-        std::vector<uint32_t> xw((size_t)xp[(size_t)h->ncl]);
-        for (int32_t c = 0; c < h->ncl; ++c)
-            std::copy(ex[(size_t)c].begin(), ex[(size_t)c].end(), xw.begin() + xp[(size_t)c]);
-        ALLOC(h->ex_ptr, int32_t, h->ncl + 1);
-        ALLOC(h->ex_w, uint32_t, xw.size());
-        HIPCHK(h, hipMemcpy(h->ex_ptr, xp.data(), sizeof(int32_t) * xp.size(), hipMemcpyHostToDevice));
-        if (!xw.empty())
-            HIPCHK(h, hipMemcpy(h->ex_w, xw.data(), sizeof(uint32_t) * xw.size(), hipMemcpyHostToDevice));
     }
 
     // neighbour lists: a counting pass sizes the capacity
@@ -2065,6 +2153,9 @@ int bs_destroy(bs_handle *h)
     for (hipEvent_t e : h->evpool)
         (void)hipEventDestroy(e);
     free_device(h);
+    for (void *q : {(void *)h->dsa, (void *)h->dsb, (void *)h->dsl, (void *)h->dsk})
+        if (q)
+            (void)hipFree(q);
     if (h->stream)
         (void)hipStreamDestroy(h->stream);
     delete h;
@@ -2079,7 +2170,7 @@ const char *bs_last_error(const bs_handle *h)
 }
 
 int64_t bs_num_particles(const bs_handle *h) { return h ? h->n : 0; }
-int64_t bs_num_springs(const bs_handle *h) { return h ? (int64_t)h->s1.size() : 0; }
+int64_t bs_num_springs(const bs_handle *h) { return h ? h->nsp : 0; }
 
 int bs_set_particles(bs_handle *h, int64_t n, const float *x3, const float *v3, const float *mass,
                      const float *charge, const float *radius, const float *epsilon, const float *sasa,
@@ -2119,6 +2210,7 @@ int bs_set_particles(bs_handle *h, int64_t n, const float *x3, const float *v3, 
     h->s2.clear();
     h->sl0.clear();
     h->sk.clear();
+    springs_host_changed(h);
     h->have_particles = true;
     h->steps_done = 0;
     h->nonfinite = -1;
@@ -2170,6 +2262,7 @@ int bs_set_springs(bs_handle *h, int64_t m, const uint32_t *id1, const uint32_t 
     h->s2.swap(s2);
     h->sl0.swap(l0v);
     h->sk.swap(kv);
+    springs_host_changed(h);
     h->have_springs = true;
     return BS_OK;
 }
@@ -2842,6 +2935,9 @@ int bs_set_systems(bs_handle *h, int32_t nsys, const int64_t *offsets)
     for (int32_t s = 0; s < nsys; ++s)
         for (int64_t i = offsets[s]; i < offsets[s + 1]; ++i)
             sys[(size_t)i] = s;
+    const int hrc = springs_host(h);
+    if (hrc)
+        return hrc;
     for (size_t t = 0; t < h->s1.size(); ++t)
         if (sys[h->s1[t]] != sys[h->s2[t]])
             return fail(h, BS_ERR_ARG, "bs_set_systems: spring %zu joins two systems", t);
@@ -2948,7 +3044,7 @@ int bs_adjoint_backward(bs_handle *h, const double *dl_dx3, const double *dl_dv3
     int rc = adjoint_backward_check(h, dl_dx3, dl_dv3, "bs_adjoint_backward");
     if (rc)
         return rc;
-    const bool springs = (dk || dl0) && !h->s1.empty();
+    const bool springs = (dk || dl0) && h->nsp > 0;
     if (springs)
     {
         rc = build_spring_owners(h);
@@ -2976,7 +3072,7 @@ int bs_adjoint_backward_noalloc(bs_handle *h, const double *dl_dx3, const double
         return rc;
     if (h->prof)
         return fail(h, BS_ERR_STATE, "%s: profiling creates events; turn it off first", who);
-    const bool springs = (dk || dl0) && !h->s1.empty();
+    const bool springs = (dk || dl0) && h->nsp > 0;
     if (springs && h->nslots < 0)
         return fail(h, BS_ERR_STATE, "%s: spring gradient slots missing; run bs_adjoint_forward first", who);
     char *base = nullptr;
@@ -2989,7 +3085,7 @@ int bs_adjoint_backward_noalloc(bs_handle *h, const double *dl_dx3, const double
         return fail(h, BS_ERR_ARG,
                     "%s: this system needs %zu bytes of device scratch, more than the fixed %zu (%lld particles, "
                     "%lld springs); use bs_adjoint_backward",
-                    who, bytes, NOALLOC_BYTES, (long long)h->n, (long long)h->s1.size());
+                    who, bytes, NOALLOC_BYTES, (long long)h->n, (long long)h->nsp);
     adj_carve(h, springs, true, base, b);
     return adjoint_backward_on(h, b, dl_dx3, dl_dv3, dk, dl0, dq);
 }
@@ -3006,7 +3102,7 @@ int bs_adjoint(bs_handle *h, int32_t nsteps, int32_t checkpoint_every, const flo
         return rc;
     // loss and seed lx = 2 (x_T - y) / n on the device
     const int64_t n = h->n;
-    const bool springs = (dk || dl0) && !h->s1.empty();
+    const bool springs = (dk || dl0) && h->nsp > 0;
     AdjBufs b;
     const size_t bytes = adj_carve(h, springs, false, nullptr, b);
     Scratch S;
@@ -3083,7 +3179,13 @@ static int springs_from_cutoff(bs_handle *h, double cutoff, float stiffness, con
     int64_t m = 0; // springs added
     // springs already loaded stay, with their own rest length and stiffness;
     // their pairs get no second spring (SpringCollection.hpp:138-151,226-233)
-    const int64_t m0 = h->have_springs ? (int64_t)h->s1.size() : 0;
+    const int64_t m0 = h->have_springs ? h->nsp : 0;
+    if (m0 > 0)
+    {
+        rc = springs_host(h);
+        if (rc)
+            return rc;
+    }
     if (n > 1)
     {
         std::vector<float4> ph((size_t)n);
@@ -3208,52 +3310,38 @@ static int springs_from_cutoff(bs_handle *h, double cutoff, float stiffness, con
         }
         int32_t *offs = S.get<int32_t>((size_t)n + 1, e);
         HIPCHK(h, e);
-        int32_t *jout = S.get<int32_t>((size_t)std::max<int64_t>(total, 1), e);
-        HIPCHK(h, e);
-        float *lout = S.get<float>((size_t)std::max<int64_t>(total, 1), e);
-        HIPCHK(h, e);
         HIPCHK(h, hipMemcpyAsync(offs, rows.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, st));
+        // the new springs go straight into the device table, after the
+        // loaded ones; the host copy is filled when something asks for it
+        rc = springs_device_reserve(h, m0 + std::max<int64_t>(total, 1), m0);
+        if (rc)
+            return rc;
         a.offs = offs;
         a.max_row = max_row;
-        a.jout = jout;
-        a.l0out = lout;
+        a.jout = reinterpret_cast<int32_t *>(h->dsb + m0);
+        a.l0out = h->dsl + m0;
         const size_t e2 = h->prof ? prof_mark(h) : SIZE_MAX;
         launch_topo_springs(a, st);
         if (h->prof)
             prof_span(h, BS_PROF_TOPO_FILL, e2, prof_mark(h));
-        // the new springs go straight into the tables, after the loaded ones
-        m = total;
-        h->s1.resize((size_t)(m0 + m));
-        h->s2.resize((size_t)(m0 + m));
-        h->sl0.resize((size_t)(m0 + m));
-        hipError_t ce = hipSuccess;
-        if (m > 0)
-        {
-            ce = hipMemcpyAsync(h->s2.data() + m0, jout, sizeof(int32_t) * m, hipMemcpyDeviceToHost, st);
-            if (ce == hipSuccess)
-                ce = hipMemcpyAsync(h->sl0.data() + m0, lout, sizeof(float) * m, hipMemcpyDeviceToHost, st);
-        }
-        uint32_t *s1 = h->s1.data() + m0;
-#pragma omp parallel for schedule(static, 1024)
-        for (int64_t i = 0; i < n; ++i)
-            for (int32_t t = rows[(size_t)i]; t < rows[(size_t)i + 1]; ++t)
-                s1[t] = (uint32_t)i;
-        if (ce == hipSuccess)
-            ce = hipStreamSynchronize(st);
-        if (ce != hipSuccess)
-        {
-            h->s1.resize((size_t)m0);
-            h->s2.resize((size_t)m0);
-            h->sl0.resize((size_t)m0);
-            HIPCHK(h, ce);
-        }
+        launch_rows_expand(offs, (int32_t)n, h->dsa + m0, st);
+        launch_fill_i32(reinterpret_cast<int32_t *>(h->dsk + m0), __builtin_bit_cast(int32_t, stiffness), total,
+                        nullptr, st);
+        HIPCHK(h, hipStreamSynchronize(st));
         if (h->prof)
             prof_collect(h);
+        m = total;
+        h->nsp = m0 + m;
+        h->sp_dev = true;
+        h->sp_host = false;
         if (h->sys_off.size() > 2)
         {
             // independent systems (bs_set_systems): no spring joins two of
             // them; rows are sorted by j > i, so a row keeps its j below the
             // end of i's system
+            rc = springs_host(h);
+            if (rc)
+                return rc;
             size_t w = (size_t)m0, sy = 0;
             for (int64_t i = 0; i < n; ++i)
             {
@@ -3266,6 +3354,7 @@ static int springs_from_cutoff(bs_handle *h, double cutoff, float stiffness, con
                         h->s1[w] = h->s1[(size_t)(m0 + t)];
                         h->s2[w] = h->s2[(size_t)(m0 + t)];
                         h->sl0[w] = h->sl0[(size_t)(m0 + t)];
+                        h->sk[w] = h->sk[(size_t)(m0 + t)];
                         ++w;
                     }
             }
@@ -3273,10 +3362,10 @@ static int springs_from_cutoff(bs_handle *h, double cutoff, float stiffness, con
             h->s1.resize(w);
             h->s2.resize(w);
             h->sl0.resize(w);
+            h->sk.resize(w);
+            springs_host_changed(h);
         }
     }
-    h->sk.resize((size_t)m0);
-    h->sk.resize((size_t)(m0 + m), stiffness);
     free_device(h);
     h->have_springs = true;
     if (count_out)
@@ -3288,6 +3377,10 @@ int bs_get_springs(const bs_handle *h, uint32_t *id1, uint32_t *id2, float *equi
 {
     if (!h)
         return BS_ERR_ARG;
+    // filling the host copy changes no observable state
+    const int hrc = springs_host(const_cast<bs_handle *>(h));
+    if (hrc)
+        return hrc;
     const size_t m = h->s1.size();
     if (id1 && m)
         std::memcpy(id1, h->s1.data(), sizeof(uint32_t) * m);
@@ -3357,6 +3450,7 @@ int bs_autotune(bs_handle *h, int32_t steps, int32_t *group_out, int32_t *graph_
             t->s2 = h->s2;
             t->sl0 = h->sl0;
             t->sk = h->sk;
+            springs_host_changed(t);
             t->bodies = h->bodies;
             t->body_of = h->body_of;
             t->burying = h->burying;
@@ -3518,6 +3612,9 @@ int bs_add_trajectory(bs_handle *h, int32_t kind, const char *path, int64_t freq
     ctx->echarge_default.resize((size_t)h->n);
     for (size_t i = 0; i < (size_t)h->n; ++i)
         ctx->echarge_default[i] = (int32_t)h->charge[i]; // ParticleProperty::setCharge
+    const int hrc = springs_host(h);
+    if (hrc)
+        return hrc;
     ctx->conect.reserve(h->s1.size());
     for (size_t t = 0; t < h->s1.size(); ++t)
         ctx->conect.emplace_back(h->s1[t], h->s2[t]);
@@ -3645,6 +3742,9 @@ int bs_save_netcdf(bs_handle *h, const char *path)
     t.resname = h->labels.resname;
     t.chain = h->labels.chain;
     t.resid = h->labels.resid;
+    const int hrc = springs_host(h);
+    if (hrc)
+        return hrc;
     t.s1 = h->s1;
     t.s2 = h->s2;
     t.l0 = h->sl0;

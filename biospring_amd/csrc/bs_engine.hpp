@@ -391,6 +391,31 @@ void launch_spring_grads(float *dke, float *dle, int64_t nslots, double *dk, dou
 void launch_spring_out(const int32_t *slot_of, int64_t m, const double *dk, const double *dl0, double *outk,
                        double *outl, hipStream_t s);
 void launch_topo_springs(const TopoArgs &a, hipStream_t s);
+// Spring CSR on the device from the spring table (sa, sb: external ids; m
+// springs; iperm: external -> internal).  Count pass (cnt zeroed: both ends
+// of each spring), fill pass (cursor zeroed; 64-bit keys partner << 32 |
+// spring << 1 | side at the rows' offsets ptr), then the rows sorted by key
+// and written as partner (bit 31: static partner), stiffness, rest length.
+void launch_edge_count(const uint32_t *sa, const uint32_t *sb, int64_t m, const int32_t *iperm, int32_t *cnt,
+                       hipStream_t s);
+void launch_edge_fill(const uint32_t *sa, const uint32_t *sb, int64_t m, const int32_t *iperm, const int32_t *ptr,
+                      int32_t *cursor, uint64_t *key, hipStream_t s);
+void launch_edge_sort(const int32_t *ptr, int32_t n, const uint64_t *key, const float4 *jprm, const float *sl,
+                      const float *sk, int32_t *ej, float *ek, float *el, hipStream_t s);
+// Exclusions of each i-cluster from its rows: merged (j << 8 | row bit, in
+// j order) into merged[ptr[8c] ..) with the count of distinct j per cluster,
+// then the distinct words (row bits of equal j or-ed) at ex_ptr[c].
+void launch_ex_merge(const int32_t *ptr, const int32_t *ej, int32_t n, int32_t ncl, uint32_t *merged,
+                     int32_t *ucount, hipStream_t s);
+void launch_ex_fill(const int32_t *ptr, int32_t n, int32_t ncl, const uint32_t *merged, const int32_t *ex_ptr,
+                    uint32_t *ex_w, hipStream_t s);
+// Rows without the edges joining two rigid members (rigid[i] != 0): fptr
+// null counts them into cnt; otherwise they are copied to fptr's offsets.
+void launch_edge_keep(const int32_t *ptr, int32_t n, const int32_t *ej, const float *ek, const float *el,
+                      const uint8_t *rigid, const int32_t *fptr, int32_t *cnt, int32_t *fej, float *fek, float *fel,
+                      hipStream_t s);
+// ida[t] = i for t in [offs[i], offs[i + 1]).
+void launch_rows_expand(const int32_t *offs, int32_t n, uint32_t *ida, hipStream_t s);
 void launch_pack_positions(const float4 *pos, const float4 *plo, const int32_t *perm, int32_t n, float *out3,
                            hipStream_t s);
 // status[7] (the drift flag) = 1 when any particle is more than half the list skin from xref.

@@ -1476,6 +1476,196 @@ __global__ void k_fill_i32(int32_t *p, int32_t v, int64_t n, const int32_t *gate
         p[i] = v;
 }
 
+// ---------------------------------------------------------------------------
+// Spring CSR built on the device from the spring table (external ids): both
+// directions, each row sorted by (partner, spring, side); the exclusions of
+// each i-cluster merged from its rows; springs between two rigid members
+// left out of the force rows.
+// ---------------------------------------------------------------------------
+__global__ void k_edge_count(const uint32_t *sa, const uint32_t *sb, int64_t m, const int32_t *iperm, int32_t *cnt)
+{
+    const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s >= m)
+        return;
+    atomicAdd(&cnt[iperm[sa[s]]], 1);
+    atomicAdd(&cnt[iperm[sb[s]]], 1);
+}
+
+// key = partner << 32 | (spring << 1 | side), placed at an arbitrary slot of
+// the row (k_edge_sort orders it)
+__global__ void k_edge_fill(const uint32_t *sa, const uint32_t *sb, int64_t m, const int32_t *iperm,
+                            const int32_t *ptr, int32_t *cursor, uint64_t *key)
+{
+    const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s >= m)
+        return;
+    const int32_t a = iperm[sa[s]], b = iperm[sb[s]];
+    const uint32_t w = (uint32_t)s << 1;
+    key[ptr[a] + atomicAdd(&cursor[a], 1)] = (uint64_t)(uint32_t)b << 32 | w;
+    key[ptr[b] + atomicAdd(&cursor[b], 1)] = (uint64_t)(uint32_t)a << 32 | (w | 1u);
+}
+
+constexpr int32_t EDGE_WAVES = BLOCK / 64;
+constexpr int32_t EDGE_LDS = 1024; // keys per wave held in LDS; longer rows rank from global memory
+
+// One wave per row: each key's rank is the number of smaller keys in the row
+// (keys are distinct).  Writes the partner (bit 31: static partner), its
+// stiffness and rest length at the rank.
+__global__ void __launch_bounds__(BLOCK) k_edge_sort(const int32_t *ptr, int32_t n, const uint64_t *key,
+                                                     const float4 *jprm, const float *sl, const float *sk, int32_t *ej,
+                                                     float *ek, float *el)
+{
+    __shared__ uint64_t lk[EDGE_WAVES][EDGE_LDS];
+    const int32_t lane = (int32_t)(threadIdx.x & 63u), w = (int32_t)(threadIdx.x >> 6);
+    const int32_t row = (int32_t)blockIdx.x * EDGE_WAVES + w;
+    if (row >= n)
+        return; // whole wave; no block barrier below
+    const int32_t b = ptr[row], len = ptr[row + 1] - b;
+    const uint64_t *src = key + b;
+    if (len <= EDGE_LDS)
+    {
+        for (int32_t t = lane; t < len; t += 64)
+            lk[w][t] = key[b + t];
+        wave_lds_sync();
+        src = lk[w];
+    }
+    for (int32_t e = lane; e < len; e += 64)
+    {
+        const uint64_t kv = src[e];
+        int32_t rank = 0;
+        for (int32_t t = 0; t < len; ++t)
+            rank += src[t] < kv ? 1 : 0;
+        const int32_t j = (int32_t)(kv >> 32);
+        const uint32_t s = (uint32_t)kv >> 1;
+        ej[b + rank] = (fbits(jprm[j].w) & PF_STATIC) ? (int32_t)((uint32_t)j | 0x80000000u) : j;
+        ek[b + rank] = sk[s];
+        el[b + rank] = sl[s];
+    }
+}
+
+__device__ inline int32_t row_lower_bound(const int32_t *ej, int32_t lo, int32_t hi, int32_t j, bool upper)
+{
+    while (lo < hi)
+    {
+        const int32_t mid = (lo + hi) >> 1;
+        const int32_t v = ej[mid] & 0x7fffffff;
+        if (v < j || (upper && v == j))
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// One wave per i-cluster: the words j << 8 | 1 << k of its rows k = 0..7
+// merged in j order (stable in k) into merged[ptr[8c] ..), then the number
+// of distinct j.
+__global__ void k_ex_merge(const int32_t *ptr, const int32_t *ej, int32_t n, int32_t ncl, uint32_t *merged,
+                           int32_t *ucount)
+{
+    const int32_t lane = (int32_t)(threadIdx.x & 63u);
+    const int32_t c = (int32_t)((blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6);
+    if (c >= ncl)
+        return;
+    int32_t rb[CLUSTER + 1];
+#pragma unroll
+    for (int k = 0; k <= CLUSTER; ++k)
+        rb[k] = ptr[min(c * CLUSTER + k, n)];
+    const int32_t base = rb[0], end = rb[CLUSTER];
+    for (int32_t e = base + lane; e < end; e += 64)
+    {
+        int32_t k = 0;
+#pragma unroll
+        for (int q = 1; q < CLUSTER; ++q)
+            k = e >= rb[q] ? q : k;
+        const int32_t j = ej[e] & 0x7fffffff;
+        // rank: its place in its own row, plus the smaller words of the
+        // other rows (for rows before its own, the equal ones too)
+        int32_t r = e - rb[k];
+#pragma unroll
+        for (int q = 0; q < CLUSTER; ++q)
+            if (q != k)
+                r += row_lower_bound(ej, rb[q], rb[q + 1], j, q < k) - rb[q];
+        merged[base + r] = (uint32_t)j << 8 | 1u << k;
+    }
+    // the wave reads back what its own lanes stored (one CU)
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    int32_t cnt = 0;
+    for (int32_t t0 = base; t0 < end; t0 += 64)
+    {
+        const int32_t t = t0 + lane;
+        const bool first = t < end && (t == base || (merged[t - 1] >> 8) != (merged[t] >> 8));
+        cnt += (int32_t)__popcll(__ballot(first));
+    }
+    if (lane == 0)
+        ucount[c] = cnt;
+}
+
+// The distinct words of each cluster, masks of equal j or-ed, at ex_ptr[c].
+__global__ void k_ex_fill(const int32_t *ptr, int32_t n, int32_t ncl, const uint32_t *merged, const int32_t *ex_ptr,
+                          uint32_t *ex_w)
+{
+    const int32_t lane = (int32_t)(threadIdx.x & 63u);
+    const int32_t c = (int32_t)((blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6);
+    if (c >= ncl)
+        return;
+    const int32_t base = ptr[min(c * CLUSTER, n)], end = ptr[min(c * CLUSTER + CLUSTER, n)];
+    int32_t out = ex_ptr[c];
+    for (int32_t t0 = base; t0 < end; t0 += 64)
+    {
+        const int32_t t = t0 + lane;
+        const bool first = t < end && (t == base || (merged[t - 1] >> 8) != (merged[t] >> 8));
+        const uint64_t bal = __ballot(first);
+        if (first)
+        {
+            uint32_t wv = merged[t];
+            for (int32_t u = t + 1; u < end && (merged[u] >> 8) == (wv >> 8); ++u)
+                wv |= merged[u] & 0xFFu;
+            ex_w[out + (int32_t)__popcll(bal & ((1ull << lane) - 1ull))] = wv;
+        }
+        out += (int32_t)__popcll(bal);
+    }
+}
+
+// Force rows: a row's edges but those joining two rigid-body members.
+__global__ void k_edge_keep(const int32_t *ptr, int32_t n, const int32_t *ej, const float *ek, const float *el,
+                            const uint8_t *rigid, const int32_t *fptr, int32_t *cnt, int32_t *fej, float *fek,
+                            float *fel)
+{
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const bool ri = rigid[i] != 0;
+    int32_t o = fptr ? fptr[i] : 0, c = 0;
+    for (int32_t t = ptr[i]; t < ptr[i + 1]; ++t)
+    {
+        if (ri && rigid[ej[t] & 0x7fffffff])
+            continue;
+        if (fptr)
+        {
+            fej[o] = ej[t];
+            fek[o] = ek[t];
+            fel[o] = el[t];
+            ++o;
+        }
+        ++c;
+    }
+    if (!fptr)
+        cnt[i] = c;
+}
+
+// Row i of the constructed springs at offs[i]: id1 = i.
+__global__ void k_rows_expand(const int32_t *offs, int32_t n, uint32_t *ida)
+{
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    for (int32_t t = offs[i]; t < offs[i + 1]; ++t)
+        ida[t] = (uint32_t)i;
+}
+
 inline int32_t nblk(int64_t n, int32_t b = BLOCK) { return (int32_t)((n + b - 1) / b); }
 
 } // namespace
@@ -1666,4 +1856,54 @@ void launch_fill_i32(int32_t *p, int32_t v, int64_t n, const int32_t *gate, hipS
     }
 }
 
+void launch_edge_count(const uint32_t *sa, const uint32_t *sb, int64_t m, const int32_t *iperm, int32_t *cnt,
+                       hipStream_t s)
+{
+    if (m > 0)
+        k_edge_count<<<(int32_t)((m + BLOCK - 1) / BLOCK), BLOCK, 0, s>>>(sa, sb, m, iperm, cnt);
+}
+
+void launch_edge_fill(const uint32_t *sa, const uint32_t *sb, int64_t m, const int32_t *iperm, const int32_t *ptr,
+                      int32_t *cursor, uint64_t *key, hipStream_t s)
+{
+    if (m > 0)
+        k_edge_fill<<<(int32_t)((m + BLOCK - 1) / BLOCK), BLOCK, 0, s>>>(sa, sb, m, iperm, ptr, cursor, key);
+}
+
+void launch_edge_sort(const int32_t *ptr, int32_t n, const uint64_t *key, const float4 *jprm, const float *sl,
+                      const float *sk, int32_t *ej, float *ek, float *el, hipStream_t s)
+{
+    if (n > 0)
+        k_edge_sort<<<(n + EDGE_WAVES - 1) / EDGE_WAVES, BLOCK, 0, s>>>(ptr, n, key, jprm, sl, sk, ej, ek, el);
+}
+
+void launch_ex_merge(const int32_t *ptr, const int32_t *ej, int32_t n, int32_t ncl, uint32_t *merged,
+                     int32_t *ucount, hipStream_t s)
+{
+    if (ncl > 0)
+        k_ex_merge<<<(ncl + EDGE_WAVES - 1) / EDGE_WAVES, BLOCK, 0, s>>>(ptr, ej, n, ncl, merged, ucount);
+}
+
+void launch_ex_fill(const int32_t *ptr, int32_t n, int32_t ncl, const uint32_t *merged, const int32_t *ex_ptr,
+                    uint32_t *ex_w, hipStream_t s)
+{
+    if (ncl > 0)
+        k_ex_fill<<<(ncl + EDGE_WAVES - 1) / EDGE_WAVES, BLOCK, 0, s>>>(ptr, n, ncl, merged, ex_ptr, ex_w);
+}
+
+void launch_edge_keep(const int32_t *ptr, int32_t n, const int32_t *ej, const float *ek, const float *el,
+                      const uint8_t *rigid, const int32_t *fptr, int32_t *cnt, int32_t *fej, float *fek, float *fel,
+                      hipStream_t s)
+{
+    if (n > 0)
+        k_edge_keep<<<nblk(n), BLOCK, 0, s>>>(ptr, n, ej, ek, el, rigid, fptr, cnt, fej, fek, fel);
+}
+
+void launch_rows_expand(const int32_t *offs, int32_t n, uint32_t *ida, hipStream_t s)
+{
+    if (n > 0)
+        k_rows_expand<<<nblk(n), BLOCK, 0, s>>>(offs, n, ida);
+}
+
 } // namespace bs
+

@@ -931,6 +931,184 @@ void launch_unperm_f64(const double *in, const int32_t *perm, int32_t n, double 
 }
 
 // Springs from a cutoff, computed like the kernel (same grid walk and test).
+// Spring CSR, exclusions and construction expansion: the device kernels'
+// results computed serially, with every access checked.
+void launch_edge_count(const uint32_t *sa, const uint32_t *sb, int64_t m, const int32_t *iperm, int32_t *cnt,
+                       hipStream_t)
+{
+    need(sa, sizeof(uint32_t) * m, "edge_count: sa");
+    need(sb, sizeof(uint32_t) * m, "edge_count: sb");
+    for (int64_t s = 0; s < m; ++s)
+        for (uint32_t id : {sa[s], sb[s]})
+        {
+            need(iperm + id, 4, "edge_count: iperm");
+            need(cnt + iperm[id], 4, "edge_count: cnt");
+            ++cnt[iperm[id]];
+        }
+}
+
+void launch_edge_fill(const uint32_t *sa, const uint32_t *sb, int64_t m, const int32_t *iperm, const int32_t *ptr,
+                      int32_t *cursor, uint64_t *key, hipStream_t)
+{
+    need(sa, sizeof(uint32_t) * m, "edge_fill: sa");
+    need(sb, sizeof(uint32_t) * m, "edge_fill: sb");
+    for (int64_t s = 0; s < m; ++s)
+    {
+        const int32_t a = iperm[sa[s]], b = iperm[sb[s]];
+        for (int side = 0; side < 2; ++side)
+        {
+            const int32_t r = side ? b : a, o = side ? a : b;
+            need(ptr + r, 8, "edge_fill: ptr");
+            need(cursor + r, 4, "edge_fill: cursor");
+            const int32_t slot = ptr[r] + cursor[r]++;
+            if (slot >= ptr[r + 1])
+                hc_fail("edge_fill: row overflow");
+            need(key + slot, 8, "edge_fill: key");
+            key[slot] = (uint64_t)(uint32_t)o << 32 | ((uint32_t)s << 1 | (uint32_t)side);
+        }
+    }
+}
+
+void launch_edge_sort(const int32_t *ptr, int32_t n, const uint64_t *key, const float4 *jprm, const float *sl,
+                      const float *sk, int32_t *ej, float *ek, float *el, hipStream_t)
+{
+    need(ptr, sizeof(int32_t) * (n + 1), "edge_sort: ptr");
+    const int32_t E = n > 0 ? ptr[n] : 0;
+    need(key, sizeof(uint64_t) * E, "edge_sort: key");
+    need(ej, sizeof(int32_t) * E, "edge_sort: ej");
+    need(ek, sizeof(float) * E, "edge_sort: ek");
+    need(el, sizeof(float) * E, "edge_sort: el");
+    need(jprm, sizeof(float4) * n, "edge_sort: jprm");
+    for (int32_t i = 0; i < n; ++i)
+    {
+        std::vector<uint64_t> row(key + ptr[i], key + ptr[i + 1]);
+        std::sort(row.begin(), row.end());
+        for (size_t t = 0; t < row.size(); ++t)
+        {
+            if (t > 0 && row[t] == row[t - 1])
+                hc_fail("edge_sort: repeated key");
+            const int32_t j = (int32_t)(row[t] >> 32);
+            const uint32_t s = (uint32_t)row[t] >> 1;
+            if (j < 0 || j >= n)
+                hc_fail("edge_sort: partner out of range");
+            need(sl + s, 4, "edge_sort: rest length");
+            need(sk + s, 4, "edge_sort: stiffness");
+            const int32_t o = ptr[i] + (int32_t)t;
+            ej[o] = (fbits(jprm[j].w) & PF_STATIC) ? (int32_t)((uint32_t)j | 0x80000000u) : j;
+            ek[o] = sk[s];
+            el[o] = sl[s];
+        }
+    }
+}
+
+static std::vector<uint32_t> hc_cluster_words(const int32_t *ptr, const int32_t *ej, int32_t n, int32_t c)
+{
+    std::vector<uint32_t> w;
+    for (int32_t k = 0; k < CLUSTER && c * CLUSTER + k < n; ++k)
+    {
+        const int32_t i = c * CLUSTER + k;
+        for (int32_t t = ptr[i]; t < ptr[i + 1]; ++t)
+            w.push_back((uint32_t)(ej[t] & 0x7fffffff) << 8 | (1u << k));
+    }
+    std::sort(w.begin(), w.end());
+    return w;
+}
+
+void launch_ex_merge(const int32_t *ptr, const int32_t *ej, int32_t n, int32_t ncl, uint32_t *merged,
+                     int32_t *ucount, hipStream_t)
+{
+    need(ptr, sizeof(int32_t) * (n + 1), "ex_merge: ptr");
+    need(ej, sizeof(int32_t) * (n > 0 ? ptr[n] : 0), "ex_merge: ej");
+    need(merged, sizeof(uint32_t) * (n > 0 ? ptr[n] : 0), "ex_merge: merged");
+    need(ucount, sizeof(int32_t) * ncl, "ex_merge: ucount");
+    if (ncl != (n + CLUSTER - 1) / CLUSTER)
+        hc_fail("ex_merge: cluster count");
+    for (int32_t c = 0; c < ncl; ++c)
+    {
+        const std::vector<uint32_t> w = hc_cluster_words(ptr, ej, n, c);
+        const int32_t base = ptr[c * CLUSTER];
+        int32_t u = 0;
+        for (size_t t = 0; t < w.size(); ++t)
+        {
+            merged[base + (int32_t)t] = w[t];
+            u += (t == 0 || (w[t] >> 8) != (w[t - 1] >> 8)) ? 1 : 0;
+        }
+        ucount[c] = u;
+    }
+}
+
+void launch_ex_fill(const int32_t *ptr, int32_t n, int32_t ncl, const uint32_t *merged, const int32_t *ex_ptr,
+                    uint32_t *ex_w, hipStream_t)
+{
+    need(ex_ptr, sizeof(int32_t) * (ncl + 1), "ex_fill: ex_ptr");
+    need(ex_w, sizeof(uint32_t) * ex_ptr[ncl], "ex_fill: ex_w");
+    for (int32_t c = 0; c < ncl; ++c)
+    {
+        const int32_t base = ptr[std::min(c * CLUSTER, n)], end = ptr[std::min(c * CLUSTER + CLUSTER, n)];
+        int32_t o = ex_ptr[c];
+        for (int32_t t = base; t < end; ++t)
+        {
+            if (t > base && (merged[t] >> 8) == (merged[t - 1] >> 8))
+            {
+                ex_w[o - 1] |= merged[t] & 0xFFu;
+                continue;
+            }
+            if (o >= ex_ptr[c + 1])
+                hc_fail("ex_fill: more words than counted");
+            ex_w[o++] = merged[t];
+        }
+        if (o != ex_ptr[c + 1])
+            hc_fail("ex_fill: fewer words than counted");
+    }
+}
+
+void launch_edge_keep(const int32_t *ptr, int32_t n, const int32_t *ej, const float *ek, const float *el,
+                      const uint8_t *rigid, const int32_t *fptr, int32_t *cnt, int32_t *fej, float *fek, float *fel,
+                      hipStream_t)
+{
+    need(ptr, sizeof(int32_t) * (n + 1), "edge_keep: ptr");
+    need(rigid, (size_t)n, "edge_keep: rigid");
+    if (fptr)
+    {
+        need(fptr, sizeof(int32_t) * (n + 1), "edge_keep: fptr");
+        need(fej, sizeof(int32_t) * fptr[n], "edge_keep: fej");
+        need(fek, sizeof(float) * fptr[n], "edge_keep: fek");
+        need(fel, sizeof(float) * fptr[n], "edge_keep: fel");
+    }
+    else
+        need(cnt, sizeof(int32_t) * n, "edge_keep: cnt");
+    for (int32_t i = 0; i < n; ++i)
+    {
+        int32_t o = fptr ? fptr[i] : 0, c = 0;
+        for (int32_t t = ptr[i]; t < ptr[i + 1]; ++t)
+        {
+            if (rigid[i] && rigid[ej[t] & 0x7fffffff])
+                continue;
+            if (fptr)
+            {
+                fej[o] = ej[t];
+                fek[o] = ek[t];
+                fel[o] = el[t];
+                ++o;
+            }
+            ++c;
+        }
+        if (fptr && o != fptr[i + 1])
+            hc_fail("edge_keep: row length differs from its count");
+        if (!fptr)
+            cnt[i] = c;
+    }
+}
+
+void launch_rows_expand(const int32_t *offs, int32_t n, uint32_t *ida, hipStream_t)
+{
+    need(offs, sizeof(int32_t) * (n + 1), "rows_expand: offs");
+    need(ida, sizeof(uint32_t) * (n > 0 ? offs[n] : 0), "rows_expand: ida");
+    for (int32_t i = 0; i < n; ++i)
+        for (int32_t t = offs[i]; t < offs[i + 1]; ++t)
+            ida[t] = (uint32_t)i;
+}
+
 void launch_topo_springs(const TopoArgs &a, hipStream_t)
 {
     const int32_t n = a.n;

@@ -152,6 +152,55 @@ def test_constructed_network_steps_like_the_oracle():
     net.close()
 
 
+def test_device_table_steps_like_an_uploaded_one():
+    """Construction leaves the spring table on the device and the setup builds
+    the spring rows and exclusions from it there; the same table given from
+    the host (bs_set_springs) steps to the same bits, with static beads and
+    rigid bodies (whose internal springs leave the force rows), and after
+    loaded springs plus a construction."""
+    s = workloads.generate(0, seed=23, n=4000)
+    fl = s.flags.copy()
+    fl[::11] |= _abi.FLAG_STATIC
+    p = s.params.copy()
+    p.rigidbody_enable = 1
+    bodies = (np.arange(100, 400), np.arange(2000, 2300))
+
+    def make():
+        net = SpringNetwork(p)
+        net.setParticles(s.x, s.v, s.mass, s.charge, s.radius, s.eps, s.sasa, s.transfer, s.hydro, fl)
+        return net
+
+    a = make()
+    m = a.springsFromCutoff(11.0, 0.7)
+    for g in bodies:
+        a.addRigidBody(g)
+    a.run(12)
+    ids = a.getSprings()
+    assert len(ids[0]) == m
+    b = make()
+    b.setSprings(*ids)
+    for g in bodies:
+        b.addRigidBody(g)
+    b.run(12)
+    for u, v in zip(a.getState(), b.getState()):
+        assert np.array_equal(u, v)
+    assert np.array_equal(a.computeForces()[0], b.computeForces()[0])
+    # loaded springs kept ahead of constructed ones, then stepped
+    c = make()
+    c.setSprings(s.id1[:500], s.id2[:500], s.l0[:500], s.k[:500])
+    mc = c.springsFromCutoff(9.0)
+    got = c.getSprings()
+    assert len(got[0]) == 500 + mc
+    assert np.array_equal(got[0][:500], s.id1[:500]) and np.array_equal(got[3][:500], s.k[:500])
+    c.run(3)
+    d = make()
+    d.setSprings(*got)
+    d.run(3)
+    assert np.array_equal(c.getState()[0], d.getState()[0])
+    for net in (a, b, c, d):
+        net.close()
+
+
 def test_systems_stay_apart():
     rng = np.random.default_rng(10)
     xa = rng.uniform(-30, 30, (6000, 3)).astype(np.float32)
