@@ -451,7 +451,7 @@ def main():
 def bench_construction(s, cutoff=8.0, reps=3, cpu=True, cpu_sample=5000):
     """Spring network from a cutoff (Topology::add_springs_from_cutoff) over
     the workload's own positions: the GPU build (wall time of
-    bs_springs_from_cutoff, which ends with the table on the host, and HIP-event
+    bs_springs_from_cutoff, which leaves the table on the device, and HIP-event
     times of its two passes), checked spring for spring against the
     generator's host build; beside it the reference's own Topology on a
     bounded sample of the same density."""
@@ -488,7 +488,8 @@ def bench_construction(s, cutoff=8.0, reps=3, cpu=True, cpu_sample=5000):
            "roofline": {"bound": "hbm", "achieved": b_alg / (kern_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": b_alg / (kern_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "alg_bytes": b_alg,
                         "note": "27-cell candidate gathers are served from L2 and not counted"},
-           "wall_note": "wall time includes the download of the spring table to the host"}
+           "wall_note": "wall time of the call: cell grid, both passes and the table written on the device, where "
+                        "bs_setup reads it; the host copy is downloaded only when asked for (bs_get_springs)"}
     if cpu:
         from oracle import pyoracle as po
         if po.ref_available():

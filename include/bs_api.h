@@ -198,8 +198,10 @@ const char *bs_last_error(const bs_handle *h);
  * them first.  New springs follow the loaded ones, ordered by (first,
  * second) original id, first < second; the reference orders them by first
  * id, then by its cell scan.  With independent systems declared
- * (bs_set_systems), no spring joins two systems.  *count_out (nullable) =
- * number of springs added. */
+ * (bs_set_systems), no spring joins two systems.  The table stays on the
+ * device, where bs_setup builds the spring rows from it; a host copy is made
+ * only when asked for (bs_get_springs, bs_save_netcdf, PDB CONECT records).
+ * *count_out (nullable) = number of springs added. */
 int bs_springs_from_cutoff(bs_handle *h, double cutoff, float stiffness, int64_t *count_out);
 
 /* Replaces Topology::add_springs_between_topologies_from_cutoff
