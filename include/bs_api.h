@@ -576,7 +576,7 @@ int bs_adjoint_backward(bs_handle *h, const double *dl_dx3, const double *dl_dv3
 
 /* bs_adjoint_backward that allocates nothing: the same arguments, results
  * and errors, with the sweep's device buffers (about 192 bytes per particle
- * plus 48 per spring when dk or dl0 is wanted) taken from a fixed 64 MiB
+ * plus 40 per spring when dk or dl0 is wanted) taken from a fixed 64 MiB
  * array that the library holds on each GPU until the process ends.  Larger
  * systems are refused with BS_ERR_ARG and a message giving the size needed.
  * Calls on one GPU share that array: calls that overlap on one GPU race.
