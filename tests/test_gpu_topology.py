@@ -201,6 +201,40 @@ def test_device_table_steps_like_an_uploaded_one():
         net.close()
 
 
+def test_dense_rows_match_fp64_oracle():
+    """Rows longer than the spring-row sort's LDS stage (1,024 keys) and
+    clusters whose exclusions merge thousands of words: a 2,000-bead sphere
+    linked within 30 A (up to 1,999 springs a bead), then stretched by
+    0.3 A (rms).  Forces and energies against the FP64 oracle on the same
+    springs."""
+    s = workloads.generate(0, seed=24, n=2000)
+    p = s.params.copy()
+    p.viscosity_enable = 0
+    net = SpringNetwork(p)
+    net.setParticles(*s.particle_arrays())
+    m = net.springsFromCutoff(30.0, 0.05)
+    xs = (s.x.reshape(-1, 3) + np.random.default_rng(5).normal(0, 0.3, (s.n, 3))).astype(np.float32)
+    net.setState(xs)
+    a, b, l0, k = net.getSprings()
+    rows = np.bincount(np.concatenate([a, b]), minlength=s.n)
+    _log(f"dense network: {m} springs, longest row {rows.max()}")
+    assert rows.max() > 1024
+    t = workloads.System(x=s.x, v=s.v, mass=s.mass, charge=s.charge, radius=s.radius, eps=s.eps, sasa=s.sasa,
+                         transfer=s.transfer, hydro=s.hydro, flags=s.flags, id1=a, id2=b, l0=l0, k=k)
+    t.params = p
+    F, E = net.computeForces()
+    o = po.Oracle(t, params=p)
+    o.set_state(xs, None)
+    Fd, Ed = o.compute_forces(po.F64)
+    o.close()
+    err = np.linalg.norm(F - Fd, axis=1).max() / np.linalg.norm(Fd, axis=1).max()
+    _log(f"  force err {err:.2e}, spring energy {E[0]:.6g} vs {Ed[0]:.6g}")
+    assert err <= 1e-5
+    for c in range(3):
+        assert abs(E[c] - Ed[c]) <= 1e-5 * max(abs(Ed[c]), 1e-3)
+    net.close()
+
+
 def test_systems_stay_apart():
     rng = np.random.default_rng(10)
     xa = rng.uniform(-30, 30, (6000, 3)).astype(np.float32)
