@@ -712,16 +712,22 @@ void build_list(bs_handle *h, List &l, bool gated, int32_t cap_override)
         launch_rows(l.cl, l.cln, h->ncl, l.cap, l.rows, l.rlen, l.rcap, g, s);
 }
 
-// The gated list rebuild.  prepped: the gate was just set by a step's
-// prestep; otherwise it is set here from the drift flag alone.
+// The gate kernel: with lists, it also writes the box partials of the
+// current positions for the chain's cell binning.
+void launch_gate(bs_handle *h, int32_t interval, int32_t advance)
+{
+    const bool lists = !h->lists.empty();
+    launch_prestep(h->status, h->counters, interval, advance, lists ? h->pos[h->cur] : nullptr, h->shift,
+                   (int32_t)h->n, lists ? h->box_part : nullptr, h->box_blocks, h->stream);
+}
+
+// The gated list rebuild.  prepped: the gate (and the box partials) were
+// just set by a step's prestep; otherwise they are set here from the drift
+// flag alone.
 void rebuild_chain(bs_handle *h, bool gated, bool prepped = false)
 {
     if (!prepped)
-        launch_prestep(h->status, h->counters, 0, 0, h->stream);
-    if (h->lists.empty())
-        return;
-    const int32_t *g = gate_of(h, gated);
-    launch_box_partial(h->pos[h->cur], h->shift, (int32_t)h->n, h->box_part, h->box_blocks, g, h->stream);
+        launch_gate(h, 0, 0);
     for (auto &l : h->lists)
         build_list(h, l, gated, -1);
 }
@@ -1082,7 +1088,7 @@ int regrow_lists(bs_handle *h)
 void launch_step(bs_handle *h, bool energy)
 {
     const size_t e0 = h->prof ? prof_mark(h) : SIZE_MAX;
-    launch_prestep(h->status, h->counters, h->p.rebuild_interval, 1, h->stream);
+    launch_gate(h, h->p.rebuild_interval, 1);
     rebuild_chain(h, true, true);
     const size_t e1 = h->prof ? prof_mark(h) : SIZE_MAX;
     if (!h->lists.empty())

@@ -326,12 +326,8 @@ struct ListArgs
 // keeps independent systems of a batch apart on the engine's grid.
 void launch_box_reduce(const float4 *pos, const float4 *shift, int32_t n, double *partial, double *box,
                        int32_t nblocks, const int32_t *gate, hipStream_t s);
-// The first half of launch_box_reduce: nblocks (<= BLOCK) block partials,
-// reduced by launch_cell_ids (box_part).
-void launch_box_partial(const float4 *pos, const float4 *shift, int32_t n, double *partial, int32_t nblocks,
-                        const int32_t *gate, hipStream_t s);
-// box_part (nullable): the box is first reduced from box_nb partials and
-// stored in box; otherwise box is read.  xref (nullable): positions copied
+// box_part (nullable): the box is first reduced from box_nb partials (of
+// launch_prestep) and stored in box; otherwise box is read.  xref (nullable): positions copied
 // there as well.
 void launch_cell_ids(const float4 *pos, const float4 *shift, const float4 *jprm, uint32_t mask, int32_t n,
                      double *box, const double *box_part, int32_t box_nb, float4 *xref, int32_t dims0,
@@ -367,8 +363,11 @@ void launch_probe(const ProbeArgs &p, int32_t nparts, float dt, int32_t visc_ena
 // overflow, [3] longest list, [5] stop, [6] steps completed in the pass, [7]
 // drift flag (a rebuild at the next gate).  launch_prestep sets the gate from
 // the drift flag and, at the start of a step (advance), from the rebuild
-// interval; counters[0] counts steps, counters[1] rebuilds.
-void launch_prestep(int32_t *status, int32_t *counters, int32_t interval, int32_t advance, hipStream_t s);
+// interval; counters[0] counts steps, counters[1] rebuilds.  partial
+// (nullable): it also writes nblocks (<= BLOCK) box partials of pos (+ shift)
+// for launch_cell_ids, ungated.
+void launch_prestep(int32_t *status, int32_t *counters, int32_t interval, int32_t advance, const float4 *pos,
+                    const float4 *shift, int32_t n, double *partial, int32_t nblocks, hipStream_t s);
 void launch_rigid(const RigidArgs &a, hipStream_t s);
 void launch_adj_pair(const AdjPairArgs &a, int32_t steric_mode, int32_t elec_mode, int32_t nblocks, hipStream_t s);
 void launch_adj_spring(const AdjSpringArgs &a, int32_t group, int32_t nblocks, hipStream_t s);

@@ -17,11 +17,8 @@ namespace
 // ---------------------------------------------------------------------------
 // Bounding box, FP64 of float coordinates (box.hpp:37-69); min/max are exact.
 // ---------------------------------------------------------------------------
-__global__ void k_box_partial(const float4 *pos, const float4 *shift, int32_t n, double *partial,
-                              const int32_t *gate)
+__device__ void box_partial_block(const float4 *pos, const float4 *shift, int32_t n, double *partial)
 {
-    if (gated_off(gate))
-        return;
     __shared__ float sh[6][BLOCK / 64];
     float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
@@ -71,6 +68,14 @@ __global__ void k_box_partial(const float4 *pos, const float4 *shift, int32_t n,
     }
 }
 
+__global__ void __launch_bounds__(BLOCK) k_box_partial(const float4 *pos, const float4 *shift, int32_t n,
+                                                       double *partial, const int32_t *gate)
+{
+    if (gated_off(gate))
+        return;
+    box_partial_block(pos, shift, n, partial);
+}
+
 // Combines up to BLOCK block partials (one per thread) with wave shuffles
 // into out[6]; every thread of the (BLOCK-wide) block takes part.
 __device__ void block_box_final(const double *partial, int32_t nb, double *out)
@@ -116,7 +121,7 @@ __global__ void __launch_bounds__(BLOCK) k_box_final(const double *partial, int3
 // exact_mode 0: the engine's own grid, edge >= min_edge on every axis.
 // ---------------------------------------------------------------------------
 //
-// box_part (optional): the block partials of k_box_partial; each workgroup
+// box_part (optional): box block partials (k_prestep, k_box_partial); each workgroup
 // then reduces them itself (the first one also stores the box for later
 // kernels), which saves the launch of k_box_final.  xref (optional): the
 // positions are also copied there (the list's reference positions).
@@ -744,9 +749,10 @@ __global__ void __launch_bounds__(BLOCK) k_probe(ProbeArgs p, int32_t nparts, fl
 }
 
 // ---------------------------------------------------------------------------
-// Rigid bodies.  k_rigid_solve: one workgroup per body sums F and the torque
-// (x_i - X) x F_i over its dynamic particles (RigidBody.cpp:567-580) in a
-// fixed order, then one lane advances the body (RigidBody.cpp:430-484: linear
+// Rigid bodies.  k_rigid_solve: one workgroup of 1024 per body sums F and the
+// torque (x_i - X) x F_i over its dynamic particles (RigidBody.cpp:567-580)
+// in a fixed order (per lane, then wave shuffles, then the 16 waves in
+// order), then one lane advances the body (RigidBody.cpp:430-484: linear
 // Euler with the a dt^2/2 term, L += tau dt, omega = I^-1 L, rotation by the
 // quaternion of axis omega -- not normalised -- and angle |omega| dt).
 // k_rigid_place: x_i = X + q (p0_i, 0) q^-1 and v_i = (x_i - x_prev) / dt
@@ -768,11 +774,13 @@ __device__ inline void qnormalize(double q[4])
             q[c] /= m;
 }
 
-__global__ void __launch_bounds__(BLOCK) k_rigid_solve(RigidArgs a)
+constexpr int RIGID_BLOCK = 1024;
+
+__global__ void __launch_bounds__(RIGID_BLOCK) k_rigid_solve(RigidArgs a)
 {
     if (halted(a.status))
         return;
-    __shared__ double sh[BLOCK / 64];
+    __shared__ double sh[RIGID_BLOCK / 64][6];
     const int32_t b = blockIdx.x;
     RigidBody &B = a.bodies[b];
     const double X0 = B.X[0], X1 = B.X[1], X2 = B.X[2];
@@ -789,11 +797,18 @@ __global__ void __launch_bounds__(BLOCK) k_rigid_solve(RigidArgs a)
         acc[4] += r2 * f.x - r0 * f.z;
         acc[5] += r0 * f.y - r1 * f.x;
     }
-    double tot[6];
     for (int k = 0; k < 6; ++k)
-        tot[k] = block_sum(acc[k], sh);
+        acc[k] = wave_sum(acc[k]);
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 6; ++k)
+            sh[threadIdx.x >> 6][k] = acc[k];
+    __syncthreads();
     if (threadIdx.x != 0)
         return;
+    double tot[6] = {0, 0, 0, 0, 0, 0};
+    for (int w = 0; w < RIGID_BLOCK / 64; ++w)
+        for (int k = 0; k < 6; ++k)
+            tot[k] += sh[w][k];
     const double dt = a.dt;
     double acc3[3];
     for (int c = 0; c < 3; ++c)
@@ -1449,7 +1464,7 @@ __global__ void k_topo_sort_long(const int32_t *offs, int32_t n, int32_t *jout, 
 // rebuild counted (counters[1]).  advance: the start of a step, which also
 // stops the call after a non-finite position and counts the step
 // (counters[0]).
-__global__ void k_prestep(int32_t *status, int32_t *counters, int32_t interval, int32_t advance)
+__device__ void prestep_gate(int32_t *status, int32_t *counters, int32_t interval, int32_t advance)
 {
     if (halted(status))
         return;
@@ -1466,6 +1481,19 @@ __global__ void k_prestep(int32_t *status, int32_t *counters, int32_t interval, 
     counters[1] += r;
     if (advance)
         counters[0] = step + 1;
+}
+
+// The gate (first lane) and, with partial, the box partials of the current
+// positions, whether or not a rebuild follows: reading the positions costs
+// about what a second launch would.
+__global__ void __launch_bounds__(BLOCK) k_prestep(int32_t *status, int32_t *counters, int32_t interval,
+                                                   int32_t advance, const float4 *pos, const float4 *shift,
+                                                   int32_t n, double *partial)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        prestep_gate(status, counters, interval, advance);
+    if (partial)
+        box_partial_block(pos, shift, n, partial);
 }
 
 __global__ void k_fill_i32(int32_t *p, int32_t v, int64_t n, const int32_t *gate)
@@ -1679,12 +1707,6 @@ void launch_box_reduce(const float4 *pos, const float4 *shift, int32_t n, double
     k_box_final<<<1, BLOCK, 0, s>>>(partial, nblocks, box, gate);
 }
 
-void launch_box_partial(const float4 *pos, const float4 *shift, int32_t n, double *partial, int32_t nblocks,
-                        const int32_t *gate, hipStream_t s)
-{
-    k_box_partial<<<nblocks, BLOCK, 0, s>>>(pos, shift, n, partial, gate);
-}
-
 void launch_cell_ids(const float4 *pos, const float4 *shift, const float4 *jprm, uint32_t mask, int32_t n,
                      double *box, const double *box_part, int32_t box_nb, float4 *xref, int32_t d0, int32_t d1,
                      int32_t d2, double edge_exact, float min_edge, int32_t exact_mode, int32_t *cell_of,
@@ -1826,9 +1848,13 @@ void launch_probe(const ProbeArgs &p, int32_t nparts, float dt, int32_t visc_ena
     k_probe<<<1, BLOCK, 0, s>>>(p, nparts, dt, visc_enable, visc, c_kinetic, integrate);
 }
 
-void launch_prestep(int32_t *status, int32_t *counters, int32_t interval, int32_t advance, hipStream_t s)
+void launch_prestep(int32_t *status, int32_t *counters, int32_t interval, int32_t advance, const float4 *pos,
+                    const float4 *shift, int32_t n, double *partial, int32_t nblocks, hipStream_t s)
 {
-    k_prestep<<<1, 1, 0, s>>>(status, counters, interval, advance);
+    if (partial && n > 0)
+        k_prestep<<<nblocks, BLOCK, 0, s>>>(status, counters, interval, advance, pos, shift, n, partial);
+    else
+        k_prestep<<<1, BLOCK, 0, s>>>(status, counters, interval, advance, nullptr, nullptr, 0, nullptr);
 }
 
 void launch_rigid(const RigidArgs &a, hipStream_t s)
@@ -1836,7 +1862,7 @@ void launch_rigid(const RigidArgs &a, hipStream_t s)
     if (a.nb <= 0)
         return;
     if (a.solve)
-        k_rigid_solve<<<a.nb, BLOCK, 0, s>>>(a);
+        k_rigid_solve<<<a.nb, RIGID_BLOCK, 0, s>>>(a);
     if (a.nr > 0)
         k_rigid_place<<<nblk(a.nr), BLOCK, 0, s>>>(a);
 }

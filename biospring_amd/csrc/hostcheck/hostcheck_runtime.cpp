@@ -310,8 +310,8 @@ void launch_box_reduce(const float4 *pos, const float4 *shift, int32_t n, double
 
 // One partial per block, as the device writes them: the whole box in
 // partial 0, empty boxes in the others (their min/max change nothing).
-void launch_box_partial(const float4 *pos, const float4 *shift, int32_t n, double *partial, int32_t nblocks,
-                        const int32_t *gate, hipStream_t s)
+static void launch_box_partial(const float4 *pos, const float4 *shift, int32_t n, double *partial, int32_t nblocks,
+                               const int32_t *gate, hipStream_t s)
 {
     need(partial, sizeof(double) * 6 * nblocks, "box_partial: partial");
     if (nblocks < 1 || nblocks > 256)
@@ -1314,10 +1314,13 @@ void launch_centroid(const float4 *pos, int32_t n, double *partial, int32_t nblo
         out3[k] = n > 0 ? c[k] / n : 0.0;
 }
 
-void launch_prestep(int32_t *status, int32_t *counters, int32_t interval, int32_t advance, hipStream_t)
+void launch_prestep(int32_t *status, int32_t *counters, int32_t interval, int32_t advance, const float4 *pos,
+                    const float4 *shift, int32_t n, double *partial, int32_t nblocks, hipStream_t st)
 {
     need(status, 32, "prestep: status");
     need(counters, 8, "prestep: counters");
+    if (partial && n > 0)
+        launch_box_partial(pos, shift, n, partial, nblocks, nullptr, st);
     if (halted(status))
         return;
     if (advance && status[1] != 0x7fffffff)
