@@ -301,10 +301,12 @@ __global__ void __launch_bounds__(BLOCK) k_cluster_list(ListArgs a)
         const Deal d = deal_of(kept);
         const uint32_t *src = a.clt + (size_t)ci * a.cap;
         uint32_t *dst = a.cl + (size_t)ci * a.cap;
-        for (int32_t p = lane; p < kept; p += 64)
+        // chunk by chunk (no division): chunk c holds entries c, c + nch, ...
+        for (int32_t c = 0; c < d.nch; ++c)
         {
-            const int32_t c = p % d.nch, t = p / d.nch;
-            dst[c * d.q + min(c, d.r) + t] = src[p];
+            const int32_t base = c * d.q + min(c, d.r), size = d.q + (c < d.r ? 1 : 0);
+            for (int32_t t = lane; t < size; t += 64)
+                dst[base + t] = src[t * d.nch + c];
         }
     }
     // zero words up to the next multiple of 16, read unconditionally by the
