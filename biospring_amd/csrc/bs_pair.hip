@@ -182,30 +182,51 @@ __global__ void __launch_bounds__(BLOCK) k_cluster_list(ListArgs a)
         // order.  A three-stage pipeline runs over it: while batch b is
         // tested, batch b+1's positions and batch b+2's particle indices are
         // in flight, so a batch never waits for its own loads.
-        int32_t gz = clo[2], gy = clo[1], rt = 0, re = 0; // generator: next row, current row's rest
-        auto next_batch = [&](int32_t &bt, int32_t &be) {
-            while (rt >= re && gz <= chi[2])
-            { // the next row within the radius
-                const int32_t cz = gz, cy = gy;
-                if (++gy > chi[1])
-                {
-                    gy = clo[1];
-                    ++gz;
-                }
-                const double gzv = gap(2, cz);
-                if (gzv > rad)
-                    continue;
-                const double gyv = gap(1, cy);
+        // Row setup is lane-parallel: lane l takes row rbase + l of the
+        // (cy fastest, then cz) order, computes its x range and member run,
+        // and a ballot marks the rows with members; the generator then takes
+        // the marked rows in order, 64 rows per setup.
+        const int32_t ny = chi[1] - clo[1] + 1, nrows = ny * (chi[2] - clo[2] + 1);
+        int32_t rbase = 0, my_rt = 0, my_re = 0, rt = 0, re = 0; // generator: row setup, current row's rest
+        uint64_t rmask = 0ull;
+        auto setup_rows = [&]() {
+            const int32_t r = rbase + lane;
+            my_rt = my_re = 0;
+            if (r < nrows)
+            {
+                const int32_t cy = clo[1] + r % ny, cz = clo[2] + r / ny;
+                const double gzv = gap(2, cz), gyv = gap(1, cy);
                 const double r2x = rad * rad - gzv * gzv - gyv * gyv;
-                if (r2x < 0.0)
+                if (gzv <= rad && r2x >= 0.0)
+                {
+                    const double rx = sqrt(r2x);
+                    const double mn0 = a.box[0], len0 = a.box[3] - a.box[0];
+                    const int32_t cx0 =
+                        internal_coord((float)((double)bmin[0] - rx), mn0, len0, a.dims[0], a.cell_edge);
+                    const int32_t cx1 =
+                        internal_coord((float)((double)bmax[0] + rx), mn0, len0, a.dims[0], a.cell_edge);
+                    const int32_t row = cy * a.dims[0] + cz * a.dims[0] * a.dims[1];
+                    my_rt = a.cell_start[row + cx0];
+                    my_re = a.cell_start[row + cx1 + 1];
+                }
+            }
+            rmask = __ballot(my_rt < my_re);
+            rbase += 64;
+        };
+        auto next_batch = [&](int32_t &bt, int32_t &be) {
+            while (rt >= re)
+            { // the next row with members
+                if (rmask == 0ull)
+                {
+                    if (rbase >= nrows)
+                        break;
+                    setup_rows();
                     continue;
-                const double rx = sqrt(r2x);
-                const double mn0 = a.box[0], len0 = a.box[3] - a.box[0];
-                const int32_t cx0 = internal_coord((float)((double)bmin[0] - rx), mn0, len0, a.dims[0], a.cell_edge);
-                const int32_t cx1 = internal_coord((float)((double)bmax[0] + rx), mn0, len0, a.dims[0], a.cell_edge);
-                const int32_t row = cy * a.dims[0] + cz * a.dims[0] * a.dims[1];
-                rt = a.cell_start[row + cx0];
-                re = a.cell_start[row + cx1 + 1];
+                }
+                const int32_t f = __ffsll((long long)rmask) - 1;
+                rmask &= rmask - 1ull;
+                rt = __shfl(my_rt, f, 64);
+                re = __shfl(my_re, f, 64);
             }
             if (rt < re)
             {
