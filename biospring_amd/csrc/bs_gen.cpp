@@ -323,6 +323,7 @@ void bsg_params(int config, bs_params * p)
         p->steric_cutoff = 4.0f;
         p->imp_enable = 1;
         p->list_skin = 1.0f;
+        p->graph_steps = 16; // a short, launch-bound step: replayed from a graph
         break;
     default:
         break;
