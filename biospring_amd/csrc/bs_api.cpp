@@ -3413,6 +3413,9 @@ int bs_autotune(bs_handle *h, int32_t steps, int32_t *group_out, int32_t *graph_
     rc = bs_get_state(h, x.data(), v.data());
     if (rc)
         return rc;
+    rc = springs_host(h); // the trials copy the host spring table
+    if (rc)
+        return rc;
     // first the pair layout (at the current lanes and graph setting), then
     // lanes per spring row x graph replay with that layout
     struct Trial
