@@ -1594,15 +1594,44 @@ int adjoint_sweep(bs_handle *h, AdjBufs &b, double *dk, double *dl0, double *dq)
         // sweep, so that two sweeps of one record give the same bits.
         if (!recompute || c == ncp - 1)
             launch_fill_i32(h->status + 7, 1, 1, nullptr, st);
+        // After the first reversed state, one test of all the segment's other
+        // states against the lists' reference positions: when none drifted,
+        // no rebuild would happen, so their drift tests and gated rebuild
+        // launches are skipped (same lists, same bits) and the kernels read
+        // the stored positions in place.
+        bool still = false;
+        struct PosRestore
+        {
+            bs_handle *h;
+            int cur;
+            float4 *p;
+            ~PosRestore() { h->pos[cur] = p; }
+        } restore{h, h->cur, h->pos[h->cur]};
         for (int32_t s = s1 - 1; s >= s0; --s)
         {
-            HIPCHK(h, hipMemcpyAsync(h->pos[h->cur], tp.seg + (size_t)(s - s0) * n, s4, hipMemcpyDeviceToDevice, st));
+            float4 *xs = tp.seg + (size_t)(s - s0) * n;
+            if (still)
+                h->pos[h->cur] = xs;
+            else
+                HIPCHK(h, hipMemcpyAsync(restore.p, xs, s4, hipMemcpyDeviceToDevice, st));
             rc = prof_fold(h);
             if (rc)
                 return rc;
             const size_t e0 = h->prof ? prof_mark(h) : SIZE_MAX;
-            launch_drift_check(h->pos[h->cur], h->xref, (int32_t)n, half_skin2, h->status, st);
-            rebuild_chain(h, true);
+            if (!still)
+            {
+                launch_drift_check(h->pos[h->cur], h->xref, (int32_t)n, half_skin2, h->status, st);
+                rebuild_chain(h, true);
+            }
+            if (s == s1 - 1 && s > s0 && !h->lists.empty())
+            {
+                launch_fill_i32(h->status + 4, 0, 1, nullptr, st);
+                launch_drift_states(tp.seg, s - s0, (int32_t)n, h->xref, half_skin2, h->status + 4, st);
+                int32_t moved = 1;
+                HIPCHK(h, hipMemcpyAsync(&moved, h->status + 4, sizeof moved, hipMemcpyDeviceToHost, st));
+                HIPCHK(h, hipStreamSynchronize(st));
+                still = moved == 0;
+            }
             const size_t e1 = h->prof ? prof_mark(h) : SIZE_MAX;
             if (hwp)
                 launch_adj_pairs(h, wb[wc], hwp, dqd);

@@ -360,7 +360,8 @@ void launch_probe(const ProbeArgs &p, int32_t nparts, float dt, int32_t visc_ena
                   int32_t integrate, hipStream_t s);
 // Status words: [0] rebuild gate of the current step's list kernels, [1]
 // lowest particle with a non-finite position (INT_MAX: none), [2] list
-// overflow, [3] longest list, [5] stop, [6] steps completed in the pass, [7]
+// overflow, [3] longest list, [4] the adjoint sweep's drift test of a
+// segment's stored states, [5] stop, [6] steps completed in the pass, [7]
 // drift flag (a rebuild at the next gate).  launch_prestep sets the gate from
 // the drift flag and, at the start of a step (advance), from the rebuild
 // interval; counters[0] counts steps, counters[1] rebuilds.  partial
@@ -417,6 +418,10 @@ void launch_edge_keep(const int32_t *ptr, int32_t n, const int32_t *ej, const fl
 void launch_rows_expand(const int32_t *offs, int32_t n, uint32_t *ida, hipStream_t s);
 void launch_pack_positions(const float4 *pos, const float4 *plo, const int32_t *perm, int32_t n, float *out3,
                            hipStream_t s);
+// flag = 1 when a particle of any of nstates stored states (states[k * n + i])
+// is more than half the list skin from xref.
+void launch_drift_states(const float4 *states, int32_t nstates, int32_t n, const float4 *xref, float half_skin2,
+                         int32_t *flag, hipStream_t s);
 // status[7] (the drift flag) = 1 when any particle is more than half the list skin from xref.
 void launch_drift_check(const float4 *pos, const float4 *xref, int32_t n, float half_skin2, int32_t *status,
                         hipStream_t s);

@@ -1074,6 +1074,27 @@ __global__ void k_drift_check(const float4 *pos, const float4 *xref, int32_t n, 
         atomicOr(&status[7], 1);
 }
 
+// Any of nstates stored states (states[k * n + i]) more than half the list
+// skin from the lists' reference positions: flag = 1.
+__global__ void k_drift_states(const float4 *states, int32_t nstates, int32_t n, const float4 *xref,
+                               float half_skin2, int32_t *flag)
+{
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool far = false;
+    if (i < n)
+    {
+        const float4 r = xref[i];
+        for (int32_t k = blockIdx.y; k < nstates; k += gridDim.y)
+        {
+            const float4 p = states[(size_t)k * n + i];
+            const float ex = p.x - r.x, ey = p.y - r.y, ez = p.z - r.z;
+            far |= !(ex * ex + ey * ey + ez * ez <= half_skin2);
+        }
+    }
+    if (__ballot(far) != 0ull && (threadIdx.x & 63) == 0)
+        atomicOr(flag, 1);
+}
+
 template <int G>
 __global__ void __launch_bounds__(BLOCK) k_adj_spring(AdjSpringArgs a)
 {
@@ -1825,6 +1846,14 @@ void launch_pack_positions(const float4 *pos, const float4 *plo, const int32_t *
 {
     if (n > 0)
         k_pack_positions<<<nblk(n), BLOCK, 0, s>>>(pos, plo, perm, n, out3);
+}
+
+void launch_drift_states(const float4 *states, int32_t nstates, int32_t n, const float4 *xref, float half_skin2,
+                         int32_t *flag, hipStream_t s)
+{
+    if (n > 0 && nstates > 0)
+        k_drift_states<<<dim3(nblk(n), nstates < 64 ? nstates : 64), BLOCK, 0, s>>>(states, nstates, n, xref,
+                                                                                     half_skin2, flag);
 }
 
 void launch_drift_check(const float4 *pos, const float4 *xref, int32_t n, float half_skin2, int32_t *status,

@@ -1218,6 +1218,22 @@ void launch_pack_positions(const float4 *pos, const float4 *plo, const int32_t *
     }
 }
 
+void launch_drift_states(const float4 *states, int32_t nstates, int32_t n, const float4 *xref, float half_skin2,
+                         int32_t *flag, hipStream_t)
+{
+    need(states, sizeof(float4) * (size_t)n * (size_t)std::max(nstates, 0), "drift_states: states");
+    need(xref, sizeof(float4) * n, "drift_states: xref");
+    need(flag, sizeof(int32_t), "drift_states: flag");
+    for (int32_t k = 0; k < nstates; ++k)
+        for (int32_t i = 0; i < n; ++i)
+        {
+            const float4 p = states[(size_t)k * n + i], r = xref[i];
+            const float ex = p.x - r.x, ey = p.y - r.y, ez = p.z - r.z;
+            if (!(ex * ex + ey * ey + ez * ez <= half_skin2))
+                *flag |= 1;
+        }
+}
+
 void launch_drift_check(const float4 *pos, const float4 *xref, int32_t n, float, int32_t *status, hipStream_t)
 {
     need(pos, sizeof(float4) * n, "drift_check: pos");
