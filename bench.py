@@ -304,13 +304,17 @@ def main():
     spring_ms = prof["spring"]["ms"] / max(prof["spring"]["launches"], 1)
     pair_ms = prof["pair"]["ms"] / max(prof["pair"]["launches"], 1)
     rebuild_ms = prof["rebuild"]["ms"] / max(prof["rebuild"]["launches"], 1)
-    b_alg = 4 * (n + 1) + 12 * edges + 64 * n
+    # {j, k, L0} per directed edge; {j, L0} when every spring has one
+    # stiffness (the engine then reads no per-edge k)
+    k_uniform = bool(s.m > 0 and np.all(s.k.view(np.uint32) == s.k.view(np.uint32)[0]))
+    b_alg = 4 * (n + 1) + (8 if k_uniform else 12) * edges + 64 * n
     f_alg = PAIR_FLOP * pairs_unique
     spring_gbs = b_alg / (spring_ms * 1e-3) / 1e9
     pair_tflops = f_alg / (pair_ms * 1e-3) / 1e12
     kernels = {
         "spring_integrate": {"avg_ms": spring_ms, "bound": "hbm", "achieved": spring_gbs, "peak": HBM_PEAK_GBS,
-                             "unit": "GB/s", "frac": spring_gbs / HBM_PEAK_GBS, "alg_bytes": b_alg},
+                             "unit": "GB/s", "frac": spring_gbs / HBM_PEAK_GBS, "alg_bytes": b_alg,
+                             "uniform_stiffness": k_uniform},
         "pair": {"avg_ms": pair_ms, "bound": "valu", "achieved": pair_tflops, "peak": FP32_PEAK_TFLOPS,
                  "unit": "TFLOP/s", "frac": pair_tflops / FP32_PEAK_TFLOPS, "alg_flops": f_alg,
                  "note": "compute-bound FP32/FP64 vector ALU work priced against the FP32 peak"},
@@ -643,8 +647,11 @@ def adjoint_leg(seed=1, n=0, T=200, K=20, warmup=5, fp64=True, cpu=True, check_n
     ap_ms = prof["adj_pair"]["ms"] / max(prof["adj_pair"]["launches"], 1)
     as_ms = prof["adj_spring"]["ms"] / max(prof["adj_spring"]["launches"], 1)
     f_alg = ADJ_PAIR_FLOP * pairs_unique
-    owned = int(s.m)  # springs with a dynamic end: one read-modify-write of dk, dL0 each per step
-    b_alg = 4 * (n + 1) + 12 * edges + 32 * owned + 280 * n
+    # springs with a dynamic end: one read-modify-write of their FP32 dk, dL0
+    # sums per step (16 B); {j, k, L0} per edge, {j, L0} with one stiffness
+    owned = int(s.m)
+    k_uniform = bool(s.m > 0 and np.all(s.k.view(np.uint32) == s.k.view(np.uint32)[0]))
+    b_alg = 4 * (n + 1) + (8 if k_uniform else 12) * edges + 16 * owned + 280 * n
     kernels = {
         "adj_pair": {"avg_ms": ap_ms, "bound": "valu", "achieved": f_alg / (ap_ms * 1e-3) / 1e12,
                      "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "alg_flops": f_alg,

@@ -144,6 +144,8 @@ struct bs_handle
     double4 *fnb = nullptr;
     int32_t *sp_ptr = nullptr, *sp_j = nullptr, *perm_d = nullptr;
     float *sp_k = nullptr, *sp_l0 = nullptr;
+    bool k_uniform = false; // every spring has the stiffness k_uni: the kernels read no per-edge k
+    float k_uni = 0.0f;
     int64_t nedges = 0;
     int32_t ncl = 0;                                  // i-clusters of 8
     // bs_adjoint_forward's record: checkpoints, the last segment's positions
@@ -834,7 +836,8 @@ void launch_springs(bs_handle *h, bool integrate, bool energy)
     a.xref = h->xref;
     a.sp_ptr = h->sp_ptr;
     a.sp_j = h->sp_j;
-    a.sp_k = h->sp_k;
+    a.sp_k = h->k_uniform ? nullptr : h->sp_k;
+    a.k_uni = h->k_uni;
     a.sp_l0 = h->sp_l0;
     a.perm = h->perm_d;
     a.fout = h->fout;
@@ -1642,7 +1645,8 @@ int adjoint_sweep(bs_handle *h, AdjBufs &b, double *dk, double *dl0, double *dq)
             a.iprm = h->iprm;
             a.sp_ptr = h->sp_ptr;
             a.sp_j = h->sp_j;
-            a.sp_k = h->sp_k;
+            a.sp_k = h->k_uniform ? nullptr : h->sp_k;
+            a.k_uni = h->k_uni;
             a.sp_l0 = h->sp_l0;
             a.w = wb[wc];
             a.hwp = hwp;
@@ -1890,6 +1894,21 @@ int do_setup(bs_handle *h)
         ALLOC(h->sp_l0, float, ekeep);
         launch_edge_keep(fptr, (int32_t)n, fej, fek, fel, rg_d, h->sp_ptr, nullptr, h->sp_j, h->sp_k, h->sp_l0, st);
         h->nedges = ekeep;
+    }
+    // one stiffness for every spring (a network built from a cutoff, an
+    // elastic network model): the spring kernels read no per-edge k
+    h->k_uniform = false;
+    if (m > 0)
+    {
+        int32_t *diff = S.get<int32_t>(1, e);
+        HIPCHK(h, e);
+        launch_fill_i32(diff, 0, 1, nullptr, st);
+        launch_differs_f32(h->dsk, m, diff, st);
+        int32_t dv = 1;
+        HIPCHK(h, hipMemcpyAsync(&dv, diff, sizeof dv, hipMemcpyDeviceToHost, st));
+        HIPCHK(h, hipMemcpyAsync(&h->k_uni, h->dsk, sizeof(float), hipMemcpyDeviceToHost, st));
+        HIPCHK(h, hipStreamSynchronize(st));
+        h->k_uniform = dv == 0;
     }
     HIPCHK(h, hipStreamSynchronize(st)); // the scratch is freed on return
     rc = upload_positions(h);

@@ -204,8 +204,9 @@ struct SpringArgs
     const float4 *xref;
     const int32_t *sp_ptr;
     const int32_t *sp_j;
-    const float *sp_k;
+    const float *sp_k;   // null: every spring has stiffness k_uni (not read per edge)
     const float *sp_l0;
+    float k_uni;
     const int32_t *perm;
     float4 *fout;        // forces-only mode
     double4 *frig;       // integrate mode: total force of rigid particles (not integrated here)
@@ -258,7 +259,8 @@ struct AdjSpringArgs
     const float4 *iprm;
     const int32_t *sp_ptr;
     const int32_t *sp_j;
-    const float *sp_k;
+    const float *sp_k;    // null: every spring has stiffness k_uni
+    float k_uni;
     const float *sp_l0;
     const float4 *w;      // [N] weights of this step (FP32)
     const double4 *hwp;   // nullable: pair part of H w
@@ -418,6 +420,8 @@ void launch_edge_keep(const int32_t *ptr, int32_t n, const int32_t *ej, const fl
 void launch_rows_expand(const int32_t *offs, int32_t n, uint32_t *ida, hipStream_t s);
 void launch_pack_positions(const float4 *pos, const float4 *plo, const int32_t *perm, int32_t n, float *out3,
                            hipStream_t s);
+// flag = 1 when some v[s] differs from v[0] (bitwise), s < m.
+void launch_differs_f32(const float *v, int64_t m, int32_t *flag, hipStream_t s);
 // flag = 1 when a particle of any of nstates stored states (states[k * n + i])
 // is more than half the list skin from xref.
 void launch_drift_states(const float4 *states, int32_t nstates, int32_t n, const float4 *xref, float half_skin2,

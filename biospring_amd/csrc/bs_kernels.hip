@@ -506,7 +506,7 @@ __global__ void __launch_bounds__(BLOCK) k_spring(SpringArgs a)
             const int32_t e = beg + lane + u * G;
             const bool ok = e < end;
             jr[u] = ok ? a.sp_j[e] : 0;
-            kk[u] = ok ? a.sp_k[e] : 0.0f;
+            kk[u] = ok ? (a.sp_k ? a.sp_k[e] : a.k_uni) : 0.0f;
             ll[u] = ok ? a.sp_l0[e] : 0.0f;
         }
         for (int32_t e0 = beg + lane; e0 < end; e0 += U * G)
@@ -523,7 +523,7 @@ __global__ void __launch_bounds__(BLOCK) k_spring(SpringArgs a)
                 const int32_t e = e0 + (U + u) * G;
                 const bool ok = e < end;
                 jn[u] = ok ? a.sp_j[e] : 0;
-                kn[u] = ok ? a.sp_k[e] : 0.0f;
+                kn[u] = ok ? (a.sp_k ? a.sp_k[e] : a.k_uni) : 0.0f;
                 ln[u] = ok ? a.sp_l0[e] : 0.0f;
             }
 #pragma unroll
@@ -1074,6 +1074,16 @@ __global__ void k_drift_check(const float4 *pos, const float4 *xref, int32_t n, 
         atomicOr(&status[7], 1);
 }
 
+__global__ void k_differs_f32(const float *v, int64_t m, int32_t *flag)
+{
+    const uint32_t v0 = fbits(v[0]);
+    bool d = false;
+    for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < m; s += (int64_t)gridDim.x * blockDim.x)
+        d |= fbits(v[s]) != v0;
+    if (__ballot(d) != 0ull && (threadIdx.x & 63) == 0)
+        atomicOr(flag, 1);
+}
+
 // Any of nstates stored states (states[k * n + i]) more than half the list
 // skin from the lists' reference positions: flag = 1.
 __global__ void k_drift_states(const float4 *states, int32_t nstates, int32_t n, const float4 *xref,
@@ -1134,7 +1144,7 @@ __global__ void __launch_bounds__(BLOCK) k_adj_spring(AdjSpringArgs a)
             const int32_t e = beg + lane + u * G;
             const bool ok = e < end;
             jr[u] = ok ? a.sp_j[e] : row;
-            kk[u] = ok ? a.sp_k[e] : 0.0f;
+            kk[u] = ok ? (a.sp_k ? a.sp_k[e] : a.k_uni) : 0.0f;
             ll[u] = ok ? a.sp_l0[e] : 0.0f;
         }
         for (int32_t e0 = beg + lane; e0 < end; e0 += U * G)
@@ -1154,7 +1164,7 @@ __global__ void __launch_bounds__(BLOCK) k_adj_spring(AdjSpringArgs a)
                 const int32_t e = e0 + (U + u) * G;
                 const bool ok = e < end;
                 jn[u] = ok ? a.sp_j[e] : row;
-                kn[u] = ok ? a.sp_k[e] : 0.0f;
+                kn[u] = ok ? (a.sp_k ? a.sp_k[e] : a.k_uni) : 0.0f;
                 ln[u] = ok ? a.sp_l0[e] : 0.0f;
             }
 #pragma unroll
@@ -1846,6 +1856,15 @@ void launch_pack_positions(const float4 *pos, const float4 *plo, const int32_t *
 {
     if (n > 0)
         k_pack_positions<<<nblk(n), BLOCK, 0, s>>>(pos, plo, perm, n, out3);
+}
+
+void launch_differs_f32(const float *v, int64_t m, int32_t *flag, hipStream_t s)
+{
+    if (m > 0)
+    {
+        const int64_t b = (m + BLOCK - 1) / BLOCK;
+        k_differs_f32<<<(int32_t)(b < 2048 ? b : 2048), BLOCK, 0, s>>>(v, m, flag);
+    }
 }
 
 void launch_drift_states(const float4 *states, int32_t nstates, int32_t n, const float4 *xref, float half_skin2,

@@ -719,7 +719,8 @@ void launch_spring(const SpringArgs &a, int32_t group, int32_t nb, hipStream_t)
     if (ne > 0)
     {
         need(a.sp_j, sizeof(int32_t) * ne, "spring: sp_j");
-        need(a.sp_k, sizeof(float) * ne, "spring: sp_k");
+        if (a.sp_k)
+            need(a.sp_k, sizeof(float) * ne, "spring: sp_k");
         need(a.sp_l0, sizeof(float) * ne, "spring: sp_l0");
     }
     for (int32_t e = 0; e < ne; ++e)
@@ -796,7 +797,8 @@ void launch_adj_spring(const AdjSpringArgs &a, int32_t group, int32_t nb, hipStr
     if (ne > 0)
     {
         need(a.sp_j, sizeof(int32_t) * ne, "adj_spring: sp_j");
-        need(a.sp_k, sizeof(float) * ne, "adj_spring: sp_k");
+        if (a.sp_k)
+            need(a.sp_k, sizeof(float) * ne, "adj_spring: sp_k");
         need(a.sp_l0, sizeof(float) * ne, "adj_spring: sp_l0");
     }
     if (a.dk)
@@ -1216,6 +1218,15 @@ void launch_pack_positions(const float4 *pos, const float4 *plo, const int32_t *
         out3[3 * i + 1] = p.y;
         out3[3 * i + 2] = p.z;
     }
+}
+
+void launch_differs_f32(const float *v, int64_t m, int32_t *flag, hipStream_t)
+{
+    need(v, sizeof(float) * m, "differs: v");
+    need(flag, sizeof(int32_t), "differs: flag");
+    for (int64_t s = 1; s < m; ++s)
+        if (std::memcmp(&v[s], &v[0], 4) != 0)
+            *flag |= 1;
 }
 
 void launch_drift_states(const float4 *states, int32_t nstates, int32_t n, const float4 *xref, float half_skin2,
