@@ -156,10 +156,13 @@ struct bs_handle
         int32_t T = 0, K = 0, ncp = 0;
         int32_t seg_of = -1; // the segment whose positions seg holds
         int ext = 0;
-        int64_t cap_cp = 0, cap_seg = 0;
+        int64_t cap_cp = 0, cap_seg = 0, cap_log = 0;
         float4 *cp_pos = nullptr, *cp_vel = nullptr, *seg = nullptr, *fin_pos = nullptr, *fin_vel = nullptr;
         int32_t *cp_cnt = nullptr, *fin_cnt = nullptr;
+        int32_t *gate_log = nullptr;  // [T] the rebuild gate of each recorded step
+        std::vector<char> rebuilt;    // its host copy: a recomputation launches the list rebuild there only
     } tape;
+    int32_t *gate_log = nullptr; // while recording: where the next step's gate is logged
     // bs_adjoint: springs with a dynamic end in the order of their owning CSR
     // edge ("slots"; the owning edge is in the dynamic end's row, or in the
     // lower index's when both ends move): slot_ptr[i] = first slot of row i,
@@ -310,7 +313,7 @@ void traj_release(bs_handle *h)
 void tape_free(bs_handle::Tape &tp)
 {
     for (void *q : {(void *)tp.cp_pos, (void *)tp.cp_vel, (void *)tp.seg, (void *)tp.fin_pos, (void *)tp.fin_vel,
-                    (void *)tp.cp_cnt, (void *)tp.fin_cnt})
+                    (void *)tp.cp_cnt, (void *)tp.fin_cnt, (void *)tp.gate_log})
         if (q)
             (void)hipFree(q);
     tp = bs_handle::Tape{};
@@ -720,7 +723,8 @@ void launch_gate(bs_handle *h, int32_t interval, int32_t advance)
 {
     const bool lists = !h->lists.empty();
     launch_prestep(h->status, h->counters, interval, advance, lists ? h->pos[h->cur] : nullptr, h->shift,
-                   (int32_t)h->n, lists ? h->box_part : nullptr, h->box_blocks, h->stream);
+                   (int32_t)h->n, lists ? h->box_part : nullptr, h->box_blocks, advance ? h->gate_log : nullptr,
+                   h->stream);
 }
 
 // The gated list rebuild.  prepped: the gate (and the box partials) were
@@ -1111,6 +1115,27 @@ void launch_step(bs_handle *h, bool energy)
     h->cur ^= 1;
 }
 
+// A step whose list rebuild is known not to happen (a recomputation of a
+// recorded step that did not rebuild): no gate, no gated chain.  The step
+// counter is not advanced; the recomputation restores the counters.
+void launch_step_unlisted(bs_handle *h)
+{
+    const size_t e1 = h->prof ? prof_mark(h) : SIZE_MAX;
+    if (!h->lists.empty())
+        launch_pairs(h, false);
+    const size_t e2 = h->prof ? prof_mark(h) : SIZE_MAX;
+    launch_springs(h, true, false);
+    launch_bodies(h);
+    launch_probe_step(h, true);
+    if (h->prof)
+    {
+        if (!h->lists.empty())
+            prof_span(h, BS_PROF_PAIR, e1, e2);
+        prof_span(h, BS_PROF_SPRING, e2, prof_mark(h));
+    }
+    h->cur ^= 1;
+}
+
 // Captures G steps (G even, starting at cur = 0) into a HIP graph.  The step
 // holds no host synchronisation and no memset nodes; rebuild decisions stay
 // on the device.
@@ -1404,8 +1429,8 @@ int adjoint_record(bs_handle *h, int32_t T, int32_t checkpoint_every)
     hipStream_t st = h->stream;
     hipError_t e = hipSuccess;
     // buffers grow as needed and are reused
-    const int64_t need_cp = (int64_t)std::max(ncp, 1), need_seg = K;
-    if (need_cp > tp.cap_cp || need_seg > tp.cap_seg || !tp.fin_pos)
+    const int64_t need_cp = (int64_t)std::max(ncp, 1), need_seg = K, need_log = std::max(T, 1);
+    if (need_cp > tp.cap_cp || need_seg > tp.cap_seg || need_log > tp.cap_log || !tp.fin_pos)
     {
         tape_free(tp);
         tp.cp_pos = dalloc<float4>((size_t)(n * need_cp), e);
@@ -1422,9 +1447,13 @@ int adjoint_record(bs_handle *h, int32_t T, int32_t checkpoint_every)
         HIPCHK(h, e);
         tp.fin_cnt = dalloc<int32_t>(4, e);
         HIPCHK(h, e);
+        tp.gate_log = dalloc<int32_t>((size_t)need_log, e);
+        HIPCHK(h, e);
         tp.cap_cp = need_cp;
         tp.cap_seg = need_seg;
+        tp.cap_log = need_log;
     }
+    launch_fill_i32(tp.gate_log, 0, need_log, nullptr, st);
     const int32_t s_last = (ncp - 1) * K;
     for (int32_t s = 0; s < T; ++s)
     {
@@ -1443,7 +1472,9 @@ int adjoint_record(bs_handle *h, int32_t T, int32_t checkpoint_every)
         const int rc = prof_fold(h);
         if (rc)
             return rc;
+        h->gate_log = tp.gate_log + s;
         launch_step(h, s + 1 == T);
+        h->gate_log = nullptr;
     }
     if (T > 0)
     {
@@ -1456,6 +1487,14 @@ int adjoint_record(bs_handle *h, int32_t T, int32_t checkpoint_every)
     HIPCHK(h, hipMemcpyAsync(tp.fin_pos, h->pos[h->cur], s4, hipMemcpyDeviceToDevice, st));
     HIPCHK(h, hipMemcpyAsync(tp.fin_vel, h->vel, s4, hipMemcpyDeviceToDevice, st));
     HIPCHK(h, hipMemcpyAsync(tp.fin_cnt, h->counters, 4 * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
+    {
+        std::vector<int32_t> g((size_t)std::max(T, 1));
+        HIPCHK(h, hipMemcpyAsync(g.data(), tp.gate_log, sizeof(int32_t) * g.size(), hipMemcpyDeviceToHost, st));
+        HIPCHK(h, hipStreamSynchronize(st));
+        tp.rebuilt.assign((size_t)T, 0);
+        for (int32_t s = 0; s < T; ++s)
+            tp.rebuilt[(size_t)s] = g[(size_t)s] != 0;
+    }
     tp.T = T;
     tp.K = K;
     tp.ncp = ncp;
@@ -1576,6 +1615,9 @@ int adjoint_sweep(bs_handle *h, AdjBufs &b, double *dk, double *dl0, double *dq)
                                      hipMemcpyDeviceToDevice, st));
             launch_fill_i32(h->status + 7, 1, 1, nullptr, st);
         }
+        // the recomputed steps rebuild the lists where the recorded steps
+        // did (the same states give the same decisions), so a step that did
+        // not is launched without its gate and gated rebuild
         for (int32_t s = s0; s < s1 && recompute; ++s)
         {
             HIPCHK(h, hipMemcpyAsync(tp.seg + (size_t)(s - s0) * n, h->pos[h->cur], s4, hipMemcpyDeviceToDevice, st));
@@ -1584,7 +1626,13 @@ int adjoint_sweep(bs_handle *h, AdjBufs &b, double *dk, double *dl0, double *dq)
                 rc = prof_fold(h);
                 if (rc)
                     return rc;
-                launch_step(h, false);
+                if (tp.rebuilt[(size_t)s] || s == s0)
+                {
+                    launch_fill_i32(h->status + 7, 1, 1, nullptr, st);
+                    launch_step(h, false);
+                }
+                else
+                    launch_step_unlisted(h);
             }
         }
         if (recompute)

@@ -368,9 +368,10 @@ void launch_probe(const ProbeArgs &p, int32_t nparts, float dt, int32_t visc_ena
 // the drift flag and, at the start of a step (advance), from the rebuild
 // interval; counters[0] counts steps, counters[1] rebuilds.  partial
 // (nullable): it also writes nblocks (<= BLOCK) box partials of pos (+ shift)
-// for launch_cell_ids, ungated.
+// for launch_cell_ids, ungated.  log (nullable): the gate is also written
+// there.
 void launch_prestep(int32_t *status, int32_t *counters, int32_t interval, int32_t advance, const float4 *pos,
-                    const float4 *shift, int32_t n, double *partial, int32_t nblocks, hipStream_t s);
+                    const float4 *shift, int32_t n, double *partial, int32_t nblocks, int32_t *log, hipStream_t s);
 void launch_rigid(const RigidArgs &a, hipStream_t s);
 void launch_adj_pair(const AdjPairArgs &a, int32_t steric_mode, int32_t elec_mode, int32_t nblocks, hipStream_t s);
 void launch_adj_spring(const AdjSpringArgs &a, int32_t group, int32_t nblocks, hipStream_t s);

@@ -1495,7 +1495,7 @@ __global__ void k_topo_sort_long(const int32_t *offs, int32_t n, int32_t *jout, 
 // rebuild counted (counters[1]).  advance: the start of a step, which also
 // stops the call after a non-finite position and counts the step
 // (counters[0]).
-__device__ void prestep_gate(int32_t *status, int32_t *counters, int32_t interval, int32_t advance)
+__device__ void prestep_gate(int32_t *status, int32_t *counters, int32_t interval, int32_t advance, int32_t *log)
 {
     if (halted(status))
         return;
@@ -1512,6 +1512,8 @@ __device__ void prestep_gate(int32_t *status, int32_t *counters, int32_t interva
     counters[1] += r;
     if (advance)
         counters[0] = step + 1;
+    if (log)
+        *log = r;
 }
 
 // The gate (first lane) and, with partial, the box partials of the current
@@ -1519,10 +1521,10 @@ __device__ void prestep_gate(int32_t *status, int32_t *counters, int32_t interva
 // about what a second launch would.
 __global__ void __launch_bounds__(BLOCK) k_prestep(int32_t *status, int32_t *counters, int32_t interval,
                                                    int32_t advance, const float4 *pos, const float4 *shift,
-                                                   int32_t n, double *partial)
+                                                   int32_t n, double *partial, int32_t *log)
 {
     if (blockIdx.x == 0 && threadIdx.x == 0)
-        prestep_gate(status, counters, interval, advance);
+        prestep_gate(status, counters, interval, advance, log);
     if (partial)
         box_partial_block(pos, shift, n, partial);
 }
@@ -1897,12 +1899,12 @@ void launch_probe(const ProbeArgs &p, int32_t nparts, float dt, int32_t visc_ena
 }
 
 void launch_prestep(int32_t *status, int32_t *counters, int32_t interval, int32_t advance, const float4 *pos,
-                    const float4 *shift, int32_t n, double *partial, int32_t nblocks, hipStream_t s)
+                    const float4 *shift, int32_t n, double *partial, int32_t nblocks, int32_t *log, hipStream_t s)
 {
     if (partial && n > 0)
-        k_prestep<<<nblocks, BLOCK, 0, s>>>(status, counters, interval, advance, pos, shift, n, partial);
+        k_prestep<<<nblocks, BLOCK, 0, s>>>(status, counters, interval, advance, pos, shift, n, partial, log);
     else
-        k_prestep<<<1, BLOCK, 0, s>>>(status, counters, interval, advance, nullptr, nullptr, 0, nullptr);
+        k_prestep<<<1, BLOCK, 0, s>>>(status, counters, interval, advance, nullptr, nullptr, 0, nullptr, log);
 }
 
 void launch_rigid(const RigidArgs &a, hipStream_t s)

@@ -1342,8 +1342,10 @@ void launch_centroid(const float4 *pos, int32_t n, double *partial, int32_t nblo
 }
 
 void launch_prestep(int32_t *status, int32_t *counters, int32_t interval, int32_t advance, const float4 *pos,
-                    const float4 *shift, int32_t n, double *partial, int32_t nblocks, hipStream_t st)
+                    const float4 *shift, int32_t n, double *partial, int32_t nblocks, int32_t *log, hipStream_t st)
 {
+    if (log)
+        need(log, 4, "prestep: log");
     need(status, 32, "prestep: status");
     need(counters, 8, "prestep: counters");
     if (partial && n > 0)
@@ -1362,6 +1364,8 @@ void launch_prestep(int32_t *status, int32_t *counters, int32_t interval, int32_
     counters[1] += r;
     if (advance)
         counters[0] += 1;
+    if (log)
+        *log = r;
 }
 
 void launch_fill_i32(int32_t *p, int32_t v, int64_t n, const int32_t *gate, hipStream_t)
