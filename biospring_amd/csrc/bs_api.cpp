@@ -36,8 +36,7 @@ struct List
     int32_t dims[3] = {1, 1, 1};
     int32_t ncells = 1;
     int32_t cap = 0;          // entries per i-cluster
-    uint32_t *cl = nullptr;   // [ncl * cap] j | mask << 24 (dealt order for the queue layout)
-    uint32_t *clt = nullptr;  // [ncl * cap] the build's entries in search order, before dealing
+    uint32_t *cl = nullptr;   // [ncl * cap] j | mask << 24, in search order
     int32_t *cln = nullptr;   // [ncl]
     uint32_t *rows = nullptr; // row layout (BS_PAIR_ROWS): [row blocks * rcap * 64]
     int32_t *rlen = nullptr;  // [row blocks * 64]
@@ -362,7 +361,6 @@ void free_device(bs_handle *h)
     for (auto &l : h->lists)
     {
         f(l.cl);
-        f(l.clt);
         f(l.cln);
         f(l.rows);
         f(l.rlen);
@@ -407,7 +405,7 @@ int check_device(bs_handle *h)
     if (h->ncl != (int32_t)((h->n + CLUSTER - 1) / CLUSTER))
         return fail(h, BS_ERR_STATE, "internal: cluster count mismatch");
     for (const auto &l : h->lists)
-        if (!l.cl || !l.clt || !l.cln || l.cap <= 0 || l.ncells > h->cell_cap ||
+        if (!l.cl || !l.cln || l.cap <= 0 || l.ncells > h->cell_cap ||
             (h->p.pair_layout == BS_PAIR_ROWS && (!l.rows || !l.rlen || l.rcap < l.cap)))
             return fail(h, BS_ERR_STATE, "internal: neighbour list buffers missing");
     return BS_OK;
@@ -696,7 +694,6 @@ void build_list(bs_handle *h, List &l, bool gated, int32_t cap_override)
     a.ex_ptr = h->ex_ptr;
     a.ex_w = h->ex_w;
     a.cl = l.cl;
-    a.clt = l.clt;
     a.cln = l.cln;
     a.status = h->status;
     a.n = n;
@@ -710,7 +707,6 @@ void build_list(bs_handle *h, List &l, bool gated, int32_t cap_override)
     a.i_mask = l.imask;
     a.j_mask = l.jmask;
     a.exclude_bonded = h->p.spring_enable ? 1 : 0;
-    a.deal = h->p.pair_layout == BS_PAIR_QUEUES ? 1 : 0;
     a.gate = g;
     launch_list_build(a, s);
     if (l.rows && cap_override < 0)
@@ -1066,12 +1062,9 @@ int regrow_lists(bs_handle *h)
         if (l.cap >= need)
             continue;
         (void)hipFree(l.cl);
-        (void)hipFree(l.clt);
-        l.cl = l.clt = nullptr;
+        l.cl = nullptr;
         l.cap = need;
         l.cl = dalloc<uint32_t>((size_t)h->ncl * l.cap, e);
-        HIPCHK(h, e);
-        l.clt = dalloc<uint32_t>((size_t)h->ncl * l.cap, e);
         HIPCHK(h, e);
         if (l.rows)
         {
@@ -2133,7 +2126,6 @@ int do_setup(bs_handle *h)
         const int32_t mx = st[3];
         l.cap = (std::max<int32_t>(16, (int32_t)std::ceil(mx * 1.25) + 64) + 15) / 16 * 16;
         ALLOC(l.cl, uint32_t, (int64_t)h->ncl * l.cap);
-        ALLOC(l.clt, uint32_t, (int64_t)h->ncl * l.cap);
         if (h->p.pair_layout == BS_PAIR_ROWS)
         {
             const int64_t nrb = (n + 63) / 64;

@@ -79,21 +79,18 @@ inline Consts make_consts()
 constexpr int32_t CLUSTER = 8;
 constexpr uint32_t CL_JMASK = 0x00FFFFFFu;
 
-// A cluster's entries are stored in dealt order: its `kept` entries form
-// nch = ceil(kept / QCHUNK) chunks, and entry p of the search order goes to
-// chunk p mod nch.  Chunk c holds q + (c < r) entries from offset
-// c q + min(c, r) (q = kept / nch, r = kept mod nch), so the chunks tile the
-// list densely and each samples all of it: the spatial order of the search
-// (entries on one side of the cluster, then the other) would otherwise give
-// a chunk entries that only some of the cluster's particles see.
+// The queue layout reads a cluster's `kept` entries (in search order) in
+// nch = ceil(kept / QCHUNK) chunks of nearly equal size: chunk c holds
+// q + (c < r) entries from offset c q + min(c, r) (q = kept / nch,
+// r = kept mod nch).
 constexpr int32_t QCHUNK = 128;
-struct Deal
+struct Chunks
 {
     int32_t nch, q, r;
 };
-constexpr Deal deal_of(int32_t kept) // constexpr: callable on host and device
+constexpr Chunks chunks_of(int32_t kept) // constexpr: callable on host and device
 {
-    Deal d{0, 0, 0};
+    Chunks d{0, 0, 0};
     d.nch = kept > 0 ? (kept + QCHUNK - 1) / QCHUNK : 0;
     d.q = kept > 0 ? kept / d.nch : 0;
     d.r = kept > 0 ? kept - d.q * d.nch : 0;
@@ -242,7 +239,7 @@ struct AdjPairArgs
     double4 *hw;          // [N] H w of this list (overwritten, or added with accumulate)
     double *dq;           // [N] dL/dq accumulator
     int32_t n, ncl, cap, accumulate;
-    int32_t queues;       // the lists in dealt order, read through per-particle LDS queues (k_adj_pair_q)
+    int32_t queues;       // the lists read in chunks through per-particle LDS queues (k_adj_pair_q)
     float rc, rc2lo, rc2hi;
     float steric_scale;
     double kf;            // Coulomb force prefactor incl. scale and 1/eps
@@ -308,8 +305,7 @@ struct ListArgs
     const double *box;         // [6] current box (device)
     const int32_t *ex_ptr;     // [ncl + 1] static exclusions per i-cluster
     const uint32_t *ex_w;      // j << 8 | bonded mask, sorted by j
-    uint32_t *cl;              // [ncl * cap] search order, or dealt order when deal is set
-    uint32_t *clt;             // [ncl * cap] the entries in search order before dealing (deal only)
+    uint32_t *cl;              // [ncl * cap] search order
     int32_t *cln;              // [ncl]
     int32_t *status;           // [2] overflow, [3] max count
     int32_t n, ncl, cap;
@@ -319,7 +315,6 @@ struct ListArgs
     float r2list;
     uint32_t i_mask, j_mask;
     int32_t exclude_bonded;
-    int32_t deal;              // store the lists in dealt order (the queue layout reads chunks)
     const int32_t *gate;       // nullable: skip unless *gate != 0
 };
 

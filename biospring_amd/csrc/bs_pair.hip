@@ -295,7 +295,7 @@ __global__ void __launch_bounds__(BLOCK) k_cluster_list(ListArgs a)
                 {
                     const int32_t slot = count + off;
                     if (slot < a.cap)
-                        (a.deal ? a.clt : a.cl)[(size_t)ci * a.cap + slot] = (uint32_t)j | (m << 24);
+                        a.cl[(size_t)ci * a.cap + slot] = (uint32_t)j | (m << 24);
                 }
                 count += __popcll(bal);
             }
@@ -308,26 +308,6 @@ __global__ void __launch_bounds__(BLOCK) k_cluster_list(ListArgs a)
             p0 = p1;
             o0 = o1;
             f0 = f1;
-        }
-    }
-    // Deal the entries into their chunks (dealt order: entry p to chunk
-    // p mod nch).  The wave reads back what its own lanes stored; a
-    // workgroup-scope release/acquire makes them visible (one CU).
-    const int32_t kept = count < a.cap ? count : a.cap;
-    if (a.deal && kept > 0)
-    {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-        const Deal d = deal_of(kept);
-        const uint32_t *src = a.clt + (size_t)ci * a.cap;
-        uint32_t *dst = a.cl + (size_t)ci * a.cap;
-        // chunk by chunk (no division): chunk c holds entries c, c + nch, ...
-        for (int32_t c = 0; c < d.nch; ++c)
-        {
-            const int32_t base = c * d.q + min(c, d.r), size = d.q + (c < d.r ? 1 : 0);
-            for (int32_t t = lane; t < size; t += 64)
-                dst[base + t] = src[t * d.nch + c];
         }
     }
     // zero words up to the next multiple of 16, read unconditionally by the
@@ -899,16 +879,15 @@ void launch_list_build(const ListArgs &a, hipStream_t s)
 }
 
 // ---------------------------------------------------------------------------
-// Queue layout (BS_PAIR_QUEUES).  A wave takes its cluster's list one dealt
-// chunk (at most QCHUNK = 128 words) at a time.  Each word's j data is loaded
+// Queue layout (BS_PAIR_QUEUES).  A wave takes its cluster's list one chunk
+// (at most QCHUNK = 128 words) at a time.  Each word's j data is loaded
 // once (one lane per word) into LDS, and each of the 8 cluster particles gets
 // an LDS queue of the chunk's words it sees (one ballot per particle and 64
 // words).  The 8 lanes of a particle then drain its queue, 8 entries per
 // round, so the lanes evaluate listed pairs instead of idling on entries
 // their particle does not see (an entry is seen by about half of a
 // cluster's particles when springs empty the inner shell).  A chunk takes as
-// many rounds as its fullest queue; the dealt order gives every chunk a
-// sample of the whole list, so the queues of a chunk are nearly equal.  Each
+// many rounds as its fullest queue.  Each
 // lane sums a fixed subset of its particle's entries in a fixed order and
 // the 8 partial sums are combined as in k_pair: deterministic.  The next
 // chunk's words and j data are loaded while the current one is evaluated.
@@ -938,7 +917,7 @@ __global__ void __launch_bounds__(BLOCK) k_pair_q(PairArgs a)
     const PairI c = pair_i<SMODE, EMODE>(pi, ri, a);
     double es = 0.0, ee = 0.0, eh = 0.0;
     double fx = 0.0, fy = 0.0, fz = 0.0;
-    const Deal d = deal_of(__builtin_amdgcn_readfirstlane(a.cln[ci]));
+    const Chunks d = chunks_of(__builtin_amdgcn_readfirstlane(a.cln[ci]));
     const uint32_t *L = a.cl + (size_t)ci * a.cap;
     // chunk 0 in flight (a chunk holds at least one word)
     int32_t cs = 0, cn = d.q + (d.r > 0 ? 1 : 0);
@@ -1224,7 +1203,7 @@ void launch_pair(const PairArgs &a, int32_t elec_mode, int32_t hydro, int32_t nb
 }
 
 // ---------------------------------------------------------------------------
-// The adjoint's pair terms in the queue layout (dealt lists, as k_pair_q):
+// The adjoint's pair terms in the queue layout (as k_pair_q):
 // each chunk's j data (position, parameters, adjoint weight; the index j in
 // the weight's unused fourth word, for a static partner's dq) is staged in
 // LDS once, queued per cluster particle, and drained 8 entries per round.
@@ -1254,8 +1233,7 @@ __global__ void __launch_bounds__(BLOCK) k_adj_pair_q(AdjPairArgs a)
     double hx = 0.0, hy = 0.0, hz = 0.0, dqi = 0.0;
     const int32_t cnt = __builtin_amdgcn_readfirstlane(a.cln[ci]);
     const uint32_t *L = a.cl + (size_t)ci * a.cap;
-    // 64 entries at a time in list order (the dealt order spreads each
-    // 64-entry stretch over the whole list too); the next stretch in flight
+    // 64 entries at a time in list order; the next stretch in flight
     uint32_t w = lane < cnt ? L[lane] : 0u;
     uint32_t j = w & CL_JMASK;
     float4 pj = a.pos[j], rj = a.jprm[j], vj = a.w[j];

@@ -475,8 +475,6 @@ void launch_list_build(const ListArgs &a, hipStream_t)
     if (a.cap > 0)
     {
         need(a.cl, sizeof(uint32_t) * (size_t)a.ncl * a.cap, "list: cl");
-        if (a.deal)
-            need(a.clt, sizeof(uint32_t) * (size_t)a.ncl * a.cap, "list: clt");
     }
     if ((int64_t)a.ncl * CLUSTER < a.n || (int64_t)(a.ncl - 1) * CLUSTER >= a.n)
         hc_fail("list: bad cluster count");
@@ -562,7 +560,7 @@ void launch_list_build(const ListArgs &a, hipStream_t)
                         if (!m)
                             continue;
                         if (count < a.cap)
-                            (a.deal ? a.clt : a.cl)[(size_t)ci * a.cap + count] = (uint32_t)j | (m << 24);
+                            a.cl[(size_t)ci * a.cap + count] = (uint32_t)j | (m << 24);
                         ++count;
                     }
                 }
@@ -570,19 +568,6 @@ void launch_list_build(const ListArgs &a, hipStream_t)
         if (a.cap % 16)
             hc_fail("list: capacity not a multiple of 16");
         const int32_t kept = std::min(count, a.cap);
-        if (a.deal && kept > 0)
-        { // dealt order; every slot below kept written exactly once
-            const Deal d = deal_of(kept);
-            std::vector<char> hit((size_t)kept, 0);
-            for (int32_t p = 0; p < kept; ++p)
-            {
-                const int32_t c = p % d.nch, t = p / d.nch, q = c * d.q + std::min(c, d.r) + t;
-                if (q < 0 || q >= kept || hit[(size_t)q] || t >= d.q + (c < d.r ? 1 : 0))
-                    hc_fail("list: dealt position out of range or repeated");
-                hit[(size_t)q] = 1;
-                a.cl[(size_t)ci * a.cap + q] = a.clt[(size_t)ci * a.cap + p];
-            }
-        }
         for (int32_t slot = count; slot < std::min((count + 15) & ~15, a.cap); ++slot)
             a.cl[(size_t)ci * a.cap + slot] = 0u;
         a.cln[ci] = std::min(count, a.cap);

@@ -132,9 +132,8 @@ typedef struct bs_params {
     int32_t graph_steps;
 
     /* Engine option: layout the pair kernels read.  BS_PAIR_QUEUES (the
-       default): the cluster lists stored in dealt order (chunks of at most
-       128 entries, each a sample of the whole list) and read a chunk at a
-       time, each entry's j data loaded once into LDS and queued for the
+       default): the cluster lists read in chunks of at most 128 entries,
+       each entry's j data loaded once into LDS and queued for the
        particles that see it, so that lanes evaluate listed pairs at the
        cluster layout's load count.  BS_PAIR_CLUSTERS: the cluster lists in
        search order, 8 particles x 8 entries per wave, one load of j's data

@@ -93,8 +93,7 @@ def test_cluster_lists_cover_reference_pairs(hostcheck_lib):
     """The host-built exclusion lists and the cluster search ranges, run
     through the checker's CPU emulation of the list build, reproduce the
     oracle's processed pair sets exactly (also after motion and a rebuild),
-    in search order and in the queue layout's dealt order, whose chunk
-    queues the checker's pair launcher verifies entry by entry."""
+    for the cluster and the queue layouts."""
     _run("""
         from oracle import pyoracle as po
         for cfg, n, seed, lay in ((0, 0, 2, 0), (1, 3000, 4, 0), (1, 12000, 5, 0), (0, 0, 2, 2), (1, 12000, 5, 2)):
