@@ -64,25 +64,33 @@ int magic(int k) { return kMagic[std::min(std::max(k, 0), kLastIdx - 1)]; }
 struct BitSink
 {
     std::vector<uint8_t> bytes;
-    uint32_t acc = 0;
-    int nacc = 0; // bits pending in acc (< 8)
+    uint64_t acc = 0;
+    int nacc = 0; // bits pending in acc (< 8 between calls)
 
+    // nbits <= 32 bits of v, most significant first
+    void put32(int nbits, uint32_t v)
+    {
+        if (nbits <= 0)
+            return;
+        const uint64_t m = nbits == 32 ? 0xffffffffull : ((1ull << nbits) - 1ull);
+        acc = (acc << nbits) | ((uint64_t)v & m);
+        nacc += nbits;
+        while (nacc >= 8)
+        {
+            nacc -= 8;
+            bytes.push_back((uint8_t)(acc >> nacc));
+        }
+        acc &= (1ull << nacc) - 1ull;
+    }
+    // any number of bits; beyond 32 the leading bits are zero (v is 32 bits)
     void put(int nbits, uint32_t v)
     {
-        while (nbits > 0)
+        while (nbits > 32)
         {
-            const int take = std::min(nbits, 8 - nacc);
-            const uint32_t chunk = (v >> (nbits - take)) & ((1u << take) - 1u);
-            acc = (acc << take) | chunk;
-            nacc += take;
-            nbits -= take;
-            if (nacc == 8)
-            {
-                bytes.push_back((uint8_t)acc);
-                acc = 0;
-                nacc = 0;
-            }
+            put32(32, 0u);
+            nbits -= 32;
         }
+        put32(nbits, v);
     }
     void finish()
     {
@@ -97,13 +105,8 @@ using u128 = unsigned __int128;
 
 int bitlen(u128 v)
 {
-    int b = 0;
-    while (v != 0)
-    {
-        ++b;
-        v >>= 1;
-    }
-    return b;
+    const uint64_t hi = (uint64_t)(v >> 64), lo = (uint64_t)v;
+    return hi ? 128 - __builtin_clzll(hi) : lo ? 64 - __builtin_clzll(lo) : 0;
 }
 
 // Bits of the integer that sizeofint reports: the smallest b with 2^b > size.
@@ -211,6 +214,7 @@ void xtc_coords(std::vector<uint8_t> &o, int32_t n, const float *x, float prec)
     const int larger = magic(maxidx) / 2;
 
     BitSink b;
+    b.bytes.reserve((size_t)n * 4);
     int32_t prev[3] = {0, 0, 0};
     int prevrun = -1;
     int32_t i = 0;
