@@ -934,22 +934,37 @@ __global__ void __launch_bounds__(BLOCK) k_seg_energies(const double4 *epp, cons
     }
 }
 
+constexpr int32_t SYS_LDS_SEGS = 1024; // segment partials staged in LDS (48 KiB)
+
 __global__ void __launch_bounds__(BLOCK) k_sys_energies(const double *seg_part, const int32_t *sys_seg,
                                                         int32_t nsys, double *out_sys, double *out_total,
                                                         const ProbeDev *probe)
 {
-    for (int32_t s = threadIdx.x; s < nsys; s += blockDim.x)
-        for (int k = 0; k < 6; ++k)
-        {
-            double t = 0.0;
-            for (int32_t g = sys_seg[s]; g < sys_seg[s + 1]; ++g)
-                t += seg_part[6 * g + k];
-            // the probe (a single network only): its pair energies and its
-            // kinetic energy (SpringNetwork.cpp:163-166,180)
-            if (probe && s == 0)
-                t += k == 2 ? probe->e[0] : k == 1 ? probe->e[1] : k == 3 ? probe->e[2] : 0.0;
-            out_sys[6 * s + k] = t;
-        }
+    // the partials are loaded by every lane at once, so the per-system sums
+    // (in segment order) wait on LDS rather than on a chain of global loads
+    __shared__ double sp[6 * SYS_LDS_SEGS];
+    const int32_t nseg = sys_seg[nsys];
+    const bool staged = nseg <= SYS_LDS_SEGS;
+    if (staged)
+    {
+        for (int32_t t = threadIdx.x; t < 6 * nseg; t += blockDim.x)
+            sp[t] = seg_part[t];
+        __syncthreads();
+    }
+    const double *part = staged ? sp : seg_part;
+    // one lane per (system, energy): each sum in segment order
+    for (int32_t sk = threadIdx.x; sk < 6 * nsys; sk += blockDim.x)
+    {
+        const int32_t s = sk / 6, k = sk % 6;
+        double t = 0.0;
+        for (int32_t g = sys_seg[s]; g < sys_seg[s + 1]; ++g)
+            t += part[6 * g + k];
+        // the probe (a single network only): its pair energies and its
+        // kinetic energy (SpringNetwork.cpp:163-166,180)
+        if (probe && s == 0)
+            t += k == 2 ? probe->e[0] : k == 1 ? probe->e[1] : k == 3 ? probe->e[2] : 0.0;
+        out_sys[6 * s + k] = t;
+    }
     __syncthreads();
     if (threadIdx.x < 6)
     {
