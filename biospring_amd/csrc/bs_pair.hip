@@ -58,6 +58,9 @@ __device__ inline uint32_t bonded_mask_lds(const uint32_t *tab, int32_t j)
     }
 }
 
+// lane k's value, in a scalar register (k wave-uniform)
+__device__ inline float lane_f(float x, int k) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), k)); }
+
 // ---------------------------------------------------------------------------
 // List build: one wave per i-cluster, one candidate j per lane.  Candidates
 // come from the cells covering the cluster's bounding box grown by the list
@@ -125,8 +128,8 @@ __global__ void __launch_bounds__(BLOCK) k_cluster_list(ListArgs a)
     while (todo != 0u)
     {
         const int f = __ffs(todo) - 1;
-        const uint32_t tag0 = __shfl(tag, f, 64);
-        const float fx = __shfl(pi.x, f, 64), fy = __shfl(pi.y, f, 64), fz = __shfl(pi.z, f, 64);
+        const uint32_t tag0 = (uint32_t)__builtin_amdgcn_readlane((int32_t)tag, f);
+        const float fx = lane_f(pi.x, f), fy = lane_f(pi.y, f), fz = lane_f(pi.z, f);
         // the first particle always belongs to its group (also for a
         // non-finite position, so that the loop ends)
         const bool near = lane == f ||
@@ -139,9 +142,9 @@ __global__ void __launch_bounds__(BLOCK) k_cluster_list(ListArgs a)
 #pragma unroll
         for (int k = 0; k < CLUSTER; ++k)
         {
-            xi[k] = __shfl(pi.x, k, 64);
-            yi[k] = __shfl(pi.y, k, 64);
-            zi[k] = __shfl(pi.z, k, 64);
+            xi[k] = lane_f(pi.x, k); // wave-uniform: scalar registers
+            yi[k] = lane_f(pi.y, k);
+            zi[k] = lane_f(pi.z, k);
             if ((vmask >> k) & 1u)
             {
                 bmin[0] = fminf(bmin[0], xi[k]);
@@ -177,21 +180,23 @@ __global__ void __launch_bounds__(BLOCK) k_cluster_list(ListArgs a)
             const double hi = c == a.dims[q] - 1 ? INFINITY : cmn[q] + (c + 1) * ce[q];
             return fmax(0.0, fmax(lo - (double)bmax[q], (double)bmin[q] - hi));
         };
-        // The candidates form a stream of batches of at most 64 (a row's
-        // cells cx0..cx1 are contiguous in the cell array), generated in row
-        // order.  A three-stage pipeline runs over it: while batch b is
-        // tested, batch b+1's positions and batch b+2's particle indices are
-        // in flight, so a batch never waits for its own loads.
+        // The candidates form a stream, generated in row order (a row's
+        // cells cx0..cx1 are contiguous in the cell array), cut into batches
+        // of 64 that run across row ends, so that short rows do not leave
+        // lanes idle.  A three-stage pipeline runs over the batches: while
+        // batch b is tested, batch b+1's positions and batch b+2's particle
+        // indices are in flight, so a batch never waits for its own loads.
         // Row setup is lane-parallel: lane l takes row rbase + l of the
         // (cy fastest, then cz) order, computes its x range and member run,
-        // and a ballot marks the rows with members; the generator then takes
-        // the marked rows in order, 64 rows per setup.
+        // and a wave prefix sum places the runs in the stream, 64 rows per
+        // setup; a batch never spans two setups.
         const int32_t ny = chi[1] - clo[1] + 1, nrows = ny * (chi[2] - clo[2] + 1);
-        int32_t rbase = 0, my_rt = 0, my_re = 0, rt = 0, re = 0; // generator: row setup, current row's rest
-        uint64_t rmask = 0ull;
+        int32_t rbase = 0, my_rt = 0, my_pre = 0, my_end = 0; // lane's row: first cell entry, stream range
+        int32_t total = 0, spos = 0, row = 0;                  // wave-uniform: the setup's stream length, position, row
         auto setup_rows = [&]() {
             const int32_t r = rbase + lane;
-            my_rt = my_re = 0;
+            int32_t re = 0;
+            my_rt = 0;
             if (r < nrows)
             {
                 const int32_t cy = clo[1] + r % ny, cz = clo[2] + r / ny;
@@ -205,63 +210,79 @@ __global__ void __launch_bounds__(BLOCK) k_cluster_list(ListArgs a)
                         internal_coord((float)((double)bmin[0] - rx), mn0, len0, a.dims[0], a.cell_edge);
                     const int32_t cx1 =
                         internal_coord((float)((double)bmax[0] + rx), mn0, len0, a.dims[0], a.cell_edge);
-                    const int32_t row = cy * a.dims[0] + cz * a.dims[0] * a.dims[1];
-                    my_rt = a.cell_start[row + cx0];
-                    my_re = a.cell_start[row + cx1 + 1];
+                    const int32_t row0 = cy * a.dims[0] + cz * a.dims[0] * a.dims[1];
+                    my_rt = a.cell_start[row0 + cx0];
+                    re = a.cell_start[row0 + cx1 + 1];
                 }
             }
-            rmask = __ballot(my_rt < my_re);
+            const int32_t len = re > my_rt ? re - my_rt : 0;
+            int32_t inc = len;
+            for (int d = 1; d < 64; d <<= 1)
+            {
+                const int32_t o = __shfl_up(inc, d, 64);
+                inc += lane >= d ? o : 0;
+            }
+            my_end = inc;
+            my_pre = inc - len;
+            total = __builtin_amdgcn_readlane(inc, 63);
+            spos = 0;
+            row = 0;
             rbase += 64;
         };
-        auto next_batch = [&](int32_t &bt, int32_t &be) {
-            while (rt >= re)
-            { // the next row with members
-                if (rmask == 0ull)
-                {
-                    if (rbase >= nrows)
-                        break;
-                    setup_rows();
-                    continue;
-                }
-                const int32_t f = __ffsll((long long)rmask) - 1;
-                rmask &= rmask - 1ull;
-                rt = __shfl(my_rt, f, 64);
-                re = __shfl(my_re, f, 64);
-            }
-            if (rt < re)
+        // The next batch: the lane's cell-array entry, or -1 past the
+        // batch's end; false once the stream is exhausted.
+        auto next_batch = [&](int32_t &tl) -> bool {
+            tl = -1;
+            while (spos >= total)
             {
-                bt = rt;
-                be = re;
-                rt += 64;
+                if (rbase >= nrows)
+                    return false;
+                setup_rows();
             }
-            else
-                bt = be = 0; // exhausted
+            const int32_t p = spos + lane, lim = min(spos + 64, total);
+            for (;;) // the rows overlapping [spos, lim), in order
+            {
+                const int32_t pre = __builtin_amdgcn_readlane(my_pre, row);
+                const int32_t end = __builtin_amdgcn_readlane(my_end, row);
+                const int32_t rt0 = __builtin_amdgcn_readlane(my_rt, row);
+                if (p >= pre && p < end)
+                    tl = rt0 + (p - pre);
+                if (end >= lim)
+                {
+                    if (end == lim && row < 63)
+                        ++row; // the row ends with the batch
+                    break;
+                }
+                ++row;
+            }
+            spos = lim;
+            return true;
         };
         // Index loads are clamped into the cell array (n entries); a lane
         // past its batch's end gathers particle 0 and tests nothing.
-        int32_t t0, e0, t1, e1, t2, e2;
-        next_batch(t0, e0);
-        next_batch(t1, e1);
+        int32_t t0, t1, t2;
+        bool h0 = next_batch(t0);
+        bool h1 = next_batch(t1);
         const int32_t last = a.n > 0 ? a.n - 1 : 0;
-        int32_t j0 = a.cell_idx[min(t0 + lane, last)];
-        int32_t j1 = a.cell_idx[min(t1 + lane, last)];
-        j0 = t0 + lane < e0 ? j0 : 0;
+        int32_t j0 = a.cell_idx[min(max(t0, 0), last)];
+        int32_t j1 = a.cell_idx[min(max(t1, 0), last)];
+        j0 = t0 >= 0 ? j0 : 0;
         float4 p0 = a.pos[j0];
         float4 o0 = a.shift ? a.shift[j0] : make_float4(0.f, 0.f, 0.f, 0.f);
         uint32_t f0 = fbits(a.jprm[j0].w);
-        while (t0 < e0) // wave-uniform
+        while (h0) // wave-uniform
         {
-            next_batch(t2, e2);
+            const bool h2 = next_batch(t2);
             // batch b+1's positions and b+2's indices in flight
-            j1 = t1 + lane < e1 ? j1 : 0;
+            j1 = t1 >= 0 ? j1 : 0;
             const float4 p1 = a.pos[j1];
             const float4 o1 = a.shift ? a.shift[j1] : make_float4(0.f, 0.f, 0.f, 0.f);
             const uint32_t f1 = fbits(a.jprm[j1].w);
-            const int32_t j2 = a.cell_idx[min(t2 + lane, last)];
+            const int32_t j2 = a.cell_idx[min(max(t2, 0), last)];
             // batch b
             uint32_t m = 0u;
             const int32_t j = j0;
-            if (t0 + lane < e0)
+            if (t0 >= 0)
             {
                 float4 pj = p0;
                 bool same_sys = true;
@@ -300,9 +321,9 @@ __global__ void __launch_bounds__(BLOCK) k_cluster_list(ListArgs a)
                 count += __popcll(bal);
             }
             t0 = t1;
-            e0 = e1;
+            h0 = h1;
             t1 = t2;
-            e1 = e2;
+            h1 = h2;
             j0 = j1;
             j1 = j2;
             p0 = p1;
